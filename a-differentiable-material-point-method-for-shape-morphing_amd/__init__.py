@@ -1,2 +1,378 @@
-"""MI355X-native differentiable MLS-MPM step (DiffMPMLib3D hot path)."""
+"""MI355X-native differentiable MLS-MPM step (the DiffMPMLib3D hot path).
+
+Python binding of the C ABI in ``include/dmpm.h`` (``lib/libdmpm.so``, HIP
+kernels for gfx950).  There is no CPU fallback: loading raises when the
+library is missing or no GPU is present.
+
+``Morph`` mirrors CompGraph (CompGraph.h:17-52): ``forward_pass`` =
+ComputeForwardPass, ``end_layer_mass_loss`` = EndLayerMassLoss,
+``backward_pass`` = ComputeBackwardPass, with layer states, controls and
+gradients in the reference's per-particle layout (x, v (n,3); F, C, dFc
+(n,3,3)).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
 from . import scenes  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libdmpm.so")
+
+PREC_F64 = 0
+PREC_F32 = 1
+TAPE_STORE_ALL = 0
+TAPE_CHECKPOINT = 1
+COMPAT_DUP_FBAR = 1
+COMPAT_NO_1MS = 2
+COMPAT_NO_DRAG_X = 4
+COMPAT_DLDM_NO_FEXT = 8
+COMPAT_REFERENCE = 15
+
+K_P2G, K_GRID_UPDATE, K_G2P, K_G2P_ADJ, K_GRID_ADJ, K_P2G_ADJ, K_LOSS, K_FILL = range(8)
+KERNEL_NAMES = ["p2g", "grid_update", "g2p", "g2p_adj", "grid_adj", "p2g_adj", "loss", "fill"]
+
+# C-ABI symbols declared in include/dmpm.h
+EXPORTS = [
+    "dmpm_version", "dmpm_last_error", "dmpm_config_init", "dmpm_create", "dmpm_destroy",
+    "dmpm_set_particles", "dmpm_compute_volumes", "dmpm_get_volumes", "dmpm_set_control", "dmpm_get_control",
+    "dmpm_set_target_grid", "dmpm_target_from_points", "dmpm_get_target_grid", "dmpm_forward_pass",
+    "dmpm_end_layer_mass_loss", "dmpm_backward_pass", "dmpm_forward_step", "dmpm_backward_step",
+    "dmpm_get_layer_state", "dmpm_set_layer_state", "dmpm_get_layer_grads", "dmpm_get_control_grad",
+    "dmpm_control_grad_norm", "dmpm_get_grid", "dmpm_chain_episode", "dmpm_synchronize", "dmpm_memory_bytes",
+    "dmpm_profile_enable", "dmpm_profile_reset", "dmpm_profile_read", "dmpm_eval_pk1", "dmpm_eval_hessian",
+    "dmpm_eval_svd",
+]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int32),
+        ("n_particles", ctypes.c_int32),
+        ("dim", ctypes.c_int32 * 3),
+        ("num_layers", ctypes.c_int32),
+        ("dx", ctypes.c_double),
+        ("min_point", ctypes.c_double * 3),
+        ("dt", ctypes.c_double),
+        ("drag", ctypes.c_double),
+        ("smoothing", ctypes.c_double),
+        ("f_ext", ctypes.c_double * 3),
+        ("compat", ctypes.c_int32),
+        ("tape_mode", ctypes.c_int32),
+        ("ckpt_stride", ctypes.c_int32),
+        ("keep_control_grads", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class DmpmError(RuntimeError):
+    pass
+
+
+_LIB = None
+_DP = ctypes.POINTER(ctypes.c_double)
+
+
+def load_library(path: str = LIB_PATH):
+    """Load ``libdmpm.so`` (raises if it is not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise DmpmError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    L = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    i32 = ctypes.c_int32
+    sig = {
+        "dmpm_version": ([], ctypes.c_int),
+        "dmpm_last_error": ([], ctypes.c_char_p),
+        "dmpm_config_init": ([ctypes.POINTER(Config)], ctypes.c_int),
+        "dmpm_create": ([ctypes.POINTER(Config), ctypes.POINTER(vp)], ctypes.c_int),
+        "dmpm_destroy": ([vp], ctypes.c_int),
+        "dmpm_set_particles": ([vp] + [_DP] * 8, ctypes.c_int),
+        "dmpm_compute_volumes": ([vp], ctypes.c_int),
+        "dmpm_get_volumes": ([vp, _DP], ctypes.c_int),
+        "dmpm_set_control": ([vp, i32, _DP], ctypes.c_int),
+        "dmpm_get_control": ([vp, i32, _DP], ctypes.c_int),
+        "dmpm_set_target_grid": ([vp, _DP], ctypes.c_int),
+        "dmpm_target_from_points": ([vp, _DP, _DP, i32], ctypes.c_int),
+        "dmpm_get_target_grid": ([vp, _DP], ctypes.c_int),
+        "dmpm_forward_pass": ([vp, i32], ctypes.c_int),
+        "dmpm_end_layer_mass_loss": ([vp, _DP], ctypes.c_int),
+        "dmpm_backward_pass": ([vp, i32], ctypes.c_int),
+        "dmpm_forward_step": ([vp, i32], ctypes.c_int),
+        "dmpm_backward_step": ([vp, i32], ctypes.c_int),
+        "dmpm_get_layer_state": ([vp, i32] + [_DP] * 4, ctypes.c_int),
+        "dmpm_set_layer_state": ([vp, i32] + [_DP] * 4, ctypes.c_int),
+        "dmpm_get_layer_grads": ([vp, i32] + [_DP] * 4, ctypes.c_int),
+        "dmpm_get_control_grad": ([vp, i32, _DP], ctypes.c_int),
+        "dmpm_control_grad_norm": ([vp, i32, _DP], ctypes.c_int),
+        "dmpm_get_grid": ([vp, i32, _DP, _DP], ctypes.c_int),
+        "dmpm_chain_episode": ([vp], ctypes.c_int),
+        "dmpm_synchronize": ([vp], ctypes.c_int),
+        "dmpm_memory_bytes": ([vp, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+        "dmpm_profile_enable": ([vp, i32], ctypes.c_int),
+        "dmpm_profile_reset": ([vp], ctypes.c_int),
+        "dmpm_profile_read": ([vp, i32, _DP, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "dmpm_eval_pk1": ([i32, _DP, _DP, _DP, _DP], ctypes.c_int),
+        "dmpm_eval_hessian": ([i32, _DP, _DP, _DP, _DP, _DP, i32], ctypes.c_int),
+        "dmpm_eval_svd": ([i32, _DP, _DP, _DP, _DP], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _LIB = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = _LIB.dmpm_last_error().decode(errors="replace")
+        raise DmpmError(f"dmpm error {rc}: {msg}")
+
+
+def _in(a, shape_tail=None):
+    if a is None:
+        return None, None
+    arr = np.ascontiguousarray(a, dtype=np.float64)
+    return arr, arr.ctypes.data_as(_DP)
+
+
+def _ptr(arr):
+    return None if arr is None else arr.ctypes.data_as(_DP)
+
+
+# ----------------------------------------------------- batched evaluators --
+def eval_pk1(F, lam, mu):
+    """PK_FixedCorotatedElasticity on the GPU for a batch of (n,3,3) F."""
+    L = load_library()
+    F, pF = _in(np.asarray(F).reshape(-1, 9))
+    n = F.shape[0]
+    lam, pl = _in(np.broadcast_to(np.asarray(lam, float), (n,)))
+    mu, pm = _in(np.broadcast_to(np.asarray(mu, float), (n,)))
+    out = np.zeros((n, 9))
+    _check(L.dmpm_eval_pk1(n, pF, pl, pm, _ptr(out)))
+    return out.reshape(n, 3, 3)
+
+
+def eval_hessian(F, lam, mu, dF, exact=True):
+    """d2_FCE_psi_dF2_mult_by_dF on the GPU for a batch."""
+    L = load_library()
+    F, pF = _in(np.asarray(F).reshape(-1, 9))
+    n = F.shape[0]
+    lam, pl = _in(np.broadcast_to(np.asarray(lam, float), (n,)))
+    mu, pm = _in(np.broadcast_to(np.asarray(mu, float), (n,)))
+    dF, pd = _in(np.asarray(dF).reshape(-1, 9))
+    out = np.zeros((n, 9))
+    _check(L.dmpm_eval_hessian(n, pF, pl, pm, pd, _ptr(out), int(exact)))
+    return out.reshape(n, 3, 3)
+
+
+def eval_svd(F):
+    """3x3 SVD on the GPU: (U, sigma, V) with U, V rotations."""
+    L = load_library()
+    F, pF = _in(np.asarray(F).reshape(-1, 9))
+    n = F.shape[0]
+    U = np.zeros((n, 9))
+    s = np.zeros((n, 3))
+    V = np.zeros((n, 9))
+    _check(L.dmpm_eval_svd(n, pF, _ptr(U), _ptr(s), _ptr(V)))
+    return U.reshape(n, 3, 3), s, V.reshape(n, 3, 3)
+
+
+# ------------------------------------------------------------------ Morph --
+class Morph:
+    """One morph (CompGraph) on one GPU, backed by a ``dmpm_ctx``."""
+
+    def __init__(self, n, dim, dx, num_layers, *, min_point=(0.0, 0.0, 0.0), dt=1.0 / 120.0, drag=0.5,
+                 smoothing=0.1, f_ext=(0.0, 0.0, 0.0), precision=PREC_F64, compat=0, tape=TAPE_STORE_ALL,
+                 ckpt_stride=0, keep_control_grads=False, device=0):
+        L = load_library()
+        cfg = Config()
+        _check(L.dmpm_config_init(ctypes.byref(cfg)))
+        cfg.precision = int(precision)
+        cfg.n_particles = int(n)
+        for a in range(3):
+            cfg.dim[a] = int(dim[a])
+            cfg.min_point[a] = float(min_point[a])
+            cfg.f_ext[a] = float(f_ext[a])
+        cfg.num_layers = int(num_layers)
+        cfg.dx = float(dx)
+        cfg.dt = float(dt)
+        cfg.drag = float(drag)
+        cfg.smoothing = float(smoothing)
+        cfg.compat = int(compat)
+        cfg.tape_mode = int(tape)
+        cfg.ckpt_stride = int(ckpt_stride)
+        cfg.keep_control_grads = int(bool(keep_control_grads))
+        cfg.device = int(device)
+        self.cfg = cfg
+        self.n = int(n)
+        self.G = int(dim[0]) * int(dim[1]) * int(dim[2])
+        self.num_layers = int(num_layers)
+        h = ctypes.c_void_p()
+        _check(L.dmpm_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._L = L
+
+    @classmethod
+    def from_scene(cls, sc, **kw):
+        kw.setdefault("dt", sc.dt)
+        kw.setdefault("drag", sc.drag)
+        kw.setdefault("smoothing", sc.smoothing)
+        kw.setdefault("f_ext", sc.f_ext)
+        kw.setdefault("min_point", sc.min_point)
+        return cls(sc.n, sc.dim, sc.dx, kw.pop("num_layers", sc.num_layers), **kw)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.dmpm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- inputs ---------------------------------------------------------
+    def set_particles(self, x, v, F, C, m, vol, lam, mu):
+        arrs = [_in(a) for a in (x, v, F, C, m, vol, lam, mu)]
+        _check(self._L.dmpm_set_particles(self._h, *[p for _, p in arrs]))
+
+    def compute_volumes(self):
+        _check(self._L.dmpm_compute_volumes(self._h))
+
+    def volumes(self):
+        out = np.zeros(self.n)
+        _check(self._L.dmpm_get_volumes(self._h, _ptr(out)))
+        return out
+
+    def set_control(self, layer, dFc):
+        a, p = _in(dFc)
+        _check(self._L.dmpm_set_control(self._h, int(layer), p))
+
+    def control(self, layer):
+        out = np.zeros((self.n, 3, 3))
+        _check(self._L.dmpm_get_control(self._h, int(layer), _ptr(out)))
+        return out
+
+    def set_target_grid(self, mass):
+        a, p = _in(mass)
+        _check(self._L.dmpm_set_target_grid(self._h, p))
+
+    def target_from_points(self, x, m):
+        xa, px = _in(x)
+        ma, pm = _in(m)
+        _check(self._L.dmpm_target_from_points(self._h, px, pm, int(len(ma))))
+
+    def target_grid(self):
+        out = np.zeros(self.G)
+        _check(self._L.dmpm_get_target_grid(self._h, _ptr(out)))
+        return out
+
+    # -- passes ---------------------------------------------------------
+    def forward_pass(self, start_layer=0):
+        _check(self._L.dmpm_forward_pass(self._h, int(start_layer)))
+
+    def end_layer_mass_loss(self):
+        loss = ctypes.c_double()
+        _check(self._L.dmpm_end_layer_mass_loss(self._h, ctypes.byref(loss)))
+        return loss.value
+
+    def backward_pass(self, control_layer=0):
+        _check(self._L.dmpm_backward_pass(self._h, int(control_layer)))
+
+    def forward_step(self, layer):
+        _check(self._L.dmpm_forward_step(self._h, int(layer)))
+
+    def backward_step(self, layer):
+        _check(self._L.dmpm_backward_step(self._h, int(layer)))
+
+    def chain_episode(self):
+        _check(self._L.dmpm_chain_episode(self._h))
+
+    def synchronize(self):
+        _check(self._L.dmpm_synchronize(self._h))
+
+    # -- outputs --------------------------------------------------------
+    def layer_state(self, layer):
+        x = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        F = np.zeros((self.n, 3, 3))
+        C = np.zeros((self.n, 3, 3))
+        _check(self._L.dmpm_get_layer_state(self._h, int(layer), _ptr(x), _ptr(v), _ptr(F), _ptr(C)))
+        return x, v, F, C
+
+    def set_layer_state(self, layer, x=None, v=None, F=None, C=None):
+        arrs = [_in(a) for a in (x, v, F, C)]
+        _check(self._L.dmpm_set_layer_state(self._h, int(layer), *[p for _, p in arrs]))
+
+    def layer_grads(self, layer):
+        x = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        F = np.zeros((self.n, 3, 3))
+        C = np.zeros((self.n, 3, 3))
+        _check(self._L.dmpm_get_layer_grads(self._h, int(layer), _ptr(x), _ptr(v), _ptr(F), _ptr(C)))
+        return x, v, F, C
+
+    def control_grad(self, layer):
+        out = np.zeros((self.n, 3, 3))
+        _check(self._L.dmpm_get_control_grad(self._h, int(layer), _ptr(out)))
+        return out
+
+    def control_grad_norm(self, layer):
+        out = ctypes.c_double()
+        _check(self._L.dmpm_control_grad_norm(self._h, int(layer), ctypes.byref(out)))
+        return out.value
+
+    def grid(self, step):
+        m = np.zeros(self.G)
+        v = np.zeros((self.G, 3))
+        _check(self._L.dmpm_get_grid(self._h, int(step), _ptr(m), _ptr(v)))
+        return m, v
+
+    def memory_bytes(self):
+        b = ctypes.c_uint64()
+        _check(self._L.dmpm_memory_bytes(self._h, ctypes.byref(b)))
+        return int(b.value)
+
+    # -- profiling ------------------------------------------------------
+    def profile(self, on=True):
+        _check(self._L.dmpm_profile_enable(self._h, int(bool(on))))
+
+    def profile_reset(self):
+        _check(self._L.dmpm_profile_reset(self._h))
+
+    def profile_read(self):
+        out = {}
+        for k, name in enumerate(KERNEL_NAMES):
+            ms = ctypes.c_double()
+            n = ctypes.c_int64()
+            _check(self._L.dmpm_profile_read(self._h, k, ctypes.byref(ms), ctypes.byref(n)))
+            out[name] = (ms.value, int(n.value))
+        return out
+
+
+def morph_from_scene(sc, dFc=None, **kw):
+    """Build a ``Morph`` for a ``scenes.Scene``: layer 0 particles, reference-formula
+    volumes (computed on the GPU), controls and the target grid (GPU P2G)."""
+    mo = Morph.from_scene(sc, **kw)
+    x, v, C, F = sc.state0()
+    mo.set_particles(x, v, F, C, sc.m, None, sc.lam, sc.mu)
+    mo.compute_volumes()
+    if dFc is not None:
+        mo.set_control(-1, dFc)
+    mo.target_from_points(sc.target_x, sc.target_m)
+    return mo

@@ -1,0 +1,72 @@
+// Kernel-side parameter blocks and host launch wrappers of the MLS-MPM step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace dmpm {
+
+// Particle layer in HBM: structure of arrays, field f of particle p at
+// base[f * stride + p].  Fields: x0..2, v0..2, C00..C22, F00..F22.
+enum : int { FX = 0, FV = 3, FC = 6, FF = 15, kLayerFields = 24 };
+// Per-particle constants: m, vol, lam, mu.
+enum : int { KM = 0, KVOL = 1, KLAM = 2, KMU = 3, kConstFields = 4 };
+// Grid of one step: m then v (or momentum before the update) per axis.
+enum : int { kGridFields = 4 };
+
+struct StepParams {
+    int n;            // particles
+    int stride;       // SoA field stride (>= n)
+    int dim[3];
+    long long G;      // nodes
+    double dx, inv_dx;
+    double min_point[3];
+    double dt, drag, smoothing;
+    double f_ext[3];
+    int compat;       // reference-compat bits (same as the C ABI)
+};
+
+template <typename Real>
+struct StepIO {
+    const Real* cur;   // layer n state
+    Real* next;        // layer n+1 state (forward)
+    const Real* ctrl;  // dFc_n (9 fields)
+    const Real* cst;   // constants
+    Real* grid;        // grid m, v/p (kGridFields * G)
+};
+
+template <typename Real>
+struct AdjIO {
+    const Real* cur;     // layer n state
+    const Real* next;    // layer n+1 state (C' is read)
+    const Real* ctrl;    // dFc_n
+    const Real* cst;
+    const Real* grid;    // grid of step n after the update (m, v)
+    Real* agrid;         // adjoint grid: vbar -> pbar (3G), mbar (G)
+    const Real* g1;      // gradients of layer n+1 (24 fields, same layout as a layer)
+    Real* g0;            // gradients of layer n
+    Real* gctrl;         // dL/d(dFc_n) (9 fields)
+    Real* tmp;           // per particle vt (3) and Ct (9)
+};
+
+template <typename Real> void launch_fill(Real* p, long long count, Real value, hipStream_t s);
+template <typename Real> void launch_p2g(const StepParams& sp, const StepIO<Real>& io, hipStream_t s);
+template <typename Real> void launch_grid_update(const StepParams& sp, Real* grid, hipStream_t s);
+template <typename Real> void launch_g2p(const StepParams& sp, const StepIO<Real>& io, hipStream_t s);
+template <typename Real> void launch_p2g_mass(const StepParams& sp, const Real* x, const Real* m, int stride, Real* gm, hipStream_t s);
+template <typename Real> void launch_volumes(const StepParams& sp, const Real* x, Real* cst, const Real* gm, hipStream_t s);
+template <typename Real> void launch_loss(const StepParams& sp, const Real* gm, const Real* target, Real* dLdm,
+                                          double* loss_acc, hipStream_t s);
+template <typename Real> void launch_loss_particles(const StepParams& sp, const Real* layer, const Real* cst,
+                                                    const Real* dLdm, Real* g, hipStream_t s);
+template <typename Real> void launch_g2p_adj(const StepParams& sp, const AdjIO<Real>& io, hipStream_t s);
+template <typename Real> void launch_grid_adj(const StepParams& sp, const AdjIO<Real>& io, hipStream_t s);
+template <typename Real> void launch_p2g_adj(const StepParams& sp, const AdjIO<Real>& io, hipStream_t s);
+
+// batched constitutive evaluations (AoS 9 doubles per matrix)
+void launch_eval_pk1(int n, const double* F, const double* lam, const double* mu, double* P, hipStream_t s);
+void launch_eval_hess(int n, const double* F, const double* lam, const double* mu, const double* dF, double* out,
+                      int exact, hipStream_t s);
+void launch_eval_svd(int n, const double* F, double* U, double* sig, double* V, hipStream_t s);
+
+}  // namespace dmpm

@@ -1,0 +1,1033 @@
+// C ABI, device memory and episode driver of the differentiable MLS-MPM step.
+//
+// A context holds one morph on one GPU: per-particle constants, per-layer
+// controls, the forward trajectory (every layer, or checkpoints plus one
+// segment tape), two ping-pong gradient layers and the work grids.  The
+// passes follow CompGraph (CompGraph.cpp:246-267): forward over layers
+// [start, L-1), EndLayerMassLoss on the last layer, backward from L-2 down to
+// the control layer.  In checkpoint mode the forward keeps layer 0, every
+// ckpt_stride-th layer and the whole last segment; the backward recomputes
+// one segment at a time into the tape before stepping back through it.
+#include "../../include/dmpm.h"
+#include "dmpm_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+using namespace dmpm;
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) return fail(DMPM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct ProfRec {
+    int kernel;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct dmpm_ctx {
+    dmpm_config cfg{};
+    bool f32 = false;
+    size_t esz = 8;
+    int N = 0, S = 0, L = 0, K = 0;
+    long long G = 0;
+    hipStream_t stream = nullptr;
+    // device buffers
+    void* cst = nullptr;      // 4 x S
+    void* ctrl = nullptr;     // (L-1) x 9 x S
+    void* layers = nullptr;   // layer slots x 24 x S
+    void* grids = nullptr;    // grid slots x 4 x G
+    void* wgrid = nullptr;    // 4 x G: loss mass + dLdm / checkpoint-mode forward grid
+    void* agrid = nullptr;    // 4 x G: vbar/pbar + mbar
+    void* gbuf[2] = {nullptr, nullptr};
+    void* gctrl = nullptr;    // (L-1) x 9 x S or 9 x S
+    void* tmp = nullptr;      // 12 x S
+    void* target = nullptr;   // G
+    double* dacc = nullptr;   // FP64 accumulators
+    uint64_t bytes = 0;
+    // checkpoint-mode slot layout
+    int n_ckpt = 0, seg_last = 0, tape_base = 0, roll_base = 0, n_slots = 0;
+    int tape_seg = -1;
+    // validity
+    bool have_particles = false;
+    bool have_target = false;
+    int fwd_valid = -1;      // layers [0, fwd_valid] are consistent with the controls
+    bool loss_valid = false; // last layer gradients seeded
+    int grads_lo = -1;       // lowest layer holding gradients
+    int bwd_control = -1;    // control layer of the last backward pass
+    // profiling
+    bool prof = false;
+    std::vector<ProfRec> recs;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    double prof_ms[DMPM_K_COUNT] = {0};
+    int64_t prof_n[DMPM_K_COUNT] = {0};
+};
+
+namespace {
+
+int dalloc(dmpm_ctx* c, void** p, size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(DMPM_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) + " B) failed: " + hipGetErrorString(e));
+    }
+    c->bytes += bytes;
+    return DMPM_OK;
+}
+
+hipEvent_t pool_event(dmpm_ctx* c)
+{
+    if (c->pool_used == c->pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->pool.push_back(e);
+    }
+    return c->pool[c->pool_used++];
+}
+
+// Run a launch, bracketed by events when profiling is on.
+template <typename Fn>
+void timed(dmpm_ctx* c, int kernel, Fn&& fn)
+{
+    if (!c->prof) {
+        fn();
+        return;
+    }
+    hipEvent_t a = pool_event(c), b = pool_event(c);
+    if (!a || !b) {
+        fn();
+        return;
+    }
+    hipEventRecord(a, c->stream);
+    fn();
+    hipEventRecord(b, c->stream);
+    c->recs.push_back({kernel, a, b});
+}
+
+int drain_profile(dmpm_ctx* c)
+{
+    if (c->recs.empty()) return DMPM_OK;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (auto& r : c->recs) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+        c->prof_ms[r.kernel] += ms;
+        c->prof_n[r.kernel] += 1;
+    }
+    c->recs.clear();
+    c->pool_used = 0;
+    return DMPM_OK;
+}
+
+StepParams step_params(const dmpm_ctx* c)
+{
+    StepParams sp{};
+    sp.n = c->N;
+    sp.stride = c->S;
+    for (int a = 0; a < 3; ++a) {
+        sp.dim[a] = c->cfg.dim[a];
+        sp.min_point[a] = c->cfg.min_point[a];
+        sp.f_ext[a] = c->cfg.f_ext[a];
+    }
+    sp.G = c->G;
+    sp.dx = c->cfg.dx;
+    sp.inv_dx = 1.0 / c->cfg.dx;
+    sp.dt = c->cfg.dt;
+    sp.drag = c->cfg.drag;
+    sp.smoothing = c->cfg.smoothing;
+    sp.compat = c->cfg.compat;
+    return sp;
+}
+
+// ------------------------------------------------------------ slot layout --
+template <typename Real>
+Real* slot(dmpm_ctx* c, int s)
+{
+    return static_cast<Real*>(c->layers) + (size_t)s * kLayerFields * c->S;
+}
+
+template <typename Real>
+Real* tape_layer(dmpm_ctx* c, int j)  // layer j inside the current tape segment
+{
+    if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) return slot<Real>(c, j);
+    if (j % c->K == 0) return slot<Real>(c, j / c->K);
+    return slot<Real>(c, c->tape_base + (j - (j / c->K) * c->K));
+}
+
+template <typename Real>
+Real* grid_of_step(dmpm_ctx* c, int i)
+{
+    if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) return static_cast<Real*>(c->grids) + (size_t)i * kGridFields * c->G;
+    int seg = i / c->K;
+    if (seg == c->tape_seg || (c->tape_seg < 0 && seg == c->seg_last))
+        return static_cast<Real*>(c->grids) + (size_t)(i - seg * c->K) * kGridFields * c->G;
+    return static_cast<Real*>(c->wgrid);
+}
+
+template <typename Real>
+Real* ctrl_of(dmpm_ctx* c, int layer)
+{
+    return static_cast<Real*>(c->ctrl) + (size_t)layer * 9 * c->S;
+}
+
+template <typename Real>
+Real* grads_of(dmpm_ctx* c, int layer)
+{
+    return static_cast<Real*>(c->gbuf[layer & 1]);
+}
+
+template <typename Real>
+Real* gctrl_of(dmpm_ctx* c, int layer)
+{
+    return static_cast<Real*>(c->gctrl) + (c->cfg.keep_control_grads ? (size_t)layer * 9 * c->S : 0);
+}
+
+// --------------------------------------------------------------- kernels ---
+template <typename Real>
+void fill(dmpm_ctx* c, Real* p, long long count, Real v)
+{
+    timed(c, DMPM_K_FILL, [&] { launch_fill<Real>(p, count, v, c->stream); });
+}
+
+// One ForwardTimeStep: cur -> nxt, grid of the step in g.
+template <typename Real>
+void step_forward(dmpm_ctx* c, const Real* cur, Real* nxt, const Real* ctrl, Real* g)
+{
+    StepParams sp = step_params(c);
+    StepIO<Real> io{cur, nxt, ctrl, static_cast<const Real*>(c->cst), g};
+    fill<Real>(c, g, (long long)kGridFields * c->G, Real(0));
+    timed(c, DMPM_K_P2G, [&] { launch_p2g<Real>(sp, io, c->stream); });
+    timed(c, DMPM_K_GRID_UPDATE, [&] { launch_grid_update<Real>(sp, g, c->stream); });
+    timed(c, DMPM_K_G2P, [&] { launch_g2p<Real>(sp, io, c->stream); });
+}
+
+// One Back_Timestep: gradients of layer i+1 -> layer i.
+template <typename Real>
+void step_backward(dmpm_ctx* c, int i, const Real* cur, const Real* nxt, const Real* g)
+{
+    StepParams sp = step_params(c);
+    AdjIO<Real> io{};
+    io.cur = cur;
+    io.next = nxt;
+    io.ctrl = ctrl_of<Real>(c, i);
+    io.cst = static_cast<const Real*>(c->cst);
+    io.grid = g;
+    io.agrid = static_cast<Real*>(c->agrid);
+    io.g1 = grads_of<Real>(c, i + 1);
+    io.g0 = grads_of<Real>(c, i);
+    io.gctrl = gctrl_of<Real>(c, i);
+    io.tmp = static_cast<Real*>(c->tmp);
+    fill<Real>(c, io.agrid, 3 * c->G, Real(0));
+    timed(c, DMPM_K_G2P_ADJ, [&] { launch_g2p_adj<Real>(sp, io, c->stream); });
+    timed(c, DMPM_K_GRID_ADJ, [&] { launch_grid_adj<Real>(sp, io, c->stream); });
+    timed(c, DMPM_K_P2G_ADJ, [&] { launch_p2g_adj<Real>(sp, io, c->stream); });
+}
+
+// Bring layer j (<= fwd_valid) into a readable slot and return it.  Checkpoint
+// mode recomputes from the checkpoint below j into the rolling slots when j
+// is neither a checkpoint nor in the current tape segment.
+template <typename Real>
+Real* materialize(dmpm_ctx* c, int j)
+{
+    if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) return slot<Real>(c, j);
+    if (j % c->K == 0) return slot<Real>(c, j / c->K);
+    if (c->tape_seg == j / c->K) return tape_layer<Real>(c, j);
+    const int j0 = (j / c->K) * c->K;
+    Real* prev = slot<Real>(c, j0 / c->K);
+    for (int i = j0; i < j; ++i) {
+        Real* dst = slot<Real>(c, c->roll_base + ((i + 1) & 1));
+        step_forward<Real>(c, prev, dst, ctrl_of<Real>(c, i), static_cast<Real*>(c->wgrid));
+        prev = dst;
+    }
+    return prev;
+}
+
+// Recompute segment seg into the tape (checkpoint mode).
+template <typename Real>
+void recompute_segment(dmpm_ctx* c, int seg)
+{
+    const int s0 = seg * c->K, s1 = std::min(s0 + c->K, c->L - 1);
+    Real* grids = static_cast<Real*>(c->grids);
+    for (int i = s0; i < s1; ++i) {
+        const Real* cur = (i == s0) ? slot<Real>(c, s0 / c->K) : slot<Real>(c, c->tape_base + (i - s0));
+        Real* dst = ((i + 1) % c->K == 0) ? slot<Real>(c, c->roll_base) : slot<Real>(c, c->tape_base + (i + 1 - s0));
+        step_forward<Real>(c, cur, dst, ctrl_of<Real>(c, i), grids + (size_t)(i - s0) * kGridFields * c->G);
+    }
+    c->tape_seg = seg;
+}
+
+// -------------------------------------------------------------- passes -----
+template <typename Real>
+int forward_pass(dmpm_ctx* c, int start)
+{
+    if (!c->have_particles) return fail(DMPM_ERR_STATE, "forward pass before dmpm_set_particles");
+    if (start < 0 || start >= c->L - 1) return fail(DMPM_ERR_INVALID, "start_layer out of range");
+    int from = std::min(start, c->fwd_valid);
+    if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) {
+        for (int i = from; i < c->L - 1; ++i)
+            step_forward<Real>(c, slot<Real>(c, i), slot<Real>(c, i + 1), ctrl_of<Real>(c, i), grid_of_step<Real>(c, i));
+    } else {
+        from = (from / c->K) * c->K;  // restart from a checkpoint so the tape holds whole segments
+        const int s0_last = c->seg_last * c->K;
+        Real* grids = static_cast<Real*>(c->grids);
+        for (int i = from; i < c->L - 1; ++i) {
+            const Real* cur;
+            if (i % c->K == 0) cur = slot<Real>(c, i / c->K);
+            else if (i / c->K == c->seg_last) cur = slot<Real>(c, c->tape_base + (i - s0_last));
+            else cur = slot<Real>(c, c->roll_base + (i & 1));
+            Real* dst;
+            const int j = i + 1;
+            if (j % c->K == 0) dst = slot<Real>(c, j / c->K);
+            else if (j / c->K == c->seg_last) dst = slot<Real>(c, c->tape_base + (j - s0_last));
+            else dst = slot<Real>(c, c->roll_base + (j & 1));
+            Real* g = (i >= s0_last) ? grids + (size_t)(i - s0_last) * kGridFields * c->G : static_cast<Real*>(c->wgrid);
+            step_forward<Real>(c, cur, dst, ctrl_of<Real>(c, i), g);
+        }
+        c->tape_seg = c->seg_last;
+    }
+    HIP_TRY(hipGetLastError());
+    c->fwd_valid = c->L - 1;
+    c->loss_valid = false;
+    c->grads_lo = -1;
+    return DMPM_OK;
+}
+
+template <typename Real>
+int end_loss(dmpm_ctx* c, double* loss)
+{
+    if (!c->have_target) return fail(DMPM_ERR_STATE, "loss before the target grid is set");
+    if (c->fwd_valid < c->L - 1) return fail(DMPM_ERR_STATE, "loss before the forward pass reached the last layer");
+    const int last = c->L - 1;
+    Real* lay = (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) ? slot<Real>(c, last) : materialize<Real>(c, last);
+    StepParams sp = step_params(c);
+    Real* gm = static_cast<Real*>(c->wgrid);
+    Real* dLdm = gm + c->G;
+    fill<Real>(c, gm, c->G, Real(0));
+    HIP_TRY(hipMemsetAsync(c->dacc, 0, sizeof(double), c->stream));
+    const Real* cst = static_cast<const Real*>(c->cst);
+    timed(c, DMPM_K_LOSS, [&] {
+        launch_p2g_mass<Real>(sp, lay, cst + KM * c->S, c->S, gm, c->stream);
+        launch_loss<Real>(sp, gm, static_cast<const Real*>(c->target), dLdm, c->dacc, c->stream);
+        launch_loss_particles<Real>(sp, lay, cst, dLdm, grads_of<Real>(c, last), c->stream);
+    });
+    HIP_TRY(hipGetLastError());
+    double h = 0.0;
+    HIP_TRY(hipMemcpyAsync(&h, c->dacc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (loss) *loss = h;
+    c->loss_valid = true;
+    c->grads_lo = last;
+    c->bwd_control = -1;
+    return DMPM_OK;
+}
+
+template <typename Real>
+int backward_pass(dmpm_ctx* c, int control)
+{
+    if (!c->loss_valid) return fail(DMPM_ERR_STATE, "backward pass before EndLayerMassLoss");
+    if (control < 0 || control > c->L - 2) return fail(DMPM_ERR_INVALID, "control_layer out of range");
+    if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) {
+        for (int i = c->L - 2; i >= control; --i)
+            step_backward<Real>(c, i, slot<Real>(c, i), slot<Real>(c, i + 1), grid_of_step<Real>(c, i));
+    } else {
+        Real* grids = static_cast<Real*>(c->grids);
+        for (int seg = c->seg_last; seg >= control / c->K; --seg) {
+            const int s0 = seg * c->K, s1 = std::min(s0 + c->K, c->L - 1);
+            if (c->tape_seg != seg) recompute_segment<Real>(c, seg);
+            for (int i = s1 - 1; i >= std::max(s0, control); --i) {
+                const Real* cur = tape_layer<Real>(c, i);
+                const Real* nxt = tape_layer<Real>(c, i + 1);
+                step_backward<Real>(c, i, cur, nxt, grids + (size_t)(i - s0) * kGridFields * c->G);
+            }
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    c->grads_lo = control;
+    c->bwd_control = control;
+    return DMPM_OK;
+}
+
+// ------------------------------------------------------- host conversion ---
+template <typename Real>
+int upload_fields(dmpm_ctx* c, Real* dst, int field0, int nfield, const double* src, int n)
+{
+    // AoS [n][nfield] double -> SoA fields field0.. of stride S
+    std::vector<Real> h((size_t)nfield * c->S, Real(0));
+    for (int p = 0; p < n; ++p)
+        for (int f = 0; f < nfield; ++f) h[(size_t)f * c->S + p] = Real(src[(size_t)p * nfield + f]);
+    HIP_TRY(hipMemcpyAsync(dst + (size_t)field0 * c->S, h.data(), h.size() * sizeof(Real), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return DMPM_OK;
+}
+
+template <typename Real>
+int download_fields(dmpm_ctx* c, const Real* src, int field0, int nfield, double* dst, int n)
+{
+    std::vector<Real> h((size_t)nfield * c->S);
+    HIP_TRY(hipMemcpyAsync(h.data(), src + (size_t)field0 * c->S, h.size() * sizeof(Real), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int p = 0; p < n; ++p)
+        for (int f = 0; f < nfield; ++f) dst[(size_t)p * nfield + f] = double(h[(size_t)f * c->S + p]);
+    return DMPM_OK;
+}
+
+template <typename Real>
+int set_particles(dmpm_ctx* c, const double* x, const double* v, const double* F, const double* C, const double* m,
+                  const double* vol, const double* lam, const double* mu)
+{
+    if (!x || !v || !F || !C || !m || !lam || !mu) return fail(DMPM_ERR_INVALID, "null particle array");
+    Real* l0 = slot<Real>(c, 0);
+    int rc;
+    if ((rc = upload_fields<Real>(c, l0, FX, 3, x, c->N))) return rc;
+    if ((rc = upload_fields<Real>(c, l0, FV, 3, v, c->N))) return rc;
+    if ((rc = upload_fields<Real>(c, l0, FC, 9, C, c->N))) return rc;
+    if ((rc = upload_fields<Real>(c, l0, FF, 9, F, c->N))) return rc;
+    Real* cst = static_cast<Real*>(c->cst);
+    if ((rc = upload_fields<Real>(c, cst, KM, 1, m, c->N))) return rc;
+    if (vol) {
+        if ((rc = upload_fields<Real>(c, cst, KVOL, 1, vol, c->N))) return rc;
+    } else {
+        fill<Real>(c, cst + (size_t)KVOL * c->S, c->S, Real(0));
+    }
+    if ((rc = upload_fields<Real>(c, cst, KLAM, 1, lam, c->N))) return rc;
+    if ((rc = upload_fields<Real>(c, cst, KMU, 1, mu, c->N))) return rc;
+    c->have_particles = true;
+    c->fwd_valid = 0;
+    c->loss_valid = false;
+    c->grads_lo = -1;
+    c->tape_seg = -1;
+    return DMPM_OK;
+}
+
+template <typename Real>
+int compute_volumes(dmpm_ctx* c)
+{
+    if (!c->have_particles) return fail(DMPM_ERR_STATE, "volumes before dmpm_set_particles");
+    StepParams sp = step_params(c);
+    Real* gm = static_cast<Real*>(c->wgrid);
+    Real* cst = static_cast<Real*>(c->cst);
+    fill<Real>(c, gm, c->G, Real(0));
+    launch_p2g_mass<Real>(sp, slot<Real>(c, 0), cst + KM * c->S, c->S, gm, c->stream);
+    launch_volumes<Real>(sp, slot<Real>(c, 0), cst, gm, c->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->fwd_valid = std::min(c->fwd_valid, 0);
+    return DMPM_OK;
+}
+
+template <typename Real>
+int target_from_points(dmpm_ctx* c, const double* x, const double* m, int n)
+{
+    if (n < 0 || (n > 0 && (!x || !m))) return fail(DMPM_ERR_INVALID, "bad target point cloud");
+    StepParams sp = step_params(c);
+    sp.n = n;
+    Real* tg = static_cast<Real*>(c->target);
+    fill<Real>(c, tg, c->G, Real(0));
+    if (n > 0) {
+        void* buf = nullptr;
+        hipError_t e = hipMalloc(&buf, (size_t)4 * n * sizeof(Real));
+        if (e != hipSuccess) return fail(DMPM_ERR_NOMEM, std::string("target points: ") + hipGetErrorString(e));
+        std::vector<Real> h((size_t)4 * n);
+        for (int p = 0; p < n; ++p) {
+            for (int a = 0; a < 3; ++a) h[(size_t)a * n + p] = Real(x[3 * (size_t)p + a]);
+            h[(size_t)3 * n + p] = Real(m[p]);
+        }
+        Real* d = static_cast<Real*>(buf);
+        hipError_t e2 = hipMemcpyAsync(d, h.data(), h.size() * sizeof(Real), hipMemcpyHostToDevice, c->stream);
+        if (e2 == hipSuccess) launch_p2g_mass<Real>(sp, d, d + (size_t)3 * n, n, tg, c->stream);
+        hipError_t e3 = hipStreamSynchronize(c->stream);
+        hipFree(buf);
+        if (e2 != hipSuccess || e3 != hipSuccess)
+            return fail(DMPM_ERR_HIP, std::string("target P2G: ") + hipGetErrorString(e2 != hipSuccess ? e2 : e3));
+    }
+    HIP_TRY(hipGetLastError());
+    c->have_target = true;
+    c->loss_valid = false;
+    return DMPM_OK;
+}
+
+template <typename Real>
+int grid_download(dmpm_ctx* c, const Real* src, long long count, double* dst)
+{
+    std::vector<Real> h(count);
+    HIP_TRY(hipMemcpyAsync(h.data(), src, count * sizeof(Real), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (long long i = 0; i < count; ++i) dst[i] = double(h[i]);
+    return DMPM_OK;
+}
+
+template <typename Real>
+int grid_upload(dmpm_ctx* c, Real* dst, long long count, const double* src)
+{
+    std::vector<Real> h(count);
+    for (long long i = 0; i < count; ++i) h[i] = Real(src[i]);
+    HIP_TRY(hipMemcpyAsync(dst, h.data(), count * sizeof(Real), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return DMPM_OK;
+}
+
+template <typename Real>
+int get_layer_state(dmpm_ctx* c, int layer, double* x, double* v, double* F, double* C)
+{
+    if (layer < 0 || layer >= c->L) return fail(DMPM_ERR_INVALID, "layer out of range");
+    if (layer > c->fwd_valid) return fail(DMPM_ERR_STATE, "layer not computed by a forward pass");
+    Real* l = (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) ? slot<Real>(c, layer) : materialize<Real>(c, layer);
+    int rc;
+    if (x && (rc = download_fields<Real>(c, l, FX, 3, x, c->N))) return rc;
+    if (v && (rc = download_fields<Real>(c, l, FV, 3, v, c->N))) return rc;
+    if (F && (rc = download_fields<Real>(c, l, FF, 9, F, c->N))) return rc;
+    if (C && (rc = download_fields<Real>(c, l, FC, 9, C, c->N))) return rc;
+    return DMPM_OK;
+}
+
+template <typename Real>
+int set_layer_state(dmpm_ctx* c, int layer, const double* x, const double* v, const double* F, const double* C)
+{
+    if (layer < 0 || layer >= c->L) return fail(DMPM_ERR_INVALID, "layer out of range");
+    if (layer > 0 && c->cfg.tape_mode != DMPM_TAPE_STORE_ALL)
+        return fail(DMPM_ERR_STATE, "checkpoint tape: only layer 0 can be written");
+    if (layer > c->fwd_valid) return fail(DMPM_ERR_STATE, "layer not computed by a forward pass");
+    Real* l = slot<Real>(c, layer);
+    int rc;
+    if (x && (rc = upload_fields<Real>(c, l, FX, 3, x, c->N))) return rc;
+    if (v && (rc = upload_fields<Real>(c, l, FV, 3, v, c->N))) return rc;
+    if (F && (rc = upload_fields<Real>(c, l, FF, 9, F, c->N))) return rc;
+    if (C && (rc = upload_fields<Real>(c, l, FC, 9, C, c->N))) return rc;
+    c->fwd_valid = layer;
+    c->loss_valid = false;
+    c->grads_lo = -1;
+    if (c->cfg.tape_mode != DMPM_TAPE_STORE_ALL) c->tape_seg = -1;
+    return DMPM_OK;
+}
+
+template <typename Real>
+int get_layer_grads(dmpm_ctx* c, int layer, double* gx, double* gv, double* gF, double* gC)
+{
+    if (c->grads_lo < 0 || layer < c->grads_lo || layer > std::min(c->grads_lo + 1, c->L - 1))
+        return fail(DMPM_ERR_STATE, "gradients of this layer are not held (only the lowest layer of the last pass and the one above)");
+    Real* g = grads_of<Real>(c, layer);
+    int rc;
+    if (gx && (rc = download_fields<Real>(c, g, FX, 3, gx, c->N))) return rc;
+    if (gv && (rc = download_fields<Real>(c, g, FV, 3, gv, c->N))) return rc;
+    if (gF && (rc = download_fields<Real>(c, g, FF, 9, gF, c->N))) return rc;
+    if (gC && (rc = download_fields<Real>(c, g, FC, 9, gC, c->N))) return rc;
+    return DMPM_OK;
+}
+
+template <typename Real>
+int get_control_grad(dmpm_ctx* c, int layer, double* out)
+{
+    if (c->bwd_control < 0) return fail(DMPM_ERR_STATE, "no backward pass has run");
+    bool ok = c->cfg.keep_control_grads ? (layer >= c->bwd_control && layer <= c->L - 2) : (layer == c->bwd_control);
+    if (!ok) return fail(DMPM_ERR_STATE, "control gradient of this layer is not kept");
+    return download_fields<Real>(c, gctrl_of<Real>(c, layer), 0, 9, out, c->N);
+}
+
+template <typename Real>
+int chain_episode(dmpm_ctx* c)
+{
+    if (c->fwd_valid < c->L - 1) return fail(DMPM_ERR_STATE, "chain before the forward pass reached the last layer");
+    Real* last = (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) ? slot<Real>(c, c->L - 1) : materialize<Real>(c, c->L - 1);
+    HIP_TRY(hipMemcpyAsync(slot<Real>(c, 0), last, (size_t)kLayerFields * c->S * sizeof(Real), hipMemcpyDeviceToDevice, c->stream));
+    fill<Real>(c, static_cast<Real*>(c->ctrl), (long long)(c->L - 1) * 9 * c->S, Real(0));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->fwd_valid = 0;
+    c->loss_valid = false;
+    c->grads_lo = -1;
+    c->bwd_control = -1;
+    c->tape_seg = -1;
+    return DMPM_OK;
+}
+
+#define DISPATCH(ctx, fn, ...) ((ctx)->f32 ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
+
+int check_ctx(dmpm_ctx* c)
+{
+    if (!c) return fail(DMPM_ERR_INVALID, "null context");
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    return DMPM_OK;
+}
+
+}  // namespace
+
+// ================================================================ C ABI ====
+extern "C" {
+
+int dmpm_version(void) { return 100; }
+
+const char* dmpm_last_error(void) { return g_err.c_str(); }
+
+int dmpm_config_init(dmpm_config* cfg)
+{
+    if (!cfg) return fail(DMPM_ERR_INVALID, "null config");
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->precision = DMPM_PREC_F64;
+    cfg->num_layers = 2;
+    cfg->dx = 1.0;
+    cfg->dt = 1.0 / 120.0;
+    cfg->drag = 0.5;
+    cfg->smoothing = 0.1;
+    cfg->tape_mode = DMPM_TAPE_STORE_ALL;
+    return DMPM_OK;
+}
+
+int dmpm_create(const dmpm_config* cfg, dmpm_ctx** out)
+{
+    if (!cfg || !out) return fail(DMPM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (cfg->precision != DMPM_PREC_F64 && cfg->precision != DMPM_PREC_F32) return fail(DMPM_ERR_INVALID, "precision");
+    if (cfg->n_particles < 0) return fail(DMPM_ERR_INVALID, "n_particles < 0");
+    if (cfg->dim[0] <= 0 || cfg->dim[1] <= 0 || cfg->dim[2] <= 0) return fail(DMPM_ERR_INVALID, "grid dims must be > 0");
+    if (!(cfg->dx > 0.0)) return fail(DMPM_ERR_INVALID, "dx must be > 0");
+    if (cfg->num_layers < 2) return fail(DMPM_ERR_INVALID, "num_layers must be >= 2");
+    if (cfg->tape_mode != DMPM_TAPE_STORE_ALL && cfg->tape_mode != DMPM_TAPE_CHECKPOINT) return fail(DMPM_ERR_INVALID, "tape_mode");
+    if (cfg->compat < 0 || cfg->compat > DMPM_COMPAT_REFERENCE) return fail(DMPM_ERR_INVALID, "compat bits");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (cfg->device < 0 || cfg->device >= ndev) return fail(DMPM_ERR_INVALID, "device ordinal out of range");
+    HIP_TRY(hipSetDevice(cfg->device));
+
+    dmpm_ctx* c = new dmpm_ctx();
+    c->cfg = *cfg;
+    c->f32 = cfg->precision == DMPM_PREC_F32;
+    c->esz = c->f32 ? 4 : 8;
+    c->N = cfg->n_particles;
+    c->S = ((c->N + 63) / 64) * 64;
+    if (c->S == 0) c->S = 64;
+    c->L = cfg->num_layers;
+    c->G = (long long)cfg->dim[0] * cfg->dim[1] * cfg->dim[2];
+    const int steps = c->L - 1;
+    if (cfg->tape_mode == DMPM_TAPE_CHECKPOINT) {
+        int K = cfg->ckpt_stride > 0 ? cfg->ckpt_stride : (int)std::lround(std::sqrt((double)steps));
+        if (K < 1) K = 1;
+        if (K > steps) K = steps;
+        c->K = K;
+        c->n_ckpt = (c->L - 1) / K + 1;  // layers 0, K, 2K, ... <= L-1
+        c->seg_last = (c->L - 2) / K;
+        c->tape_base = c->n_ckpt;         // K+1 tape slots (slot 0 unused: the segment start is a checkpoint)
+        c->roll_base = c->tape_base + K + 1;
+        c->n_slots = c->roll_base + 2;
+    } else {
+        c->K = steps;
+        c->n_slots = c->L;
+    }
+    const size_t e = c->esz;
+    const size_t S = c->S;
+    const size_t G = c->G;
+    int rc = DMPM_OK;
+    hipError_t he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        delete c;
+        return fail(DMPM_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
+    }
+    const size_t grid_slots = cfg->tape_mode == DMPM_TAPE_CHECKPOINT ? (size_t)c->K : (size_t)steps;
+    const size_t gctrl_layers = cfg->keep_control_grads ? (size_t)steps : 1;
+    if ((rc = dalloc(c, &c->cst, kConstFields * S * e)) || (rc = dalloc(c, &c->ctrl, (size_t)steps * 9 * S * e)) ||
+        (rc = dalloc(c, &c->layers, (size_t)c->n_slots * kLayerFields * S * e)) ||
+        (rc = dalloc(c, &c->grids, grid_slots * kGridFields * G * e)) || (rc = dalloc(c, &c->wgrid, kGridFields * G * e)) ||
+        (rc = dalloc(c, &c->agrid, kGridFields * G * e)) || (rc = dalloc(c, &c->gbuf[0], kLayerFields * S * e)) ||
+        (rc = dalloc(c, &c->gbuf[1], kLayerFields * S * e)) || (rc = dalloc(c, &c->gctrl, gctrl_layers * 9 * S * e)) ||
+        (rc = dalloc(c, &c->tmp, 12 * S * e)) || (rc = dalloc(c, &c->target, G * e)) ||
+        (rc = dalloc(c, (void**)&c->dacc, 64 * sizeof(double)))) {
+        std::string msg = g_err;
+        dmpm_destroy(c);
+        return fail(rc, msg);
+    }
+    // zero controls and layer 0 so a context is well defined before uploads
+    if (c->f32) {
+        launch_fill<float>(static_cast<float*>(c->ctrl), (long long)steps * 9 * S, 0.f, c->stream);
+        launch_fill<float>(static_cast<float*>(c->layers), (long long)kLayerFields * S, 0.f, c->stream);
+        launch_fill<float>(static_cast<float*>(c->cst), (long long)kConstFields * S, 0.f, c->stream);
+    } else {
+        launch_fill<double>(static_cast<double*>(c->ctrl), (long long)steps * 9 * S, 0.0, c->stream);
+        launch_fill<double>(static_cast<double*>(c->layers), (long long)kLayerFields * S, 0.0, c->stream);
+        launch_fill<double>(static_cast<double*>(c->cst), (long long)kConstFields * S, 0.0, c->stream);
+    }
+    he = hipStreamSynchronize(c->stream);
+    if (he != hipSuccess) {
+        dmpm_destroy(c);
+        return fail(DMPM_ERR_HIP, std::string("init: ") + hipGetErrorString(he));
+    }
+    *out = c;
+    return DMPM_OK;
+}
+
+int dmpm_destroy(dmpm_ctx* c)
+{
+    if (!c) return DMPM_OK;
+    hipSetDevice(c->cfg.device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    void* bufs[] = {c->cst, c->ctrl, c->layers, c->grids, c->wgrid, c->agrid, c->gbuf[0], c->gbuf[1], c->gctrl, c->tmp, c->target, c->dacc};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+    for (hipEvent_t ev : c->pool) hipEventDestroy(ev);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return DMPM_OK;
+}
+
+int dmpm_set_particles(dmpm_ctx* c, const double* x, const double* v, const double* F, const double* C,
+                       const double* m, const double* vol, const double* lam, const double* mu)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    return DISPATCH(c, set_particles, c, x, v, F, C, m, vol, lam, mu);
+}
+
+int dmpm_compute_volumes(dmpm_ctx* c)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    return DISPATCH(c, compute_volumes, c);
+}
+
+int dmpm_get_volumes(dmpm_ctx* c, double* vol)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!vol) return fail(DMPM_ERR_INVALID, "null output");
+    if (c->f32) return download_fields<float>(c, static_cast<float*>(c->cst), KVOL, 1, vol, c->N);
+    return download_fields<double>(c, static_cast<double*>(c->cst), KVOL, 1, vol, c->N);
+}
+
+int dmpm_set_control(dmpm_ctx* c, int32_t layer, const double* dFc)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!dFc) return fail(DMPM_ERR_INVALID, "null control");
+    if (layer < -1 || layer > c->L - 2) return fail(DMPM_ERR_INVALID, "control layer out of range");
+    const int lo = layer < 0 ? 0 : layer, hi = layer < 0 ? c->L - 2 : layer;
+    for (int l = lo; l <= hi; ++l) {
+        rc = c->f32 ? upload_fields<float>(c, ctrl_of<float>(c, l), 0, 9, dFc, c->N)
+                    : upload_fields<double>(c, ctrl_of<double>(c, l), 0, 9, dFc, c->N);
+        if (rc) return rc;
+    }
+    c->fwd_valid = std::min(c->fwd_valid, lo);
+    c->loss_valid = false;
+    if (c->cfg.tape_mode != DMPM_TAPE_STORE_ALL && c->tape_seg >= 0 && c->tape_seg * c->K + c->K > lo) c->tape_seg = -1;
+    return DMPM_OK;
+}
+
+int dmpm_get_control(dmpm_ctx* c, int32_t layer, double* dFc)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!dFc) return fail(DMPM_ERR_INVALID, "null output");
+    if (layer < 0 || layer > c->L - 2) return fail(DMPM_ERR_INVALID, "control layer out of range");
+    return c->f32 ? download_fields<float>(c, ctrl_of<float>(c, layer), 0, 9, dFc, c->N)
+                  : download_fields<double>(c, ctrl_of<double>(c, layer), 0, 9, dFc, c->N);
+}
+
+int dmpm_set_target_grid(dmpm_ctx* c, const double* mass)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!mass) return fail(DMPM_ERR_INVALID, "null target");
+    rc = c->f32 ? grid_upload<float>(c, static_cast<float*>(c->target), c->G, mass)
+                : grid_upload<double>(c, static_cast<double*>(c->target), c->G, mass);
+    if (rc) return rc;
+    c->have_target = true;
+    c->loss_valid = false;
+    return DMPM_OK;
+}
+
+int dmpm_target_from_points(dmpm_ctx* c, const double* x, const double* m, int32_t n)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    return DISPATCH(c, target_from_points, c, x, m, n);
+}
+
+int dmpm_get_target_grid(dmpm_ctx* c, double* mass)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!mass) return fail(DMPM_ERR_INVALID, "null output");
+    return c->f32 ? grid_download<float>(c, static_cast<float*>(c->target), c->G, mass)
+                  : grid_download<double>(c, static_cast<double*>(c->target), c->G, mass);
+}
+
+int dmpm_forward_pass(dmpm_ctx* c, int32_t start)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    return DISPATCH(c, forward_pass, c, start);
+}
+
+int dmpm_end_layer_mass_loss(dmpm_ctx* c, double* loss)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    return DISPATCH(c, end_loss, c, loss);
+}
+
+int dmpm_backward_pass(dmpm_ctx* c, int32_t control)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    return DISPATCH(c, backward_pass, c, control);
+}
+
+int dmpm_forward_step(dmpm_ctx* c, int32_t layer)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->cfg.tape_mode != DMPM_TAPE_STORE_ALL) return fail(DMPM_ERR_STATE, "single steps need the store-all tape");
+    if (layer < 0 || layer > c->L - 2) return fail(DMPM_ERR_INVALID, "layer out of range");
+    if (layer > c->fwd_valid) return fail(DMPM_ERR_STATE, "input layer not computed");
+    if (c->f32)
+        step_forward<float>(c, slot<float>(c, layer), slot<float>(c, layer + 1), ctrl_of<float>(c, layer), grid_of_step<float>(c, layer));
+    else
+        step_forward<double>(c, slot<double>(c, layer), slot<double>(c, layer + 1), ctrl_of<double>(c, layer), grid_of_step<double>(c, layer));
+    HIP_TRY(hipGetLastError());
+    c->fwd_valid = layer + 1;
+    c->loss_valid = false;
+    return DMPM_OK;
+}
+
+int dmpm_backward_step(dmpm_ctx* c, int32_t layer)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->cfg.tape_mode != DMPM_TAPE_STORE_ALL) return fail(DMPM_ERR_STATE, "single steps need the store-all tape");
+    if (layer < 0 || layer > c->L - 2) return fail(DMPM_ERR_INVALID, "layer out of range");
+    if (c->grads_lo != layer + 1) return fail(DMPM_ERR_STATE, "gradients of layer+1 are not the current ones");
+    if (layer + 1 > c->fwd_valid) return fail(DMPM_ERR_STATE, "forward layers missing");
+    if (c->f32)
+        step_backward<float>(c, layer, slot<float>(c, layer), slot<float>(c, layer + 1), grid_of_step<float>(c, layer));
+    else
+        step_backward<double>(c, layer, slot<double>(c, layer), slot<double>(c, layer + 1), grid_of_step<double>(c, layer));
+    HIP_TRY(hipGetLastError());
+    c->grads_lo = layer;
+    c->bwd_control = layer;
+    return DMPM_OK;
+}
+
+int dmpm_get_layer_state(dmpm_ctx* c, int32_t layer, double* x, double* v, double* F, double* C)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    return DISPATCH(c, get_layer_state, c, layer, x, v, F, C);
+}
+
+int dmpm_set_layer_state(dmpm_ctx* c, int32_t layer, const double* x, const double* v, const double* F, const double* C)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    return DISPATCH(c, set_layer_state, c, layer, x, v, F, C);
+}
+
+int dmpm_get_layer_grads(dmpm_ctx* c, int32_t layer, double* dLdx, double* dLdv, double* dLdF, double* dLdC)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    return DISPATCH(c, get_layer_grads, c, layer, dLdx, dLdv, dLdF, dLdC);
+}
+
+int dmpm_get_control_grad(dmpm_ctx* c, int32_t layer, double* out)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!out) return fail(DMPM_ERR_INVALID, "null output");
+    return DISPATCH(c, get_control_grad, c, layer, out);
+}
+
+int dmpm_control_grad_norm(dmpm_ctx* c, int32_t layer, double* norm)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!norm) return fail(DMPM_ERR_INVALID, "null output");
+    std::vector<double> g((size_t)9 * c->N);
+    rc = DISPATCH(c, get_control_grad, c, layer, g.data());
+    if (rc) return rc;
+    double s = 0.0;
+    for (double v : g) s += v * v;
+    *norm = std::sqrt(s);
+    return DMPM_OK;
+}
+
+int dmpm_get_grid(dmpm_ctx* c, int32_t step, double* m, double* v)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->cfg.tape_mode != DMPM_TAPE_STORE_ALL) return fail(DMPM_ERR_STATE, "grids are kept by the store-all tape only");
+    if (step < 0 || step > c->L - 2 || step + 1 > c->fwd_valid) return fail(DMPM_ERR_STATE, "grid of this step not computed");
+    if (c->f32) {
+        float* g = grid_of_step<float>(c, step);
+        if (m && (rc = grid_download<float>(c, g, c->G, m))) return rc;
+        if (v) {
+            std::vector<double> t((size_t)3 * c->G);
+            if ((rc = grid_download<float>(c, g + c->G, 3 * c->G, t.data()))) return rc;
+            for (long long i = 0; i < c->G; ++i)
+                for (int a = 0; a < 3; ++a) v[3 * i + a] = t[(size_t)a * c->G + i];
+        }
+    } else {
+        double* g = grid_of_step<double>(c, step);
+        if (m && (rc = grid_download<double>(c, g, c->G, m))) return rc;
+        if (v) {
+            std::vector<double> t((size_t)3 * c->G);
+            if ((rc = grid_download<double>(c, g + c->G, 3 * c->G, t.data()))) return rc;
+            for (long long i = 0; i < c->G; ++i)
+                for (int a = 0; a < 3; ++a) v[3 * i + a] = t[(size_t)a * c->G + i];
+        }
+    }
+    return DMPM_OK;
+}
+
+int dmpm_chain_episode(dmpm_ctx* c)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    return DISPATCH(c, chain_episode, c);
+}
+
+int dmpm_synchronize(dmpm_ctx* c)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return DMPM_OK;
+}
+
+int dmpm_memory_bytes(dmpm_ctx* c, uint64_t* bytes)
+{
+    if (!c || !bytes) return fail(DMPM_ERR_INVALID, "null argument");
+    *bytes = c->bytes;
+    return DMPM_OK;
+}
+
+int dmpm_profile_enable(dmpm_ctx* c, int32_t on)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!on && c->prof) {
+        rc = drain_profile(c);
+        if (rc) return rc;
+    }
+    c->prof = on != 0;
+    return DMPM_OK;
+}
+
+int dmpm_profile_reset(dmpm_ctx* c)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    rc = drain_profile(c);
+    if (rc) return rc;
+    for (int k = 0; k < DMPM_K_COUNT; ++k) {
+        c->prof_ms[k] = 0.0;
+        c->prof_n[k] = 0;
+    }
+    return DMPM_OK;
+}
+
+int dmpm_profile_read(dmpm_ctx* c, int32_t kernel, double* total_ms, int64_t* launches)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (kernel < 0 || kernel >= DMPM_K_COUNT) return fail(DMPM_ERR_INVALID, "kernel id");
+    rc = drain_profile(c);
+    if (rc) return rc;
+    if (total_ms) *total_ms = c->prof_ms[kernel];
+    if (launches) *launches = c->prof_n[kernel];
+    return DMPM_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------ batched evaluators --
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf()
+    {
+        if (p) hipFree(p);
+    }
+};
+
+int eval_common(int n, std::initializer_list<std::pair<const double*, size_t>> ins,
+                std::initializer_list<std::pair<double*, size_t>> outs,
+                const std::function<void(int, std::vector<double*>&)>& launch)
+{
+    if (n < 0) return fail(DMPM_ERR_INVALID, "n < 0");
+    if (n == 0) return DMPM_OK;
+    HIP_TRY(hipSetDevice(0));
+    std::vector<DevBuf> bufs(ins.size() + outs.size());
+    std::vector<double*> ptrs;
+    size_t k = 0;
+    for (auto& in : ins) {
+        if (!in.first) return fail(DMPM_ERR_INVALID, "null input");
+        HIP_TRY(hipMalloc(&bufs[k].p, in.second * n * sizeof(double)));
+        HIP_TRY(hipMemcpy(bufs[k].p, in.first, in.second * n * sizeof(double), hipMemcpyHostToDevice));
+        ptrs.push_back(static_cast<double*>(bufs[k].p));
+        ++k;
+    }
+    for (auto& o : outs) {
+        if (!o.first) return fail(DMPM_ERR_INVALID, "null output");
+        HIP_TRY(hipMalloc(&bufs[k].p, o.second * n * sizeof(double)));
+        ptrs.push_back(static_cast<double*>(bufs[k].p));
+        ++k;
+    }
+    launch(n, ptrs);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    k = ins.size();
+    for (auto& o : outs) {
+        HIP_TRY(hipMemcpy(o.first, ptrs[k], o.second * n * sizeof(double), hipMemcpyDeviceToHost));
+        ++k;
+    }
+    return DMPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dmpm_eval_pk1(int32_t n, const double* F, const double* lam, const double* mu, double* P)
+{
+    return eval_common(n, {{F, 9}, {lam, 1}, {mu, 1}}, {{P, 9}}, [](int n_, std::vector<double*>& p) {
+        launch_eval_pk1(n_, p[0], p[1], p[2], p[3], nullptr);
+    });
+}
+
+int dmpm_eval_hessian(int32_t n, const double* F, const double* lam, const double* mu, const double* dF, double* out,
+                      int32_t exact)
+{
+    return eval_common(n, {{F, 9}, {lam, 1}, {mu, 1}, {dF, 9}}, {{out, 9}}, [exact](int n_, std::vector<double*>& p) {
+        launch_eval_hess(n_, p[0], p[1], p[2], p[3], p[4], exact, nullptr);
+    });
+}
+
+int dmpm_eval_svd(int32_t n, const double* F, double* U, double* sigma, double* V)
+{
+    return eval_common(n, {{F, 9}}, {{U, 9}, {sigma, 3}, {V, 9}}, [](int n_, std::vector<double*>& p) {
+        launch_eval_svd(n_, p[0], p[1], p[2], p[3], nullptr);
+    });
+}
+
+}  // extern "C"

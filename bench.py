@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Forward+adjoint particle-steps/s of the differentiable MLS-MPM step on one MI355X.
+
+Workload (default ``morph_1m_128``): 1M particles in a seeded sphere, 128^3
+grid (dx = 1/128), E log-uniform in [5e3, 5e4], nu = 0.3, rho = 1, 100-step
+episodes (101 layers), target the P2G mass of a seeded 12-sphere union, seeded
+non-rest control dFc ~ N(0, 1e-2) on layer 0.  One bench step is one
+gradient evaluation of an episode: ComputeForwardPass(0) + EndLayerMassLoss +
+ComputeBackwardPass(0).  particle-steps/s = particles x 100 x steps / time.
+
+Timed paths: FP32 state with FP64 polar/SVD/H:dF and checkpoint+recompute
+(the reported value) and the plain FP64 path with every layer stored; the
+control-gradient error is FP32 vs FP64 on the same inputs.  The CPU baseline is
+the FP64 oracle (a port) on a bounded sample of the same scene.
+
+Stability: the scene's own dt (2e-4) with the reference's dimensionless
+particle volume exceeds the explicit CFL limit many times over; the bench uses
+volumes scaled by dx^3 and dt = 1e-5 (CFL ~0.33 at E = 5e4).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from __graft_entry__ import _load_pkg  # noqa: E402
+
+# Algorithmic HBM bytes per FP32 particle / per touched node / per dense grid node
+# of each kernel (FP64: twice).  Particle fields: x3 v3 C9 F9 dFc9 + m vol lam mu.
+KBYTES = {
+    "p2g": (148, 16, 0),        # read x,v,C,F,dFc,m,vol,lam,mu; m,p scatter per node
+    "grid_update": (0, 0, 28),  # read m,p write v per node (dense)
+    "g2p": (180, 12, 0),        # read x,F,dFc write x,v,C,F; gather v
+    "g2p_adj": (228, 12, 0),    # read x,F,dFc,x',v',F',C' write vt,Ct; scatter vbar
+    "grid_adj": (0, 0, 44),     # read m,v,vbar write pbar,mbar (dense)
+    "p2g_adj": (412, 28, 0),    # read x,v,C,F,dFc,consts,C',F',x',vt,Ct write grads; gather pbar,mbar,v
+}
+HBM_PEAK_GBS = 8000.0
+
+
+def touched_nodes(sc):
+    """Nodes inside the 4^3 stencil of at least one particle of layer 0."""
+    mark = np.zeros(sc.dim, dtype=bool)
+    base = np.floor((sc.x - np.asarray(sc.min_point)) / sc.dx).astype(np.int64) - 1
+    for i in range(4):
+        for j in range(4):
+            for k in range(4):
+                idx = base + np.array([i, j, k])
+                ok = np.all((idx >= 0) & (idx < np.asarray(sc.dim)), axis=1)
+                mark[idx[ok, 0], idx[ok, 1], idx[ok, 2]] = True
+    return int(mark.sum())
+
+
+def make_scene(pkg, name, dt):
+    S = pkg.scenes
+    if name == "morph_1m_128":
+        return S.morph_1m(dt=dt)
+    if name == "morph_130k_64":
+        return S.morph_130k(dt=dt)
+    if name == "morph_8k_32":
+        return S.morph_small(dt=dt)
+    if name == "morph_4m_256":
+        return S.morph_4m(dt=dt)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def setup(pkg, sc, vol, dFc, precision, tape, num_layers):
+    mo = pkg.Morph.from_scene(sc, precision=precision, tape=tape, num_layers=num_layers)
+    x, v, C, F = sc.state0()
+    mo.set_particles(x, v, F, C, sc.m, vol, sc.lam, sc.mu)
+    mo.set_control(0, dFc)
+    mo.target_from_points(sc.target_x, sc.target_m)
+    return mo
+
+
+def episode(mo):
+    mo.forward_pass(0)
+    loss = mo.end_layer_mass_loss()
+    mo.backward_pass(0)
+    return loss
+
+
+def run_path(pkg, sc, vol, dFc, precision, tape, L, warmup, steps, profile):
+    mo = setup(pkg, sc, vol, dFc, precision, tape, L)
+    for _ in range(warmup):
+        episode(mo)
+    mo.synchronize()
+    if profile:
+        mo.profile(True)
+        mo.profile_reset()
+    mo.synchronize()
+    t0 = time.perf_counter()
+    loss = None
+    for _ in range(steps):
+        loss = episode(mo)
+    mo.synchronize()
+    dt = time.perf_counter() - t0
+    prof = mo.profile_read() if profile else None
+    if profile:
+        mo.profile(False)
+    grad = mo.control_grad(0)
+    mem = mo.memory_bytes()
+    mo.close()
+    return dict(time=dt, loss=loss, grad=grad, prof=prof, mem=mem)
+
+
+def cpu_baseline(sc, vol, dFc, threads, steps):
+    from oracle import oracle as orc
+
+    orc.set_threads(threads)
+    prm = orc.Params(n=sc.n, dim=sc.dim, dx=sc.dx, min_point=sc.min_point, dt=sc.dt)
+    x, v, C, F = sc.state0()
+    st = orc.State(x, v, C, F)
+    cs = orc.Consts(sc.m, vol, sc.lam, sc.mu)
+    target = orc.p2g_mass(prm, sc.target_x, sc.target_m)
+    controls = [dFc] + [np.zeros_like(dFc)] * (steps - 1)
+    t0 = time.perf_counter()
+    orc.episode(prm, st, cs, controls, target, steps + 1)
+    dt = time.perf_counter() - t0
+    return sc.n * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="morph_1m_128")
+    ap.add_argument("--episode-steps", type=int, default=100)
+    ap.add_argument("--dt", type=float, default=1e-5)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a design note only)"}))
+        sys.exit(2)
+
+    pkg = _load_pkg()
+    sc = make_scene(pkg, args.workload, args.dt)
+    L = args.episode_steps + 1
+    # reference-formula volumes (computed on the GPU) scaled by dx^3
+    mo = pkg.Morph.from_scene(sc, num_layers=2)
+    x, v, C, F = sc.state0()
+    mo.set_particles(x, v, F, C, sc.m, None, sc.lam, sc.mu)
+    mo.compute_volumes()
+    vol = mo.volumes() * sc.dx ** 3
+    mo.close()
+    dFc = pkg.scenes.random_controls(sc.n, 1e-2, 1)
+    touched = touched_nodes(sc)
+    print(f"# workload {sc.name}: n={sc.n} grid={sc.dim} L={L} touched_nodes={touched}", file=sys.stderr, flush=True)
+
+    r32 = run_path(pkg, sc, vol, dFc, pkg.PREC_F32, pkg.TAPE_CHECKPOINT, L, args.warmup, args.steps, True)
+    value = sc.n * args.episode_steps * args.steps / r32["time"]
+    print(f"# fp32+ckpt: {r32['time'] / args.steps * 1e3:.2f} ms/episode, {value:.4e} particle-steps/s", file=sys.stderr, flush=True)
+
+    r64 = None
+    if not args.no_fp64:
+        r64 = run_path(pkg, sc, vol, dFc, pkg.PREC_F64, pkg.TAPE_STORE_ALL, L, 1, args.fp64_steps, False)
+        v64 = sc.n * args.episode_steps * args.fp64_steps / r64["time"]
+        print(f"# fp64 plain: {r64['time'] / args.fp64_steps * 1e3:.2f} ms/episode, {v64:.4e} particle-steps/s", file=sys.stderr, flush=True)
+
+    # roofline of the dominant kernel
+    prof = r32["prof"]
+    dom = max((k for k in KBYTES), key=lambda k: prof[k][0])
+    ms, launches = prof[dom]
+    pb, nb, gb = KBYTES[dom]
+    G = sc.G
+    bytes_per_launch = pb * sc.n + nb * touched + gb * G
+    avg_s = ms / 1e3 / max(launches, 1)
+    achieved = bytes_per_launch / avg_s / 1e9
+    kernels = {k: {"ms_total": round(prof[k][0], 3), "launches": prof[k][1],
+                   "avg_us": round(prof[k][0] * 1e3 / max(prof[k][1], 1), 2)} for k in prof}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        cv, cdt = cpu_baseline(sc, vol, dFc, threads, args.cpu_steps)
+        cpu = {"value": cv, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": f"FP64 oracle, one {args.cpu_steps}-step episode fwd+loss+adjoint of {sc.name} ({cdt:.1f} s)"}
+
+    out = {
+        "metric": "fwd+adjoint particle-steps/s (FP32+ckpt vs plain FP64) at 1M pts; grad err",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["time"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{sc.name}: {sc.n} particles, {sc.dim[0]}^3 grid, {args.episode_steps}-step episodes, "
+                               f"FP32 state + FP64 polar/SVD, checkpoint+recompute; step = one fwd+loss+adjoint episode",
+                   "dt": args.dt, "volume": "reference formula x dx^3", "control": "dFc ~ N(0,1e-2) on layer 0"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu,
+        "kernels_fp32": kernels,
+        "touched_nodes": touched,
+        "memory_fp32_ckpt_bytes": r32["mem"],
+    }
+    if r64 is not None:
+        v64 = sc.n * args.episode_steps * args.fp64_steps / r64["time"]
+        g32, g64 = r32["grad"], r64["grad"]
+        out["fp64_plain"] = {"value": v64, "ms_per_step": r64["time"] / args.fp64_steps * 1e3, "memory_bytes": r64["mem"],
+                             "loss": r64["loss"]}
+        out["speedup_fp32_vs_fp64"] = value / v64
+        out["grad_rel_err_fp32_vs_fp64"] = float(np.linalg.norm(g32 - g64) / max(np.linalg.norm(g64), 1e-300))
+        out["loss_fp32"] = r32["loss"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

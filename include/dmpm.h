@@ -1,0 +1,159 @@
+/*
+ * dmpm.h -- C ABI of the MI355X-native differentiable MLS-MPM step.
+ *
+ * The library replaces the hot path of DiffMPMLib3D (paths relative to the
+ * reference root): the forward step, the end-layer mass loss and the
+ * adjoint step, driven over one episode of layers.  Each entry point names
+ * the reference interface it stands in for.  Host arrays are row-major AoS
+ * doubles: x, v are [n][3]; F, C, dFc are [n][3][3] with (i,j) at 3*i+j;
+ * grids are flat [dim_x*dim_y*dim_z] with index i*dy*dz + j*dz + k
+ * (Grid.h:22-31) and node position min_point + (i,j,k)*dx (Grid.cpp:22).
+ *
+ * Every call returns DMPM_OK (0) or a negative error code; dmpm_last_error()
+ * returns the message of the calling thread's last failure.  A context owns
+ * one HIP stream; calls on one context are serialised on it, distinct
+ * contexts are independent.  Calls that return host data synchronise.
+ */
+#ifndef DMPM_H
+#define DMPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMPM_OK 0
+#define DMPM_ERR_INVALID (-1) /* bad argument or configuration */
+#define DMPM_ERR_HIP (-2)     /* HIP runtime failure (message has the HIP error) */
+#define DMPM_ERR_STATE (-3)   /* call order: e.g. backward before the loss */
+#define DMPM_ERR_NOMEM (-4)   /* device allocation failed */
+
+/* precision of particle and grid state */
+#define DMPM_PREC_F64 0 /* plain FP64 path: the drop-in */
+#define DMPM_PREC_F32 1 /* FP32 state, FP64 polar/SVD, PK1, H:dF and reductions */
+
+/* tape: how the forward trajectory is kept for the adjoint */
+#define DMPM_TAPE_STORE_ALL 0  /* every layer and grid (the reference's CompGraph, CompGraph.cpp:7-17) */
+#define DMPM_TAPE_CHECKPOINT 1 /* checkpoints every ckpt_stride layers + one recomputed segment */
+
+/* reference-compat switches for the adjoint (BackPropagation.cpp), 0 = exact adjoint */
+#define DMPM_COMPAT_DUP_FBAR 1     /* (I+dtC')^T dLdF' added twice, :20 and :121 */
+#define DMPM_COMPAT_NO_1MS 2       /* dLdC_next without (1-s), :24 */
+#define DMPM_COMPAT_NO_DRAG_X 4    /* x-gradient uses m v, not m(1-dt drag) v, :109 */
+#define DMPM_COMPAT_DLDM_NO_FEXT 8 /* dLdm uses v, not v - dt f_ext, :57 */
+#define DMPM_COMPAT_REFERENCE 15
+
+/* kernel ids for the profiling counters */
+#define DMPM_K_P2G 0
+#define DMPM_K_GRID_UPDATE 1
+#define DMPM_K_G2P 2
+#define DMPM_K_G2P_ADJ 3
+#define DMPM_K_GRID_ADJ 4
+#define DMPM_K_P2G_ADJ 5
+#define DMPM_K_LOSS 6
+#define DMPM_K_FILL 7
+#define DMPM_K_COUNT 8
+
+typedef struct dmpm_ctx dmpm_ctx;
+
+typedef struct dmpm_config {
+    int32_t precision;          /* DMPM_PREC_* */
+    int32_t n_particles;
+    int32_t dim[3];             /* grid nodes per axis (Grid::dim_x/y/z) */
+    int32_t num_layers;         /* layers per episode, >= 2 (CompGraph::SetUpCompGraph num_layers) */
+    double dx;                  /* Grid::dx */
+    double min_point[3];        /* Grid::min_point */
+    double dt, drag, smoothing; /* CompGraph.h:55-58 */
+    double f_ext[3];
+    int32_t compat;             /* DMPM_COMPAT_* bits */
+    int32_t tape_mode;          /* DMPM_TAPE_* */
+    int32_t ckpt_stride;        /* checkpoint mode: layers between checkpoints, 0 = round(sqrt(L-1)) */
+    int32_t keep_control_grads; /* 1: keep dL/d(dFc_i) of every layer of the backward pass */
+    int32_t device;             /* HIP device ordinal */
+    int32_t reserved;
+} dmpm_config;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int dmpm_version(void);
+const char* dmpm_last_error(void);
+
+/* Defaults of CompGraph.h:55-58 (dt = 1/120, drag = 0.5, s = 0.1, f_ext = 0), FP64, store-all. */
+int dmpm_config_init(dmpm_config* cfg);
+
+/* Lifetime: replaces CompGraph::CompGraph (CompGraph.cpp:269-278) + SetUpCompGraph (:7-17). */
+int dmpm_create(const dmpm_config* cfg, dmpm_ctx** out);
+int dmpm_destroy(dmpm_ctx* ctx);
+
+/* Layer 0 particles: MaterialPoint x, v, F, C, m, vol, lam, mu (MaterialPoint.h:18-28).
+ * vol may be NULL (then call dmpm_compute_volumes).  Invalidates every later layer. */
+int dmpm_set_particles(dmpm_ctx* ctx, const double* x, const double* v, const double* F, const double* C,
+                       const double* m, const double* vol, const double* lam, const double* mu);
+/* SingleThreadMPM::CalculatePointCloudVolumes (ForwardSimulation.cpp:262-290) on layer 0. */
+int dmpm_compute_volumes(dmpm_ctx* ctx);
+int dmpm_get_volumes(dmpm_ctx* ctx, double* vol);
+
+/* Control dFc of one layer (MaterialPoint::dFc, MaterialPoint.h:31); layer -1 sets all layers,
+ * as SetUpCompGraph's copy of layer 0 does.  Invalidates the layers after it. */
+int dmpm_set_control(dmpm_ctx* ctx, int32_t layer, const double* dFc);
+int dmpm_get_control(dmpm_ctx* ctx, int32_t layer, double* dFc);
+
+/* Target grid of EndLayerMassLoss (CompGraph::target_grid). */
+int dmpm_set_target_grid(dmpm_ctx* ctx, const double* mass);
+/* Target grid as the P2G mass of a point cloud (MPMPolyscope.cpp:156-160). */
+int dmpm_target_from_points(dmpm_ctx* ctx, const double* x, const double* m, int32_t n);
+int dmpm_get_target_grid(dmpm_ctx* ctx, double* mass);
+
+/* CompGraph::ComputeForwardPass(start_layer) (CompGraph.cpp:246-256). */
+int dmpm_forward_pass(dmpm_ctx* ctx, int32_t start_layer);
+/* CompGraph::EndLayerMassLoss (CompGraph.cpp:110-244): loss and the adjoint seed. */
+int dmpm_end_layer_mass_loss(dmpm_ctx* ctx, double* loss);
+/* CompGraph::ComputeBackwardPass(control_layer) (CompGraph.cpp:258-267). */
+int dmpm_backward_pass(dmpm_ctx* ctx, int32_t control_layer);
+
+/* One SingleThreadMPM::ForwardTimeStep layer -> layer+1 (ForwardSimulation.h:18) and one
+ * Back_Timestep layer+1 -> layer (BackPropagation.h:7); store-all tape only. */
+int dmpm_forward_step(dmpm_ctx* ctx, int32_t layer);
+int dmpm_backward_step(dmpm_ctx* ctx, int32_t layer);
+
+/* State of a layer (any pointer may be NULL); checkpoint mode recomputes it when needed. */
+int dmpm_get_layer_state(dmpm_ctx* ctx, int32_t layer, double* x, double* v, double* F, double* C);
+/* Store-all tape: replace a computed layer's state (the C++ host mirror's write-back). */
+int dmpm_set_layer_state(dmpm_ctx* ctx, int32_t layer, const double* x, const double* v, const double* F,
+                         const double* C);
+/* Gradients dLdx, dLdv, dLdF, dLdC (MaterialPoint.h:34-37) of the lowest layer reached by the last
+ * backward pass or of the layer above it (the last layer after the loss). */
+int dmpm_get_layer_grads(dmpm_ctx* ctx, int32_t layer, double* dLdx, double* dLdv, double* dLdF, double* dLdC);
+/* dL/d(dFc_layer): the control layer of the last backward pass, or any of its layers with
+ * keep_control_grads.  Exact mode: dL/d(dFc); compat mode: the reference's dLdF. */
+int dmpm_get_control_grad(dmpm_ctx* ctx, int32_t layer, double* out);
+/* PointCloud::Compute_dLdF_Norm (PointCloud.cpp:24-35) of a control gradient, FP64 reduction. */
+int dmpm_control_grad_norm(dmpm_ctx* ctx, int32_t layer, double* norm);
+/* Grid of one step after G_op (node m and v), store-all tape. */
+int dmpm_get_grid(dmpm_ctx* ctx, int32_t step, double* m, double* v);
+
+/* Episode chaining of DiffMPMAnimator3D.cpp:424-429: layer 0 <- last layer, controls <- 0. */
+int dmpm_chain_episode(dmpm_ctx* ctx);
+
+int dmpm_synchronize(dmpm_ctx* ctx);
+/* Device bytes held by the context. */
+int dmpm_memory_bytes(dmpm_ctx* ctx, uint64_t* bytes);
+/* HIP events around every launch on the context's stream; per-kernel totals in milliseconds. */
+int dmpm_profile_enable(dmpm_ctx* ctx, int32_t on);
+int dmpm_profile_reset(dmpm_ctx* ctx);
+int dmpm_profile_read(dmpm_ctx* ctx, int32_t kernel, double* total_ms, int64_t* launches);
+
+/* Batched constitutive model on the GPU (device 0, host arrays, n matrices):
+ * PK_FixedCorotatedElasticity (Elasticity.cpp:88-98),
+ * d2_FCE_psi_dF2_mult_by_dF (Elasticity.cpp:100-178; exact = 1 differentiates the J clamp),
+ * and the 3x3 SVD of qrsvd/ImplicitQRSVD.h:687-850 (U, V rotations, sign on sigma[2]). */
+int dmpm_eval_pk1(int32_t n, const double* F, const double* lam, const double* mu, double* P);
+int dmpm_eval_hessian(int32_t n, const double* F, const double* lam, const double* mu, const double* dF,
+                      double* out, int32_t exact);
+int dmpm_eval_svd(int32_t n, const double* F, double* U, double* sigma, double* V);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DMPM_H */

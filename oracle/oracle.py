@@ -94,6 +94,12 @@ def _c(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def _n(prm, *arrs):
+    """Every per-particle array must hold prm.n rows (the C code trusts it)."""
+    for a in arrs:
+        assert np.shape(a)[0] == prm.n, (np.shape(a), prm.n)
+
+
 def set_threads(n: int) -> None:
     lib().orc_set_threads(int(n))
 
@@ -225,20 +231,30 @@ class Consts:
 
 
 def p2g_mass(prm: Params, x, m):
+    """P2G mass of an arbitrary point cloud (its own count, not prm.n)."""
+    x = _c(x).reshape(-1, 3)
+    m = _c(m)
+    assert len(m) == len(x)
     out = np.zeros(prm.G)
     P = prm.c()
+    P.n = len(x)
     lib().orc_p2g_mass(ctypes.byref(P), _p(_c(x)), _p(_c(m)), _p(out))
     return out
 
 
 def point_volumes(prm: Params, x, m):
-    out = np.zeros(prm.n)
+    x = _c(x).reshape(-1, 3)
+    m = _c(m)
+    assert len(m) == len(x)
+    out = np.zeros(len(x))
     P = prm.c()
+    P.n = len(x)
     lib().orc_point_volumes(ctypes.byref(P), _p(_c(x)), _p(_c(m)), _p(out))
     return out
 
 
 def p2g(prm: Params, st: State, cs: Consts, dFc, dt=None, drag=None):
+    _n(prm, st.x, st.v, st.C, st.F, dFc, cs.m, cs.vol, cs.lam, cs.mu)
     gm = np.zeros(prm.G)
     gp = np.zeros((prm.G, 3))
     P = prm.c()
@@ -250,6 +266,7 @@ def p2g(prm: Params, st: State, cs: Consts, dFc, dt=None, drag=None):
 
 def forward_step(prm: Params, st: State, cs: Consts, dFc):
     """ForwardTimeStep (ForwardSimulation.cpp:136-157). Returns (next, grid_m, grid_v)."""
+    _n(prm, st.x, st.v, st.C, st.F, dFc, cs.m, cs.vol, cs.lam, cs.mu)
     n = prm.n
     nxt = State.zeros(n)
     gm = np.zeros(prm.G)
@@ -263,6 +280,8 @@ def forward_step(prm: Params, st: State, cs: Consts, dFc):
 
 def end_mass_loss(prm: Params, x, m, target):
     """EndLayerMassLoss. Returns (loss, grid_m, grid_dLdm, dLdx)."""
+    _n(prm, x, m)
+    assert np.size(target) == prm.G
     gm = np.zeros(prm.G)
     gd = np.zeros(prm.G)
     dLdx = np.zeros((prm.n, 3))
@@ -286,6 +305,8 @@ class Grads:
 
 def backward_step(prm: Params, st: State, cs: Consts, dFc, grid_m, grid_v, C1, g1: Grads) -> Grads:
     """Back_Timestep (BackPropagation.cpp:6-125) for layer n from layer n+1's gradients."""
+    _n(prm, st.x, st.v, st.C, st.F, dFc, cs.m, cs.vol, cs.lam, cs.mu, C1, g1.x, g1.v, g1.F, g1.C)
+    assert np.size(grid_m) == prm.G and np.size(grid_v) == 3 * prm.G
     n = prm.n
     out = Grads.zeros(n)
     wv = np.zeros((prm.G, 3))

@@ -1,0 +1,49 @@
+"""C-ABI library checks that need no GPU: it loads and exports every symbol
+declared in include/dmpm.h; the Python binding covers all of them."""
+import ctypes
+import os
+import re
+
+from conftest import PKG_DIR, ROOT, load_pkg
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "dmpm.h")).read()
+    return sorted(set(re.findall(r"\b(dmpm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declarations_exported():
+    lib = ctypes.CDLL(os.path.join(PKG_DIR, "lib", "libdmpm.so"))
+    names = _declared()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    pkg = load_pkg()
+    assert sorted(pkg.EXPORTS) == _declared()
+
+
+def test_config_layout_matches_header():
+    pkg = load_pkg()
+    lib = pkg.load_library()
+    cfg = pkg.Config()
+    assert lib.dmpm_config_init(ctypes.byref(cfg)) == 0
+    assert cfg.dt == 1.0 / 120.0 and cfg.drag == 0.5 and cfg.smoothing == 0.1
+    assert cfg.num_layers == 2 and cfg.dx == 1.0
+    assert ctypes.sizeof(pkg.Config) == 4 * 6 + 8 * 10 + 4 * 6  # int32 x6, double x10, int32 x6
+
+
+def test_create_without_gpu_fails_cleanly():
+    """No GPU in this container: dmpm_create returns an error, it does not fall back to the CPU."""
+    pkg = load_pkg()
+    import torch
+    if torch.cuda.is_available():
+        return
+    try:
+        pkg.Morph(10, (4, 4, 4), 0.25, 3)
+    except pkg.DmpmError as e:
+        assert "dmpm error" in str(e)
+    else:
+        raise AssertionError("context created without a GPU")

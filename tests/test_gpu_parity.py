@@ -1,0 +1,265 @@
+"""HIP path vs the CPU FP64 oracle on the same seeded inputs (needs an MI355X).
+
+Tolerances (stated per test):
+* plain FP64 path vs oracle: 1e-12 relative on forward state and grids, 1e-10 on
+  gradients (only the order of FP64 atomic additions differs);
+* FP32 path vs FP64 oracle: 2e-5 relative on forward state, 1e-4 relative
+  (L2 over all entries) on the control gradient.
+"""
+import numpy as np
+import pytest
+
+from _scenes import gpu_morph, make_case, maxrel, orc, pkg, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand_F(n, seed, scale):
+    rng = np.random.default_rng(seed)
+    return np.eye(3) + scale * rng.normal(size=(n, 3, 3))
+
+
+# --------------------------------------------------------- constitutive ----
+def test_svd_conventions_gpu():
+    F = np.concatenate([_rand_F(500, 1, 0.5), np.diag([2.0, -0.5, 1.0])[None], np.eye(3)[None],
+                        np.diag([3.0, 3.0, 1e-9])[None]])
+    U, s, V = pkg.eval_svd(F)
+    for k in range(len(F)):
+        assert np.allclose(U[k] @ np.diag(s[k]) @ V[k].T, F[k], atol=1e-12)
+        assert np.linalg.det(U[k]) == pytest.approx(1.0, abs=1e-12)
+        assert np.linalg.det(V[k]) == pytest.approx(1.0, abs=1e-12)
+        ref = np.linalg.svd(F[k], compute_uv=False)
+        assert np.allclose(np.abs(s[k]), ref, rtol=1e-11, atol=1e-12)
+        assert s[k][0] >= 0 and s[k][1] >= 0
+
+
+def test_pk1_gpu_vs_oracle():
+    F = _rand_F(2000, 2, 0.4)
+    rng = np.random.default_rng(3)
+    lam = rng.uniform(1e3, 1e5, 2000)
+    mu = rng.uniform(1e3, 1e5, 2000)
+    P = pkg.eval_pk1(F, lam, mu)
+    ref = orc.pk1_batch(F, lam, mu)
+    assert maxrel(P, ref) < 1e-12
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_hessian_gpu_vs_oracle(exact):
+    F = _rand_F(2000, 4, 0.4)
+    rng = np.random.default_rng(5)
+    lam = rng.uniform(1e3, 1e5, 2000)
+    mu = rng.uniform(1e3, 1e5, 2000)
+    dF = rng.normal(size=(2000, 3, 3))
+    H = pkg.eval_hessian(F, lam, mu, dF, exact)
+    ref = orc.hess_mult_batch(F, lam, mu, dF, exact)
+    assert maxrel(H, ref) < 1e-10
+
+
+# ----------------------------------------------------------- FP64 step -----
+def test_volumes_and_target_gpu():
+    case = make_case(physical_vol=False)
+    sc = case.sc
+    mo = pkg.Morph(sc.n, sc.dim, sc.dx, 2, precision=pkg.PREC_F64)
+    x, v, C, F = sc.state0()
+    mo.set_particles(x, v, F, C, sc.m, None, sc.lam, sc.mu)
+    mo.compute_volumes()
+    assert maxrel(mo.volumes(), case.cs.vol) < 1e-12
+    mo.target_from_points(sc.target_x, sc.target_m)
+    assert maxrel(mo.target_grid(), case.target) < 1e-12
+
+
+@pytest.mark.parametrize("phys", [dict(), dict(drag=0.3, smoothing=0.1, f_ext=(0.3, -1.0, 0.5), vel=0.2)])
+def test_forward_step_fp64(phys):
+    case = make_case(L=3, per_layer_ctrl=True, **phys)
+    mo = gpu_morph(case)
+    mo.forward_pass(0)
+    layers, grids = orc.forward_pass(case.prm, case.st0, case.cs, case.controls, case.L)
+    for j in (1, 2):
+        x, v, F, C = mo.layer_state(j)
+        assert maxrel(x, layers[j].x) < 1e-14
+        assert maxrel(v, layers[j].v) < 1e-12
+        assert maxrel(F, layers[j].F) < 1e-12
+        assert maxrel(C, layers[j].C) < 1e-12
+        gm, gv = mo.grid(j - 1)
+        assert maxrel(gm, grids[j - 1][0]) < 1e-12
+        assert maxrel(gv, grids[j - 1][1]) < 1e-12
+
+
+def test_reference_config_rest_and_first_step():
+    """As written (reference volume formula, dt = 1e-3): the rest state is static, one step matches."""
+    case = make_case(dt=1e-3, physical_vol=False, L=2, ctrl=0.0)
+    mo = gpu_morph(case)
+    mo.forward_pass(0)
+    x, v, F, C = mo.layer_state(1)
+    assert np.array_equal(x, case.st0.x) and np.abs(v).max() == 0.0 and np.abs(C).max() == 0.0
+    assert np.abs(F - np.eye(3)).max() == 0.0
+    case = make_case(dt=1e-3, physical_vol=False, L=2, ctrl=1e-2)
+    mo = gpu_morph(case)
+    mo.forward_pass(0)
+    layers, _ = orc.forward_pass(case.prm, case.st0, case.cs, case.controls, case.L)
+    x, v, F, C = mo.layer_state(1)
+    assert maxrel(v, layers[1].v) < 1e-12 and maxrel(F, layers[1].F) < 1e-12
+
+
+def test_loss_fp64():
+    case = make_case(L=3)
+    mo = gpu_morph(case)
+    mo.forward_pass(0)
+    loss = mo.end_layer_mass_loss()
+    layers, _ = orc.forward_pass(case.prm, case.st0, case.cs, case.controls, case.L)
+    ref, _, _, dLdx = orc.end_mass_loss(case.prm, layers[-1].x, case.cs.m, case.target)
+    assert loss == pytest.approx(ref, rel=1e-12)
+    gx, gv, gF, gC = mo.layer_grads(case.L - 1)
+    assert maxrel(gx, dLdx) < 1e-10
+    assert np.abs(gv).max() == 0 and np.abs(gF).max() == 0 and np.abs(gC).max() == 0
+
+
+@pytest.mark.parametrize("phys", [dict(), dict(drag=0.3, smoothing=0.1, f_ext=(0.3, -1.0, 0.5), vel=0.2)])
+@pytest.mark.parametrize("compat", [0, pkg.COMPAT_REFERENCE])
+def test_episode_gradient_fp64(phys, compat):
+    case = make_case(L=6, per_layer_ctrl=True, **phys)
+    case.prm.compat = compat
+    mo = gpu_morph(case, compat=compat)
+    mo.forward_pass(0)
+    loss = mo.end_layer_mass_loss()
+    mo.backward_pass(0)
+    ref_loss, layers, gd, g0 = orc.episode(case.prm, case.st0, case.cs, case.controls, case.target, case.L)
+    assert loss == pytest.approx(ref_loss, rel=1e-12)
+    for i in range(case.L - 1):
+        assert rel(mo.control_grad(i), gd[i]) < 1e-10, i
+    gx, gv, gF, gC = mo.layer_grads(0)
+    assert rel(gx, g0.x) < 1e-10 and rel(gv, g0.v) < 1e-10
+    assert rel(gF, g0.F) < 1e-10 and rel(gC, g0.C) < 1e-10
+
+
+def test_single_step_api_matches_pass():
+    case = make_case(L=4, per_layer_ctrl=True)
+    a = gpu_morph(case)
+    a.forward_pass(0)
+    la = a.end_layer_mass_loss()
+    a.backward_pass(0)
+    b = gpu_morph(case)
+    for i in range(case.L - 1):
+        b.forward_step(i)
+    lb = b.end_layer_mass_loss()
+    for i in range(case.L - 2, -1, -1):
+        b.backward_step(i)
+    assert la == pytest.approx(lb, rel=1e-13)
+    assert rel(a.control_grad(0), b.control_grad(0)) < 1e-12
+
+
+# ------------------------------------------------------------ checkpoint ---
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 2e-5)])
+@pytest.mark.parametrize("stride", [0, 1, 4, 7])
+def test_checkpoint_matches_store_all(prec, tol, stride):
+    case = make_case(L=12, per_layer_ctrl=True)
+    a = gpu_morph(case, precision=prec)
+    a.forward_pass(0)
+    la = a.end_layer_mass_loss()
+    a.backward_pass(0)
+    b = gpu_morph(case, precision=prec, tape=pkg.TAPE_CHECKPOINT, ckpt_stride=stride)
+    b.forward_pass(0)
+    lb = b.end_layer_mass_loss()
+    b.backward_pass(0)
+    assert lb == pytest.approx(la, rel=1e-12 if prec == 0 else 1e-6)
+    for i in range(case.L - 1):
+        assert rel(b.control_grad(i), a.control_grad(i)) < tol, i
+    xa = a.layer_state(5)[0]
+    xb = b.layer_state(5)[0]
+    assert maxrel(xb, xa) < (1e-12 if prec == 0 else 1e-6)
+
+
+def test_checkpoint_memory_smaller():
+    case = make_case(L=41)
+    a = gpu_morph(case, keep_control_grads=False)
+    b = gpu_morph(case, tape=pkg.TAPE_CHECKPOINT, keep_control_grads=False)
+    assert b.memory_bytes() < 0.6 * a.memory_bytes()
+
+
+def test_checkpoint_control_layer_restart():
+    """Optimiser pattern: change one control, forward from it, backward to it (CompGraph.cpp:365-373)."""
+    case = make_case(L=9, per_layer_ctrl=True)
+    b = gpu_morph(case, tape=pkg.TAPE_CHECKPOINT, ckpt_stride=3)
+    b.forward_pass(0)
+    b.end_layer_mass_loss()
+    b.backward_pass(0)
+    new = case.controls[4] + 1e-3
+    case.controls[4] = new
+    b.set_control(4, new)
+    b.forward_pass(4)
+    lb = b.end_layer_mass_loss()
+    b.backward_pass(4)
+    ref_loss, _, gd, _ = orc.episode(case.prm, case.st0, case.cs, case.controls, case.target, case.L, control_layer=4)
+    assert lb == pytest.approx(ref_loss, rel=1e-12)
+    assert rel(b.control_grad(4), gd[4]) < 1e-10
+
+
+# ------------------------------------------------------------------ FP32 ---
+def test_fp32_episode_vs_fp64_oracle():
+    case = make_case(L=11)
+    mo = gpu_morph(case, precision=pkg.PREC_F32, tape=pkg.TAPE_CHECKPOINT)
+    mo.forward_pass(0)
+    loss = mo.end_layer_mass_loss()
+    mo.backward_pass(0)
+    ref_loss, layers, gd, _ = orc.episode(case.prm, case.st0, case.cs, case.controls, case.target, case.L)
+    x, v, F, C = mo.layer_state(case.L - 1)
+    assert maxrel(x, layers[-1].x) < 2e-7
+    assert maxrel(F, layers[-1].F) < 2e-5
+    assert loss == pytest.approx(ref_loss, rel=1e-5)
+    err = rel(mo.control_grad(0), gd[0])
+    print(f"fp32 control-gradient rel err {err:.3e}")
+    assert err < 1e-4
+
+
+# ------------------------------------------------------------- edge cases --
+def test_particles_near_and_outside_grid_fp64():
+    """Stencil nodes outside the grid are skipped (ForwardSimulation.cpp:34-36)."""
+    rng = np.random.default_rng(7)
+    n = 300
+    sc = pkg.scenes.morph_small()
+    x = rng.uniform(-0.05, 1.05, size=(n, 3))
+    m = np.full(n, sc.m[0])
+    lam = np.full(n, sc.lam[0])
+    mu = np.full(n, sc.mu[0])
+    prm = orc.Params(n=n, dim=sc.dim, dx=sc.dx, dt=1e-4)
+    vol = orc.point_volumes(prm, x, m) * sc.dx ** 3
+    cs = orc.Consts(m, vol, lam, mu)
+    st = orc.State(x, 0.1 * rng.normal(size=(n, 3)), np.zeros((n, 3, 3)), np.broadcast_to(np.eye(3), (n, 3, 3)).copy())
+    dFc = 1e-2 * rng.normal(size=(n, 3, 3))
+    target = orc.p2g_mass(prm, sc.target_x, sc.target_m)
+    L = 4
+    controls = [dFc] + [np.zeros_like(dFc)] * (L - 2)
+    mo = pkg.Morph(n, sc.dim, sc.dx, L, dt=1e-4, drag=0.0, smoothing=0.0)
+    mo.set_particles(st.x, st.v, st.F, st.C, m, vol, lam, mu)
+    mo.set_control(0, dFc)
+    mo.set_target_grid(target)
+    mo.forward_pass(0)
+    loss = mo.end_layer_mass_loss()
+    mo.backward_pass(0)
+    ref_loss, layers, gd, _ = orc.episode(prm, st, cs, controls, target, L)
+    assert loss == pytest.approx(ref_loss, rel=1e-12)
+    assert maxrel(mo.layer_state(L - 1)[0], layers[-1].x) < 1e-14
+    assert rel(mo.control_grad(0), gd[0]) < 1e-10
+
+
+def test_chain_episode():
+    case = make_case(L=4)
+    mo = gpu_morph(case)
+    mo.forward_pass(0)
+    last = mo.layer_state(case.L - 1)
+    mo.chain_episode()
+    first = mo.layer_state(0)
+    for a, b in zip(first, last):
+        assert np.array_equal(a, b)
+    assert np.abs(mo.control(0)).max() == 0.0
+
+
+def test_error_paths():
+    case = make_case(L=3)
+    mo = gpu_morph(case)
+    with pytest.raises(pkg.DmpmError):
+        mo.backward_pass(0)  # before the loss
+    with pytest.raises(pkg.DmpmError):
+        mo.forward_pass(5)   # out of range
+    with pytest.raises(pkg.DmpmError):
+        pkg.Morph(10, (0, 4, 4), 0.1, 3)
