@@ -55,13 +55,17 @@ template <typename Real> void launch_grid_update(const StepParams& sp, Real* gri
 template <typename Real> void launch_g2p(const StepParams& sp, const StepIO<Real>& io, hipStream_t s);
 template <typename Real> void launch_p2g_mass(const StepParams& sp, const Real* x, const Real* m, int stride, Real* gm, hipStream_t s);
 template <typename Real> void launch_volumes(const StepParams& sp, const Real* x, Real* cst, const Real* gm, hipStream_t s);
-template <typename Real> void launch_loss(const StepParams& sp, const Real* gm, const Real* target, Real* dLdm,
+template <typename Real> void launch_loss(const StepParams& sp, const Real* gm, const Real* target, double* dLdm,
                                           double* loss_acc, hipStream_t s);
 template <typename Real> void launch_loss_particles(const StepParams& sp, const Real* layer, const Real* cst,
-                                                    const Real* dLdm, Real* g, hipStream_t s);
+                                                    const double* dLdm, Real* g, hipStream_t s);
 template <typename Real> void launch_g2p_adj(const StepParams& sp, const AdjIO<Real>& io, hipStream_t s);
 template <typename Real> void launch_grid_adj(const StepParams& sp, const AdjIO<Real>& io, hipStream_t s);
 template <typename Real> void launch_p2g_adj(const StepParams& sp, const AdjIO<Real>& io, hipStream_t s);
+
+// FP32 path: LDS-tiled scatters (cap = LDS window capacity in nodes)
+void launch_p2g_tiled(const StepParams& sp, const StepIO<float>& io, int cap, hipStream_t s);
+void launch_g2p_adj_tiled(const StepParams& sp, const AdjIO<float>& io, int cap, hipStream_t s);
 
 // batched constitutive evaluations (AoS 9 doubles per matrix)
 void launch_eval_pk1(int n, const double* F, const double* lam, const double* mu, double* P, hipStream_t s);

@@ -66,6 +66,10 @@ struct dmpm_ctx {
     void* target = nullptr;   // G
     double* dacc = nullptr;   // FP64 accumulators
     uint64_t bytes = 0;
+    // FP32 path: device order of the particles (sorted by 4^3-cell block);
+    // pos[p] is the device slot of API particle p (empty = identity)
+    std::vector<int> pos;
+    int lds_cap = 2560;       // LDS window of the tiled scatters, in nodes
     // checkpoint-mode slot layout
     int n_ckpt = 0, seg_last = 0, tape_base = 0, roll_base = 0, n_slots = 0;
     int tape_seg = -1;
@@ -213,6 +217,13 @@ void fill(dmpm_ctx* c, Real* p, long long count, Real v)
     timed(c, DMPM_K_FILL, [&] { launch_fill<Real>(p, count, v, c->stream); });
 }
 
+// Scatter kernels: the plain FP64 path uses direct HBM atomics, the FP32
+// path the LDS-tiled kernels over block-sorted particles.
+inline void p2g_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<double>& io) { launch_p2g<double>(sp, io, c->stream); }
+inline void p2g_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<float>& io) { launch_p2g_tiled(sp, io, c->lds_cap, c->stream); }
+inline void g2p_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<double>& io) { launch_g2p_adj<double>(sp, io, c->stream); }
+inline void g2p_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<float>& io) { launch_g2p_adj_tiled(sp, io, c->lds_cap, c->stream); }
+
 // One ForwardTimeStep: cur -> nxt, grid of the step in g.
 template <typename Real>
 void step_forward(dmpm_ctx* c, const Real* cur, Real* nxt, const Real* ctrl, Real* g)
@@ -220,7 +231,7 @@ void step_forward(dmpm_ctx* c, const Real* cur, Real* nxt, const Real* ctrl, Rea
     StepParams sp = step_params(c);
     StepIO<Real> io{cur, nxt, ctrl, static_cast<const Real*>(c->cst), g};
     fill<Real>(c, g, (long long)kGridFields * c->G, Real(0));
-    timed(c, DMPM_K_P2G, [&] { launch_p2g<Real>(sp, io, c->stream); });
+    timed(c, DMPM_K_P2G, [&] { p2g_dispatch(c, sp, io); });
     timed(c, DMPM_K_GRID_UPDATE, [&] { launch_grid_update<Real>(sp, g, c->stream); });
     timed(c, DMPM_K_G2P, [&] { launch_g2p<Real>(sp, io, c->stream); });
 }
@@ -242,7 +253,7 @@ void step_backward(dmpm_ctx* c, int i, const Real* cur, const Real* nxt, const R
     io.gctrl = gctrl_of<Real>(c, i);
     io.tmp = static_cast<Real*>(c->tmp);
     fill<Real>(c, io.agrid, 3 * c->G, Real(0));
-    timed(c, DMPM_K_G2P_ADJ, [&] { launch_g2p_adj<Real>(sp, io, c->stream); });
+    timed(c, DMPM_K_G2P_ADJ, [&] { g2p_adj_dispatch(c, sp, io); });
     timed(c, DMPM_K_GRID_ADJ, [&] { launch_grid_adj<Real>(sp, io, c->stream); });
     timed(c, DMPM_K_P2G_ADJ, [&] { launch_p2g_adj<Real>(sp, io, c->stream); });
 }
@@ -325,7 +336,8 @@ int end_loss(dmpm_ctx* c, double* loss)
     Real* lay = (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) ? slot<Real>(c, last) : materialize<Real>(c, last);
     StepParams sp = step_params(c);
     Real* gm = static_cast<Real*>(c->wgrid);
-    Real* dLdm = gm + c->G;
+    // FP64 dL/dm after 2G reals of the 4G-real work grid (8-byte aligned for both precisions)
+    double* dLdm = reinterpret_cast<double*>(gm + 2 * c->G);
     fill<Real>(c, gm, c->G, Real(0));
     HIP_TRY(hipMemsetAsync(c->dacc, 0, sizeof(double), c->stream));
     const Real* cst = static_cast<const Real*>(c->cst);
@@ -377,8 +389,11 @@ int upload_fields(dmpm_ctx* c, Real* dst, int field0, int nfield, const double* 
 {
     // AoS [n][nfield] double -> SoA fields field0.. of stride S
     std::vector<Real> h((size_t)nfield * c->S, Real(0));
-    for (int p = 0; p < n; ++p)
-        for (int f = 0; f < nfield; ++f) h[(size_t)f * c->S + p] = Real(src[(size_t)p * nfield + f]);
+    const bool perm = !c->pos.empty();
+    for (int p = 0; p < n; ++p) {
+        const size_t d = perm ? (size_t)c->pos[p] : (size_t)p;
+        for (int f = 0; f < nfield; ++f) h[(size_t)f * c->S + d] = Real(src[(size_t)p * nfield + f]);
+    }
     HIP_TRY(hipMemcpyAsync(dst + (size_t)field0 * c->S, h.data(), h.size() * sizeof(Real), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return DMPM_OK;
@@ -390,9 +405,42 @@ int download_fields(dmpm_ctx* c, const Real* src, int field0, int nfield, double
     std::vector<Real> h((size_t)nfield * c->S);
     HIP_TRY(hipMemcpyAsync(h.data(), src + (size_t)field0 * c->S, h.size() * sizeof(Real), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    for (int p = 0; p < n; ++p)
-        for (int f = 0; f < nfield; ++f) dst[(size_t)p * nfield + f] = double(h[(size_t)f * c->S + p]);
+    const bool perm = !c->pos.empty();
+    for (int p = 0; p < n; ++p) {
+        const size_t d = perm ? (size_t)c->pos[p] : (size_t)p;
+        for (int f = 0; f < nfield; ++f) dst[(size_t)p * nfield + f] = double(h[(size_t)f * c->S + d]);
+    }
     return DMPM_OK;
+}
+
+// Device order of the FP32 path: particles sorted by the 4^3-cell block of
+// their stencil base cell (blocks in x-major order, Morton order of the 64
+// cells inside a block), ties kept in API order.
+void sort_particles(dmpm_ctx* c, const double* x)
+{
+    const int N = c->N;
+    int nb[3];
+    for (int a = 0; a < 3; ++a) nb[a] = (c->cfg.dim[a] + 3) / 4;
+    std::vector<uint64_t> key(N);
+    for (int p = 0; p < N; ++p) {
+        int cell[3];
+        for (int a = 0; a < 3; ++a) {
+            double u = std::floor((x[3 * (size_t)p + a] - c->cfg.min_point[a]) / c->cfg.dx);
+            if (!(u >= 0.0)) u = 0.0;  // also maps NaN
+            if (u > c->cfg.dim[a] - 1) u = c->cfg.dim[a] - 1;
+            cell[a] = (int)u;
+        }
+        const uint64_t block = ((uint64_t)(cell[0] >> 2) * nb[1] + (cell[1] >> 2)) * nb[2] + (cell[2] >> 2);
+        uint32_t mort = 0;
+        for (int bit = 0; bit < 2; ++bit)
+            for (int a = 0; a < 3; ++a) mort |= (uint32_t)((cell[a] >> bit) & 1) << (3 * bit + (2 - a));
+        key[p] = (block << 6) | mort;
+    }
+    std::vector<int> order(N);
+    for (int p = 0; p < N; ++p) order[p] = p;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+    c->pos.assign(N, 0);
+    for (int d = 0; d < N; ++d) c->pos[order[d]] = d;
 }
 
 template <typename Real>
@@ -400,6 +448,9 @@ int set_particles(dmpm_ctx* c, const double* x, const double* v, const double* F
                   const double* vol, const double* lam, const double* mu)
 {
     if (!x || !v || !F || !C || !m || !lam || !mu) return fail(DMPM_ERR_INVALID, "null particle array");
+    if (c->f32) sort_particles(c, x);
+    // a new particle set starts with zero controls (MaterialPoint.h:31)
+    fill<Real>(c, static_cast<Real*>(c->ctrl), (long long)(c->L - 1) * 9 * c->S, Real(0));
     Real* l0 = slot<Real>(c, 0);
     int rc;
     if ((rc = upload_fields<Real>(c, l0, FX, 3, x, c->N))) return rc;
