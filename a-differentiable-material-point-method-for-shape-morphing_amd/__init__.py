@@ -41,7 +41,8 @@ EXPORTS = [
     "dmpm_set_particles", "dmpm_compute_volumes", "dmpm_get_volumes", "dmpm_set_control", "dmpm_get_control",
     "dmpm_set_target_grid", "dmpm_target_from_points", "dmpm_get_target_grid", "dmpm_forward_pass",
     "dmpm_end_layer_mass_loss", "dmpm_backward_pass", "dmpm_forward_step", "dmpm_backward_step",
-    "dmpm_get_layer_state", "dmpm_set_layer_state", "dmpm_get_layer_grads", "dmpm_get_control_grad",
+    "dmpm_get_layer_state", "dmpm_set_layer_state", "dmpm_get_layer_grads", "dmpm_set_layer_grads",
+    "dmpm_get_control_grad",
     "dmpm_control_grad_norm", "dmpm_get_grid", "dmpm_chain_episode", "dmpm_synchronize", "dmpm_memory_bytes",
     "dmpm_profile_enable", "dmpm_profile_reset", "dmpm_profile_read", "dmpm_eval_pk1", "dmpm_eval_hessian",
     "dmpm_eval_svd",
@@ -109,6 +110,7 @@ def load_library(path: str = LIB_PATH):
         "dmpm_get_layer_state": ([vp, i32] + [_DP] * 4, ctypes.c_int),
         "dmpm_set_layer_state": ([vp, i32] + [_DP] * 4, ctypes.c_int),
         "dmpm_get_layer_grads": ([vp, i32] + [_DP] * 4, ctypes.c_int),
+        "dmpm_set_layer_grads": ([vp, i32] + [_DP] * 4, ctypes.c_int),
         "dmpm_get_control_grad": ([vp, i32, _DP], ctypes.c_int),
         "dmpm_control_grad_norm": ([vp, i32, _DP], ctypes.c_int),
         "dmpm_get_grid": ([vp, i32, _DP, _DP], ctypes.c_int),
@@ -326,6 +328,10 @@ class Morph:
         C = np.zeros((self.n, 3, 3))
         _check(self._L.dmpm_get_layer_grads(self._h, int(layer), _ptr(x), _ptr(v), _ptr(F), _ptr(C)))
         return x, v, F, C
+
+    def set_layer_grads(self, layer, dLdx=None, dLdv=None, dLdF=None, dLdC=None):
+        arrs = [_in(a) for a in (dLdx, dLdv, dLdF, dLdC)]
+        _check(self._L.dmpm_set_layer_grads(self._h, int(layer), *[p for _, p in arrs]))
 
     def control_grad(self, layer):
         out = np.zeros((self.n, 3, 3))

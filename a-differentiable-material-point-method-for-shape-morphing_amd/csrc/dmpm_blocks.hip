@@ -1,0 +1,625 @@
+// FP32 path: particle-grid transfers staged per grid block in LDS (gfx950).
+//
+// Particles are stored sorted by the 4^3-cell block of their cell; one
+// workgroup processes the particle range of one block (CSR ranges built at
+// sort time).  The workgroup first reduces the bounding box of its particles'
+// stencil bases -- the node window, normally the block's 7^3 nodes -- then:
+//   scatters (P2G, transposed G2P): accumulate every particle's 64-node
+//     contribution with LDS atomics, then add each non-zero window node to
+//     HBM with one atomic;
+//   gathers (G2P, transposed P2G): copy the window's grid channels from HBM
+//     into LDS once, then every particle reads its 64 nodes from LDS.
+// The window is recomputed each launch from the current positions, so a
+// particle that drifted out of its block only widens the window; a window
+// larger than the LDS capacity falls back to direct HBM access.
+//
+// Formulas are those of the per-particle kernels (ForwardSimulation.cpp:6-124,
+// BackPropagation.cpp:14-124 in the reference) with the same FP64 polar/SVD,
+// PK1 and H:dF.
+#include "dmpm_kernels.h"
+#include "dmpm_math.h"
+
+#include <climits>
+
+namespace dmpm {
+
+namespace {
+
+constexpr int kBlk = 256;
+
+struct Win {
+    int lo[3];
+    int w[3];
+    int vol;
+    bool lds;
+};
+
+template <typename Stn>
+__device__ __forceinline__ void stencil_f(const StepParams& sp, const float x[3], Stn& st, bool slopes)
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a) axis_weights<float>((x[a] - float(sp.min_point[a])) * float(sp.inv_dx), st[a], slopes);
+}
+
+__device__ __forceinline__ int base_of(const StepParams& sp, float x, int a)
+{
+    return int(floorf((x - float(sp.min_point[a])) * float(sp.inv_dx))) - 1;
+}
+
+// Bounding box of the stencil bases of particles [r0, r1), clamped to the grid.
+__device__ Win range_window(const StepParams& sp, const float* lay, int S, int r0, int r1, int cap)
+{
+    __shared__ int red[6][kBlk / 64];
+    __shared__ Win win;
+    int v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    for (int p = r0 + (int)threadIdx.x; p < r1; p += blockDim.x) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int b = base_of(sp, lay[(long long)(FX + a) * S + p], a);
+            v[a] = min(v[a], b);
+            v[3 + a] = max(v[3 + a], b);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            v[a] = min(v[a], __shfl_xor(v[a], off, 64));
+            v[3 + a] = max(v[3 + a], __shfl_xor(v[3 + a], off, 64));
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) red[c][wid] = v[c];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Win W;
+        long long vol = 1;
+        bool empty = false;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            int lo = INT_MAX, hi = INT_MIN;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+                lo = min(lo, red[a][w]);
+                hi = max(hi, red[3 + a][w]);
+            }
+            long long l = lo < 0 ? 0 : lo;
+            long long h = (long long)hi + 3;
+            if (h > sp.dim[a] - 1) h = sp.dim[a] - 1;
+            if (h < l) empty = true;
+            W.lo[a] = (int)(l > INT_MAX ? INT_MAX : l);
+            W.w[a] = empty ? 0 : (int)(h - l + 1);
+            vol *= empty ? 0 : (h - l + 1);
+        }
+        W.vol = empty ? 0 : (int)(vol > INT_MAX ? INT_MAX : vol);
+        W.lds = !empty && vol <= cap;
+        win = W;
+    }
+    __syncthreads();
+    return win;
+}
+
+__device__ __forceinline__ long long win_node(const StepParams& sp, const Win& W, int idx)
+{
+    const int lk = idx % W.w[2];
+    const int t = idx / W.w[2];
+    const int lj = t % W.w[1];
+    const int li = t / W.w[1];
+    return ((long long)(W.lo[0] + li) * sp.dim[1] + (W.lo[1] + lj)) * sp.dim[2] + (W.lo[2] + lk);
+}
+
+__device__ __forceinline__ int win_local(const Win& W, int gi, int gj, int gk)
+{
+    return ((gi - W.lo[0]) * W.w[1] + (gj - W.lo[1])) * W.w[2] + (gk - W.lo[2]);
+}
+
+template <int NC>
+__device__ __forceinline__ void win_flush(const StepParams& sp, const Win& W, const float* lds, float* base, long long cstride)
+{
+    for (int idx = threadIdx.x; idx < W.vol; idx += blockDim.x) {
+        const long long node = win_node(sp, W, idx);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const float val = lds[c * W.vol + idx];
+            if (val != 0.f) unsafeAtomicAdd(base + c * cstride + node, val);
+        }
+    }
+}
+
+// load NC channels (channel c at src_c[node]) of the window into LDS
+template <int NC>
+__device__ __forceinline__ void win_load(const StepParams& sp, const Win& W, float* lds, const float* const src[NC])
+{
+    for (int idx = threadIdx.x; idx < W.vol; idx += blockDim.x) {
+        const long long node = win_node(sp, W, idx);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) lds[c * W.vol + idx] = src[c][node];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void ld3(const T* base, int field, int S, int p, T out[3])
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a) out[a] = base[(long long)(field + a) * S + p];
+}
+
+template <typename T>
+__device__ __forceinline__ M3<T> ldm(const T* base, int field, int S, int p)
+{
+    M3<T> M;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) M.a[i] = base[(long long)(field + i) * S + p];
+    return M;
+}
+
+template <typename T>
+__device__ __forceinline__ void stm(T* base, int field, int S, int p, const M3<T>& M)
+{
+#pragma unroll
+    for (int i = 0; i < 9; ++i) base[(long long)(field + i) * S + p] = M.a[i];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ P2G ----
+__global__ __launch_bounds__(kBlk) void k_p2g_blk(StepParams sp, StepIO<float> io, const int2* ranges, int cap)
+{
+    extern __shared__ float lds[];
+    const int2 rg = ranges[blockIdx.x];
+    const int S = sp.stride;
+    const Win W = range_window(sp, io.cur, S, rg.x, rg.y, cap);
+    if (W.lds)
+        for (int i = threadIdx.x; i < 4 * W.vol; i += blockDim.x) lds[i] = 0.f;
+    __syncthreads();
+    const float dx = float(sp.dx);
+    const float sdrag = 1.f - float(sp.dt) * float(sp.drag);
+    const double kst = -3.0 * sp.inv_dx * sp.inv_dx * sp.dt;
+    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) {
+        float x[3], v[3];
+        ld3(io.cur, FX, S, p, x);
+        ld3(io.cur, FV, S, p, v);
+        const M3<float> C = ldm(io.cur, FC, S, p);
+        const M3<float> F = ldm(io.cur, FF, S, p);
+        const M3<float> dF = ldm(io.ctrl, 0, S, p);
+        const float m = io.cst[KM * S + p];
+        const double vol = io.cst[KVOL * S + p];
+        const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
+        M3<float> A;
+        {
+            M3<double> Fc;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
+            FCE e;
+            fce_prepare(Fc, e);
+            const M3<double> PF = mult(fce_pk1(Fc, e, lam, mu), Fc);
+            const double k = kst * vol;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) A.a[i] = float(k * PF.a[i]) + m * C.a[i];
+        }
+        float mv[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) mv[a] = m * v[a] * sdrag;
+        Axis<float> st[3];
+        stencil_f(sp, x, st, false);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int gi = st[0].base + i;
+            if (gi < 0 || gi >= sp.dim[0]) continue;
+            const float d0 = (float(i - 1) - st[0].f) * dx;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int gj = st[1].base + j;
+                if (gj < 0 || gj >= sp.dim[1]) continue;
+                const float d1 = (float(j - 1) - st[1].f) * dx;
+                const float wij = st[0].w[i] * st[1].w[j];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const int gk = st[2].base + kk;
+                    if (gk < 0 || gk >= sp.dim[2]) continue;
+                    const float d2 = (float(kk - 1) - st[2].f) * dx;
+                    const float w = wij * st[2].w[kk];
+                    float val[4];
+                    val[0] = w * m;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) val[1 + a] = w * (mv[a] + A.a[3 * a] * d0 + A.a[3 * a + 1] * d1 + A.a[3 * a + 2] * d2);
+                    if (W.lds) {
+                        const int li = win_local(W, gi, gj, gk);
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) atomicAdd(&lds[c * W.vol + li], val[c]);
+                    } else {
+                        const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) unsafeAtomicAdd(io.grid + c * sp.G + node, val[c]);
+                    }
+                }
+            }
+        }
+    }
+    if (W.lds) {
+        __syncthreads();
+        win_flush<4>(sp, W, lds, io.grid, sp.G);
+    }
+}
+
+// ------------------------------------------------------------------ G2P ----
+__global__ __launch_bounds__(kBlk) void k_g2p_blk(StepParams sp, StepIO<float> io, const int2* ranges, int cap)
+{
+    extern __shared__ float lds[];
+    const int2 rg = ranges[blockIdx.x];
+    const int S = sp.stride;
+    const Win W = range_window(sp, io.cur, S, rg.x, rg.y, cap);
+    const float* gv = io.grid + sp.G;
+    if (W.lds) {
+        const float* src[3] = {gv, gv + sp.G, gv + 2 * sp.G};
+        win_load<3>(sp, W, lds, src);
+    }
+    __syncthreads();
+    const float Dd = float(3.0 * sp.inv_dx);
+    const float dt = float(sp.dt), s = float(sp.smoothing);
+    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) {
+        float x[3];
+        ld3(io.cur, FX, S, p, x);
+        Axis<float> st[3];
+        stencil_f(sp, x, st, false);
+        float vp[3] = {0.f, 0.f, 0.f};
+        M3<float> Cp = m3_zero<float>();
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int gi = st[0].base + i;
+            if (gi < 0 || gi >= sp.dim[0]) continue;
+            const float e0 = (float(i - 1) - st[0].f) * Dd;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int gj = st[1].base + j;
+                if (gj < 0 || gj >= sp.dim[1]) continue;
+                const float e1 = (float(j - 1) - st[1].f) * Dd;
+                const float wij = st[0].w[i] * st[1].w[j];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const int gk = st[2].base + kk;
+                    if (gk < 0 || gk >= sp.dim[2]) continue;
+                    const float e2 = (float(kk - 1) - st[2].f) * Dd;
+                    const float w = wij * st[2].w[kk];
+                    float vi[3];
+                    if (W.lds) {
+                        const int li = win_local(W, gi, gj, gk);
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) vi[a] = lds[a * W.vol + li];
+                    } else {
+                        const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) vi[a] = gv[a * sp.G + node];
+                    }
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const float wv = w * vi[a];
+                        vp[a] += wv;
+                        Cp.a[3 * a] += wv * e0;
+                        Cp.a[3 * a + 1] += wv * e1;
+                        Cp.a[3 * a + 2] += wv * e2;
+                    }
+                }
+            }
+        }
+        const M3<float> F = ldm(io.cur, FF, S, p);
+        const M3<float> dF = ldm(io.ctrl, 0, S, p);
+        M3<float> Fc, IC;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            Fc.a[i] = F.a[i] + dF.a[i];
+            IC.a[i] = dt * Cp.a[i];
+        }
+        IC.a[0] += 1.f;
+        IC.a[4] += 1.f;
+        IC.a[8] += 1.f;
+        M3<float> Fn = mul(IC, Fc);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Fn.a[i] = Fn.a[i] * (1.f - s) + F.a[i] * s;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            io.next[(long long)(FX + a) * S + p] = x[a] + dt * vp[a];
+            io.next[(long long)(FV + a) * S + p] = vp[a];
+        }
+        stm(io.next, FC, S, p, Cp);
+        stm(io.next, FF, S, p, Fn);
+    }
+}
+
+// -------------------------------------------------------- transposed G2P ---
+__global__ __launch_bounds__(kBlk) void k_g2p_adj_blk(StepParams sp, AdjIO<float> io, const int2* ranges, int cap)
+{
+    extern __shared__ float lds[];
+    const int2 rg = ranges[blockIdx.x];
+    const int S = sp.stride;
+    const Win W = range_window(sp, io.cur, S, rg.x, rg.y, cap);
+    if (W.lds)
+        for (int i = threadIdx.x; i < 3 * W.vol; i += blockDim.x) lds[i] = 0.f;
+    __syncthreads();
+    const float Dd = float(3.0 * sp.inv_dx);
+    const float dt = float(sp.dt);
+    const float cs = (sp.compat & 2) ? 1.f : 1.f - float(sp.smoothing);
+    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) {
+        float x[3], gx1[3], gv1[3];
+        ld3(io.cur, FX, S, p, x);
+        ld3(io.g1, FX, S, p, gx1);
+        ld3(io.g1, FV, S, p, gv1);
+        const M3<float> F = ldm(io.cur, FF, S, p);
+        const M3<float> dF = ldm(io.ctrl, 0, S, p);
+        const M3<float> gF1 = ldm(io.g1, FF, S, p);
+        const M3<float> gC1 = ldm(io.g1, FC, S, p);
+        M3<float> Fc;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Fc.a[i] = F.a[i] + dF.a[i];
+        const M3<float> FF_ = mult(gF1, Fc);
+        M3<float> Ct;
+        float vt[3];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Ct.a[i] = gC1.a[i] + cs * dt * FF_.a[i];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) vt[a] = gv1[a] + dt * gx1[a];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) io.tmp[(long long)a * S + p] = vt[a];
+        stm(io.tmp, 3, S, p, Ct);
+        Axis<float> st[3];
+        stencil_f(sp, x, st, false);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int gi = st[0].base + i;
+            if (gi < 0 || gi >= sp.dim[0]) continue;
+            const float e0 = (float(i - 1) - st[0].f) * Dd;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int gj = st[1].base + j;
+                if (gj < 0 || gj >= sp.dim[1]) continue;
+                const float e1 = (float(j - 1) - st[1].f) * Dd;
+                const float wij = st[0].w[i] * st[1].w[j];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const int gk = st[2].base + kk;
+                    if (gk < 0 || gk >= sp.dim[2]) continue;
+                    const float e2 = (float(kk - 1) - st[2].f) * Dd;
+                    const float w = wij * st[2].w[kk];
+                    float val[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) val[a] = w * (vt[a] + Ct.a[3 * a] * e0 + Ct.a[3 * a + 1] * e1 + Ct.a[3 * a + 2] * e2);
+                    if (W.lds) {
+                        const int li = win_local(W, gi, gj, gk);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) atomicAdd(&lds[c * W.vol + li], val[c]);
+                    } else {
+                        const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) unsafeAtomicAdd(io.agrid + c * sp.G + node, val[c]);
+                    }
+                }
+            }
+        }
+    }
+    if (W.lds) {
+        __syncthreads();
+        win_flush<3>(sp, W, lds, io.agrid, sp.G);
+    }
+}
+
+// ------------------------------------------------------- transposed P2G ----
+// Gather half of P2G^T.  With G = -k P Fc^T + m C and b = m(1 - dt drag) v:
+//   Gb   = sum w pbar d^T          -> kept in tmp for the elastic kernel
+//   Cbar = m Gb,  vbar = m (1 - dt drag) sum w pbar,
+//   xbar = x' + sum [gw (m mbar + (b + G d).pbar + v_i.vt + D (Ct : v_i d^T))
+//                    - w (G^T pbar + D Ct^T v_i)].
+__global__ __launch_bounds__(kBlk) void k_p2g_adj_blk(StepParams sp, AdjIO<float> io, const int2* ranges, int cap)
+{
+    extern __shared__ float lds[];
+    const int2 rg = ranges[blockIdx.x];
+    const int S = sp.stride;
+    const int cp = sp.compat;
+    const Win W = range_window(sp, io.cur, S, rg.x, rg.y, cap);
+    const float* gv = io.grid + sp.G;
+    const float* pb = io.agrid;
+    const float* mb = io.agrid + 3 * sp.G;
+    if (W.lds) {
+        const float* src[7] = {pb, pb + sp.G, pb + 2 * sp.G, mb, gv, gv + sp.G, gv + 2 * sp.G};
+        win_load<7>(sp, W, lds, src);
+    }
+    __syncthreads();
+    const float dx = float(sp.dx);
+    const float Dinv = float(3.0 * sp.inv_dx * sp.inv_dx);
+    const float mgw = -float(sp.inv_dx);
+    const float dt = float(sp.dt);
+    const float sdrag = 1.f - dt * float(sp.drag);
+    const float bsc = (cp & 4) ? 1.f : sdrag;
+    const double kst = -3.0 * sp.inv_dx * sp.inv_dx * sp.dt;
+    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) {
+        const float m = io.cst[KM * S + p];
+        M3<float> Gm;
+        {
+            const M3<float> C = ldm(io.cur, FC, S, p);
+            const M3<float> F = ldm(io.cur, FF, S, p);
+            const M3<float> dF = ldm(io.ctrl, 0, S, p);
+            const double vol = io.cst[KVOL * S + p];
+            const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
+            M3<double> Fc;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
+            FCE e;
+            fce_prepare(Fc, e);
+            const M3<double> PF = mult(fce_pk1(Fc, e, lam, mu), Fc);
+            const double k = kst * vol;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Gm.a[i] = float(k * PF.a[i]) + m * C.a[i];
+        }
+        float x[3], v[3], vt[3];
+        ld3(io.cur, FX, S, p, x);
+        ld3(io.cur, FV, S, p, v);
+        ld3(io.tmp, 0, S, p, vt);
+        const M3<float> Ct = ldm(io.tmp, 3, S, p);
+        float b[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) b[a] = m * bsc * v[a];
+        Axis<float> st[3];
+        stencil_f(sp, x, st, true);
+        M3<float> Gb = m3_zero<float>();
+        float xb[3] = {0.f, 0.f, 0.f}, pacc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int gi = st[0].base + i;
+            if (gi < 0 || gi >= sp.dim[0]) continue;
+            const float d0 = (float(i - 1) - st[0].f) * dx;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int gj = st[1].base + j;
+                if (gj < 0 || gj >= sp.dim[1]) continue;
+                const float d1 = (float(j - 1) - st[1].f) * dx;
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const int gk = st[2].base + kk;
+                    if (gk < 0 || gk >= sp.dim[2]) continue;
+                    const float d2 = (float(kk - 1) - st[2].f) * dx;
+                    const float w0 = st[0].w[i], w1 = st[1].w[j], w2 = st[2].w[kk];
+                    const float w = w0 * w1 * w2;
+                    const float gw[3] = {mgw * st[0].dw[i] * w1 * w2, mgw * w0 * st[1].dw[j] * w2, mgw * w0 * w1 * st[2].dw[kk]};
+                    float pbi[3], vi[3], mbi;
+                    if (W.lds) {
+                        const int li = win_local(W, gi, gj, gk);
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            pbi[a] = lds[a * W.vol + li];
+                            vi[a] = lds[(4 + a) * W.vol + li];
+                        }
+                        mbi = lds[3 * W.vol + li];
+                    } else {
+                        const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            pbi[a] = pb[a * sp.G + node];
+                            vi[a] = gv[a * sp.G + node];
+                        }
+                        mbi = mb[node];
+                    }
+                    float bgd = 0.f, vdot = 0.f, cvd = 0.f;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const float gd = Gm.a[3 * a] * d0 + Gm.a[3 * a + 1] * d1 + Gm.a[3 * a + 2] * d2;
+                        bgd += (b[a] + gd) * pbi[a];
+                        vdot += vi[a] * vt[a];
+                        cvd += vi[a] * (Ct.a[3 * a] * d0 + Ct.a[3 * a + 1] * d1 + Ct.a[3 * a + 2] * d2);
+                        const float wp = w * pbi[a];
+                        pacc[a] += wp;
+                        Gb.a[3 * a] += wp * d0;
+                        Gb.a[3 * a + 1] += wp * d1;
+                        Gb.a[3 * a + 2] += wp * d2;
+                    }
+                    const float sgw = m * mbi + bgd + vdot + Dinv * cvd;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const float GTp = Gm.a[a] * pbi[0] + Gm.a[3 + a] * pbi[1] + Gm.a[6 + a] * pbi[2];
+                        const float CTv = Ct.a[a] * vi[0] + Ct.a[3 + a] * vi[1] + Ct.a[6 + a] * vi[2];
+                        xb[a] += gw[a] * sgw - w * (GTp + Dinv * CTv);
+                    }
+                }
+            }
+        }
+        float gx1[3];
+        ld3(io.g1, FX, S, p, gx1);
+        const float vsc = m * sdrag;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            io.g0[(long long)(FX + a) * S + p] = gx1[a] + xb[a];
+            io.g0[(long long)(FV + a) * S + p] = vsc * pacc[a];
+        }
+        M3<float> gC;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) gC.a[i] = m * Gb.a[i];
+        stm(io.g0, FC, S, p, gC);
+        stm(io.tmp, 0, S, p, Gb);  // vt, Ct are dead for this particle
+    }
+}
+
+// Elastic half of P2G^T + P_op_1^T, one thread per particle:
+//   Pbar = -k Gb Fc,  Fcbar = (1-s)(I + dt C')^T F' - k Gb^T P + H(Fc):Pbar.
+__global__ __launch_bounds__(kBlk) void k_elastic_adj(StepParams sp, AdjIO<float> io)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= sp.n) return;
+    const int S = sp.stride;
+    const int cp = sp.compat;
+    const M3<float> F = ldm(io.cur, FF, S, p);
+    const M3<float> dF = ldm(io.ctrl, 0, S, p);
+    const M3<float> Gb = ldm(io.tmp, 0, S, p);
+    const double vol = io.cst[KVOL * S + p];
+    const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
+    M3<double> Fc;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
+    FCE e;
+    fce_prepare(Fc, e);
+    const M3<double> P = fce_pk1(Fc, e, lam, mu);
+    const double kk_ = 3.0 * sp.inv_dx * sp.inv_dx * sp.dt * vol;
+    const M3<double> Gbd = cast<double>(Gb);
+    M3<double> Pb = mul(Gbd, Fc);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Pb.a[i] *= -kk_;
+    const M3<double> H = fce_hess_mult(e, lam, mu, Pb, cp == 0);
+    const M3<double> GTP = tmul(Gbd, P);
+    const M3<float> C1 = ldm(io.next, FC, S, p);
+    const M3<float> gF1 = ldm(io.g1, FF, S, p);
+    const float dt = float(sp.dt);
+    M3<float> IC;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) IC.a[i] = dt * C1.a[i];
+    IC.a[0] += 1.f;
+    IC.a[4] += 1.f;
+    IC.a[8] += 1.f;
+    const M3<float> base = tmul(IC, gF1);
+    const float s = float(sp.smoothing);
+    M3<float> gF, gdFc;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const float rest = float(-kk_ * GTP.a[i] + H.a[i]);
+        if (cp & 1) {
+            const float f = (1.f - s) * base.a[i] + s * gF1.a[i] + base.a[i] + rest;
+            gF.a[i] = f;
+            gdFc.a[i] = f;
+        } else {
+            gdFc.a[i] = (1.f - s) * base.a[i] + rest;
+            gF.a[i] = gdFc.a[i] + s * gF1.a[i];
+        }
+    }
+    stm(io.g0, FF, S, p, gF);
+    stm(io.gctrl, 0, S, p, gdFc);
+}
+
+// ------------------------------------------------------------ launchers ----
+void launch_p2g_blk(const StepParams& sp, const StepIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s)
+{
+    if (nranges <= 0) return;
+    hipLaunchKernelGGL(k_p2g_blk, dim3(nranges), dim3(kBlk), 4 * cap * sizeof(float), s, sp, io, ranges, cap);
+}
+
+void launch_g2p_blk(const StepParams& sp, const StepIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s)
+{
+    if (nranges <= 0) return;
+    hipLaunchKernelGGL(k_g2p_blk, dim3(nranges), dim3(kBlk), 3 * cap * sizeof(float), s, sp, io, ranges, cap);
+}
+
+void launch_g2p_adj_blk(const StepParams& sp, const AdjIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s)
+{
+    if (nranges <= 0) return;
+    hipLaunchKernelGGL(k_g2p_adj_blk, dim3(nranges), dim3(kBlk), 3 * cap * sizeof(float), s, sp, io, ranges, cap);
+}
+
+void launch_p2g_adj_blk(const StepParams& sp, const AdjIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s)
+{
+    if (nranges <= 0) return;
+    hipLaunchKernelGGL(k_p2g_adj_blk, dim3(nranges), dim3(kBlk), 7 * cap * sizeof(float), s, sp, io, ranges, cap);
+}
+
+void launch_elastic_adj(const StepParams& sp, const AdjIO<float>& io, hipStream_t s)
+{
+    if (sp.n <= 0) return;
+    hipLaunchKernelGGL(k_elastic_adj, dim3((sp.n + kBlk - 1) / kBlk), dim3(kBlk), 0, s, sp, io);
+}
+
+}  // namespace dmpm

@@ -63,9 +63,15 @@ template <typename Real> void launch_g2p_adj(const StepParams& sp, const AdjIO<R
 template <typename Real> void launch_grid_adj(const StepParams& sp, const AdjIO<Real>& io, hipStream_t s);
 template <typename Real> void launch_p2g_adj(const StepParams& sp, const AdjIO<Real>& io, hipStream_t s);
 
-// FP32 path: LDS-tiled scatters (cap = LDS window capacity in nodes)
-void launch_p2g_tiled(const StepParams& sp, const StepIO<float>& io, int cap, hipStream_t s);
-void launch_g2p_adj_tiled(const StepParams& sp, const AdjIO<float>& io, int cap, hipStream_t s);
+// FP32 path: one workgroup per particle range of a 4^3-cell block (ranges[i] =
+// {begin, end} in device order), node window staged in LDS (cap = window
+// capacity in nodes; larger windows fall back to direct HBM access)
+void launch_p2g_blk(const StepParams& sp, const StepIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s);
+void launch_g2p_blk(const StepParams& sp, const StepIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s);
+void launch_g2p_adj_blk(const StepParams& sp, const AdjIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s);
+void launch_p2g_adj_blk(const StepParams& sp, const AdjIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s);
+// FP32 path: elastic half of the transposed P2G (Gb from tmp -> dLdF, dL/d(dFc))
+void launch_elastic_adj(const StepParams& sp, const AdjIO<float>& io, hipStream_t s);
 
 // batched constitutive evaluations (AoS 9 doubles per matrix)
 void launch_eval_pk1(int n, const double* F, const double* lam, const double* mu, double* P, hipStream_t s);

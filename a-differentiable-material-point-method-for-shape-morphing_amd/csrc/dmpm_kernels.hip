@@ -18,8 +18,6 @@
 #include "dmpm_kernels.h"
 #include "dmpm_math.h"
 
-#include <climits>
-
 namespace dmpm {
 
 constexpr int kBlock = 256;
@@ -683,283 +681,6 @@ void launch_p2g_adj(const StepParams& sp, const AdjIO<Real>& io, hipStream_t s)
 {
     if (sp.n <= 0) return;
     hipLaunchKernelGGL(k_p2g_adj<Real>, dim3(nblocks(sp.n, kBlock)), dim3(kBlock), 0, s, sp, io);
-}
-
-// ------------------------------------------------- LDS-tiled scatters -----
-// FP32 path.  Particles are stored sorted by 4^3-cell block (Morton order
-// inside a block), so the 256 particles of a workgroup cover a few dozen
-// cells.  The workgroup reduces the bounding box of its particles' stencil
-// bases; when the box (plus the 3-node halo) fits the LDS window it
-// accumulates all 64-node contributions with LDS atomics and flushes each
-// non-zero window node to HBM once, otherwise it falls back to direct HBM
-// atomics.  The window is recomputed every launch, so correctness never
-// depends on how well the particles are sorted.
-struct Window {
-    int lo[3];
-    int w[3];
-    int vol;
-    bool lds;
-};
-
-__device__ __forceinline__ Window block_window(const StepParams& sp, const int base[3], bool valid, int cap)
-{
-    __shared__ int red[6][kBlock / 64];
-    __shared__ Window win;
-    int v[6];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        v[a] = valid ? base[a] : INT_MAX;
-        v[3 + a] = valid ? base[a] : INT_MIN;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            v[a] = min(v[a], __shfl_xor(v[a], off, 64));
-            v[3 + a] = max(v[3 + a], __shfl_xor(v[3 + a], off, 64));
-        }
-    }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0)
-#pragma unroll
-        for (int c = 0; c < 6; ++c) red[c][wid] = v[c];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int lo[3], hi[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = INT_MAX;
-            hi[a] = INT_MIN;
-            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-                lo[a] = min(lo[a], red[a][w]);
-                hi[a] = max(hi[a], red[3 + a][w]);
-            }
-        }
-        Window W;
-        long long vol = 1;
-        bool empty = false;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            // clamp in 64-bit: bases of far-away particles may sit near INT_MIN/INT_MAX
-            long long l = lo[a] < 0 ? 0 : lo[a];
-            long long h = (long long)hi[a] + 3;
-            if (h > sp.dim[a] - 1) h = sp.dim[a] - 1;
-            if (h < l) empty = true;
-            W.lo[a] = (int)l;
-            W.w[a] = empty ? 0 : (int)(h - l + 1);
-            vol *= empty ? 0 : (h - l + 1);
-        }
-        W.vol = empty ? 0 : (int)(vol > INT_MAX ? INT_MAX : vol);
-        W.lds = !empty && vol <= cap;
-        win = W;
-    }
-    __syncthreads();
-    return win;
-}
-
-__device__ __forceinline__ long long window_node(const StepParams& sp, const Window& W, int idx)
-{
-    const int lk = idx % W.w[2];
-    const int t = idx / W.w[2];
-    const int lj = t % W.w[1];
-    const int li = t / W.w[1];
-    return ((long long)(W.lo[0] + li) * sp.dim[1] + (W.lo[1] + lj)) * sp.dim[2] + (W.lo[2] + lk);
-}
-
-template <int NC>
-__device__ __forceinline__ void window_flush(const StepParams& sp, const Window& W, const float* lds, float* const dst[NC])
-{
-    for (int idx = threadIdx.x; idx < W.vol; idx += blockDim.x) {
-        const long long node = window_node(sp, W, idx);
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const float v = lds[c * W.vol + idx];
-            if (v != 0.f) unsafeAtomicAdd(dst[c] + node, v);
-        }
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_p2g_tiled(StepParams sp, StepIO<float> io, int cap)
-{
-    extern __shared__ float lds[];
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = p < sp.n;
-    const int S = sp.stride;
-    const int q = valid ? p : 0;
-    float x[3];
-    load_vec3(io.cur, FX, S, q, x);
-    Stencil<float> st;
-    make_stencil(sp, x, st, false);
-    const int base[3] = {st.ax[0].base, st.ax[1].base, st.ax[2].base};
-    const Window W = block_window(sp, base, valid, cap);
-    if (W.lds)
-        for (int i = threadIdx.x; i < 4 * W.vol; i += blockDim.x) lds[i] = 0.f;
-
-    float v[3];
-    load_vec3(io.cur, FV, S, q, v);
-    M3<float> C = load_mat(io.cur, FC, S, q);
-    M3<float> F = load_mat(io.cur, FF, S, q);
-    M3<float> dF = load_mat(io.ctrl, 0, S, q);
-    const float m = io.cst[KM * S + q];
-    const double vol = io.cst[KVOL * S + q];
-    const double lam = io.cst[KLAM * S + q], mu = io.cst[KMU * S + q];
-    M3<double> Fc;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
-    FCE e;
-    fce_prepare(Fc, e);
-    M3<double> P = fce_pk1(Fc, e, lam, mu);
-    M3<double> PF = mult(P, Fc);
-    const double k = -3.0 * sp.inv_dx * sp.inv_dx * sp.dt * vol;
-    M3<float> A;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) A.a[i] = float(k * PF.a[i]) + m * C.a[i];
-    const float sdrag = 1.f - float(sp.dt) * float(sp.drag);
-    float mv[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) mv[a] = m * v[a] * sdrag;
-    if (W.lds) __syncthreads();
-
-    if (valid) {
-        const float dx = float(sp.dx);
-        float* gm = io.grid;
-        float* gp = io.grid + sp.G;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int gi = st.ax[0].base + i;
-            if (gi < 0 || gi >= sp.dim[0]) continue;
-            const float d0 = (float(i - 1) - st.ax[0].f) * dx;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int gj = st.ax[1].base + j;
-                if (gj < 0 || gj >= sp.dim[1]) continue;
-                const float d1 = (float(j - 1) - st.ax[1].f) * dx;
-                const float wij = st.ax[0].w[i] * st.ax[1].w[j];
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const int gk = st.ax[2].base + kk;
-                    if (gk < 0 || gk >= sp.dim[2]) continue;
-                    const float d2 = (float(kk - 1) - st.ax[2].f) * dx;
-                    const float w = wij * st.ax[2].w[kk];
-                    float val[4];
-                    val[0] = w * m;
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) val[1 + a] = w * (mv[a] + A.a[3 * a] * d0 + A.a[3 * a + 1] * d1 + A.a[3 * a + 2] * d2);
-                    if (W.lds) {
-                        const int li = ((gi - W.lo[0]) * W.w[1] + (gj - W.lo[1])) * W.w[2] + (gk - W.lo[2]);
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) atomicAdd(&lds[c * W.vol + li], val[c]);
-                    } else {
-                        const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
-                        unsafeAtomicAdd(gm + node, val[0]);
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) unsafeAtomicAdd(gp + a * sp.G + node, val[1 + a]);
-                    }
-                }
-            }
-        }
-    }
-    if (W.lds) {
-        __syncthreads();
-        float* const dst[4] = {io.grid, io.grid + sp.G, io.grid + 2 * sp.G, io.grid + 3 * sp.G};
-        window_flush<4>(sp, W, lds, dst);
-    }
-}
-
-void launch_p2g_tiled(const StepParams& sp, const StepIO<float>& io, int cap, hipStream_t s)
-{
-    if (sp.n <= 0) return;
-    hipLaunchKernelGGL(k_p2g_tiled, dim3(nblocks(sp.n, kBlock)), dim3(kBlock), 4 * cap * sizeof(float), s, sp, io, cap);
-}
-
-__global__ __launch_bounds__(kBlock) void k_g2p_adj_tiled(StepParams sp, AdjIO<float> io, int cap)
-{
-    extern __shared__ float lds[];
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = p < sp.n;
-    const int S = sp.stride;
-    const int q = valid ? p : 0;
-    float x[3];
-    load_vec3(io.cur, FX, S, q, x);
-    Stencil<float> st;
-    make_stencil(sp, x, st, false);
-    const int base[3] = {st.ax[0].base, st.ax[1].base, st.ax[2].base};
-    const Window W = block_window(sp, base, valid, cap);
-    if (W.lds)
-        for (int i = threadIdx.x; i < 3 * W.vol; i += blockDim.x) lds[i] = 0.f;
-
-    float gx1[3], gv1[3];
-    load_vec3(io.g1, FX, S, q, gx1);
-    load_vec3(io.g1, FV, S, q, gv1);
-    M3<float> F = load_mat(io.cur, FF, S, q);
-    M3<float> dF = load_mat(io.ctrl, 0, S, q);
-    M3<float> gF1 = load_mat(io.g1, FF, S, q);
-    M3<float> gC1 = load_mat(io.g1, FC, S, q);
-    M3<float> Fc;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Fc.a[i] = F.a[i] + dF.a[i];
-    const float cs = (sp.compat & 2) ? 1.f : 1.f - float(sp.smoothing);
-    M3<float> FF_ = mult(gF1, Fc);
-    M3<float> Ct;
-    float vt[3];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Ct.a[i] = gC1.a[i] + cs * float(sp.dt) * FF_.a[i];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) vt[a] = gv1[a] + float(sp.dt) * gx1[a];
-    if (valid) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) io.tmp[(long long)a * S + p] = vt[a];
-        store_mat(io.tmp, 3, S, p, Ct);
-    }
-    if (W.lds) __syncthreads();
-
-    if (valid) {
-        const float Dd = float(3.0 * sp.inv_dx);
-        float* vb = io.agrid;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int gi = st.ax[0].base + i;
-            if (gi < 0 || gi >= sp.dim[0]) continue;
-            const float e0 = (float(i - 1) - st.ax[0].f) * Dd;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int gj = st.ax[1].base + j;
-                if (gj < 0 || gj >= sp.dim[1]) continue;
-                const float e1 = (float(j - 1) - st.ax[1].f) * Dd;
-                const float wij = st.ax[0].w[i] * st.ax[1].w[j];
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const int gk = st.ax[2].base + kk;
-                    if (gk < 0 || gk >= sp.dim[2]) continue;
-                    const float e2 = (float(kk - 1) - st.ax[2].f) * Dd;
-                    const float w = wij * st.ax[2].w[kk];
-                    float val[3];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) val[a] = w * (vt[a] + Ct.a[3 * a] * e0 + Ct.a[3 * a + 1] * e1 + Ct.a[3 * a + 2] * e2);
-                    if (W.lds) {
-                        const int li = ((gi - W.lo[0]) * W.w[1] + (gj - W.lo[1])) * W.w[2] + (gk - W.lo[2]);
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) atomicAdd(&lds[c * W.vol + li], val[c]);
-                    } else {
-                        const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) unsafeAtomicAdd(vb + a * sp.G + node, val[a]);
-                    }
-                }
-            }
-        }
-    }
-    if (W.lds) {
-        __syncthreads();
-        float* const dst[3] = {io.agrid, io.agrid + sp.G, io.agrid + 2 * sp.G};
-        window_flush<3>(sp, W, lds, dst);
-    }
-}
-
-void launch_g2p_adj_tiled(const StepParams& sp, const AdjIO<float>& io, int cap, hipStream_t s)
-{
-    if (sp.n <= 0) return;
-    hipLaunchKernelGGL(k_g2p_adj_tiled, dim3(nblocks(sp.n, kBlock)), dim3(kBlock), 3 * cap * sizeof(float), s, sp, io, cap);
 }
 
 // ------------------------------------------------- batched constitutive ----

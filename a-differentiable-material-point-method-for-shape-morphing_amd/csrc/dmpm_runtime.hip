@@ -69,7 +69,10 @@ struct dmpm_ctx {
     // FP32 path: device order of the particles (sorted by 4^3-cell block);
     // pos[p] is the device slot of API particle p (empty = identity)
     std::vector<int> pos;
-    int lds_cap = 2560;       // LDS window of the tiled scatters, in nodes
+    int2* ranges = nullptr;   // per-workgroup particle ranges of the block kernels
+    std::vector<int2> host_ranges;
+    int n_ranges = 0;
+    int lds_cap = 1024;       // LDS node window of the block kernels
     // checkpoint-mode slot layout
     int n_ckpt = 0, seg_last = 0, tape_base = 0, roll_base = 0, n_slots = 0;
     int tape_seg = -1;
@@ -217,12 +220,21 @@ void fill(dmpm_ctx* c, Real* p, long long count, Real v)
     timed(c, DMPM_K_FILL, [&] { launch_fill<Real>(p, count, v, c->stream); });
 }
 
-// Scatter kernels: the plain FP64 path uses direct HBM atomics, the FP32
-// path the LDS-tiled kernels over block-sorted particles.
+// Particle-grid kernels: the plain FP64 path runs one thread per particle
+// with direct HBM atomics; the FP32 path runs one workgroup per block range
+// with the node window staged in LDS.
 inline void p2g_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<double>& io) { launch_p2g<double>(sp, io, c->stream); }
-inline void p2g_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<float>& io) { launch_p2g_tiled(sp, io, c->lds_cap, c->stream); }
+inline void p2g_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<float>& io) { launch_p2g_blk(sp, io, c->ranges, c->n_ranges, c->lds_cap, c->stream); }
+inline void g2p_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<double>& io) { launch_g2p<double>(sp, io, c->stream); }
+inline void g2p_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<float>& io) { launch_g2p_blk(sp, io, c->ranges, c->n_ranges, c->lds_cap, c->stream); }
 inline void g2p_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<double>& io) { launch_g2p_adj<double>(sp, io, c->stream); }
-inline void g2p_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<float>& io) { launch_g2p_adj_tiled(sp, io, c->lds_cap, c->stream); }
+inline void g2p_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<float>& io) { launch_g2p_adj_blk(sp, io, c->ranges, c->n_ranges, c->lds_cap, c->stream); }
+inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<double>& io) { launch_p2g_adj<double>(sp, io, c->stream); }
+inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<float>& io)
+{
+    launch_p2g_adj_blk(sp, io, c->ranges, c->n_ranges, c->lds_cap, c->stream);
+    launch_elastic_adj(sp, io, c->stream);
+}
 
 // One ForwardTimeStep: cur -> nxt, grid of the step in g.
 template <typename Real>
@@ -233,7 +245,7 @@ void step_forward(dmpm_ctx* c, const Real* cur, Real* nxt, const Real* ctrl, Rea
     fill<Real>(c, g, (long long)kGridFields * c->G, Real(0));
     timed(c, DMPM_K_P2G, [&] { p2g_dispatch(c, sp, io); });
     timed(c, DMPM_K_GRID_UPDATE, [&] { launch_grid_update<Real>(sp, g, c->stream); });
-    timed(c, DMPM_K_G2P, [&] { launch_g2p<Real>(sp, io, c->stream); });
+    timed(c, DMPM_K_G2P, [&] { g2p_dispatch(c, sp, io); });
 }
 
 // One Back_Timestep: gradients of layer i+1 -> layer i.
@@ -255,7 +267,7 @@ void step_backward(dmpm_ctx* c, int i, const Real* cur, const Real* nxt, const R
     fill<Real>(c, io.agrid, 3 * c->G, Real(0));
     timed(c, DMPM_K_G2P_ADJ, [&] { g2p_adj_dispatch(c, sp, io); });
     timed(c, DMPM_K_GRID_ADJ, [&] { launch_grid_adj<Real>(sp, io, c->stream); });
-    timed(c, DMPM_K_P2G_ADJ, [&] { launch_p2g_adj<Real>(sp, io, c->stream); });
+    timed(c, DMPM_K_P2G_ADJ, [&] { p2g_adj_dispatch(c, sp, io); });
 }
 
 // Bring layer j (<= fwd_valid) into a readable slot and return it.  Checkpoint
@@ -413,15 +425,18 @@ int download_fields(dmpm_ctx* c, const Real* src, int field0, int nfield, double
     return DMPM_OK;
 }
 
-// Device order of the FP32 path: particles sorted by the 4^3-cell block of
-// their stencil base cell (blocks in x-major order, Morton order of the 64
-// cells inside a block), ties kept in API order.
+// Device order of the FP32 path.  Particles are grouped by the 4^3-cell
+// block of their cell (blocks in x-major order); inside a block they are
+// ordered by their rank within their cell ("slot"), then by the cell's Morton
+// index.  A 64-lane wave then holds one particle of each of the block's 64
+// cells: its LDS atomics hit 64 distinct nodes instead of 8 particles of one
+// cell hitting the same node.  Ties keep the API order.
 void sort_particles(dmpm_ctx* c, const double* x)
 {
     const int N = c->N;
     int nb[3];
     for (int a = 0; a < 3; ++a) nb[a] = (c->cfg.dim[a] + 3) / 4;
-    std::vector<uint64_t> key(N);
+    std::vector<uint64_t> cellkey(N);
     for (int p = 0; p < N; ++p) {
         int cell[3];
         for (int a = 0; a < 3; ++a) {
@@ -434,13 +449,32 @@ void sort_particles(dmpm_ctx* c, const double* x)
         uint32_t mort = 0;
         for (int bit = 0; bit < 2; ++bit)
             for (int a = 0; a < 3; ++a) mort |= (uint32_t)((cell[a] >> bit) & 1) << (3 * bit + (2 - a));
-        key[p] = (block << 6) | mort;
+        cellkey[p] = (block << 6) | mort;
     }
     std::vector<int> order(N);
     for (int p = 0; p < N; ++p) order[p] = p;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cellkey[a] < cellkey[b]; });
+    // slot = rank inside the cell; final key (block, slot, Morton cell)
+    std::vector<uint64_t> key(N);
+    for (int d = 0; d < N; ++d) {
+        const int p = order[d];
+        const uint32_t slot = (d > 0 && cellkey[order[d - 1]] == cellkey[p]) ? (uint32_t)(key[order[d - 1]] >> 6 & 0xffffu) + 1u : 0u;
+        key[p] = ((cellkey[p] >> 6) << 22) | ((uint64_t)(slot & 0xffffu) << 6) | (cellkey[p] & 63u);
+    }
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
     c->pos.assign(N, 0);
     for (int d = 0; d < N; ++d) c->pos[order[d]] = d;
+    // one workgroup range per block (split above kMaxRange particles)
+    constexpr int kMaxRange = 1024;
+    std::vector<int2> rg;
+    int d0 = 0;
+    for (int d = 1; d <= N; ++d) {
+        if (d == N || (key[order[d]] >> 22) != (key[order[d0]] >> 22) || d - d0 == kMaxRange) {
+            rg.push_back(make_int2(d0, d));
+            d0 = d;
+        }
+    }
+    c->host_ranges.swap(rg);
 }
 
 template <typename Real>
@@ -448,7 +482,18 @@ int set_particles(dmpm_ctx* c, const double* x, const double* v, const double* F
                   const double* vol, const double* lam, const double* mu)
 {
     if (!x || !v || !F || !C || !m || !lam || !mu) return fail(DMPM_ERR_INVALID, "null particle array");
-    if (c->f32) sort_particles(c, x);
+    if (c->f32) {
+        sort_particles(c, x);
+        if (c->ranges) {
+            hipFree(c->ranges);
+            c->ranges = nullptr;
+        }
+        c->n_ranges = (int)c->host_ranges.size();
+        if (c->n_ranges > 0) {
+            HIP_TRY(hipMalloc(&c->ranges, c->host_ranges.size() * sizeof(int2)));
+            HIP_TRY(hipMemcpy(c->ranges, c->host_ranges.data(), c->host_ranges.size() * sizeof(int2), hipMemcpyHostToDevice));
+        }
+    }
     // a new particle set starts with zero controls (MaterialPoint.h:31)
     fill<Real>(c, static_cast<Real*>(c->ctrl), (long long)(c->L - 1) * 9 * c->S, Real(0));
     Real* l0 = slot<Real>(c, 0);
@@ -586,6 +631,25 @@ int get_layer_grads(dmpm_ctx* c, int layer, double* gx, double* gv, double* gF, 
     if (gv && (rc = download_fields<Real>(c, g, FV, 3, gv, c->N))) return rc;
     if (gF && (rc = download_fields<Real>(c, g, FF, 9, gF, c->N))) return rc;
     if (gC && (rc = download_fields<Real>(c, g, FC, 9, gC, c->N))) return rc;
+    return DMPM_OK;
+}
+
+template <typename Real>
+int set_layer_grads(dmpm_ctx* c, int layer, const double* gx, const double* gv, const double* gF, const double* gC)
+{
+    if (layer < 0 || layer >= c->L) return fail(DMPM_ERR_INVALID, "layer out of range");
+    if (layer > c->fwd_valid) return fail(DMPM_ERR_STATE, "layer not computed by a forward pass");
+    Real* g = grads_of<Real>(c, layer);
+    fill<Real>(c, g, (long long)kLayerFields * c->S, Real(0));
+    int rc;
+    if (gx && (rc = upload_fields<Real>(c, g, FX, 3, gx, c->N))) return rc;
+    if (gv && (rc = upload_fields<Real>(c, g, FV, 3, gv, c->N))) return rc;
+    if (gF && (rc = upload_fields<Real>(c, g, FF, 9, gF, c->N))) return rc;
+    if (gC && (rc = upload_fields<Real>(c, g, FC, 9, gC, c->N))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->grads_lo = layer;
+    c->bwd_control = -1;
+    c->loss_valid = (layer == c->L - 1);
     return DMPM_OK;
 }
 
@@ -732,7 +796,7 @@ int dmpm_destroy(dmpm_ctx* c)
     if (!c) return DMPM_OK;
     hipSetDevice(c->cfg.device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    void* bufs[] = {c->cst, c->ctrl, c->layers, c->grids, c->wgrid, c->agrid, c->gbuf[0], c->gbuf[1], c->gctrl, c->tmp, c->target, c->dacc};
+    void* bufs[] = {c->cst, c->ctrl, c->layers, c->grids, c->wgrid, c->agrid, c->gbuf[0], c->gbuf[1], c->gctrl, c->tmp, c->target, c->dacc, c->ranges};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t ev : c->pool) hipEventDestroy(ev);
@@ -897,6 +961,14 @@ int dmpm_get_layer_grads(dmpm_ctx* c, int32_t layer, double* dLdx, double* dLdv,
     int rc = check_ctx(c);
     if (rc) return rc;
     return DISPATCH(c, get_layer_grads, c, layer, dLdx, dLdv, dLdF, dLdC);
+}
+
+int dmpm_set_layer_grads(dmpm_ctx* c, int32_t layer, const double* dLdx, const double* dLdv, const double* dLdF,
+                         const double* dLdC)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    return DISPATCH(c, set_layer_grads, c, layer, dLdx, dLdv, dLdF, dLdC);
 }
 
 int dmpm_get_control_grad(dmpm_ctx* c, int32_t layer, double* out)

@@ -125,6 +125,11 @@ int dmpm_set_layer_state(dmpm_ctx* ctx, int32_t layer, const double* x, const do
 /* Gradients dLdx, dLdv, dLdF, dLdC (MaterialPoint.h:34-37) of the lowest layer reached by the last
  * backward pass or of the layer above it (the last layer after the loss). */
 int dmpm_get_layer_grads(dmpm_ctx* ctx, int32_t layer, double* dLdx, double* dLdv, double* dLdF, double* dLdC);
+/* Seed the gradients of a layer (MaterialPoint dLdx, dLdv, dLdF, dLdC; NULL = zero): on the last
+ * layer it replaces EndLayerMassLoss's seed (a caller-defined loss) for dmpm_backward_pass; on any
+ * layer it is the input of dmpm_backward_step(layer-1) (Back_Timestep reads layer n+1's gradients). */
+int dmpm_set_layer_grads(dmpm_ctx* ctx, int32_t layer, const double* dLdx, const double* dLdv, const double* dLdF,
+                         const double* dLdC);
 /* dL/d(dFc_layer): the control layer of the last backward pass, or any of its layers with
  * keep_control_grads.  Exact mode: dL/d(dFc); compat mode: the reference's dLdF. */
 int dmpm_get_control_grad(dmpm_ctx* ctx, int32_t layer, double* out);
