@@ -21,6 +21,14 @@ ARCH = os.environ.get("DMPM_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["dmpm_kernels.hip", "dmpm_blocks.hip", "dmpm_runtime.hip"]
 
+# C++ host mirror of the reference API (g++, links libdmpm.so) and its self-test driver
+HOST_LIB = os.path.join(LIBDIR, "libdmpm_cpp.so")
+HOST_SOURCES = [os.path.join(HERE, "host", "dmpm_host.cpp")]
+SELFTEST_SRC = os.path.join(ROOT, "tests", "cpp", "compgraph_selftest.cpp")
+SELFTEST_BIN = os.path.join(ROOT, "tests", "cpp", "compgraph_selftest")
+CXX = os.environ.get("CXX", "g++")
+CXXFLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include")]
+
 FLAGS = [
     "-O3",
     "-std=c++17",
@@ -54,6 +62,12 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, *FLAGS, "-shared", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)
+    tmp = HOST_LIB + ".tmp"
+    _run([CXX, *CXXFLAGS, "-shared", *HOST_SOURCES, "-L", LIBDIR, "-ldmpm", "-Wl,-rpath,$ORIGIN", "-o", tmp], verbose)
+    os.replace(tmp, HOST_LIB)
+    _run([CXX, *CXXFLAGS, SELFTEST_SRC, "-L", LIBDIR, "-ldmpm_cpp", "-ldmpm", f"-Wl,-rpath,{LIBDIR}",
+          "-Wl,-rpath,$ORIGIN/../../a-differentiable-material-point-method-for-shape-morphing_amd/lib", "-o", SELFTEST_BIN],
+         verbose)
     return LIB
 
 

@@ -1,0 +1,100 @@
+"""The C++ host mirror of the reference API (include/DiffMPMLib3D/DiffMPMLib3D.h).
+
+CPU: the .mpmbin format round trip through the C++ reader/writer.
+GPU: CompGraph passes, EndLayerMassLoss, Back_Timestep, ForwardTimeStep and
+PK1 through the mirror, against the FP64 oracle in reference-compat mode
+(the mirror's default).  Tolerances: the mirror stores float fields like the
+reference, so states compare to 1e-6 and gradients to 1e-5 relative.
+"""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from _scenes import make_case, maxrel, orc, pkg, rel
+
+BIN = os.path.join(ROOT, "tests", "cpp", "compgraph_selftest")
+mpmbin = pkg.mpmbin
+
+
+def _rand_cloud(n, seed):
+    rng = np.random.default_rng(seed)
+    return dict(x=rng.normal(size=(n, 3)), v=rng.normal(size=(n, 3)), F=rng.normal(size=(n, 3, 3)),
+                C=rng.normal(size=(n, 3, 3)), P=rng.normal(size=(n, 3, 3)), m=rng.uniform(size=n),
+                vol=rng.uniform(size=n), lam=rng.uniform(size=n), mu=rng.uniform(size=n),
+                dFc=rng.normal(size=(n, 3, 3)), dLdx=rng.normal(size=(n, 3)), dLdv=rng.normal(size=(n, 3)),
+                dLdF=rng.normal(size=(n, 3, 3)), momentum=rng.normal(size=(n, 3, 3)),
+                vector=rng.normal(size=(n, 3, 3)), vector_max=rng.normal(size=(n, 3, 3)))
+
+
+def test_mpmbin_layout_and_cpp_roundtrip(tmp_path):
+    fields = _rand_cloud(37, 1)
+    a = tmp_path / "a.mpmbin"
+    b = tmp_path / "b.mpmbin"
+    mpmbin.write(a, **fields)
+    assert os.path.getsize(a) == 8 + 37 * 432
+    raw = open(a, "rb").read()
+    assert int.from_bytes(raw[:8], "little") == 37
+    assert int.from_bytes(raw[8:12], "little") == 3  # x rows
+    assert np.frombuffer(raw[12:24], "<f4").tolist() == np.float32(fields["x"][0]).tolist()
+    subprocess.run([BIN, "roundtrip", str(a), str(b)], check=True, timeout=60)
+    assert open(a, "rb").read() == open(b, "rb").read()
+    back = mpmbin.read(b)
+    for k, v in fields.items():
+        assert np.array_equal(back[k], np.float32(v).astype(np.float64)), k
+
+
+def test_mpmbin_rejects_truncated(tmp_path):
+    a = tmp_path / "a.mpmbin"
+    mpmbin.write(a, **_rand_cloud(3, 2))
+    raw = open(a, "rb").read()
+    open(a, "wb").write(raw[:-5])
+    with pytest.raises(ValueError):
+        mpmbin.read(a)
+    r = subprocess.run([BIN, "roundtrip", str(a), str(tmp_path / "b")], timeout=60)
+    assert r.returncode == 2
+
+
+@pytest.mark.gpu
+def test_cpp_mirror_compgraph_vs_oracle(tmp_path):
+    case = make_case(L=4, per_layer_ctrl=False, drag=0.2, smoothing=0.05)
+    sc, st, cs = case.sc, case.st0, case.cs
+    # the mirror reads float fields: give the oracle the same float-rounded inputs
+    f32 = lambda a: np.float32(a).astype(np.float64)  # noqa: E731
+    st = orc.State(f32(st.x), f32(st.v), f32(st.C), f32(st.F))
+    cs = orc.Consts(f32(cs.m), f32(cs.vol), f32(cs.lam), f32(cs.mu))
+    dFc = f32(case.controls[0])
+    prm = orc.Params(**{**case.prm.__dict__, "compat": orc.COMPAT_ALL, "dx": float(np.float32(sc.dx)),
+                        "dt": float(np.float32(case.prm.dt)), "drag": float(np.float32(0.2)),
+                        "smoothing": float(np.float32(0.05))})
+    target = f32(case.target)
+    mpmbin.write(tmp_path / "in.mpmbin", x=st.x, v=st.v, F=st.F, C=st.C, m=cs.m, vol=cs.vol, lam=cs.lam, mu=cs.mu,
+                 dFc=dFc)
+    target.astype("<f8").tofile(tmp_path / "target.bin")
+    with open(tmp_path / "params.txt", "w") as fh:
+        fh.write(f"{sc.dim[0]} {sc.dim[1]} {sc.dim[2]} {sc.dx!r} 0 0 0 {prm.dt!r} {prm.drag!r} {prm.smoothing!r} 0 0 0 "
+                 f"{case.L} {orc.COMPAT_ALL}\n")
+    subprocess.run([BIN, "run", str(tmp_path)], check=True, timeout=300)
+
+    controls = [dFc] * (case.L - 1)  # SetUpCompGraph copies layer 0 (and its dFc) into every layer
+    ref_loss, layers, gd, g0 = orc.episode(prm, st, cs, controls, target, case.L)
+    loss = float(open(tmp_path / "loss.txt").read())
+    assert loss == pytest.approx(ref_loss, rel=1e-6)
+    last = mpmbin.read(tmp_path / "last.mpmbin")
+    assert maxrel(last["x"], layers[-1].x) < 1e-6
+    assert maxrel(last["F"], layers[-1].F) < 1e-6
+    l0 = mpmbin.read(tmp_path / "l0.mpmbin")
+    assert rel(l0["dLdF"], g0.F) < 1e-5
+    assert rel(l0["dLdx"], g0.x) < 1e-5
+    bt = mpmbin.read(tmp_path / "bt.mpmbin")
+    assert rel(bt["dLdF"], g0.F) < 1e-5 and rel(bt["dLdx"], g0.x) < 1e-5
+    step = mpmbin.read(tmp_path / "step.mpmbin")
+    assert maxrel(step["x"], layers[1].x) < 1e-6 and maxrel(step["F"], layers[1].F) < 1e-6
+    P = np.array([float(t) for t in open(tmp_path / "pk1.txt").read().split()]).reshape(3, 3)
+    s = 0.8
+    R = np.array([[1, 0, 0], [0, math.cos(math.pi / 4), -math.sin(math.pi / 4)], [0, math.sin(math.pi / 4), math.cos(math.pi / 4)]])
+    lam, mu = 58333.0, 38888.9
+    assert np.allclose(P, (2 * mu * (s - 1) + lam * (s ** 3 - 1) * s ** 2) * R, rtol=1e-5, atol=1e-2)
