@@ -25,7 +25,9 @@ namespace dmpm {
 
 namespace {
 
-constexpr int kBlk = 256;
+constexpr int kBlk = 256;      // gather kernels
+constexpr int kScat = 512;     // scatter kernels: one particle per thread of a block range
+constexpr int kMaxWaves = 8;
 
 struct Win {
     int lo[3];
@@ -49,7 +51,7 @@ __device__ __forceinline__ int base_of(const StepParams& sp, float x, int a)
 // Bounding box of the stencil bases of particles [r0, r1), clamped to the grid.
 __device__ Win range_window(const StepParams& sp, const float* lay, int S, int r0, int r1, int cap)
 {
-    __shared__ int red[6][kBlk / 64];
+    __shared__ int red[6][kMaxWaves];
     __shared__ Win win;
     int v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
     for (int p = r0 + (int)threadIdx.x; p < r1; p += blockDim.x) {
@@ -114,19 +116,6 @@ __device__ __forceinline__ int win_local(const Win& W, int gi, int gj, int gk)
     return ((gi - W.lo[0]) * W.w[1] + (gj - W.lo[1])) * W.w[2] + (gk - W.lo[2]);
 }
 
-template <int NC>
-__device__ __forceinline__ void win_flush(const StepParams& sp, const Win& W, const float* lds, float* base, long long cstride)
-{
-    for (int idx = threadIdx.x; idx < W.vol; idx += blockDim.x) {
-        const long long node = win_node(sp, W, idx);
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const float val = lds[c * W.vol + idx];
-            if (val != 0.f) unsafeAtomicAdd(base + c * cstride + node, val);
-        }
-    }
-}
-
 // load NC channels (channel c at src_c[node]) of the window into LDS
 template <int NC>
 __device__ __forceinline__ void win_load(const StepParams& sp, const Win& W, float* lds, const float* const src[NC])
@@ -163,44 +152,132 @@ __device__ __forceinline__ void stm(T* base, int field, int S, int p, const M3<T
 
 }  // namespace
 
-// ------------------------------------------------------------------ P2G ----
-__global__ __launch_bounds__(kBlk) void k_p2g_blk(StepParams sp, StepIO<float> io, const int2* ranges, int cap)
+// Scatter accumulation.  LDS float atomics are serialised lane by lane on
+// gfx950 (ds_add_f32 ~0.3 lane-ops/clk/CU against ~9 for ds_add_u64), so the
+// window is accumulated in 64-bit fixed point: each channel c of a workgroup
+// gets a power-of-two scale s_c with sum_p B_pc * s_c <= 2^61, where B_pc
+// bounds particle p's total contribution to channel c (its weights sum to 1),
+// every contribution is rounded to an integer multiple of 1/s_c and added
+// exactly with ds_add_u64.  The in-window sums are therefore exact and
+// independent of the order of the additions; the float rounding of each term
+// (|rel| <= 2^-24) is the only error.  A workgroup whose bound is not finite
+// or whose window overflows the LDS capacity falls back to HBM float atomics.
+template <int NC>
+struct Scales {
+    float s[NC];
+    float inv[NC];
+    bool ok;
+};
+
+// block-wide sum of NC per-thread bounds -> power-of-two scales (all threads)
+template <int NC>
+__device__ Scales<NC> block_scales(const float bound[NC])
 {
-    extern __shared__ float lds[];
+    __shared__ float red[NC][kMaxWaves];
+    __shared__ Scales<NC> sc;
+    float v[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = bound[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[c] += __shfl_xor(v[c], off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) red[c][wid] = v[c];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Scales<NC> S;
+        S.ok = true;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            float t = 0.f;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[c][w];
+            if (!(t < 3.0e38f)) S.ok = false;  // inf or NaN
+            int e = 61;
+            if (t > 0.f) {
+                int ex;
+                frexpf(t * 2.f, &ex);  // 2t <= 2^ex
+                e = 61 - ex;
+            }
+            e = e < -120 ? -120 : (e > 120 ? 120 : e);
+            S.s[c] = ldexpf(1.f, e);
+            S.inv[c] = ldexpf(1.f, -e);
+        }
+        sc = S;
+    }
+    __syncthreads();
+    return sc;
+}
+
+__device__ __forceinline__ unsigned long long to_fixed(float v, float scale)
+{
+    return (unsigned long long)__float2ll_rn(v * scale);
+}
+
+template <int NC>
+__device__ __forceinline__ void win_flush_fixed(const StepParams& sp, const Win& W, const unsigned long long* lds,
+                                                const Scales<NC>& S, float* base, long long cstride)
+{
+    for (int idx = threadIdx.x; idx < W.vol; idx += blockDim.x) {
+        const long long node = win_node(sp, W, idx);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const long long q = (long long)lds[c * W.vol + idx];
+            if (q != 0) unsafeAtomicAdd(base + c * cstride + node, (float)((double)q * (double)S.inv[c]));
+        }
+    }
+}
+
+// ------------------------------------------------------------------ P2G ----
+__global__ __launch_bounds__(kScat) void k_p2g_blk(StepParams sp, StepIO<float> io, const int2* ranges, int cap)
+{
+    extern __shared__ unsigned long long lds64[];
     const int2 rg = ranges[blockIdx.x];
     const int S = sp.stride;
-    const Win W = range_window(sp, io.cur, S, rg.x, rg.y, cap);
-    if (W.lds)
-        for (int i = threadIdx.x; i < 4 * W.vol; i += blockDim.x) lds[i] = 0.f;
-    __syncthreads();
+    const int p = rg.x + (int)threadIdx.x;
+    const bool valid = p < rg.y;
+    const Win W0 = range_window(sp, io.cur, S, rg.x, rg.y, cap);
     const float dx = float(sp.dx);
     const float sdrag = 1.f - float(sp.dt) * float(sp.drag);
     const double kst = -3.0 * sp.inv_dx * sp.inv_dx * sp.dt;
-    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) {
-        float x[3], v[3];
+    float x[3] = {0.f, 0.f, 0.f}, mv[3] = {0.f, 0.f, 0.f}, m = 0.f;
+    M3<float> A = m3_zero<float>();
+    float bound[4] = {0.f, 0.f, 0.f, 0.f};
+    if (valid) {
+        float v[3];
         ld3(io.cur, FX, S, p, x);
         ld3(io.cur, FV, S, p, v);
         const M3<float> C = ldm(io.cur, FC, S, p);
         const M3<float> F = ldm(io.cur, FF, S, p);
         const M3<float> dF = ldm(io.ctrl, 0, S, p);
-        const float m = io.cst[KM * S + p];
+        m = io.cst[KM * S + p];
         const double vol = io.cst[KVOL * S + p];
         const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
-        M3<float> A;
-        {
-            M3<double> Fc;
+        M3<double> Fc;
 #pragma unroll
-            for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
-            FCE e;
-            fce_prepare(Fc, e);
-            const M3<double> PF = mult(fce_pk1(Fc, e, lam, mu), Fc);
-            const double k = kst * vol;
+        for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
+        FCE e;
+        fce_prepare(Fc, e);
+        const M3<double> PF = mult(fce_pk1(Fc, e, lam, mu), Fc);
+        const double k = kst * vol;
 #pragma unroll
-            for (int i = 0; i < 9; ++i) A.a[i] = float(k * PF.a[i]) + m * C.a[i];
-        }
-        float mv[3];
+        for (int i = 0; i < 9; ++i) A.a[i] = float(k * PF.a[i]) + m * C.a[i];
 #pragma unroll
         for (int a = 0; a < 3; ++a) mv[a] = m * v[a] * sdrag;
+        // sum over the stencil of |w (mv + A d)| <= |mv| + 2 dx sum_b |A_ab|  (sum w = 1, |d| <= 2 dx)
+        bound[0] = fabsf(m);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            bound[1 + a] = fabsf(mv[a]) + 2.f * dx * (fabsf(A.a[3 * a]) + fabsf(A.a[3 * a + 1]) + fabsf(A.a[3 * a + 2]));
+    }
+    const Scales<4> SC = block_scales<4>(bound);
+    const bool use_lds = W0.lds && SC.ok;
+    if (use_lds)
+        for (int i = threadIdx.x; i < 4 * W0.vol; i += blockDim.x) lds64[i] = 0ull;
+    __syncthreads();
+    if (valid) {
         Axis<float> st[3];
         stencil_f(sp, x, st, false);
 #pragma unroll
@@ -224,10 +301,10 @@ __global__ __launch_bounds__(kBlk) void k_p2g_blk(StepParams sp, StepIO<float> i
                     val[0] = w * m;
 #pragma unroll
                     for (int a = 0; a < 3; ++a) val[1 + a] = w * (mv[a] + A.a[3 * a] * d0 + A.a[3 * a + 1] * d1 + A.a[3 * a + 2] * d2);
-                    if (W.lds) {
-                        const int li = win_local(W, gi, gj, gk);
+                    if (use_lds) {
+                        const int li = win_local(W0, gi, gj, gk);
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) atomicAdd(&lds[c * W.vol + li], val[c]);
+                        for (int c = 0; c < 4; ++c) atomicAdd(&lds64[c * W0.vol + li], to_fixed(val[c], SC.s[c]));
                     } else {
                         const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
 #pragma unroll
@@ -237,9 +314,9 @@ __global__ __launch_bounds__(kBlk) void k_p2g_blk(StepParams sp, StepIO<float> i
             }
         }
     }
-    if (W.lds) {
+    if (use_lds) {
         __syncthreads();
-        win_flush<4>(sp, W, lds, io.grid, sp.G);
+        win_flush_fixed<4>(sp, W0, lds64, SC, io.grid, sp.G);
     }
 }
 
@@ -328,20 +405,22 @@ __global__ __launch_bounds__(kBlk) void k_g2p_blk(StepParams sp, StepIO<float> i
 }
 
 // -------------------------------------------------------- transposed G2P ---
-__global__ __launch_bounds__(kBlk) void k_g2p_adj_blk(StepParams sp, AdjIO<float> io, const int2* ranges, int cap)
+__global__ __launch_bounds__(kScat) void k_g2p_adj_blk(StepParams sp, AdjIO<float> io, const int2* ranges, int cap)
 {
-    extern __shared__ float lds[];
+    extern __shared__ unsigned long long lds64[];
     const int2 rg = ranges[blockIdx.x];
     const int S = sp.stride;
-    const Win W = range_window(sp, io.cur, S, rg.x, rg.y, cap);
-    if (W.lds)
-        for (int i = threadIdx.x; i < 3 * W.vol; i += blockDim.x) lds[i] = 0.f;
-    __syncthreads();
+    const int p = rg.x + (int)threadIdx.x;
+    const bool valid = p < rg.y;
+    const Win W0 = range_window(sp, io.cur, S, rg.x, rg.y, cap);
     const float Dd = float(3.0 * sp.inv_dx);
     const float dt = float(sp.dt);
     const float cs = (sp.compat & 2) ? 1.f : 1.f - float(sp.smoothing);
-    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) {
-        float x[3], gx1[3], gv1[3];
+    float x[3] = {0.f, 0.f, 0.f}, vt[3] = {0.f, 0.f, 0.f};
+    M3<float> Ct = m3_zero<float>();
+    float bound[3] = {0.f, 0.f, 0.f};
+    if (valid) {
+        float gx1[3], gv1[3];
         ld3(io.cur, FX, S, p, x);
         ld3(io.g1, FX, S, p, gx1);
         ld3(io.g1, FV, S, p, gv1);
@@ -353,8 +432,6 @@ __global__ __launch_bounds__(kBlk) void k_g2p_adj_blk(StepParams sp, AdjIO<float
 #pragma unroll
         for (int i = 0; i < 9; ++i) Fc.a[i] = F.a[i] + dF.a[i];
         const M3<float> FF_ = mult(gF1, Fc);
-        M3<float> Ct;
-        float vt[3];
 #pragma unroll
         for (int i = 0; i < 9; ++i) Ct.a[i] = gC1.a[i] + cs * dt * FF_.a[i];
 #pragma unroll
@@ -362,6 +439,17 @@ __global__ __launch_bounds__(kBlk) void k_g2p_adj_blk(StepParams sp, AdjIO<float
 #pragma unroll
         for (int a = 0; a < 3; ++a) io.tmp[(long long)a * S + p] = vt[a];
         stm(io.tmp, 3, S, p, Ct);
+        // sum over the stencil of |w (vt + Ct e)| <= |vt| + (6/dx) sum_b |Ct_ab|  (sum w = 1, |e| = |D d| <= 6/dx)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            bound[a] = fabsf(vt[a]) + 2.f * Dd * (fabsf(Ct.a[3 * a]) + fabsf(Ct.a[3 * a + 1]) + fabsf(Ct.a[3 * a + 2]));
+    }
+    const Scales<3> SC = block_scales<3>(bound);
+    const bool use_lds = W0.lds && SC.ok;
+    if (use_lds)
+        for (int i = threadIdx.x; i < 3 * W0.vol; i += blockDim.x) lds64[i] = 0ull;
+    __syncthreads();
+    if (valid) {
         Axis<float> st[3];
         stencil_f(sp, x, st, false);
 #pragma unroll
@@ -384,10 +472,10 @@ __global__ __launch_bounds__(kBlk) void k_g2p_adj_blk(StepParams sp, AdjIO<float
                     float val[3];
 #pragma unroll
                     for (int a = 0; a < 3; ++a) val[a] = w * (vt[a] + Ct.a[3 * a] * e0 + Ct.a[3 * a + 1] * e1 + Ct.a[3 * a + 2] * e2);
-                    if (W.lds) {
-                        const int li = win_local(W, gi, gj, gk);
+                    if (use_lds) {
+                        const int li = win_local(W0, gi, gj, gk);
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) atomicAdd(&lds[c * W.vol + li], val[c]);
+                        for (int c = 0; c < 3; ++c) atomicAdd(&lds64[c * W0.vol + li], to_fixed(val[c], SC.s[c]));
                     } else {
                         const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
 #pragma unroll
@@ -397,9 +485,9 @@ __global__ __launch_bounds__(kBlk) void k_g2p_adj_blk(StepParams sp, AdjIO<float
             }
         }
     }
-    if (W.lds) {
+    if (use_lds) {
         __syncthreads();
-        win_flush<3>(sp, W, lds, io.agrid, sp.G);
+        win_flush_fixed<3>(sp, W0, lds64, SC, io.agrid, sp.G);
     }
 }
 
@@ -595,7 +683,7 @@ __global__ __launch_bounds__(kBlk) void k_elastic_adj(StepParams sp, AdjIO<float
 void launch_p2g_blk(const StepParams& sp, const StepIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s)
 {
     if (nranges <= 0) return;
-    hipLaunchKernelGGL(k_p2g_blk, dim3(nranges), dim3(kBlk), 4 * cap * sizeof(float), s, sp, io, ranges, cap);
+    hipLaunchKernelGGL(k_p2g_blk, dim3(nranges), dim3(kScat), 4 * cap * sizeof(unsigned long long), s, sp, io, ranges, cap);
 }
 
 void launch_g2p_blk(const StepParams& sp, const StepIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s)
@@ -607,7 +695,7 @@ void launch_g2p_blk(const StepParams& sp, const StepIO<float>& io, const int2* r
 void launch_g2p_adj_blk(const StepParams& sp, const AdjIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s)
 {
     if (nranges <= 0) return;
-    hipLaunchKernelGGL(k_g2p_adj_blk, dim3(nranges), dim3(kBlk), 3 * cap * sizeof(float), s, sp, io, ranges, cap);
+    hipLaunchKernelGGL(k_g2p_adj_blk, dim3(nranges), dim3(kScat), 3 * cap * sizeof(unsigned long long), s, sp, io, ranges, cap);
 }
 
 void launch_p2g_adj_blk(const StepParams& sp, const AdjIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s)

@@ -64,7 +64,7 @@ template <typename Real> void launch_grid_adj(const StepParams& sp, const AdjIO<
 template <typename Real> void launch_p2g_adj(const StepParams& sp, const AdjIO<Real>& io, hipStream_t s);
 
 // FP32 path: one workgroup per particle range of a 4^3-cell block (ranges[i] =
-// {begin, end} in device order), node window staged in LDS (cap = window
+// {begin, end} in device order, at most 512 particles), node window staged in LDS (cap = window
 // capacity in nodes; larger windows fall back to direct HBM access)
 void launch_p2g_blk(const StepParams& sp, const StepIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s);
 void launch_g2p_blk(const StepParams& sp, const StepIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s);

@@ -464,8 +464,9 @@ void sort_particles(dmpm_ctx* c, const double* x)
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
     c->pos.assign(N, 0);
     for (int d = 0; d < N; ++d) c->pos[order[d]] = d;
-    // one workgroup range per block (split above kMaxRange particles)
-    constexpr int kMaxRange = 1024;
+    // one workgroup range per block, at most 512 particles (the scatter
+    // kernels hold one particle per thread of a 512-thread workgroup)
+    constexpr int kMaxRange = 512;
     std::vector<int2> rg;
     int d0 = 0;
     for (int d = 1; d <= N; ++d) {
