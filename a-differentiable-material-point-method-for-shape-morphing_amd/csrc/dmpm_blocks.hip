@@ -2,16 +2,30 @@
 //
 // Particles are stored sorted by the 4^3-cell block of their cell; one
 // workgroup processes the particle range of one block (CSR ranges built at
-// sort time).  The workgroup first reduces the bounding box of its particles'
-// stencil bases -- the node window, normally the block's 7^3 nodes -- then:
+// sort time, at most 512 particles).  The workgroup first reduces the
+// bounding box of its particles' stencil bases, then:
 //   scatters (P2G, transposed G2P): accumulate every particle's 64-node
-//     contribution with LDS atomics, then add each non-zero window node to
+//     contribution in an LDS window, then add each non-zero window node to
 //     HBM with one atomic;
 //   gathers (G2P, transposed P2G): copy the window's grid channels from HBM
 //     into LDS once, then every particle reads its 64 nodes from LDS.
-// The window is recomputed each launch from the current positions, so a
-// particle that drifted out of its block only widens the window; a window
-// larger than the LDS capacity falls back to direct HBM access.
+// Fast path: when the bases span at most 5 cells per axis (particles within
+// one cell of their block) the window is a fixed 8^3 box at the smallest
+// base.  Nodes outside the grid are kept as zero-padded LDS slots (a gather
+// reads 0, a scatter's contribution is never flushed), so the 64-node loops
+// carry no bounds checks and every LDS offset is a compile-time constant
+// relative to the particle's base slot.  Otherwise a general window (the
+// clamped box, up to `cap` nodes, with per-node bounds checks) is used, and
+// past that, direct HBM access.
+//
+// Scatter accumulation is 64-bit fixed point: LDS float atomics are
+// serialised lane by lane on gfx950 (ds_add_f32 ~0.3 lane-ops/clk/CU against
+// ~9 for ds_add_u64).  Each channel c of a workgroup gets a power-of-two scale
+// s_c with s_c * max_p B_pc <= 2^30, where B_pc bounds every contribution of
+// particle p to channel c; each contribution is rounded to an int32 multiple
+// of 1/s_c and summed exactly in int64 (ds_add_u64).  Window sums are thus
+// independent of the order of the additions; the flush converts them back to
+// float for one HBM atomic per node and channel.
 //
 // Formulas are those of the per-particle kernels (ForwardSimulation.cpp:6-124,
 // BackPropagation.cpp:14-124 in the reference) with the same FP64 polar/SVD,
@@ -25,19 +39,22 @@ namespace dmpm {
 
 namespace {
 
-constexpr int kBlk = 256;      // gather kernels
-constexpr int kScat = 512;     // scatter kernels: one particle per thread of a block range
+constexpr int kGat = 256;    // gather kernels
+constexpr int kScat = 512;   // scatter kernels: one particle per thread of a block range
 constexpr int kMaxWaves = 8;
+constexpr int kFW = 8;       // fixed window edge (nodes)
+constexpr int kFW3 = kFW * kFW * kFW;
 
 struct Win {
-    int lo[3];
+    int lo[3];   // general window: clamped box
     int w[3];
     int vol;
     bool lds;
+    int flo[3];  // fixed window origin (smallest base, unclamped)
+    bool fixed;
 };
 
-template <typename Stn>
-__device__ __forceinline__ void stencil_f(const StepParams& sp, const float x[3], Stn& st, bool slopes)
+__device__ __forceinline__ void stencil_f(const StepParams& sp, const float x[3], Axis<float> st[3], bool slopes)
 {
 #pragma unroll
     for (int a = 0; a < 3; ++a) axis_weights<float>((x[a] - float(sp.min_point[a])) * float(sp.inv_dx), st[a], slopes);
@@ -48,7 +65,7 @@ __device__ __forceinline__ int base_of(const StepParams& sp, float x, int a)
     return int(floorf((x - float(sp.min_point[a])) * float(sp.inv_dx))) - 1;
 }
 
-// Bounding box of the stencil bases of particles [r0, r1), clamped to the grid.
+// Bounding box of the stencil bases of particles [r0, r1).
 __device__ Win range_window(const StepParams& sp, const float* lay, int S, int r0, int r1, int cap)
 {
     __shared__ int red[6][kMaxWaves];
@@ -79,6 +96,7 @@ __device__ Win range_window(const StepParams& sp, const float* lay, int S, int r
         Win W;
         long long vol = 1;
         bool empty = false;
+        W.fixed = true;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             int lo = INT_MAX, hi = INT_MIN;
@@ -86,6 +104,8 @@ __device__ Win range_window(const StepParams& sp, const float* lay, int S, int r
                 lo = min(lo, red[a][w]);
                 hi = max(hi, red[3 + a][w]);
             }
+            W.flo[a] = lo;
+            if ((long long)hi - (long long)lo > kFW - 4) W.fixed = false;
             long long l = lo < 0 ? 0 : lo;
             long long h = (long long)hi + 3;
             if (h > sp.dim[a] - 1) h = sp.dim[a] - 1;
@@ -116,14 +136,123 @@ __device__ __forceinline__ int win_local(const Win& W, int gi, int gj, int gk)
     return ((gi - W.lo[0]) * W.w[1] + (gj - W.lo[1])) * W.w[2] + (gk - W.lo[2]);
 }
 
+// fixed window: node of slot idx, or -1 outside the grid
+__device__ __forceinline__ long long fixed_node(const StepParams& sp, const Win& W, int idx)
+{
+    const int g0 = W.flo[0] + (idx >> 6), g1 = W.flo[1] + ((idx >> 3) & 7), g2 = W.flo[2] + (idx & 7);
+    if (g0 < 0 || g0 >= sp.dim[0] || g1 < 0 || g1 >= sp.dim[1] || g2 < 0 || g2 >= sp.dim[2]) return -1;
+    return ((long long)g0 * sp.dim[1] + g1) * sp.dim[2] + g2;
+}
+
+__device__ __forceinline__ int fixed_slot(const Win& W, const Axis<float> st[3])
+{
+    return ((st[0].base - W.flo[0]) * kFW + (st[1].base - W.flo[1])) * kFW + (st[2].base - W.flo[2]);
+}
+
 // load NC channels (channel c at src_c[node]) of the window into LDS
 template <int NC>
 __device__ __forceinline__ void win_load(const StepParams& sp, const Win& W, float* lds, const float* const src[NC])
 {
-    for (int idx = threadIdx.x; idx < W.vol; idx += blockDim.x) {
-        const long long node = win_node(sp, W, idx);
+    if (W.fixed) {
+        for (int idx = threadIdx.x; idx < kFW3; idx += blockDim.x) {
+            const long long node = fixed_node(sp, W, idx);
 #pragma unroll
-        for (int c = 0; c < NC; ++c) lds[c * W.vol + idx] = src[c][node];
+            for (int c = 0; c < NC; ++c) lds[c * kFW3 + idx] = node >= 0 ? src[c][node] : 0.f;
+        }
+    } else if (W.lds) {
+        for (int idx = threadIdx.x; idx < W.vol; idx += blockDim.x) {
+            const long long node = win_node(sp, W, idx);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) lds[c * W.vol + idx] = src[c][node];
+        }
+    }
+}
+
+template <int NC>
+struct Scales {
+    float s[NC];
+    float inv[NC];
+    bool ok;
+};
+
+// block-wide max of NC per-thread contribution bounds -> power-of-two scales
+// with s_c * max <= 2^30 (all threads)
+template <int NC>
+__device__ Scales<NC> block_scales(const float bound[NC])
+{
+    __shared__ float red[NC][kMaxWaves];
+    __shared__ Scales<NC> sc;
+    float v[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = bound[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const float o = __shfl_xor(v[c], off, 64);
+            v[c] = (v[c] == v[c] && !(o > v[c])) ? v[c] : o;  // max that keeps NaN
+        }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) red[c][wid] = v[c];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Scales<NC> S;
+        S.ok = true;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            float t = 0.f;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+                const float r = red[c][w];
+                if (!(r < 3.0e38f)) S.ok = false;  // inf or NaN somewhere in the block
+                else t = fmaxf(t, r);
+            }
+            int e = 30;
+            if (t > 0.f) {
+                int ex;
+                frexpf(t, &ex);  // t < 2^ex
+                e = 30 - ex;
+            }
+            e = e < -120 ? -120 : (e > 120 ? 120 : e);
+            S.s[c] = ldexpf(1.f, e);
+            S.inv[c] = ldexpf(1.f, -e);
+        }
+        sc = S;
+    }
+    __syncthreads();
+    return sc;
+}
+
+// |y| < 2^30: rounded to an int32, sign-extended to the int64 accumulator
+__device__ __forceinline__ unsigned long long fix32(float y)
+{
+    return (unsigned long long)(long long)(int)rintf(y);
+}
+
+template <int NC>
+__device__ __forceinline__ void flush_fixed(const StepParams& sp, const Win& W, const unsigned long long* lds,
+                                            const Scales<NC>& S, float* base, long long cstride)
+{
+    if (W.fixed) {
+        for (int idx = threadIdx.x; idx < kFW3; idx += blockDim.x) {
+            const long long node = fixed_node(sp, W, idx);
+            if (node < 0) continue;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const long long q = (long long)lds[c * kFW3 + idx];
+                if (q != 0) unsafeAtomicAdd(base + c * cstride + node, (float)((double)q * (double)S.inv[c]));
+            }
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < W.vol; idx += blockDim.x) {
+            const long long node = win_node(sp, W, idx);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const long long q = (long long)lds[c * W.vol + idx];
+                if (q != 0) unsafeAtomicAdd(base + c * cstride + node, (float)((double)q * (double)S.inv[c]));
+            }
+        }
     }
 }
 
@@ -150,85 +279,93 @@ __device__ __forceinline__ void stm(T* base, int field, int S, int p, const M3<T
     for (int i = 0; i < 9; ++i) base[(long long)(field + i) * S + p] = M.a[i];
 }
 
-}  // namespace
-
-// Scatter accumulation.  LDS float atomics are serialised lane by lane on
-// gfx950 (ds_add_f32 ~0.3 lane-ops/clk/CU against ~9 for ds_add_u64), so the
-// window is accumulated in 64-bit fixed point: each channel c of a workgroup
-// gets a power-of-two scale s_c with sum_p B_pc * s_c <= 2^61, where B_pc
-// bounds particle p's total contribution to channel c (its weights sum to 1),
-// every contribution is rounded to an integer multiple of 1/s_c and added
-// exactly with ds_add_u64.  The in-window sums are therefore exact and
-// independent of the order of the additions; the float rounding of each term
-// (|rel| <= 2^-24) is the only error.  A workgroup whose bound is not finite
-// or whose window overflows the LDS capacity falls back to HBM float atomics.
-template <int NC>
-struct Scales {
-    float s[NC];
-    float inv[NC];
-    bool ok;
-};
-
-// block-wide sum of NC per-thread bounds -> power-of-two scales (all threads)
-template <int NC>
-__device__ Scales<NC> block_scales(const float bound[NC])
+// Scatter of NC channels over a particle's 64 nodes.  val(i, j, k, out)
+// fills the NC contributions (already in scaled units for the LDS modes).
+// MODE 0: fixed window, 1: general LDS window, 2: HBM float atomics.
+template <int NC, int MODE, typename ValFn>
+__device__ __forceinline__ void scatter64(const StepParams& sp, const Win& W, const Axis<float> st[3],
+                                          unsigned long long* lds, float* hbm, ValFn&& val)
 {
-    __shared__ float red[NC][kMaxWaves];
-    __shared__ Scales<NC> sc;
-    float v[NC];
+    const int slot = MODE == 0 ? fixed_slot(W, st) : 0;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) v[c] = bound[c];
+    for (int i = 0; i < 4; ++i) {
+        const int gi = st[0].base + i;
+        if (MODE != 0 && (gi < 0 || gi >= sp.dim[0])) continue;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
+        for (int j = 0; j < 4; ++j) {
+            const int gj = st[1].base + j;
+            if (MODE != 0 && (gj < 0 || gj >= sp.dim[1])) continue;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) v[c] += __shfl_xor(v[c], off, 64);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0)
+            for (int kk = 0; kk < 4; ++kk) {
+                const int gk = st[2].base + kk;
+                if (MODE != 0 && (gk < 0 || gk >= sp.dim[2])) continue;
+                float v[NC];
+                val(i, j, kk, v);
+                if (MODE == 0) {
+                    const int o = slot + i * kFW * kFW + j * kFW + kk;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) red[c][wid] = v[c];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        Scales<NC> S;
-        S.ok = true;
+                    for (int c = 0; c < NC; ++c) atomicAdd(&lds[c * kFW3 + o], fix32(v[c]));
+                } else if (MODE == 1) {
+                    const int o = win_local(W, gi, gj, gk);
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            float t = 0.f;
-            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[c][w];
-            if (!(t < 3.0e38f)) S.ok = false;  // inf or NaN
-            int e = 61;
-            if (t > 0.f) {
-                int ex;
-                frexpf(t * 2.f, &ex);  // 2t <= 2^ex
-                e = 61 - ex;
+                    for (int c = 0; c < NC; ++c) atomicAdd(&lds[c * W.vol + o], fix32(v[c]));
+                } else {
+                    const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) unsafeAtomicAdd(hbm + c * sp.G + node, v[c]);
+                }
             }
-            e = e < -120 ? -120 : (e > 120 ? 120 : e);
-            S.s[c] = ldexpf(1.f, e);
-            S.inv[c] = ldexpf(1.f, -e);
         }
-        sc = S;
     }
-    __syncthreads();
-    return sc;
 }
 
-__device__ __forceinline__ unsigned long long to_fixed(float v, float scale)
+// Gather over a particle's 64 nodes: fn(i, j, k, ch) with ch[c] the node's
+// channel c.  MODE 0: fixed window, 1: general LDS window, 2: HBM.
+template <int NC, int MODE, typename Fn>
+__device__ __forceinline__ void gather64(const StepParams& sp, const Win& W, const Axis<float> st[3], const float* lds,
+                                         const float* const hbm[NC], Fn&& fn)
 {
-    return (unsigned long long)__float2ll_rn(v * scale);
-}
-
-template <int NC>
-__device__ __forceinline__ void win_flush_fixed(const StepParams& sp, const Win& W, const unsigned long long* lds,
-                                                const Scales<NC>& S, float* base, long long cstride)
-{
-    for (int idx = threadIdx.x; idx < W.vol; idx += blockDim.x) {
-        const long long node = win_node(sp, W, idx);
+    const int slot = MODE == 0 ? fixed_slot(W, st) : 0;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const long long q = (long long)lds[c * W.vol + idx];
-            if (q != 0) unsafeAtomicAdd(base + c * cstride + node, (float)((double)q * (double)S.inv[c]));
+    for (int i = 0; i < 4; ++i) {
+        const int gi = st[0].base + i;
+        if (MODE != 0 && (gi < 0 || gi >= sp.dim[0])) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gj = st[1].base + j;
+            if (MODE != 0 && (gj < 0 || gj >= sp.dim[1])) continue;
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const int gk = st[2].base + kk;
+                if (MODE != 0 && (gk < 0 || gk >= sp.dim[2])) continue;
+                float ch[NC];
+                if (MODE == 0) {
+                    const int o = slot + i * kFW * kFW + j * kFW + kk;
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) ch[c] = lds[c * kFW3 + o];
+                } else if (MODE == 1) {
+                    const int o = win_local(W, gi, gj, gk);
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) ch[c] = lds[c * W.vol + o];
+                } else {
+                    const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) ch[c] = hbm[c][node];
+                }
+                fn(i, j, kk, ch);
+            }
         }
     }
 }
+
+// node offsets (o - 1 - f) * h of one axis
+__device__ __forceinline__ void offsets4(const Axis<float>& ax, float h, float out[4])
+{
+#pragma unroll
+    for (int o = 0; o < 4; ++o) out[o] = (float(o - 1) - ax.f) * h;
+}
+
+}  // namespace
 
 // ------------------------------------------------------------------ P2G ----
 __global__ __launch_bounds__(kScat) void k_p2g_blk(StepParams sp, StepIO<float> io, const int2* ranges, int cap)
@@ -238,7 +375,7 @@ __global__ __launch_bounds__(kScat) void k_p2g_blk(StepParams sp, StepIO<float> 
     const int S = sp.stride;
     const int p = rg.x + (int)threadIdx.x;
     const bool valid = p < rg.y;
-    const Win W0 = range_window(sp, io.cur, S, rg.x, rg.y, cap);
+    const Win W = range_window(sp, io.cur, S, rg.x, rg.y, cap);
     const float dx = float(sp.dx);
     const float sdrag = 1.f - float(sp.dt) * float(sp.drag);
     const double kst = -3.0 * sp.inv_dx * sp.inv_dx * sp.dt;
@@ -266,72 +403,61 @@ __global__ __launch_bounds__(kScat) void k_p2g_blk(StepParams sp, StepIO<float> 
         for (int i = 0; i < 9; ++i) A.a[i] = float(k * PF.a[i]) + m * C.a[i];
 #pragma unroll
         for (int a = 0; a < 3; ++a) mv[a] = m * v[a] * sdrag;
-        // sum over the stencil of |w (mv + A d)| <= |mv| + 2 dx sum_b |A_ab|  (sum w = 1, |d| <= 2 dx)
+        // every contribution: |w (mv + A d)| <= |mv| + 2 dx sum_b |A_ab|  (w <= 1, |d| <= 2 dx)
         bound[0] = fabsf(m);
 #pragma unroll
         for (int a = 0; a < 3; ++a)
             bound[1 + a] = fabsf(mv[a]) + 2.f * dx * (fabsf(A.a[3 * a]) + fabsf(A.a[3 * a + 1]) + fabsf(A.a[3 * a + 2]));
     }
     const Scales<4> SC = block_scales<4>(bound);
-    const bool use_lds = W0.lds && SC.ok;
-    if (use_lds)
-        for (int i = threadIdx.x; i < 4 * W0.vol; i += blockDim.x) lds64[i] = 0ull;
+    const int mode = (W.fixed && SC.ok) ? 0 : ((W.lds && SC.ok) ? 1 : 2);
+    if (mode == 0)
+        for (int i = threadIdx.x; i < 4 * kFW3; i += blockDim.x) lds64[i] = 0ull;
+    else if (mode == 1)
+        for (int i = threadIdx.x; i < 4 * W.vol; i += blockDim.x) lds64[i] = 0ull;
     __syncthreads();
     if (valid) {
         Axis<float> st[3];
         stencil_f(sp, x, st, false);
+        if (mode != 2) {  // scaled units
+            m *= SC.s[0];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int gi = st[0].base + i;
-            if (gi < 0 || gi >= sp.dim[0]) continue;
-            const float d0 = (float(i - 1) - st[0].f) * dx;
+            for (int a = 0; a < 3; ++a) {
+                mv[a] *= SC.s[1 + a];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int gj = st[1].base + j;
-                if (gj < 0 || gj >= sp.dim[1]) continue;
-                const float d1 = (float(j - 1) - st[1].f) * dx;
-                const float wij = st[0].w[i] * st[1].w[j];
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const int gk = st[2].base + kk;
-                    if (gk < 0 || gk >= sp.dim[2]) continue;
-                    const float d2 = (float(kk - 1) - st[2].f) * dx;
-                    const float w = wij * st[2].w[kk];
-                    float val[4];
-                    val[0] = w * m;
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) val[1 + a] = w * (mv[a] + A.a[3 * a] * d0 + A.a[3 * a + 1] * d1 + A.a[3 * a + 2] * d2);
-                    if (use_lds) {
-                        const int li = win_local(W0, gi, gj, gk);
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) atomicAdd(&lds64[c * W0.vol + li], to_fixed(val[c], SC.s[c]));
-                    } else {
-                        const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) unsafeAtomicAdd(io.grid + c * sp.G + node, val[c]);
-                    }
-                }
+                for (int b = 0; b < 3; ++b) A.a[3 * a + b] *= SC.s[1 + a];
             }
         }
+        float d0s[4], d1s[4], d2s[4];
+        offsets4(st[0], dx, d0s);
+        offsets4(st[1], dx, d1s);
+        offsets4(st[2], dx, d2s);
+        auto val = [&](int i, int j, int kk, float* out) {
+            const float w = st[0].w[i] * st[1].w[j] * st[2].w[kk];
+            out[0] = w * m;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) out[1 + a] = w * (mv[a] + A.a[3 * a] * d0s[i] + A.a[3 * a + 1] * d1s[j] + A.a[3 * a + 2] * d2s[kk]);
+        };
+        if (mode == 0) scatter64<4, 0>(sp, W, st, lds64, io.grid, val);
+        else if (mode == 1) scatter64<4, 1>(sp, W, st, lds64, io.grid, val);
+        else scatter64<4, 2>(sp, W, st, lds64, io.grid, val);
     }
-    if (use_lds) {
+    if (mode != 2) {
         __syncthreads();
-        win_flush_fixed<4>(sp, W0, lds64, SC, io.grid, sp.G);
+        flush_fixed<4>(sp, W, lds64, SC, io.grid, sp.G);
     }
 }
 
 // ------------------------------------------------------------------ G2P ----
-__global__ __launch_bounds__(kBlk) void k_g2p_blk(StepParams sp, StepIO<float> io, const int2* ranges, int cap)
+__global__ __launch_bounds__(kGat) void k_g2p_blk(StepParams sp, StepIO<float> io, const int2* ranges, int cap)
 {
     extern __shared__ float lds[];
     const int2 rg = ranges[blockIdx.x];
     const int S = sp.stride;
     const Win W = range_window(sp, io.cur, S, rg.x, rg.y, cap);
     const float* gv = io.grid + sp.G;
-    if (W.lds) {
-        const float* src[3] = {gv, gv + sp.G, gv + 2 * sp.G};
-        win_load<3>(sp, W, lds, src);
-    }
+    const float* const src[3] = {gv, gv + sp.G, gv + 2 * sp.G};
+    win_load<3>(sp, W, lds, src);
     __syncthreads();
     const float Dd = float(3.0 * sp.inv_dx);
     const float dt = float(sp.dt), s = float(sp.smoothing);
@@ -340,46 +466,26 @@ __global__ __launch_bounds__(kBlk) void k_g2p_blk(StepParams sp, StepIO<float> i
         ld3(io.cur, FX, S, p, x);
         Axis<float> st[3];
         stencil_f(sp, x, st, false);
+        float e0s[4], e1s[4], e2s[4];
+        offsets4(st[0], Dd, e0s);
+        offsets4(st[1], Dd, e1s);
+        offsets4(st[2], Dd, e2s);
         float vp[3] = {0.f, 0.f, 0.f};
         M3<float> Cp = m3_zero<float>();
+        auto acc = [&](int i, int j, int kk, const float* vi) {
+            const float w = st[0].w[i] * st[1].w[j] * st[2].w[kk];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int gi = st[0].base + i;
-            if (gi < 0 || gi >= sp.dim[0]) continue;
-            const float e0 = (float(i - 1) - st[0].f) * Dd;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int gj = st[1].base + j;
-                if (gj < 0 || gj >= sp.dim[1]) continue;
-                const float e1 = (float(j - 1) - st[1].f) * Dd;
-                const float wij = st[0].w[i] * st[1].w[j];
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const int gk = st[2].base + kk;
-                    if (gk < 0 || gk >= sp.dim[2]) continue;
-                    const float e2 = (float(kk - 1) - st[2].f) * Dd;
-                    const float w = wij * st[2].w[kk];
-                    float vi[3];
-                    if (W.lds) {
-                        const int li = win_local(W, gi, gj, gk);
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) vi[a] = lds[a * W.vol + li];
-                    } else {
-                        const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) vi[a] = gv[a * sp.G + node];
-                    }
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        const float wv = w * vi[a];
-                        vp[a] += wv;
-                        Cp.a[3 * a] += wv * e0;
-                        Cp.a[3 * a + 1] += wv * e1;
-                        Cp.a[3 * a + 2] += wv * e2;
-                    }
-                }
+            for (int a = 0; a < 3; ++a) {
+                const float wv = w * vi[a];
+                vp[a] += wv;
+                Cp.a[3 * a] += wv * e0s[i];
+                Cp.a[3 * a + 1] += wv * e1s[j];
+                Cp.a[3 * a + 2] += wv * e2s[kk];
             }
-        }
+        };
+        if (W.fixed) gather64<3, 0>(sp, W, st, lds, src, acc);
+        else if (W.lds) gather64<3, 1>(sp, W, st, lds, src, acc);
+        else gather64<3, 2>(sp, W, st, lds, src, acc);
         const M3<float> F = ldm(io.cur, FF, S, p);
         const M3<float> dF = ldm(io.ctrl, 0, S, p);
         M3<float> Fc, IC;
@@ -405,6 +511,8 @@ __global__ __launch_bounds__(kBlk) void k_g2p_blk(StepParams sp, StepIO<float> i
 }
 
 // -------------------------------------------------------- transposed G2P ---
+// P_op_2^T + G2P^T: vt = v' + dt x', Ct = C' + (1-s) dt F' Fc^T (kept in tmp),
+// vbar_i += w (vt + D Ct d).
 __global__ __launch_bounds__(kScat) void k_g2p_adj_blk(StepParams sp, AdjIO<float> io, const int2* ranges, int cap)
 {
     extern __shared__ unsigned long long lds64[];
@@ -412,7 +520,7 @@ __global__ __launch_bounds__(kScat) void k_g2p_adj_blk(StepParams sp, AdjIO<floa
     const int S = sp.stride;
     const int p = rg.x + (int)threadIdx.x;
     const bool valid = p < rg.y;
-    const Win W0 = range_window(sp, io.cur, S, rg.x, rg.y, cap);
+    const Win W = range_window(sp, io.cur, S, rg.x, rg.y, cap);
     const float Dd = float(3.0 * sp.inv_dx);
     const float dt = float(sp.dt);
     const float cs = (sp.compat & 2) ? 1.f : 1.f - float(sp.smoothing);
@@ -439,55 +547,45 @@ __global__ __launch_bounds__(kScat) void k_g2p_adj_blk(StepParams sp, AdjIO<floa
 #pragma unroll
         for (int a = 0; a < 3; ++a) io.tmp[(long long)a * S + p] = vt[a];
         stm(io.tmp, 3, S, p, Ct);
-        // sum over the stencil of |w (vt + Ct e)| <= |vt| + (6/dx) sum_b |Ct_ab|  (sum w = 1, |e| = |D d| <= 6/dx)
+        // every contribution: |w (vt + Ct e)| <= |vt| + (6/dx) sum_b |Ct_ab|  (w <= 1, |e| = |D d| <= 6/dx)
 #pragma unroll
         for (int a = 0; a < 3; ++a)
             bound[a] = fabsf(vt[a]) + 2.f * Dd * (fabsf(Ct.a[3 * a]) + fabsf(Ct.a[3 * a + 1]) + fabsf(Ct.a[3 * a + 2]));
     }
     const Scales<3> SC = block_scales<3>(bound);
-    const bool use_lds = W0.lds && SC.ok;
-    if (use_lds)
-        for (int i = threadIdx.x; i < 3 * W0.vol; i += blockDim.x) lds64[i] = 0ull;
+    const int mode = (W.fixed && SC.ok) ? 0 : ((W.lds && SC.ok) ? 1 : 2);
+    if (mode == 0)
+        for (int i = threadIdx.x; i < 3 * kFW3; i += blockDim.x) lds64[i] = 0ull;
+    else if (mode == 1)
+        for (int i = threadIdx.x; i < 3 * W.vol; i += blockDim.x) lds64[i] = 0ull;
     __syncthreads();
     if (valid) {
         Axis<float> st[3];
         stencil_f(sp, x, st, false);
+        if (mode != 2) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int gi = st[0].base + i;
-            if (gi < 0 || gi >= sp.dim[0]) continue;
-            const float e0 = (float(i - 1) - st[0].f) * Dd;
+            for (int a = 0; a < 3; ++a) {
+                vt[a] *= SC.s[a];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int gj = st[1].base + j;
-                if (gj < 0 || gj >= sp.dim[1]) continue;
-                const float e1 = (float(j - 1) - st[1].f) * Dd;
-                const float wij = st[0].w[i] * st[1].w[j];
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const int gk = st[2].base + kk;
-                    if (gk < 0 || gk >= sp.dim[2]) continue;
-                    const float e2 = (float(kk - 1) - st[2].f) * Dd;
-                    const float w = wij * st[2].w[kk];
-                    float val[3];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) val[a] = w * (vt[a] + Ct.a[3 * a] * e0 + Ct.a[3 * a + 1] * e1 + Ct.a[3 * a + 2] * e2);
-                    if (use_lds) {
-                        const int li = win_local(W0, gi, gj, gk);
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) atomicAdd(&lds64[c * W0.vol + li], to_fixed(val[c], SC.s[c]));
-                    } else {
-                        const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) unsafeAtomicAdd(io.agrid + c * sp.G + node, val[c]);
-                    }
-                }
+                for (int b = 0; b < 3; ++b) Ct.a[3 * a + b] *= SC.s[a];
             }
         }
+        float e0s[4], e1s[4], e2s[4];
+        offsets4(st[0], Dd, e0s);
+        offsets4(st[1], Dd, e1s);
+        offsets4(st[2], Dd, e2s);
+        auto val = [&](int i, int j, int kk, float* out) {
+            const float w = st[0].w[i] * st[1].w[j] * st[2].w[kk];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) out[a] = w * (vt[a] + Ct.a[3 * a] * e0s[i] + Ct.a[3 * a + 1] * e1s[j] + Ct.a[3 * a + 2] * e2s[kk]);
+        };
+        if (mode == 0) scatter64<3, 0>(sp, W, st, lds64, io.agrid, val);
+        else if (mode == 1) scatter64<3, 1>(sp, W, st, lds64, io.agrid, val);
+        else scatter64<3, 2>(sp, W, st, lds64, io.agrid, val);
     }
-    if (use_lds) {
+    if (mode != 2) {
         __syncthreads();
-        win_flush_fixed<3>(sp, W0, lds64, SC, io.agrid, sp.G);
+        flush_fixed<3>(sp, W, lds64, SC, io.agrid, sp.G);
     }
 }
 
@@ -497,7 +595,7 @@ __global__ __launch_bounds__(kScat) void k_g2p_adj_blk(StepParams sp, AdjIO<floa
 //   Cbar = m Gb,  vbar = m (1 - dt drag) sum w pbar,
 //   xbar = x' + sum [gw (m mbar + (b + G d).pbar + v_i.vt + D (Ct : v_i d^T))
 //                    - w (G^T pbar + D Ct^T v_i)].
-__global__ __launch_bounds__(kBlk) void k_p2g_adj_blk(StepParams sp, AdjIO<float> io, const int2* ranges, int cap)
+__global__ __launch_bounds__(kGat) void k_p2g_adj_blk(StepParams sp, AdjIO<float> io, const int2* ranges, int cap)
 {
     extern __shared__ float lds[];
     const int2 rg = ranges[blockIdx.x];
@@ -507,10 +605,8 @@ __global__ __launch_bounds__(kBlk) void k_p2g_adj_blk(StepParams sp, AdjIO<float
     const float* gv = io.grid + sp.G;
     const float* pb = io.agrid;
     const float* mb = io.agrid + 3 * sp.G;
-    if (W.lds) {
-        const float* src[7] = {pb, pb + sp.G, pb + 2 * sp.G, mb, gv, gv + sp.G, gv + 2 * sp.G};
-        win_load<7>(sp, W, lds, src);
-    }
+    const float* const src[7] = {pb, pb + sp.G, pb + 2 * sp.G, mb, gv, gv + sp.G, gv + 2 * sp.G};
+    win_load<7>(sp, W, lds, src);
     __syncthreads();
     const float dx = float(sp.dx);
     const float Dinv = float(3.0 * sp.inv_dx * sp.inv_dx);
@@ -548,67 +644,44 @@ __global__ __launch_bounds__(kBlk) void k_p2g_adj_blk(StepParams sp, AdjIO<float
         for (int a = 0; a < 3; ++a) b[a] = m * bsc * v[a];
         Axis<float> st[3];
         stencil_f(sp, x, st, true);
+        float d0s[4], d1s[4], d2s[4];
+        offsets4(st[0], dx, d0s);
+        offsets4(st[1], dx, d1s);
+        offsets4(st[2], dx, d2s);
         M3<float> Gb = m3_zero<float>();
         float xb[3] = {0.f, 0.f, 0.f}, pacc[3] = {0.f, 0.f, 0.f};
+        auto acc = [&](int i, int j, int kk, const float* ch) {
+            const float d0 = d0s[i], d1 = d1s[j], d2 = d2s[kk];
+            const float w0 = st[0].w[i], w1 = st[1].w[j], w2 = st[2].w[kk];
+            const float w = w0 * w1 * w2;
+            const float gw[3] = {mgw * st[0].dw[i] * w1 * w2, mgw * w0 * st[1].dw[j] * w2, mgw * w0 * w1 * st[2].dw[kk]};
+            const float pbi[3] = {ch[0], ch[1], ch[2]};
+            const float mbi = ch[3];
+            const float vi[3] = {ch[4], ch[5], ch[6]};
+            float bgd = 0.f, vdot = 0.f, cvd = 0.f;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int gi = st[0].base + i;
-            if (gi < 0 || gi >= sp.dim[0]) continue;
-            const float d0 = (float(i - 1) - st[0].f) * dx;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int gj = st[1].base + j;
-                if (gj < 0 || gj >= sp.dim[1]) continue;
-                const float d1 = (float(j - 1) - st[1].f) * dx;
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const int gk = st[2].base + kk;
-                    if (gk < 0 || gk >= sp.dim[2]) continue;
-                    const float d2 = (float(kk - 1) - st[2].f) * dx;
-                    const float w0 = st[0].w[i], w1 = st[1].w[j], w2 = st[2].w[kk];
-                    const float w = w0 * w1 * w2;
-                    const float gw[3] = {mgw * st[0].dw[i] * w1 * w2, mgw * w0 * st[1].dw[j] * w2, mgw * w0 * w1 * st[2].dw[kk]};
-                    float pbi[3], vi[3], mbi;
-                    if (W.lds) {
-                        const int li = win_local(W, gi, gj, gk);
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            pbi[a] = lds[a * W.vol + li];
-                            vi[a] = lds[(4 + a) * W.vol + li];
-                        }
-                        mbi = lds[3 * W.vol + li];
-                    } else {
-                        const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            pbi[a] = pb[a * sp.G + node];
-                            vi[a] = gv[a * sp.G + node];
-                        }
-                        mbi = mb[node];
-                    }
-                    float bgd = 0.f, vdot = 0.f, cvd = 0.f;
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        const float gd = Gm.a[3 * a] * d0 + Gm.a[3 * a + 1] * d1 + Gm.a[3 * a + 2] * d2;
-                        bgd += (b[a] + gd) * pbi[a];
-                        vdot += vi[a] * vt[a];
-                        cvd += vi[a] * (Ct.a[3 * a] * d0 + Ct.a[3 * a + 1] * d1 + Ct.a[3 * a + 2] * d2);
-                        const float wp = w * pbi[a];
-                        pacc[a] += wp;
-                        Gb.a[3 * a] += wp * d0;
-                        Gb.a[3 * a + 1] += wp * d1;
-                        Gb.a[3 * a + 2] += wp * d2;
-                    }
-                    const float sgw = m * mbi + bgd + vdot + Dinv * cvd;
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        const float GTp = Gm.a[a] * pbi[0] + Gm.a[3 + a] * pbi[1] + Gm.a[6 + a] * pbi[2];
-                        const float CTv = Ct.a[a] * vi[0] + Ct.a[3 + a] * vi[1] + Ct.a[6 + a] * vi[2];
-                        xb[a] += gw[a] * sgw - w * (GTp + Dinv * CTv);
-                    }
-                }
+            for (int a = 0; a < 3; ++a) {
+                const float gd = Gm.a[3 * a] * d0 + Gm.a[3 * a + 1] * d1 + Gm.a[3 * a + 2] * d2;
+                bgd += (b[a] + gd) * pbi[a];
+                vdot += vi[a] * vt[a];
+                cvd += vi[a] * (Ct.a[3 * a] * d0 + Ct.a[3 * a + 1] * d1 + Ct.a[3 * a + 2] * d2);
+                const float wp = w * pbi[a];
+                pacc[a] += wp;
+                Gb.a[3 * a] += wp * d0;
+                Gb.a[3 * a + 1] += wp * d1;
+                Gb.a[3 * a + 2] += wp * d2;
             }
-        }
+            const float sgw = m * mbi + bgd + vdot + Dinv * cvd;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float GTp = Gm.a[a] * pbi[0] + Gm.a[3 + a] * pbi[1] + Gm.a[6 + a] * pbi[2];
+                const float CTv = Ct.a[a] * vi[0] + Ct.a[3 + a] * vi[1] + Ct.a[6 + a] * vi[2];
+                xb[a] += gw[a] * sgw - w * (GTp + Dinv * CTv);
+            }
+        };
+        if (W.fixed) gather64<7, 0>(sp, W, st, lds, src, acc);
+        else if (W.lds) gather64<7, 1>(sp, W, st, lds, src, acc);
+        else gather64<7, 2>(sp, W, st, lds, src, acc);
         float gx1[3];
         ld3(io.g1, FX, S, p, gx1);
         const float vsc = m * sdrag;
@@ -627,7 +700,7 @@ __global__ __launch_bounds__(kBlk) void k_p2g_adj_blk(StepParams sp, AdjIO<float
 
 // Elastic half of P2G^T + P_op_1^T, one thread per particle:
 //   Pbar = -k Gb Fc,  Fcbar = (1-s)(I + dt C')^T F' - k Gb^T P + H(Fc):Pbar.
-__global__ __launch_bounds__(kBlk) void k_elastic_adj(StepParams sp, AdjIO<float> io)
+__global__ __launch_bounds__(kGat) void k_elastic_adj(StepParams sp, AdjIO<float> io)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= sp.n) return;
@@ -680,34 +753,39 @@ __global__ __launch_bounds__(kBlk) void k_elastic_adj(StepParams sp, AdjIO<float
 }
 
 // ------------------------------------------------------------ launchers ----
+// LDS: NC channels of max(cap, 8^3) nodes (int64 for scatters, float for gathers)
 void launch_p2g_blk(const StepParams& sp, const StepIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s)
 {
     if (nranges <= 0) return;
-    hipLaunchKernelGGL(k_p2g_blk, dim3(nranges), dim3(kScat), 4 * cap * sizeof(unsigned long long), s, sp, io, ranges, cap);
+    const size_t lds = 4 * (size_t)(cap > kFW3 ? cap : kFW3) * sizeof(unsigned long long);
+    hipLaunchKernelGGL(k_p2g_blk, dim3(nranges), dim3(kScat), lds, s, sp, io, ranges, cap);
 }
 
 void launch_g2p_blk(const StepParams& sp, const StepIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s)
 {
     if (nranges <= 0) return;
-    hipLaunchKernelGGL(k_g2p_blk, dim3(nranges), dim3(kBlk), 3 * cap * sizeof(float), s, sp, io, ranges, cap);
+    const size_t lds = 3 * (size_t)(cap > kFW3 ? cap : kFW3) * sizeof(float);
+    hipLaunchKernelGGL(k_g2p_blk, dim3(nranges), dim3(kGat), lds, s, sp, io, ranges, cap);
 }
 
 void launch_g2p_adj_blk(const StepParams& sp, const AdjIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s)
 {
     if (nranges <= 0) return;
-    hipLaunchKernelGGL(k_g2p_adj_blk, dim3(nranges), dim3(kScat), 3 * cap * sizeof(unsigned long long), s, sp, io, ranges, cap);
+    const size_t lds = 3 * (size_t)(cap > kFW3 ? cap : kFW3) * sizeof(unsigned long long);
+    hipLaunchKernelGGL(k_g2p_adj_blk, dim3(nranges), dim3(kScat), lds, s, sp, io, ranges, cap);
 }
 
 void launch_p2g_adj_blk(const StepParams& sp, const AdjIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s)
 {
     if (nranges <= 0) return;
-    hipLaunchKernelGGL(k_p2g_adj_blk, dim3(nranges), dim3(kBlk), 7 * cap * sizeof(float), s, sp, io, ranges, cap);
+    const size_t lds = 7 * (size_t)(cap > kFW3 ? cap : kFW3) * sizeof(float);
+    hipLaunchKernelGGL(k_p2g_adj_blk, dim3(nranges), dim3(kGat), lds, s, sp, io, ranges, cap);
 }
 
 void launch_elastic_adj(const StepParams& sp, const AdjIO<float>& io, hipStream_t s)
 {
     if (sp.n <= 0) return;
-    hipLaunchKernelGGL(k_elastic_adj, dim3((sp.n + kBlk - 1) / kBlk), dim3(kBlk), 0, s, sp, io);
+    hipLaunchKernelGGL(k_elastic_adj, dim3((sp.n + kGat - 1) / kGat), dim3(kGat), 0, s, sp, io);
 }
 
 }  // namespace dmpm

@@ -125,6 +125,25 @@ def cpu_baseline(sc, vol, dFc, threads, steps):
     return sc.n * steps / dt, dt
 
 
+def copy_probe(nbytes):
+    """One device-to-device copy of ``nbytes`` through the HIP runtime's copy kernel:
+    the known-size reference for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a = ctypes.c_void_p()
+    b = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": nbytes}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,7 +156,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--copy-probe", type=int, default=0, help="only run a D2D copy of this many bytes")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a design note only)"}))
