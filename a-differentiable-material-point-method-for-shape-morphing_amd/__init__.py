@@ -45,7 +45,8 @@ EXPORTS = [
     "dmpm_get_control_grad",
     "dmpm_control_grad_norm", "dmpm_get_grid", "dmpm_chain_episode", "dmpm_synchronize", "dmpm_memory_bytes",
     "dmpm_profile_enable", "dmpm_profile_reset", "dmpm_profile_read", "dmpm_eval_pk1", "dmpm_eval_hessian",
-    "dmpm_eval_svd",
+    "dmpm_eval_svd", "dmpm_create_batch", "dmpm_num_instances", "dmpm_get_instance_losses",
+    "dmpm_target_from_points_batch",
 ]
 
 
@@ -67,6 +68,16 @@ class Config(ctypes.Structure):
         ("keep_control_grads", ctypes.c_int32),
         ("device", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+    ]
+
+
+class Instance(ctypes.Structure):
+    """``dmpm_instance``: one morph of a batched context."""
+    _fields_ = [
+        ("n_particles", ctypes.c_int32),
+        ("dim", ctypes.c_int32 * 3),
+        ("dx", ctypes.c_double),
+        ("min_point", ctypes.c_double * 3),
     ]
 
 
@@ -123,6 +134,10 @@ def load_library(path: str = LIB_PATH):
         "dmpm_eval_pk1": ([i32, _DP, _DP, _DP, _DP], ctypes.c_int),
         "dmpm_eval_hessian": ([i32, _DP, _DP, _DP, _DP, _DP, i32], ctypes.c_int),
         "dmpm_eval_svd": ([i32, _DP, _DP, _DP, _DP], ctypes.c_int),
+        "dmpm_create_batch": ([ctypes.POINTER(Config), i32, ctypes.POINTER(Instance), ctypes.POINTER(vp)], ctypes.c_int),
+        "dmpm_num_instances": ([vp, ctypes.POINTER(i32)], ctypes.c_int),
+        "dmpm_get_instance_losses": ([vp, _DP], ctypes.c_int),
+        "dmpm_target_from_points_batch": ([vp, _DP, _DP, ctypes.POINTER(i32)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -188,6 +203,25 @@ def eval_svd(F):
 
 
 # ------------------------------------------------------------------ Morph --
+def _make_config(L, num_layers, dt, drag, smoothing, f_ext, precision, compat, tape, ckpt_stride,
+                 keep_control_grads, device):
+    cfg = Config()
+    _check(L.dmpm_config_init(ctypes.byref(cfg)))
+    cfg.precision = int(precision)
+    for a in range(3):
+        cfg.f_ext[a] = float(f_ext[a])
+    cfg.num_layers = int(num_layers)
+    cfg.dt = float(dt)
+    cfg.drag = float(drag)
+    cfg.smoothing = float(smoothing)
+    cfg.compat = int(compat)
+    cfg.tape_mode = int(tape)
+    cfg.ckpt_stride = int(ckpt_stride)
+    cfg.keep_control_grads = int(bool(keep_control_grads))
+    cfg.device = int(device)
+    return cfg
+
+
 class Morph:
     """One morph (CompGraph) on one GPU, backed by a ``dmpm_ctx``."""
 
@@ -195,24 +229,13 @@ class Morph:
                  smoothing=0.1, f_ext=(0.0, 0.0, 0.0), precision=PREC_F64, compat=0, tape=TAPE_STORE_ALL,
                  ckpt_stride=0, keep_control_grads=False, device=0):
         L = load_library()
-        cfg = Config()
-        _check(L.dmpm_config_init(ctypes.byref(cfg)))
-        cfg.precision = int(precision)
+        cfg = _make_config(L, num_layers, dt, drag, smoothing, f_ext, precision, compat, tape, ckpt_stride,
+                           keep_control_grads, device)
         cfg.n_particles = int(n)
         for a in range(3):
             cfg.dim[a] = int(dim[a])
             cfg.min_point[a] = float(min_point[a])
-            cfg.f_ext[a] = float(f_ext[a])
-        cfg.num_layers = int(num_layers)
         cfg.dx = float(dx)
-        cfg.dt = float(dt)
-        cfg.drag = float(drag)
-        cfg.smoothing = float(smoothing)
-        cfg.compat = int(compat)
-        cfg.tape_mode = int(tape)
-        cfg.ckpt_stride = int(ckpt_stride)
-        cfg.keep_control_grads = int(bool(keep_control_grads))
-        cfg.device = int(device)
         self.cfg = cfg
         self.n = int(n)
         self.G = int(dim[0]) * int(dim[1]) * int(dim[2])
@@ -382,3 +405,87 @@ def morph_from_scene(sc, dFc=None, **kw):
         mo.set_control(-1, dFc)
     mo.target_from_points(sc.target_x, sc.target_m)
     return mo
+
+
+# ------------------------------------------------------------- MorphBatch --
+class MorphBatch(Morph):
+    """Independent morphs advanced together (FP32 path): every launch covers
+    all instances.  Particle arrays are the instances' particles concatenated
+    in instance order and grid arrays their grids concatenated; ``split`` and
+    ``split_grid`` cut them back per instance.  ``end_layer_mass_loss`` is the
+    sum of the instance losses; ``instance_losses`` gives each."""
+
+    def __init__(self, instances, num_layers, *, dt=1.0 / 120.0, drag=0.5, smoothing=0.1, f_ext=(0.0, 0.0, 0.0),
+                 compat=0, tape=TAPE_STORE_ALL, ckpt_stride=0, keep_control_grads=False, device=0):
+        """``instances``: sequence of (n, dim, dx, min_point)."""
+        L = load_library()
+        cfg = _make_config(L, num_layers, dt, drag, smoothing, f_ext, PREC_F32, compat, tape, ckpt_stride,
+                           keep_control_grads, device)
+        arr = (Instance * len(instances))()
+        for i, (n, dim, dx, mp) in enumerate(instances):
+            arr[i].n_particles = int(n)
+            arr[i].dx = float(dx)
+            for a in range(3):
+                arr[i].dim[a] = int(dim[a])
+                arr[i].min_point[a] = float(mp[a])
+        self.cfg = cfg
+        self.counts = [int(t[0]) for t in instances]
+        self.gcounts = [int(t[1][0]) * int(t[1][1]) * int(t[1][2]) for t in instances]
+        self.B = len(instances)
+        self.n = sum(self.counts)
+        self.G = sum(self.gcounts)
+        self.num_layers = int(num_layers)
+        h = ctypes.c_void_p()
+        _check(L.dmpm_create_batch(ctypes.byref(cfg), len(instances), arr, ctypes.byref(h)))
+        self._h = h
+        self._L = L
+
+    @classmethod
+    def from_scenes(cls, scs, **kw):
+        s0 = scs[0]
+        for k in ("dt", "drag", "smoothing", "f_ext"):
+            kw.setdefault(k, getattr(s0, k))
+        inst = [(sc.n, sc.dim, sc.dx, sc.min_point) for sc in scs]
+        return cls(inst, kw.pop("num_layers", s0.num_layers), **kw)
+
+    def split(self, a):
+        out, p = [], 0
+        for n in self.counts:
+            out.append(a[p:p + n])
+            p += n
+        return out
+
+    def split_grid(self, a):
+        out, q = [], 0
+        for g in self.gcounts:
+            out.append(a[q:q + g])
+            q += g
+        return out
+
+    def target_from_points(self, xs, ms):
+        """Target grids from one point cloud per instance (lists of (n_i,3) and (n_i,))."""
+        counts = (ctypes.c_int32 * self.B)(*[len(m) for m in ms])
+        xa, px = _in(np.concatenate([np.asarray(x, float).reshape(-1, 3) for x in xs]))
+        ma, pm = _in(np.concatenate([np.asarray(m, float).reshape(-1) for m in ms]))
+        _check(self._L.dmpm_target_from_points_batch(self._h, px, pm, counts))
+
+    def instance_losses(self):
+        out = np.zeros(self.B)
+        _check(self._L.dmpm_get_instance_losses(self._h, _ptr(out)))
+        return out
+
+
+def batch_from_scenes(scs, dFcs=None, **kw):
+    """``MorphBatch`` over ``scenes.Scene`` instances: layer 0 particles, GPU volumes,
+    controls (one (n_i,3,3) array per instance, or None) and the target grids."""
+    mb = MorphBatch.from_scenes(scs, **kw)
+    st = [sc.state0() for sc in scs]
+    cat = lambda k: np.concatenate([s[k] for s in st])  # noqa: E731
+    mb.set_particles(cat(0), cat(1), cat(3), cat(2), np.concatenate([sc.m for sc in scs]), None,
+                     np.concatenate([sc.lam for sc in scs]), np.concatenate([sc.mu for sc in scs]))
+    mb.compute_volumes()
+    if dFcs is not None:
+        mb.set_control(-1, np.concatenate([np.asarray(d).reshape(-1, 3, 3) for d in dFcs]))
+    mb.target_from_points([sc.target_x for sc in scs], [sc.target_m for sc in scs])
+    return mb
+

@@ -368,10 +368,13 @@ __device__ __forceinline__ void offsets4(const Axis<float>& ax, float h, float o
 }  // namespace
 
 // ------------------------------------------------------------------ P2G ----
-__global__ __launch_bounds__(kScat) void k_p2g_blk(StepParams sp, StepIO<float> io, const int2* ranges, int cap)
+__global__ __launch_bounds__(kScat) void k_p2g_blk(Batch bt, StepIO<float> io)
 {
     extern __shared__ unsigned long long lds64[];
-    const int2 rg = ranges[blockIdx.x];
+    const int4 rg = bt.ranges[blockIdx.x];
+    const StepParams sp = bt.sps[rg.z];
+    io.grid += 4 * bt.nofs[rg.z];
+    const int cap = bt.cap;
     const int S = sp.stride;
     const int p = rg.x + (int)threadIdx.x;
     const bool valid = p < rg.y;
@@ -449,10 +452,13 @@ __global__ __launch_bounds__(kScat) void k_p2g_blk(StepParams sp, StepIO<float> 
 }
 
 // ------------------------------------------------------------------ G2P ----
-__global__ __launch_bounds__(kGat) void k_g2p_blk(StepParams sp, StepIO<float> io, const int2* ranges, int cap)
+__global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io)
 {
     extern __shared__ float lds[];
-    const int2 rg = ranges[blockIdx.x];
+    const int4 rg = bt.ranges[blockIdx.x];
+    const StepParams sp = bt.sps[rg.z];
+    io.grid += 4 * bt.nofs[rg.z];
+    const int cap = bt.cap;
     const int S = sp.stride;
     const Win W = range_window(sp, io.cur, S, rg.x, rg.y, cap);
     const float* gv = io.grid + sp.G;
@@ -513,10 +519,13 @@ __global__ __launch_bounds__(kGat) void k_g2p_blk(StepParams sp, StepIO<float> i
 // -------------------------------------------------------- transposed G2P ---
 // P_op_2^T + G2P^T: vt = v' + dt x', Ct = C' + (1-s) dt F' Fc^T (kept in tmp),
 // vbar_i += w (vt + D Ct d).
-__global__ __launch_bounds__(kScat) void k_g2p_adj_blk(StepParams sp, AdjIO<float> io, const int2* ranges, int cap)
+__global__ __launch_bounds__(kScat) void k_g2p_adj_blk(Batch bt, AdjIO<float> io)
 {
     extern __shared__ unsigned long long lds64[];
-    const int2 rg = ranges[blockIdx.x];
+    const int4 rg = bt.ranges[blockIdx.x];
+    const StepParams sp = bt.sps[rg.z];
+    io.agrid += 4 * bt.nofs[rg.z];
+    const int cap = bt.cap;
     const int S = sp.stride;
     const int p = rg.x + (int)threadIdx.x;
     const bool valid = p < rg.y;
@@ -595,10 +604,14 @@ __global__ __launch_bounds__(kScat) void k_g2p_adj_blk(StepParams sp, AdjIO<floa
 //   Cbar = m Gb,  vbar = m (1 - dt drag) sum w pbar,
 //   xbar = x' + sum [gw (m mbar + (b + G d).pbar + v_i.vt + D (Ct : v_i d^T))
 //                    - w (G^T pbar + D Ct^T v_i)].
-__global__ __launch_bounds__(kGat) void k_p2g_adj_blk(StepParams sp, AdjIO<float> io, const int2* ranges, int cap)
+__global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
 {
     extern __shared__ float lds[];
-    const int2 rg = ranges[blockIdx.x];
+    const int4 rg = bt.ranges[blockIdx.x];
+    const StepParams sp = bt.sps[rg.z];
+    io.grid += 4 * bt.nofs[rg.z];
+    io.agrid += 4 * bt.nofs[rg.z];
+    const int cap = bt.cap;
     const int S = sp.stride;
     const int cp = sp.compat;
     const Win W = range_window(sp, io.cur, S, rg.x, rg.y, cap);
@@ -698,94 +711,262 @@ __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(StepParams sp, AdjIO<float
     }
 }
 
-// Elastic half of P2G^T + P_op_1^T, one thread per particle:
+// Elastic half of P2G^T + P_op_1^T, one thread per particle of a range:
 //   Pbar = -k Gb Fc,  Fcbar = (1-s)(I + dt C')^T F' - k Gb^T P + H(Fc):Pbar.
-__global__ __launch_bounds__(kGat) void k_elastic_adj(StepParams sp, AdjIO<float> io)
+__global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= sp.n) return;
+    const int4 rg = bt.ranges[blockIdx.x];
+    const StepParams sp = bt.sps[rg.z];
     const int S = sp.stride;
     const int cp = sp.compat;
-    const M3<float> F = ldm(io.cur, FF, S, p);
-    const M3<float> dF = ldm(io.ctrl, 0, S, p);
-    const M3<float> Gb = ldm(io.tmp, 0, S, p);
-    const double vol = io.cst[KVOL * S + p];
-    const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
-    M3<double> Fc;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
-    FCE e;
-    fce_prepare(Fc, e);
-    const M3<double> P = fce_pk1(Fc, e, lam, mu);
-    const double kk_ = 3.0 * sp.inv_dx * sp.inv_dx * sp.dt * vol;
-    const M3<double> Gbd = cast<double>(Gb);
-    M3<double> Pb = mul(Gbd, Fc);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Pb.a[i] *= -kk_;
-    const M3<double> H = fce_hess_mult(e, lam, mu, Pb, cp == 0);
-    const M3<double> GTP = tmul(Gbd, P);
-    const M3<float> C1 = ldm(io.next, FC, S, p);
-    const M3<float> gF1 = ldm(io.g1, FF, S, p);
     const float dt = float(sp.dt);
-    M3<float> IC;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) IC.a[i] = dt * C1.a[i];
-    IC.a[0] += 1.f;
-    IC.a[4] += 1.f;
-    IC.a[8] += 1.f;
-    const M3<float> base = tmul(IC, gF1);
     const float s = float(sp.smoothing);
-    M3<float> gF, gdFc;
+    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) {
+        const M3<float> F = ldm(io.cur, FF, S, p);
+        const M3<float> dF = ldm(io.ctrl, 0, S, p);
+        const M3<float> Gb = ldm(io.tmp, 0, S, p);
+        const double vol = io.cst[KVOL * S + p];
+        const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
+        M3<double> Fc;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        const float rest = float(-kk_ * GTP.a[i] + H.a[i]);
-        if (cp & 1) {
-            const float f = (1.f - s) * base.a[i] + s * gF1.a[i] + base.a[i] + rest;
-            gF.a[i] = f;
-            gdFc.a[i] = f;
-        } else {
-            gdFc.a[i] = (1.f - s) * base.a[i] + rest;
-            gF.a[i] = gdFc.a[i] + s * gF1.a[i];
+        for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
+        FCE e;
+        fce_prepare(Fc, e);
+        const M3<double> P = fce_pk1(Fc, e, lam, mu);
+        const double kk_ = 3.0 * sp.inv_dx * sp.inv_dx * sp.dt * vol;
+        const M3<double> Gbd = cast<double>(Gb);
+        M3<double> Pb = mul(Gbd, Fc);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Pb.a[i] *= -kk_;
+        const M3<double> H = fce_hess_mult(e, lam, mu, Pb, cp == 0);
+        const M3<double> GTP = tmul(Gbd, P);
+        const M3<float> C1 = ldm(io.next, FC, S, p);
+        const M3<float> gF1 = ldm(io.g1, FF, S, p);
+        M3<float> IC;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) IC.a[i] = dt * C1.a[i];
+        IC.a[0] += 1.f;
+        IC.a[4] += 1.f;
+        IC.a[8] += 1.f;
+        const M3<float> base = tmul(IC, gF1);
+        M3<float> gF, gdFc;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const float rest = float(-kk_ * GTP.a[i] + H.a[i]);
+            if (cp & 1) {
+                const float f = (1.f - s) * base.a[i] + s * gF1.a[i] + base.a[i] + rest;
+                gF.a[i] = f;
+                gdFc.a[i] = f;
+            } else {
+                gdFc.a[i] = (1.f - s) * base.a[i] + rest;
+                gF.a[i] = gdFc.a[i] + s * gF1.a[i];
+            }
         }
+        stm(io.g0, FF, S, p, gF);
+        stm(io.gctrl, 0, S, p, gdFc);
     }
-    stm(io.g0, FF, S, p, gF);
-    stm(io.gctrl, 0, S, p, gdFc);
+}
+
+// --------------------------------------------------- batched grid kernels --
+namespace {
+// instance owning flat node q (nofs ascending, B+1 entries)
+__device__ __forceinline__ int instance_of(const Batch& b, long long q)
+{
+    int lo = 0, hi = b.B - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.nofs[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+}  // namespace
+
+// G_op (ForwardSimulation.cpp:53-58): v = p/m + dt f where |m| > FLT_EPSILON, else 0
+__global__ __launch_bounds__(kGat) void k_grid_update_b(Batch bt, float* grid)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= bt.Gtot) return;
+    const int inst = instance_of(bt, q);
+    const StepParams& sp = bt.sps[inst];
+    const long long l = q - bt.nofs[inst];
+    float* g = grid + 4 * bt.nofs[inst];
+    const float m = g[l];
+    if (fabsf(m) > float(kMassEps)) {
+        const float inv = 1.f / m;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) g[(1 + a) * sp.G + l] = g[(1 + a) * sp.G + l] * inv + float(sp.dt) * float(sp.f_ext[a]);
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) g[(1 + a) * sp.G + l] = 0.f;
+    }
+}
+
+// G_op^T (BackPropagation.cpp:46-59): pbar = vbar/m, mbar = -((v - dt f).vbar)/m
+__global__ __launch_bounds__(kGat) void k_grid_adj_b(Batch bt, const float* grid, float* agrid)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= bt.Gtot) return;
+    const int inst = instance_of(bt, q);
+    const StepParams& sp = bt.sps[inst];
+    const long long l = q - bt.nofs[inst];
+    const float* g = grid + 4 * bt.nofs[inst];
+    float* ag = agrid + 4 * bt.nofs[inst];
+    const float m = g[l];
+    if (fabsf(m) > float(kMassEps)) {
+        const float inv = 1.f / m;
+        float dot = 0.f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float vi = g[(1 + a) * sp.G + l];
+            if (!(sp.compat & 8)) vi -= float(sp.dt) * float(sp.f_ext[a]);
+            const float vb = ag[a * sp.G + l];
+            dot += vi * vb;
+            ag[a * sp.G + l] = vb * inv;
+        }
+        ag[3 * sp.G + l] = -dot * inv;
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ag[a * sp.G + l] = 0.f;
+        ag[3 * sp.G + l] = 0.f;
+    }
+}
+
+// EndLayerMassLoss node terms (CompGraph.cpp:143-187) in FP64.  blockIdx.y is
+// the instance; gm and target hold one node mass per node (instance i at
+// nofs[i]); per-instance loss sums go to acc[i].
+__global__ __launch_bounds__(kGat) void k_loss_b(Batch bt, const float* gm, const float* target, double* dLdm, double* acc)
+{
+    __shared__ double red[kGat / 64];
+    const int inst = blockIdx.y;
+    const long long n0 = bt.nofs[inst], n1 = bt.nofs[inst + 1];
+    double local = 0.0;
+    for (long long q = n0 + (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n1; q += (long long)gridDim.x * blockDim.x) {
+        const double cm = gm[q], tm = target[q];
+        const double c = log(cm + 1.0 + 1e-4) - log(tm + 1.0 + 1e-4);
+        double l = 0.5 * c * c;
+        double g = c / (cm + 1.0 + 1e-4);
+        if (cm < 1e-3) {
+            const double diff = 1e-3 - cm;
+            l += 10.0 * diff * diff;
+            g -= 20.0 * diff;
+        }
+        local += l;
+        dLdm[q] = g;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+#pragma unroll
+        for (int w = 0; w < kGat / 64; ++w) sum += red[w];
+        atomicAdd(acc + inst, sum);
+    }
+}
+
+// EndLayerMassLoss particle gather (CompGraph.cpp:192-241) in FP64 over the ranges
+__global__ __launch_bounds__(kGat) void k_loss_particles_b(Batch bt, const float* layer, const float* cst, const double* dLdm, float* g)
+{
+    const int4 rg = bt.ranges[blockIdx.x];
+    const StepParams sp = bt.sps[rg.z];
+    const int S = sp.stride;
+    const double* dl = dLdm + bt.nofs[rg.z];
+    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) {
+        Axis<double> st[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            axis_weights<double>((double(layer[(long long)(FX + a) * S + p]) - sp.min_point[a]) * sp.inv_dx, st[a], true);
+        double gx[3] = {0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int gi = st[0].base + i;
+            if (gi < 0 || gi >= sp.dim[0]) continue;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int gj = st[1].base + j;
+                if (gj < 0 || gj >= sp.dim[1]) continue;
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const int gk = st[2].base + kk;
+                    if (gk < 0 || gk >= sp.dim[2]) continue;
+                    const double d = dl[((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk];
+                    gx[0] += d * st[0].dw[i] * st[1].w[j] * st[2].w[kk];
+                    gx[1] += d * st[0].w[i] * st[1].dw[j] * st[2].w[kk];
+                    gx[2] += d * st[0].w[i] * st[1].w[j] * st[2].dw[kk];
+                }
+            }
+        }
+        const double sc = -double(cst[KM * S + p]) * sp.inv_dx;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) g[(long long)(FX + a) * S + p] = float(sc * gx[a]);
+#pragma unroll
+        for (int f = FV; f < kLayerFields; ++f) g[(long long)f * S + p] = 0.f;
+    }
 }
 
 // ------------------------------------------------------------ launchers ----
 // LDS: NC channels of max(cap, 8^3) nodes (int64 for scatters, float for gathers)
-void launch_p2g_blk(const StepParams& sp, const StepIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s)
+static size_t win_nodes(int cap) { return (size_t)(cap > kFW3 ? cap : kFW3); }
+
+void launch_p2g_blk(const Batch& b, const StepIO<float>& io, hipStream_t s)
 {
-    if (nranges <= 0) return;
-    const size_t lds = 4 * (size_t)(cap > kFW3 ? cap : kFW3) * sizeof(unsigned long long);
-    hipLaunchKernelGGL(k_p2g_blk, dim3(nranges), dim3(kScat), lds, s, sp, io, ranges, cap);
+    if (b.nranges <= 0) return;
+    hipLaunchKernelGGL(k_p2g_blk, dim3(b.nranges), dim3(kScat), 4 * win_nodes(b.cap) * sizeof(unsigned long long), s, b, io);
 }
 
-void launch_g2p_blk(const StepParams& sp, const StepIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s)
+void launch_g2p_blk(const Batch& b, const StepIO<float>& io, hipStream_t s)
 {
-    if (nranges <= 0) return;
-    const size_t lds = 3 * (size_t)(cap > kFW3 ? cap : kFW3) * sizeof(float);
-    hipLaunchKernelGGL(k_g2p_blk, dim3(nranges), dim3(kGat), lds, s, sp, io, ranges, cap);
+    if (b.nranges <= 0) return;
+    hipLaunchKernelGGL(k_g2p_blk, dim3(b.nranges), dim3(kGat), 3 * win_nodes(b.cap) * sizeof(float), s, b, io);
 }
 
-void launch_g2p_adj_blk(const StepParams& sp, const AdjIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s)
+void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
 {
-    if (nranges <= 0) return;
-    const size_t lds = 3 * (size_t)(cap > kFW3 ? cap : kFW3) * sizeof(unsigned long long);
-    hipLaunchKernelGGL(k_g2p_adj_blk, dim3(nranges), dim3(kScat), lds, s, sp, io, ranges, cap);
+    if (b.nranges <= 0) return;
+    hipLaunchKernelGGL(k_g2p_adj_blk, dim3(b.nranges), dim3(kScat), 3 * win_nodes(b.cap) * sizeof(unsigned long long), s, b, io);
 }
 
-void launch_p2g_adj_blk(const StepParams& sp, const AdjIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s)
+void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
 {
-    if (nranges <= 0) return;
-    const size_t lds = 7 * (size_t)(cap > kFW3 ? cap : kFW3) * sizeof(float);
-    hipLaunchKernelGGL(k_p2g_adj_blk, dim3(nranges), dim3(kGat), lds, s, sp, io, ranges, cap);
+    if (b.nranges <= 0) return;
+    hipLaunchKernelGGL(k_p2g_adj_blk, dim3(b.nranges), dim3(kGat), 7 * win_nodes(b.cap) * sizeof(float), s, b, io);
 }
 
-void launch_elastic_adj(const StepParams& sp, const AdjIO<float>& io, hipStream_t s)
+void launch_elastic_adj(const Batch& b, const AdjIO<float>& io, hipStream_t s)
 {
-    if (sp.n <= 0) return;
-    hipLaunchKernelGGL(k_elastic_adj, dim3((sp.n + kGat - 1) / kGat), dim3(kGat), 0, s, sp, io);
+    if (b.nranges <= 0) return;
+    hipLaunchKernelGGL(k_elastic_adj, dim3(b.nranges), dim3(kGat), 0, s, b, io);
+}
+
+void launch_grid_update_b(const Batch& b, float* grid, hipStream_t s)
+{
+    if (b.Gtot <= 0) return;
+    hipLaunchKernelGGL(k_grid_update_b, dim3((unsigned)((b.Gtot + kGat - 1) / kGat)), dim3(kGat), 0, s, b, grid);
+}
+
+void launch_grid_adj_b(const Batch& b, const float* grid, float* agrid, hipStream_t s)
+{
+    if (b.Gtot <= 0) return;
+    hipLaunchKernelGGL(k_grid_adj_b, dim3((unsigned)((b.Gtot + kGat - 1) / kGat)), dim3(kGat), 0, s, b, grid, agrid);
+}
+
+void launch_loss_b(const Batch& b, const float* gm, const float* target, double* dLdm, double* acc, long long gmax, hipStream_t s)
+{
+    if (b.Gtot <= 0 || b.B <= 0) return;
+    long long nb = (gmax + kGat - 1) / kGat;
+    const long long lim = 4096 / b.B > 16 ? 4096 / b.B : 16;
+    if (nb > lim) nb = lim;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_loss_b, dim3((unsigned)nb, (unsigned)b.B), dim3(kGat), 0, s, b, gm, target, dLdm, acc);
+}
+
+void launch_loss_particles_b(const Batch& b, const float* layer, const float* cst, const double* dLdm, float* g, hipStream_t s)
+{
+    if (b.nranges <= 0) return;
+    hipLaunchKernelGGL(k_loss_particles_b, dim3(b.nranges), dim3(kGat), 0, s, b, layer, cst, dLdm, g);
 }
 
 }  // namespace dmpm

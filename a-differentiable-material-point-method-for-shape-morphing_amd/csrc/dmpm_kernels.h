@@ -63,15 +63,36 @@ template <typename Real> void launch_g2p_adj(const StepParams& sp, const AdjIO<R
 template <typename Real> void launch_grid_adj(const StepParams& sp, const AdjIO<Real>& io, hipStream_t s);
 template <typename Real> void launch_p2g_adj(const StepParams& sp, const AdjIO<Real>& io, hipStream_t s);
 
-// FP32 path: one workgroup per particle range of a 4^3-cell block (ranges[i] =
-// {begin, end} in device order, at most 512 particles), node window staged in LDS (cap = window
-// capacity in nodes; larger windows fall back to direct HBM access)
-void launch_p2g_blk(const StepParams& sp, const StepIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s);
-void launch_g2p_blk(const StepParams& sp, const StepIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s);
-void launch_g2p_adj_blk(const StepParams& sp, const AdjIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s);
-void launch_p2g_adj_blk(const StepParams& sp, const AdjIO<float>& io, const int2* ranges, int nranges, int cap, hipStream_t s);
-// FP32 path: elastic half of the transposed P2G (Gb from tmp -> dLdF, dL/d(dFc))
-void launch_elastic_adj(const StepParams& sp, const AdjIO<float>& io, hipStream_t s);
+// FP32 path: many instances (independent morphs) per launch.  Particles of
+// all instances share one SoA layer (instance-major, block-sorted within an
+// instance); each instance's grid is a block of the concatenated grids.
+// Work items are particle ranges of one 4^3-cell block of one instance
+// (ranges[i] = {begin, end, instance, 0} in device order, at most 512
+// particles); the node window is staged in LDS (cap = general-window
+// capacity in nodes; larger windows fall back to direct HBM access).
+struct Batch {
+    const StepParams* sps;  // per instance: dims, G, dx, min point, dt, ...
+    const long long* nofs;  // node prefix, B+1 entries: instance i owns nodes [nofs[i], nofs[i+1])
+    const int4* ranges;
+    int nranges;
+    int B;
+    long long Gtot;         // total nodes
+    int cap;
+};
+// 4-channel grids (m or mbar, then 3 vector channels) of instance i start at 4 * nofs[i];
+// 1-channel arrays (target, dL/dm) at nofs[i].
+void launch_p2g_blk(const Batch& b, const StepIO<float>& io, hipStream_t s);
+void launch_g2p_blk(const Batch& b, const StepIO<float>& io, hipStream_t s);
+void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s);
+void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s);
+// elastic half of the transposed P2G (Gb from tmp -> dLdF, dL/d(dFc)), over the ranges
+void launch_elastic_adj(const Batch& b, const AdjIO<float>& io, hipStream_t s);
+void launch_grid_update_b(const Batch& b, float* grid, hipStream_t s);
+void launch_grid_adj_b(const Batch& b, const float* grid, float* agrid, hipStream_t s);
+// loss over all instances from node masses gm (instance i at nofs[i]);
+// per-instance sums added to acc[0..B); gmax = largest instance node count
+void launch_loss_b(const Batch& b, const float* gm, const float* target, double* dLdm, double* acc, long long gmax, hipStream_t s);
+void launch_loss_particles_b(const Batch& b, const float* layer, const float* cst, const double* dLdm, float* g, hipStream_t s);
 
 // batched constitutive evaluations (AoS 9 doubles per matrix)
 void launch_eval_pk1(int n, const double* F, const double* lam, const double* mu, double* P, hipStream_t s);

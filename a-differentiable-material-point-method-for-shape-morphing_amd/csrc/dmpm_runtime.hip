@@ -1,6 +1,7 @@
 // C ABI, device memory and episode driver of the differentiable MLS-MPM step.
 //
-// A context holds one morph on one GPU: per-particle constants, per-layer
+// A context holds one morph (or, FP32, a batch of independent morphs, each
+// with its own particles and grid) on one GPU: per-particle constants, per-layer
 // controls, the forward trajectory (every layer, or checkpoints plus one
 // segment tape), two ping-pong gradient layers and the work grids.  The
 // passes follow CompGraph (CompGraph.cpp:246-267): forward over layers
@@ -66,13 +67,24 @@ struct dmpm_ctx {
     void* target = nullptr;   // G
     double* dacc = nullptr;   // FP64 accumulators
     uint64_t bytes = 0;
-    // FP32 path: device order of the particles (sorted by 4^3-cell block);
-    // pos[p] is the device slot of API particle p (empty = identity)
+    // instances: particles of instance i are API/device slots [poff[i], poff[i+1]),
+    // its grid nodes [nofs[i], nofs[i+1]) (4-channel grids at 4 * nofs[i])
+    int B = 1;
+    std::vector<dmpm_instance> inst;
+    std::vector<int> poff;
+    std::vector<long long> nofs;
+    long long gmax = 0;
+    std::vector<double> inst_loss;
+    // FP32 path: device order of the particles (sorted by 4^3-cell block per
+    // instance); pos[p] is the device slot of API particle p (empty = identity)
     std::vector<int> pos;
-    int2* ranges = nullptr;   // per-workgroup particle ranges of the block kernels
-    std::vector<int2> host_ranges;
+    int4* ranges = nullptr;   // per-workgroup {begin, end, instance} of the block kernels
+    std::vector<int4> host_ranges;
     int n_ranges = 0;
     int lds_cap = 1024;       // LDS node window of the block kernels
+    StepParams* d_sps = nullptr;
+    long long* d_nofs = nullptr;
+    Batch batch{};
     // checkpoint-mode slot layout
     int n_ckpt = 0, seg_last = 0, tape_base = 0, roll_base = 0, n_slots = 0;
     int tape_seg = -1;
@@ -150,25 +162,30 @@ int drain_profile(dmpm_ctx* c)
     return DMPM_OK;
 }
 
-StepParams step_params(const dmpm_ctx* c)
+// Step parameters of instance i (particle count, grid, time step).
+StepParams inst_params(const dmpm_ctx* c, int i)
 {
+    const dmpm_instance& in = c->inst[i];
     StepParams sp{};
-    sp.n = c->N;
+    sp.n = in.n_particles;
     sp.stride = c->S;
     for (int a = 0; a < 3; ++a) {
-        sp.dim[a] = c->cfg.dim[a];
-        sp.min_point[a] = c->cfg.min_point[a];
+        sp.dim[a] = in.dim[a];
+        sp.min_point[a] = in.min_point[a];
         sp.f_ext[a] = c->cfg.f_ext[a];
     }
-    sp.G = c->G;
-    sp.dx = c->cfg.dx;
-    sp.inv_dx = 1.0 / c->cfg.dx;
+    sp.G = c->nofs[i + 1] - c->nofs[i];
+    sp.dx = in.dx;
+    sp.inv_dx = 1.0 / in.dx;
     sp.dt = c->cfg.dt;
     sp.drag = c->cfg.drag;
     sp.smoothing = c->cfg.smoothing;
     sp.compat = c->cfg.compat;
     return sp;
 }
+
+// Parameters of the single-instance (FP64) path.
+StepParams step_params(const dmpm_ctx* c) { return inst_params(c, 0); }
 
 // ------------------------------------------------------------ slot layout --
 template <typename Real>
@@ -224,16 +241,20 @@ void fill(dmpm_ctx* c, Real* p, long long count, Real v)
 // with direct HBM atomics; the FP32 path runs one workgroup per block range
 // with the node window staged in LDS.
 inline void p2g_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<double>& io) { launch_p2g<double>(sp, io, c->stream); }
-inline void p2g_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<float>& io) { launch_p2g_blk(sp, io, c->ranges, c->n_ranges, c->lds_cap, c->stream); }
+inline void p2g_dispatch(dmpm_ctx* c, const StepParams&, const StepIO<float>& io) { launch_p2g_blk(c->batch, io, c->stream); }
+inline void grid_update_dispatch(dmpm_ctx* c, const StepParams& sp, double* g) { launch_grid_update<double>(sp, g, c->stream); }
+inline void grid_update_dispatch(dmpm_ctx* c, const StepParams&, float* g) { launch_grid_update_b(c->batch, g, c->stream); }
 inline void g2p_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<double>& io) { launch_g2p<double>(sp, io, c->stream); }
-inline void g2p_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<float>& io) { launch_g2p_blk(sp, io, c->ranges, c->n_ranges, c->lds_cap, c->stream); }
+inline void g2p_dispatch(dmpm_ctx* c, const StepParams&, const StepIO<float>& io) { launch_g2p_blk(c->batch, io, c->stream); }
 inline void g2p_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<double>& io) { launch_g2p_adj<double>(sp, io, c->stream); }
-inline void g2p_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<float>& io) { launch_g2p_adj_blk(sp, io, c->ranges, c->n_ranges, c->lds_cap, c->stream); }
+inline void g2p_adj_dispatch(dmpm_ctx* c, const StepParams&, const AdjIO<float>& io) { launch_g2p_adj_blk(c->batch, io, c->stream); }
+inline void grid_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<double>& io) { launch_grid_adj<double>(sp, io, c->stream); }
+inline void grid_adj_dispatch(dmpm_ctx* c, const StepParams&, const AdjIO<float>& io) { launch_grid_adj_b(c->batch, io.grid, io.agrid, c->stream); }
 inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<double>& io) { launch_p2g_adj<double>(sp, io, c->stream); }
-inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<float>& io)
+inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams&, const AdjIO<float>& io)
 {
-    launch_p2g_adj_blk(sp, io, c->ranges, c->n_ranges, c->lds_cap, c->stream);
-    launch_elastic_adj(sp, io, c->stream);
+    launch_p2g_adj_blk(c->batch, io, c->stream);
+    launch_elastic_adj(c->batch, io, c->stream);
 }
 
 // One ForwardTimeStep: cur -> nxt, grid of the step in g.
@@ -244,7 +265,7 @@ void step_forward(dmpm_ctx* c, const Real* cur, Real* nxt, const Real* ctrl, Rea
     StepIO<Real> io{cur, nxt, ctrl, static_cast<const Real*>(c->cst), g};
     fill<Real>(c, g, (long long)kGridFields * c->G, Real(0));
     timed(c, DMPM_K_P2G, [&] { p2g_dispatch(c, sp, io); });
-    timed(c, DMPM_K_GRID_UPDATE, [&] { launch_grid_update<Real>(sp, g, c->stream); });
+    timed(c, DMPM_K_GRID_UPDATE, [&] { grid_update_dispatch(c, sp, g); });
     timed(c, DMPM_K_G2P, [&] { g2p_dispatch(c, sp, io); });
 }
 
@@ -264,9 +285,10 @@ void step_backward(dmpm_ctx* c, int i, const Real* cur, const Real* nxt, const R
     io.g0 = grads_of<Real>(c, i);
     io.gctrl = gctrl_of<Real>(c, i);
     io.tmp = static_cast<Real*>(c->tmp);
-    fill<Real>(c, io.agrid, 3 * c->G, Real(0));
+    // vbar channels (one instance: the first 3G reals; a batch interleaves them per instance)
+    fill<Real>(c, io.agrid, (c->B > 1 ? kGridFields : 3) * c->G, Real(0));
     timed(c, DMPM_K_G2P_ADJ, [&] { g2p_adj_dispatch(c, sp, io); });
-    timed(c, DMPM_K_GRID_ADJ, [&] { launch_grid_adj<Real>(sp, io, c->stream); });
+    timed(c, DMPM_K_GRID_ADJ, [&] { grid_adj_dispatch(c, sp, io); });
     timed(c, DMPM_K_P2G_ADJ, [&] { p2g_adj_dispatch(c, sp, io); });
 }
 
@@ -339,6 +361,22 @@ int forward_pass(dmpm_ctx* c, int start)
     return DMPM_OK;
 }
 
+// Loss node terms and particle seed: per-particle kernels (FP64 path) or the
+// batched kernels over the block ranges (FP32 path).
+void loss_kernels(dmpm_ctx* c, const double* lay, const double* gm, double* dLdm)
+{
+    const StepParams sp = step_params(c);
+    const double* cst = static_cast<const double*>(c->cst);
+    launch_loss<double>(sp, gm, static_cast<const double*>(c->target), dLdm, c->dacc, c->stream);
+    launch_loss_particles<double>(sp, lay, cst, dLdm, grads_of<double>(c, c->L - 1), c->stream);
+}
+
+void loss_kernels(dmpm_ctx* c, const float* lay, const float* gm, double* dLdm)
+{
+    launch_loss_b(c->batch, gm, static_cast<const float*>(c->target), dLdm, c->dacc, c->gmax, c->stream);
+    launch_loss_particles_b(c->batch, lay, static_cast<const float*>(c->cst), dLdm, grads_of<float>(c, c->L - 1), c->stream);
+}
+
 template <typename Real>
 int end_loss(dmpm_ctx* c, double* loss)
 {
@@ -346,22 +384,24 @@ int end_loss(dmpm_ctx* c, double* loss)
     if (c->fwd_valid < c->L - 1) return fail(DMPM_ERR_STATE, "loss before the forward pass reached the last layer");
     const int last = c->L - 1;
     Real* lay = (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) ? slot<Real>(c, last) : materialize<Real>(c, last);
-    StepParams sp = step_params(c);
     Real* gm = static_cast<Real*>(c->wgrid);
-    // FP64 dL/dm after 2G reals of the 4G-real work grid (8-byte aligned for both precisions)
+    // node masses in the first G reals; FP64 dL/dm after 2G reals of the
+    // 4G-real work grid (8-byte aligned for both precisions)
     double* dLdm = reinterpret_cast<double*>(gm + 2 * c->G);
     fill<Real>(c, gm, c->G, Real(0));
-    HIP_TRY(hipMemsetAsync(c->dacc, 0, sizeof(double), c->stream));
+    launch_fill<double>(c->dacc, c->B, 0.0, c->stream);
     const Real* cst = static_cast<const Real*>(c->cst);
     timed(c, DMPM_K_LOSS, [&] {
-        launch_p2g_mass<Real>(sp, lay, cst + KM * c->S, c->S, gm, c->stream);
-        launch_loss<Real>(sp, gm, static_cast<const Real*>(c->target), dLdm, c->dacc, c->stream);
-        launch_loss_particles<Real>(sp, lay, cst, dLdm, grads_of<Real>(c, last), c->stream);
+        for (int i = 0; i < c->B; ++i)
+            launch_p2g_mass<Real>(inst_params(c, i), lay + c->poff[i], cst + KM * c->S + c->poff[i], c->S, gm + c->nofs[i], c->stream);
+        loss_kernels(c, lay, gm, dLdm);
     });
     HIP_TRY(hipGetLastError());
-    double h = 0.0;
-    HIP_TRY(hipMemcpyAsync(&h, c->dacc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    c->inst_loss.assign(c->B, 0.0);
+    HIP_TRY(hipMemcpyAsync(c->inst_loss.data(), c->dacc, c->B * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    double h = 0.0;
+    for (double v : c->inst_loss) h += v;
     if (loss) *loss = h;
     c->loss_valid = true;
     c->grads_lo = last;
@@ -425,57 +465,60 @@ int download_fields(dmpm_ctx* c, const Real* src, int field0, int nfield, double
     return DMPM_OK;
 }
 
-// Device order of the FP32 path.  Particles are grouped by the 4^3-cell
-// block of their cell (blocks in x-major order); inside a block they are
-// ordered by their rank within their cell ("slot"), then by the cell's Morton
-// index.  A 64-lane wave then holds one particle of each of the block's 64
-// cells: its LDS atomics hit 64 distinct nodes instead of 8 particles of one
-// cell hitting the same node.  Ties keep the API order.
+// Device order of the FP32 path, per instance.  Particles are grouped by the
+// 4^3-cell block of their cell (blocks in x-major order); inside a block they
+// are ordered by their rank within their cell ("slot"), then by the cell's
+// Morton index.  A 64-lane wave then holds one particle of each of the
+// block's 64 cells: its LDS atomics hit 64 distinct nodes instead of 8
+// particles of one cell hitting the same node.  Ties keep the API order.
+// Instances keep their API slot range [poff[i], poff[i+1]).
 void sort_particles(dmpm_ctx* c, const double* x)
 {
-    const int N = c->N;
-    int nb[3];
-    for (int a = 0; a < 3; ++a) nb[a] = (c->cfg.dim[a] + 3) / 4;
-    std::vector<uint64_t> cellkey(N);
-    for (int p = 0; p < N; ++p) {
-        int cell[3];
-        for (int a = 0; a < 3; ++a) {
-            double u = std::floor((x[3 * (size_t)p + a] - c->cfg.min_point[a]) / c->cfg.dx);
-            if (!(u >= 0.0)) u = 0.0;  // also maps NaN
-            if (u > c->cfg.dim[a] - 1) u = c->cfg.dim[a] - 1;
-            cell[a] = (int)u;
-        }
-        const uint64_t block = ((uint64_t)(cell[0] >> 2) * nb[1] + (cell[1] >> 2)) * nb[2] + (cell[2] >> 2);
-        uint32_t mort = 0;
-        for (int bit = 0; bit < 2; ++bit)
-            for (int a = 0; a < 3; ++a) mort |= (uint32_t)((cell[a] >> bit) & 1) << (3 * bit + (2 - a));
-        cellkey[p] = (block << 6) | mort;
-    }
-    std::vector<int> order(N);
-    for (int p = 0; p < N; ++p) order[p] = p;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cellkey[a] < cellkey[b]; });
-    // slot = rank inside the cell; final key (block, slot, Morton cell)
-    std::vector<uint64_t> key(N);
-    for (int d = 0; d < N; ++d) {
-        const int p = order[d];
-        const uint32_t slot = (d > 0 && cellkey[order[d - 1]] == cellkey[p]) ? (uint32_t)(key[order[d - 1]] >> 6 & 0xffffu) + 1u : 0u;
-        key[p] = ((cellkey[p] >> 6) << 22) | ((uint64_t)(slot & 0xffffu) << 6) | (cellkey[p] & 63u);
-    }
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
-    c->pos.assign(N, 0);
-    for (int d = 0; d < N; ++d) c->pos[order[d]] = d;
+    c->pos.assign(c->N, 0);
+    c->host_ranges.clear();
     // one workgroup range per block, at most 512 particles (the scatter
     // kernels hold one particle per thread of a 512-thread workgroup)
     constexpr int kMaxRange = 512;
-    std::vector<int2> rg;
-    int d0 = 0;
-    for (int d = 1; d <= N; ++d) {
-        if (d == N || (key[order[d]] >> 22) != (key[order[d0]] >> 22) || d - d0 == kMaxRange) {
-            rg.push_back(make_int2(d0, d));
-            d0 = d;
+    for (int in = 0; in < c->B; ++in) {
+        const dmpm_instance& g = c->inst[in];
+        const int p0 = c->poff[in], N = g.n_particles;
+        int nb[3];
+        for (int a = 0; a < 3; ++a) nb[a] = (g.dim[a] + 3) / 4;
+        std::vector<uint64_t> cellkey(N);
+        for (int p = 0; p < N; ++p) {
+            int cell[3];
+            for (int a = 0; a < 3; ++a) {
+                double u = std::floor((x[3 * (size_t)(p0 + p) + a] - g.min_point[a]) / g.dx);
+                if (!(u >= 0.0)) u = 0.0;  // also maps NaN
+                if (u > g.dim[a] - 1) u = g.dim[a] - 1;
+                cell[a] = (int)u;
+            }
+            const uint64_t block = ((uint64_t)(cell[0] >> 2) * nb[1] + (cell[1] >> 2)) * nb[2] + (cell[2] >> 2);
+            uint32_t mort = 0;
+            for (int bit = 0; bit < 2; ++bit)
+                for (int a = 0; a < 3; ++a) mort |= (uint32_t)((cell[a] >> bit) & 1) << (3 * bit + (2 - a));
+            cellkey[p] = (block << 6) | mort;
+        }
+        std::vector<int> order(N);
+        for (int p = 0; p < N; ++p) order[p] = p;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cellkey[a] < cellkey[b]; });
+        // slot = rank inside the cell; final key (block, slot, Morton cell)
+        std::vector<uint64_t> key(N);
+        for (int d = 0; d < N; ++d) {
+            const int p = order[d];
+            const uint32_t sl = (d > 0 && cellkey[order[d - 1]] == cellkey[p]) ? (uint32_t)(key[order[d - 1]] >> 6 & 0xffffu) + 1u : 0u;
+            key[p] = ((cellkey[p] >> 6) << 22) | ((uint64_t)(sl & 0xffffu) << 6) | (cellkey[p] & 63u);
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+        for (int d = 0; d < N; ++d) c->pos[p0 + order[d]] = p0 + d;
+        int d0 = 0;
+        for (int d = 1; d <= N; ++d) {
+            if (d == N || (key[order[d]] >> 22) != (key[order[d0]] >> 22) || d - d0 == kMaxRange) {
+                c->host_ranges.push_back(make_int4(p0 + d0, p0 + d, in, 0));
+                d0 = d;
+            }
         }
     }
-    c->host_ranges.swap(rg);
 }
 
 template <typename Real>
@@ -491,9 +534,11 @@ int set_particles(dmpm_ctx* c, const double* x, const double* v, const double* F
         }
         c->n_ranges = (int)c->host_ranges.size();
         if (c->n_ranges > 0) {
-            HIP_TRY(hipMalloc(&c->ranges, c->host_ranges.size() * sizeof(int2)));
-            HIP_TRY(hipMemcpy(c->ranges, c->host_ranges.data(), c->host_ranges.size() * sizeof(int2), hipMemcpyHostToDevice));
+            HIP_TRY(hipMalloc(&c->ranges, c->host_ranges.size() * sizeof(int4)));
+            HIP_TRY(hipMemcpy(c->ranges, c->host_ranges.data(), c->host_ranges.size() * sizeof(int4), hipMemcpyHostToDevice));
         }
+        c->batch.ranges = c->ranges;
+        c->batch.nranges = c->n_ranges;
     }
     // a new particle set starts with zero controls (MaterialPoint.h:31)
     fill<Real>(c, static_cast<Real*>(c->ctrl), (long long)(c->L - 1) * 9 * c->S, Real(0));
@@ -524,38 +569,57 @@ template <typename Real>
 int compute_volumes(dmpm_ctx* c)
 {
     if (!c->have_particles) return fail(DMPM_ERR_STATE, "volumes before dmpm_set_particles");
-    StepParams sp = step_params(c);
     Real* gm = static_cast<Real*>(c->wgrid);
     Real* cst = static_cast<Real*>(c->cst);
     fill<Real>(c, gm, c->G, Real(0));
-    launch_p2g_mass<Real>(sp, slot<Real>(c, 0), cst + KM * c->S, c->S, gm, c->stream);
-    launch_volumes<Real>(sp, slot<Real>(c, 0), cst, gm, c->stream);
+    for (int i = 0; i < c->B; ++i) {
+        const StepParams sp = inst_params(c, i);
+        const int p0 = c->poff[i];
+        launch_p2g_mass<Real>(sp, slot<Real>(c, 0) + p0, cst + KM * c->S + p0, c->S, gm + c->nofs[i], c->stream);
+        launch_volumes<Real>(sp, slot<Real>(c, 0) + p0, cst + p0, gm + c->nofs[i], c->stream);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->fwd_valid = std::min(c->fwd_valid, 0);
     return DMPM_OK;
 }
 
+// P2G mass of point clouds into the target grids: counts[i] points for
+// instance i, concatenated in instance order.
 template <typename Real>
-int target_from_points(dmpm_ctx* c, const double* x, const double* m, int n)
+int target_from_points(dmpm_ctx* c, const double* x, const double* m, const int* counts)
 {
-    if (n < 0 || (n > 0 && (!x || !m))) return fail(DMPM_ERR_INVALID, "bad target point cloud");
-    StepParams sp = step_params(c);
-    sp.n = n;
+    long long n = 0;
+    for (int i = 0; i < c->B; ++i) {
+        if (counts[i] < 0) return fail(DMPM_ERR_INVALID, "bad target point count");
+        n += counts[i];
+    }
+    if (n > INT32_MAX) return fail(DMPM_ERR_INVALID, "too many target points");
+    if (n > 0 && (!x || !m)) return fail(DMPM_ERR_INVALID, "bad target point cloud");
     Real* tg = static_cast<Real*>(c->target);
     fill<Real>(c, tg, c->G, Real(0));
     if (n > 0) {
         void* buf = nullptr;
         hipError_t e = hipMalloc(&buf, (size_t)4 * n * sizeof(Real));
         if (e != hipSuccess) return fail(DMPM_ERR_NOMEM, std::string("target points: ") + hipGetErrorString(e));
+        // SoA [4][n]: x, y, z, m; instance i's points at column offset sum(counts[<i])
         std::vector<Real> h((size_t)4 * n);
-        for (int p = 0; p < n; ++p) {
+        for (long long p = 0; p < n; ++p) {
             for (int a = 0; a < 3; ++a) h[(size_t)a * n + p] = Real(x[3 * (size_t)p + a]);
             h[(size_t)3 * n + p] = Real(m[p]);
         }
         Real* d = static_cast<Real*>(buf);
         hipError_t e2 = hipMemcpyAsync(d, h.data(), h.size() * sizeof(Real), hipMemcpyHostToDevice, c->stream);
-        if (e2 == hipSuccess) launch_p2g_mass<Real>(sp, d, d + (size_t)3 * n, n, tg, c->stream);
+        if (e2 == hipSuccess) {
+            long long q0 = 0;
+            for (int i = 0; i < c->B; ++i) {
+                StepParams sp = inst_params(c, i);
+                sp.n = counts[i];
+                if (counts[i] > 0)
+                    launch_p2g_mass<Real>(sp, d + q0, d + (size_t)3 * n + q0, (int)n, tg + c->nofs[i], c->stream);
+                q0 += counts[i];
+            }
+        }
         hipError_t e3 = hipStreamSynchronize(c->stream);
         hipFree(buf);
         if (e2 != hipSuccess || e3 != hipSuccess)
@@ -679,6 +743,26 @@ int chain_episode(dmpm_ctx* c)
     return DMPM_OK;
 }
 
+// Grid of one step: node masses (G) and velocities ([G][3]), instances concatenated.
+template <typename Real>
+int get_grid(dmpm_ctx* c, int step, double* m, double* v)
+{
+    const Real* g = grid_of_step<Real>(c, step);
+    std::vector<double> t((size_t)kGridFields * c->G);
+    int rc = grid_download<Real>(c, g, kGridFields * c->G, t.data());
+    if (rc) return rc;
+    for (int i = 0; i < c->B; ++i) {
+        const long long n0 = c->nofs[i], Gi = c->nofs[i + 1] - n0;
+        const double* gi = t.data() + 4 * n0;
+        for (long long q = 0; q < Gi; ++q) {
+            if (m) m[n0 + q] = gi[q];
+            if (v)
+                for (int a = 0; a < 3; ++a) v[3 * (n0 + q) + a] = gi[(1 + a) * Gi + q];
+        }
+    }
+    return DMPM_OK;
+}
+
 #define DISPATCH(ctx, fn, ...) ((ctx)->f32 ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
 
 int check_ctx(dmpm_ctx* c)
@@ -711,14 +795,22 @@ int dmpm_config_init(dmpm_config* cfg)
     return DMPM_OK;
 }
 
-int dmpm_create(const dmpm_config* cfg, dmpm_ctx** out)
+}  // extern "C"
+
+namespace {
+
+int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts, dmpm_ctx** out)
 {
-    if (!cfg || !out) return fail(DMPM_ERR_INVALID, "null argument");
     *out = nullptr;
     if (cfg->precision != DMPM_PREC_F64 && cfg->precision != DMPM_PREC_F32) return fail(DMPM_ERR_INVALID, "precision");
-    if (cfg->n_particles < 0) return fail(DMPM_ERR_INVALID, "n_particles < 0");
-    if (cfg->dim[0] <= 0 || cfg->dim[1] <= 0 || cfg->dim[2] <= 0) return fail(DMPM_ERR_INVALID, "grid dims must be > 0");
-    if (!(cfg->dx > 0.0)) return fail(DMPM_ERR_INVALID, "dx must be > 0");
+    long long ntot = 0;
+    for (const dmpm_instance& in : insts) {
+        if (in.n_particles < 0) return fail(DMPM_ERR_INVALID, "n_particles < 0");
+        if (in.dim[0] <= 0 || in.dim[1] <= 0 || in.dim[2] <= 0) return fail(DMPM_ERR_INVALID, "grid dims must be > 0");
+        if (!(in.dx > 0.0)) return fail(DMPM_ERR_INVALID, "dx must be > 0");
+        ntot += in.n_particles;
+    }
+    if (ntot > INT32_MAX - 64) return fail(DMPM_ERR_INVALID, "too many particles");
     if (cfg->num_layers < 2) return fail(DMPM_ERR_INVALID, "num_layers must be >= 2");
     if (cfg->tape_mode != DMPM_TAPE_STORE_ALL && cfg->tape_mode != DMPM_TAPE_CHECKPOINT) return fail(DMPM_ERR_INVALID, "tape_mode");
     if (cfg->compat < 0 || cfg->compat > DMPM_COMPAT_REFERENCE) return fail(DMPM_ERR_INVALID, "compat bits");
@@ -731,11 +823,23 @@ int dmpm_create(const dmpm_config* cfg, dmpm_ctx** out)
     c->cfg = *cfg;
     c->f32 = cfg->precision == DMPM_PREC_F32;
     c->esz = c->f32 ? 4 : 8;
-    c->N = cfg->n_particles;
+    c->B = (int)insts.size();
+    c->inst = insts;
+    c->poff.assign(c->B + 1, 0);
+    c->nofs.assign(c->B + 1, 0);
+    for (int i = 0; i < c->B; ++i) {
+        const dmpm_instance& in = insts[i];
+        const long long Gi = (long long)in.dim[0] * in.dim[1] * in.dim[2];
+        c->poff[i + 1] = c->poff[i] + in.n_particles;
+        c->nofs[i + 1] = c->nofs[i] + Gi;
+        c->gmax = std::max(c->gmax, Gi);
+    }
+    c->N = c->poff[c->B];
+    c->cfg.n_particles = c->N;
     c->S = ((c->N + 63) / 64) * 64;
     if (c->S == 0) c->S = 64;
     c->L = cfg->num_layers;
-    c->G = (long long)cfg->dim[0] * cfg->dim[1] * cfg->dim[2];
+    c->G = c->nofs[c->B];
     const int steps = c->L - 1;
     if (cfg->tape_mode == DMPM_TAPE_CHECKPOINT) {
         int K = cfg->ckpt_stride > 0 ? cfg->ckpt_stride : (int)std::lround(std::sqrt((double)steps));
@@ -768,7 +872,9 @@ int dmpm_create(const dmpm_config* cfg, dmpm_ctx** out)
         (rc = dalloc(c, &c->agrid, kGridFields * G * e)) || (rc = dalloc(c, &c->gbuf[0], kLayerFields * S * e)) ||
         (rc = dalloc(c, &c->gbuf[1], kLayerFields * S * e)) || (rc = dalloc(c, &c->gctrl, gctrl_layers * 9 * S * e)) ||
         (rc = dalloc(c, &c->tmp, 12 * S * e)) || (rc = dalloc(c, &c->target, G * e)) ||
-        (rc = dalloc(c, (void**)&c->dacc, 64 * sizeof(double)))) {
+        (rc = dalloc(c, (void**)&c->dacc, std::max(64, c->B) * sizeof(double))) ||
+        (rc = dalloc(c, (void**)&c->d_sps, c->B * sizeof(StepParams))) ||
+        (rc = dalloc(c, (void**)&c->d_nofs, (c->B + 1) * sizeof(long long)))) {
         std::string msg = g_err;
         dmpm_destroy(c);
         return fail(rc, msg);
@@ -783,7 +889,21 @@ int dmpm_create(const dmpm_config* cfg, dmpm_ctx** out)
         launch_fill<double>(static_cast<double*>(c->layers), (long long)kLayerFields * S, 0.0, c->stream);
         launch_fill<double>(static_cast<double*>(c->cst), (long long)kConstFields * S, 0.0, c->stream);
     }
-    he = hipStreamSynchronize(c->stream);
+    {
+        std::vector<StepParams> sps(c->B);
+        for (int i = 0; i < c->B; ++i) sps[i] = inst_params(c, i);
+        hipError_t e1 = hipMemcpyAsync(c->d_sps, sps.data(), sps.size() * sizeof(StepParams), hipMemcpyHostToDevice, c->stream);
+        hipError_t e2 = hipMemcpyAsync(c->d_nofs, c->nofs.data(), c->nofs.size() * sizeof(long long), hipMemcpyHostToDevice, c->stream);
+        he = hipStreamSynchronize(c->stream);
+        if (he == hipSuccess) he = e1 != hipSuccess ? e1 : e2;
+    }
+    c->batch.sps = c->d_sps;
+    c->batch.nofs = c->d_nofs;
+    c->batch.ranges = nullptr;
+    c->batch.nranges = 0;
+    c->batch.B = c->B;
+    c->batch.Gtot = c->G;
+    c->batch.cap = c->lds_cap;
     if (he != hipSuccess) {
         dmpm_destroy(c);
         return fail(DMPM_ERR_HIP, std::string("init: ") + hipGetErrorString(he));
@@ -792,12 +912,41 @@ int dmpm_create(const dmpm_config* cfg, dmpm_ctx** out)
     return DMPM_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int dmpm_create(const dmpm_config* cfg, dmpm_ctx** out)
+{
+    if (!cfg || !out) return fail(DMPM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (cfg->n_particles < 0) return fail(DMPM_ERR_INVALID, "n_particles < 0");
+    dmpm_instance in{};
+    in.n_particles = cfg->n_particles;
+    for (int a = 0; a < 3; ++a) {
+        in.dim[a] = cfg->dim[a];
+        in.min_point[a] = cfg->min_point[a];
+    }
+    in.dx = cfg->dx;
+    return create_impl(cfg, {in}, out);
+}
+
+int dmpm_create_batch(const dmpm_config* cfg, int32_t n_instances, const dmpm_instance* instances, dmpm_ctx** out)
+{
+    if (!cfg || !out || !instances) return fail(DMPM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (n_instances < 1) return fail(DMPM_ERR_INVALID, "n_instances must be >= 1");
+    if (cfg->precision != DMPM_PREC_F32) return fail(DMPM_ERR_INVALID, "batched contexts run the FP32 path (precision DMPM_PREC_F32)");
+    return create_impl(cfg, std::vector<dmpm_instance>(instances, instances + n_instances), out);
+}
+
 int dmpm_destroy(dmpm_ctx* c)
 {
     if (!c) return DMPM_OK;
     hipSetDevice(c->cfg.device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    void* bufs[] = {c->cst, c->ctrl, c->layers, c->grids, c->wgrid, c->agrid, c->gbuf[0], c->gbuf[1], c->gctrl, c->tmp, c->target, c->dacc, c->ranges};
+    void* bufs[] = {c->cst,  c->ctrl, c->layers, c->grids, c->wgrid, c->agrid, c->gbuf[0], c->gbuf[1],
+                    c->gctrl, c->tmp, c->target, c->dacc,  c->ranges, c->d_sps, c->d_nofs};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t ev : c->pool) hipEventDestroy(ev);
@@ -875,7 +1024,32 @@ int dmpm_target_from_points(dmpm_ctx* c, const double* x, const double* m, int32
 {
     int rc = check_ctx(c);
     if (rc) return rc;
-    return DISPATCH(c, target_from_points, c, x, m, n);
+    if (c->B != 1) return fail(DMPM_ERR_STATE, "batched context: use dmpm_target_from_points_batch");
+    const int counts[1] = {n};
+    return DISPATCH(c, target_from_points, c, x, m, counts);
+}
+
+int dmpm_target_from_points_batch(dmpm_ctx* c, const double* x, const double* m, const int32_t* counts)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!counts) return fail(DMPM_ERR_INVALID, "null counts");
+    return DISPATCH(c, target_from_points, c, x, m, counts);
+}
+
+int dmpm_num_instances(dmpm_ctx* c, int32_t* n)
+{
+    if (!c || !n) return fail(DMPM_ERR_INVALID, "null argument");
+    *n = c->B;
+    return DMPM_OK;
+}
+
+int dmpm_get_instance_losses(dmpm_ctx* c, double* losses)
+{
+    if (!c || !losses) return fail(DMPM_ERR_INVALID, "null argument");
+    if ((int)c->inst_loss.size() != c->B) return fail(DMPM_ERR_STATE, "no loss has been evaluated");
+    for (int i = 0; i < c->B; ++i) losses[i] = c->inst_loss[i];
+    return DMPM_OK;
 }
 
 int dmpm_get_target_grid(dmpm_ctx* c, double* mass)
@@ -1000,26 +1174,7 @@ int dmpm_get_grid(dmpm_ctx* c, int32_t step, double* m, double* v)
     if (rc) return rc;
     if (c->cfg.tape_mode != DMPM_TAPE_STORE_ALL) return fail(DMPM_ERR_STATE, "grids are kept by the store-all tape only");
     if (step < 0 || step > c->L - 2 || step + 1 > c->fwd_valid) return fail(DMPM_ERR_STATE, "grid of this step not computed");
-    if (c->f32) {
-        float* g = grid_of_step<float>(c, step);
-        if (m && (rc = grid_download<float>(c, g, c->G, m))) return rc;
-        if (v) {
-            std::vector<double> t((size_t)3 * c->G);
-            if ((rc = grid_download<float>(c, g + c->G, 3 * c->G, t.data()))) return rc;
-            for (long long i = 0; i < c->G; ++i)
-                for (int a = 0; a < 3; ++a) v[3 * i + a] = t[(size_t)a * c->G + i];
-        }
-    } else {
-        double* g = grid_of_step<double>(c, step);
-        if (m && (rc = grid_download<double>(c, g, c->G, m))) return rc;
-        if (v) {
-            std::vector<double> t((size_t)3 * c->G);
-            if ((rc = grid_download<double>(c, g + c->G, 3 * c->G, t.data()))) return rc;
-            for (long long i = 0; i < c->G; ++i)
-                for (int a = 0; a < 3; ++a) v[3 * i + a] = t[(size_t)a * c->G + i];
-        }
-    }
-    return DMPM_OK;
+    return DISPATCH(c, get_grid, c, step, m, v);
 }
 
 int dmpm_chain_episode(dmpm_ctx* c)

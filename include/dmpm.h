@@ -86,6 +86,28 @@ int dmpm_config_init(dmpm_config* cfg);
 int dmpm_create(const dmpm_config* cfg, dmpm_ctx** out);
 int dmpm_destroy(dmpm_ctx* ctx);
 
+/* One morph of a batched context: its particle count and grid (Grid.h:22-31, Grid.cpp:22). */
+typedef struct dmpm_instance {
+    int32_t n_particles;
+    int32_t dim[3];
+    double dx;
+    double min_point[3];
+} dmpm_instance;
+
+/* Batched context: n_instances independent morphs advanced together, every launch covering all
+ * of them (the reference runs one CompGraph per morph, DiffMPMAnimator3D.cpp:421-471).  FP32 only.
+ * cfg gives precision, num_layers, dt, drag, smoothing, f_ext, compat, tape and device; its
+ * n_particles, dim, dx and min_point are ignored.  Every particle array of the other calls holds
+ * the instances' particles concatenated in instance order; grid arrays (target, dmpm_get_grid)
+ * hold the instances' grids concatenated.  dmpm_end_layer_mass_loss returns the sum of the
+ * instance losses (instances are independent, so each instance's gradients are its own). */
+int dmpm_create_batch(const dmpm_config* cfg, int32_t n_instances, const dmpm_instance* instances, dmpm_ctx** out);
+int dmpm_num_instances(dmpm_ctx* ctx, int32_t* n);
+/* EndLayerMassLoss of every instance from the last dmpm_end_layer_mass_loss. */
+int dmpm_get_instance_losses(dmpm_ctx* ctx, double* losses);
+/* dmpm_target_from_points per instance: point clouds concatenated in instance order, counts[i] each. */
+int dmpm_target_from_points_batch(dmpm_ctx* ctx, const double* x, const double* m, const int32_t* counts);
+
 /* Layer 0 particles: MaterialPoint x, v, F, C, m, vol, lam, mu (MaterialPoint.h:18-28).
  * vol may be NULL (then call dmpm_compute_volumes).  Invalidates every later layer. */
 int dmpm_set_particles(dmpm_ctx* ctx, const double* x, const double* v, const double* F, const double* C,
