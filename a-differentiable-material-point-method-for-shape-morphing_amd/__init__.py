@@ -32,6 +32,10 @@ COMPAT_NO_DRAG_X = 4
 COMPAT_DLDM_NO_FEXT = 8
 COMPAT_REFERENCE = 15
 
+OPT_GRAPHS = 1
+OPT_RANGE_MAX = 2
+OPT_LDS_CAP = 3
+
 K_P2G, K_GRID_UPDATE, K_G2P, K_G2P_ADJ, K_GRID_ADJ, K_P2G_ADJ, K_LOSS, K_FILL = range(8)
 KERNEL_NAMES = ["p2g", "grid_update", "g2p", "g2p_adj", "grid_adj", "p2g_adj", "loss", "fill"]
 
@@ -46,7 +50,7 @@ EXPORTS = [
     "dmpm_control_grad_norm", "dmpm_get_grid", "dmpm_chain_episode", "dmpm_synchronize", "dmpm_memory_bytes",
     "dmpm_profile_enable", "dmpm_profile_reset", "dmpm_profile_read", "dmpm_eval_pk1", "dmpm_eval_hessian",
     "dmpm_eval_svd", "dmpm_create_batch", "dmpm_num_instances", "dmpm_get_instance_losses",
-    "dmpm_target_from_points_batch",
+    "dmpm_target_from_points_batch", "dmpm_set_option", "dmpm_get_option", "dmpm_autotune",
 ]
 
 
@@ -138,6 +142,9 @@ def load_library(path: str = LIB_PATH):
         "dmpm_num_instances": ([vp, ctypes.POINTER(i32)], ctypes.c_int),
         "dmpm_get_instance_losses": ([vp, _DP], ctypes.c_int),
         "dmpm_target_from_points_batch": ([vp, _DP, _DP, ctypes.POINTER(i32)], ctypes.c_int),
+        "dmpm_set_option": ([vp, i32, ctypes.c_int64], ctypes.c_int),
+        "dmpm_get_option": ([vp, i32, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "dmpm_autotune": ([vp, _DP], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -376,6 +383,26 @@ class Morph:
         b = ctypes.c_uint64()
         _check(self._L.dmpm_memory_bytes(self._h, ctypes.byref(b)))
         return int(b.value)
+
+    # -- execution options ----------------------------------------------
+    def set_option(self, option, value):
+        _check(self._L.dmpm_set_option(self._h, int(option), int(value)))
+
+    def option(self, option):
+        v = ctypes.c_int64()
+        _check(self._L.dmpm_get_option(self._h, int(option), ctypes.byref(v)))
+        return int(v.value)
+
+    def set_graphs(self, on=True):
+        """Capture each pass into a HIP graph on first use and replay it afterwards."""
+        self.set_option(OPT_GRAPHS, int(bool(on)))
+
+    def autotune(self):
+        """FP32: pick the fastest range size / LDS capacity; returns its step time (ms).
+        Invalidates the layers after 0."""
+        ms = ctypes.c_double()
+        _check(self._L.dmpm_autotune(self._h, ctypes.byref(ms)))
+        return ms.value
 
     # -- profiling ------------------------------------------------------
     def profile(self, on=True):

@@ -45,6 +45,11 @@ struct ProfRec {
     hipEvent_t a, b;
 };
 
+struct GraphEntry {
+    long long key;
+    hipGraphExec_t exec;
+};
+
 }  // namespace
 
 struct dmpm_ctx {
@@ -78,10 +83,16 @@ struct dmpm_ctx {
     // FP32 path: device order of the particles (sorted by 4^3-cell block per
     // instance); pos[p] is the device slot of API particle p (empty = identity)
     std::vector<int> pos;
-    int4* ranges = nullptr;   // per-workgroup {begin, end, instance} of the block kernels
+    std::vector<int4> host_blocks;  // {begin, end, instance} of every non-empty 4^3-cell block
+    int4* ranges = nullptr;         // per-workgroup {begin, end, instance}: blocks split to <= range_max
     std::vector<int4> host_ranges;
     int n_ranges = 0;
-    int lds_cap = 1024;       // LDS node window of the block kernels
+    int range_max = 512;            // particles per workgroup range (64..512)
+    int lds_cap = 1024;             // general-window LDS capacity of the block kernels (nodes)
+    // captured passes, replayed while graphs are on (kernel arguments are
+    // fixed per context; the cache is dropped when ranges or options change)
+    bool use_graphs = false;
+    std::vector<GraphEntry> graphs;
     StepParams* d_sps = nullptr;
     long long* d_nofs = nullptr;
     Batch batch{};
@@ -163,6 +174,44 @@ int drain_profile(dmpm_ctx* c)
 }
 
 // Step parameters of instance i (particle count, grid, time step).
+void clear_graphs(dmpm_ctx* c)
+{
+    for (auto& g : c->graphs) hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+}
+
+// Run the launches of body() on the context's stream.  With graphs on (and
+// profiling off) the first run of a key is captured into a HIP graph and
+// every run replays it; body's host-side effects happen only when captured,
+// so callers set host state themselves after the call.
+template <typename Fn>
+int run_graph(dmpm_ctx* c, long long key, Fn&& body)
+{
+    if (!c->use_graphs || c->prof) {
+        body();
+        return DMPM_OK;
+    }
+    for (auto& g : c->graphs)
+        if (g.key == key) {
+            HIP_TRY(hipGraphLaunch(g.exec, c->stream));
+            return DMPM_OK;
+        }
+    constexpr size_t kMaxGraphs = 64;
+    if (c->graphs.size() >= kMaxGraphs) clear_graphs(c);
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    body();
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (e != hipSuccess) return fail(DMPM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(DMPM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    c->graphs.push_back({key, exec});
+    HIP_TRY(hipGraphLaunch(exec, c->stream));
+    return DMPM_OK;
+}
+
 StepParams inst_params(const dmpm_ctx* c, int i)
 {
     const dmpm_instance& in = c->inst[i];
@@ -332,13 +381,18 @@ int forward_pass(dmpm_ctx* c, int start)
     if (!c->have_particles) return fail(DMPM_ERR_STATE, "forward pass before dmpm_set_particles");
     if (start < 0 || start >= c->L - 1) return fail(DMPM_ERR_INVALID, "start_layer out of range");
     int from = std::min(start, c->fwd_valid);
+    int rc;
     if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) {
-        for (int i = from; i < c->L - 1; ++i)
-            step_forward<Real>(c, slot<Real>(c, i), slot<Real>(c, i + 1), ctrl_of<Real>(c, i), grid_of_step<Real>(c, i));
+        rc = run_graph(c, from, [&] {
+            for (int i = from; i < c->L - 1; ++i)
+                step_forward<Real>(c, slot<Real>(c, i), slot<Real>(c, i + 1), ctrl_of<Real>(c, i), grid_of_step<Real>(c, i));
+        });
+        if (rc) return rc;
     } else {
         from = (from / c->K) * c->K;  // restart from a checkpoint so the tape holds whole segments
         const int s0_last = c->seg_last * c->K;
         Real* grids = static_cast<Real*>(c->grids);
+        rc = run_graph(c, from, [&] {
         for (int i = from; i < c->L - 1; ++i) {
             const Real* cur;
             if (i % c->K == 0) cur = slot<Real>(c, i / c->K);
@@ -352,6 +406,8 @@ int forward_pass(dmpm_ctx* c, int start)
             Real* g = (i >= s0_last) ? grids + (size_t)(i - s0_last) * kGridFields * c->G : static_cast<Real*>(c->wgrid);
             step_forward<Real>(c, cur, dst, ctrl_of<Real>(c, i), g);
         }
+        });
+        if (rc) return rc;
         c->tape_seg = c->seg_last;
     }
     HIP_TRY(hipGetLastError());
@@ -414,20 +470,30 @@ int backward_pass(dmpm_ctx* c, int control)
 {
     if (!c->loss_valid) return fail(DMPM_ERR_STATE, "backward pass before EndLayerMassLoss");
     if (control < 0 || control > c->L - 2) return fail(DMPM_ERR_INVALID, "control_layer out of range");
+    // graph key: backward flag, control layer, tape segment held at the start
+    const long long key = (1LL << 40) | ((long long)control << 20) | (long long)(c->tape_seg + 1);
+    int rc;
     if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) {
-        for (int i = c->L - 2; i >= control; --i)
-            step_backward<Real>(c, i, slot<Real>(c, i), slot<Real>(c, i + 1), grid_of_step<Real>(c, i));
+        rc = run_graph(c, key, [&] {
+            for (int i = c->L - 2; i >= control; --i)
+                step_backward<Real>(c, i, slot<Real>(c, i), slot<Real>(c, i + 1), grid_of_step<Real>(c, i));
+        });
+        if (rc) return rc;
     } else {
         Real* grids = static_cast<Real*>(c->grids);
-        for (int seg = c->seg_last; seg >= control / c->K; --seg) {
-            const int s0 = seg * c->K, s1 = std::min(s0 + c->K, c->L - 1);
-            if (c->tape_seg != seg) recompute_segment<Real>(c, seg);
-            for (int i = s1 - 1; i >= std::max(s0, control); --i) {
-                const Real* cur = tape_layer<Real>(c, i);
-                const Real* nxt = tape_layer<Real>(c, i + 1);
-                step_backward<Real>(c, i, cur, nxt, grids + (size_t)(i - s0) * kGridFields * c->G);
+        rc = run_graph(c, key, [&] {
+            for (int seg = c->seg_last; seg >= control / c->K; --seg) {
+                const int s0 = seg * c->K, s1 = std::min(s0 + c->K, c->L - 1);
+                if (c->tape_seg != seg) recompute_segment<Real>(c, seg);
+                for (int i = s1 - 1; i >= std::max(s0, control); --i) {
+                    const Real* cur = tape_layer<Real>(c, i);
+                    const Real* nxt = tape_layer<Real>(c, i + 1);
+                    step_backward<Real>(c, i, cur, nxt, grids + (size_t)(i - s0) * kGridFields * c->G);
+                }
             }
-        }
+        });
+        if (rc) return rc;
+        c->tape_seg = control / c->K;  // the pass ends holding the control layer's segment
     }
     HIP_TRY(hipGetLastError());
     c->grads_lo = control;
@@ -475,10 +541,7 @@ int download_fields(dmpm_ctx* c, const Real* src, int field0, int nfield, double
 void sort_particles(dmpm_ctx* c, const double* x)
 {
     c->pos.assign(c->N, 0);
-    c->host_ranges.clear();
-    // one workgroup range per block, at most 512 particles (the scatter
-    // kernels hold one particle per thread of a 512-thread workgroup)
-    constexpr int kMaxRange = 512;
+    c->host_blocks.clear();
     for (int in = 0; in < c->B; ++in) {
         const dmpm_instance& g = c->inst[in];
         const int p0 = c->poff[in], N = g.n_particles;
@@ -513,12 +576,40 @@ void sort_particles(dmpm_ctx* c, const double* x)
         for (int d = 0; d < N; ++d) c->pos[p0 + order[d]] = p0 + d;
         int d0 = 0;
         for (int d = 1; d <= N; ++d) {
-            if (d == N || (key[order[d]] >> 22) != (key[order[d0]] >> 22) || d - d0 == kMaxRange) {
-                c->host_ranges.push_back(make_int4(p0 + d0, p0 + d, in, 0));
+            if (d == N || (key[order[d]] >> 22) != (key[order[d0]] >> 22)) {
+                c->host_blocks.push_back(make_int4(p0 + d0, p0 + d, in, 0));
                 d0 = d;
             }
         }
     }
+}
+
+// Workgroup ranges: every block split into equal parts of at most range_max
+// particles (the scatter kernels hold one particle per thread of a
+// 512-thread workgroup, so range_max <= 512).
+int build_ranges(dmpm_ctx* c)
+{
+    std::vector<int4> rg;
+    for (const int4& b : c->host_blocks) {
+        const int len = b.y - b.x;
+        const int parts = (len + c->range_max - 1) / c->range_max;
+        for (int k = 0; k < parts; ++k)
+            rg.push_back(make_int4(b.x + (int)((long long)len * k / parts), b.x + (int)((long long)len * (k + 1) / parts), b.z, 0));
+    }
+    clear_graphs(c);
+    if (c->ranges) {
+        hipFree(c->ranges);
+        c->ranges = nullptr;
+    }
+    c->host_ranges.swap(rg);
+    c->n_ranges = (int)c->host_ranges.size();
+    if (c->n_ranges > 0) {
+        HIP_TRY(hipMalloc(&c->ranges, c->host_ranges.size() * sizeof(int4)));
+        HIP_TRY(hipMemcpy(c->ranges, c->host_ranges.data(), c->host_ranges.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
+    c->batch.ranges = c->ranges;
+    c->batch.nranges = c->n_ranges;
+    return DMPM_OK;
 }
 
 template <typename Real>
@@ -528,17 +619,8 @@ int set_particles(dmpm_ctx* c, const double* x, const double* v, const double* F
     if (!x || !v || !F || !C || !m || !lam || !mu) return fail(DMPM_ERR_INVALID, "null particle array");
     if (c->f32) {
         sort_particles(c, x);
-        if (c->ranges) {
-            hipFree(c->ranges);
-            c->ranges = nullptr;
-        }
-        c->n_ranges = (int)c->host_ranges.size();
-        if (c->n_ranges > 0) {
-            HIP_TRY(hipMalloc(&c->ranges, c->host_ranges.size() * sizeof(int4)));
-            HIP_TRY(hipMemcpy(c->ranges, c->host_ranges.data(), c->host_ranges.size() * sizeof(int4), hipMemcpyHostToDevice));
-        }
-        c->batch.ranges = c->ranges;
-        c->batch.nranges = c->n_ranges;
+        const int rc0 = build_ranges(c);
+        if (rc0) return rc0;
     }
     // a new particle set starts with zero controls (MaterialPoint.h:31)
     fill<Real>(c, static_cast<Real*>(c->ctrl), (long long)(c->L - 1) * 9 * c->S, Real(0));
@@ -945,6 +1027,7 @@ int dmpm_destroy(dmpm_ctx* c)
     if (!c) return DMPM_OK;
     hipSetDevice(c->cfg.device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    clear_graphs(c);
     void* bufs[] = {c->cst,  c->ctrl, c->layers, c->grids, c->wgrid, c->agrid, c->gbuf[0], c->gbuf[1],
                     c->gctrl, c->tmp, c->target, c->dacc,  c->ranges, c->d_sps, c->d_nofs};
     for (void* b : bufs)
@@ -1182,6 +1265,112 @@ int dmpm_chain_episode(dmpm_ctx* c)
     int rc = check_ctx(c);
     if (rc) return rc;
     return DISPATCH(c, chain_episode, c);
+}
+
+int dmpm_set_option(dmpm_ctx* c, int32_t option, int64_t value)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    switch (option) {
+    case DMPM_OPT_GRAPHS:
+        if (value != 0 && value != 1) return fail(DMPM_ERR_INVALID, "DMPM_OPT_GRAPHS is 0 or 1");
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->use_graphs = value != 0;
+        clear_graphs(c);
+        return DMPM_OK;
+    case DMPM_OPT_RANGE_MAX:
+        if (value < 64 || value > 512) return fail(DMPM_ERR_INVALID, "DMPM_OPT_RANGE_MAX must be in [64, 512]");
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->range_max = (int)value;
+        return c->f32 && !c->host_blocks.empty() ? build_ranges(c) : DMPM_OK;
+    case DMPM_OPT_LDS_CAP:
+        if (value < 512 || value > 4096) return fail(DMPM_ERR_INVALID, "DMPM_OPT_LDS_CAP must be in [512, 4096]");
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->lds_cap = (int)value;
+        c->batch.cap = c->lds_cap;
+        clear_graphs(c);
+        return DMPM_OK;
+    default:
+        return fail(DMPM_ERR_INVALID, "unknown option");
+    }
+}
+
+int dmpm_get_option(dmpm_ctx* c, int32_t option, int64_t* value)
+{
+    if (!c || !value) return fail(DMPM_ERR_INVALID, "null argument");
+    switch (option) {
+    case DMPM_OPT_GRAPHS: *value = c->use_graphs ? 1 : 0; return DMPM_OK;
+    case DMPM_OPT_RANGE_MAX: *value = c->range_max; return DMPM_OK;
+    case DMPM_OPT_LDS_CAP: *value = c->lds_cap; return DMPM_OK;
+    default: return fail(DMPM_ERR_INVALID, "unknown option");
+    }
+}
+
+int dmpm_autotune(dmpm_ctx* c, double* best_ms)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->f32) return fail(DMPM_ERR_INVALID, "autotune tunes the FP32 block kernels");
+    if (!c->have_particles) return fail(DMPM_ERR_STATE, "autotune before dmpm_set_particles");
+    const bool prof = c->prof;
+    c->prof = false;
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    float* l0 = slot<float>(c, 0);
+    float* l1 = slot<float>(c, 1);
+    float* g = static_cast<float*>(c->grids);
+    const int ranges[] = {128, 256, 512};
+    const int caps[] = {512, 1024, 2048};
+    double best = 1e30;
+    int best_r = c->range_max, best_c = c->lds_cap;
+    for (int cap : caps) {
+        c->lds_cap = cap;
+        c->batch.cap = cap;
+        for (int r : ranges) {
+            c->range_max = r;
+            if ((rc = build_ranges(c))) break;
+            float ms_min = 1e30f;
+            for (int rep = 0; rep < 3; ++rep) {  // first repetition warms up
+                launch_fill<float>(grads_of<float>(c, 1), (long long)kLayerFields * c->S, 0.f, c->stream);
+                hipEventRecord(a, c->stream);
+                step_forward<float>(c, l0, l1, ctrl_of<float>(c, 0), g);
+                step_backward<float>(c, 0, l0, l1, g);
+                hipEventRecord(b, c->stream);
+                if (hipEventSynchronize(b) != hipSuccess) {
+                    rc = fail(DMPM_ERR_HIP, "autotune step failed");
+                    break;
+                }
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, a, b);
+                if (rep > 0) ms_min = std::min(ms_min, ms);
+            }
+            if (rc) break;
+            if (ms_min < best) {
+                best = ms_min;
+                best_r = r;
+                best_c = cap;
+            }
+        }
+        if (rc) break;
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    c->prof = prof;
+    c->lds_cap = best_c;
+    c->batch.cap = best_c;
+    c->range_max = best_r;
+    const int rc2 = build_ranges(c);
+    c->fwd_valid = 0;
+    c->loss_valid = false;
+    c->grads_lo = -1;
+    c->bwd_control = -1;
+    c->tape_seg = -1;
+    if (rc) return rc;
+    if (rc2) return rc2;
+    HIP_TRY(hipGetLastError());
+    if (best_ms) *best_ms = best;
+    return DMPM_OK;
 }
 
 int dmpm_synchronize(dmpm_ctx* c)

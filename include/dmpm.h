@@ -163,6 +163,17 @@ int dmpm_get_grid(dmpm_ctx* ctx, int32_t step, double* m, double* v);
 /* Episode chaining of DiffMPMAnimator3D.cpp:424-429: layer 0 <- last layer, controls <- 0. */
 int dmpm_chain_episode(dmpm_ctx* ctx);
 
+/* Execution options (no reference counterpart: launch configuration of this library). */
+#define DMPM_OPT_GRAPHS 1    /* 1: capture each pass into a HIP graph on first use, then replay it (default 0) */
+#define DMPM_OPT_RANGE_MAX 2 /* FP32: most particles per workgroup range, 64..512 (default 512) */
+#define DMPM_OPT_LDS_CAP 3   /* FP32: node capacity of the general LDS window, 512..4096 (default 1024) */
+int dmpm_set_option(dmpm_ctx* ctx, int32_t option, int64_t value);
+int dmpm_get_option(dmpm_ctx* ctx, int32_t option, int64_t* value);
+/* FP32: time a forward and a backward step of layer 0 for every candidate range size and LDS
+ * capacity, keep the fastest (best_ms: its time).  Needs particles; the forward trajectory
+ * (layers after 0) and gradients are invalidated. */
+int dmpm_autotune(dmpm_ctx* ctx, double* best_ms);
+
 int dmpm_synchronize(dmpm_ctx* ctx);
 /* Device bytes held by the context. */
 int dmpm_memory_bytes(dmpm_ctx* ctx, uint64_t* bytes);
