@@ -9,9 +9,15 @@ gradient evaluation of an episode: ComputeForwardPass(0) + EndLayerMassLoss +
 ComputeBackwardPass(0).  particle-steps/s = particles x 100 x steps / time.
 
 Timed paths: FP32 state with FP64 polar/SVD/H:dF and checkpoint+recompute
-(the reported value) and the plain FP64 path with every layer stored; the
-control-gradient error is FP32 vs FP64 on the same inputs.  The CPU baseline is
-the FP64 oracle (a port) on a bounded sample of the same scene.
+(the reported value: autotuned range size / LDS capacity, passes replayed
+from HIP graphs), the FP32 path storing every layer (memory and speed of the
+two tapes), and the plain FP64 path with every layer stored; the
+control-gradient error is FP32 vs FP64 on the same inputs.  Per-kernel
+durations come from HIP events around every launch of eager episodes run
+right after the timed region (graph replays cannot be bracketed per kernel;
+the kernels are the same).  A batched run advances 24 uneven morphs
+(5k-300k particles, 32^3-96^3 grids, 50-step episodes) together.  The CPU
+baseline is the FP64 oracle (a port) on a bounded sample of the same scene.
 
 Stability: the scene's own dt (2e-4) with the reference's dimensionless
 particle volume exceeds the explicit CFL limit many times over; the bench uses
@@ -29,6 +35,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from __graft_entry__ import _load_pkg  # noqa: E402
+
+OPT_RANGE_MAX, OPT_LDS_CAP = 2, 3
 
 # Algorithmic HBM bytes per FP32 particle / per touched node / per dense grid node
 # of each kernel (FP64: twice).  Particle fields: x3 v3 C9 F9 dFc9 + m vol lam mu.
@@ -70,7 +78,7 @@ def make_scene(pkg, name, dt):
 
 
 def setup(pkg, sc, vol, dFc, precision, tape, num_layers):
-    mo = pkg.Morph.from_scene(sc, precision=precision, tape=tape, num_layers=num_layers)
+    mo = pkg.Morph.from_scene(sc, precision=precision, tape=tape, num_layers=num_layers, dt=sc.dt)
     x, v, C, F = sc.state0()
     mo.set_particles(x, v, F, C, sc.m, vol, sc.lam, sc.mu)
     mo.set_control(0, dFc)
@@ -85,14 +93,14 @@ def episode(mo):
     return loss
 
 
-def run_path(pkg, sc, vol, dFc, precision, tape, L, warmup, steps, profile):
-    mo = setup(pkg, sc, vol, dFc, precision, tape, L)
+def timed_episodes(mo, warmup, steps, prof_steps=0, tune=False, graphs=False):
+    """Warm up, time ``steps`` episodes, then (optionally) profile ``prof_steps`` eager episodes."""
+    tuned = None
+    if tune:
+        tuned = {"step_ms": mo.autotune(), "range_max": mo.option(OPT_RANGE_MAX), "lds_cap": mo.option(OPT_LDS_CAP)}
+    mo.set_graphs(graphs)
     for _ in range(warmup):
         episode(mo)
-    mo.synchronize()
-    if profile:
-        mo.profile(True)
-        mo.profile_reset()
     mo.synchronize()
     t0 = time.perf_counter()
     loss = None
@@ -100,13 +108,48 @@ def run_path(pkg, sc, vol, dFc, precision, tape, L, warmup, steps, profile):
         loss = episode(mo)
     mo.synchronize()
     dt = time.perf_counter() - t0
-    prof = mo.profile_read() if profile else None
-    if profile:
+    prof = None
+    if prof_steps:
+        mo.profile(True)
+        mo.profile_reset()
+        for _ in range(prof_steps):
+            episode(mo)
+        prof = mo.profile_read()
         mo.profile(False)
-    grad = mo.control_grad(0)
-    mem = mo.memory_bytes()
+    return dict(time=dt, loss=loss, prof=prof, tuned=tuned)
+
+
+def run_path(pkg, sc, vol, dFc, precision, tape, L, warmup, steps, prof_steps=0, tune=False, graphs=False):
+    mo = setup(pkg, sc, vol, dFc, precision, tape, L)
+    r = timed_episodes(mo, warmup, steps, prof_steps, tune, graphs)
+    r["grad"] = mo.control_grad(0)
+    r["mem"] = mo.memory_bytes()
     mo.close()
-    return dict(time=dt, loss=loss, grad=grad, prof=prof, mem=mem)
+    return r
+
+
+def run_batched(pkg, count, L, dt, warmup, steps):
+    """Uneven instances advanced together: one FP32 checkpointed context, graphs, autotuned."""
+    scs = pkg.scenes.batched_instances(count, dt=dt)
+    mb = pkg.MorphBatch.from_scenes(scs, num_layers=L, dt=dt, tape=pkg.TAPE_CHECKPOINT)
+    st = [sc.state0() for sc in scs]
+    cat = lambda k: np.concatenate([t[k] for t in st])  # noqa: E731
+    m = np.concatenate([sc.m for sc in scs])
+    lam = np.concatenate([sc.lam for sc in scs])
+    mu = np.concatenate([sc.mu for sc in scs])
+    mb.set_particles(cat(0), cat(1), cat(3), cat(2), m, None, lam, mu)
+    mb.compute_volumes()
+    vol = np.concatenate([v * sc.dx ** 3 for v, sc in zip(mb.split(mb.volumes()), scs)])
+    mb.set_particles(cat(0), cat(1), cat(3), cat(2), m, vol, lam, mu)
+    mb.set_control(0, np.concatenate([pkg.scenes.random_controls(sc.n, 1e-2, 7 + i) for i, sc in enumerate(scs)]))
+    mb.target_from_points([sc.target_x for sc in scs], [sc.target_m for sc in scs])
+    r = timed_episodes(mb, warmup, steps, 0, tune=True, graphs=True)
+    n = mb.n
+    grids = sorted({sc.dim[0] for sc in scs})
+    mb.close()
+    return {"value": n * (L - 1) * steps / r["time"], "unit": "particle-steps/s", "instances": count,
+            "particles": n, "grids": grids, "ms_per_step": r["time"] / steps * 1e3, "episode_steps": L - 1,
+            "tuned": r["tuned"], "loss": r["loss"]}
 
 
 def cpu_baseline(sc, vol, dFc, threads, steps):
@@ -156,6 +199,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--prof-steps", type=int, default=2)
+    ap.add_argument("--store-all-steps", type=int, default=3)
+    ap.add_argument("--batched-steps", type=int, default=3)
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a D2D copy of this many bytes")
     args = ap.parse_args()
     if args.copy_probe:
@@ -180,27 +228,46 @@ def main():
     touched = touched_nodes(sc)
     print(f"# workload {sc.name}: n={sc.n} grid={sc.dim} L={L} touched_nodes={touched}", file=sys.stderr, flush=True)
 
-    r32 = run_path(pkg, sc, vol, dFc, pkg.PREC_F32, pkg.TAPE_CHECKPOINT, L, args.warmup, args.steps, True)
+    r32 = run_path(pkg, sc, vol, dFc, pkg.PREC_F32, pkg.TAPE_CHECKPOINT, L, args.warmup, args.steps, args.prof_steps,
+                   tune=not args.no_tune, graphs=not args.no_graphs)
     value = sc.n * args.episode_steps * args.steps / r32["time"]
-    print(f"# fp32+ckpt: {r32['time'] / args.steps * 1e3:.2f} ms/episode, {value:.4e} particle-steps/s", file=sys.stderr, flush=True)
+    print(f"# fp32+ckpt: {r32['time'] / args.steps * 1e3:.2f} ms/episode, {value:.4e} particle-steps/s "
+          f"(tuned {r32['tuned']})", file=sys.stderr, flush=True)
+
+    rsa = None
+    if args.store_all_steps:
+        rsa = run_path(pkg, sc, vol, dFc, pkg.PREC_F32, pkg.TAPE_STORE_ALL, L, 1, args.store_all_steps,
+                       tune=not args.no_tune, graphs=not args.no_graphs)
+        vsa = sc.n * args.episode_steps * args.store_all_steps / rsa["time"]
+        print(f"# fp32 store-all: {rsa['time'] / args.store_all_steps * 1e3:.2f} ms/episode, {vsa:.4e} particle-steps/s, "
+              f"{rsa['mem'] / 2**30:.2f} GiB vs ckpt {r32['mem'] / 2**30:.2f} GiB", file=sys.stderr, flush=True)
 
     r64 = None
     if not args.no_fp64:
-        r64 = run_path(pkg, sc, vol, dFc, pkg.PREC_F64, pkg.TAPE_STORE_ALL, L, 1, args.fp64_steps, False)
+        r64 = run_path(pkg, sc, vol, dFc, pkg.PREC_F64, pkg.TAPE_STORE_ALL, L, 1, args.fp64_steps)
         v64 = sc.n * args.episode_steps * args.fp64_steps / r64["time"]
         print(f"# fp64 plain: {r64['time'] / args.fp64_steps * 1e3:.2f} ms/episode, {v64:.4e} particle-steps/s", file=sys.stderr, flush=True)
 
     # roofline of the dominant kernel
     prof = r32["prof"]
-    dom = max((k for k in KBYTES), key=lambda k: prof[k][0])
-    ms, launches = prof[dom]
-    pb, nb, gb = KBYTES[dom]
-    G = sc.G
-    bytes_per_launch = pb * sc.n + nb * touched + gb * G
-    avg_s = ms / 1e3 / max(launches, 1)
-    achieved = bytes_per_launch / avg_s / 1e9
-    kernels = {k: {"ms_total": round(prof[k][0], 3), "launches": prof[k][1],
-                   "avg_us": round(prof[k][0] * 1e3 / max(prof[k][1], 1), 2)} for k in prof}
+    roof, kernels = None, None
+    if prof is not None:
+        dom = max((k for k in KBYTES), key=lambda k: prof[k][0])
+        ms, launches = prof[dom]
+        pb, nb, gb = KBYTES[dom]
+        bytes_per_launch = pb * sc.n + nb * touched + gb * sc.G
+        avg_s = ms / 1e3 / max(launches, 1)
+        achieved = bytes_per_launch / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": bytes_per_launch}
+        kernels = {k: {"ms_total": round(prof[k][0], 3), "launches": prof[k][1],
+                       "avg_us": round(prof[k][0] * 1e3 / max(prof[k][1], 1), 2)} for k in prof}
+
+    batched = None
+    if args.batched_steps:
+        batched = run_batched(pkg, 24, 51, args.dt, 1, args.batched_steps)
+        print(f"# batched 24: {batched['ms_per_step']:.2f} ms/episode, {batched['value']:.4e} particle-steps/s "
+              f"({batched['particles']} particles)", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -225,13 +292,20 @@ def main():
         "config": {"workload": f"{sc.name}: {sc.n} particles, {sc.dim[0]}^3 grid, {args.episode_steps}-step episodes, "
                                f"FP32 state + FP64 polar/SVD, checkpoint+recompute; step = one fwd+loss+adjoint episode",
                    "dt": args.dt, "volume": "reference formula x dx^3", "control": "dFc ~ N(0,1e-2) on layer 0"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": bytes_per_launch},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_fp32": kernels,
+        "kernel_timing": f"HIP events around every launch of {args.prof_steps} eager episodes after the timed region",
+        "tuned": r32["tuned"],
+        "graphs": not args.no_graphs,
         "touched_nodes": touched,
         "memory_fp32_ckpt_bytes": r32["mem"],
     }
+    if rsa is not None:
+        out["fp32_store_all"] = {"value": sc.n * args.episode_steps * args.store_all_steps / rsa["time"],
+                                 "ms_per_step": rsa["time"] / args.store_all_steps * 1e3, "memory_bytes": rsa["mem"]}
+    if batched is not None:
+        out["batched"] = batched
     if r64 is not None:
         v64 = sc.n * args.episode_steps * args.fp64_steps / r64["time"]
         g32, g64 = r32["grad"], r64["grad"]
