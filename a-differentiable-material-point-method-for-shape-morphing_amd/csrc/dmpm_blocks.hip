@@ -368,7 +368,7 @@ __device__ __forceinline__ void offsets4(const Axis<float>& ax, float h, float o
 }  // namespace
 
 // ------------------------------------------------------------------ P2G ----
-__global__ __launch_bounds__(kScat) void k_p2g_blk(Batch bt, StepIO<float> io)
+__global__ __launch_bounds__(kScat, 4) void k_p2g_blk(Batch bt, StepIO<float> io)
 {
     extern __shared__ unsigned long long lds64[];
     const int4 rg = bt.ranges[blockIdx.x];
@@ -404,6 +404,7 @@ __global__ __launch_bounds__(kScat) void k_p2g_blk(Batch bt, StepIO<float> io)
         const double k = kst * vol;
 #pragma unroll
         for (int i = 0; i < 9; ++i) A.a[i] = float(k * PF.a[i]) + m * C.a[i];
+        if (io.aux) stm(io.aux, FA, S, p, A);
 #pragma unroll
         for (int a = 0; a < 3; ++a) mv[a] = m * v[a] * sdrag;
         // every contribution: |w (mv + A d)| <= |mv| + 2 dx sum_b |A_ab|  (w <= 1, |d| <= 2 dx)
@@ -599,11 +600,14 @@ __global__ __launch_bounds__(kScat) void k_g2p_adj_blk(Batch bt, AdjIO<float> io
 }
 
 // ------------------------------------------------------- transposed P2G ----
-// Gather half of P2G^T.  With G = -k P Fc^T + m C and b = m(1 - dt drag) v:
-//   Gb   = sum w pbar d^T          -> kept in tmp for the elastic kernel
-//   Cbar = m Gb,  vbar = m (1 - dt drag) sum w pbar,
+// P2G^T + P_op_1^T, one thread per particle of the range.  With the P2G
+// affine matrix G = -k P Fc^T + m C (cached in the layer by P2G) and
+// b = m(1 - dt drag) v, the gather over the 64 nodes gives
+//   Gb   = sum w pbar d^T,  Cbar = m Gb,  vbar = m (1 - dt drag) sum w pbar,
 //   xbar = x' + sum [gw (m mbar + (b + G d).pbar + v_i.vt + D (Ct : v_i d^T))
-//                    - w (G^T pbar + D Ct^T v_i)].
+//                    - w (G^T pbar + D Ct^T v_i)],
+// then the elastic part (FP64 polar, PK1 and H:dF):
+//   Pbar = -k Gb Fc,  Fcbar = (1-s)(I + dt C')^T F' - k Gb^T P + H(Fc):Pbar.
 __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
 {
     extern __shared__ float lds[];
@@ -630,23 +634,7 @@ __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
     const double kst = -3.0 * sp.inv_dx * sp.inv_dx * sp.dt;
     for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) {
         const float m = io.cst[KM * S + p];
-        M3<float> Gm;
-        {
-            const M3<float> C = ldm(io.cur, FC, S, p);
-            const M3<float> F = ldm(io.cur, FF, S, p);
-            const M3<float> dF = ldm(io.ctrl, 0, S, p);
-            const double vol = io.cst[KVOL * S + p];
-            const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
-            M3<double> Fc;
-#pragma unroll
-            for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
-            FCE e;
-            fce_prepare(Fc, e);
-            const M3<double> PF = mult(fce_pk1(Fc, e, lam, mu), Fc);
-            const double k = kst * vol;
-#pragma unroll
-            for (int i = 0; i < 9; ++i) Gm.a[i] = float(k * PF.a[i]) + m * C.a[i];
-        }
+        const M3<float> Gm = ldm(io.cur, FA, S, p);
         float x[3], v[3], vt[3];
         ld3(io.cur, FX, S, p, x);
         ld3(io.cur, FV, S, p, v);
@@ -707,24 +695,9 @@ __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
 #pragma unroll
         for (int i = 0; i < 9; ++i) gC.a[i] = m * Gb.a[i];
         stm(io.g0, FC, S, p, gC);
-        stm(io.tmp, 0, S, p, Gb);  // vt, Ct are dead for this particle
-    }
-}
-
-// Elastic half of P2G^T + P_op_1^T, one thread per particle of a range:
-//   Pbar = -k Gb Fc,  Fcbar = (1-s)(I + dt C')^T F' - k Gb^T P + H(Fc):Pbar.
-__global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io)
-{
-    const int4 rg = bt.ranges[blockIdx.x];
-    const StepParams sp = bt.sps[rg.z];
-    const int S = sp.stride;
-    const int cp = sp.compat;
-    const float dt = float(sp.dt);
-    const float s = float(sp.smoothing);
-    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) {
+        // elastic part
         const M3<float> F = ldm(io.cur, FF, S, p);
         const M3<float> dF = ldm(io.ctrl, 0, S, p);
-        const M3<float> Gb = ldm(io.tmp, 0, S, p);
         const double vol = io.cst[KVOL * S + p];
         const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
         M3<double> Fc;
@@ -733,7 +706,7 @@ __global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io)
         FCE e;
         fce_prepare(Fc, e);
         const M3<double> P = fce_pk1(Fc, e, lam, mu);
-        const double kk_ = 3.0 * sp.inv_dx * sp.inv_dx * sp.dt * vol;
+        const double kk_ = -kst * vol;  // 3 dt vol / dx^2
         const M3<double> Gbd = cast<double>(Gb);
         M3<double> Pb = mul(Gbd, Fc);
 #pragma unroll
@@ -749,6 +722,7 @@ __global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io)
         IC.a[4] += 1.f;
         IC.a[8] += 1.f;
         const M3<float> base = tmul(IC, gF1);
+        const float s = float(sp.smoothing);
         M3<float> gF, gdFc;
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
@@ -933,12 +907,6 @@ void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
 {
     if (b.nranges <= 0) return;
     hipLaunchKernelGGL(k_p2g_adj_blk, dim3(b.nranges), dim3(kGat), 7 * win_nodes(b.cap) * sizeof(float), s, b, io);
-}
-
-void launch_elastic_adj(const Batch& b, const AdjIO<float>& io, hipStream_t s)
-{
-    if (b.nranges <= 0) return;
-    hipLaunchKernelGGL(k_elastic_adj, dim3(b.nranges), dim3(kGat), 0, s, b, io);
 }
 
 void launch_grid_update_b(const Batch& b, float* grid, hipStream_t s)

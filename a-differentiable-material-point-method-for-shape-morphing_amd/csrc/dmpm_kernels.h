@@ -9,6 +9,10 @@ namespace dmpm {
 // Particle layer in HBM: structure of arrays, field f of particle p at
 // base[f * stride + p].  Fields: x0..2, v0..2, C00..C22, F00..F22.
 enum : int { FX = 0, FV = 3, FC = 6, FF = 15, kLayerFields = 24 };
+// FP32 layers carry 9 more fields: the P2G affine matrix
+// A = m C - (3/dx^2) dt vol P(Fc) Fc^T of the step that starts at the layer,
+// written by P2G and read back by the transposed P2G (no second polar solve).
+enum : int { FA = 24, kLayerFieldsF32 = 33 };
 // Per-particle constants: m, vol, lam, mu.
 enum : int { KM = 0, KVOL = 1, KLAM = 2, KMU = 3, kConstFields = 4 };
 // Grid of one step: m then v (or momentum before the update) per axis.
@@ -33,6 +37,7 @@ struct StepIO {
     const Real* ctrl;  // dFc_n (9 fields)
     const Real* cst;   // constants
     Real* grid;        // grid m, v/p (kGridFields * G)
+    Real* aux;         // writable alias of cur for derived fields (FP32: A), or null
 };
 
 template <typename Real>
@@ -46,7 +51,7 @@ struct AdjIO {
     const Real* g1;      // gradients of layer n+1 (24 fields, same layout as a layer)
     Real* g0;            // gradients of layer n
     Real* gctrl;         // dL/d(dFc_n) (9 fields)
-    Real* tmp;           // per particle vt (3) and Ct (9)
+    Real* tmp;           // per particle vt (3) and Ct (9), from the transposed G2P
 };
 
 template <typename Real> void launch_fill(Real* p, long long count, Real value, hipStream_t s);
@@ -85,8 +90,6 @@ void launch_p2g_blk(const Batch& b, const StepIO<float>& io, hipStream_t s);
 void launch_g2p_blk(const Batch& b, const StepIO<float>& io, hipStream_t s);
 void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s);
 void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s);
-// elastic half of the transposed P2G (Gb from tmp -> dLdF, dL/d(dFc)), over the ranges
-void launch_elastic_adj(const Batch& b, const AdjIO<float>& io, hipStream_t s);
 void launch_grid_update_b(const Batch& b, float* grid, hipStream_t s);
 void launch_grid_adj_b(const Batch& b, const float* grid, float* agrid, hipStream_t s);
 // loss over all instances from node masses gm (instance i at nofs[i]);

@@ -56,6 +56,7 @@ struct dmpm_ctx {
     dmpm_config cfg{};
     bool f32 = false;
     size_t esz = 8;
+    int lf = kLayerFields;  // fields per layer slot (FP32: + the cached P2G affine matrix)
     int N = 0, S = 0, L = 0, K = 0;
     long long G = 0;
     hipStream_t stream = nullptr;
@@ -240,7 +241,7 @@ StepParams step_params(const dmpm_ctx* c) { return inst_params(c, 0); }
 template <typename Real>
 Real* slot(dmpm_ctx* c, int s)
 {
-    return static_cast<Real*>(c->layers) + (size_t)s * kLayerFields * c->S;
+    return static_cast<Real*>(c->layers) + (size_t)s * c->lf * c->S;
 }
 
 template <typename Real>
@@ -303,7 +304,6 @@ inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<doub
 inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams&, const AdjIO<float>& io)
 {
     launch_p2g_adj_blk(c->batch, io, c->stream);
-    launch_elastic_adj(c->batch, io, c->stream);
 }
 
 // One ForwardTimeStep: cur -> nxt, grid of the step in g.
@@ -311,7 +311,8 @@ template <typename Real>
 void step_forward(dmpm_ctx* c, const Real* cur, Real* nxt, const Real* ctrl, Real* g)
 {
     StepParams sp = step_params(c);
-    StepIO<Real> io{cur, nxt, ctrl, static_cast<const Real*>(c->cst), g};
+    // FP32: P2G caches its affine matrix in the input layer
+    StepIO<Real> io{cur, nxt, ctrl, static_cast<const Real*>(c->cst), g, c->f32 ? const_cast<Real*>(cur) : nullptr};
     fill<Real>(c, g, (long long)kGridFields * c->G, Real(0));
     timed(c, DMPM_K_P2G, [&] { p2g_dispatch(c, sp, io); });
     timed(c, DMPM_K_GRID_UPDATE, [&] { grid_update_dispatch(c, sp, g); });
@@ -905,6 +906,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
     c->cfg = *cfg;
     c->f32 = cfg->precision == DMPM_PREC_F32;
     c->esz = c->f32 ? 4 : 8;
+    c->lf = c->f32 ? kLayerFieldsF32 : kLayerFields;
     c->B = (int)insts.size();
     c->inst = insts;
     c->poff.assign(c->B + 1, 0);
@@ -949,7 +951,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
     const size_t grid_slots = cfg->tape_mode == DMPM_TAPE_CHECKPOINT ? (size_t)c->K : (size_t)steps;
     const size_t gctrl_layers = cfg->keep_control_grads ? (size_t)steps : 1;
     if ((rc = dalloc(c, &c->cst, kConstFields * S * e)) || (rc = dalloc(c, &c->ctrl, (size_t)steps * 9 * S * e)) ||
-        (rc = dalloc(c, &c->layers, (size_t)c->n_slots * kLayerFields * S * e)) ||
+        (rc = dalloc(c, &c->layers, (size_t)c->n_slots * c->lf * S * e)) ||
         (rc = dalloc(c, &c->grids, grid_slots * kGridFields * G * e)) || (rc = dalloc(c, &c->wgrid, kGridFields * G * e)) ||
         (rc = dalloc(c, &c->agrid, kGridFields * G * e)) || (rc = dalloc(c, &c->gbuf[0], kLayerFields * S * e)) ||
         (rc = dalloc(c, &c->gbuf[1], kLayerFields * S * e)) || (rc = dalloc(c, &c->gctrl, gctrl_layers * 9 * S * e)) ||
@@ -964,7 +966,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
     // zero controls and layer 0 so a context is well defined before uploads
     if (c->f32) {
         launch_fill<float>(static_cast<float*>(c->ctrl), (long long)steps * 9 * S, 0.f, c->stream);
-        launch_fill<float>(static_cast<float*>(c->layers), (long long)kLayerFields * S, 0.f, c->stream);
+        launch_fill<float>(static_cast<float*>(c->layers), (long long)c->lf * S, 0.f, c->stream);
         launch_fill<float>(static_cast<float*>(c->cst), (long long)kConstFields * S, 0.f, c->stream);
     } else {
         launch_fill<double>(static_cast<double*>(c->ctrl), (long long)steps * 9 * S, 0.0, c->stream);
