@@ -44,6 +44,19 @@ constexpr int kScat = 512;   // scatter kernels: one particle per thread of a bl
 constexpr int kMaxWaves = 8;
 constexpr int kFW = 8;       // fixed window edge (nodes)
 constexpr int kFW3 = kFW * kFW * kFW;
+// LDS layout of the fixed window: node (i, j, k) at slot i*kSI + j*kSJ + k.
+// With kSJ = 12 and kSI = 112 the 64 cells of a 4^3 block (one per lane of a
+// wave at a given stencil offset) fall on 32 distinct banks mod 32 and 16
+// distinct 8-byte bank pairs: the fewest conflicts possible for 64 lanes,
+// half (floats) or a quarter (64-bit) of those of the dense 8x8x8 layout.
+constexpr int kSJ = 12;
+constexpr int kSI = 112;
+constexpr int kFWS = kFW * kSI;  // slots per channel
+
+__device__ __forceinline__ int fixed_slot_of(int idx)  // idx: x-major node of the 8^3 window
+{
+    return (idx >> 6) * kSI + ((idx >> 3) & 7) * kSJ + (idx & 7);
+}
 
 struct Win {
     int lo[3];   // general window: clamped box
@@ -146,7 +159,7 @@ __device__ __forceinline__ long long fixed_node(const StepParams& sp, const Win&
 
 __device__ __forceinline__ int fixed_slot(const Win& W, const Axis<float> st[3])
 {
-    return ((st[0].base - W.flo[0]) * kFW + (st[1].base - W.flo[1])) * kFW + (st[2].base - W.flo[2]);
+    return (st[0].base - W.flo[0]) * kSI + (st[1].base - W.flo[1]) * kSJ + (st[2].base - W.flo[2]);
 }
 
 // load NC channels (channel c at src_c[node]) of the window into LDS
@@ -157,7 +170,7 @@ __device__ __forceinline__ void win_load(const StepParams& sp, const Win& W, flo
         for (int idx = threadIdx.x; idx < kFW3; idx += blockDim.x) {
             const long long node = fixed_node(sp, W, idx);
 #pragma unroll
-            for (int c = 0; c < NC; ++c) lds[c * kFW3 + idx] = node >= 0 ? src[c][node] : 0.f;
+            for (int c = 0; c < NC; ++c) lds[c * kFWS + fixed_slot_of(idx)] = node >= 0 ? src[c][node] : 0.f;
         }
     } else if (W.lds) {
         for (int idx = threadIdx.x; idx < W.vol; idx += blockDim.x) {
@@ -240,7 +253,7 @@ __device__ __forceinline__ void flush_fixed(const StepParams& sp, const Win& W, 
             if (node < 0) continue;
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-                const long long q = (long long)lds[c * kFW3 + idx];
+                const long long q = (long long)lds[c * kFWS + fixed_slot_of(idx)];
                 if (q != 0) unsafeAtomicAdd(base + c * cstride + node, (float)((double)q * (double)S.inv[c]));
             }
         }
@@ -302,9 +315,9 @@ __device__ __forceinline__ void scatter64(const StepParams& sp, const Win& W, co
                 float v[NC];
                 val(i, j, kk, v);
                 if (MODE == 0) {
-                    const int o = slot + i * kFW * kFW + j * kFW + kk;
+                    const int o = slot + i * kSI + j * kSJ + kk;
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) atomicAdd(&lds[c * kFW3 + o], fix32(v[c]));
+                    for (int c = 0; c < NC; ++c) atomicAdd(&lds[c * kFWS + o], fix32(v[c]));
                 } else if (MODE == 1) {
                     const int o = win_local(W, gi, gj, gk);
 #pragma unroll
@@ -340,9 +353,9 @@ __device__ __forceinline__ void gather64(const StepParams& sp, const Win& W, con
                 if (MODE != 0 && (gk < 0 || gk >= sp.dim[2])) continue;
                 float ch[NC];
                 if (MODE == 0) {
-                    const int o = slot + i * kFW * kFW + j * kFW + kk;
+                    const int o = slot + i * kSI + j * kSJ + kk;
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) ch[c] = lds[c * kFW3 + o];
+                    for (int c = 0; c < NC; ++c) ch[c] = lds[c * kFWS + o];
                 } else if (MODE == 1) {
                     const int o = win_local(W, gi, gj, gk);
 #pragma unroll
@@ -416,7 +429,7 @@ __global__ __launch_bounds__(kScat, 4) void k_p2g_blk(Batch bt, StepIO<float> io
     const Scales<4> SC = block_scales<4>(bound);
     const int mode = (W.fixed && SC.ok) ? 0 : ((W.lds && SC.ok) ? 1 : 2);
     if (mode == 0)
-        for (int i = threadIdx.x; i < 4 * kFW3; i += blockDim.x) lds64[i] = 0ull;
+        for (int i = threadIdx.x; i < 4 * kFWS; i += blockDim.x) lds64[i] = 0ull;
     else if (mode == 1)
         for (int i = threadIdx.x; i < 4 * W.vol; i += blockDim.x) lds64[i] = 0ull;
     __syncthreads();
@@ -565,7 +578,7 @@ __global__ __launch_bounds__(kScat) void k_g2p_adj_blk(Batch bt, AdjIO<float> io
     const Scales<3> SC = block_scales<3>(bound);
     const int mode = (W.fixed && SC.ok) ? 0 : ((W.lds && SC.ok) ? 1 : 2);
     if (mode == 0)
-        for (int i = threadIdx.x; i < 3 * kFW3; i += blockDim.x) lds64[i] = 0ull;
+        for (int i = threadIdx.x; i < 3 * kFWS; i += blockDim.x) lds64[i] = 0ull;
     else if (mode == 1)
         for (int i = threadIdx.x; i < 3 * W.vol; i += blockDim.x) lds64[i] = 0ull;
     __syncthreads();
@@ -882,8 +895,8 @@ __global__ __launch_bounds__(kGat) void k_loss_particles_b(Batch bt, const float
 }
 
 // ------------------------------------------------------------ launchers ----
-// LDS: NC channels of max(cap, 8^3) nodes (int64 for scatters, float for gathers)
-static size_t win_nodes(int cap) { return (size_t)(cap > kFW3 ? cap : kFW3); }
+// LDS: NC channels of max(cap, fixed-window slots) (int64 for scatters, float for gathers)
+static size_t win_nodes(int cap) { return (size_t)(cap > kFWS ? cap : kFWS); }
 
 void launch_p2g_blk(const Batch& b, const StepIO<float>& io, hipStream_t s)
 {
