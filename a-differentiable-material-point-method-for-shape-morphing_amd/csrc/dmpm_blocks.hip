@@ -662,40 +662,45 @@ __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
         offsets4(st[0], dx, d0s);
         offsets4(st[1], dx, d1s);
         offsets4(st[2], dx, d2s);
+        // the sums over nodes are regrouped so that the node loop does the
+        // least work: sum_i w_i G^T pbar_i = G^T (sum_i w_i pbar_i) and
+        // sum_i w_i D Ct^T v_i = D Ct^T (sum_i w_i v_i) leave the loop
+        M3<float> CtD;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) CtD.a[i] = Dinv * Ct.a[i];
         M3<float> Gb = m3_zero<float>();
-        float xb[3] = {0.f, 0.f, 0.f}, pacc[3] = {0.f, 0.f, 0.f};
+        float xg[3] = {0.f, 0.f, 0.f}, pacc[3] = {0.f, 0.f, 0.f}, vacc[3] = {0.f, 0.f, 0.f};
         auto acc = [&](int i, int j, int kk, const float* ch) {
             const float d0 = d0s[i], d1 = d1s[j], d2 = d2s[kk];
-            const float w0 = st[0].w[i], w1 = st[1].w[j], w2 = st[2].w[kk];
-            const float w = w0 * w1 * w2;
-            const float gw[3] = {mgw * st[0].dw[i] * w1 * w2, mgw * w0 * st[1].dw[j] * w2, mgw * w0 * w1 * st[2].dw[kk]};
-            const float pbi[3] = {ch[0], ch[1], ch[2]};
-            const float mbi = ch[3];
-            const float vi[3] = {ch[4], ch[5], ch[6]};
-            float bgd = 0.f, vdot = 0.f, cvd = 0.f;
+            const float w01 = st[0].w[i] * st[1].w[j];
+            const float w = w01 * st[2].w[kk];
+            const float gw[3] = {(mgw * st[0].dw[i] * st[1].w[j]) * st[2].w[kk], (mgw * st[0].w[i] * st[1].dw[j]) * st[2].w[kk],
+                                 (mgw * w01) * st[2].dw[kk]};
+            float sgw = m * ch[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                const float gd = Gm.a[3 * a] * d0 + Gm.a[3 * a + 1] * d1 + Gm.a[3 * a + 2] * d2;
-                bgd += (b[a] + gd) * pbi[a];
-                vdot += vi[a] * vt[a];
-                cvd += vi[a] * (Ct.a[3 * a] * d0 + Ct.a[3 * a + 1] * d1 + Ct.a[3 * a + 2] * d2);
-                const float wp = w * pbi[a];
+                // q = b + G d, r = vt + D Ct d (the (i, j) parts are shared by the 4 nodes of a k-row)
+                const float q = (b[a] + Gm.a[3 * a] * d0 + Gm.a[3 * a + 1] * d1) + Gm.a[3 * a + 2] * d2;
+                const float r = (vt[a] + CtD.a[3 * a] * d0 + CtD.a[3 * a + 1] * d1) + CtD.a[3 * a + 2] * d2;
+                sgw += ch[a] * q + ch[4 + a] * r;
+                const float wp = w * ch[a];
                 pacc[a] += wp;
                 Gb.a[3 * a] += wp * d0;
                 Gb.a[3 * a + 1] += wp * d1;
                 Gb.a[3 * a + 2] += wp * d2;
+                vacc[a] += w * ch[4 + a];
             }
-            const float sgw = m * mbi + bgd + vdot + Dinv * cvd;
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const float GTp = Gm.a[a] * pbi[0] + Gm.a[3 + a] * pbi[1] + Gm.a[6 + a] * pbi[2];
-                const float CTv = Ct.a[a] * vi[0] + Ct.a[3 + a] * vi[1] + Ct.a[6 + a] * vi[2];
-                xb[a] += gw[a] * sgw - w * (GTp + Dinv * CTv);
-            }
+            for (int a = 0; a < 3; ++a) xg[a] += gw[a] * sgw;
         };
         if (W.fixed) gather64<7, 0>(sp, W, st, lds, src, acc);
         else if (W.lds) gather64<7, 1>(sp, W, st, lds, src, acc);
         else gather64<7, 2>(sp, W, st, lds, src, acc);
+        float xb[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            xb[a] = xg[a] - (Gm.a[a] * pacc[0] + Gm.a[3 + a] * pacc[1] + Gm.a[6 + a] * pacc[2]) -
+                    (CtD.a[a] * vacc[0] + CtD.a[3 + a] * vacc[1] + CtD.a[6 + a] * vacc[2]);
         float gx1[3];
         ld3(io.g1, FX, S, p, gx1);
         const float vsc = m * sdrag;
