@@ -17,7 +17,7 @@ import os
 
 import numpy as np
 
-from . import mpmbin, scenes  # noqa: F401
+from . import mpmbin, optimize, scenes  # noqa: F401
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libdmpm.so")
@@ -31,6 +31,8 @@ COMPAT_NO_1MS = 2
 COMPAT_NO_DRAG_X = 4
 COMPAT_DLDM_NO_FEXT = 8
 COMPAT_REFERENCE = 15
+
+DESCEND_ADAM, DESCEND_AMSGRAD, DESCEND_LION, DESCEND_GD = range(4)
 
 OPT_GRAPHS = 1
 OPT_RANGE_MAX = 2
@@ -50,7 +52,7 @@ EXPORTS = [
     "dmpm_control_grad_norm", "dmpm_get_grid", "dmpm_chain_episode", "dmpm_synchronize", "dmpm_memory_bytes",
     "dmpm_profile_enable", "dmpm_profile_reset", "dmpm_profile_read", "dmpm_eval_pk1", "dmpm_eval_hessian",
     "dmpm_eval_svd", "dmpm_create_batch", "dmpm_num_instances", "dmpm_get_instance_losses",
-    "dmpm_target_from_points_batch", "dmpm_set_option", "dmpm_get_option", "dmpm_autotune",
+    "dmpm_target_from_points_batch", "dmpm_set_option", "dmpm_get_option", "dmpm_autotune", "dmpm_descend",
 ]
 
 
@@ -145,6 +147,7 @@ def load_library(path: str = LIB_PATH):
         "dmpm_set_option": ([vp, i32, ctypes.c_int64], ctypes.c_int),
         "dmpm_get_option": ([vp, i32, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "dmpm_autotune": ([vp, _DP], ctypes.c_int),
+        "dmpm_descend": ([vp, i32, i32] + [ctypes.c_float] * 5 + [i32], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -372,6 +375,11 @@ class Morph:
         out = ctypes.c_double()
         _check(self._L.dmpm_control_grad_norm(self._h, int(layer), ctypes.byref(out)))
         return out.value
+
+    def descend(self, layer, rule, alpha, gradient_norm, beta1=0.9, beta2=0.999, epsilon=1e-5, timestep=1):
+        """PointCloud::Descend_* on the device for the control of ``layer`` (rule: DESCEND_*)."""
+        _check(self._L.dmpm_descend(self._h, int(layer), int(rule), float(alpha), float(gradient_norm), float(beta1),
+                                    float(beta2), float(epsilon), int(timestep)))
 
     def grid(self, step):
         m = np.zeros(self.G)

@@ -54,7 +54,17 @@ struct AdjIO {
     Real* tmp;           // per particle vt (3) and Ct (9), from the transposed G2P
 };
 
+// control update of one layer (PointCloud::Descend_*): rule 0 Adam, 1 AMSGrad, 2 Lion, 3 gradient descent
+struct DescendParams {
+    int rule;
+    float alpha, gradient_norm, beta1, beta2, epsilon;
+    float bc1, bc2;  // 1 - beta1^t, 1 - beta2^t
+};
+
 template <typename Real> void launch_fill(Real* p, long long count, Real value, hipStream_t s);
+template <typename Real> void launch_descend(long long count, const DescendParams& dp, Real* dFc, const Real* g, Real* mom,
+                                             Real* vec, Real* vmax, hipStream_t s);
+template <typename Real> void launch_sumsq(const Real* a, long long count, double* acc, hipStream_t s);
 template <typename Real> void launch_p2g(const StepParams& sp, const StepIO<Real>& io, hipStream_t s);
 template <typename Real> void launch_grid_update(const StepParams& sp, Real* grid, hipStream_t s);
 template <typename Real> void launch_g2p(const StepParams& sp, const StepIO<Real>& io, hipStream_t s);

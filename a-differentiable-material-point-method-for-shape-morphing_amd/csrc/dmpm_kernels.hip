@@ -754,7 +754,92 @@ void launch_eval_svd(int n, const double* F, double* U, double* sig, double* V, 
 }
 
 // ---------------------------------------------------- instantiations -------
+// ------------------------------------------------------------ optimizer ----
+// PointCloud::Descend_* (PointCloud.cpp:83-179) per control entry, in the
+// path's precision (the reference's float on the FP32 path):
+// grad = dL/d(dFc) / gradient_norm, then
+//   Adam:    m = b1 m + (1-b1) grad, v = b2 v + (1-b2) grad^2,
+//            dFc -= alpha (m / bc1) / (sqrt(v / bc2) + eps)
+//   AMSGrad: as Adam with vmax = max(vmax, v / bc2) in the denominator
+//   Lion:    m = b1 m + (1-b1) sign(grad), dFc -= alpha m
+//   GD:      dFc -= alpha grad
+// with bc1 = 1 - b1^t, bc2 = 1 - b2^t.  State m, v, vmax (MaterialPoint
+// momentum, vector, vector_max) is per particle and control layer.
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_descend(long long count, DescendParams dp, Real* dFc, const Real* g, Real* mom, Real* vec,
+                                                    Real* vmax)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const Real b1 = Real(dp.beta1), b2 = Real(dp.beta2), alpha = Real(dp.alpha);
+    const Real grad = g[i] / Real(dp.gradient_norm);
+    Real d = dFc[i];
+    if (dp.rule == 2) {  // Lion
+        const Real sg = grad > Real(0) ? Real(1) : (grad < Real(0) ? Real(-1) : Real(0));
+        const Real m = b1 * mom[i] + (Real(1) - b1) * sg;
+        mom[i] = m;
+        d -= alpha * m;
+    } else if (dp.rule == 3) {  // gradient descent
+        d -= alpha * grad;
+    } else {  // Adam, AMSGrad
+        const Real m = b1 * mom[i] + (Real(1) - b1) * grad;
+        const Real v = b2 * vec[i] + (Real(1) - b2) * grad * grad;
+        mom[i] = m;
+        vec[i] = v;
+        const Real mh = m / Real(dp.bc1), vh = v / Real(dp.bc2);
+        Real den = vh;
+        if (dp.rule == 1) {
+            den = vmax[i] > vh ? vmax[i] : vh;
+            vmax[i] = den;
+        }
+        d -= alpha * (mh / (sqrt(den) + Real(dp.epsilon)));
+    }
+    dFc[i] = d;
+}
+
+template <typename Real>
+void launch_descend(long long count, const DescendParams& dp, Real* dFc, const Real* g, Real* mom, Real* vec, Real* vmax,
+                    hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_descend<Real>, dim3(nblocks(count, kBlock)), dim3(kBlock), 0, s, count, dp, dFc, g, mom, vec, vmax);
+}
+
+// sum of squares in FP64 (PointCloud::Compute_dLdF_Norm, PointCloud.cpp:24-35), added to acc
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_sumsq(const Real* a, long long count, double* acc)
+{
+    __shared__ double red[kBlock / 64];
+    double local = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (long long)gridDim.x * blockDim.x) {
+        const double v = double(a[i]);
+        local += v * v;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) t += red[w];
+        atomicAdd(acc, t);
+    }
+}
+
+template <typename Real>
+void launch_sumsq(const Real* a, long long count, double* acc, hipStream_t s)
+{
+    if (count <= 0) return;
+    unsigned nb = nblocks(count, kBlock);
+    if (nb > 1024) nb = 1024;
+    hipLaunchKernelGGL(k_sumsq<Real>, dim3(nb), dim3(kBlock), 0, s, a, count, acc);
+}
+
 #define DMPM_INSTANTIATE(R)                                                                                        \
+    template void launch_descend<R>(long long, const DescendParams&, R*, const R*, R*, R*, R*, hipStream_t);      \
+    template void launch_sumsq<R>(const R*, long long, double*, hipStream_t);                                      \
     template void launch_fill<R>(R*, long long, R, hipStream_t);                                                   \
     template void launch_p2g<R>(const StepParams&, const StepIO<R>&, hipStream_t);                                 \
     template void launch_grid_update<R>(const StepParams&, R*, hipStream_t);                                       \

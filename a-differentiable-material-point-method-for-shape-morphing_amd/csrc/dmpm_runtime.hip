@@ -71,7 +71,8 @@ struct dmpm_ctx {
     void* gctrl = nullptr;    // (L-1) x 9 x S or 9 x S
     void* tmp = nullptr;      // 12 x S
     void* target = nullptr;   // G
-    double* dacc = nullptr;   // FP64 accumulators
+    double* dacc = nullptr;   // FP64 accumulators: per-instance losses [0, B), norm at B
+    std::vector<void*> opt;   // per control layer: optimizer state m, v, vmax (27 x S), allocated on first use
     uint64_t bytes = 0;
     // instances: particles of instance i are API/device slots [poff[i], poff[i+1]),
     // its grid nodes [nofs[i], nofs[i+1]) (4-channel grids at 4 * nofs[i])
@@ -462,7 +463,8 @@ int end_loss(dmpm_ctx* c, double* loss)
     if (loss) *loss = h;
     c->loss_valid = true;
     c->grads_lo = last;
-    c->bwd_control = -1;
+    // the control gradients of the last backward pass stay held (as the
+    // reference's per-point dLdF): a line search reads them after new losses
     return DMPM_OK;
 }
 
@@ -801,12 +803,21 @@ int set_layer_grads(dmpm_ctx* c, int layer, const double* gx, const double* gv, 
     return DMPM_OK;
 }
 
-template <typename Real>
-int get_control_grad(dmpm_ctx* c, int layer, double* out)
+// The control gradient of `layer` is held: the control layer of the last
+// backward pass, or any layer it reached when every layer's gradient is kept.
+int check_control_grad(dmpm_ctx* c, int layer)
 {
     if (c->bwd_control < 0) return fail(DMPM_ERR_STATE, "no backward pass has run");
     bool ok = c->cfg.keep_control_grads ? (layer >= c->bwd_control && layer <= c->L - 2) : (layer == c->bwd_control);
     if (!ok) return fail(DMPM_ERR_STATE, "control gradient of this layer is not kept");
+    return DMPM_OK;
+}
+
+template <typename Real>
+int get_control_grad(dmpm_ctx* c, int layer, double* out)
+{
+    const int rc = check_control_grad(c, layer);
+    if (rc) return rc;
     return download_fields<Real>(c, gctrl_of<Real>(c, layer), 0, 9, out, c->N);
 }
 
@@ -956,7 +967,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         (rc = dalloc(c, &c->agrid, kGridFields * G * e)) || (rc = dalloc(c, &c->gbuf[0], kLayerFields * S * e)) ||
         (rc = dalloc(c, &c->gbuf[1], kLayerFields * S * e)) || (rc = dalloc(c, &c->gctrl, gctrl_layers * 9 * S * e)) ||
         (rc = dalloc(c, &c->tmp, 12 * S * e)) || (rc = dalloc(c, &c->target, G * e)) ||
-        (rc = dalloc(c, (void**)&c->dacc, std::max(64, c->B) * sizeof(double))) ||
+        (rc = dalloc(c, (void**)&c->dacc, std::max(64, c->B + 8) * sizeof(double))) ||
         (rc = dalloc(c, (void**)&c->d_sps, c->B * sizeof(StepParams))) ||
         (rc = dalloc(c, (void**)&c->d_nofs, (c->B + 1) * sizeof(long long)))) {
         std::string msg = g_err;
@@ -965,10 +976,12 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
     }
     // zero controls and layer 0 so a context is well defined before uploads
     if (c->f32) {
+        launch_fill<float>(static_cast<float*>(c->gctrl), (long long)gctrl_layers * 9 * S, 0.f, c->stream);
         launch_fill<float>(static_cast<float*>(c->ctrl), (long long)steps * 9 * S, 0.f, c->stream);
         launch_fill<float>(static_cast<float*>(c->layers), (long long)c->lf * S, 0.f, c->stream);
         launch_fill<float>(static_cast<float*>(c->cst), (long long)kConstFields * S, 0.f, c->stream);
     } else {
+        launch_fill<double>(static_cast<double*>(c->gctrl), (long long)gctrl_layers * 9 * S, 0.0, c->stream);
         launch_fill<double>(static_cast<double*>(c->ctrl), (long long)steps * 9 * S, 0.0, c->stream);
         launch_fill<double>(static_cast<double*>(c->layers), (long long)kLayerFields * S, 0.0, c->stream);
         launch_fill<double>(static_cast<double*>(c->cst), (long long)kConstFields * S, 0.0, c->stream);
@@ -1033,6 +1046,8 @@ int dmpm_destroy(dmpm_ctx* c)
     void* bufs[] = {c->cst,  c->ctrl, c->layers, c->grids, c->wgrid, c->agrid, c->gbuf[0], c->gbuf[1],
                     c->gctrl, c->tmp, c->target, c->dacc,  c->ranges, c->d_sps, c->d_nofs};
     for (void* b : bufs)
+        if (b) hipFree(b);
+    for (void* b : c->opt)
         if (b) hipFree(b);
     for (hipEvent_t ev : c->pool) hipEventDestroy(ev);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -1244,12 +1259,54 @@ int dmpm_control_grad_norm(dmpm_ctx* c, int32_t layer, double* norm)
     int rc = check_ctx(c);
     if (rc) return rc;
     if (!norm) return fail(DMPM_ERR_INVALID, "null output");
-    std::vector<double> g((size_t)9 * c->N);
-    rc = DISPATCH(c, get_control_grad, c, layer, g.data());
+    if ((rc = check_control_grad(c, layer))) return rc;
+    double* acc = c->dacc + c->B;
+    launch_fill<double>(acc, 1, 0.0, c->stream);
+    if (c->f32) launch_sumsq<float>(gctrl_of<float>(c, layer), 9LL * c->S, acc, c->stream);
+    else launch_sumsq<double>(gctrl_of<double>(c, layer), 9LL * c->S, acc, c->stream);
+    HIP_TRY(hipGetLastError());
+    double h = 0.0;
+    HIP_TRY(hipMemcpyAsync(&h, acc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *norm = std::sqrt(h);
+    return DMPM_OK;
+}
+
+int dmpm_descend(dmpm_ctx* c, int32_t layer, int32_t rule, float alpha, float gradient_norm, float beta1, float beta2,
+                 float epsilon, int32_t timestep)
+{
+    int rc = check_ctx(c);
     if (rc) return rc;
-    double s = 0.0;
-    for (double v : g) s += v * v;
-    *norm = std::sqrt(s);
+    if (rule < DMPM_DESCEND_ADAM || rule > DMPM_DESCEND_GD) return fail(DMPM_ERR_INVALID, "descend rule");
+    if ((rc = check_control_grad(c, layer))) return rc;
+    if (c->opt.empty()) c->opt.assign(c->L - 1, nullptr);
+    const size_t n = (size_t)9 * c->S;
+    if (!c->opt[layer]) {
+        if ((rc = dalloc(c, &c->opt[layer], 3 * n * c->esz))) return rc;
+        if (c->f32) launch_fill<float>(static_cast<float*>(c->opt[layer]), 3 * (long long)n, 0.f, c->stream);
+        else launch_fill<double>(static_cast<double*>(c->opt[layer]), 3 * (long long)n, 0.0, c->stream);
+    }
+    DescendParams dp{};
+    dp.rule = rule;
+    dp.alpha = alpha;
+    dp.gradient_norm = gradient_norm;
+    dp.beta1 = beta1;
+    dp.beta2 = beta2;
+    dp.epsilon = epsilon;
+    dp.bc1 = (float)(1.0 - std::pow((double)beta1, (double)timestep));
+    dp.bc2 = (float)(1.0 - std::pow((double)beta2, (double)timestep));
+    if (c->f32) {
+        float* st = static_cast<float*>(c->opt[layer]);
+        launch_descend<float>((long long)n, dp, ctrl_of<float>(c, layer), gctrl_of<float>(c, layer), st, st + n, st + 2 * n, c->stream);
+    } else {
+        double* st = static_cast<double*>(c->opt[layer]);
+        launch_descend<double>((long long)n, dp, ctrl_of<double>(c, layer), gctrl_of<double>(c, layer), st, st + n, st + 2 * n, c->stream);
+    }
+    HIP_TRY(hipGetLastError());
+    // the control changed: layers after `layer` are stale (as dmpm_set_control)
+    c->fwd_valid = std::min(c->fwd_valid, (int)layer);
+    c->loss_valid = false;
+    if (c->cfg.tape_mode != DMPM_TAPE_STORE_ALL && c->tape_seg >= 0 && c->tape_seg * c->K + c->K > layer) c->tape_seg = -1;
     return DMPM_OK;
 }
 

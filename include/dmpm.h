@@ -155,8 +155,19 @@ int dmpm_set_layer_grads(dmpm_ctx* ctx, int32_t layer, const double* dLdx, const
 /* dL/d(dFc_layer): the control layer of the last backward pass, or any of its layers with
  * keep_control_grads.  Exact mode: dL/d(dFc); compat mode: the reference's dLdF. */
 int dmpm_get_control_grad(dmpm_ctx* ctx, int32_t layer, double* out);
-/* PointCloud::Compute_dLdF_Norm (PointCloud.cpp:24-35) of a control gradient, FP64 reduction. */
+/* PointCloud::Compute_dLdF_Norm (PointCloud.cpp:24-35) of a control gradient, FP64 reduction on the device. */
 int dmpm_control_grad_norm(dmpm_ctx* ctx, int32_t layer, double* norm);
+/* PointCloud::Descend_Adam / Descend_AMSGrad / Descend_Lion / Descend_GradientDescent
+ * (PointCloud.cpp:83-179) on the device: dFc of `layer` updated from its control gradient of the
+ * last backward pass (dmpm_get_control_grad) divided by gradient_norm, with the per-particle
+ * momentum / vector / vector_max of that layer (zero at first use, MaterialPoint.h).  Float
+ * arithmetic as the reference; timestep t enters as 1 - beta^t.  Invalidates later layers. */
+#define DMPM_DESCEND_ADAM 0
+#define DMPM_DESCEND_AMSGRAD 1
+#define DMPM_DESCEND_LION 2 /* beta1 is Lion's beta */
+#define DMPM_DESCEND_GD 3
+int dmpm_descend(dmpm_ctx* ctx, int32_t layer, int32_t rule, float alpha, float gradient_norm, float beta1, float beta2,
+                 float epsilon, int32_t timestep);
 /* Grid of one step after G_op (node m and v), store-all tape. */
 int dmpm_get_grid(dmpm_ctx* ctx, int32_t step, double* m, double* v);
 

