@@ -826,8 +826,8 @@ __global__ __launch_bounds__(kGat) void k_grid_adj_b(Batch bt, const float* grid
 
 // EndLayerMassLoss node terms (CompGraph.cpp:143-187) in FP64.  blockIdx.y is
 // the instance; gm and target hold one node mass per node (instance i at
-// nofs[i]); per-instance loss sums go to acc[i].
-__global__ __launch_bounds__(kGat) void k_loss_b(Batch bt, const float* gm, const float* target, double* dLdm, double* acc)
+// nofs[i]); workgroup partial sums go to part[instance * gridDim.x + x].
+__global__ __launch_bounds__(kGat) void k_loss_b(Batch bt, const float* gm, const float* target, double* dLdm, double* part)
 {
     __shared__ double red[kGat / 64];
     const int inst = blockIdx.y;
@@ -855,7 +855,7 @@ __global__ __launch_bounds__(kGat) void k_loss_b(Batch bt, const float* gm, cons
         double sum = 0.0;
 #pragma unroll
         for (int w = 0; w < kGat / 64; ++w) sum += red[w];
-        atomicAdd(acc + inst, sum);
+        part[(long long)inst * gridDim.x + blockIdx.x] = sum;
     }
 }
 
@@ -939,14 +939,16 @@ void launch_grid_adj_b(const Batch& b, const float* grid, float* agrid, hipStrea
     hipLaunchKernelGGL(k_grid_adj_b, dim3((unsigned)((b.Gtot + kGat - 1) / kGat)), dim3(kGat), 0, s, b, grid, agrid);
 }
 
-void launch_loss_b(const Batch& b, const float* gm, const float* target, double* dLdm, double* acc, long long gmax, hipStream_t s)
+void launch_loss_b(const Batch& b, const float* gm, const float* target, double* dLdm, double* part, double* acc, long long gmax,
+                   hipStream_t s)
 {
     if (b.Gtot <= 0 || b.B <= 0) return;
     long long nb = (gmax + kGat - 1) / kGat;
-    const long long lim = 4096 / b.B > 16 ? 4096 / b.B : 16;
+    const long long lim = loss_partials(b.B) / b.B;
     if (nb > lim) nb = lim;
     if (nb < 1) nb = 1;
-    hipLaunchKernelGGL(k_loss_b, dim3((unsigned)nb, (unsigned)b.B), dim3(kGat), 0, s, b, gm, target, dLdm, acc);
+    hipLaunchKernelGGL(k_loss_b, dim3((unsigned)nb, (unsigned)b.B), dim3(kGat), 0, s, b, gm, target, dLdm, part);
+    launch_sum_rows(part, b.B, (int)nb, acc, s);
 }
 
 void launch_loss_particles_b(const Batch& b, const float* layer, const float* cst, const double* dLdm, float* g, hipStream_t s)

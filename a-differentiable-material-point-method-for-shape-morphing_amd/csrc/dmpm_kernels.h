@@ -54,6 +54,9 @@ struct AdjIO {
     Real* tmp;           // per particle vt (3) and Ct (9), from the transposed G2P
 };
 
+// most workgroups of a one-row FP64 reduction (loss, norm)
+constexpr int kLossBlocks = 2048;
+
 // control update of one layer (PointCloud::Descend_*): rule 0 Adam, 1 AMSGrad, 2 Lion, 3 gradient descent
 struct DescendParams {
     int rule;
@@ -64,14 +67,19 @@ struct DescendParams {
 template <typename Real> void launch_fill(Real* p, long long count, Real value, hipStream_t s);
 template <typename Real> void launch_descend(long long count, const DescendParams& dp, Real* dFc, const Real* g, Real* mom,
                                              Real* vec, Real* vmax, hipStream_t s);
-template <typename Real> void launch_sumsq(const Real* a, long long count, double* acc, hipStream_t s);
+// sum of squares in FP64 (partials in part, at most kLossBlocks), fixed-order total in *out
+template <typename Real> void launch_sumsq(const Real* a, long long count, double* part, double* out, hipStream_t s);
 template <typename Real> void launch_p2g(const StepParams& sp, const StepIO<Real>& io, hipStream_t s);
 template <typename Real> void launch_grid_update(const StepParams& sp, Real* grid, hipStream_t s);
 template <typename Real> void launch_g2p(const StepParams& sp, const StepIO<Real>& io, hipStream_t s);
 template <typename Real> void launch_p2g_mass(const StepParams& sp, const Real* x, const Real* m, int stride, Real* gm, hipStream_t s);
 template <typename Real> void launch_volumes(const StepParams& sp, const Real* x, Real* cst, const Real* gm, hipStream_t s);
+// loss node terms: dL/dm in FP64, per-workgroup partial sums (at most
+// kLossBlocks) in part, their fixed-order sum in *loss
 template <typename Real> void launch_loss(const StepParams& sp, const Real* gm, const Real* target, double* dLdm,
-                                          double* loss_acc, hipStream_t s);
+                                          double* part, double* loss, hipStream_t s);
+// out[r] = sum of part[r * len .. r * len + len) in a fixed order
+void launch_sum_rows(const double* part, int rows, int len, double* out, hipStream_t s);
 template <typename Real> void launch_loss_particles(const StepParams& sp, const Real* layer, const Real* cst,
                                                     const double* dLdm, Real* g, hipStream_t s);
 template <typename Real> void launch_g2p_adj(const StepParams& sp, const AdjIO<Real>& io, hipStream_t s);
@@ -102,9 +110,12 @@ void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s);
 void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s);
 void launch_grid_update_b(const Batch& b, float* grid, hipStream_t s);
 void launch_grid_adj_b(const Batch& b, const float* grid, float* agrid, hipStream_t s);
-// loss over all instances from node masses gm (instance i at nofs[i]);
-// per-instance sums added to acc[0..B); gmax = largest instance node count
-void launch_loss_b(const Batch& b, const float* gm, const float* target, double* dLdm, double* acc, long long gmax, hipStream_t s);
+// loss over all instances from node masses gm (instance i at nofs[i]):
+// per-instance losses in acc[0..B) (fixed-order sums of workgroup partials
+// kept in part, at most loss_partials(B) doubles); gmax = largest instance node count
+inline long long loss_partials(int B) { return (long long)B * (4096 / B > 16 ? 4096 / B : 16); }
+void launch_loss_b(const Batch& b, const float* gm, const float* target, double* dLdm, double* part, double* acc, long long gmax,
+                   hipStream_t s);
 void launch_loss_particles_b(const Batch& b, const float* layer, const float* cst, const double* dLdm, float* g, hipStream_t s);
 
 // batched constitutive evaluations (AoS 9 doubles per matrix)

@@ -341,7 +341,7 @@ void launch_volumes(const StepParams& sp, const Real* x, Real* cst, const Real* 
 // soft minimum, dL/dm ~ -0.02 + O(m), and the particle gather below relies on
 // sum_i grad w = 0 to cancel the constant part exactly.
 template <typename Real>
-__global__ __launch_bounds__(kBlock) void k_loss(StepParams sp, const Real* gm, const Real* target, double* dLdm, double* acc)
+__global__ __launch_bounds__(kBlock) void k_loss(StepParams sp, const Real* gm, const Real* target, double* dLdm, double* part)
 {
     __shared__ double red[kBlock / 64];
     double local = 0.0;
@@ -367,16 +367,45 @@ __global__ __launch_bounds__(kBlock) void k_loss(StepParams sp, const Real* gm, 
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < kBlock / 64; ++w) s += red[w];
-        atomicAdd(acc, s);
+        part[blockIdx.x] = s;
     }
 }
 
+// Row sums of partials in a fixed order (one workgroup per row): the loss is
+// the same bit for bit whenever the node masses are.
+__global__ __launch_bounds__(kBlock) void k_sum_rows(const double* part, int len, double* out)
+{
+    __shared__ double red[kBlock / 64];
+    const double* row = part + (long long)blockIdx.x * len;
+    double local = 0.0;
+    for (int i = threadIdx.x; i < len; i += blockDim.x) local += row[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) s += red[w];
+        out[blockIdx.x] = s;
+    }
+}
+
+void launch_sum_rows(const double* part, int rows, int len, double* out, hipStream_t s)
+{
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(k_sum_rows, dim3(rows), dim3(kBlock), 0, s, part, len, out);
+}
+
 template <typename Real>
-void launch_loss(const StepParams& sp, const Real* gm, const Real* target, double* dLdm, double* loss_acc, hipStream_t s)
+void launch_loss(const StepParams& sp, const Real* gm, const Real* target, double* dLdm, double* part, double* loss, hipStream_t s)
 {
     unsigned nb = nblocks(sp.G, kBlock);
-    if (nb > 2048) nb = 2048;
-    hipLaunchKernelGGL(k_loss<Real>, dim3(nb), dim3(kBlock), 0, s, sp, gm, target, dLdm, loss_acc);
+    if (nb > kLossBlocks) nb = kLossBlocks;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_loss<Real>, dim3(nb), dim3(kBlock), 0, s, sp, gm, target, dLdm, part);
+    launch_sum_rows(part, 1, (int)nb, loss, s);
 }
 
 // CompGraph.cpp:192-241: dLdx = sum_i m dLdm_i grad w; dLdv = dLdF = dLdC = 0.
@@ -805,9 +834,10 @@ void launch_descend(long long count, const DescendParams& dp, Real* dFc, const R
     hipLaunchKernelGGL(k_descend<Real>, dim3(nblocks(count, kBlock)), dim3(kBlock), 0, s, count, dp, dFc, g, mom, vec, vmax);
 }
 
-// sum of squares in FP64 (PointCloud::Compute_dLdF_Norm, PointCloud.cpp:24-35), added to acc
+// sum of squares in FP64 (PointCloud::Compute_dLdF_Norm, PointCloud.cpp:24-35):
+// workgroup partials in part, their fixed-order sum in *out
 template <typename Real>
-__global__ __launch_bounds__(kBlock) void k_sumsq(const Real* a, long long count, double* acc)
+__global__ __launch_bounds__(kBlock) void k_sumsq(const Real* a, long long count, double* part)
 {
     __shared__ double red[kBlock / 64];
     double local = 0.0;
@@ -824,29 +854,29 @@ __global__ __launch_bounds__(kBlock) void k_sumsq(const Real* a, long long count
         double t = 0.0;
 #pragma unroll
         for (int w = 0; w < kBlock / 64; ++w) t += red[w];
-        atomicAdd(acc, t);
+        part[blockIdx.x] = t;
     }
 }
 
 template <typename Real>
-void launch_sumsq(const Real* a, long long count, double* acc, hipStream_t s)
+void launch_sumsq(const Real* a, long long count, double* part, double* out, hipStream_t s)
 {
-    if (count <= 0) return;
-    unsigned nb = nblocks(count, kBlock);
-    if (nb > 1024) nb = 1024;
-    hipLaunchKernelGGL(k_sumsq<Real>, dim3(nb), dim3(kBlock), 0, s, a, count, acc);
+    unsigned nb = count > 0 ? nblocks(count, kBlock) : 1;
+    if (nb > kLossBlocks) nb = kLossBlocks;
+    hipLaunchKernelGGL(k_sumsq<Real>, dim3(nb), dim3(kBlock), 0, s, a, count, part);
+    launch_sum_rows(part, 1, (int)nb, out, s);
 }
 
 #define DMPM_INSTANTIATE(R)                                                                                        \
     template void launch_descend<R>(long long, const DescendParams&, R*, const R*, R*, R*, R*, hipStream_t);      \
-    template void launch_sumsq<R>(const R*, long long, double*, hipStream_t);                                      \
+    template void launch_sumsq<R>(const R*, long long, double*, double*, hipStream_t);                             \
     template void launch_fill<R>(R*, long long, R, hipStream_t);                                                   \
     template void launch_p2g<R>(const StepParams&, const StepIO<R>&, hipStream_t);                                 \
     template void launch_grid_update<R>(const StepParams&, R*, hipStream_t);                                       \
     template void launch_g2p<R>(const StepParams&, const StepIO<R>&, hipStream_t);                                 \
     template void launch_p2g_mass<R>(const StepParams&, const R*, const R*, int, R*, hipStream_t);                 \
     template void launch_volumes<R>(const StepParams&, const R*, R*, const R*, hipStream_t);                       \
-    template void launch_loss<R>(const StepParams&, const R*, const R*, double*, double*, hipStream_t);                 \
+    template void launch_loss<R>(const StepParams&, const R*, const R*, double*, double*, double*, hipStream_t);        \
     template void launch_loss_particles<R>(const StepParams&, const R*, const R*, const double*, R*, hipStream_t);      \
     template void launch_g2p_adj<R>(const StepParams&, const AdjIO<R>&, hipStream_t);                              \
     template void launch_grid_adj<R>(const StepParams&, const AdjIO<R>&, hipStream_t);                             \

@@ -72,6 +72,7 @@ struct dmpm_ctx {
     void* tmp = nullptr;      // 12 x S
     void* target = nullptr;   // G
     double* dacc = nullptr;   // FP64 accumulators: per-instance losses [0, B), norm at B
+    double* dpart = nullptr;  // per-workgroup partial sums of the loss
     std::vector<void*> opt;   // per control layer: optimizer state m, v, vmax (27 x S), allocated on first use
     uint64_t bytes = 0;
     // instances: particles of instance i are API/device slots [poff[i], poff[i+1]),
@@ -425,13 +426,13 @@ void loss_kernels(dmpm_ctx* c, const double* lay, const double* gm, double* dLdm
 {
     const StepParams sp = step_params(c);
     const double* cst = static_cast<const double*>(c->cst);
-    launch_loss<double>(sp, gm, static_cast<const double*>(c->target), dLdm, c->dacc, c->stream);
+    launch_loss<double>(sp, gm, static_cast<const double*>(c->target), dLdm, c->dpart, c->dacc, c->stream);
     launch_loss_particles<double>(sp, lay, cst, dLdm, grads_of<double>(c, c->L - 1), c->stream);
 }
 
 void loss_kernels(dmpm_ctx* c, const float* lay, const float* gm, double* dLdm)
 {
-    launch_loss_b(c->batch, gm, static_cast<const float*>(c->target), dLdm, c->dacc, c->gmax, c->stream);
+    launch_loss_b(c->batch, gm, static_cast<const float*>(c->target), dLdm, c->dpart, c->dacc, c->gmax, c->stream);
     launch_loss_particles_b(c->batch, lay, static_cast<const float*>(c->cst), dLdm, grads_of<float>(c, c->L - 1), c->stream);
 }
 
@@ -447,7 +448,6 @@ int end_loss(dmpm_ctx* c, double* loss)
     // 4G-real work grid (8-byte aligned for both precisions)
     double* dLdm = reinterpret_cast<double*>(gm + 2 * c->G);
     fill<Real>(c, gm, c->G, Real(0));
-    launch_fill<double>(c->dacc, c->B, 0.0, c->stream);
     const Real* cst = static_cast<const Real*>(c->cst);
     timed(c, DMPM_K_LOSS, [&] {
         for (int i = 0; i < c->B; ++i)
@@ -968,6 +968,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         (rc = dalloc(c, &c->gbuf[1], kLayerFields * S * e)) || (rc = dalloc(c, &c->gctrl, gctrl_layers * 9 * S * e)) ||
         (rc = dalloc(c, &c->tmp, 12 * S * e)) || (rc = dalloc(c, &c->target, G * e)) ||
         (rc = dalloc(c, (void**)&c->dacc, std::max(64, c->B + 8) * sizeof(double))) ||
+        (rc = dalloc(c, (void**)&c->dpart, std::max<long long>(kLossBlocks, loss_partials(c->B)) * sizeof(double))) ||
         (rc = dalloc(c, (void**)&c->d_sps, c->B * sizeof(StepParams))) ||
         (rc = dalloc(c, (void**)&c->d_nofs, (c->B + 1) * sizeof(long long)))) {
         std::string msg = g_err;
@@ -1044,7 +1045,7 @@ int dmpm_destroy(dmpm_ctx* c)
     if (c->stream) hipStreamSynchronize(c->stream);
     clear_graphs(c);
     void* bufs[] = {c->cst,  c->ctrl, c->layers, c->grids, c->wgrid, c->agrid, c->gbuf[0], c->gbuf[1],
-                    c->gctrl, c->tmp, c->target, c->dacc,  c->ranges, c->d_sps, c->d_nofs};
+                    c->gctrl, c->tmp, c->target, c->dacc,  c->ranges, c->d_sps, c->d_nofs, c->dpart};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (void* b : c->opt)
@@ -1261,9 +1262,8 @@ int dmpm_control_grad_norm(dmpm_ctx* c, int32_t layer, double* norm)
     if (!norm) return fail(DMPM_ERR_INVALID, "null output");
     if ((rc = check_control_grad(c, layer))) return rc;
     double* acc = c->dacc + c->B;
-    launch_fill<double>(acc, 1, 0.0, c->stream);
-    if (c->f32) launch_sumsq<float>(gctrl_of<float>(c, layer), 9LL * c->S, acc, c->stream);
-    else launch_sumsq<double>(gctrl_of<double>(c, layer), 9LL * c->S, acc, c->stream);
+    if (c->f32) launch_sumsq<float>(gctrl_of<float>(c, layer), 9LL * c->S, c->dpart, acc, c->stream);
+    else launch_sumsq<double>(gctrl_of<double>(c, layer), 9LL * c->S, c->dpart, acc, c->stream);
     HIP_TRY(hipGetLastError());
     double h = 0.0;
     HIP_TRY(hipMemcpyAsync(&h, acc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
