@@ -41,14 +41,33 @@ OPT_RANGE_MAX, OPT_LDS_CAP = 2, 3
 # Algorithmic HBM bytes per FP32 particle / per touched node / per dense grid node
 # of each kernel (FP64: twice).  Particle fields: x3 v3 C9 F9 dFc9 + m vol lam mu.
 KBYTES = {
-    "p2g": (148, 16, 0),        # read x,v,C,F,dFc,m,vol,lam,mu; m,p scatter per node
+    "p2g": (184, 16, 0),        # read x,v,C,F,dFc,m,vol,lam,mu, write the affine matrix A; m,p scatter per node
     "grid_update": (0, 0, 28),  # read m,p write v per node (dense)
     "g2p": (180, 12, 0),        # read x,F,dFc write x,v,C,F; gather v
     "g2p_adj": (228, 12, 0),    # read x,F,dFc,x',v',F',C' write vt,Ct; scatter vbar
     "grid_adj": (0, 0, 44),     # read m,v,vbar write pbar,mbar (dense)
-    "p2g_adj": (412, 28, 0),    # read x,v,C,F,dFc,consts,C',F',x',vt,Ct write grads; gather pbar,mbar,v
+    "p2g_adj": (412, 28, 0),    # read m,A,x,v,vt,Ct,x',F,dFc,vol,lam,mu,C',F' write x,v,C,F grads + dL/d(dFc); gather pbar,mbar,v
 }
 HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes of one forward + one adjoint particle-step (FP32): per particle
+# 328 + 376, per touched node 56 + 72, per dense grid node 44 (reset + update) + 56
+# (adjoint reset + transposed update).
+STEP_BYTES = (704, 128, 100)
+# measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a
+# device-to-device copy of known size) of the 1M-particle scene's kernels
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_morph_1m_128.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` from the committed counter summary, or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as fh:
+        for line in fh:
+            f = line.split()
+            if f and not f[0].startswith("#") and f[0] == kernel:
+                return int(f[1])
+    return None
 
 
 def touched_nodes(sc):
@@ -258,8 +277,10 @@ def main():
         bytes_per_launch = pb * sc.n + nb * touched + gb * sc.G
         avg_s = ms / 1e3 / max(launches, 1)
         achieved = bytes_per_launch / avg_s / 1e9
+        traffic = measured_traffic(dom) if sc.name == "morph_1m_128" else None
         roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": bytes_per_launch}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch": bytes_per_launch,
+                "traffic_source": "profiles/traffic_morph_1m_128.txt (rocprofv3 FETCH_SIZE/WRITE_SIZE)" if traffic else None}
         kernels = {k: {"ms_total": round(prof[k][0], 3), "launches": prof[k][1],
                        "avg_us": round(prof[k][0] * 1e3 / max(prof[k][1], 1), 2)} for k in prof}
 
@@ -294,6 +315,13 @@ def main():
                    "dt": args.dt, "volume": "reference formula x dx^3", "control": "dFc ~ N(0,1e-2) on layer 0"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "step_roofline": {"bytes_per_particle_step": (STEP_BYTES[0] * sc.n + STEP_BYTES[1] * touched + STEP_BYTES[2] * sc.G)
+                          / sc.n,
+                          "achieved_GBs": (STEP_BYTES[0] * sc.n + STEP_BYTES[1] * touched + STEP_BYTES[2] * sc.G)
+                          * args.episode_steps * args.steps / r32["time"] / 1e9,
+                          "frac": (STEP_BYTES[0] * sc.n + STEP_BYTES[1] * touched + STEP_BYTES[2] * sc.G)
+                          * args.episode_steps * args.steps / r32["time"] / 1e9 / HBM_PEAK_GBS,
+                          "note": "one forward + one adjoint per particle-step; recompute counts in the time"},
         "kernels_fp32": kernels,
         "kernel_timing": f"HIP events around every launch of {args.prof_steps} eager episodes after the timed region",
         "tuned": r32["tuned"],
