@@ -217,7 +217,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=25)
     ap.add_argument("--prof-steps", type=int, default=2)
     ap.add_argument("--store-all-steps", type=int, default=3)
     ap.add_argument("--batched-steps", type=int, default=3)
