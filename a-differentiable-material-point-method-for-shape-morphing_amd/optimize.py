@@ -67,11 +67,15 @@ def optimize_control_sequence(mo, control_stride, max_gd_iters, max_line_search_
     return initial_loss, final_loss, history
 
 
-def gradient_descent(mo, iters, alpha, control_layer=0, max_line_search_iters=8, log=None):
-    """``iters`` iterations of normalised gradient descent on the control of
-    ``control_layer`` (Descend_GradientDescent, PointCloud.cpp:164-179) with a
-    halving line search that keeps the last accepted step size.  Returns the
-    loss after every iteration (index 0: the initial loss)."""
+def gradient_descent(mo, iters, alpha, control_layer=0, max_line_search_iters=8, log=None, rule=DESCEND_GD,
+                     beta1=0.9, beta2=0.999, epsilon=1e-5):
+    """``iters`` iterations of gradient descent on the control of
+    ``control_layer`` with a halving line search that keeps the last accepted
+    step size.  ``rule`` DESCEND_GD is Descend_GradientDescent
+    (PointCloud.cpp:164-179: step alpha g / |g|); DESCEND_ADAM is
+    Descend_Adam (PointCloud.cpp:83-99) with the timestep advanced per accepted
+    step and a failed trial undone by the reference's own -alpha call.
+    Returns the loss after every iteration (index 0: the initial loss)."""
     mo.forward_pass(0)
     loss = mo.end_layer_mass_loss()
     losses = [loss]
@@ -82,14 +86,14 @@ def gradient_descent(mo, iters, alpha, control_layer=0, max_line_search_iters=8,
             break
         accepted = False
         for _ in range(max_line_search_iters):
-            mo.descend(control_layer, DESCEND_GD, alpha, gn)
+            mo.descend(control_layer, rule, alpha, gn, beta1, beta2, epsilon, it + 1)
             mo.forward_pass(control_layer)
             trial = mo.end_layer_mass_loss()
             if trial < loss:
                 loss = trial
                 accepted = True
                 break
-            mo.descend(control_layer, DESCEND_GD, -alpha, gn)
+            mo.descend(control_layer, rule, -alpha, gn, beta1, beta2, epsilon, it + 1)
             alpha *= 0.5
         if not accepted:
             mo.forward_pass(control_layer)

@@ -171,6 +171,33 @@ def run_batched(pkg, count, L, dt, warmup, steps):
             "tuned": r["tuned"], "loss": r["loss"]}
 
 
+def loss_curves(pkg, sc, vol, dFc, L, args):
+    """Descent on the layer-0 control from the same seed on the FP32+checkpoint and
+    plain FP64 (checkpointed, to fit) paths; one JSON line with both loss curves."""
+    out = {"workload": sc.name, "particles": sc.n, "grid": list(sc.dim), "episode_steps": L - 1, "dt": sc.dt,
+           "rho": args.rho,
+           "iterations": args.gd_iters, "rule": "Descend_Adam with halving line search", "alpha": args.gd_alpha}
+    paths = {"f32": (pkg.PREC_F32, "fp32_ckpt"), "f64": (pkg.PREC_F64, "fp64_ckpt")}
+    for key in (["f32", "f64"] if args.gd_precision == "both" else [args.gd_precision]):
+        prec, name = paths[key]
+        mo = setup(pkg, sc, vol, dFc, prec, pkg.TAPE_CHECKPOINT, L)
+        if prec == pkg.PREC_F32:
+            mo.autotune()
+            mo.set_graphs(True)
+        t0 = time.perf_counter()
+        curve = pkg.optimize.gradient_descent(
+            mo, args.gd_iters, args.gd_alpha, rule=pkg.DESCEND_ADAM,
+            log=lambda m, n=name: print(f"# {n} {m}", file=sys.stderr, flush=True))
+        mo.synchronize()
+        out[name] = {"loss": curve, "seconds": time.perf_counter() - t0, "memory_bytes": mo.memory_bytes()}
+        mo.close()
+    if "fp32_ckpt" in out and "fp64_ckpt" in out:
+        a, b = np.array(out["fp32_ckpt"]["loss"]), np.array(out["fp64_ckpt"]["loss"])
+        k = min(len(a), len(b))
+        out["max_rel_diff_fp32_vs_fp64"] = float(np.max(np.abs(a[:k] - b[:k]) / np.abs(b[:k])))
+    print(json.dumps(out))
+
+
 def cpu_baseline(sc, vol, dFc, threads, steps):
     from oracle import oracle as orc
 
@@ -223,6 +250,11 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=3)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--gd-iters", type=int, default=0,
+                    help="loss-curve mode: this many descent iterations on the layer-0 control (FP32+ckpt and FP64)")
+    ap.add_argument("--gd-precision", default="both", choices=["both", "f32", "f64"])
+    ap.add_argument("--gd-alpha", type=float, default=1e-3)
+    ap.add_argument("--rho", type=float, default=1.0, help="density (scales particle and target masses)")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a D2D copy of this many bytes")
     args = ap.parse_args()
     if args.copy_probe:
@@ -235,6 +267,9 @@ def main():
 
     pkg = _load_pkg()
     sc = make_scene(pkg, args.workload, args.dt)
+    if args.rho != 1.0:
+        sc.m = sc.m * args.rho
+        sc.target_m = sc.target_m * args.rho
     L = args.episode_steps + 1
     # reference-formula volumes (computed on the GPU) scaled by dx^3
     mo = pkg.Morph.from_scene(sc, num_layers=2)
@@ -244,6 +279,9 @@ def main():
     vol = mo.volumes() * sc.dx ** 3
     mo.close()
     dFc = pkg.scenes.random_controls(sc.n, 1e-2, 1)
+    if args.gd_iters:
+        loss_curves(pkg, sc, vol, dFc, L, args)
+        return
     touched = touched_nodes(sc)
     print(f"# workload {sc.name}: n={sc.n} grid={sc.dim} L={L} touched_nodes={touched}", file=sys.stderr, flush=True)
 
