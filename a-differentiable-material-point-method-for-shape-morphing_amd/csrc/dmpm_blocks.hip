@@ -42,6 +42,11 @@ namespace {
 constexpr int kGat = 256;    // gather kernels
 constexpr int kScat = 512;   // scatter kernels: one particle per thread of a block range
 constexpr int kMaxWaves = 8;
+// The elastic adjoint runs as a kernel of its own after the transposed P2G
+// gather (Gb through tmp): the gather then needs 167 VGPRs instead of 253 and
+// the pair of kernels measured 177 us against 189 for the fused one (1M
+// particles, 128^3).  true fuses them.
+constexpr bool kFuseElastic = false;
 constexpr int kFW = 8;       // fixed window edge (nodes)
 constexpr int kFW3 = kFW * kFW * kFW;
 // LDS layout of the fixed window: node (i, j, k) at slot i*kSI + j*kSJ + k.
@@ -612,6 +617,58 @@ __global__ __launch_bounds__(kScat) void k_g2p_adj_blk(Batch bt, AdjIO<float> io
     }
 }
 
+// Elastic half of P2G^T + P_op_1^T for particle p (FP64 polar, PK1, H:dF):
+//   Pbar = -k Gb Fc,  Fcbar = (1-s)(I + dt C')^T F' - k Gb^T P + H(Fc):Pbar.
+__device__ __forceinline__ void elastic_adj(const StepParams& sp, const AdjIO<float>& io, int p, const M3<float>& Gb)
+{
+    const int S = sp.stride;
+    const int cp = sp.compat;
+    const float dt = float(sp.dt);
+    const double kst = -3.0 * sp.inv_dx * sp.inv_dx * sp.dt;
+    const M3<float> F = ldm(io.cur, FF, S, p);
+    const M3<float> dF = ldm(io.ctrl, 0, S, p);
+    const double vol = io.cst[KVOL * S + p];
+    const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
+    M3<double> Fc;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
+    FCE e;
+    fce_prepare(Fc, e);
+    const M3<double> P = fce_pk1(Fc, e, lam, mu);
+    const double kk_ = -kst * vol;  // 3 dt vol / dx^2
+    const M3<double> Gbd = cast<double>(Gb);
+    M3<double> Pb = mul(Gbd, Fc);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Pb.a[i] *= -kk_;
+    const M3<double> H = fce_hess_mult(e, lam, mu, Pb, cp == 0);
+    const M3<double> GTP = tmul(Gbd, P);
+    const M3<float> C1 = ldm(io.next, FC, S, p);
+    const M3<float> gF1 = ldm(io.g1, FF, S, p);
+    M3<float> IC;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) IC.a[i] = dt * C1.a[i];
+    IC.a[0] += 1.f;
+    IC.a[4] += 1.f;
+    IC.a[8] += 1.f;
+    const M3<float> base = tmul(IC, gF1);
+    const float s = float(sp.smoothing);
+    M3<float> gF, gdFc;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const float rest = float(-kk_ * GTP.a[i] + H.a[i]);
+        if (cp & 1) {
+            const float f = (1.f - s) * base.a[i] + s * gF1.a[i] + base.a[i] + rest;
+            gF.a[i] = f;
+            gdFc.a[i] = f;
+        } else {
+            gdFc.a[i] = (1.f - s) * base.a[i] + rest;
+            gF.a[i] = gdFc.a[i] + s * gF1.a[i];
+        }
+    }
+    stm(io.g0, FF, S, p, gF);
+    stm(io.gctrl, 0, S, p, gdFc);
+}
+
 // ------------------------------------------------------- transposed P2G ----
 // P2G^T + P_op_1^T, one thread per particle of the range.  With the P2G
 // affine matrix G = -k P Fc^T + m C (cached in the layer by P2G) and
@@ -621,6 +678,7 @@ __global__ __launch_bounds__(kScat) void k_g2p_adj_blk(Batch bt, AdjIO<float> io
 //                    - w (G^T pbar + D Ct^T v_i)],
 // then the elastic part (FP64 polar, PK1 and H:dF):
 //   Pbar = -k Gb Fc,  Fcbar = (1-s)(I + dt C')^T F' - k Gb^T P + H(Fc):Pbar.
+template <bool FUSE>
 __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
 {
     extern __shared__ float lds[];
@@ -713,50 +771,18 @@ __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
 #pragma unroll
         for (int i = 0; i < 9; ++i) gC.a[i] = m * Gb.a[i];
         stm(io.g0, FC, S, p, gC);
-        // elastic part
-        const M3<float> F = ldm(io.cur, FF, S, p);
-        const M3<float> dF = ldm(io.ctrl, 0, S, p);
-        const double vol = io.cst[KVOL * S + p];
-        const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
-        M3<double> Fc;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
-        FCE e;
-        fce_prepare(Fc, e);
-        const M3<double> P = fce_pk1(Fc, e, lam, mu);
-        const double kk_ = -kst * vol;  // 3 dt vol / dx^2
-        const M3<double> Gbd = cast<double>(Gb);
-        M3<double> Pb = mul(Gbd, Fc);
-#pragma unroll
-        for (int i = 0; i < 9; ++i) Pb.a[i] *= -kk_;
-        const M3<double> H = fce_hess_mult(e, lam, mu, Pb, cp == 0);
-        const M3<double> GTP = tmul(Gbd, P);
-        const M3<float> C1 = ldm(io.next, FC, S, p);
-        const M3<float> gF1 = ldm(io.g1, FF, S, p);
-        M3<float> IC;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) IC.a[i] = dt * C1.a[i];
-        IC.a[0] += 1.f;
-        IC.a[4] += 1.f;
-        IC.a[8] += 1.f;
-        const M3<float> base = tmul(IC, gF1);
-        const float s = float(sp.smoothing);
-        M3<float> gF, gdFc;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            const float rest = float(-kk_ * GTP.a[i] + H.a[i]);
-            if (cp & 1) {
-                const float f = (1.f - s) * base.a[i] + s * gF1.a[i] + base.a[i] + rest;
-                gF.a[i] = f;
-                gdFc.a[i] = f;
-            } else {
-                gdFc.a[i] = (1.f - s) * base.a[i] + rest;
-                gF.a[i] = gdFc.a[i] + s * gF1.a[i];
-            }
-        }
-        stm(io.g0, FF, S, p, gF);
-        stm(io.gctrl, 0, S, p, gdFc);
+        if (FUSE) elastic_adj(sp, io, p, Gb);
+        else stm(io.tmp, 0, S, p, Gb);  // vt, Ct are dead for this particle
     }
+}
+
+// Elastic half as a kernel of its own (Gb from tmp), one thread per particle of a range.
+__global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io)
+{
+    const int4 rg = bt.ranges[blockIdx.x];
+    const StepParams sp = bt.sps[rg.z];
+    const int S = sp.stride;
+    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) elastic_adj(sp, io, p, ldm(io.tmp, 0, S, p));
 }
 
 // --------------------------------------------------- batched grid kernels --
@@ -924,7 +950,12 @@ void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
 void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
 {
     if (b.nranges <= 0) return;
-    hipLaunchKernelGGL(k_p2g_adj_blk, dim3(b.nranges), dim3(kGat), 7 * win_nodes(b.cap) * sizeof(float), s, b, io);
+    if (kFuseElastic) {
+        hipLaunchKernelGGL(k_p2g_adj_blk<true>, dim3(b.nranges), dim3(kGat), 7 * win_nodes(b.cap) * sizeof(float), s, b, io);
+    } else {
+        hipLaunchKernelGGL(k_p2g_adj_blk<false>, dim3(b.nranges), dim3(kGat), 7 * win_nodes(b.cap) * sizeof(float), s, b, io);
+        hipLaunchKernelGGL(k_elastic_adj, dim3(b.nranges), dim3(kGat), 0, s, b, io);
+    }
 }
 
 void launch_grid_update_b(const Batch& b, float* grid, hipStream_t s)
