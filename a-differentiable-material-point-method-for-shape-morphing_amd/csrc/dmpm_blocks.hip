@@ -248,14 +248,21 @@ __device__ __forceinline__ unsigned long long fix32(float y)
     return (unsigned long long)(long long)(int)rintf(y);
 }
 
-template <int NC>
+// MASS: channel 0 is the node mass; a node whose window mass is zero adds
+// nothing (its other channels hold only contributions of weights below the
+// fixed-point resolution), so a node left at mass 0 keeps every channel at 0.
+// gm (optional): skip nodes whose mass there is 0 (the transposed G2P skips
+// nodes the forward P2G left empty: their pbar is 0 whatever vbar is).
+template <int NC, bool MASS>
 __device__ __forceinline__ void flush_fixed(const StepParams& sp, const Win& W, const unsigned long long* lds,
-                                            const Scales<NC>& S, float* base, long long cstride)
+                                            const Scales<NC>& S, float* base, long long cstride, const float* gm = nullptr)
 {
     if (W.fixed) {
         for (int idx = threadIdx.x; idx < kFW3; idx += blockDim.x) {
             const long long node = fixed_node(sp, W, idx);
             if (node < 0) continue;
+            if (MASS && lds[fixed_slot_of(idx)] == 0ull) continue;
+            if (gm && gm[node] == 0.f) continue;
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
                 const long long q = (long long)lds[c * kFWS + fixed_slot_of(idx)];
@@ -265,6 +272,8 @@ __device__ __forceinline__ void flush_fixed(const StepParams& sp, const Win& W, 
     } else {
         for (int idx = threadIdx.x; idx < W.vol; idx += blockDim.x) {
             const long long node = win_node(sp, W, idx);
+            if (MASS && lds[idx] == 0ull) continue;
+            if (gm && gm[node] == 0.f) continue;
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
                 const long long q = (long long)lds[c * W.vol + idx];
@@ -299,10 +308,11 @@ __device__ __forceinline__ void stm(T* base, int field, int S, int p, const M3<T
 
 // Scatter of NC channels over a particle's 64 nodes.  val(i, j, k, out)
 // fills the NC contributions (already in scaled units for the LDS modes).
-// MODE 0: fixed window, 1: general LDS window, 2: HBM float atomics.
+// MODE 0: fixed window, 1: general LDS window, 2: HBM float atomics (nodes
+// where gate is 0 are skipped there, as the LDS modes' flush skips them).
 template <int NC, int MODE, typename ValFn>
 __device__ __forceinline__ void scatter64(const StepParams& sp, const Win& W, const Axis<float> st[3],
-                                          unsigned long long* lds, float* hbm, ValFn&& val)
+                                          unsigned long long* lds, float* hbm, ValFn&& val, const float* gate = nullptr)
 {
     const int slot = MODE == 0 ? fixed_slot(W, st) : 0;
 #pragma unroll
@@ -329,6 +339,7 @@ __device__ __forceinline__ void scatter64(const StepParams& sp, const Win& W, co
                     for (int c = 0; c < NC; ++c) atomicAdd(&lds[c * W.vol + o], fix32(v[c]));
                 } else {
                     const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
+                    if (gate && gate[node] == 0.f) continue;
 #pragma unroll
                     for (int c = 0; c < NC; ++c) unsafeAtomicAdd(hbm + c * sp.G + node, v[c]);
                 }
@@ -466,7 +477,7 @@ __global__ __launch_bounds__(kScat, 4) void k_p2g_blk(Batch bt, StepIO<float> io
     }
     if (mode != 2) {
         __syncthreads();
-        flush_fixed<4>(sp, W, lds64, SC, io.grid, sp.G);
+        flush_fixed<4, true>(sp, W, lds64, SC, io.grid, sp.G);
     }
 }
 
@@ -609,11 +620,11 @@ __global__ __launch_bounds__(kScat) void k_g2p_adj_blk(Batch bt, AdjIO<float> io
         };
         if (mode == 0) scatter64<3, 0>(sp, W, st, lds64, io.agrid, val);
         else if (mode == 1) scatter64<3, 1>(sp, W, st, lds64, io.agrid, val);
-        else scatter64<3, 2>(sp, W, st, lds64, io.agrid, val);
+        else scatter64<3, 2>(sp, W, st, lds64, io.agrid, val, io.grid + 4 * bt.nofs[rg.z]);
     }
     if (mode != 2) {
         __syncthreads();
-        flush_fixed<3>(sp, W, lds64, SC, io.agrid, sp.G);
+        flush_fixed<3, false>(sp, W, lds64, SC, io.agrid, sp.G, io.grid + 4 * bt.nofs[rg.z]);
     }
 }
 
@@ -800,54 +811,84 @@ __device__ __forceinline__ int instance_of(const Batch& b, long long q)
 }
 }  // namespace
 
-// G_op (ForwardSimulation.cpp:53-58): v = p/m + dt f where |m| > FLT_EPSILON, else 0
+// G_op (ForwardSimulation.cpp:53-58): v = p/m + dt f where |m| > FLT_EPSILON,
+// else 0.  blockIdx.y is the instance; the x-blocks stride over its nodes.
 __global__ __launch_bounds__(kGat) void k_grid_update_b(Batch bt, float* grid)
 {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= bt.Gtot) return;
-    const int inst = instance_of(bt, q);
+    const int inst = blockIdx.y;
     const StepParams& sp = bt.sps[inst];
-    const long long l = q - bt.nofs[inst];
+    const long long G = sp.G;
     float* g = grid + 4 * bt.nofs[inst];
-    const float m = g[l];
-    if (fabsf(m) > float(kMassEps)) {
-        const float inv = 1.f / m;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) g[(1 + a) * sp.G + l] = g[(1 + a) * sp.G + l] * inv + float(sp.dt) * float(sp.f_ext[a]);
-    } else {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) g[(1 + a) * sp.G + l] = 0.f;
+    const float dt = float(sp.dt);
+    const float fx = dt * float(sp.f_ext[0]), fy = dt * float(sp.f_ext[1]), fz = dt * float(sp.f_ext[2]);
+    auto node = [&](long long l, float m) {
+        if (m == 0.f) return;  // no particle reached the node: p = 0, v stays 0
+        if (fabsf(m) > float(kMassEps)) {
+            const float inv = 1.f / m;
+            g[G + l] = g[G + l] * inv + fx;
+            g[2 * G + l] = g[2 * G + l] * inv + fy;
+            g[3 * G + l] = g[3 * G + l] * inv + fz;
+        } else {
+            g[G + l] = 0.f;
+            g[2 * G + l] = 0.f;
+            g[3 * G + l] = 0.f;
+        }
+    };
+    // the mass channel starts at a multiple of 4 nodes: read it 4 nodes at a time
+    const float4* m4 = reinterpret_cast<const float4*>(g);
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < G / 4; q += (long long)gridDim.x * blockDim.x) {
+        const float4 m = m4[q];
+        node(4 * q, m.x);
+        node(4 * q + 1, m.y);
+        node(4 * q + 2, m.z);
+        node(4 * q + 3, m.w);
     }
+    const long long t = (G / 4) * 4 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < G) node(t, g[t]);
 }
 
-// G_op^T (BackPropagation.cpp:46-59): pbar = vbar/m, mbar = -((v - dt f).vbar)/m
+// G_op^T (BackPropagation.cpp:46-59): pbar = vbar/m, mbar = -((v - dt f).vbar)/m.
+// Nodes of zero forward mass hold vbar = pbar = mbar = 0 already (fill, and
+// the transposed G2P flush skips them).
 __global__ __launch_bounds__(kGat) void k_grid_adj_b(Batch bt, const float* grid, float* agrid)
 {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= bt.Gtot) return;
-    const int inst = instance_of(bt, q);
+    const int inst = blockIdx.y;
     const StepParams& sp = bt.sps[inst];
-    const long long l = q - bt.nofs[inst];
+    const long long G = sp.G;
     const float* g = grid + 4 * bt.nofs[inst];
     float* ag = agrid + 4 * bt.nofs[inst];
-    const float m = g[l];
-    if (fabsf(m) > float(kMassEps)) {
-        const float inv = 1.f / m;
-        float dot = 0.f;
+    const float dt = float(sp.dt);
+    const bool fsub = !(sp.compat & 8);
+    auto node = [&](long long l, float m) {
+        if (m == 0.f) return;
+        if (fabsf(m) > float(kMassEps)) {
+            const float inv = 1.f / m;
+            float dot = 0.f;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            float vi = g[(1 + a) * sp.G + l];
-            if (!(sp.compat & 8)) vi -= float(sp.dt) * float(sp.f_ext[a]);
-            const float vb = ag[a * sp.G + l];
-            dot += vi * vb;
-            ag[a * sp.G + l] = vb * inv;
+            for (int a = 0; a < 3; ++a) {
+                float vi = g[(1 + a) * G + l];
+                if (fsub) vi -= dt * float(sp.f_ext[a]);
+                const float vb = ag[a * G + l];
+                dot += vi * vb;
+                ag[a * G + l] = vb * inv;
+            }
+            ag[3 * G + l] = -dot * inv;
+        } else {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) ag[a * G + l] = 0.f;
+            ag[3 * G + l] = 0.f;
         }
-        ag[3 * sp.G + l] = -dot * inv;
-    } else {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) ag[a * sp.G + l] = 0.f;
-        ag[3 * sp.G + l] = 0.f;
+    };
+    const float4* m4 = reinterpret_cast<const float4*>(g);
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < G / 4; q += (long long)gridDim.x * blockDim.x) {
+        const float4 m = m4[q];
+        node(4 * q, m.x);
+        node(4 * q + 1, m.y);
+        node(4 * q + 2, m.z);
+        node(4 * q + 3, m.w);
     }
+    const long long t = (G / 4) * 4 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < G) node(t, g[t]);
 }
 
 // EndLayerMassLoss node terms (CompGraph.cpp:143-187) in FP64.  blockIdx.y is
@@ -958,16 +999,25 @@ void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
     }
 }
 
+// x-blocks per instance of the node-parallel kernels: 4 nodes per thread
+static unsigned node_blocks(const Batch& b)
+{
+    long long nb = (b.gmax + 4LL * kGat - 1) / (4LL * kGat);
+    const long long lim = 8192 / b.B > 8 ? 8192 / b.B : 8;
+    if (nb > lim) nb = lim;
+    return (unsigned)(nb < 1 ? 1 : nb);
+}
+
 void launch_grid_update_b(const Batch& b, float* grid, hipStream_t s)
 {
     if (b.Gtot <= 0) return;
-    hipLaunchKernelGGL(k_grid_update_b, dim3((unsigned)((b.Gtot + kGat - 1) / kGat)), dim3(kGat), 0, s, b, grid);
+    hipLaunchKernelGGL(k_grid_update_b, dim3(node_blocks(b), (unsigned)b.B), dim3(kGat), 0, s, b, grid);
 }
 
 void launch_grid_adj_b(const Batch& b, const float* grid, float* agrid, hipStream_t s)
 {
     if (b.Gtot <= 0) return;
-    hipLaunchKernelGGL(k_grid_adj_b, dim3((unsigned)((b.Gtot + kGat - 1) / kGat)), dim3(kGat), 0, s, b, grid, agrid);
+    hipLaunchKernelGGL(k_grid_adj_b, dim3(node_blocks(b), (unsigned)b.B), dim3(kGat), 0, s, b, grid, agrid);
 }
 
 void launch_loss_b(const Batch& b, const float* gm, const float* target, double* dLdm, double* part, double* acc, long long gmax,

@@ -101,6 +101,7 @@ struct Batch {
     int B;
     long long Gtot;         // total nodes
     int cap;
+    long long gmax;         // nodes of the largest instance
 };
 // 4-channel grids (m or mbar, then 3 vector channels) of instance i start at 4 * nofs[i];
 // 1-channel arrays (target, dL/dm) at nofs[i].

@@ -38,10 +38,24 @@ __global__ __launch_bounds__(kBlock) void k_fill(Real* p, long long count, Real 
     for (; i < count; i += step) p[i] = value;
 }
 
+// 16-byte stores where p is 16-byte aligned (every buffer of this library is)
+__global__ __launch_bounds__(kBlock) void k_fill4(float4* p, long long count4, float value)
+{
+    const float4 v = make_float4(value, value, value, value);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count4; i += (long long)gridDim.x * blockDim.x) p[i] = v;
+}
+
 template <typename Real>
 void launch_fill(Real* p, long long count, Real value, hipStream_t s)
 {
     if (count <= 0) return;
+    if (sizeof(Real) == 4 && ((uintptr_t)p & 15) == 0 && (count & 3) == 0) {
+        const long long c4 = count / 4;
+        unsigned nb = nblocks(c4, kBlock);
+        if (nb > 8192) nb = 8192;
+        hipLaunchKernelGGL(k_fill4, dim3(nb), dim3(kBlock), 0, s, reinterpret_cast<float4*>(p), c4, float(value));
+        return;
+    }
     unsigned nb = nblocks(count, kBlock);
     if (nb > 8192) nb = 8192;
     hipLaunchKernelGGL(k_fill<Real>, dim3(nb), dim3(kBlock), 0, s, p, count, value);

@@ -337,8 +337,9 @@ void step_backward(dmpm_ctx* c, int i, const Real* cur, const Real* nxt, const R
     io.g0 = grads_of<Real>(c, i);
     io.gctrl = gctrl_of<Real>(c, i);
     io.tmp = static_cast<Real*>(c->tmp);
-    // vbar channels (one instance: the first 3G reals; a batch interleaves them per instance)
-    fill<Real>(c, io.agrid, (c->B > 1 ? kGridFields : 3) * c->G, Real(0));
+    // FP64: the vbar channels (the first 3G reals); FP32: all four channels
+    // (the batched transposed grid update leaves mbar of untouched nodes at 0)
+    fill<Real>(c, io.agrid, (c->f32 ? kGridFields : 3) * c->G, Real(0));
     timed(c, DMPM_K_G2P_ADJ, [&] { g2p_adj_dispatch(c, sp, io); });
     timed(c, DMPM_K_GRID_ADJ, [&] { grid_adj_dispatch(c, sp, io); });
     timed(c, DMPM_K_P2G_ADJ, [&] { p2g_adj_dispatch(c, sp, io); });
@@ -1001,6 +1002,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
     c->batch.nranges = 0;
     c->batch.B = c->B;
     c->batch.Gtot = c->G;
+    c->batch.gmax = c->gmax;
     c->batch.cap = c->lds_cap;
     if (he != hipSuccess) {
         dmpm_destroy(c);
