@@ -481,6 +481,48 @@ __global__ __launch_bounds__(kScat, 4) void k_p2g_blk(Batch bt, StepIO<float> io
     }
 }
 
+// --------------------------------------------------- P2G of mass only ----
+// Node masses of a layer (the EndLayerMassLoss grid, CompGraph.cpp:110-141),
+// with the window scatter of k_p2g_blk.
+__global__ __launch_bounds__(kScat) void k_p2g_mass_blk(Batch bt, const float* lay, const float* cst, float* gm)
+{
+    extern __shared__ unsigned long long lds64[];
+    const int4 rg = bt.ranges[blockIdx.x];
+    const StepParams sp = bt.sps[rg.z];
+    gm += bt.nofs[rg.z];
+    const int S = sp.stride;
+    const int p = rg.x + (int)threadIdx.x;
+    const bool valid = p < rg.y;
+    const Win W = range_window(sp, lay, S, rg.x, rg.y, bt.cap);
+    float x[3] = {0.f, 0.f, 0.f}, m = 0.f;
+    float bound[1] = {0.f};
+    if (valid) {
+        ld3(lay, FX, S, p, x);
+        m = cst[KM * S + p];
+        bound[0] = fabsf(m);
+    }
+    const Scales<1> SC = block_scales<1>(bound);
+    const int mode = (W.fixed && SC.ok) ? 0 : ((W.lds && SC.ok) ? 1 : 2);
+    if (mode == 0)
+        for (int i = threadIdx.x; i < kFWS; i += blockDim.x) lds64[i] = 0ull;
+    else if (mode == 1)
+        for (int i = threadIdx.x; i < W.vol; i += blockDim.x) lds64[i] = 0ull;
+    __syncthreads();
+    if (valid) {
+        Axis<float> st[3];
+        stencil_f(sp, x, st, false);
+        if (mode != 2) m *= SC.s[0];
+        auto val = [&](int i, int j, int kk, float* out) { out[0] = st[0].w[i] * st[1].w[j] * st[2].w[kk] * m; };
+        if (mode == 0) scatter64<1, 0>(sp, W, st, lds64, gm, val);
+        else if (mode == 1) scatter64<1, 1>(sp, W, st, lds64, gm, val);
+        else scatter64<1, 2>(sp, W, st, lds64, gm, val);
+    }
+    if (mode != 2) {
+        __syncthreads();
+        flush_fixed<1, true>(sp, W, lds64, SC, gm, sp.G);
+    }
+}
+
 // ------------------------------------------------------------------ G2P ----
 __global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io)
 {
@@ -974,6 +1016,13 @@ void launch_p2g_blk(const Batch& b, const StepIO<float>& io, hipStream_t s)
 {
     if (b.nranges <= 0) return;
     hipLaunchKernelGGL(k_p2g_blk, dim3(b.nranges), dim3(kScat), 4 * win_nodes(b.cap) * sizeof(unsigned long long), s, b, io);
+}
+
+void launch_p2g_mass_blk(const Batch& b, const float* lay, const float* cst, float* gm, hipStream_t s)
+{
+    if (b.nranges <= 0) return;
+    hipLaunchKernelGGL(k_p2g_mass_blk, dim3(b.nranges), dim3(kScat), win_nodes(b.cap) * sizeof(unsigned long long), s, b, lay, cst,
+                       gm);
 }
 
 void launch_g2p_blk(const Batch& b, const StepIO<float>& io, hipStream_t s)

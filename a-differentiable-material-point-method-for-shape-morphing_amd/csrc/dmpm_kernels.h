@@ -106,6 +106,7 @@ struct Batch {
 // 4-channel grids (m or mbar, then 3 vector channels) of instance i start at 4 * nofs[i];
 // 1-channel arrays (target, dL/dm) at nofs[i].
 void launch_p2g_blk(const Batch& b, const StepIO<float>& io, hipStream_t s);
+void launch_p2g_mass_blk(const Batch& b, const float* lay, const float* cst, float* gm, hipStream_t s);
 void launch_g2p_blk(const Batch& b, const StepIO<float>& io, hipStream_t s);
 void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s);
 void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s);

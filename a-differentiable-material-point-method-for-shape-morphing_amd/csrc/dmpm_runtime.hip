@@ -431,6 +431,20 @@ void loss_kernels(dmpm_ctx* c, const double* lay, const double* gm, double* dLdm
     launch_loss_particles<double>(sp, lay, cst, dLdm, grads_of<double>(c, c->L - 1), c->stream);
 }
 
+// node masses of the end layer: per-particle HBM atomics (FP64) or the
+// window scatter over the ranges (FP32)
+void end_masses(dmpm_ctx* c, const double* lay, double* gm)
+{
+    const double* cst = static_cast<const double*>(c->cst);
+    for (int i = 0; i < c->B; ++i)
+        launch_p2g_mass<double>(inst_params(c, i), lay + c->poff[i], cst + KM * c->S + c->poff[i], c->S, gm + c->nofs[i], c->stream);
+}
+
+void end_masses(dmpm_ctx* c, const float* lay, float* gm)
+{
+    launch_p2g_mass_blk(c->batch, lay, static_cast<const float*>(c->cst), gm, c->stream);
+}
+
 void loss_kernels(dmpm_ctx* c, const float* lay, const float* gm, double* dLdm)
 {
     launch_loss_b(c->batch, gm, static_cast<const float*>(c->target), dLdm, c->dpart, c->dacc, c->gmax, c->stream);
@@ -449,10 +463,8 @@ int end_loss(dmpm_ctx* c, double* loss)
     // 4G-real work grid (8-byte aligned for both precisions)
     double* dLdm = reinterpret_cast<double*>(gm + 2 * c->G);
     fill<Real>(c, gm, c->G, Real(0));
-    const Real* cst = static_cast<const Real*>(c->cst);
     timed(c, DMPM_K_LOSS, [&] {
-        for (int i = 0; i < c->B; ++i)
-            launch_p2g_mass<Real>(inst_params(c, i), lay + c->poff[i], cst + KM * c->S + c->poff[i], c->S, gm + c->nofs[i], c->stream);
+        end_masses(c, lay, gm);
         loss_kernels(c, lay, gm, dLdm);
     });
     HIP_TRY(hipGetLastError());
