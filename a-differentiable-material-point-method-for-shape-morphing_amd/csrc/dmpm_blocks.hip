@@ -18,6 +18,10 @@
 // clamped box, up to `cap` nodes, with per-node bounds checks) is used, and
 // past that, direct HBM access.
 //
+// The FP32 grid of a step keeps the P2G sums (m, p): G2P and the transposed
+// P2G apply the grid update (and its transpose) to each node as they load it,
+// so no node-parallel kernel runs between the particle kernels.
+//
 // Scatter accumulation is 64-bit fixed point: LDS float atomics are
 // serialised lane by lane on gfx950 (ds_add_f32 ~0.3 lane-ops/clk/CU against
 // ~9 for ds_add_u64).  Each channel c of a workgroup gets a power-of-two scale
@@ -167,23 +171,41 @@ __device__ __forceinline__ int fixed_slot(const Win& W, const Axis<float> st[3])
     return (st[0].base - W.flo[0]) * kSI + (st[1].base - W.flo[1]) * kSJ + (st[2].base - W.flo[2]);
 }
 
-// load NC channels (channel c at src_c[node]) of the window into LDS
-template <int NC>
-__device__ __forceinline__ void win_load(const StepParams& sp, const Win& W, float* lds, const float* const src[NC])
+// load NC channels of the window into LDS; ld(node, out) gives the NC
+// channel values of a grid node
+template <int NC, typename Ld>
+__device__ __forceinline__ void win_load(const StepParams& sp, const Win& W, float* lds, Ld&& ld)
 {
     if (W.fixed) {
         for (int idx = threadIdx.x; idx < kFW3; idx += blockDim.x) {
             const long long node = fixed_node(sp, W, idx);
+            float v[NC];
+            if (node >= 0) ld(node, v);
+            else
 #pragma unroll
-            for (int c = 0; c < NC; ++c) lds[c * kFWS + fixed_slot_of(idx)] = node >= 0 ? src[c][node] : 0.f;
+                for (int c = 0; c < NC; ++c) v[c] = 0.f;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) lds[c * kFWS + fixed_slot_of(idx)] = v[c];
         }
     } else if (W.lds) {
         for (int idx = threadIdx.x; idx < W.vol; idx += blockDim.x) {
-            const long long node = win_node(sp, W, idx);
+            float v[NC];
+            ld(win_node(sp, W, idx), v);
 #pragma unroll
-            for (int c = 0; c < NC; ++c) lds[c * W.vol + idx] = src[c][node];
+            for (int c = 0; c < NC; ++c) lds[c * W.vol + idx] = v[c];
         }
     }
+}
+
+// G_op (ForwardSimulation.cpp:53-58) of one node from its P2G sums m, p:
+// v = p/m + dt f where |m| > FLT_EPSILON, else 0.  FP32 grids keep m and p;
+// the kernels that read node velocities apply this as they load a node.
+__device__ __forceinline__ void node_velocity(float m, const float p[3], const float fdt[3], float v[3])
+{
+    const bool ok = fabsf(m) > float(kMassEps);
+    const float inv = 1.f / m;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[a] = ok ? fmaf(p[a], inv, fdt[a]) : 0.f;
 }
 
 template <int NC>
@@ -253,9 +275,12 @@ __device__ __forceinline__ unsigned long long fix32(float y)
 // fixed-point resolution), so a node left at mass 0 keeps every channel at 0.
 // gm (optional): skip nodes whose mass there is 0 (the transposed G2P skips
 // nodes the forward P2G left empty: their pbar is 0 whatever vbar is).
+// flags (optional): tile flags of the target grid, set for every node
+// flushed (nofs: the instance's first flat node).
 template <int NC, bool MASS>
 __device__ __forceinline__ void flush_fixed(const StepParams& sp, const Win& W, const unsigned long long* lds,
-                                            const Scales<NC>& S, float* base, long long cstride, const float* gm = nullptr)
+                                            const Scales<NC>& S, float* base, long long cstride, const float* gm = nullptr,
+                                            unsigned* flags = nullptr, long long nofs = 0)
 {
     if (W.fixed) {
         for (int idx = threadIdx.x; idx < kFW3; idx += blockDim.x) {
@@ -263,6 +288,7 @@ __device__ __forceinline__ void flush_fixed(const StepParams& sp, const Win& W, 
             if (node < 0) continue;
             if (MASS && lds[fixed_slot_of(idx)] == 0ull) continue;
             if (gm && gm[node] == 0.f) continue;
+            if (flags) flags[(nofs + node) >> kTileShift] = 1u;
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
                 const long long q = (long long)lds[c * kFWS + fixed_slot_of(idx)];
@@ -274,6 +300,7 @@ __device__ __forceinline__ void flush_fixed(const StepParams& sp, const Win& W, 
             const long long node = win_node(sp, W, idx);
             if (MASS && lds[idx] == 0ull) continue;
             if (gm && gm[node] == 0.f) continue;
+            if (flags) flags[(nofs + node) >> kTileShift] = 1u;
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
                 const long long q = (long long)lds[c * W.vol + idx];
@@ -309,10 +336,12 @@ __device__ __forceinline__ void stm(T* base, int field, int S, int p, const M3<T
 // Scatter of NC channels over a particle's 64 nodes.  val(i, j, k, out)
 // fills the NC contributions (already in scaled units for the LDS modes).
 // MODE 0: fixed window, 1: general LDS window, 2: HBM float atomics (nodes
-// where gate is 0 are skipped there, as the LDS modes' flush skips them).
+// where gate is 0 are skipped there, as the LDS modes' flush skips them; the
+// tile flags of the nodes written are set when flags is given).
 template <int NC, int MODE, typename ValFn>
 __device__ __forceinline__ void scatter64(const StepParams& sp, const Win& W, const Axis<float> st[3],
-                                          unsigned long long* lds, float* hbm, ValFn&& val, const float* gate = nullptr)
+                                          unsigned long long* lds, float* hbm, ValFn&& val, const float* gate = nullptr,
+                                          unsigned* flags = nullptr, long long nofs = 0)
 {
     const int slot = MODE == 0 ? fixed_slot(W, st) : 0;
 #pragma unroll
@@ -340,6 +369,7 @@ __device__ __forceinline__ void scatter64(const StepParams& sp, const Win& W, co
                 } else {
                     const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
                     if (gate && gate[node] == 0.f) continue;
+                    if (flags) flags[(nofs + node) >> kTileShift] = 1u;
 #pragma unroll
                     for (int c = 0; c < NC; ++c) unsafeAtomicAdd(hbm + c * sp.G + node, v[c]);
                 }
@@ -350,9 +380,9 @@ __device__ __forceinline__ void scatter64(const StepParams& sp, const Win& W, co
 
 // Gather over a particle's 64 nodes: fn(i, j, k, ch) with ch[c] the node's
 // channel c.  MODE 0: fixed window, 1: general LDS window, 2: HBM.
-template <int NC, int MODE, typename Fn>
-__device__ __forceinline__ void gather64(const StepParams& sp, const Win& W, const Axis<float> st[3], const float* lds,
-                                         const float* const hbm[NC], Fn&& fn)
+template <int NC, int MODE, typename Ld, typename Fn>
+__device__ __forceinline__ void gather64(const StepParams& sp, const Win& W, const Axis<float> st[3], const float* lds, Ld&& ld,
+                                         Fn&& fn)
 {
     const int slot = MODE == 0 ? fixed_slot(W, st) : 0;
 #pragma unroll
@@ -377,9 +407,7 @@ __device__ __forceinline__ void gather64(const StepParams& sp, const Win& W, con
 #pragma unroll
                     for (int c = 0; c < NC; ++c) ch[c] = lds[c * W.vol + o];
                 } else {
-                    const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) ch[c] = hbm[c][node];
+                    ld(((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk, ch);
                 }
                 fn(i, j, kk, ch);
             }
@@ -402,7 +430,8 @@ __global__ __launch_bounds__(kScat, 4) void k_p2g_blk(Batch bt, StepIO<float> io
     extern __shared__ unsigned long long lds64[];
     const int4 rg = bt.ranges[blockIdx.x];
     const StepParams sp = bt.sps[rg.z];
-    io.grid += 4 * bt.nofs[rg.z];
+    const long long nofs = bt.nofs[rg.z];
+    io.grid += 4 * nofs;
     const int cap = bt.cap;
     const int S = sp.stride;
     const int p = rg.x + (int)threadIdx.x;
@@ -415,13 +444,10 @@ __global__ __launch_bounds__(kScat, 4) void k_p2g_blk(Batch bt, StepIO<float> io
     M3<float> A = m3_zero<float>();
     float bound[4] = {0.f, 0.f, 0.f, 0.f};
     if (valid) {
-        float v[3];
-        ld3(io.cur, FX, S, p, x);
-        ld3(io.cur, FV, S, p, v);
-        const M3<float> C = ldm(io.cur, FC, S, p);
+        // the elastic part first; the fields it does not need are loaded
+        // after it, so that they do not occupy registers across the FP64 polar
         const M3<float> F = ldm(io.cur, FF, S, p);
         const M3<float> dF = ldm(io.ctrl, 0, S, p);
-        m = io.cst[KM * S + p];
         const double vol = io.cst[KVOL * S + p];
         const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
         M3<double> Fc;
@@ -431,8 +457,16 @@ __global__ __launch_bounds__(kScat, 4) void k_p2g_blk(Batch bt, StepIO<float> io
         fce_prepare(Fc, e);
         const M3<double> PF = mult(fce_pk1(Fc, e, lam, mu), Fc);
         const double k = kst * vol;
+        float kPF[9];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) A.a[i] = float(k * PF.a[i]) + m * C.a[i];
+        for (int i = 0; i < 9; ++i) kPF[i] = float(k * PF.a[i]);
+        float v[3];
+        ld3(io.cur, FX, S, p, x);
+        ld3(io.cur, FV, S, p, v);
+        const M3<float> C = ldm(io.cur, FC, S, p);
+        m = io.cst[KM * S + p];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) A.a[i] = kPF[i] + m * C.a[i];
         if (io.aux) stm(io.aux, FA, S, p, A);
 #pragma unroll
         for (int a = 0; a < 3; ++a) mv[a] = m * v[a] * sdrag;
@@ -473,11 +507,11 @@ __global__ __launch_bounds__(kScat, 4) void k_p2g_blk(Batch bt, StepIO<float> io
         };
         if (mode == 0) scatter64<4, 0>(sp, W, st, lds64, io.grid, val);
         else if (mode == 1) scatter64<4, 1>(sp, W, st, lds64, io.grid, val);
-        else scatter64<4, 2>(sp, W, st, lds64, io.grid, val);
+        else scatter64<4, 2>(sp, W, st, lds64, io.grid, val, nullptr, io.gflags, nofs);
     }
     if (mode != 2) {
         __syncthreads();
-        flush_fixed<4, true>(sp, W, lds64, SC, io.grid, sp.G);
+        flush_fixed<4, true>(sp, W, lds64, SC, io.grid, sp.G, nullptr, io.gflags, nofs);
     }
 }
 
@@ -533,12 +567,17 @@ __global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io)
     const int cap = bt.cap;
     const int S = sp.stride;
     const Win W = range_window(sp, io.cur, S, rg.x, rg.y, cap);
-    const float* gv = io.grid + sp.G;
-    const float* const src[3] = {gv, gv + sp.G, gv + 2 * sp.G};
-    win_load<3>(sp, W, lds, src);
+    const float dt = float(sp.dt), s = float(sp.smoothing);
+    const float fdt[3] = {dt * float(sp.f_ext[0]), dt * float(sp.f_ext[1]), dt * float(sp.f_ext[2])};
+    const float* gr = io.grid;
+    const long long G = sp.G;
+    auto ld = [&](long long node, float* v) {
+        const float p[3] = {gr[G + node], gr[2 * G + node], gr[3 * G + node]};
+        node_velocity(gr[node], p, fdt, v);
+    };
+    win_load<3>(sp, W, lds, ld);
     __syncthreads();
     const float Dd = float(3.0 * sp.inv_dx);
-    const float dt = float(sp.dt), s = float(sp.smoothing);
     for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) {
         float x[3];
         ld3(io.cur, FX, S, p, x);
@@ -561,9 +600,9 @@ __global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io)
                 Cp.a[3 * a + 2] += wv * e2s[kk];
             }
         };
-        if (W.fixed) gather64<3, 0>(sp, W, st, lds, src, acc);
-        else if (W.lds) gather64<3, 1>(sp, W, st, lds, src, acc);
-        else gather64<3, 2>(sp, W, st, lds, src, acc);
+        if (W.fixed) gather64<3, 0>(sp, W, st, lds, ld, acc);
+        else if (W.lds) gather64<3, 1>(sp, W, st, lds, ld, acc);
+        else gather64<3, 2>(sp, W, st, lds, ld, acc);
         const M3<float> F = ldm(io.cur, FF, S, p);
         const M3<float> dF = ldm(io.ctrl, 0, S, p);
         M3<float> Fc, IC;
@@ -590,13 +629,15 @@ __global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io)
 
 // -------------------------------------------------------- transposed G2P ---
 // P_op_2^T + G2P^T: vt = v' + dt x', Ct = C' + (1-s) dt F' Fc^T (kept in tmp),
-// vbar_i += w (vt + D Ct d).
+// vbar_i += w (vt + D Ct d), into the adjoint grid (3 channels; nodes of zero
+// forward mass are skipped: their pbar is 0 whatever vbar is).
 __global__ __launch_bounds__(kScat) void k_g2p_adj_blk(Batch bt, AdjIO<float> io)
 {
     extern __shared__ unsigned long long lds64[];
     const int4 rg = bt.ranges[blockIdx.x];
     const StepParams sp = bt.sps[rg.z];
-    io.agrid += 4 * bt.nofs[rg.z];
+    const long long nofs = bt.nofs[rg.z];
+    io.agrid += 4 * nofs;
     const int cap = bt.cap;
     const int S = sp.stride;
     const int p = rg.x + (int)threadIdx.x;
@@ -662,11 +703,11 @@ __global__ __launch_bounds__(kScat) void k_g2p_adj_blk(Batch bt, AdjIO<float> io
         };
         if (mode == 0) scatter64<3, 0>(sp, W, st, lds64, io.agrid, val);
         else if (mode == 1) scatter64<3, 1>(sp, W, st, lds64, io.agrid, val);
-        else scatter64<3, 2>(sp, W, st, lds64, io.agrid, val, io.grid + 4 * bt.nofs[rg.z]);
+        else scatter64<3, 2>(sp, W, st, lds64, io.agrid, val, io.grid + 4 * nofs, io.aflags, nofs);
     }
     if (mode != 2) {
         __syncthreads();
-        flush_fixed<3, false>(sp, W, lds64, SC, io.agrid, sp.G, io.grid + 4 * bt.nofs[rg.z]);
+        flush_fixed<3, false>(sp, W, lds64, SC, io.agrid, sp.G, io.grid + 4 * nofs, io.aflags, nofs);
     }
 }
 
@@ -723,7 +764,10 @@ __device__ __forceinline__ void elastic_adj(const StepParams& sp, const AdjIO<fl
 }
 
 // ------------------------------------------------------- transposed P2G ----
-// P2G^T + P_op_1^T, one thread per particle of the range.  With the P2G
+// P2G^T + G_op^T + P_op_1^T, one thread per particle of the range.  The
+// window load applies the transposed grid update (BackPropagation.cpp:46-59)
+// to each node: pbar = vbar/m, mbar = -((v - dt f).vbar)/m where
+// |m| > FLT_EPSILON, else 0, next to the node velocity v.  With the P2G
 // affine matrix G = -k P Fc^T + m C (cached in the layer by P2G) and
 // b = m(1 - dt drag) v, the gather over the 64 nodes gives
 //   Gb   = sum w pbar d^T,  Cbar = m Gb,  vbar = m (1 - dt drag) sum w pbar,
@@ -743,16 +787,34 @@ __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
     const int S = sp.stride;
     const int cp = sp.compat;
     const Win W = range_window(sp, io.cur, S, rg.x, rg.y, cap);
-    const float* gv = io.grid + sp.G;
-    const float* pb = io.agrid;
-    const float* mb = io.agrid + 3 * sp.G;
-    const float* const src[7] = {pb, pb + sp.G, pb + 2 * sp.G, mb, gv, gv + sp.G, gv + 2 * sp.G};
-    win_load<7>(sp, W, lds, src);
+    const float dt = float(sp.dt);
+    const float fdt[3] = {dt * float(sp.f_ext[0]), dt * float(sp.f_ext[1]), dt * float(sp.f_ext[2])};
+    const bool fsub = !(cp & 8);
+    const float* gr = io.grid;
+    const float* ag = io.agrid;
+    const long long G = sp.G;
+    // channels: pbar (3), mbar, v (3)
+    auto ld = [&](long long node, float* ch) {
+        const float m = gr[node];
+        const float p[3] = {gr[G + node], gr[2 * G + node], gr[3 * G + node]};
+        const float vb[3] = {ag[node], ag[G + node], ag[2 * G + node]};
+        node_velocity(m, p, fdt, ch + 4);
+        const bool ok = fabsf(m) > float(kMassEps);
+        const float inv = 1.f / m;
+        float dot = 0.f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float vi = fsub ? ch[4 + a] - fdt[a] : ch[4 + a];
+            dot += vi * vb[a];
+            ch[a] = ok ? vb[a] * inv : 0.f;
+        }
+        ch[3] = ok ? -dot * inv : 0.f;
+    };
+    win_load<7>(sp, W, lds, ld);
     __syncthreads();
     const float dx = float(sp.dx);
     const float Dinv = float(3.0 * sp.inv_dx * sp.inv_dx);
     const float mgw = -float(sp.inv_dx);
-    const float dt = float(sp.dt);
     const float sdrag = 1.f - dt * float(sp.drag);
     const float bsc = (cp & 4) ? 1.f : sdrag;
     const double kst = -3.0 * sp.inv_dx * sp.inv_dx * sp.dt;
@@ -804,9 +866,9 @@ __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
 #pragma unroll
             for (int a = 0; a < 3; ++a) xg[a] += gw[a] * sgw;
         };
-        if (W.fixed) gather64<7, 0>(sp, W, st, lds, src, acc);
-        else if (W.lds) gather64<7, 1>(sp, W, st, lds, src, acc);
-        else gather64<7, 2>(sp, W, st, lds, src, acc);
+        if (W.fixed) gather64<7, 0>(sp, W, st, lds, ld, acc);
+        else if (W.lds) gather64<7, 1>(sp, W, st, lds, ld, acc);
+        else gather64<7, 2>(sp, W, st, lds, ld, acc);
         float xb[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
@@ -853,84 +915,44 @@ __device__ __forceinline__ int instance_of(const Batch& b, long long q)
 }
 }  // namespace
 
-// G_op (ForwardSimulation.cpp:53-58): v = p/m + dt f where |m| > FLT_EPSILON,
-// else 0.  blockIdx.y is the instance; the x-blocks stride over its nodes.
-__global__ __launch_bounds__(kGat) void k_grid_update_b(Batch bt, float* grid)
+// Zero the 4 channels of every node of the flagged tiles and clear the flags.
+// Each wave reads the flags of 64 consecutive tiles (one per lane, so every
+// flag is read and reset by one lane), then clears the flagged ones: lanes
+// 0-31 channels 0 and 2, lanes 32-63 channels 1 and 3 of the tile's 32 nodes.
+__global__ __launch_bounds__(kGat) void k_clear_tiles(Batch bt, float* grid, unsigned* flags, long long ntiles)
 {
-    const int inst = blockIdx.y;
-    const StepParams& sp = bt.sps[inst];
-    const long long G = sp.G;
-    float* g = grid + 4 * bt.nofs[inst];
-    const float dt = float(sp.dt);
-    const float fx = dt * float(sp.f_ext[0]), fy = dt * float(sp.f_ext[1]), fz = dt * float(sp.f_ext[2]);
-    auto node = [&](long long l, float m) {
-        if (m == 0.f) return;  // no particle reached the node: p = 0, v stays 0
-        if (fabsf(m) > float(kMassEps)) {
-            const float inv = 1.f / m;
-            g[G + l] = g[G + l] * inv + fx;
-            g[2 * G + l] = g[2 * G + l] * inv + fy;
-            g[3 * G + l] = g[3 * G + l] * inv + fz;
-        } else {
-            g[G + l] = 0.f;
-            g[2 * G + l] = 0.f;
-            g[3 * G + l] = 0.f;
+    const int lane = threadIdx.x & 63;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    const int c0 = lane >> 5;
+    const long long Gtot = bt.Gtot;
+    for (long long t0 = wave * 64; t0 < ntiles; t0 += nwaves * 64) {
+        const long long t = t0 + lane;
+        const unsigned f = t < ntiles ? flags[t] : 0u;
+        unsigned long long mask = __ballot(f != 0u);
+        if (f != 0u) flags[t] = 0u;
+        while (mask) {
+            const int b = __ffsll((long long)mask) - 1;
+            mask &= mask - 1;
+            const long long q = ((t0 + b) << kTileShift) + (lane & 31);
+            if (q >= Gtot) continue;
+            long long n0 = 0, Gi = Gtot;
+            if (bt.B > 1) {
+                const int i = instance_of(bt, q);
+                n0 = bt.nofs[i];
+                Gi = bt.nofs[i + 1] - n0;
+            }
+            float* g = grid + 4 * n0 + (q - n0);
+            g[c0 * Gi] = 0.f;
+            g[(c0 + 2) * Gi] = 0.f;
         }
-    };
-    // the mass channel starts at a multiple of 4 nodes: read it 4 nodes at a time
-    const float4* m4 = reinterpret_cast<const float4*>(g);
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < G / 4; q += (long long)gridDim.x * blockDim.x) {
-        const float4 m = m4[q];
-        node(4 * q, m.x);
-        node(4 * q + 1, m.y);
-        node(4 * q + 2, m.z);
-        node(4 * q + 3, m.w);
     }
-    const long long t = (G / 4) * 4 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < G) node(t, g[t]);
 }
 
-// G_op^T (BackPropagation.cpp:46-59): pbar = vbar/m, mbar = -((v - dt f).vbar)/m.
-// Nodes of zero forward mass hold vbar = pbar = mbar = 0 already (fill, and
-// the transposed G2P flush skips them).
-__global__ __launch_bounds__(kGat) void k_grid_adj_b(Batch bt, const float* grid, float* agrid)
+__global__ __launch_bounds__(kGat) void k_set_flags(unsigned* flags, long long ntiles)
 {
-    const int inst = blockIdx.y;
-    const StepParams& sp = bt.sps[inst];
-    const long long G = sp.G;
-    const float* g = grid + 4 * bt.nofs[inst];
-    float* ag = agrid + 4 * bt.nofs[inst];
-    const float dt = float(sp.dt);
-    const bool fsub = !(sp.compat & 8);
-    auto node = [&](long long l, float m) {
-        if (m == 0.f) return;
-        if (fabsf(m) > float(kMassEps)) {
-            const float inv = 1.f / m;
-            float dot = 0.f;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                float vi = g[(1 + a) * G + l];
-                if (fsub) vi -= dt * float(sp.f_ext[a]);
-                const float vb = ag[a * G + l];
-                dot += vi * vb;
-                ag[a * G + l] = vb * inv;
-            }
-            ag[3 * G + l] = -dot * inv;
-        } else {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) ag[a * G + l] = 0.f;
-            ag[3 * G + l] = 0.f;
-        }
-    };
-    const float4* m4 = reinterpret_cast<const float4*>(g);
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < G / 4; q += (long long)gridDim.x * blockDim.x) {
-        const float4 m = m4[q];
-        node(4 * q, m.x);
-        node(4 * q + 1, m.y);
-        node(4 * q + 2, m.z);
-        node(4 * q + 3, m.w);
-    }
-    const long long t = (G / 4) * 4 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < G) node(t, g[t]);
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += (long long)gridDim.x * blockDim.x)
+        flags[t] = 1u;
 }
 
 // EndLayerMassLoss node terms (CompGraph.cpp:143-187) in FP64.  blockIdx.y is
@@ -1048,25 +1070,21 @@ void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
     }
 }
 
-// x-blocks per instance of the node-parallel kernels: 4 nodes per thread
-static unsigned node_blocks(const Batch& b)
+void launch_clear_tiles(const Batch& b, float* grid, unsigned* flags, hipStream_t s)
 {
-    long long nb = (b.gmax + 4LL * kGat - 1) / (4LL * kGat);
-    const long long lim = 8192 / b.B > 8 ? 8192 / b.B : 8;
-    if (nb > lim) nb = lim;
-    return (unsigned)(nb < 1 ? 1 : nb);
+    const long long nt = num_tiles(b.Gtot);
+    if (nt <= 0) return;
+    long long nb = (nt + 4 * 64 - 1) / (4 * 64);  // a wave per 64 tiles, 4 waves per block
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(k_clear_tiles, dim3((unsigned)nb), dim3(kGat), 0, s, b, grid, flags, nt);
 }
 
-void launch_grid_update_b(const Batch& b, float* grid, hipStream_t s)
+void launch_set_flags(unsigned* flags, long long ntiles, hipStream_t s)
 {
-    if (b.Gtot <= 0) return;
-    hipLaunchKernelGGL(k_grid_update_b, dim3(node_blocks(b), (unsigned)b.B), dim3(kGat), 0, s, b, grid);
-}
-
-void launch_grid_adj_b(const Batch& b, const float* grid, float* agrid, hipStream_t s)
-{
-    if (b.Gtot <= 0) return;
-    hipLaunchKernelGGL(k_grid_adj_b, dim3(node_blocks(b), (unsigned)b.B), dim3(kGat), 0, s, b, grid, agrid);
+    if (ntiles <= 0) return;
+    long long nb = (ntiles + kGat - 1) / kGat;
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(k_set_flags, dim3((unsigned)nb), dim3(kGat), 0, s, flags, ntiles);
 }
 
 void launch_loss_b(const Batch& b, const float* gm, const float* target, double* dLdm, double* part, double* acc, long long gmax,

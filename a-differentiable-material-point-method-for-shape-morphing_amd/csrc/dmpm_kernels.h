@@ -36,8 +36,9 @@ struct StepIO {
     Real* next;        // layer n+1 state (forward)
     const Real* ctrl;  // dFc_n (9 fields)
     const Real* cst;   // constants
-    Real* grid;        // grid m, v/p (kGridFields * G)
+    Real* grid;        // grid m, p -> v (FP64); FP32 keeps m, p (kGridFields * G)
     Real* aux;         // writable alias of cur for derived fields (FP32: A), or null
+    unsigned* gflags;  // FP32: tile flags of grid (set where P2G writes), else null
 };
 
 template <typename Real>
@@ -46,12 +47,14 @@ struct AdjIO {
     const Real* next;    // layer n+1 state (C' is read)
     const Real* ctrl;    // dFc_n
     const Real* cst;
-    const Real* grid;    // grid of step n after the update (m, v)
-    Real* agrid;         // adjoint grid: vbar -> pbar (3G), mbar (G)
+    const Real* grid;    // grid of step n: m, v after the update (FP64) or m, p (FP32)
+    Real* agrid;         // adjoint grid: vbar -> pbar (3G), mbar (G) (FP64); vbar (FP32)
     const Real* g1;      // gradients of layer n+1 (24 fields, same layout as a layer)
     Real* g0;            // gradients of layer n
     Real* gctrl;         // dL/d(dFc_n) (9 fields)
     Real* tmp;           // per particle vt (3) and Ct (9), from the transposed G2P
+    const unsigned* gflags;  // FP32: tile flags of grid, else null
+    unsigned* aflags;        // FP32: tile flags of agrid (set where the transposed G2P writes)
 };
 
 // most workgroups of a one-row FP64 reduction (loss, norm)
@@ -105,13 +108,21 @@ struct Batch {
 };
 // 4-channel grids (m or mbar, then 3 vector channels) of instance i start at 4 * nofs[i];
 // 1-channel arrays (target, dL/dm) at nofs[i].
+//
+// Tile flags (FP32 grids): flat node q (instance i's node l is q = nofs[i] + l)
+// lies in tile q >> kTileShift; a grid's flag array holds one word per tile,
+// non-zero where a kernel may have written a node of the tile since the last
+// clear.  Nodes of unflagged tiles are 0 in every channel, so the clear and
+// the node-parallel kernels visit flagged tiles only.
+constexpr int kTileShift = 5;
+inline long long num_tiles(long long Gtot) { return (Gtot + (1LL << kTileShift) - 1) >> kTileShift; }
+void launch_clear_tiles(const Batch& b, float* grid, unsigned* flags, hipStream_t s);
+void launch_set_flags(unsigned* flags, long long ntiles, hipStream_t s);
 void launch_p2g_blk(const Batch& b, const StepIO<float>& io, hipStream_t s);
 void launch_p2g_mass_blk(const Batch& b, const float* lay, const float* cst, float* gm, hipStream_t s);
 void launch_g2p_blk(const Batch& b, const StepIO<float>& io, hipStream_t s);
 void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s);
 void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s);
-void launch_grid_update_b(const Batch& b, float* grid, hipStream_t s);
-void launch_grid_adj_b(const Batch& b, const float* grid, float* agrid, hipStream_t s);
 // loss over all instances from node masses gm (instance i at nofs[i]):
 // per-instance losses in acc[0..B) (fixed-order sums of workgroup partials
 // kept in part, at most loss_partials(B) doubles); gmax = largest instance node count

@@ -67,6 +67,10 @@ struct dmpm_ctx {
     void* grids = nullptr;    // grid slots x 4 x G
     void* wgrid = nullptr;    // 4 x G: loss mass + dLdm / checkpoint-mode forward grid
     void* agrid = nullptr;    // 4 x G: vbar/pbar + mbar
+    // FP32: tile flags of every grid buffer (grid slots, then wgrid, then agrid)
+    unsigned* tflags = nullptr;
+    long long ntiles = 0;
+    size_t grid_slots = 0;
     void* gbuf[2] = {nullptr, nullptr};
     void* gctrl = nullptr;    // (L-1) x 9 x S or 9 x S
     void* tmp = nullptr;      // 12 x S
@@ -289,19 +293,44 @@ void fill(dmpm_ctx* c, Real* p, long long count, Real v)
     timed(c, DMPM_K_FILL, [&] { launch_fill<Real>(p, count, v, c->stream); });
 }
 
+// Tile flags of an FP32 grid buffer (null for the FP64 path).
+unsigned* flags_of(dmpm_ctx* c, const void* g)
+{
+    if (!c->tflags) return nullptr;
+    const size_t gb = (size_t)kGridFields * c->G * sizeof(float);
+    size_t k;
+    if (g == c->wgrid) k = c->grid_slots;
+    else if (g == c->agrid) k = c->grid_slots + 1;
+    else k = (size_t)(static_cast<const char*>(g) - static_cast<const char*>(c->grids)) / gb;
+    return c->tflags + k * (size_t)c->ntiles;
+}
+
+// Zero a 4-channel grid buffer before a scatter into it: FP32 clears the
+// flagged tiles only (every other node is already 0), FP64 the whole buffer.
+void clear_grid(dmpm_ctx* c, float* g)
+{
+    timed(c, DMPM_K_FILL, [&] { launch_clear_tiles(c->batch, g, flags_of(c, g), c->stream); });
+}
+
+void clear_grid(dmpm_ctx* c, double* g) { fill<double>(c, g, (long long)kGridFields * c->G, 0.0); }
+
+// wgrid used as scratch (node masses, dL/dm): every tile may be non-zero
+void taint_wgrid(dmpm_ctx* c)
+{
+    if (c->tflags) launch_set_flags(flags_of(c, c->wgrid), c->ntiles, c->stream);
+}
+
 // Particle-grid kernels: the plain FP64 path runs one thread per particle
 // with direct HBM atomics; the FP32 path runs one workgroup per block range
 // with the node window staged in LDS.
 inline void p2g_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<double>& io) { launch_p2g<double>(sp, io, c->stream); }
 inline void p2g_dispatch(dmpm_ctx* c, const StepParams&, const StepIO<float>& io) { launch_p2g_blk(c->batch, io, c->stream); }
 inline void grid_update_dispatch(dmpm_ctx* c, const StepParams& sp, double* g) { launch_grid_update<double>(sp, g, c->stream); }
-inline void grid_update_dispatch(dmpm_ctx* c, const StepParams&, float* g) { launch_grid_update_b(c->batch, g, c->stream); }
 inline void g2p_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<double>& io) { launch_g2p<double>(sp, io, c->stream); }
 inline void g2p_dispatch(dmpm_ctx* c, const StepParams&, const StepIO<float>& io) { launch_g2p_blk(c->batch, io, c->stream); }
 inline void g2p_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<double>& io) { launch_g2p_adj<double>(sp, io, c->stream); }
 inline void g2p_adj_dispatch(dmpm_ctx* c, const StepParams&, const AdjIO<float>& io) { launch_g2p_adj_blk(c->batch, io, c->stream); }
 inline void grid_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<double>& io) { launch_grid_adj<double>(sp, io, c->stream); }
-inline void grid_adj_dispatch(dmpm_ctx* c, const StepParams&, const AdjIO<float>& io) { launch_grid_adj_b(c->batch, io.grid, io.agrid, c->stream); }
 inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<double>& io) { launch_p2g_adj<double>(sp, io, c->stream); }
 inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams&, const AdjIO<float>& io)
 {
@@ -314,10 +343,12 @@ void step_forward(dmpm_ctx* c, const Real* cur, Real* nxt, const Real* ctrl, Rea
 {
     StepParams sp = step_params(c);
     // FP32: P2G caches its affine matrix in the input layer
-    StepIO<Real> io{cur, nxt, ctrl, static_cast<const Real*>(c->cst), g, c->f32 ? const_cast<Real*>(cur) : nullptr};
-    fill<Real>(c, g, (long long)kGridFields * c->G, Real(0));
+    StepIO<Real> io{cur, nxt, ctrl, static_cast<const Real*>(c->cst), g, c->f32 ? const_cast<Real*>(cur) : nullptr,
+                    flags_of(c, g)};
+    clear_grid(c, g);
     timed(c, DMPM_K_P2G, [&] { p2g_dispatch(c, sp, io); });
-    timed(c, DMPM_K_GRID_UPDATE, [&] { grid_update_dispatch(c, sp, g); });
+    // FP32: G2P applies the grid update as it loads the nodes
+    if constexpr (std::is_same<Real, double>::value) timed(c, DMPM_K_GRID_UPDATE, [&] { grid_update_dispatch(c, sp, g); });
     timed(c, DMPM_K_G2P, [&] { g2p_dispatch(c, sp, io); });
 }
 
@@ -337,11 +368,14 @@ void step_backward(dmpm_ctx* c, int i, const Real* cur, const Real* nxt, const R
     io.g0 = grads_of<Real>(c, i);
     io.gctrl = gctrl_of<Real>(c, i);
     io.tmp = static_cast<Real*>(c->tmp);
-    // FP64: the vbar channels (the first 3G reals); FP32: all four channels
-    // (the batched transposed grid update leaves mbar of untouched nodes at 0)
-    fill<Real>(c, io.agrid, (c->f32 ? kGridFields : 3) * c->G, Real(0));
+    io.gflags = flags_of(c, g);
+    io.aflags = flags_of(c, c->agrid);
+    // the vbar channels: FP64 the first 3G reals, FP32 the flagged tiles
+    if (c->f32) clear_grid(c, reinterpret_cast<float*>(io.agrid));
+    else fill<Real>(c, io.agrid, 3 * c->G, Real(0));
     timed(c, DMPM_K_G2P_ADJ, [&] { g2p_adj_dispatch(c, sp, io); });
-    timed(c, DMPM_K_GRID_ADJ, [&] { grid_adj_dispatch(c, sp, io); });
+    // FP32: the transposed P2G applies the transposed grid update as it loads the nodes
+    if constexpr (std::is_same<Real, double>::value) timed(c, DMPM_K_GRID_ADJ, [&] { grid_adj_dispatch(c, sp, io); });
     timed(c, DMPM_K_P2G_ADJ, [&] { p2g_adj_dispatch(c, sp, io); });
 }
 
@@ -467,6 +501,7 @@ int end_loss(dmpm_ctx* c, double* loss)
         end_masses(c, lay, gm);
         loss_kernels(c, lay, gm, dLdm);
     });
+    taint_wgrid(c);
     HIP_TRY(hipGetLastError());
     c->inst_loss.assign(c->B, 0.0);
     HIP_TRY(hipMemcpyAsync(c->inst_loss.data(), c->dacc, c->B * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -676,6 +711,7 @@ int compute_volumes(dmpm_ctx* c)
         launch_p2g_mass<Real>(sp, slot<Real>(c, 0) + p0, cst + KM * c->S + p0, c->S, gm + c->nofs[i], c->stream);
         launch_volumes<Real>(sp, slot<Real>(c, 0) + p0, cst + p0, gm + c->nofs[i], c->stream);
     }
+    taint_wgrid(c);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->fwd_valid = std::min(c->fwd_valid, 0);
@@ -850,7 +886,10 @@ int chain_episode(dmpm_ctx* c)
     return DMPM_OK;
 }
 
-// Grid of one step: node masses (G) and velocities ([G][3]), instances concatenated.
+// Grid of one step: node masses (G) and velocities ([G][3]), instances
+// concatenated.  The FP32 grid holds the P2G sums m, p: the velocities are
+// the grid update's, v = p/m + dt f where |m| > FLT_EPSILON, else 0, in the
+// float operations the kernels use (correctly rounded 1/m, then one fma).
 template <typename Real>
 int get_grid(dmpm_ctx* c, int step, double* m, double* v)
 {
@@ -861,10 +900,21 @@ int get_grid(dmpm_ctx* c, int step, double* m, double* v)
     for (int i = 0; i < c->B; ++i) {
         const long long n0 = c->nofs[i], Gi = c->nofs[i + 1] - n0;
         const double* gi = t.data() + 4 * n0;
+        const StepParams sp = inst_params(c, i);
+        float fdt[3];
+        for (int a = 0; a < 3; ++a) fdt[a] = float(sp.dt) * float(sp.f_ext[a]);
         for (long long q = 0; q < Gi; ++q) {
             if (m) m[n0 + q] = gi[q];
-            if (v)
+            if (!v) continue;
+            if (c->f32) {
+                const float mq = float(gi[q]);
+                const bool ok = std::fabs(mq) > 1.1920928955078125e-07f;  // FLT_EPSILON
+                const float inv = 1.f / mq;
+                for (int a = 0; a < 3; ++a)
+                    v[3 * (n0 + q) + a] = ok ? double(std::fmaf(float(gi[(1 + a) * Gi + q]), inv, fdt[a])) : 0.0;
+            } else {
                 for (int a = 0; a < 3; ++a) v[3 * (n0 + q) + a] = gi[(1 + a) * Gi + q];
+            }
         }
     }
     return DMPM_OK;
@@ -973,6 +1023,8 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         return fail(DMPM_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
     }
     const size_t grid_slots = cfg->tape_mode == DMPM_TAPE_CHECKPOINT ? (size_t)c->K : (size_t)steps;
+    c->grid_slots = grid_slots;
+    c->ntiles = c->f32 ? num_tiles((long long)G) : 0;
     const size_t gctrl_layers = cfg->keep_control_grads ? (size_t)steps : 1;
     if ((rc = dalloc(c, &c->cst, kConstFields * S * e)) || (rc = dalloc(c, &c->ctrl, (size_t)steps * 9 * S * e)) ||
         (rc = dalloc(c, &c->layers, (size_t)c->n_slots * c->lf * S * e)) ||
@@ -983,7 +1035,8 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         (rc = dalloc(c, (void**)&c->dacc, std::max(64, c->B + 8) * sizeof(double))) ||
         (rc = dalloc(c, (void**)&c->dpart, std::max<long long>(kLossBlocks, loss_partials(c->B)) * sizeof(double))) ||
         (rc = dalloc(c, (void**)&c->d_sps, c->B * sizeof(StepParams))) ||
-        (rc = dalloc(c, (void**)&c->d_nofs, (c->B + 1) * sizeof(long long)))) {
+        (rc = dalloc(c, (void**)&c->d_nofs, (c->B + 1) * sizeof(long long))) ||
+        (c->f32 && (rc = dalloc(c, (void**)&c->tflags, (grid_slots + 2) * (size_t)c->ntiles * sizeof(unsigned))))) {
         std::string msg = g_err;
         dmpm_destroy(c);
         return fail(rc, msg);
@@ -994,6 +1047,11 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         launch_fill<float>(static_cast<float*>(c->ctrl), (long long)steps * 9 * S, 0.f, c->stream);
         launch_fill<float>(static_cast<float*>(c->layers), (long long)c->lf * S, 0.f, c->stream);
         launch_fill<float>(static_cast<float*>(c->cst), (long long)kConstFields * S, 0.f, c->stream);
+        // grids start at 0 with no tile flagged
+        launch_fill<float>(static_cast<float*>(c->grids), (long long)(grid_slots * kGridFields * G), 0.f, c->stream);
+        launch_fill<float>(static_cast<float*>(c->wgrid), (long long)(kGridFields * G), 0.f, c->stream);
+        launch_fill<float>(static_cast<float*>(c->agrid), (long long)(kGridFields * G), 0.f, c->stream);
+        launch_fill<float>(reinterpret_cast<float*>(c->tflags), (long long)(grid_slots + 2) * c->ntiles, 0.f, c->stream);
     } else {
         launch_fill<double>(static_cast<double*>(c->gctrl), (long long)gctrl_layers * 9 * S, 0.0, c->stream);
         launch_fill<double>(static_cast<double*>(c->ctrl), (long long)steps * 9 * S, 0.0, c->stream);
@@ -1059,7 +1117,7 @@ int dmpm_destroy(dmpm_ctx* c)
     if (c->stream) hipStreamSynchronize(c->stream);
     clear_graphs(c);
     void* bufs[] = {c->cst,  c->ctrl, c->layers, c->grids, c->wgrid, c->agrid, c->gbuf[0], c->gbuf[1],
-                    c->gctrl, c->tmp, c->target, c->dacc,  c->ranges, c->d_sps, c->d_nofs, c->dpart};
+                    c->gctrl, c->tmp, c->target, c->dacc,  c->ranges, c->d_sps, c->d_nofs, c->dpart, c->tflags};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (void* b : c->opt)

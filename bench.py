@@ -42,17 +42,16 @@ OPT_RANGE_MAX, OPT_LDS_CAP = 2, 3
 # of each kernel (FP64: twice).  Particle fields: x3 v3 C9 F9 dFc9 + m vol lam mu.
 KBYTES = {
     "p2g": (184, 16, 0),        # read x,v,C,F,dFc,m,vol,lam,mu, write the affine matrix A; m,p scatter per node
-    "grid_update": (0, 0, 28),  # read m,p write v per node (dense)
-    "g2p": (180, 12, 0),        # read x,F,dFc write x,v,C,F; gather v
+    "g2p": (180, 16, 0),        # read x,F,dFc write x,v,C,F; gather m,p (grid update applied on load)
     "g2p_adj": (228, 12, 0),    # read x,F,dFc,x',v',F',C' write vt,Ct; scatter vbar
-    "grid_adj": (0, 0, 44),     # read m,v,vbar write pbar,mbar (dense)
-    "p2g_adj": (412, 28, 0),    # read m,A,x,v,vt,Ct,x',F,dFc,vol,lam,mu,C',F' write x,v,C,F grads + dL/d(dFc); gather pbar,mbar,v
+    "p2g_adj": (412, 28, 0),    # read m,A,x,v,vt,Ct,x',F,dFc,vol,lam,mu,C',F' write x,v,C,F grads + dL/d(dFc); gather m,p,vbar
 }
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of one forward + one adjoint particle-step (FP32): per particle
-# 328 + 376, per touched node 56 + 72, per dense grid node 44 (reset + update) + 56
-# (adjoint reset + transposed update).
-STEP_BYTES = (704, 128, 100)
+# 328 + 376; per touched node 76 (reset 16, scatter 16, gather 16 of m,p, ...) +
+# 84 (vbar reset 12, scatter 12, gather 28, ...); no dense grid pass (the grid
+# update and its transpose run inside the gathers, resets cover flagged tiles).
+STEP_BYTES = (704, 160, 0)
 # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a
 # device-to-device copy of known size) of the 1M-particle scene's kernels
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_morph_1m_128.txt")
