@@ -22,14 +22,23 @@
 // P2G apply the grid update (and its transpose) to each node as they load it,
 // so no node-parallel kernel runs between the particle kernels.
 //
-// Scatter accumulation is 64-bit fixed point: LDS float atomics are
-// serialised lane by lane on gfx950 (ds_add_f32 ~0.3 lane-ops/clk/CU against
-// ~9 for ds_add_u64).  Each channel c of a workgroup gets a power-of-two scale
-// s_c with s_c * max_p B_pc <= 2^30, where B_pc bounds every contribution of
-// particle p to channel c; each contribution is rounded to an int32 multiple
-// of 1/s_c and summed exactly in int64 (ds_add_u64).  Window sums are thus
-// independent of the order of the additions; the flush converts them back to
-// float for one HBM atomic per node and channel.
+// Scatter accumulation is fixed point in 64-bit LDS words: LDS float atomics
+// are serialised lane by lane on gfx950 (ds_add_f32 ~0.3 lane-ops/clk/CU
+// against ~9 for ds_add_u64).  Each channel c of a workgroup of nP particles
+// gets a power-of-two scale s_c with s_c * max_p B_pc <= 2^E, where B_pc
+// bounds |val| for every contribution w * val of particle p to channel c;
+// each contribution is rounded to an integer multiple of 1/s_c.
+//   One channel per word (P2G): E = 30, sums in int64.
+//   Two channels per word (transposed G2P, half the LDS atomics): E is the
+//   largest exponent with 2^E nP (2/3)^3 + nP/2 < 2^31 (a node receives at
+//   most nP contributions, each of weight <= (2/3)^3), so every node sum of
+//   every channel fits an int32.  Channel 2q is added in the low 32 bits of
+//   word q and channel 2q+1 in the high 32 bits (the low word's sign borrowed
+//   from the high word): the 64-bit sum is hi * 2^32 + lo and both halves
+//   come back exactly, at a resolution 2^(E-30) coarser than one channel per
+//   word (E = 23 for 512 particles).
+// Window sums are thus independent of the order of the additions; the flush
+// converts them back to float for one HBM atomic per node and channel.
 //
 // Formulas are those of the per-particle kernels (ForwardSimulation.cpp:6-124,
 // BackPropagation.cpp:14-124 in the reference) with the same FP64 polar/SVD,
@@ -216,9 +225,10 @@ struct Scales {
 };
 
 // block-wide max of NC per-thread contribution bounds -> power-of-two scales
-// with s_c * max <= 2^30 (all threads)
-template <int NC>
-__device__ Scales<NC> block_scales(const float bound[NC])
+// with s_c * max <= 2^E (all threads): E = 30 for one channel per word, else
+// the largest exponent with 2^E nP (2/3)^3 + nP/2 < 2^31 (capped at 30)
+template <int NC, bool PACK>
+__device__ Scales<NC> block_scales(const float bound[NC], int nP)
 {
     __shared__ float red[NC][kMaxWaves];
     __shared__ Scales<NC> sc;
@@ -240,6 +250,12 @@ __device__ Scales<NC> block_scales(const float bound[NC])
     if (threadIdx.x == 0) {
         Scales<NC> S;
         S.ok = true;
+        int E = 30;
+        if (PACK) {
+            // (2/3)^3 rounded up, so that float weights cannot exceed it
+            E = (int)floor(log2((2147483647.0 - 0.5 * nP) / (0.29630 * (nP > 1 ? nP : 1))));
+            if (E > 30) E = 30;
+        }
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             float t = 0.f;
@@ -248,11 +264,11 @@ __device__ Scales<NC> block_scales(const float bound[NC])
                 if (!(r < 3.0e38f)) S.ok = false;  // inf or NaN somewhere in the block
                 else t = fmaxf(t, r);
             }
-            int e = 30;
+            int e = E;
             if (t > 0.f) {
                 int ex;
                 frexpf(t, &ex);  // t < 2^ex
-                e = 30 - ex;
+                e = E - ex;
             }
             e = e < -120 ? -120 : (e > 120 ? 120 : e);
             S.s[c] = ldexpf(1.f, e);
@@ -270,6 +286,21 @@ __device__ __forceinline__ unsigned long long fix32(float y)
     return (unsigned long long)(long long)(int)rintf(y);
 }
 
+// channels lo, hi (scaled units) rounded to int32 and packed into one word:
+// (hi + borrow) * 2^32 + lo as a 64-bit integer is hi * 2^32 + lo
+__device__ __forceinline__ unsigned long long pack2(float lo, float hi)
+{
+    const int l = (int)rintf(lo), h = (int)rintf(hi);
+    return ((unsigned long long)(unsigned)(h + (l >> 31)) << 32) | (unsigned)l;
+}
+
+// a sum T = H * 2^32 + L of packed words with |L|, |H| < 2^31 -> L, H
+__device__ __forceinline__ void unpack2(unsigned long long T, int& lo, int& hi)
+{
+    lo = (int)(unsigned)T;
+    hi = (int)(((long long)T - (long long)lo) >> 32);
+}
+
 // MASS: channel 0 is the node mass; a node whose window mass is zero adds
 // nothing (its other channels hold only contributions of weights below the
 // fixed-point resolution), so a node left at mass 0 keeps every channel at 0.
@@ -277,36 +308,49 @@ __device__ __forceinline__ unsigned long long fix32(float y)
 // nodes the forward P2G left empty: their pbar is 0 whatever vbar is).
 // flags (optional): tile flags of the target grid, set for every node
 // flushed (nofs: the instance's first flat node).
-template <int NC, bool MASS>
+// PACK: two channels per word.
+template <int NC, bool MASS, bool PACK>
 __device__ __forceinline__ void flush_fixed(const StepParams& sp, const Win& W, const unsigned long long* lds,
                                             const Scales<NC>& S, float* base, long long cstride, const float* gm = nullptr,
                                             unsigned* flags = nullptr, long long nofs = 0)
 {
-    if (W.fixed) {
-        for (int idx = threadIdx.x; idx < kFW3; idx += blockDim.x) {
-            const long long node = fixed_node(sp, W, idx);
+    constexpr int NW = PACK ? (NC + 1) / 2 : NC;
+    const int n = W.fixed ? kFW3 : W.vol;
+    const int stride = W.fixed ? kFWS : W.vol;
+    for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+        long long node;
+        int slot;
+        if (W.fixed) {
+            node = fixed_node(sp, W, idx);
             if (node < 0) continue;
-            if (MASS && lds[fixed_slot_of(idx)] == 0ull) continue;
-            if (gm && gm[node] == 0.f) continue;
-            if (flags) flags[(nofs + node) >> kTileShift] = 1u;
+            slot = fixed_slot_of(idx);
+        } else {
+            node = win_node(sp, W, idx);
+            slot = idx;
+        }
+        float val[NC];
+        bool any = true;
+        if (PACK) {
+            int q[2 * NW];
+#pragma unroll
+            for (int k = 0; k < NW; ++k) unpack2(lds[k * stride + slot], q[2 * k], q[2 * k + 1]);
+            if (MASS) any = q[0] != 0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) val[c] = (float)q[c] * S.inv[c];
+        } else {
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-                const long long q = (long long)lds[c * kFWS + fixed_slot_of(idx)];
-                if (q != 0) unsafeAtomicAdd(base + c * cstride + node, (float)((double)q * (double)S.inv[c]));
+                const long long q = (long long)lds[c * stride + slot];
+                if (MASS && c == 0) any = q != 0;
+                val[c] = (float)((double)q * (double)S.inv[c]);
             }
         }
-    } else {
-        for (int idx = threadIdx.x; idx < W.vol; idx += blockDim.x) {
-            const long long node = win_node(sp, W, idx);
-            if (MASS && lds[idx] == 0ull) continue;
-            if (gm && gm[node] == 0.f) continue;
-            if (flags) flags[(nofs + node) >> kTileShift] = 1u;
+        if (!any) continue;
+        if (gm && gm[node] == 0.f) continue;
+        if (flags) flags[(nofs + node) >> kTileShift] = 1u;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                const long long q = (long long)lds[c * W.vol + idx];
-                if (q != 0) unsafeAtomicAdd(base + c * cstride + node, (float)((double)q * (double)S.inv[c]));
-            }
-        }
+        for (int c = 0; c < NC; ++c)
+            if (val[c] != 0.f) unsafeAtomicAdd(base + c * cstride + node, val[c]);
     }
 }
 
@@ -335,10 +379,11 @@ __device__ __forceinline__ void stm(T* base, int field, int S, int p, const M3<T
 
 // Scatter of NC channels over a particle's 64 nodes.  val(i, j, k, out)
 // fills the NC contributions (already in scaled units for the LDS modes).
+// PACK: two channels per LDS word.
 // MODE 0: fixed window, 1: general LDS window, 2: HBM float atomics (nodes
 // where gate is 0 are skipped there, as the LDS modes' flush skips them; the
 // tile flags of the nodes written are set when flags is given).
-template <int NC, int MODE, typename ValFn>
+template <int NC, int MODE, bool PACK, typename ValFn>
 __device__ __forceinline__ void scatter64(const StepParams& sp, const Win& W, const Axis<float> st[3],
                                           unsigned long long* lds, float* hbm, ValFn&& val, const float* gate = nullptr,
                                           unsigned* flags = nullptr, long long nofs = 0)
@@ -358,14 +403,16 @@ __device__ __forceinline__ void scatter64(const StepParams& sp, const Win& W, co
                 if (MODE != 0 && (gk < 0 || gk >= sp.dim[2])) continue;
                 float v[NC];
                 val(i, j, kk, v);
-                if (MODE == 0) {
-                    const int o = slot + i * kSI + j * kSJ + kk;
+                if (MODE != 2) {
+                    const int o = MODE == 0 ? slot + i * kSI + j * kSJ + kk : win_local(W, gi, gj, gk);
+                    const int stride = MODE == 0 ? kFWS : W.vol;
+                    if (PACK) {
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) atomicAdd(&lds[c * kFWS + o], fix32(v[c]));
-                } else if (MODE == 1) {
-                    const int o = win_local(W, gi, gj, gk);
+                        for (int c = 0; c < NC; c += 2) atomicAdd(&lds[(c / 2) * stride + o], pack2(v[c], c + 1 < NC ? v[c + 1] : 0.f));
+                    } else {
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) atomicAdd(&lds[c * W.vol + o], fix32(v[c]));
+                        for (int c = 0; c < NC; ++c) atomicAdd(&lds[c * stride + o], fix32(v[c]));
+                    }
                 } else {
                     const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
                     if (gate && gate[node] == 0.f) continue;
@@ -476,7 +523,7 @@ __global__ __launch_bounds__(kScat, 4) void k_p2g_blk(Batch bt, StepIO<float> io
         for (int a = 0; a < 3; ++a)
             bound[1 + a] = fabsf(mv[a]) + 2.f * dx * (fabsf(A.a[3 * a]) + fabsf(A.a[3 * a + 1]) + fabsf(A.a[3 * a + 2]));
     }
-    const Scales<4> SC = block_scales<4>(bound);
+    const Scales<4> SC = block_scales<4, false>(bound, rg.y - rg.x);
     const int mode = (W.fixed && SC.ok) ? 0 : ((W.lds && SC.ok) ? 1 : 2);
     if (mode == 0)
         for (int i = threadIdx.x; i < 4 * kFWS; i += blockDim.x) lds64[i] = 0ull;
@@ -505,13 +552,13 @@ __global__ __launch_bounds__(kScat, 4) void k_p2g_blk(Batch bt, StepIO<float> io
 #pragma unroll
             for (int a = 0; a < 3; ++a) out[1 + a] = w * (mv[a] + A.a[3 * a] * d0s[i] + A.a[3 * a + 1] * d1s[j] + A.a[3 * a + 2] * d2s[kk]);
         };
-        if (mode == 0) scatter64<4, 0>(sp, W, st, lds64, io.grid, val);
-        else if (mode == 1) scatter64<4, 1>(sp, W, st, lds64, io.grid, val);
-        else scatter64<4, 2>(sp, W, st, lds64, io.grid, val, nullptr, io.gflags, nofs);
+        if (mode == 0) scatter64<4, 0, false>(sp, W, st, lds64, io.grid, val);
+        else if (mode == 1) scatter64<4, 1, false>(sp, W, st, lds64, io.grid, val);
+        else scatter64<4, 2, false>(sp, W, st, lds64, io.grid, val, nullptr, io.gflags, nofs);
     }
     if (mode != 2) {
         __syncthreads();
-        flush_fixed<4, true>(sp, W, lds64, SC, io.grid, sp.G, nullptr, io.gflags, nofs);
+        flush_fixed<4, true, false>(sp, W, lds64, SC, io.grid, sp.G, nullptr, io.gflags, nofs);
     }
 }
 
@@ -535,7 +582,7 @@ __global__ __launch_bounds__(kScat) void k_p2g_mass_blk(Batch bt, const float* l
         m = cst[KM * S + p];
         bound[0] = fabsf(m);
     }
-    const Scales<1> SC = block_scales<1>(bound);
+    const Scales<1> SC = block_scales<1, false>(bound, rg.y - rg.x);
     const int mode = (W.fixed && SC.ok) ? 0 : ((W.lds && SC.ok) ? 1 : 2);
     if (mode == 0)
         for (int i = threadIdx.x; i < kFWS; i += blockDim.x) lds64[i] = 0ull;
@@ -547,13 +594,13 @@ __global__ __launch_bounds__(kScat) void k_p2g_mass_blk(Batch bt, const float* l
         stencil_f(sp, x, st, false);
         if (mode != 2) m *= SC.s[0];
         auto val = [&](int i, int j, int kk, float* out) { out[0] = st[0].w[i] * st[1].w[j] * st[2].w[kk] * m; };
-        if (mode == 0) scatter64<1, 0>(sp, W, st, lds64, gm, val);
-        else if (mode == 1) scatter64<1, 1>(sp, W, st, lds64, gm, val);
-        else scatter64<1, 2>(sp, W, st, lds64, gm, val);
+        if (mode == 0) scatter64<1, 0, false>(sp, W, st, lds64, gm, val);
+        else if (mode == 1) scatter64<1, 1, false>(sp, W, st, lds64, gm, val);
+        else scatter64<1, 2, false>(sp, W, st, lds64, gm, val);
     }
     if (mode != 2) {
         __syncthreads();
-        flush_fixed<1, true>(sp, W, lds64, SC, gm, sp.G);
+        flush_fixed<1, true, false>(sp, W, lds64, SC, gm, sp.G);
     }
 }
 
@@ -674,12 +721,12 @@ __global__ __launch_bounds__(kScat) void k_g2p_adj_blk(Batch bt, AdjIO<float> io
         for (int a = 0; a < 3; ++a)
             bound[a] = fabsf(vt[a]) + 2.f * Dd * (fabsf(Ct.a[3 * a]) + fabsf(Ct.a[3 * a + 1]) + fabsf(Ct.a[3 * a + 2]));
     }
-    const Scales<3> SC = block_scales<3>(bound);
+    const Scales<3> SC = block_scales<3, true>(bound, rg.y - rg.x);
     const int mode = (W.fixed && SC.ok) ? 0 : ((W.lds && SC.ok) ? 1 : 2);
     if (mode == 0)
-        for (int i = threadIdx.x; i < 3 * kFWS; i += blockDim.x) lds64[i] = 0ull;
+        for (int i = threadIdx.x; i < 2 * kFWS; i += blockDim.x) lds64[i] = 0ull;
     else if (mode == 1)
-        for (int i = threadIdx.x; i < 3 * W.vol; i += blockDim.x) lds64[i] = 0ull;
+        for (int i = threadIdx.x; i < 2 * W.vol; i += blockDim.x) lds64[i] = 0ull;
     __syncthreads();
     if (valid) {
         Axis<float> st[3];
@@ -701,13 +748,13 @@ __global__ __launch_bounds__(kScat) void k_g2p_adj_blk(Batch bt, AdjIO<float> io
 #pragma unroll
             for (int a = 0; a < 3; ++a) out[a] = w * (vt[a] + Ct.a[3 * a] * e0s[i] + Ct.a[3 * a + 1] * e1s[j] + Ct.a[3 * a + 2] * e2s[kk]);
         };
-        if (mode == 0) scatter64<3, 0>(sp, W, st, lds64, io.agrid, val);
-        else if (mode == 1) scatter64<3, 1>(sp, W, st, lds64, io.agrid, val);
-        else scatter64<3, 2>(sp, W, st, lds64, io.agrid, val, io.grid + 4 * nofs, io.aflags, nofs);
+        if (mode == 0) scatter64<3, 0, true>(sp, W, st, lds64, io.agrid, val);
+        else if (mode == 1) scatter64<3, 1, true>(sp, W, st, lds64, io.agrid, val);
+        else scatter64<3, 2, true>(sp, W, st, lds64, io.agrid, val, io.grid + 4 * nofs, io.aflags, nofs);
     }
     if (mode != 2) {
         __syncthreads();
-        flush_fixed<3, false>(sp, W, lds64, SC, io.agrid, sp.G, io.grid + 4 * nofs, io.aflags, nofs);
+        flush_fixed<3, false, true>(sp, W, lds64, SC, io.agrid, sp.G, io.grid + 4 * nofs, io.aflags, nofs);
     }
 }
 
@@ -1031,7 +1078,7 @@ __global__ __launch_bounds__(kGat) void k_loss_particles_b(Batch bt, const float
 }
 
 // ------------------------------------------------------------ launchers ----
-// LDS: NC channels of max(cap, fixed-window slots) (int64 for scatters, float for gathers)
+// LDS: max(cap, fixed-window slots) nodes of 64-bit words (scatters: NC, or (NC + 1) / 2 packed) or NC floats (gathers)
 static size_t win_nodes(int cap) { return (size_t)(cap > kFWS ? cap : kFWS); }
 
 void launch_p2g_blk(const Batch& b, const StepIO<float>& io, hipStream_t s)
@@ -1056,7 +1103,7 @@ void launch_g2p_blk(const Batch& b, const StepIO<float>& io, hipStream_t s)
 void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
 {
     if (b.nranges <= 0) return;
-    hipLaunchKernelGGL(k_g2p_adj_blk, dim3(b.nranges), dim3(kScat), 3 * win_nodes(b.cap) * sizeof(unsigned long long), s, b, io);
+    hipLaunchKernelGGL(k_g2p_adj_blk, dim3(b.nranges), dim3(kScat), 2 * win_nodes(b.cap) * sizeof(unsigned long long), s, b, io);
 }
 
 void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
