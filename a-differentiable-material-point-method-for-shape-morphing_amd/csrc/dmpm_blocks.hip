@@ -426,10 +426,12 @@ __device__ __forceinline__ void scatter64(const StepParams& sp, const Win& W, co
 }
 
 // Gather over a particle's 64 nodes: fn(i, j, k, ch) with ch[c] the node's
-// channel c.  MODE 0: fixed window, 1: general LDS window, 2: HBM.
-template <int NC, int MODE, typename Ld, typename Fn>
+// channel c, then row(i, j) after the nodes of each (i, j) row of 4 (the
+// kernels sum a row with the k-weights first and apply the (i, j) factors
+// once per row).  MODE 0: fixed window, 1: general LDS window, 2: HBM.
+template <int NC, int MODE, typename Ld, typename Fn, typename RowFn>
 __device__ __forceinline__ void gather64(const StepParams& sp, const Win& W, const Axis<float> st[3], const float* lds, Ld&& ld,
-                                         Fn&& fn)
+                                         Fn&& fn, RowFn&& row)
 {
     const int slot = MODE == 0 ? fixed_slot(W, st) : 0;
 #pragma unroll
@@ -458,6 +460,7 @@ __device__ __forceinline__ void gather64(const StepParams& sp, const Win& W, con
                 }
                 fn(i, j, kk, ch);
             }
+            row(i, j);
         }
     }
 }
@@ -636,20 +639,34 @@ __global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io)
         offsets4(st[2], Dd, e2s);
         float vp[3] = {0.f, 0.f, 0.f};
         M3<float> Cp = m3_zero<float>();
-        auto acc = [&](int i, int j, int kk, const float* vi) {
-            const float w = st[0].w[i] * st[1].w[j] * st[2].w[kk];
+        // row sums R = sum_k w_k v, Re = sum_k w_k e_k v
+        float we2[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) we2[k] = st[2].w[k] * e2s[k];
+        float R[3] = {0.f, 0.f, 0.f}, Re[3] = {0.f, 0.f, 0.f};
+        auto acc = [&](int, int, int kk, const float* vi) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                const float wv = w * vi[a];
-                vp[a] += wv;
-                Cp.a[3 * a] += wv * e0s[i];
-                Cp.a[3 * a + 1] += wv * e1s[j];
-                Cp.a[3 * a + 2] += wv * e2s[kk];
+                R[a] += st[2].w[kk] * vi[a];
+                Re[a] += we2[kk] * vi[a];
             }
         };
-        if (W.fixed) gather64<3, 0>(sp, W, st, lds, ld, acc);
-        else if (W.lds) gather64<3, 1>(sp, W, st, lds, ld, acc);
-        else gather64<3, 2>(sp, W, st, lds, ld, acc);
+        auto row = [&](int i, int j) {
+            const float w01 = st[0].w[i] * st[1].w[j];
+            const float c0 = w01 * e0s[i], c1 = w01 * e1s[j];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                vp[a] += w01 * R[a];
+                Cp.a[3 * a] += c0 * R[a];
+                Cp.a[3 * a + 1] += c1 * R[a];
+                Cp.a[3 * a + 2] += w01 * Re[a];
+                R[a] = 0.f;
+                Re[a] = 0.f;
+            }
+        };
+        if (W.fixed) gather64<3, 0>(sp, W, st, lds, ld, acc, row);
+        else if (W.lds) gather64<3, 1>(sp, W, st, lds, ld, acc, row);
+        else gather64<3, 2>(sp, W, st, lds, ld, acc, row);
         const M3<float> F = ldm(io.cur, FF, S, p);
         const M3<float> dF = ldm(io.ctrl, 0, S, p);
         M3<float> Fc, IC;
@@ -890,6 +907,8 @@ __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
         for (int i = 0; i < 9; ++i) CtD.a[i] = Dinv * Ct.a[i];
         M3<float> Gb = m3_zero<float>();
         float xg[3] = {0.f, 0.f, 0.f}, pacc[3] = {0.f, 0.f, 0.f}, vacc[3] = {0.f, 0.f, 0.f};
+        // per node (a row-wise contraction as in G2P needs 212 VGPRs here,
+        // occupancy 2, and measured 213 us against 181 us)
         auto acc = [&](int i, int j, int kk, const float* ch) {
             const float d0 = d0s[i], d1 = d1s[j], d2 = d2s[kk];
             const float w01 = st[0].w[i] * st[1].w[j];
@@ -913,9 +932,10 @@ __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
 #pragma unroll
             for (int a = 0; a < 3; ++a) xg[a] += gw[a] * sgw;
         };
-        if (W.fixed) gather64<7, 0>(sp, W, st, lds, ld, acc);
-        else if (W.lds) gather64<7, 1>(sp, W, st, lds, ld, acc);
-        else gather64<7, 2>(sp, W, st, lds, ld, acc);
+        auto row = [](int, int) {};
+        if (W.fixed) gather64<7, 0>(sp, W, st, lds, ld, acc, row);
+        else if (W.lds) gather64<7, 1>(sp, W, st, lds, ld, acc, row);
+        else gather64<7, 2>(sp, W, st, lds, ld, acc, row);
         float xb[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
