@@ -475,57 +475,88 @@ __device__ __forceinline__ void offsets4(const Axis<float>& ax, float h, float o
 }  // namespace
 
 // ------------------------------------------------------------------ P2G ----
-__global__ __launch_bounds__(kScat, 4) void k_p2g_blk(Batch bt, StepIO<float> io)
+// One particle's P2G inputs: position, m (1 - dt drag) v, m, and the affine
+// matrix A = -(3/dx^2) dt vol P(Fc) Fc^T + m C (FP64 polar and PK1), which is
+// also cached in the layer for the transposed P2G.
+struct P2GPart {
+    float x[3], mv[3], m;
+    M3<float> A;
+};
+
+__device__ __forceinline__ void p2g_prepare(const StepParams& sp, const StepIO<float>& io, int p, P2GPart& q, float bound[4])
+{
+    const int S = sp.stride;
+    const float dx = float(sp.dx);
+    const float sdrag = 1.f - float(sp.dt) * float(sp.drag);
+    const double kst = -3.0 * sp.inv_dx * sp.inv_dx * sp.dt;
+    // the elastic part first; the fields it does not need are loaded after
+    // it, so that they do not occupy registers across the FP64 polar
+    const M3<float> F = ldm(io.cur, FF, S, p);
+    const M3<float> dF = ldm(io.ctrl, 0, S, p);
+    const double vol = io.cst[KVOL * S + p];
+    const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
+    M3<double> Fc;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
+    FCE e;
+    fce_prepare(Fc, e);
+    const M3<double> PF = mult(fce_pk1(Fc, e, lam, mu), Fc);
+    const double k = kst * vol;
+    float kPF[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) kPF[i] = float(k * PF.a[i]);
+    float v[3];
+    ld3(io.cur, FX, S, p, q.x);
+    ld3(io.cur, FV, S, p, v);
+    const M3<float> C = ldm(io.cur, FC, S, p);
+    q.m = io.cst[KM * S + p];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) q.A.a[i] = kPF[i] + q.m * C.a[i];
+    if (io.aux) stm(io.aux, FA, S, p, q.A);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) q.mv[a] = q.m * v[a] * sdrag;
+    // every contribution: |w (mv + A d)| <= |mv| + 2 dx sum_b |A_ab|  (w <= 1, |d| <= 2 dx)
+    bound[0] = fmaxf(bound[0], fabsf(q.m));
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        bound[1 + a] = fmaxf(bound[1 + a], fabsf(q.mv[a]) + 2.f * dx * (fabsf(q.A.a[3 * a]) + fabsf(q.A.a[3 * a + 1]) +
+                                                                         fabsf(q.A.a[3 * a + 2])));
+}
+
+__device__ __forceinline__ void p2g_scale(P2GPart& q, const Scales<4>& SC)
+{
+    q.m *= SC.s[0];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        q.mv[a] *= SC.s[1 + a];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) q.A.a[3 * a + b] *= SC.s[1 + a];
+    }
+}
+
+// Two particles per thread: particles t and t + kP2G of the range.  With the
+// block-sorted order (rank in cell, then cell) these are particles of one
+// cell, so they usually share their stencil base; a wave whose lanes all do
+// (or have no second particle) adds both with one LDS atomic per node and
+// channel, which halves the atomics that bound this kernel.  Other waves add
+// the two particles one after the other.
+constexpr int kP2G = 256;
+
+__global__ __launch_bounds__(kP2G, 3) void k_p2g_blk(Batch bt, StepIO<float> io)
 {
     extern __shared__ unsigned long long lds64[];
     const int4 rg = bt.ranges[blockIdx.x];
     const StepParams sp = bt.sps[rg.z];
     const long long nofs = bt.nofs[rg.z];
     io.grid += 4 * nofs;
-    const int cap = bt.cap;
-    const int S = sp.stride;
-    const int p = rg.x + (int)threadIdx.x;
-    const bool valid = p < rg.y;
-    const Win W = range_window(sp, io.cur, S, rg.x, rg.y, cap);
+    const int p0 = rg.x + (int)threadIdx.x, p1 = p0 + kP2G;
+    const bool v0 = p0 < rg.y, v1 = p1 < rg.y;
+    const Win W = range_window(sp, io.cur, sp.stride, rg.x, rg.y, bt.cap);
     const float dx = float(sp.dx);
-    const float sdrag = 1.f - float(sp.dt) * float(sp.drag);
-    const double kst = -3.0 * sp.inv_dx * sp.inv_dx * sp.dt;
-    float x[3] = {0.f, 0.f, 0.f}, mv[3] = {0.f, 0.f, 0.f}, m = 0.f;
-    M3<float> A = m3_zero<float>();
+    P2GPart q0{}, q1{};
     float bound[4] = {0.f, 0.f, 0.f, 0.f};
-    if (valid) {
-        // the elastic part first; the fields it does not need are loaded
-        // after it, so that they do not occupy registers across the FP64 polar
-        const M3<float> F = ldm(io.cur, FF, S, p);
-        const M3<float> dF = ldm(io.ctrl, 0, S, p);
-        const double vol = io.cst[KVOL * S + p];
-        const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
-        M3<double> Fc;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
-        FCE e;
-        fce_prepare(Fc, e);
-        const M3<double> PF = mult(fce_pk1(Fc, e, lam, mu), Fc);
-        const double k = kst * vol;
-        float kPF[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) kPF[i] = float(k * PF.a[i]);
-        float v[3];
-        ld3(io.cur, FX, S, p, x);
-        ld3(io.cur, FV, S, p, v);
-        const M3<float> C = ldm(io.cur, FC, S, p);
-        m = io.cst[KM * S + p];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) A.a[i] = kPF[i] + m * C.a[i];
-        if (io.aux) stm(io.aux, FA, S, p, A);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) mv[a] = m * v[a] * sdrag;
-        // every contribution: |w (mv + A d)| <= |mv| + 2 dx sum_b |A_ab|  (w <= 1, |d| <= 2 dx)
-        bound[0] = fabsf(m);
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-            bound[1 + a] = fabsf(mv[a]) + 2.f * dx * (fabsf(A.a[3 * a]) + fabsf(A.a[3 * a + 1]) + fabsf(A.a[3 * a + 2]));
-    }
+    if (v0) p2g_prepare(sp, io, p0, q0, bound);
+    if (v1) p2g_prepare(sp, io, p1, q1, bound);
     const Scales<4> SC = block_scales<4, false>(bound, rg.y - rg.x);
     const int mode = (W.fixed && SC.ok) ? 0 : ((W.lds && SC.ok) ? 1 : 2);
     if (mode == 0)
@@ -533,31 +564,58 @@ __global__ __launch_bounds__(kScat, 4) void k_p2g_blk(Batch bt, StepIO<float> io
     else if (mode == 1)
         for (int i = threadIdx.x; i < 4 * W.vol; i += blockDim.x) lds64[i] = 0ull;
     __syncthreads();
-    if (valid) {
-        Axis<float> st[3];
-        stencil_f(sp, x, st, false);
-        if (mode != 2) {  // scaled units
-            m *= SC.s[0];
+    if (mode != 2) {  // scaled units
+        p2g_scale(q0, SC);
+        p2g_scale(q1, SC);
+    }
+    Axis<float> s0[3], s1[3];
+    stencil_f(sp, q0.x, s0, false);
+    stencil_f(sp, q1.x, s1, false);
+    const bool same = v1 && s0[0].base == s1[0].base && s0[1].base == s1[1].base && s0[2].base == s1[2].base;
+    if (mode == 0 && __all(!v1 || same)) {
+        if (v0) {
+            float a0[4], a1[4], a2[4], b0[4], b1[4], b2[4];
+            offsets4(s0[0], dx, a0);
+            offsets4(s0[1], dx, a1);
+            offsets4(s0[2], dx, a2);
+            offsets4(s1[0], dx, b0);
+            offsets4(s1[1], dx, b1);
+            offsets4(s1[2], dx, b2);
+            // without a second particle q1 is all zero: its terms vanish
+            auto val = [&](int i, int j, int kk, float* out) {
+                const float wa = s0[0].w[i] * s0[1].w[j] * s0[2].w[kk];
+                const float wb = s1[0].w[i] * s1[1].w[j] * s1[2].w[kk];
+                out[0] = wa * q0.m + wb * q1.m;
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                mv[a] *= SC.s[1 + a];
-#pragma unroll
-                for (int b = 0; b < 3; ++b) A.a[3 * a + b] *= SC.s[1 + a];
-            }
+                for (int a = 0; a < 3; ++a)
+                    out[1 + a] = wa * (q0.mv[a] + q0.A.a[3 * a] * a0[i] + q0.A.a[3 * a + 1] * a1[j] + q0.A.a[3 * a + 2] * a2[kk]) +
+                                 wb * (q1.mv[a] + q1.A.a[3 * a] * b0[i] + q1.A.a[3 * a + 1] * b1[j] + q1.A.a[3 * a + 2] * b2[kk]);
+            };
+            scatter64<4, 0, false>(sp, W, s0, lds64, io.grid, val);
         }
-        float d0s[4], d1s[4], d2s[4];
-        offsets4(st[0], dx, d0s);
-        offsets4(st[1], dx, d1s);
-        offsets4(st[2], dx, d2s);
-        auto val = [&](int i, int j, int kk, float* out) {
-            const float w = st[0].w[i] * st[1].w[j] * st[2].w[kk];
-            out[0] = w * m;
+    } else {
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            if (!(h ? v1 : v0)) continue;
+            const P2GPart& q = h ? q1 : q0;
+            Axis<float> st[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) out[1 + a] = w * (mv[a] + A.a[3 * a] * d0s[i] + A.a[3 * a + 1] * d1s[j] + A.a[3 * a + 2] * d2s[kk]);
-        };
-        if (mode == 0) scatter64<4, 0, false>(sp, W, st, lds64, io.grid, val);
-        else if (mode == 1) scatter64<4, 1, false>(sp, W, st, lds64, io.grid, val);
-        else scatter64<4, 2, false>(sp, W, st, lds64, io.grid, val, nullptr, io.gflags, nofs);
+            for (int c = 0; c < 3; ++c) st[c] = h ? s1[c] : s0[c];
+            float d0s[4], d1s[4], d2s[4];
+            offsets4(st[0], dx, d0s);
+            offsets4(st[1], dx, d1s);
+            offsets4(st[2], dx, d2s);
+            auto val = [&](int i, int j, int kk, float* out) {
+                const float w = st[0].w[i] * st[1].w[j] * st[2].w[kk];
+                out[0] = w * q.m;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    out[1 + a] = w * (q.mv[a] + q.A.a[3 * a] * d0s[i] + q.A.a[3 * a + 1] * d1s[j] + q.A.a[3 * a + 2] * d2s[kk]);
+            };
+            if (mode == 0) scatter64<4, 0, false>(sp, W, st, lds64, io.grid, val);
+            else if (mode == 1) scatter64<4, 1, false>(sp, W, st, lds64, io.grid, val);
+            else scatter64<4, 2, false>(sp, W, st, lds64, io.grid, val, nullptr, io.gflags, nofs);
+        }
     }
     if (mode != 2) {
         __syncthreads();
@@ -1104,7 +1162,7 @@ static size_t win_nodes(int cap) { return (size_t)(cap > kFWS ? cap : kFWS); }
 void launch_p2g_blk(const Batch& b, const StepIO<float>& io, hipStream_t s)
 {
     if (b.nranges <= 0) return;
-    hipLaunchKernelGGL(k_p2g_blk, dim3(b.nranges), dim3(kScat), 4 * win_nodes(b.cap) * sizeof(unsigned long long), s, b, io);
+    hipLaunchKernelGGL(k_p2g_blk, dim3(b.nranges), dim3(kP2G), 4 * win_nodes(b.cap) * sizeof(unsigned long long), s, b, io);
 }
 
 void launch_p2g_mass_blk(const Batch& b, const float* lay, const float* cst, float* gm, hipStream_t s)
