@@ -752,50 +752,72 @@ __global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io)
 // -------------------------------------------------------- transposed G2P ---
 // P_op_2^T + G2P^T: vt = v' + dt x', Ct = C' + (1-s) dt F' Fc^T (kept in tmp),
 // vbar_i += w (vt + D Ct d), into the adjoint grid (3 channels; nodes of zero
-// forward mass are skipped: their pbar is 0 whatever vbar is).
-__global__ __launch_bounds__(kScat) void k_g2p_adj_blk(Batch bt, AdjIO<float> io)
+// forward mass are skipped: their pbar is 0 whatever vbar is).  Two particles
+// per thread, paired as in P2G.
+struct G2PAdjPart {
+    float x[3], vt[3];
+    M3<float> Ct;
+};
+
+__device__ __forceinline__ void g2p_adj_prepare(const StepParams& sp, const AdjIO<float>& io, int p, G2PAdjPart& q, float bound[3])
+{
+    const int S = sp.stride;
+    const float Dd = float(3.0 * sp.inv_dx);
+    const float dt = float(sp.dt);
+    const float cs = (sp.compat & 2) ? 1.f : 1.f - float(sp.smoothing);
+    float gx1[3], gv1[3];
+    ld3(io.cur, FX, S, p, q.x);
+    ld3(io.g1, FX, S, p, gx1);
+    ld3(io.g1, FV, S, p, gv1);
+    const M3<float> F = ldm(io.cur, FF, S, p);
+    const M3<float> dF = ldm(io.ctrl, 0, S, p);
+    const M3<float> gF1 = ldm(io.g1, FF, S, p);
+    const M3<float> gC1 = ldm(io.g1, FC, S, p);
+    M3<float> Fc;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fc.a[i] = F.a[i] + dF.a[i];
+    const M3<float> FF_ = mult(gF1, Fc);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) q.Ct.a[i] = gC1.a[i] + cs * dt * FF_.a[i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) q.vt[a] = gv1[a] + dt * gx1[a];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) io.tmp[(long long)a * S + p] = q.vt[a];
+    stm(io.tmp, 3, S, p, q.Ct);
+    // every contribution: |w (vt + Ct e)| <= |vt| + (6/dx) sum_b |Ct_ab|  (w <= 1, |e| = |D d| <= 6/dx)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        bound[a] = fmaxf(bound[a], fabsf(q.vt[a]) + 2.f * Dd * (fabsf(q.Ct.a[3 * a]) + fabsf(q.Ct.a[3 * a + 1]) +
+                                                                fabsf(q.Ct.a[3 * a + 2])));
+}
+
+__device__ __forceinline__ void g2p_adj_scale(G2PAdjPart& q, const Scales<3>& SC)
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        q.vt[a] *= SC.s[a];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) q.Ct.a[3 * a + b] *= SC.s[a];
+    }
+}
+
+constexpr int kG2PAdj = 256;
+
+__global__ __launch_bounds__(kG2PAdj, 4) void k_g2p_adj_blk(Batch bt, AdjIO<float> io)
 {
     extern __shared__ unsigned long long lds64[];
     const int4 rg = bt.ranges[blockIdx.x];
     const StepParams sp = bt.sps[rg.z];
     const long long nofs = bt.nofs[rg.z];
     io.agrid += 4 * nofs;
-    const int cap = bt.cap;
-    const int S = sp.stride;
-    const int p = rg.x + (int)threadIdx.x;
-    const bool valid = p < rg.y;
-    const Win W = range_window(sp, io.cur, S, rg.x, rg.y, cap);
+    const int p0 = rg.x + (int)threadIdx.x, p1 = p0 + kG2PAdj;
+    const bool v0 = p0 < rg.y, v1 = p1 < rg.y;
+    const Win W = range_window(sp, io.cur, sp.stride, rg.x, rg.y, bt.cap);
     const float Dd = float(3.0 * sp.inv_dx);
-    const float dt = float(sp.dt);
-    const float cs = (sp.compat & 2) ? 1.f : 1.f - float(sp.smoothing);
-    float x[3] = {0.f, 0.f, 0.f}, vt[3] = {0.f, 0.f, 0.f};
-    M3<float> Ct = m3_zero<float>();
+    G2PAdjPart q0{}, q1{};
     float bound[3] = {0.f, 0.f, 0.f};
-    if (valid) {
-        float gx1[3], gv1[3];
-        ld3(io.cur, FX, S, p, x);
-        ld3(io.g1, FX, S, p, gx1);
-        ld3(io.g1, FV, S, p, gv1);
-        const M3<float> F = ldm(io.cur, FF, S, p);
-        const M3<float> dF = ldm(io.ctrl, 0, S, p);
-        const M3<float> gF1 = ldm(io.g1, FF, S, p);
-        const M3<float> gC1 = ldm(io.g1, FC, S, p);
-        M3<float> Fc;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) Fc.a[i] = F.a[i] + dF.a[i];
-        const M3<float> FF_ = mult(gF1, Fc);
-#pragma unroll
-        for (int i = 0; i < 9; ++i) Ct.a[i] = gC1.a[i] + cs * dt * FF_.a[i];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) vt[a] = gv1[a] + dt * gx1[a];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) io.tmp[(long long)a * S + p] = vt[a];
-        stm(io.tmp, 3, S, p, Ct);
-        // every contribution: |w (vt + Ct e)| <= |vt| + (6/dx) sum_b |Ct_ab|  (w <= 1, |e| = |D d| <= 6/dx)
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-            bound[a] = fabsf(vt[a]) + 2.f * Dd * (fabsf(Ct.a[3 * a]) + fabsf(Ct.a[3 * a + 1]) + fabsf(Ct.a[3 * a + 2]));
-    }
+    if (v0) g2p_adj_prepare(sp, io, p0, q0, bound);
+    if (v1) g2p_adj_prepare(sp, io, p1, q1, bound);
     const Scales<3> SC = block_scales<3, true>(bound, rg.y - rg.x);
     const int mode = (W.fixed && SC.ok) ? 0 : ((W.lds && SC.ok) ? 1 : 2);
     if (mode == 0)
@@ -803,33 +825,61 @@ __global__ __launch_bounds__(kScat) void k_g2p_adj_blk(Batch bt, AdjIO<float> io
     else if (mode == 1)
         for (int i = threadIdx.x; i < 2 * W.vol; i += blockDim.x) lds64[i] = 0ull;
     __syncthreads();
-    if (valid) {
-        Axis<float> st[3];
-        stencil_f(sp, x, st, false);
-        if (mode != 2) {
+    if (mode != 2) {
+        g2p_adj_scale(q0, SC);
+        g2p_adj_scale(q1, SC);
+    }
+    Axis<float> s0[3], s1[3];
+    stencil_f(sp, q0.x, s0, false);
+    stencil_f(sp, q1.x, s1, false);
+    const bool same = v1 && s0[0].base == s1[0].base && s0[1].base == s1[1].base && s0[2].base == s1[2].base;
+    const float* gate = io.grid + 4 * nofs;
+    if (mode == 0 && __all(!v1 || same)) {
+        if (v0) {
+            float a0[4], a1[4], a2[4], b0[4], b1[4], b2[4];
+            offsets4(s0[0], Dd, a0);
+            offsets4(s0[1], Dd, a1);
+            offsets4(s0[2], Dd, a2);
+            offsets4(s1[0], Dd, b0);
+            offsets4(s1[1], Dd, b1);
+            offsets4(s1[2], Dd, b2);
+            // without a second particle q1 is all zero: its terms vanish
+            auto val = [&](int i, int j, int kk, float* out) {
+                const float wa = s0[0].w[i] * s0[1].w[j] * s0[2].w[kk];
+                const float wb = s1[0].w[i] * s1[1].w[j] * s1[2].w[kk];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                vt[a] *= SC.s[a];
-#pragma unroll
-                for (int b = 0; b < 3; ++b) Ct.a[3 * a + b] *= SC.s[a];
-            }
+                for (int a = 0; a < 3; ++a)
+                    out[a] = wa * (q0.vt[a] + q0.Ct.a[3 * a] * a0[i] + q0.Ct.a[3 * a + 1] * a1[j] + q0.Ct.a[3 * a + 2] * a2[kk]) +
+                             wb * (q1.vt[a] + q1.Ct.a[3 * a] * b0[i] + q1.Ct.a[3 * a + 1] * b1[j] + q1.Ct.a[3 * a + 2] * b2[kk]);
+            };
+            scatter64<3, 0, true>(sp, W, s0, lds64, io.agrid, val);
         }
-        float e0s[4], e1s[4], e2s[4];
-        offsets4(st[0], Dd, e0s);
-        offsets4(st[1], Dd, e1s);
-        offsets4(st[2], Dd, e2s);
-        auto val = [&](int i, int j, int kk, float* out) {
-            const float w = st[0].w[i] * st[1].w[j] * st[2].w[kk];
+    } else {
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            if (!(h ? v1 : v0)) continue;
+            const G2PAdjPart& q = h ? q1 : q0;
+            Axis<float> st[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) out[a] = w * (vt[a] + Ct.a[3 * a] * e0s[i] + Ct.a[3 * a + 1] * e1s[j] + Ct.a[3 * a + 2] * e2s[kk]);
-        };
-        if (mode == 0) scatter64<3, 0, true>(sp, W, st, lds64, io.agrid, val);
-        else if (mode == 1) scatter64<3, 1, true>(sp, W, st, lds64, io.agrid, val);
-        else scatter64<3, 2, true>(sp, W, st, lds64, io.agrid, val, io.grid + 4 * nofs, io.aflags, nofs);
+            for (int c = 0; c < 3; ++c) st[c] = h ? s1[c] : s0[c];
+            float e0s[4], e1s[4], e2s[4];
+            offsets4(st[0], Dd, e0s);
+            offsets4(st[1], Dd, e1s);
+            offsets4(st[2], Dd, e2s);
+            auto val = [&](int i, int j, int kk, float* out) {
+                const float w = st[0].w[i] * st[1].w[j] * st[2].w[kk];
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    out[a] = w * (q.vt[a] + q.Ct.a[3 * a] * e0s[i] + q.Ct.a[3 * a + 1] * e1s[j] + q.Ct.a[3 * a + 2] * e2s[kk]);
+            };
+            if (mode == 0) scatter64<3, 0, true>(sp, W, st, lds64, io.agrid, val);
+            else if (mode == 1) scatter64<3, 1, true>(sp, W, st, lds64, io.agrid, val);
+            else scatter64<3, 2, true>(sp, W, st, lds64, io.agrid, val, gate, io.aflags, nofs);
+        }
     }
     if (mode != 2) {
         __syncthreads();
-        flush_fixed<3, false, true>(sp, W, lds64, SC, io.agrid, sp.G, io.grid + 4 * nofs, io.aflags, nofs);
+        flush_fixed<3, false, true>(sp, W, lds64, SC, io.agrid, sp.G, gate, io.aflags, nofs);
     }
 }
 
@@ -1181,7 +1231,7 @@ void launch_g2p_blk(const Batch& b, const StepIO<float>& io, hipStream_t s)
 void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
 {
     if (b.nranges <= 0) return;
-    hipLaunchKernelGGL(k_g2p_adj_blk, dim3(b.nranges), dim3(kScat), 2 * win_nodes(b.cap) * sizeof(unsigned long long), s, b, io);
+    hipLaunchKernelGGL(k_g2p_adj_blk, dim3(b.nranges), dim3(kG2PAdj), 2 * win_nodes(b.cap) * sizeof(unsigned long long), s, b, io);
 }
 
 void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
