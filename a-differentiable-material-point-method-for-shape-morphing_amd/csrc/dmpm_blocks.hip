@@ -498,9 +498,7 @@ __device__ __forceinline__ void p2g_prepare(const StepParams& sp, const StepIO<f
     M3<double> Fc;
 #pragma unroll
     for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
-    FCE e;
-    fce_prepare(Fc, e);
-    const M3<double> PF = mult(fce_pk1(Fc, e, lam, mu), Fc);
+    const M3<double> PF = pk1_times_ft(Fc, lam, mu);
     const double k = kst * vol;
     float kPF[9];
 #pragma unroll
@@ -542,7 +540,7 @@ __device__ __forceinline__ void p2g_scale(P2GPart& q, const Scales<4>& SC)
 // the two particles one after the other.
 constexpr int kP2G = 256;
 
-__global__ __launch_bounds__(kP2G, 3) void k_p2g_blk(Batch bt, StepIO<float> io)
+__global__ __launch_bounds__(kP2G, 4) void k_p2g_blk(Batch bt, StepIO<float> io)
 {
     extern __shared__ unsigned long long lds64[];
     const int4 rg = bt.ranges[blockIdx.x];

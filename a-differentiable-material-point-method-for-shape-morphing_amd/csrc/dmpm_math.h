@@ -302,17 +302,17 @@ __device__ __forceinline__ void svd3(const M3<double>& A, M3<double>& Uo, double
 // the iteration stops once a step moves X by less than 1e-10 (quadratic
 // convergence leaves the next error below rounding).  Returns false (the
 // caller then takes the SVD) unless det F > 1e-6 |F|_F^3 and the iteration
-// converged in 16 steps.  Fit = F^-T and det = det F are inputs.
+// converged in 16 steps.  detF = det F is an input.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool polar_newton(const M3<double>& F, const M3<double>& Fit, double detF, M3<double>& R)
+__device__ __forceinline__ bool polar_newton(const M3<double>& F, double detF, M3<double>& R)
 {
     double nf = 0.0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) nf += F.a[i] * F.a[i];
     if (!(detF > 1e-6 * nf * sqrt(nf))) return false;
-    M3<double> X = F, Y = Fit;
+    M3<double> X = F, Y;
     for (int it = 0; it < 16; ++it) {
-        if (it > 0) {
+        {
             const M3<double> cf = cofactor(X);
             const double inv = 1.0 / (X.a[0] * cf.a[0] + X.a[1] * cf.a[1] + X.a[2] * cf.a[2]);
 #pragma unroll
@@ -366,7 +366,7 @@ __device__ __forceinline__ void fce_prepare(const M3<double>& F, FCE& e)
     double inv = 1.0 / e.Ju;
 #pragma unroll
     for (int i = 0; i < 9; ++i) e.Fit.a[i] = cf.a[i] * inv;
-    e.svd = !polar_newton(F, e.Fit, e.Ju, e.R);
+    e.svd = !polar_newton(F, e.Ju, e.R);
     if (e.svd) {
         svd3(F, e.U, e.s, e.V);
         e.R = mult(e.U, e.V);
@@ -377,6 +377,34 @@ __device__ __forceinline__ void fce_prepare(const M3<double>& F, FCE& e)
 #pragma unroll
             for (int j = 0; j < 3; ++j) e.S.a[3 * i + j] = 0.5 * (RF.a[3 * i + j] + RF.a[3 * j + i]);
     }
+}
+
+// P(F) F^T for the PK1 below: P = 2 mu (F - R) + lam (J - 1) J F^-T with the
+// clamped J and the unclamped F^-T, and F^-T F^T = I, so
+// P F^T = 2 mu (F - R) F^T + lam (J - 1) J I: only the rotation is needed
+// (the forward P2G's affine term).
+__device__ __forceinline__ M3<double> pk1_times_ft(const M3<double>& F, double lam, double mu)
+{
+    const double Ju = det(F);
+    const double J = Ju < kJMin ? kJMin : Ju;
+    M3<double> R;
+    if (!polar_newton(F, Ju, R)) {
+        M3<double> U, V;
+        double sv[3];
+        svd3(F, U, sv, V);
+        R = mult(U, V);
+    }
+    M3<double> D;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) D.a[i] = F.a[i] - R.a[i];
+    M3<double> PF = mult(D, F);
+    const double k = lam * (J - 1.0) * J;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) PF.a[i] *= 2.0 * mu;
+    PF.a[0] += k;
+    PF.a[4] += k;
+    PF.a[8] += k;
+    return PF;
 }
 
 // Elasticity.cpp:88-98
