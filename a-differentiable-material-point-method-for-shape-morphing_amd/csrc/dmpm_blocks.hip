@@ -474,6 +474,62 @@ __device__ __forceinline__ void offsets4(const Axis<float>& ax, float h, float o
 
 }  // namespace
 
+// ------------------------------------------------------------ tile clears --
+// instance owning flat node q (nofs ascending, B+1 entries)
+__device__ __forceinline__ int instance_of(const Batch& b, long long q)
+{
+    int lo = 0, hi = b.B - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.nofs[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Zero the 4 channels of every node of the flagged tiles and clear the flags,
+// as wave `wave` of `nwaves`.  Each wave reads the flags of 64 consecutive
+// tiles (one per lane, so every flag is read and reset by one lane), then
+// clears the flagged ones: lanes 0-31 channels 0 and 2, lanes 32-63 channels
+// 1 and 3 of the tile's 32 nodes.
+__device__ __forceinline__ void clear_tiles(const Batch& bt, float* grid, unsigned* flags, long long ntiles, long long wave,
+                                            long long nwaves)
+{
+    const int lane = threadIdx.x & 63;
+    const int c0 = lane >> 5;
+    const long long Gtot = bt.Gtot;
+    for (long long t0 = wave * 64; t0 < ntiles; t0 += nwaves * 64) {
+        const long long t = t0 + lane;
+        const unsigned f = t < ntiles ? flags[t] : 0u;
+        unsigned long long mask = __ballot(f != 0u);
+        if (f != 0u) flags[t] = 0u;
+        while (mask) {
+            const int b = __ffsll((long long)mask) - 1;
+            mask &= mask - 1;
+            const long long q = ((t0 + b) << kTileShift) + (lane & 31);
+            if (q >= Gtot) continue;
+            long long n0 = 0, Gi = Gtot;
+            if (bt.B > 1) {
+                const int i = instance_of(bt, q);
+                n0 = bt.nofs[i];
+                Gi = bt.nofs[i + 1] - n0;
+            }
+            float* g = grid + 4 * n0 + (q - n0);
+            g[c0 * Gi] = 0.f;
+            g[(c0 + 2) * Gi] = 0.f;
+        }
+    }
+}
+
+// a kernel's workgroups from `first` on clear tc's grid (k_g2p_blk and
+// k_elastic_adj carry the clear of a grid the next kernels scatter into)
+__device__ __forceinline__ void appended_clear(const Batch& bt, const TileClear& tc, unsigned first)
+{
+    const long long nb = (long long)gridDim.x - first;
+    clear_tiles(bt, tc.grid, tc.flags, num_tiles(bt.Gtot), ((long long)(blockIdx.x - first) * blockDim.x + threadIdx.x) >> 6,
+                (nb * blockDim.x) >> 6);
+}
+
 // ------------------------------------------------------------------ P2G ----
 // One particle's P2G inputs: position, m (1 - dt drag) v, m, and the affine
 // matrix A = -(3/dx^2) dt vol P(Fc) Fc^T + m C (FP64 polar and PK1), which is
@@ -664,8 +720,12 @@ __global__ __launch_bounds__(kScat) void k_p2g_mass_blk(Batch bt, const float* l
 }
 
 // ------------------------------------------------------------------ G2P ----
-__global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io)
+__global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io, TileClear tc)
 {
+    if (blockIdx.x >= (unsigned)bt.nranges) {
+        appended_clear(bt, tc, (unsigned)bt.nranges);
+        return;
+    }
     extern __shared__ float lds[];
     const int4 rg = bt.ranges[blockIdx.x];
     const StepParams sp = bt.sps[rg.z];
@@ -1065,8 +1125,12 @@ __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
 }
 
 // Elastic half as a kernel of its own (Gb from tmp), one thread per particle of a range.
-__global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io)
+__global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io, TileClear tc)
 {
+    if (blockIdx.x >= (unsigned)bt.nranges) {
+        appended_clear(bt, tc, (unsigned)bt.nranges);
+        return;
+    }
     const int4 rg = bt.ranges[blockIdx.x];
     const StepParams sp = bt.sps[rg.z];
     const int S = sp.stride;
@@ -1074,52 +1138,11 @@ __global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io)
 }
 
 // --------------------------------------------------- batched grid kernels --
-namespace {
-// instance owning flat node q (nofs ascending, B+1 entries)
-__device__ __forceinline__ int instance_of(const Batch& b, long long q)
-{
-    int lo = 0, hi = b.B - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (b.nofs[mid] <= q) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-}  // namespace
-
-// Zero the 4 channels of every node of the flagged tiles and clear the flags.
-// Each wave reads the flags of 64 consecutive tiles (one per lane, so every
-// flag is read and reset by one lane), then clears the flagged ones: lanes
-// 0-31 channels 0 and 2, lanes 32-63 channels 1 and 3 of the tile's 32 nodes.
+// Zero the flagged tiles of a grid (clear_tiles) in a launch of its own.
 __global__ __launch_bounds__(kGat) void k_clear_tiles(Batch bt, float* grid, unsigned* flags, long long ntiles)
 {
-    const int lane = threadIdx.x & 63;
-    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
-    const int c0 = lane >> 5;
-    const long long Gtot = bt.Gtot;
-    for (long long t0 = wave * 64; t0 < ntiles; t0 += nwaves * 64) {
-        const long long t = t0 + lane;
-        const unsigned f = t < ntiles ? flags[t] : 0u;
-        unsigned long long mask = __ballot(f != 0u);
-        if (f != 0u) flags[t] = 0u;
-        while (mask) {
-            const int b = __ffsll((long long)mask) - 1;
-            mask &= mask - 1;
-            const long long q = ((t0 + b) << kTileShift) + (lane & 31);
-            if (q >= Gtot) continue;
-            long long n0 = 0, Gi = Gtot;
-            if (bt.B > 1) {
-                const int i = instance_of(bt, q);
-                n0 = bt.nofs[i];
-                Gi = bt.nofs[i + 1] - n0;
-            }
-            float* g = grid + 4 * n0 + (q - n0);
-            g[c0 * Gi] = 0.f;
-            g[(c0 + 2) * Gi] = 0.f;
-        }
-    }
+    clear_tiles(bt, grid, flags, ntiles, ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6,
+                ((long long)gridDim.x * blockDim.x) >> 6);
 }
 
 __global__ __launch_bounds__(kGat) void k_set_flags(unsigned* flags, long long ntiles)
@@ -1220,10 +1243,23 @@ void launch_p2g_mass_blk(const Batch& b, const float* lay, const float* cst, flo
                        gm);
 }
 
-void launch_g2p_blk(const Batch& b, const StepIO<float>& io, hipStream_t s)
+// workgroups appended for a tile clear: a wave per 64 tiles, 4 waves per block
+static unsigned clear_blocks(const Batch& b, const TileClear& tc)
 {
-    if (b.nranges <= 0) return;
-    hipLaunchKernelGGL(k_g2p_blk, dim3(b.nranges), dim3(kGat), 3 * win_nodes(b.cap) * sizeof(float), s, b, io);
+    if (!tc.grid) return 0;
+    long long nb = (num_tiles(b.Gtot) + 4 * 64 - 1) / (4 * 64);
+    if (nb > 4096) nb = 4096;
+    return (unsigned)nb;
+}
+
+void launch_g2p_blk(const Batch& b, const StepIO<float>& io, const TileClear& tc, hipStream_t s)
+{
+    if (b.nranges <= 0) {
+        if (tc.grid) launch_clear_tiles(b, tc.grid, tc.flags, s);
+        return;
+    }
+    hipLaunchKernelGGL(k_g2p_blk, dim3(b.nranges + clear_blocks(b, tc)), dim3(kGat), 3 * win_nodes(b.cap) * sizeof(float), s, b, io,
+                       tc);
 }
 
 void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
@@ -1232,14 +1268,18 @@ void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
     hipLaunchKernelGGL(k_g2p_adj_blk, dim3(b.nranges), dim3(kG2PAdj), 2 * win_nodes(b.cap) * sizeof(unsigned long long), s, b, io);
 }
 
-void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
+void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear& tc, hipStream_t s)
 {
-    if (b.nranges <= 0) return;
+    if (b.nranges <= 0) {
+        if (tc.grid) launch_clear_tiles(b, tc.grid, tc.flags, s);
+        return;
+    }
     if (kFuseElastic) {
         hipLaunchKernelGGL(k_p2g_adj_blk<true>, dim3(b.nranges), dim3(kGat), 7 * win_nodes(b.cap) * sizeof(float), s, b, io);
+        if (tc.grid) launch_clear_tiles(b, tc.grid, tc.flags, s);
     } else {
         hipLaunchKernelGGL(k_p2g_adj_blk<false>, dim3(b.nranges), dim3(kGat), 7 * win_nodes(b.cap) * sizeof(float), s, b, io);
-        hipLaunchKernelGGL(k_elastic_adj, dim3(b.nranges), dim3(kGat), 0, s, b, io);
+        hipLaunchKernelGGL(k_elastic_adj, dim3(b.nranges + clear_blocks(b, tc)), dim3(kGat), 0, s, b, io, tc);
     }
 }
 

@@ -115,14 +115,19 @@ struct Batch {
 // clear.  Nodes of unflagged tiles are 0 in every channel, so the clear and
 // the node-parallel kernels visit flagged tiles only.
 constexpr int kTileShift = 5;
-inline long long num_tiles(long long Gtot) { return (Gtot + (1LL << kTileShift) - 1) >> kTileShift; }
+__host__ __device__ inline long long num_tiles(long long Gtot) { return (Gtot + (1LL << kTileShift) - 1) >> kTileShift; }
 void launch_clear_tiles(const Batch& b, float* grid, unsigned* flags, hipStream_t s);
+// a tile clear carried by a kernel that does not touch the grid (null: none)
+struct TileClear {
+    float* grid;
+    unsigned* flags;
+};
 void launch_set_flags(unsigned* flags, long long ntiles, hipStream_t s);
 void launch_p2g_blk(const Batch& b, const StepIO<float>& io, hipStream_t s);
 void launch_p2g_mass_blk(const Batch& b, const float* lay, const float* cst, float* gm, hipStream_t s);
-void launch_g2p_blk(const Batch& b, const StepIO<float>& io, hipStream_t s);
+void launch_g2p_blk(const Batch& b, const StepIO<float>& io, const TileClear& tc, hipStream_t s);
 void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s);
-void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s);
+void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear& tc, hipStream_t s);
 // loss over all instances from node masses gm (instance i at nofs[i]):
 // per-instance losses in acc[0..B) (fixed-order sums of workgroup partials
 // kept in part, at most loss_partials(B) doubles); gmax = largest instance node count

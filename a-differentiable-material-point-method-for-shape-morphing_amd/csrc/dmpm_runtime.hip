@@ -65,9 +65,10 @@ struct dmpm_ctx {
     void* ctrl = nullptr;     // (L-1) x 9 x S
     void* layers = nullptr;   // layer slots x 24 x S
     void* grids = nullptr;    // grid slots x 4 x G
-    void* wgrid = nullptr;    // 4 x G: loss mass + dLdm / checkpoint-mode forward grid
+    void* wgrid = nullptr;    // 4 x G: loss mass + dLdm / checkpoint-mode forward grid (even steps)
+    void* wgrid2 = nullptr;   // 4 x G: checkpoint-mode forward grid (odd steps)
     void* agrid = nullptr;    // 4 x G: vbar/pbar + mbar
-    // FP32: tile flags of every grid buffer (grid slots, then wgrid, then agrid)
+    // FP32: tile flags of every grid buffer (grid slots, then wgrid, agrid, wgrid2)
     unsigned* tflags = nullptr;
     long long ntiles = 0;
     size_t grid_slots = 0;
@@ -301,6 +302,7 @@ unsigned* flags_of(dmpm_ctx* c, const void* g)
     size_t k;
     if (g == c->wgrid) k = c->grid_slots;
     else if (g == c->agrid) k = c->grid_slots + 1;
+    else if (g == c->wgrid2) k = c->grid_slots + 2;
     else k = (size_t)(static_cast<const char*>(g) - static_cast<const char*>(c->grids)) / gb;
     return c->tflags + k * (size_t)c->ntiles;
 }
@@ -327,34 +329,52 @@ inline void p2g_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<double>
 inline void p2g_dispatch(dmpm_ctx* c, const StepParams&, const StepIO<float>& io) { launch_p2g_blk(c->batch, io, c->stream); }
 inline void grid_update_dispatch(dmpm_ctx* c, const StepParams& sp, double* g) { launch_grid_update<double>(sp, g, c->stream); }
 inline void g2p_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<double>& io) { launch_g2p<double>(sp, io, c->stream); }
-inline void g2p_dispatch(dmpm_ctx* c, const StepParams&, const StepIO<float>& io) { launch_g2p_blk(c->batch, io, c->stream); }
+inline void g2p_dispatch(dmpm_ctx* c, const StepParams&, const StepIO<float>& io, const TileClear& tc)
+{
+    launch_g2p_blk(c->batch, io, tc, c->stream);
+}
 inline void g2p_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<double>& io) { launch_g2p_adj<double>(sp, io, c->stream); }
 inline void g2p_adj_dispatch(dmpm_ctx* c, const StepParams&, const AdjIO<float>& io) { launch_g2p_adj_blk(c->batch, io, c->stream); }
 inline void grid_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<double>& io) { launch_grid_adj<double>(sp, io, c->stream); }
 inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<double>& io) { launch_p2g_adj<double>(sp, io, c->stream); }
-inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams&, const AdjIO<float>& io)
+inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams&, const AdjIO<float>& io, const TileClear& tc)
 {
-    launch_p2g_adj_blk(c->batch, io, c->stream);
+    launch_p2g_adj_blk(c->batch, io, tc, c->stream);
 }
 
-// One ForwardTimeStep: cur -> nxt, grid of the step in g.
+// One ForwardTimeStep: cur -> nxt, grid of the step in g.  g_clean: the
+// previous step of the same launch sequence cleared g already.  gnext (FP32):
+// the grid of the next step of the sequence, another buffer, which G2P's
+// appended workgroups clear; returns whether they did.
 template <typename Real>
-void step_forward(dmpm_ctx* c, const Real* cur, Real* nxt, const Real* ctrl, Real* g)
+bool step_forward(dmpm_ctx* c, const Real* cur, Real* nxt, const Real* ctrl, Real* g, bool g_clean = false, Real* gnext = nullptr)
 {
     StepParams sp = step_params(c);
     // FP32: P2G caches its affine matrix in the input layer
     StepIO<Real> io{cur, nxt, ctrl, static_cast<const Real*>(c->cst), g, c->f32 ? const_cast<Real*>(cur) : nullptr,
                     flags_of(c, g)};
-    clear_grid(c, g);
+    if (!g_clean) clear_grid(c, g);
     timed(c, DMPM_K_P2G, [&] { p2g_dispatch(c, sp, io); });
-    // FP32: G2P applies the grid update as it loads the nodes
-    if constexpr (std::is_same<Real, double>::value) timed(c, DMPM_K_GRID_UPDATE, [&] { grid_update_dispatch(c, sp, g); });
-    timed(c, DMPM_K_G2P, [&] { g2p_dispatch(c, sp, io); });
+    if constexpr (std::is_same<Real, double>::value) {
+        timed(c, DMPM_K_GRID_UPDATE, [&] { grid_update_dispatch(c, sp, g); });
+        timed(c, DMPM_K_G2P, [&] { g2p_dispatch(c, sp, io); });
+        return false;
+    } else {
+        // FP32: G2P applies the grid update as it loads the nodes
+        TileClear tc{};
+        if (gnext && gnext != g) tc = TileClear{gnext, flags_of(c, gnext)};
+        timed(c, DMPM_K_G2P, [&] { g2p_dispatch(c, sp, io, tc); });
+        return tc.grid != nullptr;
+    }
 }
 
-// One Back_Timestep: gradients of layer i+1 -> layer i.
+// One Back_Timestep: gradients of layer i+1 -> layer i.  agrid_clean: the
+// previous backward step of the same launch sequence cleared the adjoint
+// grid; clear_next (FP32): the elastic adjoint's appended workgroups clear it
+// for the next one (returns whether they did).
 template <typename Real>
-void step_backward(dmpm_ctx* c, int i, const Real* cur, const Real* nxt, const Real* g)
+bool step_backward(dmpm_ctx* c, int i, const Real* cur, const Real* nxt, const Real* g, bool agrid_clean = false,
+                   bool clear_next = false)
 {
     StepParams sp = step_params(c);
     AdjIO<Real> io{};
@@ -370,13 +390,30 @@ void step_backward(dmpm_ctx* c, int i, const Real* cur, const Real* nxt, const R
     io.tmp = static_cast<Real*>(c->tmp);
     io.gflags = flags_of(c, g);
     io.aflags = flags_of(c, c->agrid);
-    // the vbar channels: FP64 the first 3G reals, FP32 the flagged tiles
-    if (c->f32) clear_grid(c, reinterpret_cast<float*>(io.agrid));
-    else fill<Real>(c, io.agrid, 3 * c->G, Real(0));
-    timed(c, DMPM_K_G2P_ADJ, [&] { g2p_adj_dispatch(c, sp, io); });
-    // FP32: the transposed P2G applies the transposed grid update as it loads the nodes
-    if constexpr (std::is_same<Real, double>::value) timed(c, DMPM_K_GRID_ADJ, [&] { grid_adj_dispatch(c, sp, io); });
-    timed(c, DMPM_K_P2G_ADJ, [&] { p2g_adj_dispatch(c, sp, io); });
+    if constexpr (std::is_same<Real, double>::value) {
+        fill<Real>(c, io.agrid, 3 * c->G, Real(0));  // the vbar channels
+        timed(c, DMPM_K_G2P_ADJ, [&] { g2p_adj_dispatch(c, sp, io); });
+        timed(c, DMPM_K_GRID_ADJ, [&] { grid_adj_dispatch(c, sp, io); });
+        timed(c, DMPM_K_P2G_ADJ, [&] { p2g_adj_dispatch(c, sp, io); });
+        return false;
+    } else {
+        // FP32: the flagged tiles; the transposed P2G applies the transposed
+        // grid update as it loads the nodes
+        if (!agrid_clean) clear_grid(c, io.agrid);
+        timed(c, DMPM_K_G2P_ADJ, [&] { g2p_adj_dispatch(c, sp, io); });
+        TileClear tc{};
+        if (clear_next) tc = TileClear{io.agrid, io.aflags};
+        timed(c, DMPM_K_P2G_ADJ, [&] { p2g_adj_dispatch(c, sp, io, tc); });
+        return clear_next;
+    }
+}
+
+// checkpoint-mode forward grid of a step outside the tape: two buffers by
+// parity, so that a step's G2P can clear the next step's grid
+template <typename Real>
+Real* wgrid_of(dmpm_ctx* c, int i)
+{
+    return static_cast<Real*>((i & 1) ? c->wgrid2 : c->wgrid);
 }
 
 // Bring layer j (<= fwd_valid) into a readable slot and return it.  Checkpoint
@@ -390,9 +427,11 @@ Real* materialize(dmpm_ctx* c, int j)
     if (c->tape_seg == j / c->K) return tape_layer<Real>(c, j);
     const int j0 = (j / c->K) * c->K;
     Real* prev = slot<Real>(c, j0 / c->K);
+    bool clean = false;
     for (int i = j0; i < j; ++i) {
         Real* dst = slot<Real>(c, c->roll_base + ((i + 1) & 1));
-        step_forward<Real>(c, prev, dst, ctrl_of<Real>(c, i), static_cast<Real*>(c->wgrid));
+        clean = step_forward<Real>(c, prev, dst, ctrl_of<Real>(c, i), wgrid_of<Real>(c, i), clean,
+                                   i + 1 < j ? wgrid_of<Real>(c, i + 1) : nullptr);
         prev = dst;
     }
     return prev;
@@ -404,10 +443,12 @@ void recompute_segment(dmpm_ctx* c, int seg)
 {
     const int s0 = seg * c->K, s1 = std::min(s0 + c->K, c->L - 1);
     Real* grids = static_cast<Real*>(c->grids);
+    bool clean = false;
     for (int i = s0; i < s1; ++i) {
         const Real* cur = (i == s0) ? slot<Real>(c, s0 / c->K) : slot<Real>(c, c->tape_base + (i - s0));
         Real* dst = ((i + 1) % c->K == 0) ? slot<Real>(c, c->roll_base) : slot<Real>(c, c->tape_base + (i + 1 - s0));
-        step_forward<Real>(c, cur, dst, ctrl_of<Real>(c, i), grids + (size_t)(i - s0) * kGridFields * c->G);
+        clean = step_forward<Real>(c, cur, dst, ctrl_of<Real>(c, i), grids + (size_t)(i - s0) * kGridFields * c->G, clean,
+                                   i + 1 < s1 ? grids + (size_t)(i + 1 - s0) * kGridFields * c->G : nullptr);
     }
     c->tape_seg = seg;
 }
@@ -422,8 +463,10 @@ int forward_pass(dmpm_ctx* c, int start)
     int rc;
     if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) {
         rc = run_graph(c, from, [&] {
+            bool clean = false;
             for (int i = from; i < c->L - 1; ++i)
-                step_forward<Real>(c, slot<Real>(c, i), slot<Real>(c, i + 1), ctrl_of<Real>(c, i), grid_of_step<Real>(c, i));
+                clean = step_forward<Real>(c, slot<Real>(c, i), slot<Real>(c, i + 1), ctrl_of<Real>(c, i), grid_of_step<Real>(c, i), clean,
+                                           i + 1 < c->L - 1 ? grid_of_step<Real>(c, i + 1) : nullptr);
         });
         if (rc) return rc;
     } else {
@@ -431,6 +474,7 @@ int forward_pass(dmpm_ctx* c, int start)
         const int s0_last = c->seg_last * c->K;
         Real* grids = static_cast<Real*>(c->grids);
         rc = run_graph(c, from, [&] {
+        bool clean = false;
         for (int i = from; i < c->L - 1; ++i) {
             const Real* cur;
             if (i % c->K == 0) cur = slot<Real>(c, i / c->K);
@@ -441,8 +485,10 @@ int forward_pass(dmpm_ctx* c, int start)
             if (j % c->K == 0) dst = slot<Real>(c, j / c->K);
             else if (j / c->K == c->seg_last) dst = slot<Real>(c, c->tape_base + (j - s0_last));
             else dst = slot<Real>(c, c->roll_base + (j & 1));
-            Real* g = (i >= s0_last) ? grids + (size_t)(i - s0_last) * kGridFields * c->G : static_cast<Real*>(c->wgrid);
-            step_forward<Real>(c, cur, dst, ctrl_of<Real>(c, i), g);
+            auto grid_at = [&](int k) {
+                return (k >= s0_last) ? grids + (size_t)(k - s0_last) * kGridFields * c->G : wgrid_of<Real>(c, k);
+            };
+            clean = step_forward<Real>(c, cur, dst, ctrl_of<Real>(c, i), grid_at(i), clean, i + 1 < c->L - 1 ? grid_at(i + 1) : nullptr);
         }
         });
         if (rc) return rc;
@@ -526,20 +572,23 @@ int backward_pass(dmpm_ctx* c, int control)
     int rc;
     if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) {
         rc = run_graph(c, key, [&] {
+            bool clean = false;
             for (int i = c->L - 2; i >= control; --i)
-                step_backward<Real>(c, i, slot<Real>(c, i), slot<Real>(c, i + 1), grid_of_step<Real>(c, i));
+                clean = step_backward<Real>(c, i, slot<Real>(c, i), slot<Real>(c, i + 1), grid_of_step<Real>(c, i), clean, i > control);
         });
         if (rc) return rc;
     } else {
         Real* grids = static_cast<Real*>(c->grids);
         rc = run_graph(c, key, [&] {
+            // the adjoint grid chain runs across the recomputed segments (they do not touch it)
+            bool clean = false;
             for (int seg = c->seg_last; seg >= control / c->K; --seg) {
                 const int s0 = seg * c->K, s1 = std::min(s0 + c->K, c->L - 1);
                 if (c->tape_seg != seg) recompute_segment<Real>(c, seg);
                 for (int i = s1 - 1; i >= std::max(s0, control); --i) {
                     const Real* cur = tape_layer<Real>(c, i);
                     const Real* nxt = tape_layer<Real>(c, i + 1);
-                    step_backward<Real>(c, i, cur, nxt, grids + (size_t)(i - s0) * kGridFields * c->G);
+                    clean = step_backward<Real>(c, i, cur, nxt, grids + (size_t)(i - s0) * kGridFields * c->G, clean, i > control);
                 }
             }
         });
@@ -1029,6 +1078,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
     if ((rc = dalloc(c, &c->cst, kConstFields * S * e)) || (rc = dalloc(c, &c->ctrl, (size_t)steps * 9 * S * e)) ||
         (rc = dalloc(c, &c->layers, (size_t)c->n_slots * c->lf * S * e)) ||
         (rc = dalloc(c, &c->grids, grid_slots * kGridFields * G * e)) || (rc = dalloc(c, &c->wgrid, kGridFields * G * e)) ||
+        (rc = dalloc(c, &c->wgrid2, kGridFields * G * e)) ||
         (rc = dalloc(c, &c->agrid, kGridFields * G * e)) || (rc = dalloc(c, &c->gbuf[0], kLayerFields * S * e)) ||
         (rc = dalloc(c, &c->gbuf[1], kLayerFields * S * e)) || (rc = dalloc(c, &c->gctrl, gctrl_layers * 9 * S * e)) ||
         (rc = dalloc(c, &c->tmp, 12 * S * e)) || (rc = dalloc(c, &c->target, G * e)) ||
@@ -1036,7 +1086,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         (rc = dalloc(c, (void**)&c->dpart, std::max<long long>(kLossBlocks, loss_partials(c->B)) * sizeof(double))) ||
         (rc = dalloc(c, (void**)&c->d_sps, c->B * sizeof(StepParams))) ||
         (rc = dalloc(c, (void**)&c->d_nofs, (c->B + 1) * sizeof(long long))) ||
-        (c->f32 && (rc = dalloc(c, (void**)&c->tflags, (grid_slots + 2) * (size_t)c->ntiles * sizeof(unsigned))))) {
+        (c->f32 && (rc = dalloc(c, (void**)&c->tflags, (grid_slots + 3) * (size_t)c->ntiles * sizeof(unsigned))))) {
         std::string msg = g_err;
         dmpm_destroy(c);
         return fail(rc, msg);
@@ -1050,8 +1100,9 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         // grids start at 0 with no tile flagged
         launch_fill<float>(static_cast<float*>(c->grids), (long long)(grid_slots * kGridFields * G), 0.f, c->stream);
         launch_fill<float>(static_cast<float*>(c->wgrid), (long long)(kGridFields * G), 0.f, c->stream);
+        launch_fill<float>(static_cast<float*>(c->wgrid2), (long long)(kGridFields * G), 0.f, c->stream);
         launch_fill<float>(static_cast<float*>(c->agrid), (long long)(kGridFields * G), 0.f, c->stream);
-        launch_fill<float>(reinterpret_cast<float*>(c->tflags), (long long)(grid_slots + 2) * c->ntiles, 0.f, c->stream);
+        launch_fill<float>(reinterpret_cast<float*>(c->tflags), (long long)(grid_slots + 3) * c->ntiles, 0.f, c->stream);
     } else {
         launch_fill<double>(static_cast<double*>(c->gctrl), (long long)gctrl_layers * 9 * S, 0.0, c->stream);
         launch_fill<double>(static_cast<double*>(c->ctrl), (long long)steps * 9 * S, 0.0, c->stream);
@@ -1116,7 +1167,7 @@ int dmpm_destroy(dmpm_ctx* c)
     hipSetDevice(c->cfg.device);
     if (c->stream) hipStreamSynchronize(c->stream);
     clear_graphs(c);
-    void* bufs[] = {c->cst,  c->ctrl, c->layers, c->grids, c->wgrid, c->agrid, c->gbuf[0], c->gbuf[1],
+    void* bufs[] = {c->cst,  c->ctrl, c->layers, c->grids, c->wgrid, c->wgrid2, c->agrid, c->gbuf[0], c->gbuf[1],
                     c->gctrl, c->tmp, c->target, c->dacc,  c->ranges, c->d_sps, c->d_nofs, c->dpart, c->tflags};
     for (void* b : bufs)
         if (b) hipFree(b);
