@@ -96,20 +96,20 @@ __device__ __forceinline__ int base_of(const StepParams& sp, float x, int a)
     return int(floorf((x - float(sp.min_point[a])) * float(sp.inv_dx))) - 1;
 }
 
-// Bounding box of the stencil bases of particles [r0, r1).
-__device__ Win range_window(const StepParams& sp, const float* lay, int S, int r0, int r1, int cap)
+// extend a thread's base box v (min x3, max x3) by the stencil base of position x
+__device__ __forceinline__ void box_add(const StepParams& sp, const float x[3], int v[6])
 {
-    __shared__ int red[6][kMaxWaves];
-    __shared__ Win win;
-    int v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
-    for (int p = r0 + (int)threadIdx.x; p < r1; p += blockDim.x) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const int b = base_of(sp, lay[(long long)(FX + a) * S + p], a);
-            v[a] = min(v[a], b);
-            v[3 + a] = max(v[3 + a], b);
-        }
+    for (int a = 0; a < 3; ++a) {
+        const int b = base_of(sp, x[a], a);
+        v[a] = min(v[a], b);
+        v[3 + a] = max(v[3 + a], b);
     }
+}
+
+// wave-wide min (first 3) / max (last 3) of the base boxes, in every lane
+__device__ __forceinline__ void wave_box(int v[6])
+{
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
@@ -118,39 +118,64 @@ __device__ Win range_window(const StepParams& sp, const float* lay, int S, int r
             v[3 + a] = max(v[3 + a], __shfl_xor(v[3 + a], off, 64));
         }
     }
+}
+
+// the window from the waves' boxes red[c][wave] (one thread)
+__device__ Win box_window(const StepParams& sp, const int (*red)[kMaxWaves], int nw, int cap)
+{
+    Win W;
+    long long vol = 1;
+    bool empty = false;
+    W.fixed = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int lo = INT_MAX, hi = INT_MIN;
+        for (int w = 0; w < nw; ++w) {
+            lo = min(lo, red[a][w]);
+            hi = max(hi, red[3 + a][w]);
+        }
+        W.flo[a] = lo;
+        if ((long long)hi - (long long)lo > kFW - 4) W.fixed = false;
+        long long l = lo < 0 ? 0 : lo;
+        long long h = (long long)hi + 3;
+        if (h > sp.dim[a] - 1) h = sp.dim[a] - 1;
+        if (h < l) empty = true;
+        W.lo[a] = (int)(l > INT_MAX ? INT_MAX : l);
+        W.w[a] = empty ? 0 : (int)(h - l + 1);
+        vol *= empty ? 0 : (h - l + 1);
+    }
+    W.vol = empty ? 0 : (int)(vol > INT_MAX ? INT_MAX : vol);
+    W.lds = !empty && vol <= cap;
+    return W;
+}
+
+// Window of a workgroup from every thread's base box (all threads).
+__device__ Win window_of(const StepParams& sp, int v[6], int cap)
+{
+    __shared__ int red[6][kMaxWaves];
+    __shared__ Win win;
+    wave_box(v);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0)
 #pragma unroll
         for (int c = 0; c < 6; ++c) red[c][wid] = v[c];
     __syncthreads();
-    if (threadIdx.x == 0) {
-        Win W;
-        long long vol = 1;
-        bool empty = false;
-        W.fixed = true;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            int lo = INT_MAX, hi = INT_MIN;
-            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-                lo = min(lo, red[a][w]);
-                hi = max(hi, red[3 + a][w]);
-            }
-            W.flo[a] = lo;
-            if ((long long)hi - (long long)lo > kFW - 4) W.fixed = false;
-            long long l = lo < 0 ? 0 : lo;
-            long long h = (long long)hi + 3;
-            if (h > sp.dim[a] - 1) h = sp.dim[a] - 1;
-            if (h < l) empty = true;
-            W.lo[a] = (int)(l > INT_MAX ? INT_MAX : l);
-            W.w[a] = empty ? 0 : (int)(h - l + 1);
-            vol *= empty ? 0 : (h - l + 1);
-        }
-        W.vol = empty ? 0 : (int)(vol > INT_MAX ? INT_MAX : vol);
-        W.lds = !empty && vol <= cap;
-        win = W;
-    }
+    if (threadIdx.x == 0) win = box_window(sp, red, (int)(blockDim.x >> 6), cap);
     __syncthreads();
     return win;
+}
+
+// Bounding box of the stencil bases of particles [r0, r1).
+__device__ Win range_window(const StepParams& sp, const float* lay, int S, int r0, int r1, int cap)
+{
+    int v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    for (int p = r0 + (int)threadIdx.x; p < r1; p += blockDim.x) {
+        float x[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) x[a] = lay[(long long)(FX + a) * S + p];
+        box_add(sp, x, v);
+    }
+    return window_of(sp, v, cap);
 }
 
 __device__ __forceinline__ long long win_node(const StepParams& sp, const Win& W, int idx)
@@ -224,9 +249,55 @@ struct Scales {
     bool ok;
 };
 
-// block-wide max of NC per-thread contribution bounds -> power-of-two scales
-// with s_c * max <= 2^E (all threads): E = 30 for one channel per word, else
-// the largest exponent with 2^E nP (2/3)^3 + nP/2 < 2^31 (capped at 30)
+// wave-wide max of NC bounds, in every lane (a NaN wins)
+template <int NC>
+__device__ __forceinline__ void wave_max(float v[NC])
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const float o = __shfl_xor(v[c], off, 64);
+            v[c] = (v[c] == v[c] && !(o > v[c])) ? v[c] : o;  // max that keeps NaN
+        }
+}
+
+// power-of-two scales from the waves' bounds red[c][wave] (one thread), with
+// s_c * max <= 2^E: E = 30 for one channel per word, else the largest
+// exponent with 2^E nP (2/3)^3 + nP/2 < 2^31 (capped at 30)
+template <int NC, bool PACK>
+__device__ Scales<NC> bound_scales(const float (*red)[kMaxWaves], int nw, int nP)
+{
+    Scales<NC> S;
+    S.ok = true;
+    int E = 30;
+    if (PACK) {
+        // (2/3)^3 rounded up, so that float weights cannot exceed it
+        E = (int)floor(log2((2147483647.0 - 0.5 * nP) / (0.29630 * (nP > 1 ? nP : 1))));
+        if (E > 30) E = 30;
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        float t = 0.f;
+        for (int w = 0; w < nw; ++w) {
+            const float r = red[c][w];
+            if (!(r < 3.0e38f)) S.ok = false;  // inf or NaN somewhere in the block
+            else t = fmaxf(t, r);
+        }
+        int e = E;
+        if (t > 0.f) {
+            int ex;
+            frexpf(t, &ex);  // t < 2^ex
+            e = E - ex;
+        }
+        e = e < -120 ? -120 : (e > 120 ? 120 : e);
+        S.s[c] = ldexpf(1.f, e);
+        S.inv[c] = ldexpf(1.f, -e);
+    }
+    return S;
+}
+
+// block-wide max of NC per-thread contribution bounds -> scales (all threads)
 template <int NC, bool PACK>
 __device__ Scales<NC> block_scales(const float bound[NC], int nP)
 {
@@ -235,49 +306,46 @@ __device__ Scales<NC> block_scales(const float bound[NC], int nP)
     float v[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) v[c] = bound[c];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const float o = __shfl_xor(v[c], off, 64);
-            v[c] = (v[c] == v[c] && !(o > v[c])) ? v[c] : o;  // max that keeps NaN
-        }
+    wave_max<NC>(v);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0)
 #pragma unroll
         for (int c = 0; c < NC; ++c) red[c][wid] = v[c];
     __syncthreads();
-    if (threadIdx.x == 0) {
-        Scales<NC> S;
-        S.ok = true;
-        int E = 30;
-        if (PACK) {
-            // (2/3)^3 rounded up, so that float weights cannot exceed it
-            E = (int)floor(log2((2147483647.0 - 0.5 * nP) / (0.29630 * (nP > 1 ? nP : 1))));
-            if (E > 30) E = 30;
-        }
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            float t = 0.f;
-            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-                const float r = red[c][w];
-                if (!(r < 3.0e38f)) S.ok = false;  // inf or NaN somewhere in the block
-                else t = fmaxf(t, r);
-            }
-            int e = E;
-            if (t > 0.f) {
-                int ex;
-                frexpf(t, &ex);  // t < 2^ex
-                e = E - ex;
-            }
-            e = e < -120 ? -120 : (e > 120 ? 120 : e);
-            S.s[c] = ldexpf(1.f, e);
-            S.inv[c] = ldexpf(1.f, -e);
-        }
-        sc = S;
-    }
+    if (threadIdx.x == 0) sc = bound_scales<NC, PACK>(red, (int)(blockDim.x >> 6), nP);
     __syncthreads();
     return sc;
+}
+
+// The window and the scales of a scatter kernel in one round of barriers.
+template <int NC, bool PACK>
+__device__ void block_setup(const StepParams& sp, int box[6], const float bound[NC], int nP, int cap, Win& W, Scales<NC>& S)
+{
+    __shared__ int redi[6][kMaxWaves];
+    __shared__ float redf[NC][kMaxWaves];
+    __shared__ Win win;
+    __shared__ Scales<NC> sc;
+    float v[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = bound[c];
+    wave_box(box);
+    wave_max<NC>(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) redi[c][wid] = box[c];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) redf[c][wid] = v[c];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (int)(blockDim.x >> 6);
+        win = box_window(sp, redi, nw, cap);
+        sc = bound_scales<NC, PACK>(redf, nw, nP);
+    }
+    __syncthreads();
+    W = win;
+    S = sc;
 }
 
 // |y| < 2^30: rounded to an int32, sign-extended to the int64 accumulator
@@ -605,13 +673,21 @@ __global__ __launch_bounds__(kP2G, 4) void k_p2g_blk(Batch bt, StepIO<float> io)
     io.grid += 4 * nofs;
     const int p0 = rg.x + (int)threadIdx.x, p1 = p0 + kP2G;
     const bool v0 = p0 < rg.y, v1 = p1 < rg.y;
-    const Win W = range_window(sp, io.cur, sp.stride, rg.x, rg.y, bt.cap);
     const float dx = float(sp.dx);
     P2GPart q0{}, q1{};
     float bound[4] = {0.f, 0.f, 0.f, 0.f};
-    if (v0) p2g_prepare(sp, io, p0, q0, bound);
-    if (v1) p2g_prepare(sp, io, p1, q1, bound);
-    const Scales<4> SC = block_scales<4, false>(bound, rg.y - rg.x);
+    int box[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    if (v0) {
+        p2g_prepare(sp, io, p0, q0, bound);
+        box_add(sp, q0.x, box);
+    }
+    if (v1) {
+        p2g_prepare(sp, io, p1, q1, bound);
+        box_add(sp, q1.x, box);
+    }
+    Win W;
+    Scales<4> SC;
+    block_setup<4, false>(sp, box, bound, rg.y - rg.x, bt.cap, W, SC);
     const int mode = (W.fixed && SC.ok) ? 0 : ((W.lds && SC.ok) ? 1 : 2);
     if (mode == 0)
         for (int i = threadIdx.x; i < 4 * kFWS; i += blockDim.x) lds64[i] = 0ull;
@@ -870,13 +946,21 @@ __global__ __launch_bounds__(kG2PAdj, 4) void k_g2p_adj_blk(Batch bt, AdjIO<floa
     io.agrid += 4 * nofs;
     const int p0 = rg.x + (int)threadIdx.x, p1 = p0 + kG2PAdj;
     const bool v0 = p0 < rg.y, v1 = p1 < rg.y;
-    const Win W = range_window(sp, io.cur, sp.stride, rg.x, rg.y, bt.cap);
     const float Dd = float(3.0 * sp.inv_dx);
     G2PAdjPart q0{}, q1{};
     float bound[3] = {0.f, 0.f, 0.f};
-    if (v0) g2p_adj_prepare(sp, io, p0, q0, bound);
-    if (v1) g2p_adj_prepare(sp, io, p1, q1, bound);
-    const Scales<3> SC = block_scales<3, true>(bound, rg.y - rg.x);
+    int box[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    if (v0) {
+        g2p_adj_prepare(sp, io, p0, q0, bound);
+        box_add(sp, q0.x, box);
+    }
+    if (v1) {
+        g2p_adj_prepare(sp, io, p1, q1, bound);
+        box_add(sp, q1.x, box);
+    }
+    Win W;
+    Scales<3> SC;
+    block_setup<3, true>(sp, box, bound, rg.y - rg.x, bt.cap, W, SC);
     const int mode = (W.fixed && SC.ok) ? 0 : ((W.lds && SC.ok) ? 1 : 2);
     if (mode == 0)
         for (int i = threadIdx.x; i < 2 * kFWS; i += blockDim.x) lds64[i] = 0ull;
