@@ -219,14 +219,14 @@ __device__ __forceinline__ void win_load(const StepParams& sp, const Win& W, flo
 #pragma unroll
                 for (int c = 0; c < NC; ++c) v[c] = 0.f;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) lds[c * kFWS + fixed_slot_of(idx)] = v[c];
+            for (int c = 0; c < NC; ++c) lds[fixed_slot_of(idx) * NC + c] = v[c];
         }
     } else if (W.lds) {
         for (int idx = threadIdx.x; idx < W.vol; idx += blockDim.x) {
             float v[NC];
             ld(win_node(sp, W, idx), v);
 #pragma unroll
-            for (int c = 0; c < NC; ++c) lds[c * W.vol + idx] = v[c];
+            for (int c = 0; c < NC; ++c) lds[idx * NC + c] = v[c];
         }
     }
 }
@@ -518,11 +518,11 @@ __device__ __forceinline__ void gather64(const StepParams& sp, const Win& W, con
                 if (MODE == 0) {
                     const int o = slot + i * kSI + j * kSJ + kk;
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) ch[c] = lds[c * kFWS + o];
+                    for (int c = 0; c < NC; ++c) ch[c] = lds[o * NC + c];
                 } else if (MODE == 1) {
                     const int o = win_local(W, gi, gj, gk);
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) ch[c] = lds[c * W.vol + o];
+                    for (int c = 0; c < NC; ++c) ch[c] = lds[o * NC + c];
                 } else {
                     ld(((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk, ch);
                 }
@@ -1159,12 +1159,14 @@ __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
         float xg[3] = {0.f, 0.f, 0.f}, pacc[3] = {0.f, 0.f, 0.f}, vacc[3] = {0.f, 0.f, 0.f};
         // per node (a row-wise contraction as in G2P needs 212 VGPRs here,
         // occupancy 2, and measured 213 us against 181 us)
+        // xg_c = sum_k u_c[k] T_c[k] with T_c[k] = sum_ij g_c(i, j) s(i, j, k): the
+        // gradient weight's k-factor u (w_k or w'_k) leaves the node loop
+        float T[3][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
         auto acc = [&](int i, int j, int kk, const float* ch) {
             const float d0 = d0s[i], d1 = d1s[j], d2 = d2s[kk];
             const float w01 = st[0].w[i] * st[1].w[j];
             const float w = w01 * st[2].w[kk];
-            const float gw[3] = {(mgw * st[0].dw[i] * st[1].w[j]) * st[2].w[kk], (mgw * st[0].w[i] * st[1].dw[j]) * st[2].w[kk],
-                                 (mgw * w01) * st[2].dw[kk]};
+            const float g[3] = {mgw * st[0].dw[i] * st[1].w[j], mgw * st[0].w[i] * st[1].dw[j], mgw * w01};
             float sgw = m * ch[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -1180,12 +1182,18 @@ __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
                 vacc[a] += w * ch[4 + a];
             }
 #pragma unroll
-            for (int a = 0; a < 3; ++a) xg[a] += gw[a] * sgw;
+            for (int a = 0; a < 3; ++a) T[a][kk] += g[a] * sgw;
         };
         auto row = [](int, int) {};
         if (W.fixed) gather64<7, 0>(sp, W, st, lds, ld, acc, row);
         else if (W.lds) gather64<7, 1>(sp, W, st, lds, ld, acc, row);
         else gather64<7, 2>(sp, W, st, lds, ld, acc, row);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            xg[0] += st[2].w[k] * T[0][k];
+            xg[1] += st[2].w[k] * T[1][k];
+            xg[2] += st[2].dw[k] * T[2][k];
+        }
         float xb[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
