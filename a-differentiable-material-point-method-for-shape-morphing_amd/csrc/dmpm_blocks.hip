@@ -348,17 +348,29 @@ __device__ void block_setup(const StepParams& sp, int box[6], const float bound[
     S = sc;
 }
 
+// |y| < 2^30 rounded to an int32: floor(y + 1/2), one v_cvt_rpi_i32_f32
+// (rintf and a conversion take two; the ties of a half go up instead of to even)
+__device__ __forceinline__ int round_i32(float y)
+{
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(y));
+    return r;
+}
+
 // |y| < 2^30: rounded to an int32, sign-extended to the int64 accumulator
+// (NONNEG: y >= 0, so the high word is 0)
+template <bool NONNEG = false>
 __device__ __forceinline__ unsigned long long fix32(float y)
 {
-    return (unsigned long long)(long long)(int)rintf(y);
+    if (NONNEG) return (unsigned long long)(unsigned)round_i32(y);
+    return (unsigned long long)(long long)round_i32(y);
 }
 
 // channels lo, hi (scaled units) rounded to int32 and packed into one word:
 // (hi + borrow) * 2^32 + lo as a 64-bit integer is hi * 2^32 + lo
 __device__ __forceinline__ unsigned long long pack2(float lo, float hi)
 {
-    const int l = (int)rintf(lo), h = (int)rintf(hi);
+    const int l = round_i32(lo), h = round_i32(hi);
     return ((unsigned long long)(unsigned)(h + (l >> 31)) << 32) | (unsigned)l;
 }
 
@@ -447,11 +459,12 @@ __device__ __forceinline__ void stm(T* base, int field, int S, int p, const M3<T
 
 // Scatter of NC channels over a particle's 64 nodes.  val(i, j, k, out)
 // fills the NC contributions (already in scaled units for the LDS modes).
-// PACK: two channels per LDS word.
+// PACK: two channels per LDS word; otherwise channel 0 may be declared
+// non-negative (NONNEG0: the mass).
 // MODE 0: fixed window, 1: general LDS window, 2: HBM float atomics (nodes
 // where gate is 0 are skipped there, as the LDS modes' flush skips them; the
 // tile flags of the nodes written are set when flags is given).
-template <int NC, int MODE, bool PACK, typename ValFn>
+template <int NC, int MODE, bool PACK, bool NONNEG0 = false, typename ValFn>
 __device__ __forceinline__ void scatter64(const StepParams& sp, const Win& W, const Axis<float> st[3],
                                           unsigned long long* lds, float* hbm, ValFn&& val, const float* gate = nullptr,
                                           unsigned* flags = nullptr, long long nofs = 0)
@@ -479,7 +492,7 @@ __device__ __forceinline__ void scatter64(const StepParams& sp, const Win& W, co
                         for (int c = 0; c < NC; c += 2) atomicAdd(&lds[(c / 2) * stride + o], pack2(v[c], c + 1 < NC ? v[c + 1] : 0.f));
                     } else {
 #pragma unroll
-                        for (int c = 0; c < NC; ++c) atomicAdd(&lds[c * stride + o], fix32(v[c]));
+                        for (int c = 0; c < NC; ++c) atomicAdd(&lds[c * stride + o], c == 0 && NONNEG0 ? fix32<true>(v[c]) : fix32(v[c]));
                     }
                 } else {
                     const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
@@ -721,7 +734,7 @@ __global__ __launch_bounds__(kP2G, 4) void k_p2g_blk(Batch bt, StepIO<float> io)
                     out[1 + a] = wa * (q0.mv[a] + q0.A.a[3 * a] * a0[i] + q0.A.a[3 * a + 1] * a1[j] + q0.A.a[3 * a + 2] * a2[kk]) +
                                  wb * (q1.mv[a] + q1.A.a[3 * a] * b0[i] + q1.A.a[3 * a + 1] * b1[j] + q1.A.a[3 * a + 2] * b2[kk]);
             };
-            scatter64<4, 0, false>(sp, W, s0, lds64, io.grid, val);
+            scatter64<4, 0, false, true>(sp, W, s0, lds64, io.grid, val);
         }
     } else {
 #pragma unroll 1
@@ -742,8 +755,8 @@ __global__ __launch_bounds__(kP2G, 4) void k_p2g_blk(Batch bt, StepIO<float> io)
                 for (int a = 0; a < 3; ++a)
                     out[1 + a] = w * (q.mv[a] + q.A.a[3 * a] * d0s[i] + q.A.a[3 * a + 1] * d1s[j] + q.A.a[3 * a + 2] * d2s[kk]);
             };
-            if (mode == 0) scatter64<4, 0, false>(sp, W, st, lds64, io.grid, val);
-            else if (mode == 1) scatter64<4, 1, false>(sp, W, st, lds64, io.grid, val);
+            if (mode == 0) scatter64<4, 0, false, true>(sp, W, st, lds64, io.grid, val);
+            else if (mode == 1) scatter64<4, 1, false, true>(sp, W, st, lds64, io.grid, val);
             else scatter64<4, 2, false>(sp, W, st, lds64, io.grid, val, nullptr, io.gflags, nofs);
         }
     }
@@ -785,8 +798,8 @@ __global__ __launch_bounds__(kScat) void k_p2g_mass_blk(Batch bt, const float* l
         stencil_f(sp, x, st, false);
         if (mode != 2) m *= SC.s[0];
         auto val = [&](int i, int j, int kk, float* out) { out[0] = st[0].w[i] * st[1].w[j] * st[2].w[kk] * m; };
-        if (mode == 0) scatter64<1, 0, false>(sp, W, st, lds64, gm, val);
-        else if (mode == 1) scatter64<1, 1, false>(sp, W, st, lds64, gm, val);
+        if (mode == 0) scatter64<1, 0, false, true>(sp, W, st, lds64, gm, val);
+        else if (mode == 1) scatter64<1, 1, false, true>(sp, W, st, lds64, gm, val);
         else scatter64<1, 2, false>(sp, W, st, lds64, gm, val);
     }
     if (mode != 2) {
