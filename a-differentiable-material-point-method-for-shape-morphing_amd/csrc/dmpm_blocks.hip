@@ -53,7 +53,7 @@ namespace dmpm {
 namespace {
 
 constexpr int kGat = 256;    // gather kernels
-constexpr int kScat = 512;   // scatter kernels: one particle per thread of a block range
+constexpr int kScat = 512;   // the loss mass scatter: one particle per thread of a block range
 constexpr int kMaxWaves = 8;
 // The elastic adjoint runs as a kernel of its own after the transposed P2G
 // gather (Gb through tmp): the gather then needs 167 VGPRs instead of 253 and
@@ -819,9 +819,20 @@ __global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io, Ti
     const int4 rg = bt.ranges[blockIdx.x];
     const StepParams sp = bt.sps[rg.z];
     io.grid += 4 * bt.nofs[rg.z];
-    const int cap = bt.cap;
     const int S = sp.stride;
-    const Win W = range_window(sp, io.cur, S, rg.x, rg.y, cap);
+    // the thread's particles (at most two: ranges hold <= 512): positions
+    // once, for the window and the gathers
+    float xs[2][3];
+    int box[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int p = rg.x + (int)threadIdx.x + h * kGat;
+        if (p < rg.y) {
+            ld3(io.cur, FX, S, p, xs[h]);
+            box_add(sp, xs[h], box);
+        }
+    }
+    const Win W = window_of(sp, box, bt.cap);
     const float dt = float(sp.dt), s = float(sp.smoothing);
     const float fdt[3] = {dt * float(sp.f_ext[0]), dt * float(sp.f_ext[1]), dt * float(sp.f_ext[2])};
     const float* gr = io.grid;
@@ -833,9 +844,13 @@ __global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io, Ti
     win_load<3>(sp, W, lds, ld);
     __syncthreads();
     const float Dd = float(3.0 * sp.inv_dx);
-    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) {
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+        const int p = rg.x + (int)threadIdx.x + h * kGat;
+        if (p >= rg.y) break;
         float x[3];
-        ld3(io.cur, FX, S, p, x);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) x[a] = h ? xs[1][a] : xs[0][a];
         Axis<float> st[3];
         stencil_f(sp, x, st, false);
         float e0s[4], e1s[4], e2s[4];

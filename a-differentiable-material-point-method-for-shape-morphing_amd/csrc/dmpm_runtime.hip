@@ -685,8 +685,8 @@ void sort_particles(dmpm_ctx* c, const double* x)
 }
 
 // Workgroup ranges: every block split into equal parts of at most range_max
-// particles (the scatter kernels hold one particle per thread of a
-// 512-thread workgroup, so range_max <= 512).
+// particles (the particle kernels hold two particles per thread of a
+// 256-thread workgroup, so range_max <= 512).
 int build_ranges(dmpm_ctx* c)
 {
     std::vector<int4> rg;
