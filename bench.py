@@ -39,19 +39,27 @@ from __graft_entry__ import _load_pkg  # noqa: E402
 OPT_RANGE_MAX, OPT_LDS_CAP = 2, 3
 
 # Algorithmic HBM bytes per FP32 particle / per touched node / per dense grid node
-# of each kernel (FP64: twice).  Particle fields: x3 v3 C9 F9 dFc9 + m vol lam mu.
+# of each kernel (FP64: twice), the per-particle-step figures of the scope
+# table.  Forward: P2G reads x, v, C, F, dFc, m, vol, lam, mu (148 B) and
+# writes m, p (16 B per touched node); G2P + update reads x, F, dFc (84 B),
+# writes x, v, C, F (96 B) and reads v (12 B per touched node).  The adjoint
+# step (376 B + 72 B per touched node) split by what each kernel needs: the
+# transposed G2P reads x, F, dFc, x', v', F', C' (180 B) and scatters vbar
+# (12 B per node); the transposed P2G with its elastic half reads v, C, m, vol,
+# lam, mu, C' and writes the x, v, C, F gradients (196 B), per node m, v read,
+# pbar/mbar write and read, reset (60 B).  What the implementation adds (the
+# cached affine matrix, vt/Ct and Gb round trips, window overlap) shows in the
+# measured traffic, not here.
 KBYTES = {
-    "p2g": (184, 16, 0),        # read x,v,C,F,dFc,m,vol,lam,mu, write the affine matrix A; m,p scatter per node
-    "g2p": (180, 16, 0),        # read x,F,dFc write x,v,C,F; gather m,p (grid update applied on load)
-    "g2p_adj": (228, 12, 0),    # read x,F,dFc,x',v',F',C' write vt,Ct; scatter vbar
-    "p2g_adj": (412, 28, 0),    # read m,A,x,v,vt,Ct,x',F,dFc,vol,lam,mu,C',F' write x,v,C,F grads + dL/d(dFc); gather m,p,vbar
+    "p2g": (148, 16, 0),
+    "g2p": (180, 12, 0),
+    "g2p_adj": (180, 12, 0),
+    "p2g_adj": (196, 60, 0),
 }
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes of one forward + one adjoint particle-step (FP32): per particle
-# 328 + 376; per touched node 76 (reset 16, scatter 16, gather 16 of m,p, ...) +
-# 84 (vbar reset 12, scatter 12, gather 28, ...); no dense grid pass (the grid
-# update and its transpose run inside the gathers, resets cover flagged tiles).
-STEP_BYTES = (704, 160, 0)
+# one forward + one adjoint particle-step: (328 + 376) B per particle, (56 + 72) B
+# per touched node; no dense grid pass (resets cover the flagged tiles only)
+STEP_BYTES = (704, 128, 0)
 # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a
 # device-to-device copy of known size) of the 1M-particle scene's kernels
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_morph_1m_128.txt")
@@ -304,11 +312,14 @@ def main():
         v64 = sc.n * args.episode_steps * args.fp64_steps / r64["time"]
         print(f"# fp64 plain: {r64['time'] / args.fp64_steps * 1e3:.2f} ms/episode, {v64:.4e} particle-steps/s", file=sys.stderr, flush=True)
 
-    # roofline of the dominant kernel
+    # roofline of the dominant kernel: the largest single kernel by total time
+    # (the "p2g_adj" timing covers two launches, the transposed P2G gather and
+    # the elastic adjoint, so it is not a candidate); every kernel's own
+    # fraction goes to kernel_rooflines
     prof = r32["prof"]
-    roof, kernels = None, None
+    roof, kernels, kroof = None, None, None
     if prof is not None:
-        dom = max((k for k in KBYTES), key=lambda k: prof[k][0])
+        dom = max((k for k in KBYTES if k != "p2g_adj"), key=lambda k: prof[k][0])
         ms, launches = prof[dom]
         pb, nb, gb = KBYTES[dom]
         bytes_per_launch = pb * sc.n + nb * touched + gb * sc.G
@@ -320,6 +331,12 @@ def main():
                 "traffic_source": "profiles/traffic_morph_1m_128.txt (rocprofv3 FETCH_SIZE/WRITE_SIZE)" if traffic else None}
         kernels = {k: {"ms_total": round(prof[k][0], 3), "launches": prof[k][1],
                        "avg_us": round(prof[k][0] * 1e3 / max(prof[k][1], 1), 2)} for k in prof}
+        kroof = {}
+        for k, (pb_, nb_, gb_) in KBYTES.items():
+            ms_, n_ = prof[k]
+            if n_:
+                gbs = (pb_ * sc.n + nb_ * touched + gb_ * sc.G) / (ms_ / 1e3 / n_) / 1e9
+                kroof[k] = {"achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
     batched = None
     if args.batched_steps:
@@ -360,6 +377,7 @@ def main():
                           * args.episode_steps * args.steps / r32["time"] / 1e9 / HBM_PEAK_GBS,
                           "note": "one forward + one adjoint per particle-step; recompute counts in the time"},
         "kernels_fp32": kernels,
+        "kernel_rooflines": kroof,
         "kernel_timing": f"HIP events around every launch of {args.prof_steps} eager episodes after the timed region",
         "tuned": r32["tuned"],
         "graphs": not args.no_graphs,
