@@ -31,6 +31,13 @@ def _instance(kind, seed):
         dim, dx, mp = (32, 32, 32), 1.0 / 32, (0.0, 0.0, 0.0)
         x = pkg.scenes.lattice_spheres([0.5, 0.5, 0.5], 0.2, dx / 2)
         tx = pkg.scenes.lattice_box([0.35, 0.4, 0.3], [0.7, 0.7, 0.65], dx / 2)
+    elif kind == "odd_grid":
+        # node count 17*19*23 = 7429, not a multiple of the 32-node clear
+        # tiles: the tiles at this instance's ends are shared with its
+        # neighbours in the batch (the clears then look the instance up per node)
+        dim, dx, mp = (17, 19, 23), 1.0 / 20, (0.0, 0.0, 0.0)
+        x = pkg.scenes.lattice_spheres([0.4, 0.45, 0.55], 0.15, dx / 2)
+        tx = pkg.scenes.lattice_spheres([0.45, 0.45, 0.5], 0.13, dx / 2)
     elif kind == "offset_box":
         # non-cubic grid with an offset origin
         dim, dx, mp = (20, 28, 24), 1.0 / 24, (-0.1, 0.05, -0.2)
@@ -83,7 +90,7 @@ def _single(t, **kw):
     return mo
 
 
-KINDS = ["sphere32", "offset_box", "clusters"]
+KINDS = ["sphere32", "odd_grid", "offset_box", "clusters"]
 
 
 @pytest.fixture(scope="module")
