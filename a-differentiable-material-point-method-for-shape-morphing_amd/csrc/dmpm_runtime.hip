@@ -345,13 +345,14 @@ inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams&, const AdjIO<float>&
 // One ForwardTimeStep: cur -> nxt, grid of the step in g.  g_clean: the
 // previous step of the same launch sequence cleared g already.  gnext (FP32):
 // the grid of the next step of the sequence, another buffer, which G2P's
-// appended workgroups clear; returns whether they did.
+// appended workgroups clear; returns whether they did.  keep_a (FP32): cur is
+// a layer the backward pass reads, so P2G caches its affine matrix there.
 template <typename Real>
-bool step_forward(dmpm_ctx* c, const Real* cur, Real* nxt, const Real* ctrl, Real* g, bool g_clean = false, Real* gnext = nullptr)
+bool step_forward(dmpm_ctx* c, const Real* cur, Real* nxt, const Real* ctrl, Real* g, bool g_clean = false, Real* gnext = nullptr,
+                  bool keep_a = true)
 {
     StepParams sp = step_params(c);
-    // FP32: P2G caches its affine matrix in the input layer
-    StepIO<Real> io{cur, nxt, ctrl, static_cast<const Real*>(c->cst), g, c->f32 ? const_cast<Real*>(cur) : nullptr,
+    StepIO<Real> io{cur, nxt, ctrl, static_cast<const Real*>(c->cst), g, c->f32 && keep_a ? const_cast<Real*>(cur) : nullptr,
                     flags_of(c, g)};
     if (!g_clean) clear_grid(c, g);
     timed(c, DMPM_K_P2G, [&] { p2g_dispatch(c, sp, io); });
@@ -431,7 +432,7 @@ Real* materialize(dmpm_ctx* c, int j)
     for (int i = j0; i < j; ++i) {
         Real* dst = slot<Real>(c, c->roll_base + ((i + 1) & 1));
         clean = step_forward<Real>(c, prev, dst, ctrl_of<Real>(c, i), wgrid_of<Real>(c, i), clean,
-                                   i + 1 < j ? wgrid_of<Real>(c, i + 1) : nullptr);
+                                   i + 1 < j ? wgrid_of<Real>(c, i + 1) : nullptr, false);
         prev = dst;
     }
     return prev;
@@ -488,7 +489,10 @@ int forward_pass(dmpm_ctx* c, int start)
             auto grid_at = [&](int k) {
                 return (k >= s0_last) ? grids + (size_t)(k - s0_last) * kGridFields * c->G : wgrid_of<Real>(c, k);
             };
-            clean = step_forward<Real>(c, cur, dst, ctrl_of<Real>(c, i), grid_at(i), clean, i + 1 < c->L - 1 ? grid_at(i + 1) : nullptr);
+            // steps before the last segment are recomputed for the backward pass:
+            // their affine matrices are not read
+            clean = step_forward<Real>(c, cur, dst, ctrl_of<Real>(c, i), grid_at(i), clean, i + 1 < c->L - 1 ? grid_at(i + 1) : nullptr,
+                                       i >= s0_last);
         }
         });
         if (rc) return rc;
