@@ -318,7 +318,12 @@ __device__ __forceinline__ bool polar_newton(const M3<double>& F, double detF, M
 #pragma unroll
             for (int i = 0; i < 9; ++i) Y.a[i] = cf.a[i] * inv;
         }
-        double g = 1.0;
+        // X <- h X + hi X^-T keeps the polar factor for any h, hi > 0 (both
+        // terms are R times a symmetric positive matrix), so the scaling is
+        // computed with the FP32 hardware reciprocal and square root: it only
+        // sets the speed of convergence.  nx = ny (the identity, the rest
+        // state) keeps g = 1 exactly, so that X stays fixed.
+        double h = 0.5, hi = 0.5;
         if (it < 2) {
             double nx = 0.0, ny = 0.0;
 #pragma unroll
@@ -326,9 +331,12 @@ __device__ __forceinline__ bool polar_newton(const M3<double>& F, double detF, M
                 nx += X.a[i] * X.a[i];
                 ny += Y.a[i] * Y.a[i];
             }
-            g = sqrt(sqrt(ny / nx));
+            if (nx != ny) {
+                const float g = __builtin_amdgcn_sqrtf(__builtin_amdgcn_sqrtf(float(ny) * __builtin_amdgcn_rcpf(float(nx))));
+                h = 0.5 * double(g);
+                hi = 0.5 * double(__builtin_amdgcn_rcpf(g));
+            }
         }
-        const double h = 0.5 * g, hi = 0.5 / g;
         double step = 0.0;
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
