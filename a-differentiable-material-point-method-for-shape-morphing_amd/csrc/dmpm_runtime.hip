@@ -347,15 +347,20 @@ inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams&, const AdjIO<float>&
 // the grid of the next step of the sequence, another buffer, which G2P's
 // appended workgroups clear; returns whether they did.  keep_a (FP32): cur is
 // a layer the backward pass reads, so P2G caches its affine matrix there.
+// with_g2p false: the grid of the step only (its next layer is held already).
 template <typename Real>
 bool step_forward(dmpm_ctx* c, const Real* cur, Real* nxt, const Real* ctrl, Real* g, bool g_clean = false, Real* gnext = nullptr,
-                  bool keep_a = true)
+                  bool keep_a = true, bool with_g2p = true)
 {
     StepParams sp = step_params(c);
     StepIO<Real> io{cur, nxt, ctrl, static_cast<const Real*>(c->cst), g, c->f32 && keep_a ? const_cast<Real*>(cur) : nullptr,
                     flags_of(c, g)};
     if (!g_clean) clear_grid(c, g);
     timed(c, DMPM_K_P2G, [&] { p2g_dispatch(c, sp, io); });
+    if (!with_g2p) {
+        if constexpr (std::is_same<Real, double>::value) timed(c, DMPM_K_GRID_UPDATE, [&] { grid_update_dispatch(c, sp, g); });
+        return false;
+    }
     if constexpr (std::is_same<Real, double>::value) {
         timed(c, DMPM_K_GRID_UPDATE, [&] { grid_update_dispatch(c, sp, g); });
         timed(c, DMPM_K_G2P, [&] { g2p_dispatch(c, sp, io); });
@@ -447,9 +452,12 @@ void recompute_segment(dmpm_ctx* c, int seg)
     bool clean = false;
     for (int i = s0; i < s1; ++i) {
         const Real* cur = (i == s0) ? slot<Real>(c, s0 / c->K) : slot<Real>(c, c->tape_base + (i - s0));
-        Real* dst = ((i + 1) % c->K == 0) ? slot<Real>(c, c->roll_base) : slot<Real>(c, c->tape_base + (i + 1 - s0));
+        // a segment that ends on a checkpoint needs only the grid of its last
+        // step: the backward pass reads the next layer from the checkpoint
+        const bool ends_on_ckpt = (i + 1) % c->K == 0;
+        Real* dst = ends_on_ckpt ? slot<Real>(c, c->roll_base) : slot<Real>(c, c->tape_base + (i + 1 - s0));
         clean = step_forward<Real>(c, cur, dst, ctrl_of<Real>(c, i), grids + (size_t)(i - s0) * kGridFields * c->G, clean,
-                                   i + 1 < s1 ? grids + (size_t)(i + 1 - s0) * kGridFields * c->G : nullptr);
+                                   i + 1 < s1 ? grids + (size_t)(i + 1 - s0) * kGridFields * c->G : nullptr, true, !ends_on_ckpt);
     }
     c->tape_seg = seg;
 }
