@@ -53,6 +53,7 @@ EXPORTS = [
     "dmpm_profile_enable", "dmpm_profile_reset", "dmpm_profile_read", "dmpm_eval_pk1", "dmpm_eval_hessian",
     "dmpm_eval_svd", "dmpm_create_batch", "dmpm_num_instances", "dmpm_get_instance_losses",
     "dmpm_target_from_points_batch", "dmpm_set_option", "dmpm_get_option", "dmpm_autotune", "dmpm_descend",
+    "dmpm_eval_energy", "dmpm_get_point_diagnostics",
 ]
 
 
@@ -140,6 +141,8 @@ def load_library(path: str = LIB_PATH):
         "dmpm_eval_pk1": ([i32, _DP, _DP, _DP, _DP], ctypes.c_int),
         "dmpm_eval_hessian": ([i32, _DP, _DP, _DP, _DP, _DP, i32], ctypes.c_int),
         "dmpm_eval_svd": ([i32, _DP, _DP, _DP, _DP], ctypes.c_int),
+        "dmpm_eval_energy": ([i32, _DP, _DP, _DP, _DP], ctypes.c_int),
+        "dmpm_get_point_diagnostics": ([vp, i32, _DP, _DP], ctypes.c_int),
         "dmpm_create_batch": ([ctypes.POINTER(Config), i32, ctypes.POINTER(Instance), ctypes.POINTER(vp)], ctypes.c_int),
         "dmpm_num_instances": ([vp, ctypes.POINTER(i32)], ctypes.c_int),
         "dmpm_get_instance_losses": ([vp, _DP], ctypes.c_int),
@@ -210,6 +213,18 @@ def eval_svd(F):
     V = np.zeros((n, 9))
     _check(L.dmpm_eval_svd(n, pF, _ptr(U), _ptr(s), _ptr(V)))
     return U.reshape(n, 3, 3), s, V.reshape(n, 3, 3)
+
+
+def eval_energy(F, lam, mu):
+    """FixedCorotatedElasticity (Elasticity.cpp:74-86) on the GPU for a batch of (n,3,3) F."""
+    L = load_library()
+    F, pF = _in(np.asarray(F).reshape(-1, 9))
+    n = F.shape[0]
+    lam, pl = _in(np.broadcast_to(np.asarray(lam, float), (n,)))
+    mu, pm = _in(np.broadcast_to(np.asarray(mu, float), (n,)))
+    out = np.zeros(n)
+    _check(L.dmpm_eval_energy(n, pF, pl, pm, _ptr(out)))
+    return out
 
 
 # ------------------------------------------------------------------ Morph --
@@ -349,6 +364,15 @@ class Morph:
         C = np.zeros((self.n, 3, 3))
         _check(self._L.dmpm_get_layer_state(self._h, int(layer), _ptr(x), _ptr(v), _ptr(F), _ptr(C)))
         return x, v, F, C
+
+    def point_diagnostics(self, layer):
+        """(J, psi) per particle of a layer: det(F + dFc) and the fixed-corotated energy at
+        F + dFc (PointCloud::GetPointDeterminants / GetPointElasticEnergies,
+        PointCloud.cpp:441-467), FP64 on the GPU."""
+        J = np.zeros(self.n)
+        psi = np.zeros(self.n)
+        _check(self._L.dmpm_get_point_diagnostics(self._h, int(layer), _ptr(J), _ptr(psi)))
+        return J, psi
 
     def set_layer_state(self, layer, x=None, v=None, F=None, C=None):
         arrs = [_in(a) for a in (x, v, F, C)]

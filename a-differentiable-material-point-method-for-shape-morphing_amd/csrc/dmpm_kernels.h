@@ -141,5 +141,9 @@ void launch_eval_pk1(int n, const double* F, const double* lam, const double* mu
 void launch_eval_hess(int n, const double* F, const double* lam, const double* mu, const double* dF, double* out,
                       int exact, hipStream_t s);
 void launch_eval_svd(int n, const double* F, double* U, double* sig, double* V, hipStream_t s);
+void launch_eval_energy(int n, const double* F, const double* lam, const double* mu, double* psi, hipStream_t s);
+// per-slot det(F + dFc) into out[0..S) and elastic energy into out[S..2S) of one layer
+template <typename Real>
+void launch_point_diag(long long S, const Real* layer, const Real* ctrl, const Real* cst, double* out, hipStream_t s);
 
 }  // namespace dmpm

@@ -796,6 +796,51 @@ void launch_eval_svd(int n, const double* F, double* U, double* sig, double* V, 
     hipLaunchKernelGGL(k_eval_svd, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, s, n, F, U, sig, V);
 }
 
+// Elasticity.cpp:74-86 for a batch
+__global__ __launch_bounds__(kBlock) void k_eval_energy(int n, const double* F, const double* lam, const double* mu, double* psi)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    M3<double> A;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) A.a[i] = F[9 * p + i];
+    FCE e;
+    fce_prepare(A, e);
+    psi[p] = fce_energy(e, lam[p], mu[p]);
+}
+
+void launch_eval_energy(int n, const double* F, const double* lam, const double* mu, double* psi, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_eval_energy, dim3(nblocks(n, kBlock)), dim3(kBlock), 0, s, n, F, lam, mu, psi);
+}
+
+// PointCloud::GetPointDeterminants / GetPointElasticEnergies (PointCloud.cpp:441-467)
+// over the S slots of one layer: Fe = F + dFc in FP64 from the stored values,
+// out[p] = det Fe, out[S + p] = psi(Fe) (Elasticity.cpp:74-86).  ctrl may be
+// null (a layer without a control: dFc = 0).
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_point_diag(long long S, const Real* layer, const Real* ctrl, const Real* cst, double* out)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= S) return;
+    M3<double> A;
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+        A.a[i] = double(layer[(long long)(FF + i) * S + p]) + (ctrl ? double(ctrl[(long long)i * S + p]) : 0.0);
+    FCE e;
+    fce_prepare(A, e);
+    out[p] = e.Ju;
+    out[S + p] = fce_energy(e, double(cst[(long long)KLAM * S + p]), double(cst[(long long)KMU * S + p]));
+}
+
+template <typename Real>
+void launch_point_diag(long long S, const Real* layer, const Real* ctrl, const Real* cst, double* out, hipStream_t s)
+{
+    if (S <= 0) return;
+    hipLaunchKernelGGL(k_point_diag<Real>, dim3(nblocks(S, kBlock)), dim3(kBlock), 0, s, S, layer, ctrl, cst, out);
+}
+
 // ---------------------------------------------------- instantiations -------
 // ------------------------------------------------------------ optimizer ----
 // PointCloud::Descend_* (PointCloud.cpp:83-179) per control entry, in the
@@ -884,6 +929,7 @@ void launch_sumsq(const Real* a, long long count, double* part, double* out, hip
 #define DMPM_INSTANTIATE(R)                                                                                        \
     template void launch_descend<R>(long long, const DescendParams&, R*, const R*, R*, R*, R*, hipStream_t);      \
     template void launch_sumsq<R>(const R*, long long, double*, double*, hipStream_t);                             \
+    template void launch_point_diag<R>(long long, const R*, const R*, const R*, double*, hipStream_t);             \
     template void launch_fill<R>(R*, long long, R, hipStream_t);                                                   \
     template void launch_p2g<R>(const StepParams&, const StepIO<R>&, hipStream_t);                                 \
     template void launch_grid_update<R>(const StepParams&, R*, hipStream_t);                                       \

@@ -415,6 +415,28 @@ __device__ __forceinline__ M3<double> pk1_times_ft(const M3<double>& F, double l
     return PF;
 }
 
+// Elasticity.cpp:74-86: psi = mu sum_i (s_i - 1)^2 + lam/2 (J - 1)^2 with the
+// clamped J and the signed singular values (the sign of det F on the last).
+// On the polar path R^T F = S = V diag(s) V^T, so sum_i (s_i - 1)^2 = |S - I|_F^2
+// without an eigen-decomposition (and without cancellation near the rest state).
+__device__ __forceinline__ double fce_energy(const FCE& e, double lam, double mu)
+{
+    double sum = 0.0;
+    if (e.svd) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) sum += (e.s[a] - 1.0) * (e.s[a] - 1.0);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double d = e.S.a[3 * i + j] - (i == j ? 1.0 : 0.0);
+                sum += d * d;
+            }
+    }
+    return mu * sum + 0.5 * lam * (e.J - 1.0) * (e.J - 1.0);
+}
+
 // Elasticity.cpp:88-98
 __device__ __forceinline__ M3<double> fce_pk1(const M3<double>& F, const FCE& e, double lam, double mu)
 {

@@ -860,6 +860,25 @@ int get_layer_state(dmpm_ctx* c, int layer, double* x, double* v, double* F, dou
     return DMPM_OK;
 }
 
+// PointCloud::GetPointDeterminants / GetPointElasticEnergies (PointCloud.cpp:441-467)
+// of one layer, in FP64 on the device; the tmp buffer (12 S values of the
+// path's precision, free between steps) holds the 2 S doubles.
+template <typename Real>
+int get_point_diagnostics(dmpm_ctx* c, int layer, double* J, double* psi)
+{
+    if (layer < 0 || layer >= c->L) return fail(DMPM_ERR_INVALID, "layer out of range");
+    if (layer > c->fwd_valid) return fail(DMPM_ERR_STATE, "layer not computed by a forward pass");
+    const Real* l = materialize<Real>(c, layer);
+    const Real* ctrl = layer < c->L - 1 ? ctrl_of<Real>(c, layer) : nullptr;
+    double* out = static_cast<double*>(c->tmp);
+    launch_point_diag<Real>(c->S, l, ctrl, static_cast<const Real*>(c->cst), out, c->stream);
+    HIP_TRY(hipGetLastError());
+    int rc;
+    if (J && (rc = download_fields<double>(c, out, 0, 1, J, c->N))) return rc;
+    if (psi && (rc = download_fields<double>(c, out, 1, 1, psi, c->N))) return rc;
+    return DMPM_OK;
+}
+
 template <typename Real>
 int set_layer_state(dmpm_ctx* c, int layer, const double* x, const double* v, const double* F, const double* C)
 {
@@ -1360,6 +1379,13 @@ int dmpm_get_layer_state(dmpm_ctx* c, int32_t layer, double* x, double* v, doubl
     return DISPATCH(c, get_layer_state, c, layer, x, v, F, C);
 }
 
+int dmpm_get_point_diagnostics(dmpm_ctx* c, int32_t layer, double* J, double* psi)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    return DISPATCH(c, get_point_diagnostics, c, layer, J, psi);
+}
+
 int dmpm_set_layer_state(dmpm_ctx* c, int32_t layer, const double* x, const double* v, const double* F, const double* C)
 {
     int rc = check_ctx(c);
@@ -1682,6 +1708,13 @@ int dmpm_eval_hessian(int32_t n, const double* F, const double* lam, const doubl
 {
     return eval_common(n, {{F, 9}, {lam, 1}, {mu, 1}, {dF, 9}}, {{out, 9}}, [exact](int n_, std::vector<double*>& p) {
         launch_eval_hess(n_, p[0], p[1], p[2], p[3], p[4], exact, nullptr);
+    });
+}
+
+int dmpm_eval_energy(int32_t n, const double* F, const double* lam, const double* mu, double* psi)
+{
+    return eval_common(n, {{F, 9}, {lam, 1}, {mu, 1}}, {{psi, 1}}, [](int n_, std::vector<double*>& p) {
+        launch_eval_energy(n_, p[0], p[1], p[2], p[3], nullptr);
     });
 }
 

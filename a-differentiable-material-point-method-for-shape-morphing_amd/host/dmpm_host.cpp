@@ -300,8 +300,22 @@ std::vector<Mat3> PointCloud::GetPointDefGrads() const
 std::vector<float> PointCloud::GetPointDeterminants() const
 {
     std::vector<float> r;
-    for (const MaterialPoint& mp : points) r.push_back(mp.F.determinant());
+    for (const MaterialPoint& mp : points) r.push_back((mp.F + mp.dFc).determinant());
     return r;
+}
+// all points in one batched GPU evaluation (FP64, rounded to float)
+std::vector<float> PointCloud::GetPointElasticEnergies() const
+{
+    const size_t n = points.size();
+    std::vector<double> F(9 * n), lam(n), mu(n), psi(n);
+    for (size_t p = 0; p < n; ++p) {
+        const Mat3 Fe = points[p].F + points[p].dFc;
+        for (int i = 0; i < 9; ++i) F[9 * p + i] = Fe.d[i];
+        lam[p] = points[p].lam;
+        mu[p] = points[p].mu;
+    }
+    if (n) check(dmpm_eval_energy((int32_t)n, F.data(), lam.data(), mu.data(), psi.data()), "dmpm_eval_energy");
+    return std::vector<float>(psi.begin(), psi.end());
 }
 
 // .mpmbin: cereal's binary archive of PointCloud (PointCloud.cpp:334-362,
@@ -471,6 +485,15 @@ void Grid::ResetValues()
 }
 
 // ------------------------------------------------------------ Elasticity ----
+// Elasticity.cpp:74-86 on the GPU (FP64)
+float FixedCorotatedElasticity(const Mat3& F, float lam, float mu)
+{
+    double f[9], l = lam, m = mu, psi = 0.0;
+    for (int i = 0; i < 9; ++i) f[i] = F.d[i];
+    check(dmpm_eval_energy(1, f, &l, &m, &psi), "dmpm_eval_energy");
+    return (float)psi;
+}
+
 // Elasticity.cpp:88-98 on the GPU (FP64)
 Mat3 PK_FixedCorotatedElasticity(const Mat3& F, float lam, float mu)
 {

@@ -204,7 +204,9 @@ struct PointCloud {
     std::vector<float> GetPointMasses() const;
     std::vector<float> GetPointVolumes() const;
     std::vector<Mat3> GetPointDefGrads() const;
+    // det(F + dFc) and FixedCorotatedElasticity(F + dFc, lam, mu) per point (PointCloud.cpp:441-467)
     std::vector<float> GetPointDeterminants() const;
+    std::vector<float> GetPointElasticEnergies() const;
 
     std::vector<MaterialPoint> points;
 };
@@ -311,6 +313,7 @@ void Back_Timestep(CompGraphLayer& layer_nplus1, CompGraphLayer& layer_n, float 
 
 // ------------------------------------------------------------ Elasticity ----
 // Elasticity.h:9-23
+float FixedCorotatedElasticity(const Mat3& F, float lam, float mu);
 Mat3 PK_FixedCorotatedElasticity(const Mat3& F, float lam, float mu);
 Mat3 d2_FCE_psi_dF2_mult_by_dF(const Mat3& F, float lam, float mu, const Mat3& dF);
 void CalculateLameParameters(float young_mod, float poisson, float& lam, float& mu);

@@ -141,6 +141,12 @@ int dmpm_backward_step(dmpm_ctx* ctx, int32_t layer);
 
 /* State of a layer (any pointer may be NULL); checkpoint mode recomputes it when needed. */
 int dmpm_get_layer_state(dmpm_ctx* ctx, int32_t layer, double* x, double* v, double* F, double* C);
+/* Per-particle diagnostics of a layer (either pointer may be NULL), computed on the GPU in FP64
+ * from the layer's stored F and control dFc (dFc = 0 for the last layer, which has no control):
+ * J = det(F + dFc) (PointCloud::GetPointDeterminants, PointCloud.cpp:441-452) and
+ * psi = FixedCorotatedElasticity(F + dFc, lam, mu) (PointCloud::GetPointElasticEnergies,
+ * PointCloud.cpp:455-467). */
+int dmpm_get_point_diagnostics(dmpm_ctx* ctx, int32_t layer, double* J, double* psi);
 /* Store-all tape: replace a computed layer's state (the C++ host mirror's write-back). */
 int dmpm_set_layer_state(dmpm_ctx* ctx, int32_t layer, const double* x, const double* v, const double* F,
                          const double* C);
@@ -201,6 +207,9 @@ int dmpm_eval_pk1(int32_t n, const double* F, const double* lam, const double* m
 int dmpm_eval_hessian(int32_t n, const double* F, const double* lam, const double* mu, const double* dF,
                       double* out, int32_t exact);
 int dmpm_eval_svd(int32_t n, const double* F, double* U, double* sigma, double* V);
+/* FixedCorotatedElasticity (Elasticity.cpp:74-86) for n matrices: psi = mu sum (sigma_i - 1)^2
+ * + lam/2 (J - 1)^2, J = max(det F, 1e-6), FP64. */
+int dmpm_eval_energy(int32_t n, const double* F, const double* lam, const double* mu, double* psi);
 
 #ifdef __cplusplus
 }

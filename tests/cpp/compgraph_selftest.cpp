@@ -8,7 +8,9 @@
 // outputs: loss.txt, last.mpmbin (last layer), l0.mpmbin (layer 0 with its
 // gradients after ComputeBackwardPass(0)), bt.mpmbin (layer 0 gradients from
 // Back_Timestep on layer 1's gradients), step.mpmbin (ForwardTimeStep of layer 0),
-// pk1.txt (PK_FixedCorotatedElasticity at F = 0.8 Rx(45 deg)).
+// pk1.txt (PK_FixedCorotatedElasticity, then FixedCorotatedElasticity, at
+// F = 0.8 Rx(45 deg)), diag.txt (GetPointDeterminants, GetPointElasticEnergies
+// of layer 0, one point a line).
 #include "DiffMPMLib3D/DiffMPMLib3D.h"
 
 #include <cmath>
@@ -68,6 +70,14 @@ int main(int argc, char** argv)
         std::ofstream(dir + "/loss.txt") << std::scientific << (double)loss << "\n";
         cg.layers.back().point_cloud->WriteEntirePointCloudToBinaryFile(dir + "/last.mpmbin");
         cg.layers.front().point_cloud->WriteEntirePointCloudToBinaryFile(dir + "/l0.mpmbin");
+        {
+            // per-point diagnostics of layer 0 (its F and dFc): J, then psi
+            const std::vector<float> J = cg.layers.front().point_cloud->GetPointDeterminants();
+            const std::vector<float> psi = cg.layers.front().point_cloud->GetPointElasticEnergies();
+            std::ofstream o(dir + "/diag.txt");
+            o.precision(9);
+            for (size_t p = 0; p < J.size(); ++p) o << J[p] << " " << psi[p] << "\n";
+        }
 
         // Back_Timestep on the full gradients of layer 1
         cg.ComputeForwardPass(0);
@@ -95,7 +105,8 @@ int main(int argc, char** argv)
         const Mat3 P = PK_FixedCorotatedElasticity(F, 58333.0f, 38888.9f);
         std::ofstream o(dir + "/pk1.txt");
         o.precision(9);
-        for (int i = 0; i < 9; ++i) o << P.d[i] << (i == 8 ? "\n" : " ");
+        for (int i = 0; i < 9; ++i) o << P.d[i] << " ";
+        o << FixedCorotatedElasticity(F, 58333.0f, 38888.9f) << "\n";
     } catch (const std::exception& e) {
         std::cerr << "error: " << e.what() << "\n";
         return 3;

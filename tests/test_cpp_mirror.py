@@ -1,10 +1,11 @@
 """The C++ host mirror of the reference API (include/DiffMPMLib3D/DiffMPMLib3D.h).
 
 CPU: the .mpmbin format round trip through the C++ reader/writer.
-GPU: CompGraph passes, EndLayerMassLoss, Back_Timestep, ForwardTimeStep and
-PK1 through the mirror, against the FP64 oracle in reference-compat mode
-(the mirror's default).  Tolerances: the mirror stores float fields like the
-reference, so states compare to 1e-6 and gradients to 1e-5 relative.
+GPU: CompGraph passes, EndLayerMassLoss, Back_Timestep, ForwardTimeStep, PK1,
+the elastic energy and the per-point J / energy getters through the mirror,
+against the FP64 oracle in reference-compat mode (the mirror's default).
+Tolerances: the mirror stores float fields like the reference, so states
+compare to 1e-6 and gradients to 1e-5 relative.
 """
 import math
 import os
@@ -93,8 +94,18 @@ def test_cpp_mirror_compgraph_vs_oracle(tmp_path):
     assert rel(bt["dLdF"], g0.F) < 1e-5 and rel(bt["dLdx"], g0.x) < 1e-5
     step = mpmbin.read(tmp_path / "step.mpmbin")
     assert maxrel(step["x"], layers[1].x) < 1e-6 and maxrel(step["F"], layers[1].F) < 1e-6
-    P = np.array([float(t) for t in open(tmp_path / "pk1.txt").read().split()]).reshape(3, 3)
+    vals = np.array([float(t) for t in open(tmp_path / "pk1.txt").read().split()])
+    P, psi0 = vals[:9].reshape(3, 3), vals[9]
     s = 0.8
     R = np.array([[1, 0, 0], [0, math.cos(math.pi / 4), -math.sin(math.pi / 4)], [0, math.sin(math.pi / 4), math.cos(math.pi / 4)]])
     lam, mu = 58333.0, 38888.9
     assert np.allclose(P, (2 * mu * (s - 1) + lam * (s ** 3 - 1) * s ** 2) * R, rtol=1e-5, atol=1e-2)
+    assert psi0 == pytest.approx(3 * mu * (s - 1) ** 2 + 0.5 * lam * (s ** 3 - 1) ** 2, rel=1e-5)
+    # GetPointDeterminants / GetPointElasticEnergies of layer 0: F + dFc summed in
+    # float like the reference, then det in float and the energy in FP64 (printed
+    # as float)
+    diag = np.loadtxt(tmp_path / "diag.txt").reshape(-1, 2)
+    Fe = (np.float32(st.F) + np.float32(dFc)).astype(np.float64)
+    assert maxrel(diag[:, 0], np.linalg.det(Fe)) < 1e-6
+    psi_ref = np.array([orc.energy(Fe[k], cs.lam[k], cs.mu[k]) for k in range(len(Fe))])
+    assert maxrel(diag[:, 1], psi_ref) < 1e-6

@@ -55,6 +55,58 @@ def test_hessian_gpu_vs_oracle(exact):
     assert maxrel(H, ref) < 1e-10
 
 
+def _energies(F, lam, mu):
+    return np.array([orc.energy(F[k], lam[k], mu[k]) for k in range(len(F))])
+
+
+def test_energy_gpu_vs_oracle():
+    """FixedCorotatedElasticity: random F (polar path), det F < 0 and singular F
+    (SVD path), near-rest F; 1e-12 relative to the batch's largest energy, 1e-9
+    for the near-rest batch (energies ~1e-8 mu)."""
+    rng = np.random.default_rng(6)
+    F = np.concatenate([_rand_F(1000, 6, 0.4), -_rand_F(50, 7, 0.2), np.diag([3.0, 3.0, 0.0])[None], np.eye(3)[None]])
+    lam = rng.uniform(1e3, 1e5, len(F))
+    mu = rng.uniform(1e3, 1e5, len(F))
+    assert maxrel(pkg.eval_energy(F, lam, mu), _energies(F, lam, mu)) < 1e-12
+    Fr = _rand_F(200, 8, 1e-4)
+    assert maxrel(pkg.eval_energy(Fr, lam[:200], mu[:200]), _energies(Fr, lam[:200], mu[:200])) < 1e-9
+    # closed forms: F = s R gives sigma = (s, s, s); diag(2, -0.5, 1) has signed
+    # sigma (2, 1, -0.5) and det -1 clamped to 1e-6; the identity is the rest state
+    lam0, mu0, s = 58333.0, 38888.9, 0.8
+    c, sn = np.cos(np.pi / 4), np.sin(np.pi / 4)
+    R = np.array([[1, 0, 0], [0, c, -sn], [0, sn, c]])
+    got = pkg.eval_energy(np.stack([s * R, np.diag([2.0, -0.5, 1.0]), np.eye(3)]), lam0, mu0)
+    assert got[0] == pytest.approx(3 * mu0 * (s - 1) ** 2 + 0.5 * lam0 * (s ** 3 - 1) ** 2, rel=1e-13)
+    assert got[1] == pytest.approx(3.25 * mu0 + 0.5 * lam0 * (1e-6 - 1) ** 2, rel=1e-13)
+    assert got[2] == 0.0
+
+
+@pytest.mark.parametrize("prec,tape,tol", [(0, 0, (1e-12, 1e-12)), (1, 0, (1e-12, 1e-12)), (1, 1, (1e-5, 1e-4))],
+                         ids=["f64_store_all", "f32_store_all", "f32_checkpoint"])
+def test_point_diagnostics_gpu(prec, tape, tol):
+    """det(F + dFc) and the energy at F + dFc per particle of a layer, against the
+    oracle on the layer's F and control as the context returns them (both in FP64
+    from the stored values): 1e-12 relative on a stored tape.  The checkpoint tape
+    recomputes layers 3 and 5 for each call, and an FP32 recompute is reproducible
+    only to float rounding (workgroups add their node sums to the grid with float
+    atomics in any order): 1e-5 on J, 1e-4 on the energy (|F - I| ~ 1e-2).  The
+    last layer has no control."""
+    case = make_case(L=6, per_layer_ctrl=True)
+    mo = gpu_morph(case, precision=prec, tape=tape)
+    mo.forward_pass(0)
+    lam, mu = case.cs.lam, case.cs.mu
+    if prec == 1:  # the FP32 path stores float Lame parameters
+        lam, mu = np.float32(lam).astype(float), np.float32(mu).astype(float)
+    for layer in (0, 3, case.L - 1):
+        _, _, F, _ = mo.layer_state(layer)
+        if layer < case.L - 1:
+            F = F + mo.control(layer)
+        J, psi = mo.point_diagnostics(layer)
+        assert maxrel(J, np.linalg.det(F)) < tol[0]
+        assert maxrel(psi, _energies(F, lam, mu)) < tol[1]
+    mo.close()
+
+
 # ----------------------------------------------------------- FP64 step -----
 def test_volumes_and_target_gpu():
     case = make_case(physical_vol=False)
