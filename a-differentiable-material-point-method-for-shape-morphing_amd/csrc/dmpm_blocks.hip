@@ -1055,29 +1055,26 @@ __global__ __launch_bounds__(kG2PAdj, 4) void k_g2p_adj_blk(Batch bt, AdjIO<floa
 
 // Elastic half of P2G^T + P_op_1^T for particle p (FP64 polar, PK1, H:dF):
 //   Pbar = -k Gb Fc,  Fcbar = (1-s)(I + dt C')^T F' - k Gb^T P + H(Fc):Pbar.
-__device__ __forceinline__ void elastic_adj(const StepParams& sp, const AdjIO<float>& io, int p, const M3<float>& Gb)
+// e holds det Fc, F^-T and the rotation with S (polar path, SVD false) or
+// U, sigma, V (SVD); each path is its own copy of the code, so that registers
+// are allocated for one decomposition at a time.
+template <bool SVD>
+__device__ __forceinline__ void elastic_adj_path(const StepParams& sp, const AdjIO<float>& io, int p, const M3<float>& Gb,
+                                                 const M3<double>& Fc, FCE& e, double vol, double lam, double mu)
 {
+    e.svd = SVD;
     const int S = sp.stride;
     const int cp = sp.compat;
     const float dt = float(sp.dt);
     const double kst = -3.0 * sp.inv_dx * sp.inv_dx * sp.dt;
-    const M3<float> F = ldm(io.cur, FF, S, p);
-    const M3<float> dF = ldm(io.ctrl, 0, S, p);
-    const double vol = io.cst[KVOL * S + p];
-    const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
-    M3<double> Fc;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
-    FCE e;
-    fce_prepare(Fc, e);
-    const M3<double> P = fce_pk1(Fc, e, lam, mu);
     const double kk_ = -kst * vol;  // 3 dt vol / dx^2
     const M3<double> Gbd = cast<double>(Gb);
+    // Gb^T P first, so that P dies before H:dF
+    const M3<double> GTP = tmul(Gbd, fce_pk1(Fc, e, lam, mu));
     M3<double> Pb = mul(Gbd, Fc);
 #pragma unroll
     for (int i = 0; i < 9; ++i) Pb.a[i] *= -kk_;
     const M3<double> H = fce_hess_mult(e, lam, mu, Pb, cp == 0);
-    const M3<double> GTP = tmul(Gbd, P);
     const M3<float> C1 = ldm(io.next, FC, S, p);
     const M3<float> gF1 = ldm(io.g1, FF, S, p);
     M3<float> IC;
@@ -1103,6 +1100,38 @@ __device__ __forceinline__ void elastic_adj(const StepParams& sp, const AdjIO<fl
     }
     stm(io.g0, FF, S, p, gF);
     stm(io.gctrl, 0, S, p, gdFc);
+}
+
+__device__ __forceinline__ void elastic_adj(const StepParams& sp, const AdjIO<float>& io, int p, const M3<float>& Gb)
+{
+    const int S = sp.stride;
+    const M3<float> F = ldm(io.cur, FF, S, p);
+    const M3<float> dF = ldm(io.ctrl, 0, S, p);
+    const double vol = io.cst[KVOL * S + p];
+    const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
+    M3<double> Fc;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
+    // fce_prepare, split by path
+    FCE e;
+    e.Ju = det(Fc);
+    e.J = e.Ju < kJMin ? kJMin : e.Ju;
+    const M3<double> cf = cofactor(Fc);
+    const double inv = 1.0 / e.Ju;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) e.Fit.a[i] = cf.a[i] * inv;
+    if (polar_newton(Fc, e.Ju, e.R)) {
+        const M3<double> RF = tmul(e.R, Fc);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) e.S.a[3 * i + j] = 0.5 * (RF.a[3 * i + j] + RF.a[3 * j + i]);
+        elastic_adj_path<false>(sp, io, p, Gb, Fc, e, vol, lam, mu);
+    } else {
+        svd3(Fc, e.U, e.s, e.V);
+        e.R = mult(e.U, e.V);
+        elastic_adj_path<true>(sp, io, p, Gb, Fc, e, vol, lam, mu);
+    }
 }
 
 // ------------------------------------------------------- transposed P2G ----
