@@ -17,7 +17,7 @@ import os
 
 import numpy as np
 
-from . import mpmbin, optimize, scenes  # noqa: F401
+from . import mpmbin, optimize, optinput, scenes  # noqa: F401
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libdmpm.so")
