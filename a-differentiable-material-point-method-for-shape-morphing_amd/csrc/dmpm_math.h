@@ -299,8 +299,9 @@ __device__ __forceinline__ void svd3(const M3<double>& A, M3<double>& Uo, double
 // polar decomposition -- with applications", 1986) reaches in 3-5 steps for
 // the near-rotations of an elastic solid, at a fraction of the Jacobi SVD's
 // cost.  Scaling g = sqrt(|X^-1|_F / |X|_F) is used in the first two steps;
-// the iteration stops once a step moves X by less than 1e-10 (quadratic
-// convergence leaves the next error below rounding).  Returns false (the
+// a step that moves X by at most 1e-8 ends the iteration: a step leaves X
+// within O(|X - R|^2) of R and moves it by |X - R| to first order, so the
+// error after it is ~1e-16, FP64 rounding.  Returns false (the
 // caller then takes the SVD) unless det F > 1e-6 |F|_F^3 and the iteration
 // converged in 16 steps.  detF = det F is an input.
 // ---------------------------------------------------------------------------
@@ -318,11 +319,14 @@ __device__ __forceinline__ bool polar_newton(const M3<double>& F, double detF, M
 #pragma unroll
             for (int i = 0; i < 9; ++i) Y.a[i] = cf.a[i] * inv;
         }
-        // X <- h X + hi X^-T keeps the polar factor for any h, hi > 0 (both
-        // terms are R times a symmetric positive matrix), so the scaling is
-        // computed with the FP32 hardware reciprocal and square root: it only
-        // sets the speed of convergence.  nx = ny (the identity, the rest
-        // state) keeps g = 1 exactly, so that X stays fixed.
+        // X <- (g X + X^-T / g) / 2 keeps the polar factor for any g > 0
+        // (both terms are R times a symmetric positive matrix), so g comes
+        // from the FP32 hardware reciprocal and square root: it only sets the
+        // speed of convergence.  1/g itself is FP64-accurate (hardware
+        // reciprocal, two Newton steps): h hi = 1/4 makes the step's error
+        // second order, where an FP32 1/g would leave X off R by a factor
+        // 1 + O(1e-7) for the next step to remove.  nx = ny (the identity, the
+        // rest state) keeps g = 1 exactly, so that X stays fixed.
         double h = 0.5, hi = 0.5;
         if (it < 2) {
             double nx = 0.0, ny = 0.0;
@@ -332,9 +336,12 @@ __device__ __forceinline__ bool polar_newton(const M3<double>& F, double detF, M
                 ny += Y.a[i] * Y.a[i];
             }
             if (nx != ny) {
-                const float g = __builtin_amdgcn_sqrtf(__builtin_amdgcn_sqrtf(float(ny) * __builtin_amdgcn_rcpf(float(nx))));
-                h = 0.5 * double(g);
-                hi = 0.5 * double(__builtin_amdgcn_rcpf(g));
+                const double g = double(__builtin_amdgcn_sqrtf(__builtin_amdgcn_sqrtf(float(ny) * __builtin_amdgcn_rcpf(float(nx)))));
+                double r = __builtin_amdgcn_rcp(g);
+                r = fma(r, fma(-g, r, 1.0), r);
+                r = fma(r, fma(-g, r, 1.0), r);
+                h = 0.5 * g;
+                hi = 0.5 * r;
             }
         }
         double step = 0.0;
@@ -345,7 +352,7 @@ __device__ __forceinline__ bool polar_newton(const M3<double>& F, double detF, M
             step += d * d;
             X.a[i] = xn;
         }
-        if (step <= 1e-20) {
+        if (step <= 1e-16) {
             R = X;
             return true;
         }
