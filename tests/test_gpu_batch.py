@@ -145,6 +145,25 @@ def test_batch_matches_single_contexts(insts):
         assert rel(gb[i], mo.control_grad(0)) < 1e-6
 
 
+def test_batch_point_diagnostics(insts):
+    """det(F + dFc) and the elastic energy per particle of a batch layer, in API
+    order across the instances, against the oracle on the layer's F and control
+    as the batch returns them (FP64 from the stored floats): 1e-12 relative."""
+    mb = _batch(insts)
+    mb.forward_pass(0)
+    cat = lambda f: np.concatenate([f(t) for t in insts])  # noqa: E731
+    lam = np.float32(cat(lambda t: t[2].lam)).astype(float)
+    mu = np.float32(cat(lambda t: t[2].mu)).astype(float)
+    for layer in (2, L - 1):
+        F = mb.layer_state(layer)[2]
+        if layer < L - 1:
+            F = F + mb.control(layer)
+        J, psi = mb.point_diagnostics(layer)
+        assert maxrel(J, np.linalg.det(F)) < 1e-12
+        ref = np.array([orc.energy(F[k], lam[k], mu[k]) for k in range(len(F))])
+        assert maxrel(psi, ref) < 1e-12
+
+
 def test_batch_volumes_targets_and_empty_instance(insts):
     """GPU volumes and target P2G per instance; an instance with no particles."""
     geo = [(p.n, p.dim, p.dx, p.min_point) for p, *_ in insts] + [(0, (8, 12, 10), 0.1, (0.2, -0.3, 0.0))]
