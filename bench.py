@@ -269,7 +269,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a design note only)"}))
+        print(json.dumps({"error": "bench.py measures one GPU; --gpus must be 1 (sharding across GPUs is described, not implemented)"}))
         sys.exit(2)
 
     pkg = _load_pkg()
