@@ -3,6 +3,7 @@
 // tests/test_cpp_mirror.py:
 //   compgraph_selftest roundtrip IN.mpmbin OUT.mpmbin      (no GPU)
 //   compgraph_selftest run DIR                             (GPU)
+//   compgraph_selftest optimize DIR                        (GPU)
 // DIR holds in.mpmbin (layer 0, dFc set), target.bin (float64 node masses),
 // params.txt ("dims dx minx miny minz dt drag s fx fy fz layers compat");
 // outputs: loss.txt, last.mpmbin (last layer), l0.mpmbin (layer 0 with its
@@ -10,7 +11,8 @@
 // Back_Timestep on layer 1's gradients), step.mpmbin (ForwardTimeStep of layer 0),
 // pk1.txt (PK_FixedCorotatedElasticity, then FixedCorotatedElasticity, at
 // F = 0.8 Rx(45 deg)), diag.txt (GetPointDeterminants, GetPointElasticEnergies
-// of layer 0, one point a line).
+// of layer 0, one point a line); optimize: history.txt (gradient norms of the
+// optimisation), final_loss.txt, opt_l0.mpmbin (layer 0 with its optimised dFc).
 #include "DiffMPMLib3D/DiffMPMLib3D.h"
 
 #include <cmath>
@@ -29,8 +31,9 @@ int main(int argc, char** argv)
         pc.WriteEntirePointCloudToBinaryFile(argv[3]);
         return 0;
     }
-    if (argc != 3 || std::string(argv[1]) != "run") {
-        std::cerr << "usage: compgraph_selftest roundtrip IN OUT | run DIR\n";
+    const std::string mode = argc == 3 ? std::string(argv[1]) : std::string();
+    if (mode != "run" && mode != "optimize") {
+        std::cerr << "usage: compgraph_selftest roundtrip IN OUT | run DIR | optimize DIR\n";
         return 1;
     }
     const std::string dir = argv[2];
@@ -61,6 +64,20 @@ int main(int argc, char** argv)
         }
         const Vec3 fe(f[0], f[1], f[2]);
 
+        if (mode == "optimize") {
+            // the GUI's optimisation call (DiffMPMAnimator3D.cpp:436-447) on a short
+            // sequence: control stride 1, 2 descent iterations with 3 line-search
+            // trials per control layer; the gradient-norm history and the final
+            // loss and layer-0 control are written out
+            CompGraph cg(pc, grid, target);
+            cg.OptimizeDefGradControlSequence(layers, dt, drag, fe, 1, 2, 3, 0.05f, 1e-3f, s);
+            std::ofstream h(dir + "/history.txt");
+            h.precision(9);
+            for (float g : cg.gradientNormHistory) h << g << "\n";
+            std::ofstream(dir + "/final_loss.txt") << std::scientific << (double)cg.EndLayerMassLoss() << "\n";
+            cg.layers.front().point_cloud->WriteEntirePointCloudToBinaryFile(dir + "/opt_l0.mpmbin");
+            return 0;
+        }
         CompGraph cg(pc, grid, target);
         cg.SetSimulationParameters(dt, drag, fe, s);
         cg.SetUpCompGraph(layers);

@@ -59,8 +59,9 @@ def test_mpmbin_rejects_truncated(tmp_path):
     assert r.returncode == 2
 
 
-@pytest.mark.gpu
-def test_cpp_mirror_compgraph_vs_oracle(tmp_path):
+def _write_case(tmp_path):
+    """The L=4 sphere case in the selftest's input files; returns the oracle's
+    float-rounded inputs (case, state, constants, control, params, target)."""
     case = make_case(L=4, per_layer_ctrl=False, drag=0.2, smoothing=0.05)
     sc, st, cs = case.sc, case.st0, case.cs
     # the mirror reads float fields: give the oracle the same float-rounded inputs
@@ -78,6 +79,12 @@ def test_cpp_mirror_compgraph_vs_oracle(tmp_path):
     with open(tmp_path / "params.txt", "w") as fh:
         fh.write(f"{sc.dim[0]} {sc.dim[1]} {sc.dim[2]} {sc.dx!r} 0 0 0 {prm.dt!r} {prm.drag!r} {prm.smoothing!r} 0 0 0 "
                  f"{case.L} {orc.COMPAT_ALL}\n")
+    return case, st, cs, dFc, prm, target
+
+
+@pytest.mark.gpu
+def test_cpp_mirror_compgraph_vs_oracle(tmp_path):
+    case, st, cs, dFc, prm, target = _write_case(tmp_path)
     subprocess.run([BIN, "run", str(tmp_path)], check=True, timeout=300)
 
     controls = [dFc] * (case.L - 1)  # SetUpCompGraph copies layer 0 (and its dFc) into every layer
@@ -109,3 +116,27 @@ def test_cpp_mirror_compgraph_vs_oracle(tmp_path):
     assert maxrel(diag[:, 0], np.linalg.det(Fe)) < 1e-6
     psi_ref = np.array([orc.energy(Fe[k], cs.lam[k], cs.mu[k]) for k in range(len(Fe))])
     assert maxrel(diag[:, 1], psi_ref) < 1e-6
+
+
+@pytest.mark.skip(reason="disabled: its one GPU run ended in a GPU memory fault whose cause is not found yet; "
+                         "CompGraph::OptimizeDefGradControlSequence throws until then")
+@pytest.mark.gpu
+def test_cpp_mirror_optimize_reduces_loss(tmp_path):
+    """CompGraph::OptimizeDefGradControlSequence through the mirror (CompGraph.cpp:280-444:
+    five sweeps over the control layers, Adam then Lion per line-search trial):
+    its first gradient norm is the oracle's |dL/dF| of layer 0 at the initial
+    controls (1e-5, float fields), and the loss it ends with is below the start."""
+    case, st, cs, dFc, prm, target = _write_case(tmp_path)
+    r = subprocess.run([BIN, "optimize", str(tmp_path)], check=True, timeout=600, capture_output=True, text=True)
+    controls = [dFc] * (case.L - 1)
+    ref_loss, _, _, g0 = orc.episode(prm, st, cs, controls, target, case.L)
+    initial = float(r.stdout.split("Initial loss =")[1].split()[0])
+    final = float(open(tmp_path / "final_loss.txt").read())
+    hist = np.loadtxt(tmp_path / "history.txt").reshape(-1)
+    print(f"mirror optimisation: loss {initial:.6g} -> {final:.6g}, {len(hist)} gradient evaluations")
+    assert initial == pytest.approx(ref_loss, rel=1e-5)  # printed with 6 significant digits
+    assert hist[0] == pytest.approx(np.linalg.norm(g0.F), rel=1e-5)
+    assert np.all(np.isfinite(hist)) and len(hist) >= case.L - 1
+    assert final < initial
+    l0 = mpmbin.read(tmp_path / "opt_l0.mpmbin")
+    assert rel(l0["dFc"], dFc) > 0  # the layer-0 control moved
