@@ -93,7 +93,7 @@ __device__ __forceinline__ void stencil_f(const StepParams& sp, const float x[3]
 
 __device__ __forceinline__ int base_of(const StepParams& sp, float x, int a)
 {
-    return int(floorf((x - float(sp.min_point[a])) * float(sp.inv_dx))) - 1;
+    return stencil_base(floorf((x - float(sp.min_point[a])) * float(sp.inv_dx)));
 }
 
 // extend a thread's base box v (min x3, max x3) by the stencil base of position x

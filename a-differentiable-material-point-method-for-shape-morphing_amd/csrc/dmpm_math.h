@@ -151,11 +151,25 @@ struct Axis {
     T dw[4];  // N'(u)
 };
 
+// Stencil bases are clamped to +-2^20 cells before the integer conversion: a
+// particle that far out (or at inf or NaN, which the clamp maps to -2^20) has
+// every node outside any grid either way, and base + 3 then cannot overflow.
+// Unclamped, a saturated conversion (INT_MAX) plus the node offset is signed
+// overflow, and the compiler may then test the offset node against the grid
+// bounds in a form that lets the wrapped index through.
+constexpr double kBaseClamp = 1048576.0;
+
+template <typename T>
+__device__ __forceinline__ int stencil_base(T fl)
+{
+    return int(fmin(fmax(fl, T(-kBaseClamp)), T(kBaseClamp))) - 1;
+}
+
 template <typename T>
 __device__ __forceinline__ void axis_weights(T u, Axis<T>& ax, bool slopes)
 {
     T fl = floor(u);
-    ax.base = int(fl) - 1;
+    ax.base = stencil_base(fl);
     T f = u - fl;
     ax.f = f;
     T g = T(1) - f;

@@ -742,12 +742,6 @@ void CompGraph::OptimizeDefGradControlSequence(int num_steps, float _dt, float _
                                                int max_gd_iters, int max_line_search_iters, float initial_alpha,
                                                float gd_tol, float _smoothing_factor)
 {
-    // Switched off: its one GPU run (FP64 path, 4 layers, control stride 1)
-    // ended in a GPU memory fault whose cause is not found; the device port of
-    // the same loop faulted the same way.  The passes it makes run clean on
-    // their own and its call sequence checks clean against a host build of the
-    // runtime that verifies every launch's buffer extents.
-    throw DeviceError("CompGraph::OptimizeDefGradControlSequence is disabled pending a GPU fault investigation");
     SetSimulationParameters(_dt, _drag, _f_ext, _smoothing_factor);
     SetUpCompGraph(num_steps);
     ComputeForwardPass(0);

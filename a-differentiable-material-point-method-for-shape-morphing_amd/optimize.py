@@ -19,15 +19,7 @@ def optimize_control_sequence(mo, control_stride, max_gd_iters, max_line_search_
     deliberate difference: the convergence test uses the norm of the control
     layer's own gradient, where the reference reads layer 0's (stale when the
     control layer is not 0).  Returns (initial loss, final loss, gradient-norm
-    history).
-
-    Disabled: one GPU run of this loop ended in a device memory fault whose
-    cause is not found (every call it makes passes on its own, and its launch
-    extents check clean against the host runtime); it raises until the cause
-    is shown.  The C++ host mirror's CompGraph::OptimizeDefGradControlSequence
-    and ``gradient_descent`` below are the working loops."""
-    raise RuntimeError("optimize_control_sequence is disabled pending a GPU fault investigation; "
-                       "use gradient_descent or the C++ CompGraph::OptimizeDefGradControlSequence")
+    history)."""
     L = mo.num_layers
     mo.forward_pass(0)
     initial_loss = mo.end_layer_mass_loss()

@@ -350,7 +350,10 @@ static void stencil(const orc_params* prm, const double* xp, orc_stencil* st, in
 {
     for (int a = 0; a < 3; ++a) {
         double rel = xp[a] - prm->min_point[a];
-        st->base[a] = (int)floor(rel / prm->dx) - 1;
+        /* clamped to +-2^20 cells (inf and NaN included) before the int
+         * conversion: every node of such a particle is outside the grid, and
+         * base + 3 cannot overflow */
+        st->base[a] = (int)fmin(fmax(floor(rel / prm->dx), -1048576.0), 1048576.0) - 1;
         for (int o = 0; o < 4; ++o) {
             double xi = prm->min_point[a] + (double)(st->base[a] + o) * prm->dx;
             double d = xi - xp[a];

@@ -328,6 +328,62 @@ def test_particles_near_and_outside_grid_fp32():
     assert el < 1e-5 and ex < 1e-6 and eF < 2e-5 and eg < 1e-4
 
 
+@pytest.mark.parametrize("prec", [pkg.PREC_F64, pkg.PREC_F32], ids=["fp64", "fp32"])
+def test_diverged_particles_touch_no_node(prec):
+    """Particles of a diverged simulation (positions 1e12 cells out, +-inf, NaN) add to
+    no node and read none: their stencil base is clamped before the integer
+    conversion (a saturated base plus the node offset would overflow and could
+    pass the grid bounds test as a wrapped index).  The sphere's particles then
+    step exactly as without them: loss, positions and control gradient against
+    the oracle on the same particle set (which clamps the same way), with the
+    path's tolerances."""
+    rng = np.random.default_rng(13)
+    sc = pkg.scenes.morph_small()
+    bad = np.array([[1e12 * sc.dx, 0.5, 0.5], [-1e12 * sc.dx, 0.5, 0.5], [np.inf, 0.5, 0.5], [0.5, -np.inf, 0.5],
+                    [0.5, 0.5, np.nan], [np.nan, np.nan, np.nan]])
+    x = np.concatenate([sc.x, bad])
+    n, nb = len(x), len(bad)
+    good = slice(0, n - nb)
+    m = np.full(n, sc.m[0])
+    lam = np.full(n, sc.lam[0])
+    mu = np.full(n, sc.mu[0])
+    prm = orc.Params(n=n - nb, dim=sc.dim, dx=sc.dx, dt=1e-4)
+    vol = np.concatenate([orc.point_volumes(prm, sc.x, m[good]) * sc.dx ** 3, np.full(nb, (sc.dx / 2) ** 3)])
+    v = np.concatenate([0.1 * rng.normal(size=(n - nb, 3)), np.zeros((nb, 3))])
+    dFc = 1e-2 * rng.normal(size=(n, 3, 3))
+    F = np.broadcast_to(np.eye(3), (n, 3, 3)).copy()
+    if prec == pkg.PREC_F32:
+        f32 = lambda a: np.float32(a).astype(np.float64)  # noqa: E731
+        x, v, dFc, vol, m, lam, mu = f32(x), f32(v), f32(dFc), f32(vol), f32(m), f32(lam), f32(mu)
+    L = 4
+    prm = orc.Params(n=n, dim=sc.dim, dx=sc.dx, dt=1e-4)
+    target = orc.p2g_mass(prm, sc.target_x, sc.target_m)
+    st = orc.State(x, v, np.zeros((n, 3, 3)), F)
+    cs = orc.Consts(m, vol, lam, mu)
+    controls = [dFc] + [np.zeros_like(dFc)] * (L - 2)
+    mo = pkg.Morph(n, sc.dim, sc.dx, L, dt=1e-4, drag=0.0, smoothing=0.0, precision=prec, keep_control_grads=True)
+    mo.set_particles(x, v, F, st.C, m, vol, lam, mu)
+    mo.set_control(0, dFc)
+    mo.set_target_grid(target)
+    mo.forward_pass(0)
+    loss = mo.end_layer_mass_loss()
+    mo.backward_pass(0)
+    ref_loss, layers, gd, _ = orc.episode(prm, st, cs, controls, target, L)
+    xl = mo.layer_state(L - 1)[0]
+    g = mo.control_grad(0)
+    tol = (1e-12, 1e-12, 1e-10) if prec == pkg.PREC_F64 else (1e-5, 1e-6, 1e-4)
+    el, ex, eg = abs(loss - ref_loss) / abs(ref_loss), maxrel(xl[good], layers[-1].x[good]), rel(g[good], gd[0][good])
+    print(f"diverged particles prec {prec}: loss {el:.2e} x {ex:.2e} grad {eg:.2e}")
+    assert np.isfinite(loss) and el < tol[0] and ex < tol[1] and eg < tol[2]
+    # the diverged particles gather no node: a finite one stays where it is, a non-finite
+    # one stays non-finite (FP64 skips its nodes and keeps inf; the FP32 gather's
+    # fixed window multiplies its NaN weights by the window's zeros)
+    fin = np.isfinite(x[n - nb:]).all(axis=1)
+    assert np.array_equal(xl[n - nb:][fin], x[n - nb:][fin])
+    assert not np.isfinite(xl[n - nb:][~fin]).all(axis=1).any()
+    mo.close()
+
+
 def _inverted_points(seed=9):
     """The sphere scene with one particle in 25 inverted (F = Q diag(0.9, 1, -0.3) Q^T:
     det F < 0, the SVD path, J clamped to 1e-6) and one in 25 nearly flat

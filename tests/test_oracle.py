@@ -351,3 +351,18 @@ def test_episode_gradient_vs_central_fd():
     for (p, i, j) in idx:
         fd = (4.0 * cfd(p, i, j, h / 2) - cfd(p, i, j, h)) / 3.0  # Richardson: O(h^4)
         assert gdFc[0][p, i, j] == pytest.approx(fd, rel=1e-6, abs=1e-6 * np.abs(gdFc[0]).max())
+
+
+def test_diverged_particles_reach_no_node():
+    """A particle 1e12 cells out, at +-inf or NaN adds nothing to the P2G mass: the
+    stencil base is clamped to +-2^20 cells before its integer conversion, so the
+    node offsets cannot overflow (brute force over the grid: the mass equals that
+    of the finite particles alone)."""
+    prm = orc.Params(n=3, dim=(8, 8, 8), dx=0.125)
+    xs = np.array([[0.5, 0.5, 0.5], [0.3, 0.6, 0.2], [0.9, 0.1, 0.4]])
+    ms = np.array([1.0, 2.0, 0.5])
+    ref = orc.p2g_mass(prm, xs, ms)
+    bad = np.array([[1e12, 0.5, 0.5], [-1e12, 0.5, 0.5], [np.inf, 0.5, 0.5], [0.5, -np.inf, 0.5], [0.5, 0.5, np.nan]])
+    prm2 = orc.Params(n=len(xs) + len(bad), dim=(8, 8, 8), dx=0.125)
+    got = orc.p2g_mass(prm2, np.concatenate([xs, bad]), np.concatenate([ms, np.ones(len(bad))]))
+    assert np.array_equal(got, ref)
