@@ -584,6 +584,19 @@ void P2G_Mass(const PointCloud& pc, Grid& grid)
 
 }  // namespace SingleThreadMPM
 
+namespace MultiThreadMPM {
+
+// MultiThreadForwardSimulation.cpp:7-225: P_op_1, P2G, G_op, G2P and P_op_2
+// without smoothing; next may be curr (the device keeps both layers)
+void ForwardTimeStep(PointCloud& next, PointCloud& curr, Grid& main_grid, std::vector<std::shared_ptr<Grid>>& proxy_grids,
+                     float dt, float drag, Vec3 f_ext)
+{
+    if (proxy_grids.empty()) throw std::invalid_argument("MultiThreadMPM::ForwardTimeStep: proxy_grids is empty");
+    SingleThreadMPM::ForwardTimeStep(next, curr, main_grid, 0.f, dt, drag, f_ext);
+}
+
+}  // namespace MultiThreadMPM
+
 // ------------------------------------------------------- BackPropagation ----
 // BackPropagation.cpp:6-125: gradients of layer n from those of layer n+1.
 // The step n -> n+1 is recomputed on the GPU from layer n (its grid is the

@@ -307,6 +307,18 @@ void CalculatePointCloudVolumes(PointCloud& curr_point_cloud, Grid& grid);
 void P2G_Mass(const PointCloud& point_cloud, Grid& grid);
 }  // namespace SingleThreadMPM
 
+// ----------------------------------------- MultiThreadForwardSimulation ----
+// MultiThreadForwardSimulation.h:6-11: the step the animator replays in place,
+// next and curr the same cloud (DiffMPMAnimator3D.cpp:83-111): no smoothing.
+// In the reference proxy_grids holds one grid per std::thread for a race-free
+// P2G and its size sets the thread count; the GPU scatter needs no copies, so
+// the proxy grids are not written here.  An empty vector (a division by zero
+// in the reference) raises std::invalid_argument.
+namespace MultiThreadMPM {
+void ForwardTimeStep(PointCloud& next_point_cloud, PointCloud& curr_point_cloud, Grid& main_grid,
+                     std::vector<std::shared_ptr<Grid>>& proxy_grids, float dt, float drag, Vec3 f_ext);
+}  // namespace MultiThreadMPM
+
 // ------------------------------------------------------- BackPropagation ----
 // BackPropagation.h:7
 void Back_Timestep(CompGraphLayer& layer_nplus1, CompGraphLayer& layer_n, float drag, float dt, float smoothing_factor);

@@ -380,6 +380,51 @@ static long grid_size(const orc_params* prm)
 /* ------------------------------------------------------------------ */
 /* P2G: ForwardSimulation.cpp:11-51 (after G_Reset, Grid.cpp:349-361)   */
 /* ------------------------------------------------------------------ */
+/* One particle's scatter (SingleParticle_to_grid, ForwardSimulation.cpp:
+ * 11-51), into a shared grid with OpenMP atomics (shared = 1) or into a grid
+ * only this thread writes (shared = 0: the proxy grids of
+ * MultiThreadForwardSimulation.cpp:171-181). */
+static void p2g_particle(const orc_params* prm, int p, const double* x, const double* v, const double* C,
+                         const double* F, const double* dFc, const double* m, const double* vol,
+                         const double* lam, const double* mu, double dt, double drag,
+                         double* grid_m, double* grid_p, int shared)
+{
+    double Dinv = 3.0 / (prm->dx * prm->dx);
+    double Fc[9], P[9], PF[9], A[9];
+    for (int i = 0; i < 9; ++i) Fc[i] = F[9 * p + i] + dFc[9 * p + i];
+    orc_pk1(Fc, lam[p], mu[p], P);
+    m3_mult(P, Fc, PF); /* P Fc^T */
+    for (int i = 0; i < 9; ++i) A[i] = -Dinv * dt * vol[p] * PF[i] + m[p] * C[9 * p + i];
+    double mv[3];
+    for (int a = 0; a < 3; ++a) mv[a] = m[p] * v[3 * p + a] * (1.0 - dt * drag);
+    orc_stencil st;
+    stencil(prm, x + 3 * p, &st, 0);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            for (int k = 0; k < 4; ++k) {
+                long id = node_index(prm, &st, i, j, k);
+                if (id < 0) continue;
+                double w = st.N[0][i] * st.N[1][j] * st.N[2][k];
+                double d[3] = {st.d[0][i], st.d[1][j], st.d[2][k]};
+                double dm = w * m[p];
+                if (shared) {
+#pragma omp atomic
+                    grid_m[id] += dm;
+                } else {
+                    grid_m[id] += dm;
+                }
+                for (int a = 0; a < 3; ++a) {
+                    double dp = w * (mv[a] + A[3 * a] * d[0] + A[3 * a + 1] * d[1] + A[3 * a + 2] * d[2]);
+                    if (shared) {
+#pragma omp atomic
+                        grid_p[3 * id + a] += dp;
+                    } else {
+                        grid_p[3 * id + a] += dp;
+                    }
+                }
+            }
+}
+
 void orc_p2g(const orc_params* prm, const double* x, const double* v, const double* C,
              const double* F, const double* dFc, const double* m, const double* vol,
              const double* lam, const double* mu, double dt, double drag,
@@ -388,35 +433,9 @@ void orc_p2g(const orc_params* prm, const double* x, const double* v, const doub
     long G = grid_size(prm);
     memset(grid_m, 0, sizeof(double) * G);
     memset(grid_p, 0, sizeof(double) * 3 * G);
-    double Dinv = 3.0 / (prm->dx * prm->dx);
 #pragma omp parallel for schedule(static)
-    for (int p = 0; p < prm->n; ++p) {
-        double Fc[9], P[9], PF[9], A[9];
-        for (int i = 0; i < 9; ++i) Fc[i] = F[9 * p + i] + dFc[9 * p + i];
-        orc_pk1(Fc, lam[p], mu[p], P);
-        m3_mult(P, Fc, PF); /* P Fc^T */
-        for (int i = 0; i < 9; ++i) A[i] = -Dinv * dt * vol[p] * PF[i] + m[p] * C[9 * p + i];
-        double mv[3];
-        for (int a = 0; a < 3; ++a) mv[a] = m[p] * v[3 * p + a] * (1.0 - dt * drag);
-        orc_stencil st;
-        stencil(prm, x + 3 * p, &st, 0);
-        for (int i = 0; i < 4; ++i)
-            for (int j = 0; j < 4; ++j)
-                for (int k = 0; k < 4; ++k) {
-                    long id = node_index(prm, &st, i, j, k);
-                    if (id < 0) continue;
-                    double w = st.N[0][i] * st.N[1][j] * st.N[2][k];
-                    double d[3] = {st.d[0][i], st.d[1][j], st.d[2][k]};
-                    double dm = w * m[p];
-#pragma omp atomic
-                    grid_m[id] += dm;
-                    for (int a = 0; a < 3; ++a) {
-                        double dp = w * (mv[a] + A[3 * a] * d[0] + A[3 * a + 1] * d[1] + A[3 * a + 2] * d[2]);
-#pragma omp atomic
-                        grid_p[3 * id + a] += dp;
-                    }
-                }
-    }
+    for (int p = 0; p < prm->n; ++p)
+        p2g_particle(prm, p, x, v, C, F, dFc, m, vol, lam, mu, dt, drag, grid_m, grid_p, 1);
 }
 
 void orc_p2g_mass(const orc_params* prm, const double* x, const double* m, double* grid_m)
@@ -442,28 +461,24 @@ void orc_p2g_mass(const orc_params* prm, const double* x, const double* m, doubl
 /* ------------------------------------------------------------------ */
 /* ForwardTimeStep: ForwardSimulation.cpp:136-157                       */
 /* ------------------------------------------------------------------ */
-void orc_forward_step(const orc_params* prm,
-                      const double* x, const double* v, const double* C, const double* F,
-                      const double* dFc, const double* m, const double* vol,
-                      const double* lam, const double* mu,
-                      double* grid_m, double* grid_v,
-                      double* x1, double* v1, double* C1, double* F1)
+/* G_op on one node: ForwardSimulation.cpp:53-58,184-200 (v stays 0 from
+ * the reset); grid_v holds the momentum on entry. */
+static void grid_op_node(const orc_params* prm, long i, const double* grid_m, double* grid_v)
 {
-    long G = grid_size(prm);
-    /* P_op_1 + G_Reset + P2G; grid_v holds the momentum until G_op */
-    orc_p2g(prm, x, v, C, F, dFc, m, vol, lam, mu, prm->dt, prm->drag, grid_m, grid_v);
-    /* G_op: ForwardSimulation.cpp:53-58,184-200 (v stays 0 from the reset) */
-#pragma omp parallel for schedule(static)
-    for (long i = 0; i < G; ++i) {
-        if (fabs(grid_m[i]) > ORC_MASS_EPS) {
-            for (int a = 0; a < 3; ++a) grid_v[3 * i + a] = grid_v[3 * i + a] / grid_m[i] + prm->dt * prm->f_ext[a];
-        } else {
-            for (int a = 0; a < 3; ++a) grid_v[3 * i + a] = 0.0;
-        }
+    if (fabs(grid_m[i]) > ORC_MASS_EPS) {
+        for (int a = 0; a < 3; ++a) grid_v[3 * i + a] = grid_v[3 * i + a] / grid_m[i] + prm->dt * prm->f_ext[a];
+    } else {
+        for (int a = 0; a < 3; ++a) grid_v[3 * i + a] = 0.0;
     }
-    /* G2P + P_op_2: ForwardSimulation.cpp:60-106,114-124 */
+}
+
+/* G2P + P_op_2: ForwardSimulation.cpp:60-106,108-124 (s = 0 is the
+ * overload without smoothing, :108-112) */
+static void g2p_update(const orc_params* prm, double s, const double* x, const double* F, const double* dFc,
+                       const double* grid_v, double* x1, double* v1, double* C1, double* F1)
+{
     double Dinv = 3.0 / (prm->dx * prm->dx);
-    double s = prm->smoothing, dt = prm->dt;
+    double dt = prm->dt;
 #pragma omp parallel for schedule(static)
     for (int p = 0; p < prm->n; ++p) {
         orc_stencil st;
@@ -497,6 +512,66 @@ void orc_forward_step(const orc_params* prm,
             x1[3 * p + a] = x[3 * p + a] + dt * vp[a];
         }
     }
+}
+
+void orc_forward_step(const orc_params* prm,
+                      const double* x, const double* v, const double* C, const double* F,
+                      const double* dFc, const double* m, const double* vol,
+                      const double* lam, const double* mu,
+                      double* grid_m, double* grid_v,
+                      double* x1, double* v1, double* C1, double* F1)
+{
+    long G = grid_size(prm);
+    /* P_op_1 + G_Reset + P2G; grid_v holds the momentum until G_op */
+    orc_p2g(prm, x, v, C, F, dFc, m, vol, lam, mu, prm->dt, prm->drag, grid_m, grid_v);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < G; ++i) grid_op_node(prm, i, grid_m, grid_v);
+    g2p_update(prm, prm->smoothing, x, F, dFc, grid_v, x1, v1, C1, F1);
+}
+
+/* ------------------------------------------------------------------ */
+/* MultiThreadMPM::ForwardTimeStep: MultiThreadForwardSimulation.cpp:7-225 */
+/* ------------------------------------------------------------------ */
+/* The particles are cut into `sections` contiguous runs, the first taking
+ * the remainder (:21-33); each run scatters into a proxy grid of its own
+ * with plain additions (:171-181), then every node sums the proxy grids in
+ * section order and applies G_op (:183-207); G2P and P_op_2 without
+ * smoothing follow (:210-225).  With a fixed section count the result does
+ * not depend on the thread count or schedule. */
+int orc_forward_step_proxy(const orc_params* prm, int sections,
+                           const double* x, const double* v, const double* C, const double* F,
+                           const double* dFc, const double* m, const double* vol,
+                           const double* lam, const double* mu,
+                           double* grid_m, double* grid_v,
+                           double* x1, double* v1, double* C1, double* F1)
+{
+    if (sections < 1) return -1;
+    long G = grid_size(prm);
+    double* proxy = (double*)calloc((size_t)sections * 4 * (size_t)G, sizeof(double));
+    if (!proxy) return -2;
+    int per = prm->n / sections, first = per + prm->n % sections;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int t = 0; t < sections; ++t) {
+        int begin = t == 0 ? 0 : first + (t - 1) * per, count = t == 0 ? first : per;
+        double* pm = proxy + (size_t)t * 4 * G;
+        for (int p = begin; p < begin + count; ++p)
+            p2g_particle(prm, p, x, v, C, F, dFc, m, vol, lam, mu, prm->dt, prm->drag, pm, pm + G, 0);
+    }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < G; ++i) {
+        double gm = 0.0, gp[3] = {0.0, 0.0, 0.0};
+        for (int t = 0; t < sections; ++t) {
+            const double* pm = proxy + (size_t)t * 4 * G;
+            gm += pm[i];
+            for (int a = 0; a < 3; ++a) gp[a] += pm[G + 3 * i + a];
+        }
+        grid_m[i] = gm;
+        for (int a = 0; a < 3; ++a) grid_v[3 * i + a] = gp[a];
+        grid_op_node(prm, i, grid_m, grid_v);
+    }
+    free(proxy);
+    g2p_update(prm, 0.0, x, F, dFc, grid_v, x1, v1, C1, F1);
+    return 0;
 }
 
 

@@ -82,6 +82,17 @@ void orc_forward_step(const orc_params* prm,
                       double* grid_m, double* grid_v,
                       double* x1, double* v1, double* C1, double* F1);
 
+/* MultiThreadMPM::ForwardTimeStep (MultiThreadForwardSimulation.cpp:7-225):
+ * the proxy-grid forward with `sections` particle runs, no smoothing.
+ * Returns 0, or -1 for sections < 1, -2 when the proxy grids cannot be
+ * allocated. */
+int orc_forward_step_proxy(const orc_params* prm, int sections,
+                           const double* x, const double* v, const double* C, const double* F,
+                           const double* dFc, const double* m, const double* vol,
+                           const double* lam, const double* mu,
+                           double* grid_m, double* grid_v,
+                           double* x1, double* v1, double* C1, double* F1);
+
 /* ForwardSimulation.cpp:262-290 (vol = m / sum_i w m_i, no dx^3). */
 void orc_point_volumes(const orc_params* prm, const double* x, const double* m, double* vol);
 

@@ -77,6 +77,8 @@ def lib():
         L.orc_p2g.argtypes = [pp] + [dp] * 9 + [ctypes.c_double, ctypes.c_double, dp, dp]
         L.orc_p2g_mass.argtypes = [pp, dp, dp, dp]
         L.orc_forward_step.argtypes = [pp] + [dp] * 9 + [dp, dp] + [dp] * 4
+        L.orc_forward_step_proxy.argtypes = [pp, ctypes.c_int] + [dp] * 9 + [dp, dp] + [dp] * 4
+        L.orc_forward_step_proxy.restype = ctypes.c_int
         L.orc_point_volumes.argtypes = [pp, dp, dp, dp]
         L.orc_end_mass_loss.restype = ctypes.c_double
         L.orc_end_mass_loss.argtypes = [pp, dp, dp, dp, dp, dp, dp]
@@ -275,6 +277,25 @@ def forward_step(prm: Params, st: State, cs: Consts, dFc):
     lib().orc_forward_step(ctypes.byref(P), _p(_c(st.x)), _p(_c(st.v)), _p(_c(st.C)), _p(_c(st.F)),
                            _p(_c(dFc)), _p(_c(cs.m)), _p(_c(cs.vol)), _p(_c(cs.lam)), _p(_c(cs.mu)),
                            _p(gm), _p(gv), _p(nxt.x), _p(nxt.v), _p(nxt.C), _p(nxt.F))
+    return nxt, gm, gv
+
+
+def forward_step_proxy(prm: Params, st: State, cs: Consts, dFc, sections: int):
+    """MultiThreadMPM::ForwardTimeStep (MultiThreadForwardSimulation.cpp:7-225): proxy
+    grids per particle section, no smoothing. Returns (next, grid_m, grid_v)."""
+    _n(prm, st.x, st.v, st.C, st.F, dFc, cs.m, cs.vol, cs.lam, cs.mu)
+    n = prm.n
+    nxt = State.zeros(n)
+    gm = np.zeros(prm.G)
+    gv = np.zeros((prm.G, 3))
+    P = prm.c()
+    rc = lib().orc_forward_step_proxy(ctypes.byref(P), int(sections), _p(_c(st.x)), _p(_c(st.v)), _p(_c(st.C)),
+                                      _p(_c(st.F)), _p(_c(dFc)), _p(_c(cs.m)), _p(_c(cs.vol)), _p(_c(cs.lam)),
+                                      _p(_c(cs.mu)), _p(gm), _p(gv), _p(nxt.x), _p(nxt.v), _p(nxt.C), _p(nxt.F))
+    if rc == -1:
+        raise ValueError("sections must be >= 1")
+    if rc != 0:
+        raise MemoryError("proxy grids could not be allocated")
     return nxt, gm, gv
 
 

@@ -9,6 +9,8 @@
 // outputs: loss.txt, last.mpmbin (last layer), l0.mpmbin (layer 0 with its
 // gradients after ComputeBackwardPass(0)), bt.mpmbin (layer 0 gradients from
 // Back_Timestep on layer 1's gradients), step.mpmbin (ForwardTimeStep of layer 0),
+// replay.mpmbin, replay_grid.txt (MultiThreadMPM::ForwardTimeStep of layer 0 in
+// place: the cloud, then each node's m and v, one a line),
 // pk1.txt (PK_FixedCorotatedElasticity, then FixedCorotatedElasticity, at
 // F = 0.8 Rx(45 deg)), diag.txt (GetPointDeterminants, GetPointElasticEnergies
 // of layer 0, one point a line); optimize: history.txt (gradient norms of the
@@ -110,6 +112,31 @@ int main(int argc, char** argv)
         Grid g2(*grid);
         SingleThreadMPM::ForwardTimeStep(next, *pc, g2, s, dt, drag, fe);
         next.WriteEntirePointCloudToBinaryFile(dir + "/step.mpmbin");
+
+        // the animator's in-place replay step with 4 proxy grids (DiffMPMAnimator3D.cpp:83-111)
+        PointCloud replay = *pc;
+        Grid g3(*grid);
+        std::vector<std::shared_ptr<Grid>> proxies(4);
+        for (auto& q : proxies) q = std::make_shared<Grid>(*grid);
+        MultiThreadMPM::ForwardTimeStep(replay, replay, g3, proxies, dt, drag, fe);
+        replay.WriteEntirePointCloudToBinaryFile(dir + "/replay.mpmbin");
+        {
+            std::ofstream gm(dir + "/replay_grid.txt");
+            gm.precision(9);
+            for (int i = 0; i < g3.dim_x; ++i)
+                for (int j = 0; j < g3.dim_y; ++j)
+                    for (int k = 0; k < g3.dim_z; ++k) {
+                        const GridNode& nd = g3.nodes[i][j][k];
+                        gm << nd.m << " " << nd.v[0] << " " << nd.v[1] << " " << nd.v[2] << "\n";
+                    }
+        }
+        std::vector<std::shared_ptr<Grid>> none;
+        try {
+            MultiThreadMPM::ForwardTimeStep(replay, replay, g3, none, dt, drag, fe);
+            std::cerr << "error: empty proxy_grids accepted\n";
+            return 4;
+        } catch (const std::invalid_argument&) {
+        }
 
         // Elasticity at the reference's gradient-check point (DiffMPMAnimator3D.cpp:219-225)
         Mat3 F;

@@ -1,7 +1,8 @@
 """The C++ host mirror of the reference API (include/DiffMPMLib3D/DiffMPMLib3D.h).
 
 CPU: the .mpmbin format round trip through the C++ reader/writer.
-GPU: CompGraph passes, EndLayerMassLoss, Back_Timestep, ForwardTimeStep, PK1,
+GPU: CompGraph passes, EndLayerMassLoss, Back_Timestep, ForwardTimeStep (and
+MultiThreadMPM's in place, against the oracle's proxy-grid step), PK1,
 the elastic energy and the per-point J / energy getters through the mirror,
 against the FP64 oracle in reference-compat mode (the mirror's default).
 Tolerances: the mirror stores float fields like the reference, so states
@@ -84,9 +85,13 @@ def _write_case(tmp_path):
 
 @pytest.mark.gpu
 def test_cpp_mirror_compgraph_vs_oracle(tmp_path):
-    case, st, cs, dFc, prm, target = _write_case(tmp_path)
+    inputs = _write_case(tmp_path)
     subprocess.run([BIN, "run", str(tmp_path)], check=True, timeout=300)
+    check_run_outputs(tmp_path, *inputs)
 
+
+def check_run_outputs(tmp_path, case, st, cs, dFc, prm, target):
+    """The selftest's ``run`` outputs against the oracle."""
     controls = [dFc] * (case.L - 1)  # SetUpCompGraph copies layer 0 (and its dFc) into every layer
     ref_loss, layers, gd, g0 = orc.episode(prm, st, cs, controls, target, case.L)
     loss = float(open(tmp_path / "loss.txt").read())
@@ -116,6 +121,15 @@ def test_cpp_mirror_compgraph_vs_oracle(tmp_path):
     assert maxrel(diag[:, 0], np.linalg.det(Fe)) < 1e-6
     psi_ref = np.array([orc.energy(Fe[k], cs.lam[k], cs.mu[k]) for k in range(len(Fe))])
     assert maxrel(diag[:, 1], psi_ref) < 1e-6
+    # MultiThreadMPM::ForwardTimeStep in place with 4 proxy grids: the oracle's
+    # proxy-grid step with 4 sections (no smoothing), cloud and main grid
+    rp, rgm, rgv = orc.forward_step_proxy(prm, st, cs, dFc, 4)
+    replay = mpmbin.read(tmp_path / "replay.mpmbin")
+    for f in ("x", "v", "C", "F"):
+        assert maxrel(replay[f], getattr(rp, f)) < 1e-6, f
+    assert np.array_equal(replay["dFc"], dFc)  # the control is not touched
+    g = np.loadtxt(tmp_path / "replay_grid.txt").reshape(-1, 4)
+    assert maxrel(g[:, 0], rgm) < 1e-6 and maxrel(g[:, 1:], rgv) < 1e-5
 
 
 @pytest.mark.gpu

@@ -9,8 +9,10 @@ sanitizer runtimes preloaded:
   host build is the FP64 path's arithmetic run serially);
 * diverged particles (1e12 cells out, +-inf, NaN) beside the sphere, whose stencil
   bases must be clamped before their integer conversion;
-* the C++ mirror's ``OptimizeDefGradControlSequence`` on the same case its GPU
-  run once faulted on (its trial episodes diverge).
+* the C++ mirror's selftest: every mirror call of its ``run`` mode (passes,
+  loss, Back_Timestep, both ForwardTimeStep forms, getters), then
+  ``OptimizeDefGradControlSequence`` on the same case its GPU run once faulted
+  on (its trial episodes diverge).
 """
 import os
 import shutil
@@ -49,7 +51,7 @@ def _build():
         subprocess.run([gxx, *FLAGS, "-fPIC", "-shared", "-I", CPU, "-I", CSRC, "-I", INCLUDE, "-x", "c++", rt, "-x", "none",
                         os.path.join(CPU, "host_kernels.cpp"), "-o", LIB], check=True, timeout=600)
     host = os.path.join(ROOT, "a-differentiable-material-point-method-for-shape-morphing_amd", "host", "dmpm_host.cpp")
-    if _stale(CPPLIB, [host, LIB]):
+    if _stale(CPPLIB, [host, LIB, os.path.join(INCLUDE, "DiffMPMLib3D", "DiffMPMLib3D.h")]):
         subprocess.run([gxx, *FLAGS, "-fPIC", "-shared", "-I", INCLUDE, host, "-L", BUILD, "-ldmpm_host_san",
                         f"-Wl,-rpath,{BUILD}", "-o", CPPLIB], check=True, timeout=600)
     st = os.path.join(ROOT, "tests", "cpp", "compgraph_selftest.cpp")
@@ -102,10 +104,13 @@ def test_fp64_kernels_under_sanitizers(tmp_path):
     r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=900)
     assert r.returncode == 0 and "runtime error" not in r.stderr, r.stdout[-2000:] + r.stderr[-4000:]
     assert "host build ok" in r.stdout
-    # the mirror's optimisation loop on the case its GPU run once faulted on
+    # the mirror's calls, then its optimisation loop on the case its GPU run once faulted on
     import test_cpp_mirror
-    test_cpp_mirror._write_case(tmp_path)
-    r = subprocess.run([SELFTEST, "optimize", str(tmp_path)], env={**env, "LD_PRELOAD": ""}, capture_output=True, text=True,
-                       timeout=900)
-    assert r.returncode == 0 and "runtime error" not in r.stderr, r.stdout[-2000:] + r.stderr[-4000:]
+    inputs = test_cpp_mirror._write_case(tmp_path)
+    for mode in ("run", "optimize"):
+        r = subprocess.run([SELFTEST, mode, str(tmp_path)], env={**env, "LD_PRELOAD": ""}, capture_output=True, text=True,
+                           timeout=900)
+        assert r.returncode == 0 and "runtime error" not in r.stderr, mode + r.stdout[-2000:] + r.stderr[-4000:]
+        if mode == "run":  # the same checks as the GPU test of the mirror
+            test_cpp_mirror.check_run_outputs(tmp_path, *inputs)
     assert "Final loss" in r.stdout

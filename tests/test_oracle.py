@@ -8,6 +8,7 @@ closed forms, library routines (numpy/scipy SVD and polar), invariants,
 brute-force enumeration and central finite differences.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -366,3 +367,48 @@ def test_diverged_particles_reach_no_node():
     prm2 = orc.Params(n=len(xs) + len(bad), dim=(8, 8, 8), dx=0.125)
     got = orc.p2g_mass(prm2, np.concatenate([xs, bad]), np.concatenate([ms, np.ones(len(bad))]))
     assert np.array_equal(got, ref)
+
+
+# ------------------------------------------------------- proxy-grid forward --
+def test_proxy_forward_one_section_is_the_serial_step():
+    """One section and one thread: the proxy-grid forward adds every particle into
+    one grid in particle order, as the atomic P2G does serially, so with smoothing
+    0 both give the same bits (pins the shared scatter, grid update and gather)."""
+    prm, st, cs, dFc = tiny_scene(n=50, seed=3)
+    prm.smoothing = 0.0
+    orc.set_threads(1)
+    try:
+        a, gma, gva = orc.forward_step(prm, st, cs, dFc)
+        b, gmb, gvb = orc.forward_step_proxy(prm, st, cs, dFc, 1)
+    finally:
+        orc.set_threads(os.cpu_count() or 1)
+    for f in ("x", "v", "C", "F"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    assert np.array_equal(gma, gmb) and np.array_equal(gva, gvb)
+
+
+@pytest.mark.parametrize("sections", [2, 7, 50, 64])
+def test_proxy_forward_sections(sections):
+    """Ragged sections (50 particles into 7: 8 + 6 x 7; 64 sections leave 14 empty
+    runs after a first of 50): the step equals the atomic one to summation order
+    (1e-12), ignores the smoothing factor (MultiThreadForwardSimulation.cpp:220:
+    the overload without it), and repeats bit for bit with a fixed section count
+    whatever the thread schedule."""
+    prm, st, cs, dFc = tiny_scene(n=50, seed=4)
+    ref_prm = orc.Params(**{**prm.__dict__, "smoothing": 0.0})
+    a, gma, gva = orc.forward_step(ref_prm, st, cs, dFc)
+    b, gmb, gvb = orc.forward_step_proxy(prm, st, cs, dFc, sections)
+    c, gmc, gvc = orc.forward_step_proxy(prm, st, cs, dFc, sections)
+    for f in ("x", "v", "C", "F"):
+        assert np.allclose(getattr(b, f), getattr(a, f), rtol=1e-12, atol=1e-14), f
+        assert np.array_equal(getattr(b, f), getattr(c, f)), f
+    assert np.allclose(gmb, gma, rtol=1e-12, atol=1e-18) and np.allclose(gvb, gva, rtol=1e-10, atol=1e-12)
+    assert np.array_equal(gmb, gmc) and np.array_equal(gvb, gvc)
+    smoothed, _, _ = orc.forward_step(prm, st, cs, dFc)
+    assert not np.allclose(smoothed.F, b.F, rtol=1e-6)
+
+
+def test_proxy_forward_rejects_no_sections():
+    prm, st, cs, dFc = tiny_scene(n=5)
+    with pytest.raises(ValueError):
+        orc.forward_step_proxy(prm, st, cs, dFc, 0)
