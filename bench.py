@@ -12,10 +12,11 @@ Timed paths: FP32 state with FP64 polar/SVD/H:dF and checkpoint+recompute
 (the reported value: autotuned range size / LDS capacity, passes replayed
 from HIP graphs), the FP32 path storing every layer (memory and speed of the
 two tapes), and the plain FP64 path with every layer stored; the
-control-gradient error is FP32 vs FP64 on the same inputs.  Per-kernel
-durations come from HIP events around every launch of eager episodes run
-right after the timed region (graph replays cannot be bracketed per kernel;
-the kernels are the same).  A batched run advances 24 uneven morphs
+control-gradient error is FP32 vs FP64 on the same inputs.  Both paths report
+their whole-step roofline fraction (the same algorithmic bytes, twice as wide
+in FP64).  Per-kernel durations come from HIP events around every launch of
+eager episodes run right after the timed region (one for the FP64 path;
+graph replays cannot be bracketed per kernel, the kernels are the same).  A batched run advances 24 uneven morphs
 (5k-300k particles, 32^3-96^3 grids, 50-step episodes) together.  The CPU
 baseline is the FP64 oracle (a port) on a bounded sample of the same scene.
 
@@ -308,7 +309,8 @@ def main():
 
     r64 = None
     if not args.no_fp64:
-        r64 = run_path(pkg, sc, vol, dFc, pkg.PREC_F64, pkg.TAPE_STORE_ALL, L, 1, args.fp64_steps)
+        r64 = run_path(pkg, sc, vol, dFc, pkg.PREC_F64, pkg.TAPE_STORE_ALL, L, 1, args.fp64_steps,
+                       prof_steps=1 if args.prof_steps else 0)
         v64 = sc.n * args.episode_steps * args.fp64_steps / r64["time"]
         print(f"# fp64 plain: {r64['time'] / args.fp64_steps * 1e3:.2f} ms/episode, {v64:.4e} particle-steps/s", file=sys.stderr, flush=True)
 
@@ -392,8 +394,17 @@ def main():
     if r64 is not None:
         v64 = sc.n * args.episode_steps * args.fp64_steps / r64["time"]
         g32, g64 = r32["grad"], r64["grad"]
+        # the same algorithmic bytes as the FP32 step, every field twice as wide
+        step64 = 2 * (STEP_BYTES[0] * sc.n + STEP_BYTES[1] * touched + STEP_BYTES[2] * sc.G)
+        gbs64 = step64 * args.episode_steps * args.fp64_steps / r64["time"] / 1e9
         out["fp64_plain"] = {"value": v64, "ms_per_step": r64["time"] / args.fp64_steps * 1e3, "memory_bytes": r64["mem"],
-                             "loss": r64["loss"]}
+                             "loss": r64["loss"],
+                             "step_roofline": {"bytes_per_particle_step": step64 / sc.n, "achieved_GBs": gbs64,
+                                               "frac": gbs64 / HBM_PEAK_GBS}}
+        if r64["prof"] is not None:
+            out["fp64_plain"]["kernels"] = {k: {"ms_total": round(v[0], 3), "launches": v[1],
+                                                "avg_us": round(v[0] * 1e3 / max(v[1], 1), 2)}
+                                            for k, v in r64["prof"].items() if v[1]}
         out["speedup_fp32_vs_fp64"] = value / v64
         out["grad_rel_err_fp32_vs_fp64"] = float(np.linalg.norm(g32 - g64) / max(np.linalg.norm(g64), 1e-300))
         out["loss_fp32"] = r32["loss"]
