@@ -1,5 +1,6 @@
 """C-ABI library checks that need no GPU: it loads and exports every symbol
-declared in include/dmpm.h; the Python binding covers all of them."""
+declared in include/dmpm.h; the Python binding covers all of them; the C++
+drop-in headers compile and link with the reference's signatures."""
 import ctypes
 import os
 import re
@@ -47,3 +48,35 @@ def test_create_without_gpu_fails_cleanly():
         assert "dmpm error" in str(e)
     else:
         raise AssertionError("context created without a GPU")
+
+
+REFERENCE_HEADERS = ["BackPropagation.h", "CompGraph.h", "Elasticity.h", "ForwardSimulation.h", "Grid.h", "GridNode.h",
+                     "MaterialPoint.h", "MultiThreadForwardSimulation.h", "PointCloud.h", "pch.h"]
+
+
+def test_dropin_headers_compile_and_link(tmp_path):
+    """Every header name of the reference's DiffMPMLib3D that callers include compiles, and the
+    free functions of the step keep the reference's signatures (taken as function pointers of
+    those exact types) and resolve in libdmpm_cpp.so."""
+    import subprocess
+
+    inc = os.path.join(ROOT, "include")
+    lib = os.path.join(PKG_DIR, "lib")
+    src = tmp_path / "t.cpp"
+    src.write_text("".join(f'#include "{h}"\n' for h in REFERENCE_HEADERS) + """
+using namespace DiffMPMLib3D;
+int main()
+{
+    void (*mt)(PointCloud&, PointCloud&, Grid&, std::vector<std::shared_ptr<Grid>>&, float, float, Vec3) =
+        &MultiThreadMPM::ForwardTimeStep;
+    void (*st)(PointCloud&, PointCloud&, Grid&, float, float, float, Vec3) = &SingleThreadMPM::ForwardTimeStep;
+    void (*st0)(PointCloud&, PointCloud&, Grid&, float, float, Vec3) = &SingleThreadMPM::ForwardTimeStep;
+    void (*bt)(CompGraphLayer&, CompGraphLayer&, float, float, float) = &Back_Timestep;
+    Mat3 (*pk)(const Mat3&, float, float) = &PK_FixedCorotatedElasticity;
+    return (mt && st && st0 && bt && pk) ? 0 : 1;
+}
+""")
+    exe = tmp_path / "t"
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-I", os.path.join(inc, "DiffMPMLib3D"), "-I", inc, str(src), "-L", lib,
+                    "-ldmpm_cpp", "-ldmpm", f"-Wl,-rpath,{lib}", "-o", str(exe)], check=True, timeout=300)
+    subprocess.run([str(exe)], check=True, timeout=60)
