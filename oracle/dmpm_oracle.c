@@ -12,7 +12,9 @@
  * Scatter phases (P2G and the transposed G2P) are parallel over particles
  * with OpenMP atomics, like ForwardSimulation.cpp:45-46; every other phase is
  * a per-particle or per-node gather.  With orc_set_threads(1) the oracle is
- * deterministic.
+ * deterministic.  orc_forward_step_proxy restates the proxy-grid forward of
+ * MultiThreadForwardSimulation.cpp instead: one grid per particle section,
+ * reduced in section order, deterministic for a fixed section count.
  */
 #include "dmpm_oracle.h"
 
