@@ -389,8 +389,8 @@ def test_proxy_forward_one_section_is_the_serial_step():
 
 @pytest.mark.parametrize("sections", [2, 7, 50, 64])
 def test_proxy_forward_sections(sections):
-    """Ragged sections (50 particles into 7: 8 + 6 x 7; 64 sections leave 14 empty
-    runs after a first of 50): the step equals the atomic one to summation order
+    """Ragged sections (50 particles into 7: 8 + 6 x 7; into 64: the first takes all
+    50, as the reference's split gives it the remainder, and 63 are empty): the step equals the atomic one to summation order
     (1e-12), ignores the smoothing factor (MultiThreadForwardSimulation.cpp:220:
     the overload without it), and repeats bit for bit with a fixed section count
     whatever the thread schedule."""
