@@ -241,6 +241,36 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": nbytes}))
 
 
+def copy_bandwidth(nbytes=1 << 28, reps=5):
+    """Achievable HBM bandwidth on this device: the HIP runtime's device-to-device
+    copy of ``nbytes`` (read + write) timed with HIP events on the null stream,
+    best of ``reps`` after one untimed copy."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    n = ctypes.c_size_t(nbytes)
+    assert hip.hipMalloc(ctypes.byref(a), n) == 0 and hip.hipMalloc(ctypes.byref(b), n) == 0
+    assert hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0
+    assert hip.hipMemset(a, 1, n) == 0
+    best = None
+    for r in range(reps + 1):
+        assert hip.hipEventRecord(e0, None) == 0
+        assert hip.hipMemcpyAsync(b, a, n, 3, None) == 0  # hipMemcpyDeviceToDevice
+        assert hip.hipEventRecord(e1, None) == 0
+        assert hip.hipEventSynchronize(e1) == 0
+        ms = ctypes.c_float()
+        assert hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+        if r and (best is None or ms.value < best):
+            best = ms.value
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return 2.0 * nbytes / (best / 1e3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -340,6 +370,11 @@ def main():
                 gbs = (pb_ * sc.n + nb_ * touched + gb_ * sc.G) / (ms_ / 1e3 / n_) / 1e9
                 kroof[k] = {"achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
+    copy_gbs = copy_bandwidth()
+    print(f"# device-to-device copy: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    if roof is not None:
+        roof["frac_of_copy"] = roof["achieved"] / copy_gbs
+
     batched = None
     if args.batched_steps:
         batched = run_batched(pkg, 24, 51, args.dt, 1, args.batched_steps)
@@ -371,6 +406,8 @@ def main():
                    "dt": args.dt, "volume": "reference formula x dx^3", "control": "dFc ~ N(0,1e-2) on layer 0"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "hbm_copy_probe": {"achieved_GBs": copy_gbs, "frac_of_peak": copy_gbs / HBM_PEAK_GBS,
+                           "note": "HIP runtime device-to-device copy of 256 MiB, read + write bytes, best of 5"},
         "step_roofline": {"bytes_per_particle_step": (STEP_BYTES[0] * sc.n + STEP_BYTES[1] * touched + STEP_BYTES[2] * sc.G)
                           / sc.n,
                           "achieved_GBs": (STEP_BYTES[0] * sc.n + STEP_BYTES[1] * touched + STEP_BYTES[2] * sc.G)
