@@ -131,6 +131,9 @@ int dalloc(dmpm_ctx* c, void** p, size_t bytes)
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
+        // hipMalloc also records the error as the thread's last error: clear it,
+        // or the next launch check of any context would report it again
+        (void)hipGetLastError();
         return fail(DMPM_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) + " B) failed: " + hipGetErrorString(e));
     }
     c->bytes += bytes;

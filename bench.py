@@ -339,8 +339,18 @@ def main():
 
     r64 = None
     if not args.no_fp64:
-        r64 = run_path(pkg, sc, vol, dFc, pkg.PREC_F64, pkg.TAPE_STORE_ALL, L, 1, args.fp64_steps,
-                       prof_steps=1 if args.prof_steps else 0)
+        tape64 = "store-all"
+        try:
+            r64 = run_path(pkg, sc, vol, dFc, pkg.PREC_F64, pkg.TAPE_STORE_ALL, L, 1, args.fp64_steps,
+                           prof_steps=1 if args.prof_steps else 0)
+        except pkg.DmpmError as e:
+            if "dmpm error -4:" not in str(e):  # only an out-of-memory tape falls back
+                raise
+            # every FP64 layer does not fit in HBM: the checkpoint tape holds sqrt(L) of them
+            print(f"# fp64 store-all does not fit ({e}); checkpoint tape instead", file=sys.stderr, flush=True)
+            tape64 = "checkpoint"
+            r64 = run_path(pkg, sc, vol, dFc, pkg.PREC_F64, pkg.TAPE_CHECKPOINT, L, 1, args.fp64_steps,
+                           prof_steps=1 if args.prof_steps else 0)
         v64 = sc.n * args.episode_steps * args.fp64_steps / r64["time"]
         print(f"# fp64 plain: {r64['time'] / args.fp64_steps * 1e3:.2f} ms/episode, {v64:.4e} particle-steps/s", file=sys.stderr, flush=True)
 
@@ -435,7 +445,7 @@ def main():
         step64 = 2 * (STEP_BYTES[0] * sc.n + STEP_BYTES[1] * touched + STEP_BYTES[2] * sc.G)
         gbs64 = step64 * args.episode_steps * args.fp64_steps / r64["time"] / 1e9
         out["fp64_plain"] = {"value": v64, "ms_per_step": r64["time"] / args.fp64_steps * 1e3, "memory_bytes": r64["mem"],
-                             "loss": r64["loss"],
+                             "loss": r64["loss"], "tape": tape64,
                              "step_roofline": {"bytes_per_particle_step": step64 / sc.n, "achieved_GBs": gbs64,
                                                "frac": gbs64 / HBM_PEAK_GBS}}
         if r64["prof"] is not None:

@@ -439,6 +439,24 @@ def test_error_paths():
         pkg.Morph(10, (0, 4, 4), 0.1, 3)
 
 
+@pytest.mark.parametrize("prec", [pkg.PREC_F64, pkg.PREC_F32])
+def test_out_of_memory_then_a_working_context(prec):
+    """A tape far beyond HBM (1M particles x 100 000 layers): dmpm_create returns
+    DMPM_ERR_NOMEM and frees what it had; the failed allocation leaves no error
+    behind, so the next context's episode runs and matches the oracle (1e-10
+    FP64, 1e-5 FP32 on the loss)."""
+    with pytest.raises(pkg.DmpmError, match="dmpm error -4"):
+        pkg.Morph(1_000_000, (8, 8, 8), 0.125, 100_000, precision=prec)
+    case = make_case(L=3)
+    mo = gpu_morph(case, precision=prec)
+    mo.forward_pass(0)
+    loss = mo.end_layer_mass_loss()
+    mo.backward_pass(0)
+    ref_loss = orc.episode(case.prm, case.st0, case.cs, case.controls, case.target, case.L)[0]
+    assert loss == pytest.approx(ref_loss, rel=1e-10 if prec == pkg.PREC_F64 else 1e-5)
+    mo.close()
+
+
 # ------------------------------------------------- FP32 tiled scatter paths --
 def _one_step_case(x, seed=11):
     rng = np.random.default_rng(seed)
