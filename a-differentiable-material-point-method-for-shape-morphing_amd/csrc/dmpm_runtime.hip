@@ -86,6 +86,12 @@ struct dmpm_ctx {
     std::vector<dmpm_instance> inst;
     std::vector<int> poff;
     std::vector<long long> nofs;
+    // device positions are kept relative to each instance's grid centre
+    // (origin[3 i + a]): an FP32 coordinate near 0 has a finer spacing than one
+    // near the far side of the box, and every step's weights and node offsets
+    // inherit that spacing.  Uploads subtract it, downloads add it back, and the
+    // kernels see min_point - origin.
+    std::vector<double> origin;
     long long gmax = 0;
     std::vector<double> inst_loss;
     // FP32 path: device order of the particles (sorted by 4^3-cell block per
@@ -231,7 +237,7 @@ StepParams inst_params(const dmpm_ctx* c, int i)
     sp.stride = c->S;
     for (int a = 0; a < 3; ++a) {
         sp.dim[a] = in.dim[a];
-        sp.min_point[a] = in.min_point[a];
+        sp.min_point[a] = in.min_point[a] - c->origin[3 * (size_t)i + a];
         sp.f_ext[a] = c->cfg.f_ext[a];
     }
     sp.G = c->nofs[i + 1] - c->nofs[i];
@@ -646,6 +652,29 @@ int download_fields(dmpm_ctx* c, const Real* src, int field0, int nfield, double
     return DMPM_OK;
 }
 
+// Positions (AoS [n][3], API order) to and from a layer's x field: shifted by
+// the origin of the particle's instance.
+template <typename Real>
+int upload_positions(dmpm_ctx* c, Real* layer, const double* x)
+{
+    std::vector<double> xs((size_t)3 * c->N);
+    for (int i = 0; i < c->B; ++i)
+        for (int p = c->poff[i]; p < c->poff[i + 1]; ++p)
+            for (int a = 0; a < 3; ++a) xs[3 * (size_t)p + a] = x[3 * (size_t)p + a] - c->origin[3 * (size_t)i + a];
+    return upload_fields<Real>(c, layer, FX, 3, xs.data(), c->N);
+}
+
+template <typename Real>
+int download_positions(dmpm_ctx* c, const Real* layer, double* x)
+{
+    const int rc = download_fields<Real>(c, layer, FX, 3, x, c->N);
+    if (rc) return rc;
+    for (int i = 0; i < c->B; ++i)
+        for (int p = c->poff[i]; p < c->poff[i + 1]; ++p)
+            for (int a = 0; a < 3; ++a) x[3 * (size_t)p + a] += c->origin[3 * (size_t)i + a];
+    return DMPM_OK;
+}
+
 // Device order of the FP32 path, per instance.  Particles are grouped by the
 // 4^3-cell block of their cell (blocks in x-major order); inside a block they
 // are ordered by their rank within their cell ("slot"), then by the cell's
@@ -741,7 +770,7 @@ int set_particles(dmpm_ctx* c, const double* x, const double* v, const double* F
     fill<Real>(c, static_cast<Real*>(c->ctrl), (long long)(c->L - 1) * 9 * c->S, Real(0));
     Real* l0 = slot<Real>(c, 0);
     int rc;
-    if ((rc = upload_fields<Real>(c, l0, FX, 3, x, c->N))) return rc;
+    if ((rc = upload_positions<Real>(c, l0, x))) return rc;
     if ((rc = upload_fields<Real>(c, l0, FV, 3, v, c->N))) return rc;
     if ((rc = upload_fields<Real>(c, l0, FC, 9, C, c->N))) return rc;
     if ((rc = upload_fields<Real>(c, l0, FF, 9, F, c->N))) return rc;
@@ -802,10 +831,12 @@ int target_from_points(dmpm_ctx* c, const double* x, const double* m, const int*
         if (e != hipSuccess) return fail(DMPM_ERR_NOMEM, std::string("target points: ") + hipGetErrorString(e));
         // SoA [4][n]: x, y, z, m; instance i's points at column offset sum(counts[<i])
         std::vector<Real> h((size_t)4 * n);
-        for (long long p = 0; p < n; ++p) {
-            for (int a = 0; a < 3; ++a) h[(size_t)a * n + p] = Real(x[3 * (size_t)p + a]);
-            h[(size_t)3 * n + p] = Real(m[p]);
-        }
+        long long p = 0;
+        for (int i = 0; i < c->B; ++i)
+            for (int k = 0; k < counts[i]; ++k, ++p) {
+                for (int a = 0; a < 3; ++a) h[(size_t)a * n + p] = Real(x[3 * (size_t)p + a] - c->origin[3 * (size_t)i + a]);
+                h[(size_t)3 * n + p] = Real(m[p]);
+            }
         Real* d = static_cast<Real*>(buf);
         hipError_t e2 = hipMemcpyAsync(d, h.data(), h.size() * sizeof(Real), hipMemcpyHostToDevice, c->stream);
         if (e2 == hipSuccess) {
@@ -856,7 +887,7 @@ int get_layer_state(dmpm_ctx* c, int layer, double* x, double* v, double* F, dou
     if (layer > c->fwd_valid) return fail(DMPM_ERR_STATE, "layer not computed by a forward pass");
     Real* l = (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) ? slot<Real>(c, layer) : materialize<Real>(c, layer);
     int rc;
-    if (x && (rc = download_fields<Real>(c, l, FX, 3, x, c->N))) return rc;
+    if (x && (rc = download_positions<Real>(c, l, x))) return rc;
     if (v && (rc = download_fields<Real>(c, l, FV, 3, v, c->N))) return rc;
     if (F && (rc = download_fields<Real>(c, l, FF, 9, F, c->N))) return rc;
     if (C && (rc = download_fields<Real>(c, l, FC, 9, C, c->N))) return rc;
@@ -891,7 +922,7 @@ int set_layer_state(dmpm_ctx* c, int layer, const double* x, const double* v, co
     if (layer > c->fwd_valid) return fail(DMPM_ERR_STATE, "layer not computed by a forward pass");
     Real* l = slot<Real>(c, layer);
     int rc;
-    if (x && (rc = upload_fields<Real>(c, l, FX, 3, x, c->N))) return rc;
+    if (x && (rc = upload_positions<Real>(c, l, x))) return rc;
     if (v && (rc = upload_fields<Real>(c, l, FV, 3, v, c->N))) return rc;
     if (F && (rc = upload_fields<Real>(c, l, FF, 9, F, c->N))) return rc;
     if (C && (rc = upload_fields<Real>(c, l, FC, 9, C, c->N))) return rc;
@@ -1074,6 +1105,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         c->poff[i + 1] = c->poff[i] + in.n_particles;
         c->nofs[i + 1] = c->nofs[i] + Gi;
         c->gmax = std::max(c->gmax, Gi);
+        for (int d = 0; d < 3; ++d) c->origin.push_back(in.min_point[d] + 0.5 * in.dim[d] * in.dx);
     }
     c->N = c->poff[c->B];
     c->cfg.n_particles = c->N;

@@ -74,3 +74,10 @@ def maxrel(a, b):
     b = np.asarray(b, dtype=np.float64)
     den = np.abs(b).max()
     return float(np.abs(a - b).max() / (den if den > 0 else 1.0))
+
+
+def f32_positions(x, dim, dx, min_point):
+    """Positions as an FP32 context returns them: stored on the device relative
+    to the grid centre (min_point + dim dx / 2) in float, the centre added back."""
+    c = np.asarray(min_point, dtype=np.float64) + 0.5 * np.asarray(dim, dtype=np.float64) * dx
+    return (np.asarray(x, dtype=np.float64) - c).astype(np.float32).astype(np.float64) + c

@@ -11,7 +11,7 @@ change only the launch shape: results stay within the FP32-vs-oracle bars.
 import numpy as np
 import pytest
 
-from _scenes import gpu_morph, make_case, maxrel, orc, pkg, rel
+from _scenes import f32_positions, gpu_morph, make_case, maxrel, orc, pkg, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -81,7 +81,7 @@ def test_autotune_keeps_results():
     with pytest.raises(pkg.DmpmError):
         mo.layer_state(3)  # the trajectory was invalidated
     x0 = mo.layer_state(0)[0]
-    assert np.array_equal(x0, case.st0.x.astype(np.float32).astype(np.float64))
+    assert np.array_equal(x0, f32_positions(case.st0.x, case.sc.dim, case.sc.dx, case.sc.min_point))
     mo.set_graphs(True)
     loss, g, x = _episode(mo)
     ref_loss, layers, gd, _ = orc.episode(case.prm, case.st0, case.cs, case.controls, case.target, case.L)

@@ -9,7 +9,7 @@ Tolerances (stated per test):
 import numpy as np
 import pytest
 
-from _scenes import gpu_morph, make_case, maxrel, orc, pkg, rel
+from _scenes import f32_positions, gpu_morph, make_case, maxrel, orc, pkg, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -379,7 +379,10 @@ def test_diverged_particles_touch_no_node(prec):
     # one stays non-finite (FP64 skips its nodes and keeps inf; the FP32 gather's
     # fixed window multiplies its NaN weights by the window's zeros)
     fin = np.isfinite(x[n - nb:]).all(axis=1)
-    assert np.array_equal(xl[n - nb:][fin], x[n - nb:][fin])
+    kept = x[n - nb:][fin]
+    if prec == pkg.PREC_F32:  # as the FP32 context stores it (relative to the grid centre)
+        kept = f32_positions(kept, sc.dim, sc.dx, sc.min_point)
+    assert np.array_equal(xl[n - nb:][fin], kept)
     assert not np.isfinite(xl[n - nb:][~fin]).all(axis=1).any()
     mo.close()
 
@@ -521,5 +524,5 @@ def test_fp32_api_order_roundtrip():
     mo.set_control(1, c)
     assert np.array_equal(mo.control(1), c.astype(np.float32).astype(np.float64))
     x0 = mo.layer_state(0)[0]
-    assert np.array_equal(x0, st.x[perm].astype(np.float32).astype(np.float64))
+    assert np.array_equal(x0, f32_positions(st.x[perm], case.sc.dim, case.sc.dx, case.sc.min_point))
     assert np.array_equal(mo.volumes(), case.cs.vol[perm].astype(np.float32).astype(np.float64))
