@@ -166,10 +166,14 @@ def _check(rc: int):
         raise DmpmError(f"dmpm error {rc}: {msg}")
 
 
-def _in(a, shape_tail=None):
+def _in(a, size=None, name="array"):
+    """Contiguous float64 copy of ``a`` and its pointer; ``size``: the number of values
+    the C side reads from the pointer (ValueError when ``a`` holds another count)."""
     if a is None:
         return None, None
     arr = np.ascontiguousarray(a, dtype=np.float64)
+    if size is not None and arr.size != size:
+        raise ValueError(f"{name}: expected {size} values, got {arr.size} (shape {arr.shape})")
     return arr, arr.ctypes.data_as(_DP)
 
 
@@ -197,7 +201,7 @@ def eval_hessian(F, lam, mu, dF, exact=True):
     n = F.shape[0]
     lam, pl = _in(np.broadcast_to(np.asarray(lam, float), (n,)))
     mu, pm = _in(np.broadcast_to(np.asarray(mu, float), (n,)))
-    dF, pd = _in(np.asarray(dF).reshape(-1, 9))
+    dF, pd = _in(dF, 9 * n, "dF")
     out = np.zeros((n, 9))
     _check(L.dmpm_eval_hessian(n, pF, pl, pm, pd, _ptr(out), int(exact)))
     return out.reshape(n, 3, 3)
@@ -298,7 +302,10 @@ class Morph:
 
     # -- inputs ---------------------------------------------------------
     def set_particles(self, x, v, F, C, m, vol, lam, mu):
-        arrs = [_in(a) for a in (x, v, F, C, m, vol, lam, mu)]
+        n = self.n
+        sizes = (3 * n, 3 * n, 9 * n, 9 * n, n, n, n, n)
+        names = ("x", "v", "F", "C", "m", "vol", "lam", "mu")
+        arrs = [_in(a, sz, nm) for a, sz, nm in zip((x, v, F, C, m, vol, lam, mu), sizes, names)]
         _check(self._L.dmpm_set_particles(self._h, *[p for _, p in arrs]))
 
     def compute_volumes(self):
@@ -310,7 +317,7 @@ class Morph:
         return out
 
     def set_control(self, layer, dFc):
-        a, p = _in(dFc)
+        a, p = _in(dFc, 9 * self.n, "dFc")
         _check(self._L.dmpm_set_control(self._h, int(layer), p))
 
     def control(self, layer):
@@ -319,13 +326,13 @@ class Morph:
         return out
 
     def set_target_grid(self, mass):
-        a, p = _in(mass)
+        a, p = _in(mass, self.G, "target mass")
         _check(self._L.dmpm_set_target_grid(self._h, p))
 
     def target_from_points(self, x, m):
-        xa, px = _in(x)
-        ma, pm = _in(m)
-        _check(self._L.dmpm_target_from_points(self._h, px, pm, int(len(ma))))
+        ma, pm = _in(np.asarray(m, dtype=np.float64).reshape(-1), None, "m")
+        xa, px = _in(x, 3 * ma.size, "x")
+        _check(self._L.dmpm_target_from_points(self._h, px, pm, int(ma.size)))
 
     def target_grid(self):
         out = np.zeros(self.G)
@@ -375,7 +382,8 @@ class Morph:
         return J, psi
 
     def set_layer_state(self, layer, x=None, v=None, F=None, C=None):
-        arrs = [_in(a) for a in (x, v, F, C)]
+        n = self.n
+        arrs = [_in(a, sz, nm) for a, sz, nm in zip((x, v, F, C), (3 * n, 3 * n, 9 * n, 9 * n), ("x", "v", "F", "C"))]
         _check(self._L.dmpm_set_layer_state(self._h, int(layer), *[p for _, p in arrs]))
 
     def layer_grads(self, layer):
@@ -387,7 +395,9 @@ class Morph:
         return x, v, F, C
 
     def set_layer_grads(self, layer, dLdx=None, dLdv=None, dLdF=None, dLdC=None):
-        arrs = [_in(a) for a in (dLdx, dLdv, dLdF, dLdC)]
+        n = self.n
+        arrs = [_in(a, sz, nm) for a, sz, nm in
+                zip((dLdx, dLdv, dLdF, dLdC), (3 * n, 3 * n, 9 * n, 9 * n), ("dLdx", "dLdv", "dLdF", "dLdC"))]
         _check(self._L.dmpm_set_layer_grads(self._h, int(layer), *[p for _, p in arrs]))
 
     def control_grad(self, layer):
@@ -523,9 +533,11 @@ class MorphBatch(Morph):
 
     def target_from_points(self, xs, ms):
         """Target grids from one point cloud per instance (lists of (n_i,3) and (n_i,))."""
-        counts = (ctypes.c_int32 * self.B)(*[len(m) for m in ms])
-        xa, px = _in(np.concatenate([np.asarray(x, float).reshape(-1, 3) for x in xs]))
-        ma, pm = _in(np.concatenate([np.asarray(m, float).reshape(-1) for m in ms]))
+        if len(xs) != self.B or len(ms) != self.B:
+            raise ValueError(f"target point clouds: expected {self.B} instances, got {len(xs)} and {len(ms)}")
+        counts = (ctypes.c_int32 * self.B)(*[np.asarray(m).size for m in ms])
+        ma, pm = _in(np.concatenate([np.asarray(m, float).reshape(-1) for m in ms]), None, "m")
+        xa, px = _in(np.concatenate([np.asarray(x, float).reshape(-1) for x in xs]), 3 * ma.size, "x")
         _check(self._L.dmpm_target_from_points_batch(self._h, px, pm, counts))
 
     def instance_losses(self):

@@ -131,19 +131,27 @@ struct dmpm_ctx {
 
 namespace {
 
-int dalloc(dmpm_ctx* c, void** p, size_t bytes)
+// Every device allocation of the library: a failed hipMalloc returns
+// DMPM_ERR_NOMEM and clears the error hipMalloc also records as the thread's
+// last error (the next launch check of any context would report it again).
+int dmalloc(void** p, size_t bytes, const char* what)
 {
     if (bytes == 0) bytes = 256;
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
-        // hipMalloc also records the error as the thread's last error: clear it,
-        // or the next launch check of any context would report it again
         (void)hipGetLastError();
-        return fail(DMPM_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) + " B) failed: " + hipGetErrorString(e));
+        return fail(DMPM_ERR_NOMEM, std::string(what) + ": hipMalloc(" + std::to_string(bytes) + " B) failed: " + hipGetErrorString(e));
     }
-    c->bytes += bytes;
     return DMPM_OK;
+}
+
+// a context buffer (counted in dmpm_memory_bytes)
+int dalloc(dmpm_ctx* c, void** p, size_t bytes)
+{
+    const int rc = dmalloc(p, bytes, "context buffer");
+    if (rc == DMPM_OK) c->bytes += bytes == 0 ? 256 : bytes;
+    return rc;
 }
 
 hipEvent_t pool_event(dmpm_ctx* c)
@@ -748,7 +756,14 @@ int build_ranges(dmpm_ctx* c)
     c->host_ranges.swap(rg);
     c->n_ranges = (int)c->host_ranges.size();
     if (c->n_ranges > 0) {
-        HIP_TRY(hipMalloc(&c->ranges, c->host_ranges.size() * sizeof(int4)));
+        const int rc = dmalloc((void**)&c->ranges, c->host_ranges.size() * sizeof(int4), "block ranges");
+        if (rc) {
+            c->n_ranges = 0;
+            c->host_ranges.clear();
+            c->batch.ranges = nullptr;
+            c->batch.nranges = 0;
+            return rc;
+        }
         HIP_TRY(hipMemcpy(c->ranges, c->host_ranges.data(), c->host_ranges.size() * sizeof(int4), hipMemcpyHostToDevice));
     }
     c->batch.ranges = c->ranges;
@@ -824,11 +839,14 @@ int target_from_points(dmpm_ctx* c, const double* x, const double* m, const int*
     if (n > INT32_MAX) return fail(DMPM_ERR_INVALID, "too many target points");
     if (n > 0 && (!x || !m)) return fail(DMPM_ERR_INVALID, "bad target point cloud");
     Real* tg = static_cast<Real*>(c->target);
+    void* buf = nullptr;
+    if (n > 0) {
+        // allocated before the target is touched: a failure leaves the old target in place
+        const int rc = dmalloc(&buf, (size_t)4 * n * sizeof(Real), "target points");
+        if (rc) return rc;
+    }
     fill<Real>(c, tg, c->G, Real(0));
     if (n > 0) {
-        void* buf = nullptr;
-        hipError_t e = hipMalloc(&buf, (size_t)4 * n * sizeof(Real));
-        if (e != hipSuccess) return fail(DMPM_ERR_NOMEM, std::string("target points: ") + hipGetErrorString(e));
         // SoA [4][n]: x, y, z, m; instance i's points at column offset sum(counts[<i])
         std::vector<Real> h((size_t)4 * n);
         long long p = 0;
@@ -851,8 +869,10 @@ int target_from_points(dmpm_ctx* c, const double* x, const double* m, const int*
         }
         hipError_t e3 = hipStreamSynchronize(c->stream);
         hipFree(buf);
-        if (e2 != hipSuccess || e3 != hipSuccess)
+        if (e2 != hipSuccess || e3 != hipSuccess) {
+            c->have_target = false;  // the target was cleared
             return fail(DMPM_ERR_HIP, std::string("target P2G: ") + hipGetErrorString(e2 != hipSuccess ? e2 : e3));
+        }
     }
     HIP_TRY(hipGetLastError());
     c->have_target = true;
@@ -1705,14 +1725,14 @@ int eval_common(int n, std::initializer_list<std::pair<const double*, size_t>> i
     size_t k = 0;
     for (auto& in : ins) {
         if (!in.first) return fail(DMPM_ERR_INVALID, "null input");
-        HIP_TRY(hipMalloc(&bufs[k].p, in.second * n * sizeof(double)));
+        if (const int rc = dmalloc(&bufs[k].p, in.second * n * sizeof(double), "evaluator input")) return rc;
         HIP_TRY(hipMemcpy(bufs[k].p, in.first, in.second * n * sizeof(double), hipMemcpyHostToDevice));
         ptrs.push_back(static_cast<double*>(bufs[k].p));
         ++k;
     }
     for (auto& o : outs) {
         if (!o.first) return fail(DMPM_ERR_INVALID, "null output");
-        HIP_TRY(hipMalloc(&bufs[k].p, o.second * n * sizeof(double)));
+        if (const int rc = dmalloc(&bufs[k].p, o.second * n * sizeof(double), "evaluator output")) return rc;
         ptrs.push_back(static_cast<double*>(bufs[k].p));
         ++k;
     }

@@ -66,11 +66,15 @@ def gradient_descent(mo, iters, alpha, control_layer=0, max_line_search_iters=8,
     step size.  ``rule`` DESCEND_GD is Descend_GradientDescent
     (PointCloud.cpp:164-179: step alpha g / |g|); DESCEND_ADAM is
     Descend_Adam (PointCloud.cpp:83-99) with the timestep advanced per accepted
-    step and a failed trial undone by the reference's own -alpha call.
+    step only (CompGraph.cpp:440) and a failed trial undone by the reference's
+    own -alpha call.  That undo also moves Adam's moments, so it does not restore
+    the control exactly: after a failed search the loss is evaluated again and
+    the recorded loss is always that of the control held.
     Returns the loss after every iteration (index 0: the initial loss)."""
     mo.forward_pass(0)
     loss = mo.end_layer_mass_loss()
     losses = [loss]
+    timestep = 1
     for it in range(iters):
         mo.backward_pass(control_layer)
         gn = mo.control_grad_norm(control_layer)
@@ -78,18 +82,20 @@ def gradient_descent(mo, iters, alpha, control_layer=0, max_line_search_iters=8,
             break
         accepted = False
         for _ in range(max_line_search_iters):
-            mo.descend(control_layer, rule, alpha, gn, beta1, beta2, epsilon, it + 1)
+            mo.descend(control_layer, rule, alpha, gn, beta1, beta2, epsilon, timestep)
             mo.forward_pass(control_layer)
             trial = mo.end_layer_mass_loss()
             if trial < loss:
                 loss = trial
                 accepted = True
                 break
-            mo.descend(control_layer, rule, -alpha, gn, beta1, beta2, epsilon, it + 1)
+            mo.descend(control_layer, rule, -alpha, gn, beta1, beta2, epsilon, timestep)
             alpha *= 0.5
-        if not accepted:
+        if accepted:
+            timestep += 1
+        else:
             mo.forward_pass(control_layer)
-            mo.end_layer_mass_loss()
+            loss = mo.end_layer_mass_loss()
         losses.append(loss)
         if log:
             log(f"iter {it + 1}: loss {loss:.8g} alpha {alpha:.3g}")

@@ -324,8 +324,11 @@ class Grads:
         return Grads(np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3, 3)), np.zeros((n, 3, 3)), np.zeros((n, 3, 3)))
 
 
-def backward_step(prm: Params, st: State, cs: Consts, dFc, grid_m, grid_v, C1, g1: Grads) -> Grads:
-    """Back_Timestep (BackPropagation.cpp:6-125) for layer n from layer n+1's gradients."""
+def backward_step(prm: Params, st: State, cs: Consts, dFc, grid_m, grid_v, C1, g1: Grads, return_grid=False):
+    """Back_Timestep (BackPropagation.cpp:6-125) for layer n from layer n+1's gradients.
+
+    ``return_grid``: also return the adjoint grid after the transposed grid
+    update, (pbar (G,3), mbar (G,)), as (grads, pbar, mbar)."""
     _n(prm, st.x, st.v, st.C, st.F, dFc, cs.m, cs.vol, cs.lam, cs.mu, C1, g1.x, g1.v, g1.F, g1.C)
     assert np.size(grid_m) == prm.G and np.size(grid_v) == 3 * prm.G
     n = prm.n
@@ -338,6 +341,8 @@ def backward_step(prm: Params, st: State, cs: Consts, dFc, grid_m, grid_v, C1, g
                             _p(_c(grid_m)), _p(_c(grid_v)), _p(_c(C1)), _p(_c(g1.x)), _p(_c(g1.v)),
                             _p(_c(g1.F)), _p(_c(g1.C)), _p(wv), _p(wm),
                             _p(out.x), _p(out.v), _p(out.F), _p(out.C), _p(out.dFc))
+    if return_grid:
+        return out, wv, wm
     return out
 
 

@@ -90,10 +90,25 @@ void launch_loss_particles_b(const Batch&, const float*, const float*, const dou
 }  // namespace dmpm
 
 // ------------------------------------------------ runtime API on the host --
+// the error of the last failed call, returned and cleared by hipGetLastError as
+// the HIP runtime does (a failed hipMalloc leaves it set)
+static hipError_t g_last = hipSuccess;
+
+// DMPM_HOST_MALLOC_LIMIT (bytes, read at every call): larger requests fail as
+// out of memory, so that tests can take the library's allocation-failure paths
 hipError_t hipMalloc(void** p, size_t bytes)
 {
+    const char* lim = getenv("DMPM_HOST_MALLOC_LIMIT");
+    if (lim && bytes > strtoull(lim, nullptr, 10)) {
+        *p = nullptr;
+        g_last = hipErrorOutOfMemory;
+        return hipErrorOutOfMemory;
+    }
     *p = malloc(bytes ? bytes : 1);
-    if (!*p) return hipErrorOutOfMemory;
+    if (!*p) {
+        g_last = hipErrorOutOfMemory;
+        return hipErrorOutOfMemory;
+    }
     memset(*p, 0x7f, bytes);  // not zero: a read before the first write shows
     return hipSuccess;
 }
@@ -138,8 +153,13 @@ hipError_t hipGetDeviceCount(int* n)
     return hipSuccess;
 }
 hipError_t hipSetDevice(int) { return hipSuccess; }
-hipError_t hipGetLastError() { return hipSuccess; }
-const char* hipGetErrorString(hipError_t) { return "host build"; }
+hipError_t hipGetLastError()
+{
+    const hipError_t e = g_last;
+    g_last = hipSuccess;
+    return e;
+}
+const char* hipGetErrorString(hipError_t e) { return e == hipErrorOutOfMemory ? "out of memory (host build)" : "host build"; }
 hipError_t hipDeviceSynchronize() { return hipSuccess; }
 hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipErrorInvalidValue; }
 hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t*) { return hipErrorInvalidValue; }

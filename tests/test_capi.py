@@ -80,3 +80,47 @@ int main()
     subprocess.run(["g++", "-std=c++17", "-Wall", "-I", os.path.join(inc, "DiffMPMLib3D"), "-I", inc, str(src), "-L", lib,
                     "-ldmpm_cpp", "-ldmpm", f"-Wl,-rpath,{lib}", "-o", str(exe)], check=True, timeout=300)
     subprocess.run([str(exe)], check=True, timeout=60)
+
+
+def _shell_morph(pkg, cls, n=4, G=8, B=None):
+    """A Morph whose library handle is never reached: the length checks of the binding
+    raise before any C call, so these run without a GPU."""
+    mo = object.__new__(cls)
+    mo.n, mo.G, mo._h, mo._L = n, G, None, None
+    if B is not None:
+        mo.B = B
+    return mo
+
+
+def test_binding_rejects_wrong_lengths():
+    """Arrays whose length differs from what the C side reads (n*3, n*9, n, G, len(m)*3)
+    raise ValueError instead of handing a short buffer to the library."""
+    import numpy as np
+    import pytest
+
+    pkg = load_pkg()
+    mo = _shell_morph(pkg, pkg.Morph)
+    n = 4
+    good = dict(x=np.zeros((n, 3)), v=np.zeros((n, 3)), F=np.zeros((n, 3, 3)), C=np.zeros((n, 3, 3)), m=np.ones(n),
+                vol=np.ones(n), lam=np.ones(n), mu=np.ones(n))
+    for k, short in [("x", np.zeros((n - 1, 3))), ("v", np.zeros(5)), ("F", np.zeros((n, 3))), ("C", np.zeros((n, 9, 2))),
+                     ("m", np.ones(n + 1)), ("vol", np.ones(2)), ("lam", np.ones(1)), ("mu", np.ones(n * 3))]:
+        with pytest.raises(ValueError, match=k):
+            mo.set_particles(**{**good, k: short})
+    with pytest.raises(ValueError, match="dFc"):
+        mo.set_control(0, np.zeros((n, 3)))
+    with pytest.raises(ValueError, match="target mass"):
+        mo.set_target_grid(np.zeros(7))
+    with pytest.raises(ValueError, match="x"):
+        mo.target_from_points(np.zeros((3, 3)), np.ones(4))
+    with pytest.raises(ValueError, match="F"):
+        mo.set_layer_state(1, F=np.zeros((n - 1, 3, 3)))
+    with pytest.raises(ValueError, match="dLdv"):
+        mo.set_layer_grads(1, dLdv=np.zeros((n, 2)))
+    with pytest.raises(ValueError, match="dF"):
+        pkg.eval_hessian(np.zeros((5, 3, 3)), 1.0, 1.0, np.zeros((4, 3, 3)))
+    mb = _shell_morph(pkg, pkg.MorphBatch, n=6, G=16, B=2)
+    with pytest.raises(ValueError, match="x"):
+        mb.target_from_points([np.zeros((2, 3)), np.zeros((2, 3))], [np.ones(2), np.ones(3)])
+    with pytest.raises(ValueError, match="instances"):
+        mb.target_from_points([np.zeros((2, 3))], [np.ones(2)])

@@ -114,3 +114,46 @@ def test_fp64_kernels_under_sanitizers(tmp_path):
         if mode == "run":  # the same checks as the GPU test of the mirror
             test_cpp_mirror.check_run_outputs(tmp_path, *inputs)
     assert "Final loss" in r.stdout
+
+
+NOMEM_DRIVER = r"""
+import os, sys
+sys.path.insert(0, {tests!r}); sys.path.insert(0, {root!r})
+import numpy as np
+from conftest import load_pkg
+pkg = load_pkg()
+pkg.load_library({lib!r})
+from _scenes import gpu_morph, make_case
+case = make_case(L=3)
+mo = gpu_morph(case)
+mo.forward_pass(0)
+l0 = mo.end_layer_mass_loss()
+os.environ["DMPM_HOST_MALLOC_LIMIT"] = "100000"
+for call in (lambda: mo.target_from_points(np.full((20000, 3), 0.5), np.ones(20000)),
+             lambda: pkg.eval_pk1(np.broadcast_to(np.eye(3), (2000, 3, 3)), 1.0, 1.0)):
+    try:
+        call()
+    except pkg.DmpmError as e:
+        assert "dmpm error -4:" in str(e) and "out of memory" in str(e), str(e)
+    else:
+        raise AssertionError("allocation failure not reported")
+del os.environ["DMPM_HOST_MALLOC_LIMIT"]
+# no stale HIP error reaches the next call, and the failed target call left the old target
+mo.forward_pass(0)
+l1 = mo.end_layer_mass_loss()
+assert l1 == l0, (l0, l1)
+assert pkg.eval_pk1(np.eye(3)[None], 1.0, 1.0).shape == (1, 3, 3)
+print("nomem ok")
+"""
+
+
+def test_allocation_failures_report_nomem():
+    """A failed device allocation (the host stand-in's hipMalloc refuses requests above
+    DMPM_HOST_MALLOC_LIMIT and leaves the error set, as HIP does) returns DMPM_ERR_NOMEM
+    from the target-point staging and the batched evaluators, leaves the previous target
+    grid in place, and no stale error fails the next forward pass."""
+    _build()
+    code = NOMEM_DRIVER.format(tests=os.path.join(ROOT, "tests"), root=ROOT, lib=LIB)
+    r = subprocess.run([sys.executable, "-c", code], env=_env(), capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0 and "runtime error" not in r.stderr, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "nomem ok" in r.stdout

@@ -330,6 +330,149 @@ def test_backward_compat_difference_is_defect_terms():
     assert np.allclose(e0.x, b0.x, rtol=1e-12) and np.allclose(e0.F, b0.F, rtol=1e-12)
 
 
+def _compat_case(seed):
+    """One step of the tiny scene (s = 0.1, drag = 0.3, f_ext != 0, v != 0) and a random seed lambda."""
+    prm, st, cs, dFc = tiny_scene(n=30, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    n = prm.n
+    nxt, gm, gv = orc.forward_step(prm, st, cs, dFc)
+    lam_ = orc.Grads(rng.normal(size=(n, 3)), rng.normal(size=(n, 3)), rng.normal(size=(n, 3, 3)), rng.normal(size=(n, 3, 3)))
+    return prm, st, cs, dFc, nxt, gm, gv, lam_
+
+
+def _bs(t):
+    """Cubic B-spline N and N' (Interpolation.h:5-30) from their closed forms, elementwise."""
+    a = np.abs(t)
+    N = np.where(a < 1, 0.5 * a ** 3 - a * a + 2.0 / 3.0, np.where(a < 2, (2 - a) ** 3 / 6.0, 0.0))
+    dN = np.where(a < 1, 1.5 * t * a - 2.0 * t, np.where(a < 2, -0.5 * (2 - a) ** 2 * np.sign(t), 0.0))
+    return N, dN
+
+
+def _brute_weights(prm, x):
+    """w_ip, dw_ip/dx_p (n,G,3) and d_ip = x_i - x_p (n,G,3) over EVERY node of the grid."""
+    ii, jj, kk = np.meshgrid(*[np.arange(d) for d in prm.dim], indexing="ij")
+    xi = np.asarray(prm.min_point) + prm.dx * np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1)  # Grid.cpp:22
+    d = xi[None, :, :] - x[:, None, :]
+    N, dN = _bs(-d / prm.dx)  # argument (x_p - x_i)/dx: d/dx_p N = N'/dx
+    w = N[..., 0] * N[..., 1] * N[..., 2]
+    gw = np.stack([dN[..., 0] * N[..., 1] * N[..., 2], N[..., 0] * dN[..., 1] * N[..., 2],
+                   N[..., 0] * N[..., 1] * dN[..., 2]], axis=-1) / prm.dx
+    return w, gw, d
+
+
+def _brute_pbar(prm, st, dFc, gm, lam_, cs_factor):
+    """pbar_i = vbar_i / m_i (|m_i| > FLT_EPSILON) with vbar_i = sum_p w (vt + D Ct d),
+    vt = v' + dt x', Ct = C' + cs dt F' Fc^T (BackPropagation.cpp:23-59), brute force."""
+    w, _, d = _brute_weights(prm, st.x)
+    Fc = st.F + dFc
+    vt = lam_.v + prm.dt * lam_.x
+    Ct = lam_.C + cs_factor * prm.dt * np.einsum("pik,pjk->pij", lam_.F, Fc)
+    D = 3.0 / prm.dx ** 2
+    vbar = np.einsum("pg,pa->ga", w, vt) + D * np.einsum("pg,pab,pgb->ga", w, Ct, d)
+    ok = np.abs(gm) > 1.1920928955078125e-07
+    return np.where(ok[:, None], vbar / np.where(ok, gm, 1.0)[:, None], 0.0)
+
+
+def test_compat_no_1ms_is_exact_adjoint_of_shifted_cbar():
+    """Defect (b), BackPropagation.cpp:24: dLdC_next = dt dLdF' Fc^T + dLdC' lacks the
+    (1-s) of the exact Ct = C' + (1-s) dt F' Fc^T.  The adjoint is linear in the seed
+    and C' enters only through Ct, so (b) alone must equal the exact adjoint with
+    C' + s dt F' Fc^T in every output; a wrong factor (1-s, 2s) or Fc in place of
+    Fc^T does not."""
+    prm, st, cs, dFc, nxt, gm, gv, lam_ = _compat_case(60)
+    assert prm.smoothing == pytest.approx(0.1)
+    comp = orc.backward_step(orc.Params(**{**prm.__dict__, "compat": orc.COMPAT_NO_1MS}), st, cs, dFc, gm, gv, nxt.C, lam_)
+    Fc = st.F + dFc
+
+    def shifted(extra):
+        lam2 = orc.Grads(lam_.x, lam_.v, lam_.F, lam_.C + extra)
+        return orc.backward_step(prm, st, cs, dFc, gm, gv, nxt.C, lam2)
+
+    good = shifted(prm.smoothing * prm.dt * np.einsum("pik,pjk->pij", lam_.F, Fc))
+    for f in ("x", "v", "F", "C", "dFc"):
+        a, b = getattr(comp, f), getattr(good, f)
+        assert np.abs(a - b).max() <= 1e-11 * np.abs(b).max(), f
+    ex = orc.backward_step(prm, st, cs, dFc, gm, gv, nxt.C, lam_)
+    assert np.abs(comp.x - ex.x).max() > 1e-6 * np.abs(ex.x).max()  # the defect is visible here
+    for bad in (shifted((1 - prm.smoothing) * prm.dt * np.einsum("pik,pjk->pij", lam_.F, Fc)),
+                shifted(2 * prm.smoothing * prm.dt * np.einsum("pik,pjk->pij", lam_.F, Fc)),
+                shifted(prm.smoothing * prm.dt * np.einsum("pik,pkj->pij", lam_.F, Fc))):
+        assert np.abs(comp.x - bad.x).max() > 1e-6 * np.abs(comp.x).max()
+
+
+def test_compat_no_drag_x_closed_form():
+    """Defect (c), BackPropagation.cpp:109 vs ForwardSimulation.cpp:48: the x-gradient's
+    weight-gradient term uses m v where the forward scatters m (1 - dt drag) v, so (c)
+    alone changes xbar only, by sum_i grad w_ip (dt drag m_p v_p . pbar_i), with
+    pbar and grad w by brute force over every node; every other output is unchanged."""
+    prm, st, cs, dFc, nxt, gm, gv, lam_ = _compat_case(61)
+    assert prm.drag != 0.0
+    ex, pbar_o, _ = orc.backward_step(prm, st, cs, dFc, gm, gv, nxt.C, lam_, return_grid=True)
+    comp = orc.backward_step(orc.Params(**{**prm.__dict__, "compat": orc.COMPAT_NO_DRAG_X}), st, cs, dFc, gm, gv, nxt.C, lam_)
+    pbar = _brute_pbar(prm, st, dFc, gm, lam_, 1.0 - prm.smoothing)
+    assert np.abs(pbar_o - pbar).max() <= 1e-10 * np.abs(pbar).max()
+    _, gw, _ = _brute_weights(prm, st.x)
+    db = prm.dt * prm.drag * cs.m[:, None] * st.v  # m v - m (1 - dt drag) v
+    expect = np.einsum("pga,pg->pa", gw, np.einsum("pb,gb->pg", db, pbar))
+    diff = comp.x - ex.x
+    assert np.abs(diff).max() > 1e-6 * np.abs(ex.x).max()
+    assert np.abs(diff - expect).max() <= 1e-10 * np.abs(expect).max()
+    assert np.abs(diff - 2 * expect).max() > 0.5 * np.abs(expect).max()  # a wrong factor fails
+    for f in ("v", "F", "C", "dFc"):  # the OpenMP scatter's order leaves round-off only
+        assert np.abs(getattr(comp, f) - getattr(ex, f)).max() <= 1e-12 * np.abs(getattr(ex, f)).max(), f
+
+
+def test_compat_dldm_no_fext_closed_form():
+    """Defect (d), BackPropagation.cpp:57: dLdm = -(v.dLdv)/m uses the node velocity v,
+    not v - dt f_ext, the velocity before G_op added dt f_ext; so mbar_i changes by
+    -dt (f_ext . pbar_i) and (d) alone changes xbar only, by
+    sum_i m_p grad w_ip (-dt f_ext . pbar_i) (brute force over every node)."""
+    prm, st, cs, dFc, nxt, gm, gv, lam_ = _compat_case(62)
+    f = np.asarray(prm.f_ext)
+    assert np.abs(f).max() > 0
+    ex, _, mb_ex = orc.backward_step(prm, st, cs, dFc, gm, gv, nxt.C, lam_, return_grid=True)
+    comp, _, mb_c = orc.backward_step(orc.Params(**{**prm.__dict__, "compat": orc.COMPAT_DLDM_NO_FEXT}), st, cs, dFc, gm,
+                                      gv, nxt.C, lam_, return_grid=True)
+    pbar = _brute_pbar(prm, st, dFc, gm, lam_, 1.0 - prm.smoothing)
+    dmb = -prm.dt * pbar @ f
+    assert np.abs((mb_c - mb_ex) - dmb).max() <= 1e-10 * np.abs(dmb).max()
+    _, gw, _ = _brute_weights(prm, st.x)
+    expect = cs.m[:, None] * np.einsum("pga,g->pa", gw, dmb)
+    diff = comp.x - ex.x
+    assert np.abs(diff).max() > 1e-6 * np.abs(ex.x).max()
+    assert np.abs(diff - expect).max() <= 1e-10 * np.abs(expect).max()
+    assert np.abs(diff + expect).max() > 0.5 * np.abs(expect).max()  # the opposite sign fails
+    for f_ in ("v", "F", "C", "dFc"):
+        assert np.abs(getattr(comp, f_) - getattr(ex, f_)).max() <= 1e-12 * np.abs(getattr(ex, f_)).max(), f_
+
+
+def test_compat_reference_composes_the_four_defects():
+    """All four defects at once (the reference's Back_Timestep, compat = 15), from the
+    exact adjoint and closed forms only: (b) as the exact adjoint with C' + s dt F' Fc^T,
+    whose pbar (brute force, Ct without 1-s) then carries (c) and (d) into xbar; (a)
+    adds (I + dt C')^T F' to the F-gradient; and the control gradient is the reference's
+    dLdF, the F-gradient itself."""
+    prm, st, cs, dFc, nxt, gm, gv, lam_ = _compat_case(63)
+    full = orc.backward_step(orc.Params(**{**prm.__dict__, "compat": orc.COMPAT_DUP_FBAR | orc.COMPAT_NO_1MS | orc.COMPAT_NO_DRAG_X | orc.COMPAT_DLDM_NO_FEXT}), st, cs, dFc, gm, gv, nxt.C,
+                             lam_)
+    Fc = st.F + dFc
+    lam2 = orc.Grads(lam_.x, lam_.v, lam_.F, lam_.C + prm.smoothing * prm.dt * np.einsum("pik,pjk->pij", lam_.F, Fc))
+    gb = orc.backward_step(prm, st, cs, dFc, gm, gv, nxt.C, lam2)
+    pbar = _brute_pbar(prm, st, dFc, gm, lam_, 1.0)
+    _, gw, _ = _brute_weights(prm, st.x)
+    db = prm.dt * prm.drag * cs.m[:, None] * st.v
+    dmb = -prm.dt * pbar @ np.asarray(prm.f_ext)
+    want_x = gb.x + np.einsum("pga,pg->pa", gw, np.einsum("pb,gb->pg", db, pbar)) + \
+        cs.m[:, None] * np.einsum("pga,g->pa", gw, dmb)
+    base = np.einsum("pki,pkj->pij", np.eye(3) + prm.dt * nxt.C, lam_.F)
+    want_F = gb.F + base
+    assert np.abs(full.x - want_x).max() <= 1e-10 * np.abs(want_x).max()
+    assert np.abs(full.F - want_F).max() <= 1e-10 * np.abs(want_F).max()
+    assert np.array_equal(full.dFc, full.F)
+    for f in ("v", "C"):
+        assert np.abs(getattr(full, f) - getattr(gb, f)).max() <= 1e-11 * np.abs(getattr(gb, f)).max(), f
+
+
 def test_episode_gradient_vs_central_fd():
     """Whole-episode dL/d(dFc_0) against FP64 central differences (h = 1e-4, Richardson-extrapolated)."""
     prm, st, cs, dFc = tiny_scene(n=40, seed=50, vel=0.1, ctrl=0.01)
