@@ -53,8 +53,11 @@ EXPORTS = [
     "dmpm_profile_enable", "dmpm_profile_reset", "dmpm_profile_read", "dmpm_eval_pk1", "dmpm_eval_hessian",
     "dmpm_eval_svd", "dmpm_create_batch", "dmpm_num_instances", "dmpm_get_instance_losses",
     "dmpm_target_from_points_batch", "dmpm_set_option", "dmpm_get_option", "dmpm_autotune", "dmpm_descend",
-    "dmpm_eval_energy", "dmpm_get_point_diagnostics",
+    "dmpm_eval_energy", "dmpm_get_point_diagnostics", "dmpm_target_from_points_noalloc",
 ]
+
+# most points of one dmpm_target_from_points_noalloc call (DMPM_NOALLOC_MAX_POINTS)
+NOALLOC_MAX_POINTS = 1 << 20
 
 
 class Config(ctypes.Structure):
@@ -119,6 +122,7 @@ def load_library(path: str = LIB_PATH):
         "dmpm_get_control": ([vp, i32, _DP], ctypes.c_int),
         "dmpm_set_target_grid": ([vp, _DP], ctypes.c_int),
         "dmpm_target_from_points": ([vp, _DP, _DP, i32], ctypes.c_int),
+        "dmpm_target_from_points_noalloc": ([vp, _DP, _DP, i32], ctypes.c_int),
         "dmpm_get_target_grid": ([vp, _DP], ctypes.c_int),
         "dmpm_forward_pass": ([vp, i32], ctypes.c_int),
         "dmpm_end_layer_mass_loss": ([vp, _DP], ctypes.c_int),
@@ -333,6 +337,13 @@ class Morph:
         ma, pm = _in(np.asarray(m, dtype=np.float64).reshape(-1), None, "m")
         xa, px = _in(x, 3 * ma.size, "x")
         _check(self._L.dmpm_target_from_points(self._h, px, pm, int(ma.size)))
+
+    def target_from_points_noalloc(self, x, m):
+        """``target_from_points`` staged in the library's fixed device array: allocates
+        nothing, at most NOALLOC_MAX_POINTS points."""
+        ma, pm = _in(np.asarray(m, dtype=np.float64).reshape(-1), None, "m")
+        xa, px = _in(x, 3 * ma.size, "x")
+        _check(self._L.dmpm_target_from_points_noalloc(self._h, px, pm, int(ma.size)))
 
     def target_grid(self):
         out = np.zeros(self.G)

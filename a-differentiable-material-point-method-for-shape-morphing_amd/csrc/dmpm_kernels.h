@@ -76,6 +76,9 @@ template <typename Real> void launch_p2g(const StepParams& sp, const StepIO<Real
 template <typename Real> void launch_grid_update(const StepParams& sp, Real* grid, hipStream_t s);
 template <typename Real> void launch_g2p(const StepParams& sp, const StepIO<Real>& io, hipStream_t s);
 template <typename Real> void launch_p2g_mass(const StepParams& sp, const Real* x, const Real* m, int stride, Real* gm, hipStream_t s);
+// the same from host-layout points: x AoS [n][3] doubles less org, m [n] doubles (n = sp.n)
+template <typename Real> void launch_p2g_mass_aos(const StepParams& sp, const double* x, const double* m, const double org[3], Real* gm,
+                                                  hipStream_t s);
 template <typename Real> void launch_volumes(const StepParams& sp, const Real* x, Real* cst, const Real* gm, hipStream_t s);
 // loss node terms: dL/dm in FP64, per-workgroup partial sums (at most
 // kLossBlocks) in part, their fixed-order sum in *loss

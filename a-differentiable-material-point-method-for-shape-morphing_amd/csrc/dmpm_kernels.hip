@@ -272,14 +272,10 @@ void launch_g2p(const StepParams& sp, const StepIO<Real>& io, hipStream_t s)
 }
 
 // ------------------------------------------------------- P2G mass / vol ----
+// one point's mass m at x into the node masses gm (HBM atomics)
 template <typename Real>
-__global__ __launch_bounds__(kBlock) void k_p2g_mass(StepParams sp, const Real* xs, const Real* ms, int S, Real* gm)
+__device__ __forceinline__ void p2g_mass_point(const StepParams& sp, const Real x[3], Real m, Real* gm)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= sp.n) return;
-    Real x[3];
-    load_vec3(xs, 0, S, p, x);
-    const Real m = ms[p];
     Stencil<Real> st;
     make_stencil(sp, x, st, false);
 #pragma unroll
@@ -302,10 +298,40 @@ __global__ __launch_bounds__(kBlock) void k_p2g_mass(StepParams sp, const Real* 
 }
 
 template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_p2g_mass(StepParams sp, const Real* xs, const Real* ms, int S, Real* gm)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= sp.n) return;
+    Real x[3];
+    load_vec3(xs, 0, S, p, x);
+    p2g_mass_point<Real>(sp, x, ms[p], gm);
+}
+
+template <typename Real>
 void launch_p2g_mass(const StepParams& sp, const Real* x, const Real* m, int stride, Real* gm, hipStream_t s)
 {
     if (sp.n <= 0) return;
     hipLaunchKernelGGL(k_p2g_mass<Real>, dim3(nblocks(sp.n, kBlock)), dim3(kBlock), 0, s, sp, x, m, stride, gm);
+}
+
+// P2G mass of host-layout points: AoS double positions x[3p + a] (shifted by
+// -org[a] and rounded to Real, as the host staging of the SoA kernel does) and
+// double masses m[p]
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_p2g_mass_aos(StepParams sp, const double* xa, const double* ma, double o0, double o1,
+                                                         double o2, Real* gm)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= sp.n) return;
+    const Real x[3] = {Real(xa[3 * (long long)p] - o0), Real(xa[3 * (long long)p + 1] - o1), Real(xa[3 * (long long)p + 2] - o2)};
+    p2g_mass_point<Real>(sp, x, Real(ma[p]), gm);
+}
+
+template <typename Real>
+void launch_p2g_mass_aos(const StepParams& sp, const double* x, const double* m, const double org[3], Real* gm, hipStream_t s)
+{
+    if (sp.n <= 0) return;
+    hipLaunchKernelGGL(k_p2g_mass_aos<Real>, dim3(nblocks(sp.n, kBlock)), dim3(kBlock), 0, s, sp, x, m, org[0], org[1], org[2], gm);
 }
 
 // ForwardSimulation.cpp:262-290: vol = m / sum_i w m_i
@@ -935,6 +961,7 @@ void launch_sumsq(const Real* a, long long count, double* part, double* out, hip
     template void launch_grid_update<R>(const StepParams&, R*, hipStream_t);                                       \
     template void launch_g2p<R>(const StepParams&, const StepIO<R>&, hipStream_t);                                 \
     template void launch_p2g_mass<R>(const StepParams&, const R*, const R*, int, R*, hipStream_t);                 \
+    template void launch_p2g_mass_aos<R>(const StepParams&, const double*, const double*, const double*, R*, hipStream_t); \
     template void launch_volumes<R>(const StepParams&, const R*, R*, const R*, hipStream_t);                       \
     template void launch_loss<R>(const StepParams&, const R*, const R*, double*, double*, double*, hipStream_t);        \
     template void launch_loss_particles<R>(const StepParams&, const R*, const R*, const double*, R*, hipStream_t);      \

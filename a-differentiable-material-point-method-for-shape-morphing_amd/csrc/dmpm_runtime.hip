@@ -19,14 +19,29 @@
 #include <functional>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
+
+// Staging of dmpm_target_from_points_noalloc, which allocates nothing: the
+// points' AoS positions (3 n doubles), then their masses (n doubles), for at
+// most kNoAllocPoints points (32 MiB, held on every GPU the library runs on
+// until the process ends).  Every call on one GPU shares that GPU's copy.
+constexpr long long kNoAllocPoints = 1LL << 20;
+static __device__ __align__(256) double scratch[4 * kNoAllocPoints];
 
 namespace {
 
 using namespace dmpm;
 
 thread_local std::string g_err;
+
+// Device address of each GPU's copy of scratch (the array's name is a
+// host-side stand-in, not a device address): looked up once per device, with
+// the device current, by the first dmpm_create on it.
+constexpr int kMaxDevices = 64;
+void* g_scratch_dev[kMaxDevices] = {};
+std::mutex g_scratch_mu;
 
 int fail(int code, const std::string& msg)
 {
@@ -152,6 +167,28 @@ int dalloc(dmpm_ctx* c, void** p, size_t bytes)
     const int rc = dmalloc(p, bytes, "context buffer");
     if (rc == DMPM_OK) c->bytes += bytes == 0 ? 256 : bytes;
     return rc;
+}
+
+// The device address of scratch on `device` (current): looked up on first use.
+int scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(DMPM_ERR_INVALID, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return DMPM_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return fail(DMPM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    g_scratch_dev[device] = ptr;
+    return DMPM_OK;
+}
+
+void* scratch_of(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (device >= 0 && device < kMaxDevices) ? g_scratch_dev[device] : nullptr;
 }
 
 hipEvent_t pool_event(dmpm_ctx* c)
@@ -880,6 +917,42 @@ int target_from_points(dmpm_ctx* c, const double* x, const double* m, const int*
     return DMPM_OK;
 }
 
+// target_from_points for one morph with the points staged in the file-scope
+// scratch array (device address from scratch_init): no allocation at all.
+template <typename Real>
+int target_from_points_noalloc(dmpm_ctx* c, const double* x, const double* m, int n)
+{
+    if (c->B != 1) return fail(DMPM_ERR_STATE, "batched context: use dmpm_target_from_points_batch");
+    if (n < 0) return fail(DMPM_ERR_INVALID, "bad target point count");
+    if (n > kNoAllocPoints)
+        return fail(DMPM_ERR_INVALID, "dmpm_target_from_points_noalloc takes at most " + std::to_string(kNoAllocPoints) +
+                                          " points (got " + std::to_string(n) + "); use dmpm_target_from_points");
+    if (n > 0 && (!x || !m)) return fail(DMPM_ERR_INVALID, "bad target point cloud");
+    double* sx = static_cast<double*>(scratch_of(c->cfg.device));
+    if (!sx) return fail(DMPM_ERR_STATE, "scratch not set up on this device");
+    Real* tg = static_cast<Real*>(c->target);
+    if (n > 0) {
+        double* sm = sx + 3 * (size_t)n;
+        HIP_TRY(hipMemcpyAsync(sx, x, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(sm, m, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    fill<Real>(c, tg, c->G, Real(0));
+    if (n > 0) {
+        StepParams sp = inst_params(c, 0);
+        sp.n = n;
+        launch_p2g_mass_aos<Real>(sp, sx, sx + 3 * (size_t)n, &c->origin[0], tg, c->stream);
+    }
+    HIP_TRY(hipGetLastError());
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        c->have_target = false;
+        return fail(DMPM_ERR_HIP, std::string("target P2G: ") + hipGetErrorString(e));
+    }
+    c->have_target = true;
+    c->loss_valid = false;
+    return DMPM_OK;
+}
+
 template <typename Real>
 int grid_download(dmpm_ctx* c, const Real* src, long long count, double* dst)
 {
@@ -1109,6 +1182,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (cfg->device < 0 || cfg->device >= ndev) return fail(DMPM_ERR_INVALID, "device ordinal out of range");
     HIP_TRY(hipSetDevice(cfg->device));
+    if (const int rs = scratch_init(cfg->device)) return rs;
 
     dmpm_ctx* c = new dmpm_ctx();
     c->cfg = *cfg;
@@ -1337,6 +1411,13 @@ int dmpm_target_from_points(dmpm_ctx* c, const double* x, const double* m, int32
     if (c->B != 1) return fail(DMPM_ERR_STATE, "batched context: use dmpm_target_from_points_batch");
     const int counts[1] = {n};
     return DISPATCH(c, target_from_points, c, x, m, counts);
+}
+
+int dmpm_target_from_points_noalloc(dmpm_ctx* c, const double* x, const double* m, int32_t n)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    return DISPATCH(c, target_from_points_noalloc, c, x, m, n);
 }
 
 int dmpm_target_from_points_batch(dmpm_ctx* c, const double* x, const double* m, const int32_t* counts)

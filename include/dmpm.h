@@ -126,6 +126,14 @@ int dmpm_set_target_grid(dmpm_ctx* ctx, const double* mass);
 /* Target grid as the P2G mass of a point cloud (MPMPolyscope.cpp:156-160). */
 int dmpm_target_from_points(dmpm_ctx* ctx, const double* x, const double* m, int32_t n);
 int dmpm_get_target_grid(dmpm_ctx* ctx, double* mass);
+/* dmpm_target_from_points without any allocation: no reference counterpart; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops).  The points
+ * are staged in a fixed device array of the library (32 MiB per GPU, held until the process
+ * ends) instead of a buffer allocated per call, so n is at most 1048576 (DMPM_ERR_INVALID
+ * beyond).  Every call on one GPU shares that array: calls that overlap on one GPU (contexts
+ * of one device used from several threads) race.  Single-morph contexts only. */
+#define DMPM_NOALLOC_MAX_POINTS 1048576
+int dmpm_target_from_points_noalloc(dmpm_ctx* ctx, const double* x, const double* m, int32_t n);
 
 /* CompGraph::ComputeForwardPass(start_layer) (CompGraph.cpp:246-256). */
 int dmpm_forward_pass(dmpm_ctx* ctx, int32_t start_layer);

@@ -36,6 +36,8 @@ inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c,
 #define __shared__ static
 #define __syncthreads()
 #define __launch_bounds__(...)
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) (reinterpret_cast<const void*>(&(x)))
 
 struct dim3 {
     unsigned x, y, z;
@@ -89,3 +91,9 @@ hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t s);
 hipError_t hipGraphExecDestroy(hipGraphExec_t e);
 hipError_t hipGraphDestroy(hipGraph_t g);
 template <typename T> inline hipError_t hipMalloc(T** p, size_t bytes) { return hipMalloc((void**)p, bytes); }
+// a __device__ variable is host memory here: its address is the symbol's own
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym)
+{
+    *p = const_cast<void*>(sym);
+    return hipSuccess;
+}
