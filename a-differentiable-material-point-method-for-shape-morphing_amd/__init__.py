@@ -54,6 +54,8 @@ EXPORTS = [
     "dmpm_eval_svd", "dmpm_create_batch", "dmpm_num_instances", "dmpm_get_instance_losses",
     "dmpm_target_from_points_batch", "dmpm_set_option", "dmpm_get_option", "dmpm_autotune", "dmpm_descend",
     "dmpm_eval_energy", "dmpm_get_point_diagnostics", "dmpm_target_from_points_noalloc",
+    "dmpm_phase_p2g", "dmpm_phase_grid_op", "dmpm_phase_grid_op_point", "dmpm_phase_g2p", "dmpm_phase_p_op2",
+    "dmpm_phase_p2g_mass", "dmpm_phase_g2g_mass",
 ]
 
 # most points of one dmpm_target_from_points_noalloc call (DMPM_NOALLOC_MAX_POINTS)
@@ -89,6 +91,25 @@ class Instance(ctypes.Structure):
         ("dx", ctypes.c_double),
         ("min_point", ctypes.c_double * 3),
     ]
+
+
+class GridDesc(ctypes.Structure):
+    """``dmpm_grid_desc``: the grid of the phase calls."""
+    _fields_ = [
+        ("dim", ctypes.c_int32 * 3),
+        ("reserved", ctypes.c_int32),
+        ("dx", ctypes.c_double),
+        ("min_point", ctypes.c_double * 3),
+    ]
+
+
+def grid_desc(dim, dx, min_point=(0.0, 0.0, 0.0)):
+    g = GridDesc()
+    for a in range(3):
+        g.dim[a] = int(dim[a])
+        g.min_point[a] = float(min_point[a])
+    g.dx = float(dx)
+    return g
 
 
 class DmpmError(RuntimeError):
@@ -155,6 +176,14 @@ def load_library(path: str = LIB_PATH):
         "dmpm_get_option": ([vp, i32, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "dmpm_autotune": ([vp, _DP], ctypes.c_int),
         "dmpm_descend": ([vp, i32, i32] + [ctypes.c_float] * 5 + [i32], ctypes.c_int),
+        "dmpm_phase_p2g": ([ctypes.POINTER(GridDesc), i32] + [_DP] * 8 + [ctypes.c_double] * 2 + [_DP] * 2, ctypes.c_int),
+        "dmpm_phase_grid_op": ([ctypes.POINTER(GridDesc), _DP, _DP, ctypes.c_double, _DP, _DP], ctypes.c_int),
+        "dmpm_phase_grid_op_point": ([ctypes.POINTER(GridDesc), _DP, _DP, ctypes.c_double, _DP, ctypes.c_double, _DP],
+                                     ctypes.c_int),
+        "dmpm_phase_g2p": ([ctypes.POINTER(GridDesc), i32, _DP, _DP, _DP, _DP], ctypes.c_int),
+        "dmpm_phase_p_op2": ([i32] + [_DP] * 5 + [ctypes.c_double] * 2 + [_DP] * 2, ctypes.c_int),
+        "dmpm_phase_p2g_mass": ([ctypes.POINTER(GridDesc), i32, _DP, _DP, _DP], ctypes.c_int),
+        "dmpm_phase_g2g_mass": ([ctypes.POINTER(GridDesc), _DP, ctypes.POINTER(GridDesc), _DP], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -232,6 +261,92 @@ def eval_energy(F, lam, mu):
     mu, pm = _in(np.broadcast_to(np.asarray(mu, float), (n,)))
     out = np.zeros(n)
     _check(L.dmpm_eval_energy(n, pF, pl, pm, _ptr(out)))
+    return out
+
+
+# ------------------------------------------------------------------ phases --
+# The reference's ForwardTimeStep phases one call each (SingleThreadMPM,
+# ForwardSimulation.h:23-45), FP64 on the GPU over host arrays; grids are
+# (dim, dx, min_point) with masses (G,) and momenta / velocities (G, 3).
+def _grid_in(g):
+    return ctypes.byref(grid_desc(*g))
+
+
+def _G(g):
+    return int(g[0][0]) * int(g[0][1]) * int(g[0][2])
+
+
+def phase_p2g(grid, x, v, C, F, dFc, P, m, vol, dt, drag, grid_m=None, grid_p=None):
+    """P2G (ForwardSimulation.cpp:11-51) added into grid_m, grid_p (zero when None); returns them."""
+    L = load_library()
+    n = np.asarray(m).size
+    G = _G(grid)
+    gm = np.zeros(G) if grid_m is None else np.array(grid_m, dtype=np.float64).reshape(G)
+    gp = np.zeros((G, 3)) if grid_p is None else np.array(grid_p, dtype=np.float64).reshape(G, 3)
+    arrs = [_in(a, sz, nm) for a, sz, nm in zip((x, v, C, F, dFc, P, m, vol), (3 * n, 3 * n, 9 * n, 9 * n, 9 * n, 9 * n, n, n),
+                                                 ("x", "v", "C", "F", "dFc", "P", "m", "vol"))]
+    _check(L.dmpm_phase_p2g(_grid_in(grid), n, *[p for _, p in arrs], float(dt), float(drag), _ptr(gm), _ptr(gp)))
+    return gm, gp
+
+
+def phase_grid_op(grid, grid_m, grid_p, dt, f_ext, grid_v=None, gravity_mag=None):
+    """G_op (ForwardSimulation.cpp:53-58,184-200); with ``gravity_mag`` the gravity-point form
+    (:202-224), f_ext then being the point.  Nodes at or below the mass threshold keep grid_v."""
+    L = load_library()
+    G = _G(grid)
+    gv = np.zeros((G, 3)) if grid_v is None else np.array(grid_v, dtype=np.float64).reshape(G, 3)
+    _, pm = _in(grid_m, G, "grid_m")
+    _, pp = _in(grid_p, 3 * G, "grid_p")
+    _, pf = _in(f_ext, 3, "f_ext")
+    if gravity_mag is None:
+        _check(L.dmpm_phase_grid_op(_grid_in(grid), pm, pp, float(dt), pf, _ptr(gv)))
+    else:
+        _check(L.dmpm_phase_grid_op_point(_grid_in(grid), pm, pp, float(dt), pf, float(gravity_mag), _ptr(gv)))
+    return gv
+
+
+def phase_g2p(grid, x, grid_v):
+    """G2P (ForwardSimulation.cpp:60-106): (v', C') per particle."""
+    L = load_library()
+    xa, px = _in(x, None, "x")
+    n = xa.size // 3
+    _, pv = _in(grid_v, 3 * _G(grid), "grid_v")
+    vn = np.zeros((n, 3))
+    Cn = np.zeros((n, 3, 3))
+    _check(L.dmpm_phase_g2p(_grid_in(grid), n, px, pv, _ptr(vn), _ptr(Cn)))
+    return vn, Cn
+
+
+def phase_p_op2(x, F, dFc, v_next, C_next, smoothing, dt):
+    """P_op_2 (ForwardSimulation.cpp:108-124): (x', F')."""
+    L = load_library()
+    xa, px = _in(x, None, "x")
+    n = xa.size // 3
+    arrs = [_in(a, sz, nm) for a, sz, nm in zip((F, dFc, v_next, C_next), (9 * n, 9 * n, 3 * n, 9 * n),
+                                                 ("F", "dFc", "v_next", "C_next"))]
+    xn = np.zeros((n, 3))
+    Fn = np.zeros((n, 3, 3))
+    _check(L.dmpm_phase_p_op2(n, px, *[p for _, p in arrs], float(smoothing), float(dt), _ptr(xn), _ptr(Fn)))
+    return xn, Fn
+
+
+def phase_p2g_mass(grid, x, m, grid_m=None):
+    """P2G_Mass's scatter (ForwardSimulation.cpp:292-327) added into grid_m."""
+    L = load_library()
+    ma, pm = _in(np.asarray(m, dtype=np.float64).reshape(-1), None, "m")
+    _, px = _in(x, 3 * ma.size, "x")
+    G = _G(grid)
+    gm = np.zeros(G) if grid_m is None else np.array(grid_m, dtype=np.float64).reshape(G)
+    _check(L.dmpm_phase_p2g_mass(_grid_in(grid), ma.size, px, pm, _ptr(gm)))
+    return gm
+
+
+def phase_g2g_mass(grid1, grid1_m, grid2):
+    """G2G_Mass (ForwardSimulation.cpp:329-367): grid-1 masses gathered at grid 2's nodes."""
+    L = load_library()
+    _, p1 = _in(grid1_m, _G(grid1), "grid1_m")
+    out = np.zeros(_G(grid2))
+    _check(L.dmpm_phase_g2g_mass(_grid_in(grid1), p1, _grid_in(grid2), _ptr(out)))
     return out
 
 

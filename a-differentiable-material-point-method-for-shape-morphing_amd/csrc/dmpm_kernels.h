@@ -139,6 +139,18 @@ void launch_loss_b(const Batch& b, const float* gm, const float* target, double*
                    hipStream_t s);
 void launch_loss_particles_b(const Batch& b, const float* layer, const float* cst, const double* dLdm, float* g, hipStream_t s);
 
+// the reference's ForwardTimeStep phases one by one (dmpm_phases.hip): FP64,
+// host-layout AoS arrays; g carries the grid (dim, G, dx, inv_dx, min_point),
+// n, dt, drag and f_ext (grid_op's gravity point when point_mode is set)
+void launch_phase_p2g(const StepParams& g, const double* x, const double* v, const double* C, const double* F, const double* dFc,
+                      const double* P, const double* m, const double* vol, double* gm, double* gp, hipStream_t s);
+void launch_phase_grid_op(const StepParams& g, const double* gm, const double* gp, double* gv, int point_mode, double mag,
+                          hipStream_t s);
+void launch_phase_g2p(const StepParams& g, const double* x, const double* gv, double* vn, double* Cn, hipStream_t s);
+void launch_phase_p_op2(int n, const double* x, const double* F, const double* dFc, const double* vn, const double* Cn, double sm,
+                        double dt, double* xn, double* Fn, hipStream_t s);
+void launch_phase_g2g_mass(const StepParams& g1, const StepParams& g2, const double* m1, double* m2, hipStream_t s);
+
 // batched constitutive evaluations (AoS 9 doubles per matrix)
 void launch_eval_pk1(int n, const double* F, const double* lam, const double* mu, double* P, hipStream_t s);
 void launch_eval_hess(int n, const double* F, const double* lam, const double* mu, const double* dF, double* out,

@@ -1794,37 +1794,91 @@ struct DevBuf {
     }
 };
 
+using HostIn = std::pair<const double*, size_t>;   // host array, element count
+using HostOut = std::pair<double*, size_t>;
+
+// Run launch(ptrs) on device 0 over device copies of host arrays: `ins` are
+// copied to the device, `inouts` to the device and back, `outs` back only.
+// ptrs lists the device copies in that order.
+int dev_call(const std::vector<HostIn>& ins, const std::vector<HostOut>& inouts, const std::vector<HostOut>& outs,
+             const std::function<void(std::vector<double*>&)>& launch)
+{
+    HIP_TRY(hipSetDevice(0));
+    std::vector<DevBuf> bufs(ins.size() + inouts.size() + outs.size());
+    std::vector<double*> ptrs;
+    size_t k = 0;
+    auto alloc = [&](size_t count, const char* what) -> int {
+        if (const int rc = dmalloc(&bufs[k].p, count * sizeof(double), what)) return rc;
+        ptrs.push_back(static_cast<double*>(bufs[k].p));
+        ++k;
+        return DMPM_OK;
+    };
+    for (const HostIn& in : ins) {
+        if (!in.first && in.second) return fail(DMPM_ERR_INVALID, "null input");
+        if (const int rc = alloc(in.second, "input")) return rc;
+        if (in.second) HIP_TRY(hipMemcpy(ptrs.back(), in.first, in.second * sizeof(double), hipMemcpyHostToDevice));
+    }
+    for (const HostOut& io : inouts) {
+        if (!io.first && io.second) return fail(DMPM_ERR_INVALID, "null input/output");
+        if (const int rc = alloc(io.second, "input/output")) return rc;
+        if (io.second) HIP_TRY(hipMemcpy(ptrs.back(), io.first, io.second * sizeof(double), hipMemcpyHostToDevice));
+    }
+    for (const HostOut& o : outs) {
+        if (!o.first && o.second) return fail(DMPM_ERR_INVALID, "null output");
+        if (const int rc = alloc(o.second, "output")) return rc;
+    }
+    launch(ptrs);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    k = ins.size();
+    for (const HostOut& io : inouts) {
+        if (io.second) HIP_TRY(hipMemcpy(io.first, ptrs[k], io.second * sizeof(double), hipMemcpyDeviceToHost));
+        ++k;
+    }
+    for (const HostOut& o : outs) {
+        if (o.second) HIP_TRY(hipMemcpy(o.first, ptrs[k], o.second * sizeof(double), hipMemcpyDeviceToHost));
+        ++k;
+    }
+    return DMPM_OK;
+}
+
+// n matrices: `per` doubles of each input and output per matrix
 int eval_common(int n, std::initializer_list<std::pair<const double*, size_t>> ins,
                 std::initializer_list<std::pair<double*, size_t>> outs,
                 const std::function<void(int, std::vector<double*>&)>& launch)
 {
     if (n < 0) return fail(DMPM_ERR_INVALID, "n < 0");
     if (n == 0) return DMPM_OK;
-    HIP_TRY(hipSetDevice(0));
-    std::vector<DevBuf> bufs(ins.size() + outs.size());
-    std::vector<double*> ptrs;
-    size_t k = 0;
+    std::vector<HostIn> vi;
+    std::vector<HostOut> vo;
     for (auto& in : ins) {
         if (!in.first) return fail(DMPM_ERR_INVALID, "null input");
-        if (const int rc = dmalloc(&bufs[k].p, in.second * n * sizeof(double), "evaluator input")) return rc;
-        HIP_TRY(hipMemcpy(bufs[k].p, in.first, in.second * n * sizeof(double), hipMemcpyHostToDevice));
-        ptrs.push_back(static_cast<double*>(bufs[k].p));
-        ++k;
+        vi.push_back({in.first, in.second * n});
     }
     for (auto& o : outs) {
         if (!o.first) return fail(DMPM_ERR_INVALID, "null output");
-        if (const int rc = dmalloc(&bufs[k].p, o.second * n * sizeof(double), "evaluator output")) return rc;
-        ptrs.push_back(static_cast<double*>(bufs[k].p));
-        ++k;
+        vo.push_back({o.first, o.second * n});
     }
-    launch(n, ptrs);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipDeviceSynchronize());
-    k = ins.size();
-    for (auto& o : outs) {
-        HIP_TRY(hipMemcpy(o.first, ptrs[k], o.second * n * sizeof(double), hipMemcpyDeviceToHost));
-        ++k;
+    return dev_call(vi, {}, vo, [&](std::vector<double*>& p) { launch(n, p); });
+}
+
+// A phase grid (dmpm_grid_desc) as step parameters: dims, node count, spacing, origin.
+int phase_grid(const dmpm_grid_desc* g, StepParams& sp)
+{
+    if (!g) return fail(DMPM_ERR_INVALID, "null grid");
+    sp = StepParams{};
+    long long G = 1;
+    for (int a = 0; a < 3; ++a) {
+        if (g->dim[a] <= 0) return fail(DMPM_ERR_INVALID, "grid dims must be > 0");
+        G *= g->dim[a];
+        sp.dim[a] = g->dim[a];
+        sp.min_point[a] = g->min_point[a];
     }
+    if (!(g->dx > 0.0)) return fail(DMPM_ERR_INVALID, "dx must be > 0");
+    if (G > (1LL << 34)) return fail(DMPM_ERR_INVALID, "grid too large");
+    sp.G = G;
+    sp.dx = g->dx;
+    sp.inv_dx = 1.0 / g->dx;
     return DMPM_OK;
 }
 
@@ -1859,6 +1913,102 @@ int dmpm_eval_svd(int32_t n, const double* F, double* U, double* sigma, double* 
     return eval_common(n, {{F, 9}}, {{U, 9}, {sigma, 3}, {V, 9}}, [](int n_, std::vector<double*>& p) {
         launch_eval_svd(n_, p[0], p[1], p[2], p[3], nullptr);
     });
+}
+
+// ------------------------------------------------------------- phases ------
+int dmpm_phase_p2g(const dmpm_grid_desc* g, int32_t n, const double* x, const double* v, const double* C, const double* F,
+                   const double* dFc, const double* P, const double* m, const double* vol, double dt, double drag, double* grid_m,
+                   double* grid_p)
+{
+    StepParams sp;
+    if (const int rc = phase_grid(g, sp)) return rc;
+    if (n < 0) return fail(DMPM_ERR_INVALID, "n < 0");
+    if (!grid_m || !grid_p) return fail(DMPM_ERR_INVALID, "null grid arrays");
+    if (n == 0) return DMPM_OK;
+    sp.n = n;
+    sp.dt = dt;
+    sp.drag = drag;
+    const size_t N = (size_t)n, G = (size_t)sp.G;
+    return dev_call({{x, 3 * N}, {v, 3 * N}, {C, 9 * N}, {F, 9 * N}, {dFc, 9 * N}, {P, 9 * N}, {m, N}, {vol, N}},
+                    {{grid_m, G}, {grid_p, 3 * G}}, {}, [&](std::vector<double*>& p) {
+                        launch_phase_p2g(sp, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], nullptr);
+                    });
+}
+
+static int phase_grid_op(const dmpm_grid_desc* g, const double* grid_m, const double* grid_p, double dt, const double* f3,
+                         int point_mode, double mag, double* grid_v)
+{
+    StepParams sp;
+    if (const int rc = phase_grid(g, sp)) return rc;
+    if (!f3 || !grid_m || !grid_p || !grid_v) return fail(DMPM_ERR_INVALID, "null argument");
+    sp.dt = dt;
+    for (int a = 0; a < 3; ++a) sp.f_ext[a] = f3[a];
+    const size_t G = (size_t)sp.G;
+    return dev_call({{grid_m, G}, {grid_p, 3 * G}}, {{grid_v, 3 * G}}, {}, [&](std::vector<double*>& p) {
+        launch_phase_grid_op(sp, p[0], p[1], p[2], point_mode, mag, nullptr);
+    });
+}
+
+int dmpm_phase_grid_op(const dmpm_grid_desc* g, const double* grid_m, const double* grid_p, double dt, const double* f_ext,
+                       double* grid_v)
+{
+    return phase_grid_op(g, grid_m, grid_p, dt, f_ext, 0, 0.0, grid_v);
+}
+
+int dmpm_phase_grid_op_point(const dmpm_grid_desc* g, const double* grid_m, const double* grid_p, double dt,
+                             const double* gravity_point, double gravity_mag, double* grid_v)
+{
+    return phase_grid_op(g, grid_m, grid_p, dt, gravity_point, 1, gravity_mag, grid_v);
+}
+
+int dmpm_phase_g2p(const dmpm_grid_desc* g, int32_t n, const double* x, const double* grid_v, double* v_next, double* C_next)
+{
+    StepParams sp;
+    if (const int rc = phase_grid(g, sp)) return rc;
+    if (n < 0) return fail(DMPM_ERR_INVALID, "n < 0");
+    if (!grid_v || !v_next || !C_next) return fail(DMPM_ERR_INVALID, "null argument");
+    if (n == 0) return DMPM_OK;
+    sp.n = n;
+    const size_t N = (size_t)n;
+    return dev_call({{x, 3 * N}, {grid_v, 3 * (size_t)sp.G}}, {}, {{v_next, 3 * N}, {C_next, 9 * N}},
+                    [&](std::vector<double*>& p) { launch_phase_g2p(sp, p[0], p[1], p[2], p[3], nullptr); });
+}
+
+int dmpm_phase_p_op2(int32_t n, const double* x, const double* F, const double* dFc, const double* v_next, const double* C_next,
+                     double smoothing, double dt, double* x_next, double* F_next)
+{
+    if (n < 0) return fail(DMPM_ERR_INVALID, "n < 0");
+    if (n == 0) return DMPM_OK;
+    if (!x_next || !F_next) return fail(DMPM_ERR_INVALID, "null output");
+    const size_t N = (size_t)n;
+    return dev_call({{x, 3 * N}, {F, 9 * N}, {dFc, 9 * N}, {v_next, 3 * N}, {C_next, 9 * N}}, {}, {{x_next, 3 * N}, {F_next, 9 * N}},
+                    [&](std::vector<double*>& p) {
+                        launch_phase_p_op2(n, p[0], p[1], p[2], p[3], p[4], smoothing, dt, p[5], p[6], nullptr);
+                    });
+}
+
+int dmpm_phase_p2g_mass(const dmpm_grid_desc* g, int32_t n, const double* x, const double* m, double* grid_m)
+{
+    StepParams sp;
+    if (const int rc = phase_grid(g, sp)) return rc;
+    if (n < 0) return fail(DMPM_ERR_INVALID, "n < 0");
+    if (!grid_m) return fail(DMPM_ERR_INVALID, "null grid array");
+    if (n == 0) return DMPM_OK;
+    sp.n = n;
+    const size_t N = (size_t)n;
+    const double org[3] = {0.0, 0.0, 0.0};
+    return dev_call({{x, 3 * N}, {m, N}}, {{grid_m, (size_t)sp.G}}, {},
+                    [&](std::vector<double*>& p) { launch_p2g_mass_aos<double>(sp, p[0], p[1], org, p[2], nullptr); });
+}
+
+int dmpm_phase_g2g_mass(const dmpm_grid_desc* g1, const double* grid1_m, const dmpm_grid_desc* g2, double* grid2_m)
+{
+    StepParams s1, s2;
+    if (const int rc = phase_grid(g1, s1)) return rc;
+    if (const int rc = phase_grid(g2, s2)) return rc;
+    if (!grid2_m) return fail(DMPM_ERR_INVALID, "null output");
+    return dev_call({{grid1_m, (size_t)s1.G}}, {}, {{grid2_m, (size_t)s2.G}},
+                    [&](std::vector<double*>& p) { launch_phase_g2g_mass(s1, s2, p[0], p[1], nullptr); });
 }
 
 }  // extern "C"

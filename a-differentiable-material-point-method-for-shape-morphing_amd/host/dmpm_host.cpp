@@ -8,10 +8,14 @@
 
 #include "dmpm.h"
 
+#include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <random>
+#include <stdexcept>
 
 namespace DiffMPMLib3D {
 
@@ -263,6 +267,57 @@ void PointCloud::Descend_GradientDescent(float alpha, float gradient_norm)
         for (int i = 0; i < 9; ++i) mp.dFc.d[i] -= alpha * (mp.dLdF.d[i] / gradient_norm);
 }
 
+// PointCloud.cpp:181-204: clipped direction (|g| in [1e-3, 1], sign kept) into
+// a heavy-ball momentum (0.9), dFc -= momentum
+void PointCloud::Descend_dLdF(float alpha, float gradient_norm)
+{
+    const float beta = 0.9f, lo = 1e-3f, hi = 1.0f;
+    for (MaterialPoint& mp : points)
+        for (int i = 0; i < 9; ++i) {
+            float g = mp.dLdF.d[i] / gradient_norm;
+            if (g > hi) g = hi;
+            else if (g < -hi) g = -hi;
+            else if (std::fabs(g) < lo) g = g < 0.f ? -lo : lo;
+            mp.momentum.d[i] = beta * mp.momentum.d[i] + alpha * g;
+            mp.dFc.d[i] -= mp.momentum.d[i];
+        }
+}
+
+// PointCloud.cpp:37-49: minibatch_size indices drawn without replacement, clock-seeded
+std::vector<int> PointCloud::generateMinibatchIndices(int total_size, int minibatch_size)
+{
+    std::vector<int> idx(total_size);
+    for (int i = 0; i < total_size; ++i) idx[i] = i;
+    std::mt19937 g((unsigned)std::chrono::system_clock::now().time_since_epoch().count());
+    std::shuffle(idx.begin(), idx.end(), g);
+    idx.resize(minibatch_size);
+    return idx;
+}
+
+// PointCloud.cpp:51-63 (accumulated in double)
+float PointCloud::Compute_dLdF_Norm_Stochastic(const std::vector<int>& minibatch_indices)
+{
+    double s = 0.0;
+    for (int i : minibatch_indices) s += points[i].dLdF.squaredNorm();
+    return (float)std::sqrt(s);
+}
+
+// PointCloud.cpp:65-81: Adam on the minibatch's points, bias corrections at timestep + 1
+void PointCloud::Descend_Adam_Stochastic(float alpha, float gradient_norm, float beta1, float beta2, float epsilon, int timestep,
+                                         const std::vector<int>& minibatch_indices)
+{
+    const float c1 = 1.0f - std::pow(beta1, (float)(timestep + 1)), c2 = 1.0f - std::pow(beta2, (float)(timestep + 1));
+    for (int p : minibatch_indices) {
+        MaterialPoint& mp = points[p];
+        for (int i = 0; i < 9; ++i) {
+            const float g = mp.dLdF.d[i] / gradient_norm;
+            mp.momentum.d[i] = beta1 * mp.momentum.d[i] + (1.0f - beta1) * g;
+            mp.vector.d[i] = beta2 * mp.vector.d[i] + (1.0f - beta2) * g * g;
+            mp.dFc.d[i] -= alpha * ((mp.momentum.d[i] / c1) / (std::sqrt(mp.vector.d[i] / c2) + epsilon));
+        }
+    }
+}
+
 void PointCloud::RemovePoint(size_t point_index) { points.erase(points.begin() + point_index); }
 
 bool PointCloud::IsEqualToOtherPointCloud(const PointCloud& other_pc)
@@ -429,6 +484,42 @@ Grid::Grid(int _dim_x, int _dim_y, int _dim_z, float _dx, Vec3 _min_point)
             for (int k = 0; k < dim_z; ++k) nodes[i][j][k].x = min_point + Vec3((float)i, (float)j, (float)k) * dx;
 }
 
+// the stencil of Grid.cpp:40-81 (the base clamped to +-2^20 cells before the
+// integer conversion, so that a far-away or non-finite point selects no node)
+template <typename Node, typename G>
+static std::vector<std::reference_wrapper<Node>> query_stencil(G& g, Vec3 point, std::vector<std::array<int, 3>>* indices)
+{
+    std::vector<std::reference_wrapper<Node>> ret;
+    int base[3];
+    const int dims[3] = {g.dim_x, g.dim_y, g.dim_z};
+    for (int a = 0; a < 3; ++a) {
+        double u = std::floor(((double)point[a] - (double)g.min_point[a]) / (double)g.dx);
+        if (!(u >= -1048576.0)) u = -1048576.0;  // also NaN
+        if (u > 1048576.0) u = 1048576.0;
+        base[a] = (int)u - 1;
+    }
+    for (int i = 0; i <= 3; ++i)
+        for (int j = 0; j <= 3; ++j)
+            for (int k = 0; k <= 3; ++k) {
+                const int ix[3] = {base[0] + i, base[1] + j, base[2] + k};
+                if (ix[0] < 0 || ix[0] >= dims[0] || ix[1] < 0 || ix[1] >= dims[1] || ix[2] < 0 || ix[2] >= dims[2]) continue;
+                ret.push_back(std::ref(g.nodes[ix[0]][ix[1]][ix[2]]));
+                if (indices) indices->push_back({ix[0], ix[1], ix[2]});
+            }
+    return ret;
+}
+
+std::vector<std::reference_wrapper<GridNode>> Grid::QueryPoint_CubicBSpline(Vec3 point, std::vector<std::array<int, 3>>* indices)
+{
+    return query_stencil<GridNode>(*this, point, indices);
+}
+
+std::vector<std::reference_wrapper<const GridNode>> Grid::QueryPointConst_CubicBSpline(Vec3 point,
+                                                                                      std::vector<std::array<int, 3>>* indices) const
+{
+    return query_stencil<const GridNode>(*this, point, indices);
+}
+
 std::vector<GridNode> Grid::flatten() const
 {
     std::vector<GridNode> r;
@@ -562,24 +653,258 @@ void CalculatePointCloudVolumes(PointCloud& pc, Grid& grid)
     for (size_t p = 0; p < n; ++p) pc.points[p].vol = (float)vol[p];
 }
 
-// MPMPolyscope.cpp:156-160: grid.m <- P2G mass of the point cloud
-void P2G_Mass(const PointCloud& pc, Grid& grid)
+// ----------------------------------------------------------- phases -------
+// Each phase is one GPU call (dmpm_phase_*, FP64) on the containers' values.
+namespace {
+
+dmpm_grid_desc desc_of(const Grid& g)
+{
+    dmpm_grid_desc d{};
+    d.dim[0] = g.dim_x;
+    d.dim[1] = g.dim_y;
+    d.dim[2] = g.dim_z;
+    d.dx = g.dx;
+    for (int a = 0; a < 3; ++a) d.min_point[a] = g.min_point[a];
+    return d;
+}
+
+// node fields in node order (i dy dz + j dz + k): m [G], p and v [G][3]
+struct GridArrays {
+    std::vector<double> m, p, v;
+};
+
+GridArrays grid_arrays(const Grid& g)
+{
+    GridArrays r;
+    const size_t G = (size_t)g.dim_x * g.dim_y * g.dim_z;
+    r.m.resize(G);
+    r.p.resize(3 * G);
+    r.v.resize(3 * G);
+    size_t q = 0;
+    for (int i = 0; i < g.dim_x; ++i)
+        for (int j = 0; j < g.dim_y; ++j)
+            for (int k = 0; k < g.dim_z; ++k, ++q) {
+                const GridNode& nd = g.nodes[i][j][k];
+                r.m[q] = nd.m;
+                for (int a = 0; a < 3; ++a) {
+                    r.p[3 * q + a] = nd.p[a];
+                    r.v[3 * q + a] = nd.v[a];
+                }
+            }
+    return r;
+}
+
+// write node fields back (null: leave that field)
+void put_grid(Grid& g, const std::vector<double>* m, const std::vector<double>* p, const std::vector<double>* v)
+{
+    size_t q = 0;
+    for (int i = 0; i < g.dim_x; ++i)
+        for (int j = 0; j < g.dim_y; ++j)
+            for (int k = 0; k < g.dim_z; ++k, ++q) {
+                GridNode& nd = g.nodes[i][j][k];
+                if (m) nd.m = (float)(*m)[q];
+                for (int a = 0; a < 3; ++a) {
+                    if (p) nd.p[a] = (float)(*p)[3 * q + a];
+                    if (v) nd.v[a] = (float)(*v)[3 * q + a];
+                }
+            }
+}
+
+PointCloud cloud_of(const MaterialPoint& mp)
+{
+    PointCloud pc;
+    pc.points.push_back(mp);
+    return pc;
+}
+
+}  // namespace
+
+// ForwardSimulation.cpp:6-9,159-167: P = PK1(F + dFc, lam, mu) per particle
+void P_op_1(PointCloud& pc)
 {
     const size_t n = pc.points.size();
-    Ctx ctx(make_config(1, grid, 2, 0.f, 0.f, 0.f, Vec3(), false));
+    if (n == 0) return;
+    std::vector<double> Fc(9 * n), lam(n), mu(n), P(9 * n);
+    for (size_t p = 0; p < n; ++p) {
+        const MaterialPoint& mp = pc.points[p];
+        for (int i = 0; i < 9; ++i) Fc[9 * p + i] = (double)mp.F.d[i] + (double)mp.dFc.d[i];
+        lam[p] = mp.lam;
+        mu[p] = mp.mu;
+    }
+    check(dmpm_eval_pk1((int32_t)n, Fc.data(), lam.data(), mu.data(), P.data()), "dmpm_eval_pk1");
+    for (size_t p = 0; p < n; ++p)
+        for (int i = 0; i < 9; ++i) pc.points[p].P.d[i] = (float)P[9 * p + i];
+}
+
+void SingleParticle_op_1(MaterialPoint& mp) { mp.P = PK_FixedCorotatedElasticity(mp.F + mp.dFc, mp.lam, mp.mu); }
+
+// ForwardSimulation.cpp:169-172 -> Grid.cpp:349-361
+void G_Reset(Grid& grid) { grid.ResetValues(); }
+
+// ForwardSimulation.cpp:11-51,174-182: accumulates m, p into the grid
+void P2G(const PointCloud& pc, Grid& grid, float dt, float drag)
+{
+    const size_t n = pc.points.size();
+    if (n == 0) return;
+    const Staging s = stage(pc);
+    std::vector<double> P(9 * n);
+    for (size_t p = 0; p < n; ++p)
+        for (int i = 0; i < 9; ++i) P[9 * p + i] = pc.points[p].P.d[i];
+    GridArrays ga = grid_arrays(grid);
+    const dmpm_grid_desc d = desc_of(grid);
+    check(dmpm_phase_p2g(&d, (int32_t)n, s.x.data(), s.v.data(), s.C.data(), s.F.data(), s.dFc.data(), P.data(), s.m.data(),
+                         s.vol.data(), dt, drag, ga.m.data(), ga.p.data()),
+          "dmpm_phase_p2g");
+    put_grid(grid, &ga.m, &ga.p, nullptr);
+}
+
+void SingleParticle_to_grid(const MaterialPoint& mp, Grid& grid, float dt, float drag) { P2G(cloud_of(mp), grid, dt, drag); }
+
+// ForwardSimulation.cpp:53-58,184-200
+void G_op(Grid& grid, float dt, Vec3 f_ext)
+{
+    GridArrays ga = grid_arrays(grid);
+    const dmpm_grid_desc d = desc_of(grid);
+    const double f[3] = {f_ext[0], f_ext[1], f_ext[2]};
+    check(dmpm_phase_grid_op(&d, ga.m.data(), ga.p.data(), dt, f, ga.v.data()), "dmpm_phase_grid_op");
+    put_grid(grid, nullptr, nullptr, &ga.v);
+}
+
+// ForwardSimulation.cpp:202-224
+void G_op(Grid& grid, float dt, Vec3 gravity_point, float gravity_mag)
+{
+    GridArrays ga = grid_arrays(grid);
+    const dmpm_grid_desc d = desc_of(grid);
+    const double gp[3] = {gravity_point[0], gravity_point[1], gravity_point[2]};
+    check(dmpm_phase_grid_op_point(&d, ga.m.data(), ga.p.data(), dt, gp, gravity_mag, ga.v.data()), "dmpm_phase_grid_op_point");
+    put_grid(grid, nullptr, nullptr, &ga.v);
+}
+
+// ForwardSimulation.cpp:53-58 on one node (a one-node grid at the node's position)
+void SingleNode_op(GridNode& node, float dt, Vec3 f_ext)
+{
+    dmpm_grid_desc d{};
+    d.dim[0] = d.dim[1] = d.dim[2] = 1;
+    d.dx = 1.0;
+    for (int a = 0; a < 3; ++a) d.min_point[a] = node.x[a];
+    double m = node.m, p[3], v[3];
+    const double f[3] = {f_ext[0], f_ext[1], f_ext[2]};
+    for (int a = 0; a < 3; ++a) {
+        p[a] = node.p[a];
+        v[a] = node.v[a];
+    }
+    check(dmpm_phase_grid_op(&d, &m, p, dt, f, v), "dmpm_phase_grid_op");
+    for (int a = 0; a < 3; ++a) node.v[a] = (float)v[a];
+}
+
+// ForwardSimulation.cpp:60-106,226-236: next's v and C from the grid velocities
+static void g2p_impl(PointCloud& next, const PointCloud& curr, const Grid& grid)
+{
+    const size_t n = curr.points.size();
+    if (next.points.size() != n) next.points = curr.points;
+    if (n == 0) return;
+    std::vector<double> x(3 * n), vn(3 * n), Cn(9 * n);
+    for (size_t p = 0; p < n; ++p)
+        for (int a = 0; a < 3; ++a) x[3 * p + a] = curr.points[p].x[a];
+    const GridArrays ga = grid_arrays(grid);
+    const dmpm_grid_desc d = desc_of(grid);
+    check(dmpm_phase_g2p(&d, (int32_t)n, x.data(), ga.v.data(), vn.data(), Cn.data()), "dmpm_phase_g2p");
+    for (size_t p = 0; p < n; ++p) {
+        MaterialPoint& mp = next.points[p];
+        for (int a = 0; a < 3; ++a) mp.v[a] = (float)vn[3 * p + a];
+        for (int i = 0; i < 9; ++i) mp.C.d[i] = (float)Cn[9 * p + i];
+    }
+}
+
+void G2P(PointCloud& next, const PointCloud& curr, Grid& grid) { g2p_impl(next, curr, grid); }
+
+void Grid_to_SingleParticle(MaterialPoint& next_mp, const MaterialPoint& curr_mp, const Grid& grid)
+{
+    PointCloud nx = cloud_of(next_mp);
+    g2p_impl(nx, cloud_of(curr_mp), grid);
+    next_mp.v = nx.points[0].v;
+    next_mp.C = nx.points[0].C;
+}
+
+// ForwardSimulation.cpp:108-124,238-259: next's F and x from its v and C
+void P_op_2(PointCloud& next, const PointCloud& curr, float smoothing_factor, float dt)
+{
+    const size_t n = curr.points.size();
+    if (next.points.size() != n) throw std::invalid_argument("P_op_2: next and curr differ in size");
+    if (n == 0) return;
+    const Staging s = stage(curr);
+    std::vector<double> vn(3 * n), Cn(9 * n), xn(3 * n), Fn(9 * n);
+    for (size_t p = 0; p < n; ++p) {
+        for (int a = 0; a < 3; ++a) vn[3 * p + a] = next.points[p].v[a];
+        for (int i = 0; i < 9; ++i) Cn[9 * p + i] = next.points[p].C.d[i];
+    }
+    check(dmpm_phase_p_op2((int32_t)n, s.x.data(), s.F.data(), s.dFc.data(), vn.data(), Cn.data(), smoothing_factor, dt, xn.data(),
+                           Fn.data()),
+          "dmpm_phase_p_op2");
+    for (size_t p = 0; p < n; ++p) {
+        MaterialPoint& mp = next.points[p];
+        for (int a = 0; a < 3; ++a) mp.x[a] = (float)xn[3 * p + a];
+        for (int i = 0; i < 9; ++i) mp.F.d[i] = (float)Fn[9 * p + i];
+    }
+}
+
+void P_op_2(PointCloud& next, const PointCloud& curr, float dt) { P_op_2(next, curr, 0.f, dt); }
+
+void SingleParticle_op_2(MaterialPoint& next_mp, const MaterialPoint& curr_mp, float smoothing_factor, float dt)
+{
+    PointCloud nx = cloud_of(next_mp);
+    P_op_2(nx, cloud_of(curr_mp), smoothing_factor, dt);
+    next_mp.x = nx.points[0].x;
+    next_mp.F = nx.points[0].F;
+}
+
+void SingleParticle_op_2(MaterialPoint& next_mp, const MaterialPoint& curr_mp, float dt) { SingleParticle_op_2(next_mp, curr_mp, 0.f, dt); }
+
+// ForwardSimulation.cpp:114-116
+void smooth_deformation_gradient(MaterialPoint& next_mp, const MaterialPoint& curr_mp, float smoothing_factor)
+{
+    next_mp.F = next_mp.F * (1.0f - smoothing_factor) + curr_mp.F * smoothing_factor;
+}
+
+// ForwardSimulation.cpp:292-327: grid.m <- sum w mp_m over the points (the
+// other node fields are left as they are)
+void P2G_Mass(const std::vector<Vec3> points, Grid& grid, float mp_m)
+{
+    const size_t n = points.size();
+    std::vector<double> x(3 * n), m(n, (double)mp_m);
+    for (size_t p = 0; p < n; ++p)
+        for (int a = 0; a < 3; ++a) x[3 * p + a] = points[p][a];
+    std::vector<double> gm((size_t)grid.dim_x * grid.dim_y * grid.dim_z, 0.0);
+    const dmpm_grid_desc d = desc_of(grid);
+    check(dmpm_phase_p2g_mass(&d, (int32_t)n, x.data(), m.data(), gm.data()), "dmpm_phase_p2g_mass");
+    put_grid(grid, &gm, nullptr, nullptr);
+}
+
+// ForwardSimulation.cpp:329-367: grid_2.m <- masses of grid_1 gathered at grid_2's nodes
+void G2G_Mass(Grid& grid_1, Grid& grid_2)
+{
+    const GridArrays g1 = grid_arrays(grid_1);
+    std::vector<double> m2((size_t)grid_2.dim_x * grid_2.dim_y * grid_2.dim_z);
+    const dmpm_grid_desc d1 = desc_of(grid_1), d2 = desc_of(grid_2);
+    check(dmpm_phase_g2g_mass(&d1, g1.m.data(), &d2, m2.data()), "dmpm_phase_g2g_mass");
+    put_grid(grid_2, &m2, nullptr, nullptr);
+}
+
+// grid.m <- P2G mass of the point cloud's own masses, the other node fields cleared
+// (MPMPolyscope.cpp:156-160 in one call)
+void P2G_PointCloudMass(const PointCloud& pc, Grid& grid)
+{
+    const size_t n = pc.points.size();
     std::vector<double> x(3 * n), m(n);
     for (size_t p = 0; p < n; ++p) {
         for (int a = 0; a < 3; ++a) x[3 * p + a] = pc.points[p].x[a];
         m[p] = pc.points[p].m;
     }
-    check(dmpm_target_from_points(ctx.c, x.data(), m.data(), (int32_t)n), "dmpm_target_from_points");
-    std::vector<double> gm((size_t)grid.dim_x * grid.dim_y * grid.dim_z);
-    check(dmpm_get_target_grid(ctx.c, gm.data()), "dmpm_get_target_grid");
     grid.ResetValues();
-    size_t k = 0;
-    for (int i = 0; i < grid.dim_x; ++i)
-        for (int j = 0; j < grid.dim_y; ++j)
-            for (int l = 0; l < grid.dim_z; ++l) grid.nodes[i][j][l].m = (float)gm[k++];
+    std::vector<double> gm((size_t)grid.dim_x * grid.dim_y * grid.dim_z, 0.0);
+    const dmpm_grid_desc d = desc_of(grid);
+    check(dmpm_phase_p2g_mass(&d, (int32_t)n, x.data(), m.data(), gm.data()), "dmpm_phase_p2g_mass");
+    put_grid(grid, &gm, nullptr, nullptr);
 }
 
 }  // namespace SingleThreadMPM

@@ -15,6 +15,7 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -192,6 +193,11 @@ struct PointCloud {
     void Descend_AMSGrad(float alpha, float gradient_norm, float beta1, float beta2, float epsilon, int timestep);
     void Descend_GradientDescent(float alpha, float gradient_norm);
     void Descend_Lion(float alpha, float beta, float gradient_norm);
+    void Descend_dLdF(float alpha, float gradient_norm);
+    std::vector<int> generateMinibatchIndices(int total_size, int minibatch_size);
+    float Compute_dLdF_Norm_Stochastic(const std::vector<int>& minibatch_indices);
+    void Descend_Adam_Stochastic(float alpha, float gradient_norm, float beta1, float beta2, float epsilon, int timestep,
+                                 const std::vector<int>& minibatch_indices);
     inline float SignFloat(float x) { return (x > 0.f) ? 1.f : ((x < 0.f) ? -1.f : 0.f); }
     void RemovePoint(size_t point_index);
 
@@ -230,6 +236,14 @@ struct GridNode {
 struct Grid {
     Grid(int _dim_x, int _dim_y, int dim_z, float _dx, Vec3 _min_point);
     Grid(const Grid& grid) = default;
+
+    // Grid.cpp:40-81: the grid nodes of a point's 4^3 cubic B-spline stencil (base
+    // floor((x - min)/dx) - 1, x-major), skipping those outside the grid; their
+    // (i, j, k) are appended to indices when given
+    std::vector<std::reference_wrapper<GridNode>> QueryPoint_CubicBSpline(Vec3 point,
+                                                                          std::vector<std::array<int, 3>>* indices = nullptr);
+    std::vector<std::reference_wrapper<const GridNode>> QueryPointConst_CubicBSpline(
+        Vec3 point, std::vector<std::array<int, 3>>* indices = nullptr) const;
 
     std::vector<GridNode> flatten() const;
     void unflatten(const std::vector<GridNode>& flat_nodes);
@@ -297,14 +311,43 @@ class CompGraph {
 };
 
 // ------------------------------------------------------ ForwardSimulation ----
-// ForwardSimulation.h:7-45 (the whole step and the volume utility)
+// ForwardSimulation.h:7-45.  ForwardTimeStep runs the fused step on the GPU;
+// the phase functions run each phase as its own GPU call on these containers
+// (FP64, device 0), with the reference's boundaries: P2G reads the stored P
+// and accumulates into the grid, G_op leaves nodes at or below the mass
+// threshold untouched, G2P writes v and C of next, P_op_2 its F and x.  The
+// single-particle and single-node forms make one GPU call each.
 namespace SingleThreadMPM {
+void SingleParticle_op_1(MaterialPoint& mp);
+void SingleParticle_to_grid(const MaterialPoint& mp, Grid& grid, float dt, float drag);
+void SingleNode_op(GridNode& node, float dt, Vec3 f_ext);
+void Grid_to_SingleParticle(MaterialPoint& next_timestep_mp, const MaterialPoint& curr_timestep_mp, const Grid& grid);
+void SingleParticle_op_2(MaterialPoint& next_timestep_mp, const MaterialPoint& curr_timestep_mp, float dt);
+void SingleParticle_op_2(MaterialPoint& next_timestep_mp, const MaterialPoint& curr_timestep_mp, float smoothing_factor, float dt);
+
 void ForwardTimeStep(PointCloud& next_point_cloud, PointCloud& curr_point_cloud, Grid& grid, float dt, float drag, Vec3 f_ext);
 void ForwardTimeStep(PointCloud& next_point_cloud, PointCloud& curr_point_cloud, Grid& grid, float smoothing_factor,
                      float dt, float drag, Vec3 f_ext);
+
+// F <- F (1 - s) + F_curr s (ForwardSimulation.cpp:114-116), host arithmetic on one particle
+void smooth_deformation_gradient(MaterialPoint& next_timestep_mp, const MaterialPoint& curr_timestep_mp, float smoothing_factor);
+
+void P_op_1(PointCloud& curr_point_cloud);
+void G_Reset(Grid& grid);
+void P2G(const PointCloud& curr_point_cloud, Grid& grid, float dt, float drag);
+void G_op(Grid& grid, float dt, Vec3 f_ext);
+void G_op(Grid& grid, float dt, Vec3 gravity_point, float gravity_mag);
+void G2P(PointCloud& next_point_cloud, const PointCloud& curr_point_cloud, Grid& grid);
+void P_op_2(PointCloud& next_point_cloud, const PointCloud& curr_point_cloud, float dt);
+void P_op_2(PointCloud& next_point_cloud, const PointCloud& curr_point_cloud, float smoothing_factor, float dt);
+
 void CalculatePointCloudVolumes(PointCloud& curr_point_cloud, Grid& grid);
-// P2G of the point masses only into a grid (the target-grid construction, MPMPolyscope.cpp:156-160)
-void P2G_Mass(const PointCloud& point_cloud, Grid& grid);
+void P2G_Mass(const std::vector<Vec3> points, Grid& grid, float mp_m);
+void G2G_Mass(Grid& grid_1, Grid& grid_2);
+
+// not in the reference: grid.m <- P2G mass of a point cloud's own masses, the other node
+// fields cleared (the target-grid construction of MPMPolyscope.cpp:156-160 in one call)
+void P2G_PointCloudMass(const PointCloud& point_cloud, Grid& grid);
 }  // namespace SingleThreadMPM
 
 // ----------------------------------------- MultiThreadForwardSimulation ----

@@ -219,6 +219,44 @@ int dmpm_eval_svd(int32_t n, const double* F, double* U, double* sigma, double* 
  * + lam/2 (J - 1)^2, J = max(det F, 1e-6), FP64. */
 int dmpm_eval_energy(int32_t n, const double* F, const double* lam, const double* mu, double* psi);
 
+/* ------------------------------------------------------------------------------------------
+ * The phases of one ForwardTimeStep as separate calls (SingleThreadMPM, ForwardSimulation.h:23-45),
+ * for callers that run them one by one on their own containers: FP64 on device 0, host arrays in
+ * the reference's per-particle layout (x, v [n][3]; C, F, dFc, P [n][3][3]) and grids as masses [G]
+ * and momenta / velocities [G][3] (node order and positions as above).  The episode calls above
+ * fuse these phases and keep the data on the device; these copy their arrays in and out. */
+typedef struct dmpm_grid_desc {
+    int32_t dim[3];      /* Grid::dim_x, dim_y, dim_z */
+    int32_t reserved;
+    double dx;           /* Grid::dx */
+    double min_point[3]; /* Grid::min_point */
+} dmpm_grid_desc;
+/* SingleThreadMPM::P2G (ForwardSimulation.cpp:11-51,174-182): adds each particle's w m to grid_m and
+ * w [m (1 - dt drag) v + (-(3/dx^2) dt vol P (F + dFc)^T + m C) d] to grid_p (accumulated: G_Reset
+ * clears them); P is the particle's stored stress (P_op_1). */
+int dmpm_phase_p2g(const dmpm_grid_desc* g, int32_t n, const double* x, const double* v, const double* C, const double* F,
+                   const double* dFc, const double* P, const double* m, const double* vol, double dt, double drag, double* grid_m,
+                   double* grid_p);
+/* SingleThreadMPM::G_op (ForwardSimulation.cpp:53-58,184-200): grid_v = p/m + dt f_ext where
+ * |m| > FLT_EPSILON; the other nodes keep grid_v. */
+int dmpm_phase_grid_op(const dmpm_grid_desc* g, const double* grid_m, const double* grid_p, double dt, const double* f_ext,
+                       double* grid_v);
+/* SingleThreadMPM::G_op with a gravity point (ForwardSimulation.cpp:202-224): the force on a node is
+ * gravity_mag towards gravity_point (zero at the point). */
+int dmpm_phase_grid_op_point(const dmpm_grid_desc* g, const double* grid_m, const double* grid_p, double dt,
+                             const double* gravity_point, double gravity_mag, double* grid_v);
+/* SingleThreadMPM::G2P (ForwardSimulation.cpp:60-106,226-236): v' = sum w v_i, C' = (3/dx^2) sum w v_i d^T. */
+int dmpm_phase_g2p(const dmpm_grid_desc* g, int32_t n, const double* x, const double* grid_v, double* v_next, double* C_next);
+/* SingleThreadMPM::P_op_2 (ForwardSimulation.cpp:108-124,238-259): F' = (1 - s)(I + dt C')(F + dFc) + s F,
+ * x' = x + dt v' (s = 0: the overload without smoothing). */
+int dmpm_phase_p_op2(int32_t n, const double* x, const double* F, const double* dFc, const double* v_next, const double* C_next,
+                     double smoothing, double dt, double* x_next, double* F_next);
+/* SingleThreadMPM::P2G_Mass (ForwardSimulation.cpp:292-327): adds each point's w m to grid_m. */
+int dmpm_phase_p2g_mass(const dmpm_grid_desc* g, int32_t n, const double* x, const double* m, double* grid_m);
+/* SingleThreadMPM::G2G_Mass (ForwardSimulation.cpp:329-367): every node of grid 2 gets the sum w m_i of
+ * the grid-1 node masses around its position. */
+int dmpm_phase_g2g_mass(const dmpm_grid_desc* g1, const double* grid1_m, const dmpm_grid_desc* g2, double* grid2_m);
+
 #ifdef __cplusplus
 }
 #endif

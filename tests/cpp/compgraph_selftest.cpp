@@ -4,6 +4,7 @@
 //   compgraph_selftest roundtrip IN.mpmbin OUT.mpmbin      (no GPU)
 //   compgraph_selftest run DIR                             (GPU)
 //   compgraph_selftest optimize DIR                        (GPU)
+//   compgraph_selftest phases DIR                          (GPU)
 // DIR holds in.mpmbin (layer 0, dFc set), target.bin (float64 node masses),
 // params.txt ("dims dx minx miny minz dt drag s fx fy fz layers compat");
 // outputs: loss.txt, last.mpmbin (last layer), l0.mpmbin (layer 0 with its
@@ -14,7 +15,9 @@
 // pk1.txt (PK_FixedCorotatedElasticity, then FixedCorotatedElasticity, at
 // F = 0.8 Rx(45 deg)), diag.txt (GetPointDeterminants, GetPointElasticEnergies
 // of layer 0, one point a line); optimize: history.txt (gradient norms of the
-// optimisation), final_loss.txt, opt_l0.mpmbin (layer 0 with its optimised dFc).
+// optimisation), final_loss.txt, opt_l0.mpmbin (layer 0 with its optimised dFc);
+// phases: the ForwardSimulation.h phase functions as reference callers use them
+// (outputs listed at run_phases).
 #include "DiffMPMLib3D/DiffMPMLib3D.h"
 
 #include <cmath>
@@ -25,6 +28,88 @@
 
 using namespace DiffMPMLib3D;
 
+namespace {
+
+void write_grid(const std::string& path, const Grid& g)
+{
+    std::ofstream o(path);
+    o.precision(9);
+    for (int i = 0; i < g.dim_x; ++i)
+        for (int j = 0; j < g.dim_y; ++j)
+            for (int k = 0; k < g.dim_z; ++k) {
+                const GridNode& nd = g.nodes[i][j][k];
+                o << nd.m << " " << nd.p[0] << " " << nd.p[1] << " " << nd.p[2] << " " << nd.v[0] << " " << nd.v[1] << " " << nd.v[2]
+                  << "\n";
+            }
+}
+
+// The phase functions of ForwardSimulation.h, called as the reference's callers do:
+//   phases.mpmbin, phases_grid.txt  P_op_1, G_Reset, P2G, G_op, G2P, P_op_2 of layer 0
+//                                   (ForwardSimulation.cpp:136-157 spelled out); grid lines "m p v"
+//   target_grid.txt                 Grid copy, ResetValues, P2G(cloud, grid, 0.0, 0.0)
+//                                   (MPMPolyscope.cpp:158-160)
+//   p2g_mass.txt, g2g_mass.txt      P2G_Mass(positions, grid, 1.0), then G2G_Mass onto a
+//                                   grid of (dims + 1) / 2 nodes at 2 dx (DiffMPMAnimator3D.cpp:501-505)
+//   gravity_grid.txt                G_op with a gravity point at the grid centre, magnitude 3
+//   single.mpmbin, single_grid.txt  the single-particle forms on particles 0..4: op_1 and
+//                                   to_grid into a reset grid, SingleNode_op on every node,
+//                                   Grid_to_SingleParticle and op_2 (smoothing form) from the
+//                                   phases' grid
+void run_phases(const std::string& dir, PointCloud& pc, const Grid& grid, float dt, float drag, float s, Vec3 fe)
+{
+    PointCloud curr = pc, next = pc;
+    Grid g(grid);
+    SingleThreadMPM::P_op_1(curr);
+    SingleThreadMPM::G_Reset(g);
+    SingleThreadMPM::P2G(curr, g, dt, drag);
+    SingleThreadMPM::G_op(g, dt, fe);
+    SingleThreadMPM::G2P(next, curr, g);
+    SingleThreadMPM::P_op_2(next, curr, s, dt);
+    next.WriteEntirePointCloudToBinaryFile(dir + "/phases.mpmbin");
+    write_grid(dir + "/phases_grid.txt", g);
+
+    auto target_point_cloud = std::make_shared<PointCloud>(pc);
+    auto target_grid = std::make_shared<Grid>(grid);
+    target_grid->ResetValues();
+    SingleThreadMPM::P2G(*target_point_cloud, *target_grid, 0.0, 0.0);
+    write_grid(dir + "/target_grid.txt", *target_grid);
+
+    Grid mg(grid);
+    SingleThreadMPM::P2G_Mass(pc.GetPointPositions(), mg, 1.0);
+    write_grid(dir + "/p2g_mass.txt", mg);
+    Grid coarse((grid.dim_x + 1) / 2, (grid.dim_y + 1) / 2, (grid.dim_z + 1) / 2, 2.f * grid.dx, grid.min_point);
+    SingleThreadMPM::G2G_Mass(mg, coarse);
+    write_grid(dir + "/g2g_mass.txt", coarse);
+
+    Grid gg(g);
+    const Vec3 centre = grid.min_point + Vec3((float)grid.dim_x, (float)grid.dim_y, (float)grid.dim_z) * (0.5f * grid.dx);
+    SingleThreadMPM::G_op(gg, dt, centre, 3.0f);
+    write_grid(dir + "/gravity_grid.txt", gg);
+
+    PointCloud single;
+    Grid sg(grid);
+    sg.ResetValues();
+    for (int p = 0; p < 5; ++p) {
+        MaterialPoint mp = pc.points[p];
+        SingleThreadMPM::SingleParticle_op_1(mp);
+        SingleThreadMPM::SingleParticle_to_grid(mp, sg, dt, drag);
+        MaterialPoint nx = next.points[p];
+        nx.v.setZero();
+        nx.C.setZero();
+        SingleThreadMPM::Grid_to_SingleParticle(nx, curr.points[p], g);
+        SingleThreadMPM::SingleParticle_op_2(nx, curr.points[p], s, dt);
+        single.points.push_back(nx);
+    }
+    for (int i = 0; i < sg.dim_x; ++i)
+        for (int j = 0; j < sg.dim_y; ++j)
+            for (int k = 0; k < sg.dim_z; ++k)
+                if (sg.nodes[i][j][k].m != 0.f) SingleThreadMPM::SingleNode_op(sg.nodes[i][j][k], dt, fe);
+    single.WriteEntirePointCloudToBinaryFile(dir + "/single.mpmbin");
+    write_grid(dir + "/single_grid.txt", sg);
+}
+
+}  // namespace
+
 int main(int argc, char** argv)
 {
     if (argc == 4 && std::string(argv[1]) == "roundtrip") {
@@ -34,8 +119,8 @@ int main(int argc, char** argv)
         return 0;
     }
     const std::string mode = argc == 3 ? std::string(argv[1]) : std::string();
-    if (mode != "run" && mode != "optimize") {
-        std::cerr << "usage: compgraph_selftest roundtrip IN OUT | run DIR | optimize DIR\n";
+    if (mode != "run" && mode != "optimize" && mode != "phases") {
+        std::cerr << "usage: compgraph_selftest roundtrip IN OUT | run DIR | optimize DIR | phases DIR\n";
         return 1;
     }
     const std::string dir = argv[2];
@@ -66,6 +151,10 @@ int main(int argc, char** argv)
         }
         const Vec3 fe(f[0], f[1], f[2]);
 
+        if (mode == "phases") {
+            run_phases(dir, *pc, *grid, dt, drag, s, fe);
+            return 0;
+        }
         if (mode == "optimize") {
             // the GUI's optimisation call (DiffMPMAnimator3D.cpp:436-447) on a short
             // sequence: control stride 1, 2 descent iterations with 3 line-search

@@ -17,6 +17,7 @@ dim3 blockIdx, threadIdx, blockDim, gridDim;
 #define launch_sum_rows launch_sum_rows_device
 #define launch_sumsq launch_sumsq_device
 #include "dmpm_kernels.hip"
+#include "dmpm_phases.hip"
 #undef launch_loss
 #undef launch_sum_rows
 #undef launch_sumsq

@@ -55,25 +55,122 @@ REFERENCE_HEADERS = ["BackPropagation.h", "CompGraph.h", "Elasticity.h", "Forwar
 
 
 def test_dropin_headers_compile_and_link(tmp_path):
-    """Every header name of the reference's DiffMPMLib3D that callers include compiles, and the
-    free functions of the step keep the reference's signatures (taken as function pointers of
-    those exact types) and resolve in libdmpm_cpp.so."""
+    """Every header name of the reference's DiffMPMLib3D that callers include compiles; every
+    function the reference declares for the path (ForwardSimulation.h:10-43 with the phase
+    API, BackPropagation.h:7, Elasticity.h:8-23 without the Tensor3x3x3x3 test aids, the
+    CompGraph and PointCloud members) is taken as a pointer of its exact reference type and
+    resolves in libdmpm_cpp.so; and the reference callers of the phase API compile as written
+    (MPMPolyscope.cpp:158-160, MPMPointCloudVisualization.cpp:11-12, DiffMPMAnimator3D.cpp:501-505,
+    CompGraph.cpp:134-135)."""
     import subprocess
 
     inc = os.path.join(ROOT, "include")
     lib = os.path.join(PKG_DIR, "lib")
     src = tmp_path / "t.cpp"
     src.write_text("".join(f'#include "{h}"\n' for h in REFERENCE_HEADERS) + """
+#include <memory>
+#include <vector>
 using namespace DiffMPMLib3D;
+template <typename T> bool ok(T p) { return p != nullptr; }
 int main()
 {
-    void (*mt)(PointCloud&, PointCloud&, Grid&, std::vector<std::shared_ptr<Grid>>&, float, float, Vec3) =
-        &MultiThreadMPM::ForwardTimeStep;
-    void (*st)(PointCloud&, PointCloud&, Grid&, float, float, float, Vec3) = &SingleThreadMPM::ForwardTimeStep;
-    void (*st0)(PointCloud&, PointCloud&, Grid&, float, float, Vec3) = &SingleThreadMPM::ForwardTimeStep;
-    void (*bt)(CompGraphLayer&, CompGraphLayer&, float, float, float) = &Back_Timestep;
-    Mat3 (*pk)(const Mat3&, float, float) = &PK_FixedCorotatedElasticity;
-    return (mt && st && st0 && bt && pk) ? 0 : 1;
+    namespace S = SingleThreadMPM;
+    bool all = true;
+    // ForwardSimulation.h
+    all &= ok<void (*)(MaterialPoint&)>(&S::SingleParticle_op_1);
+    all &= ok<void (*)(const MaterialPoint&, Grid&, float, float)>(&S::SingleParticle_to_grid);
+    all &= ok<void (*)(GridNode&, float, Vec3)>(&S::SingleNode_op);
+    all &= ok<void (*)(MaterialPoint&, const MaterialPoint&, const Grid&)>(&S::Grid_to_SingleParticle);
+    all &= ok<void (*)(MaterialPoint&, const MaterialPoint&, float)>(&S::SingleParticle_op_2);
+    all &= ok<void (*)(MaterialPoint&, const MaterialPoint&, float, float)>(&S::SingleParticle_op_2);
+    all &= ok<void (*)(PointCloud&, PointCloud&, Grid&, float, float, Vec3)>(&S::ForwardTimeStep);
+    all &= ok<void (*)(PointCloud&, PointCloud&, Grid&, float, float, float, Vec3)>(&S::ForwardTimeStep);
+    all &= ok<void (*)(MaterialPoint&, const MaterialPoint&, float)>(&S::smooth_deformation_gradient);
+    all &= ok<void (*)(PointCloud&)>(&S::P_op_1);
+    all &= ok<void (*)(Grid&)>(&S::G_Reset);
+    all &= ok<void (*)(const PointCloud&, Grid&, float, float)>(&S::P2G);
+    all &= ok<void (*)(Grid&, float, Vec3)>(&S::G_op);
+    all &= ok<void (*)(Grid&, float, Vec3, float)>(&S::G_op);
+    all &= ok<void (*)(PointCloud&, const PointCloud&, Grid&)>(&S::G2P);
+    all &= ok<void (*)(PointCloud&, const PointCloud&, float)>(&S::P_op_2);
+    all &= ok<void (*)(PointCloud&, const PointCloud&, float, float)>(&S::P_op_2);
+    all &= ok<void (*)(PointCloud&, Grid&)>(&S::CalculatePointCloudVolumes);
+    all &= ok<void (*)(const std::vector<Vec3>, Grid&, float)>(&S::P2G_Mass);
+    all &= ok<void (*)(Grid&, Grid&)>(&S::G2G_Mass);
+    all &= ok<void (*)(PointCloud&, PointCloud&, Grid&, std::vector<std::shared_ptr<Grid>>&, float, float, Vec3)>(
+        &MultiThreadMPM::ForwardTimeStep);
+    // BackPropagation.h, Elasticity.h
+    all &= ok<void (*)(CompGraphLayer&, CompGraphLayer&, float, float, float)>(&Back_Timestep);
+    all &= ok<float (*)(const Mat3&, float, float)>(&FixedCorotatedElasticity);
+    all &= ok<Mat3 (*)(const Mat3&, float, float)>(&PK_FixedCorotatedElasticity);
+    all &= ok<Mat3 (*)(const Mat3&, float, float, const Mat3&)>(&d2_FCE_psi_dF2_mult_by_dF);
+    all &= ok<void (*)(float, float, float&, float&)>(&CalculateLameParameters);
+    // CompGraph.h members
+    all &= ok<void (CompGraph::*)(int)>(&CompGraph::SetUpCompGraph);
+    all &= ok<float (CompGraph::*)()>(&CompGraph::EndLayerMassLoss);
+    all &= ok<void (CompGraph::*)(size_t)>(&CompGraph::ComputeForwardPass);
+    all &= ok<void (CompGraph::*)(size_t)>(&CompGraph::ComputeBackwardPass);
+    all &= ok<void (CompGraph::*)(int, float, float, Vec3, int, int, int, float, float, float)>(
+        &CompGraph::OptimizeDefGradControlSequence);
+    all &= ok<void (CompGraph::*)(int, size_t)>(&CompGraph::FiniteDifferencesGradientTest);
+    // PointCloud.h members on the path (optimisers, norms, .mpmbin)
+    all &= ok<void (PointCloud::*)()>(&PointCloud::ResetGradients);
+    all &= ok<float (PointCloud::*)()>(&PointCloud::Compute_dLdF_Norm);
+    all &= ok<void (PointCloud::*)(float, float)>(&PointCloud::Descend_dLdF);
+    all &= ok<void (PointCloud::*)(float, float, float, float, float, int)>(&PointCloud::Descend_Adam);
+    all &= ok<void (PointCloud::*)(float, float, float, float, float, int)>(&PointCloud::Descend_AMSGrad);
+    all &= ok<void (PointCloud::*)(float, float, float)>(&PointCloud::Descend_Lion);
+    all &= ok<void (PointCloud::*)(float, float)>(&PointCloud::Descend_GradientDescent);
+    all &= ok<std::vector<int> (PointCloud::*)(int, int)>(&PointCloud::generateMinibatchIndices);
+    all &= ok<float (PointCloud::*)(const std::vector<int>&)>(&PointCloud::Compute_dLdF_Norm_Stochastic);
+    all &= ok<void (PointCloud::*)(float, float, float, float, float, int, const std::vector<int>&)>(
+        &PointCloud::Descend_Adam_Stochastic);
+    all &= ok<void (PointCloud::*)(size_t)>(&PointCloud::RemovePoint);
+    all &= ok<void (PointCloud::*)(std::string)>(&PointCloud::WriteEntirePointCloudToBinaryFile);
+    all &= ok<bool (PointCloud::*)(std::string)>(&PointCloud::ReadEntirePointCloudFromBinaryFile);
+    all &= ok<bool (PointCloud::*)(const PointCloud&)>(&PointCloud::IsEqualToOtherPointCloud);
+    all &= ok<void (Grid::*)()>(&Grid::ResetValues);
+    all &= ok<void (Grid::*)()>(&Grid::ResetGradients);
+    all &= ok<std::vector<std::reference_wrapper<GridNode>> (Grid::*)(Vec3, std::vector<std::array<int, 3>>*)>(
+        &Grid::QueryPoint_CubicBSpline);
+    all &= ok<std::vector<std::reference_wrapper<const GridNode>> (Grid::*)(Vec3, std::vector<std::array<int, 3>>*) const>(
+        &Grid::QueryPointConst_CubicBSpline);
+    if (!all) return 1;
+
+    // Grid.cpp:40-81 on the host: an interior point sees 4^3 nodes from floor - 1, x-major;
+    // a point in the corner cell only the in-grid 3^3; a far point none
+    Grid q(8, 8, 8, 0.125f, Vec3(0.f, 0.f, 0.f));
+    std::vector<std::array<int, 3>> idx;
+    auto nodes = q.QueryPoint_CubicBSpline(Vec3(0.52f, 0.30f, 0.40f), &idx);
+    if (nodes.size() != 64 || idx.size() != 64) return 2;
+    if (idx[0] != std::array<int, 3>{3, 1, 2} || idx[1] != std::array<int, 3>{3, 1, 3} || idx[63] != std::array<int, 3>{6, 4, 5}) return 3;
+    if (&nodes[5].get() != &q.nodes[idx[5][0]][idx[5][1]][idx[5][2]]) return 4;
+    if (q.QueryPointConst_CubicBSpline(Vec3(0.01f, 0.01f, 0.01f)).size() != 27) return 5;
+    if (!q.QueryPoint_CubicBSpline(Vec3(1e30f, 0.5f, 0.5f)).empty()) return 6;
+
+    // the reference's callers of the phase API, as written there (not run: no GPU here)
+    volatile bool run = false;
+    if (run) {
+        auto grid = std::make_shared<Grid>(8, 8, 8, 0.125f, Vec3(0.f, 0.f, 0.f));
+        auto target_point_cloud = std::make_shared<PointCloud>();
+        auto target_grid = std::make_shared<Grid>(*grid);              // MPMPolyscope.cpp:158-160
+        target_grid->ResetValues();
+        SingleThreadMPM::P2G(*target_point_cloud, *target_grid, 0.0, 0.0);
+        PointCloud mpm_point_cloud;
+        Grid g(*grid);
+        g.ResetValues();                                               // MPMPointCloudVisualization.cpp:11-12
+        SingleThreadMPM::P2G(mpm_point_cloud, g, 0.0, 0.0);
+        std::vector<Vec3> _points = mpm_point_cloud.GetPointPositions();
+        auto mpm_grid = std::make_shared<Grid>(*grid);
+        auto high_res_mpm_grid = std::make_shared<Grid>(16, 16, 16, 0.0625f, Vec3(0.f, 0.f, 0.f));
+        SingleThreadMPM::P2G_Mass(_points, *mpm_grid, 1.0);            // DiffMPMAnimator3D.cpp:501-505
+        SingleThreadMPM::G2G_Mass(*mpm_grid, *high_res_mpm_grid);
+        PointCloud& point_cloud = mpm_point_cloud;                       // CompGraph.cpp:125-135
+        Grid& grid_ = *grid;
+        SingleThreadMPM::G_Reset(grid_);
+        SingleThreadMPM::P2G(point_cloud, grid_, 0.f, 0.f);
+    }
+    return 0;
 }
 """)
     exe = tmp_path / "t"

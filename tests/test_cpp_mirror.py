@@ -155,3 +155,77 @@ def test_cpp_mirror_optimize_runs(tmp_path):
     assert np.isfinite(final)
     l0 = mpmbin.read(tmp_path / "opt_l0.mpmbin")
     assert rel(l0["dFc"], dFc) > 0  # the layer-0 control moved
+
+
+def _grid_txt(path, n=None):
+    g = np.loadtxt(path).reshape(-1, 7)
+    return g[:, 0], g[:, 1:4], g[:, 4:7]
+
+
+def _cubic(t):
+    a = np.abs(t)
+    return np.where(a < 1, 0.5 * a ** 3 - a * a + 2.0 / 3.0, np.where(a < 2, (2 - a) ** 3 / 6.0, 0.0))
+
+
+@pytest.mark.gpu
+def test_cpp_mirror_phase_functions(tmp_path):
+    """The ForwardSimulation.h phase functions through the mirror, called as reference code
+    calls them (float containers, so 1e-6 relative unless stated):
+    * P_op_1, G_Reset, P2G, G_op, G2P, P_op_2 in turn = one oracle ForwardTimeStep (state, node
+      m and v, node p = m v where the node mass passes the FLT_EPSILON threshold);
+    * the target grid of MPMPolyscope.cpp:158-160 (grid copy, ResetValues, P2G(cloud, grid,
+      0.0, 0.0)) = the oracle's P2G at dt = drag = 0;
+    * P2G_Mass(positions, grid, 1.0) = the oracle's P2G mass of unit points, and G2G_Mass onto
+      a coarser grid = the separable closed form sum_ijk N_x N_y N_z m_ijk;
+    * G_op with a gravity point: v = p/m + dt 3 (c - x_i)/|c - x_i| where the mass passes;
+    * the single-particle forms on particles 0..4 = the cloud forms on those particles."""
+    case, st, cs, dFc, prm, target = _write_case(tmp_path)
+    subprocess.run([BIN, "phases", str(tmp_path)], check=True, timeout=300)
+    eps = 1.1920928955078125e-07
+    nxt, gm, gv = orc.forward_step(prm, st, cs, dFc)
+    ph = mpmbin.read(tmp_path / "phases.mpmbin")
+    for f in ("x", "v", "C", "F"):
+        assert maxrel(ph[f], getattr(nxt, f)) < 1e-6, f
+    m, p, v = _grid_txt(tmp_path / "phases_grid.txt")
+    assert maxrel(m, gm) < 1e-6 and maxrel(v, gv) < 1e-5
+    live = np.abs(gm) > eps
+    assert maxrel(p[live], (gm[:, None] * gv)[live]) < 1e-5
+
+    tm, tp, tv = _grid_txt(tmp_path / "target_grid.txt")
+    om, op = orc.p2g(prm, st, cs, dFc, dt=0.0, drag=0.0)
+    assert maxrel(tm, om) < 1e-6 and np.abs(tp - op).max() <= 1e-6 * max(np.abs(op).max(), 1e-30) + 1e-12
+    assert not tv.any()
+
+    pm, pp, pv = _grid_txt(tmp_path / "p2g_mass.txt")
+    assert maxrel(pm, orc.p2g_mass(prm, st.x, np.ones(prm.n))) < 1e-6
+    assert not pp.any() and not pv.any()
+    cm, _, _ = _grid_txt(tmp_path / "g2g_mass.txt")
+    dims = np.array(prm.dim)
+    cdims = (dims + 1) // 2
+    dx1 = float(np.float32(prm.dx))
+    dx2 = float(np.float32(2.0) * np.float32(dx1))
+    W = [_cubic((np.arange(dims[a])[None, :] * dx1 - np.arange(cdims[a])[:, None] * dx2) / dx1) for a in range(3)]
+    want = np.einsum("Ii,Jj,Kk,ijk->IJK", W[0], W[1], W[2], pm.reshape(tuple(dims))).ravel()
+    assert maxrel(cm, want) < 1e-6
+
+    gm2, gp2, gv2 = _grid_txt(tmp_path / "gravity_grid.txt")
+    c = (np.float32(dims) * (np.float32(0.5) * np.float32(dx1))).astype(np.float64)
+    ijk = np.stack(np.meshgrid(*[np.arange(d) for d in dims], indexing="ij"), -1).reshape(-1, 3) * dx1
+    r = c[None, :] - ijk
+    nr = np.linalg.norm(r, axis=1)
+    f = np.where(nr[:, None] > 0, 3.0 * r / np.where(nr > 0, nr, 1.0)[:, None], 0.0)
+    live2 = np.abs(gm2) > eps
+    want_v = np.where(live2[:, None], gp2 / np.where(live2, gm2, 1.0)[:, None] + prm.dt * f, v)
+    assert maxrel(gv2, want_v) < 1e-5
+
+    sg = mpmbin.read(tmp_path / "single.mpmbin")
+    for f_ in ("x", "v", "C", "F"):
+        assert maxrel(sg[f_], ph[f_][:5]) < 1e-6, f_
+    sm, sp_, sv = _grid_txt(tmp_path / "single_grid.txt")
+    sub = orc.Params(**{**prm.__dict__, "n": 5})
+    st5 = orc.State(st.x[:5], st.v[:5], st.C[:5], st.F[:5])
+    cs5 = orc.Consts(cs.m[:5], cs.vol[:5], cs.lam[:5], cs.mu[:5])
+    om5, op5 = orc.p2g(sub, st5, cs5, dFc[:5])
+    assert maxrel(sm, om5) < 1e-6 and maxrel(sp_, op5) < 1e-5
+    live5 = np.abs(sm) > eps
+    assert maxrel(sv, np.where(live5[:, None], sp_ / np.where(live5, sm, 1.0)[:, None], 0.0)) < 1e-6

@@ -45,7 +45,7 @@ def _build():
     os.makedirs(BUILD, exist_ok=True)
     rt = os.path.join(CSRC, "dmpm_runtime.hip")
     srcs = [rt, os.path.join(CPU, "host_kernels.cpp"), os.path.join(CPU, "hip", "hip_runtime.h"),
-            os.path.join(CSRC, "dmpm_kernels.hip"), os.path.join(CSRC, "dmpm_kernels.h"),
+            os.path.join(CSRC, "dmpm_kernels.hip"), os.path.join(CSRC, "dmpm_phases.hip"), os.path.join(CSRC, "dmpm_kernels.h"),
             os.path.join(CSRC, "dmpm_math.h"), os.path.join(INCLUDE, "dmpm.h")]
     if _stale(LIB, srcs):
         subprocess.run([gxx, *FLAGS, "-fPIC", "-shared", "-I", CPU, "-I", CSRC, "-I", INCLUDE, "-x", "c++", rt, "-x", "none",

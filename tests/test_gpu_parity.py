@@ -526,3 +526,34 @@ def test_fp32_api_order_roundtrip():
     x0 = mo.layer_state(0)[0]
     assert np.array_equal(x0, f32_positions(st.x[perm], case.sc.dim, case.sc.dx, case.sc.min_point))
     assert np.array_equal(mo.volumes(), case.cs.vol[perm].astype(np.float32).astype(np.float64))
+
+
+# ------------------------------------------------------------- phase calls --
+def test_phase_calls_compose_to_the_forward_step():
+    """dmpm_phase_* in ForwardTimeStep's order (P_op_1 via dmpm_eval_pk1, P2G, G_op, G2P,
+    P_op_2; ForwardSimulation.cpp:136-157) = the oracle's step at 1e-12 (states and grid),
+    with drag, smoothing, f_ext and non-zero v.  Also: P2G adds into the grid it is given,
+    G_op leaves the velocity of nodes at or below FLT_EPSILON mass, and P2G mass of unit
+    points = the oracle's."""
+    case = make_case(L=2, drag=0.2, smoothing=0.05, f_ext=(0.4, -0.3, 0.2), vel=0.1)
+    sc, st, cs, prm = case.sc, case.st0, case.cs, case.prm
+    dFc = case.controls[0]
+    grid = (sc.dim, sc.dx, sc.min_point)
+    P = pkg.eval_pk1(st.F + dFc, cs.lam, cs.mu)
+    gm, gp = pkg.phase_p2g(grid, st.x, st.v, st.C, st.F, dFc, P, cs.m, cs.vol, prm.dt, prm.drag)
+    gv = pkg.phase_grid_op(grid, gm, gp, prm.dt, prm.f_ext)
+    vn, Cn = pkg.phase_g2p(grid, st.x, gv)
+    xn, Fn = pkg.phase_p_op2(st.x, st.F, dFc, vn, Cn, prm.smoothing, prm.dt)
+    nxt, om, ov = orc.forward_step(prm, st, cs, dFc)
+    assert maxrel(gm, om) < 1e-12 and maxrel(gv, ov) < 1e-12
+    assert maxrel(vn, nxt.v) < 1e-12 and maxrel(Cn, nxt.C) < 1e-12
+    assert maxrel(xn, nxt.x) < 1e-12 and maxrel(Fn, nxt.F) < 1e-12
+    _, op = orc.p2g(prm, st, cs, dFc)
+    assert maxrel(gp, op) < 1e-12
+    gm2, gp2 = pkg.phase_p2g(grid, st.x, st.v, st.C, st.F, dFc, P, cs.m, cs.vol, prm.dt, prm.drag, gm, gp)
+    assert maxrel(gm2, 2 * gm) < 1e-12 and maxrel(gp2, 2 * gp) < 1e-12
+    keep = np.full((gm.size, 3), 7.0)
+    gvk = pkg.phase_grid_op(grid, gm, gp, prm.dt, prm.f_ext, keep)
+    dead = ~(np.abs(gm) > 1.1920928955078125e-07)
+    assert dead.any() and np.all(gvk[dead] == 7.0) and np.array_equal(gvk[~dead], gv[~dead])
+    assert maxrel(pkg.phase_p2g_mass(grid, st.x, np.ones(sc.n)), orc.p2g_mass(prm, st.x, np.ones(sc.n))) < 1e-12
