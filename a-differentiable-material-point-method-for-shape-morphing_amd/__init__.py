@@ -37,6 +37,8 @@ DESCEND_ADAM, DESCEND_AMSGRAD, DESCEND_LION, DESCEND_GD = range(4)
 OPT_GRAPHS = 1
 OPT_RANGE_MAX = 2
 OPT_LDS_CAP = 3
+OPT_RESORT = 4
+OPT_RESORT_PERMILLE = 5
 
 K_P2G, K_GRID_UPDATE, K_G2P, K_G2P_ADJ, K_GRID_ADJ, K_P2G_ADJ, K_LOSS, K_FILL = range(8)
 KERNEL_NAMES = ["p2g", "grid_update", "g2p", "g2p_adj", "grid_adj", "p2g_adj", "loss", "fill"]
@@ -55,7 +57,7 @@ EXPORTS = [
     "dmpm_target_from_points_batch", "dmpm_set_option", "dmpm_get_option", "dmpm_autotune", "dmpm_descend",
     "dmpm_eval_energy", "dmpm_get_point_diagnostics", "dmpm_target_from_points_noalloc",
     "dmpm_phase_p2g", "dmpm_phase_grid_op", "dmpm_phase_grid_op_point", "dmpm_phase_g2p", "dmpm_phase_p_op2",
-    "dmpm_phase_p2g_mass", "dmpm_phase_g2g_mass",
+    "dmpm_phase_p2g_mass", "dmpm_phase_g2g_mass", "dmpm_sort_particles", "dmpm_get_window_stats",
 ]
 
 # most points of one dmpm_target_from_points_noalloc call (DMPM_NOALLOC_MAX_POINTS)
@@ -182,6 +184,8 @@ def load_library(path: str = LIB_PATH):
                                      ctypes.c_int),
         "dmpm_phase_g2p": ([ctypes.POINTER(GridDesc), i32, _DP, _DP, _DP, _DP], ctypes.c_int),
         "dmpm_phase_p_op2": ([i32] + [_DP] * 5 + [ctypes.c_double] * 2 + [_DP] * 2, ctypes.c_int),
+        "dmpm_sort_particles": ([vp], ctypes.c_int),
+        "dmpm_get_window_stats": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "dmpm_phase_p2g_mass": ([ctypes.POINTER(GridDesc), i32, _DP, _DP, _DP], ctypes.c_int),
         "dmpm_phase_g2g_mass": ([ctypes.POINTER(GridDesc), _DP, ctypes.POINTER(GridDesc), _DP], ctypes.c_int),
     }
@@ -564,6 +568,17 @@ class Morph:
     def set_graphs(self, on=True):
         """Capture each pass into a HIP graph on first use and replay it afterwards."""
         self.set_option(OPT_GRAPHS, int(bool(on)))
+
+    def sort_particles(self):
+        """FP32: re-order the particles on the device by grid block (API order unchanged)."""
+        _check(self._L.dmpm_sort_particles(self._h))
+
+    def window_stats(self):
+        """FP32: P2G workgroups on the fixed / general / HBM paths in the last forward pass that
+        reached the loss, and the number of device sorts so far, as a dict."""
+        a = (ctypes.c_int64 * 4)()
+        _check(self._L.dmpm_get_window_stats(self._h, a))
+        return {"fixed": int(a[0]), "general": int(a[1]), "hbm": int(a[2]), "sorts": int(a[3])}
 
     def autotune(self):
         """FP32: pick the fastest range size / LDS capacity; returns its step time (ms).

@@ -19,7 +19,7 @@ LIB = os.path.join(LIBDIR, "libdmpm.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DMPM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["dmpm_kernels.hip", "dmpm_blocks.hip", "dmpm_phases.hip", "dmpm_runtime.hip"]
+SOURCES = ["dmpm_kernels.hip", "dmpm_blocks.hip", "dmpm_phases.hip", "dmpm_sort.hip", "dmpm_runtime.hip"]
 
 # C++ host mirror of the reference API (g++, links libdmpm.so) and its self-test driver
 HOST_LIB = os.path.join(LIBDIR, "libdmpm_cpp.so")

@@ -702,6 +702,7 @@ __global__ __launch_bounds__(kP2G, 4) void k_p2g_blk(Batch bt, StepIO<float> io)
     Scales<4> SC;
     block_setup<4, false>(sp, box, bound, rg.y - rg.x, bt.cap, W, SC);
     const int mode = (W.fixed && SC.ok) ? 0 : ((W.lds && SC.ok) ? 1 : 2);
+    if (threadIdx.x == 0 && bt.wstats) atomicAdd(bt.wstats + mode, 1ull);
     if (mode == 0)
         for (int i = threadIdx.x; i < 4 * kFWS; i += blockDim.x) lds64[i] = 0ull;
     else if (mode == 1)

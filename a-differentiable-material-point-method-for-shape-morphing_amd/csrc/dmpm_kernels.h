@@ -108,6 +108,7 @@ struct Batch {
     long long Gtot;         // total nodes
     int cap;
     long long gmax;         // nodes of the largest instance
+    unsigned long long* wstats;  // P2G workgroups per window mode (fixed, general, HBM), or null
 };
 // 4-channel grids (m or mbar, then 3 vector channels) of instance i start at 4 * nofs[i];
 // 1-channel arrays (target, dL/dm) at nofs[i].
@@ -150,6 +151,30 @@ void launch_phase_g2p(const StepParams& g, const double* x, const double* gv, do
 void launch_phase_p_op2(int n, const double* x, const double* F, const double* dFc, const double* vn, const double* Cn, double sm,
                         double dt, double* xn, double* Fn, hipStream_t s);
 void launch_phase_g2g_mass(const StepParams& g1, const StepParams& g2, const double* m1, double* m2, hipStream_t s);
+
+// Device order of the FP32 particles (dmpm_sort.hip): per instance by 4^3-cell
+// block, rank in cell, cell in block.  Buffers of N entries; after
+// launch_sort_order, v1[d] is the current slot of the particle that goes to
+// slot d and blk[d] its global block (instance i's blocks from boff[i]).
+struct SortArgs {
+    const StepParams* sps;   // per instance (device)
+    const int* poff;         // slot prefix, B+1 entries (device)
+    const long long* boff;   // block prefix, B+1 entries (device)
+    int B;
+    long long nblocks;       // boff[B]
+    const float* x;          // layer holding the positions (fields FX.., stride S)
+    long long S;
+    int N;
+    unsigned long long *k0, *k1;
+    int *v0, *v1, *head, *blk;
+    void* temp;
+    size_t temp_bytes;       // sort_temp_bytes(N, nblocks)
+};
+size_t sort_temp_bytes(int N, long long nblocks);
+hipError_t launch_sort_order(const SortArgs& a, hipStream_t s);
+// dst[f S + d] = src[f S + perm[d]], f < nf
+template <typename T> void launch_gather(const T* src, T* dst, long long S, int N, int nf, const int* perm, hipStream_t s);
+void launch_iota(int* a, int N, hipStream_t s);
 
 // batched constitutive evaluations (AoS 9 doubles per matrix)
 void launch_eval_pk1(int n, const double* F, const double* lam, const double* mu, double* P, hipStream_t s);

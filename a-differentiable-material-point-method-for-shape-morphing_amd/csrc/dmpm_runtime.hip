@@ -109,9 +109,23 @@ struct dmpm_ctx {
     std::vector<double> origin;
     long long gmax = 0;
     std::vector<double> inst_loss;
-    // FP32 path: device order of the particles (sorted by 4^3-cell block per
-    // instance); pos[p] is the device slot of API particle p (empty = identity)
+    // FP32 path: device order of the particles (sorted on the device by 4^3-cell
+    // block per instance); pos[p] is the device slot of API particle p (empty =
+    // identity), d_api its inverse on the device
     std::vector<int> pos;
+    int* d_api = nullptr;           // API index of each device slot (N)
+    int* d_poff = nullptr;          // slot prefix per instance (B+1)
+    long long* d_boff = nullptr;    // block prefix per instance (B+1)
+    std::vector<long long> boff;
+    void* sortbuf = nullptr;        // sort keys, values, heads, blocks and rocPRIM scratch (first sort)
+    size_t sort_temp = 0;
+    // P2G workgroups per window mode (fixed, general, HBM) since the forward
+    // pass began, and their snapshot at the last EndLayerMassLoss
+    unsigned long long* wstats = nullptr;
+    long long last_wstats[3] = {0, 0, 0};
+    int resort_mode = 2;            // 0: at set_particles only, 1: also at chain_episode, 2: also when stale
+    int resort_permille = 50;       // stale: more than this share of P2G workgroups off the fixed window
+    long long n_sorts = 0;
     std::vector<int4> host_blocks;  // {begin, end, instance} of every non-empty 4^3-cell block
     int4* ranges = nullptr;         // per-workgroup {begin, end, instance}: blocks split to <= range_max
     std::vector<int4> host_ranges;
@@ -370,6 +384,12 @@ void clear_grid(dmpm_ctx* c, float* g)
 
 void clear_grid(dmpm_ctx* c, double* g) { fill<double>(c, g, (long long)kGridFields * c->G, 0.0); }
 
+// restart the window-mode counts (a kernel, so that graphs may hold it)
+void zero_wstats(dmpm_ctx* c)
+{
+    if (c->wstats) launch_fill<double>(reinterpret_cast<double*>(c->wstats), 3, 0.0, c->stream);
+}
+
 // wgrid used as scratch (node masses, dL/dm): every tile may be non-zero
 void taint_wgrid(dmpm_ctx* c)
 {
@@ -517,15 +537,23 @@ void recompute_segment(dmpm_ctx* c, int seg)
 }
 
 // -------------------------------------------------------------- passes -----
+int resort(dmpm_ctx* c, bool controls_live);
+bool order_stale(const dmpm_ctx* c);
+
 template <typename Real>
 int forward_pass(dmpm_ctx* c, int start)
 {
     if (!c->have_particles) return fail(DMPM_ERR_STATE, "forward pass before dmpm_set_particles");
     if (start < 0 || start >= c->L - 1) return fail(DMPM_ERR_INVALID, "start_layer out of range");
+    if (c->f32 && start == 0 && c->resort_mode >= 2 && order_stale(c)) {
+        const int rs = resort(c, true);
+        if (rs) return rs;
+    }
     int from = std::min(start, c->fwd_valid);
     int rc;
     if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) {
         rc = run_graph(c, from, [&] {
+            zero_wstats(c);
             bool clean = false;
             for (int i = from; i < c->L - 1; ++i)
                 clean = step_forward<Real>(c, slot<Real>(c, i), slot<Real>(c, i + 1), ctrl_of<Real>(c, i), grid_of_step<Real>(c, i), clean,
@@ -537,6 +565,7 @@ int forward_pass(dmpm_ctx* c, int start)
         const int s0_last = c->seg_last * c->K;
         Real* grids = static_cast<Real*>(c->grids);
         rc = run_graph(c, from, [&] {
+        zero_wstats(c);
         bool clean = false;
         for (int i = from; i < c->L - 1; ++i) {
             const Real* cur;
@@ -617,7 +646,10 @@ int end_loss(dmpm_ctx* c, double* loss)
     HIP_TRY(hipGetLastError());
     c->inst_loss.assign(c->B, 0.0);
     HIP_TRY(hipMemcpyAsync(c->inst_loss.data(), c->dacc, c->B * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    unsigned long long ws[3] = {0, 0, 0};
+    if (c->wstats) HIP_TRY(hipMemcpyAsync(ws, c->wstats, sizeof(ws), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 3; ++k) c->last_wstats[k] = (long long)ws[k];
     double h = 0.0;
     for (double v : c->inst_loss) h += v;
     if (loss) *loss = h;
@@ -720,59 +752,6 @@ int download_positions(dmpm_ctx* c, const Real* layer, double* x)
     return DMPM_OK;
 }
 
-// Device order of the FP32 path, per instance.  Particles are grouped by the
-// 4^3-cell block of their cell (blocks in x-major order); inside a block they
-// are ordered by their rank within their cell ("slot"), then by the cell's
-// x-major index 16 x + 4 y + z in the block.  A 64-lane wave then holds one
-// particle of each of the block's 64 cells: its LDS atomics hit 64 distinct
-// nodes instead of 8 particles of one cell hitting the same node (x-major
-// measured 2 us faster per P2G than Morton order at 1M particles, 128^3).
-// Ties keep the API order.
-// Instances keep their API slot range [poff[i], poff[i+1]).
-void sort_particles(dmpm_ctx* c, const double* x)
-{
-    c->pos.assign(c->N, 0);
-    c->host_blocks.clear();
-    for (int in = 0; in < c->B; ++in) {
-        const dmpm_instance& g = c->inst[in];
-        const int p0 = c->poff[in], N = g.n_particles;
-        int nb[3];
-        for (int a = 0; a < 3; ++a) nb[a] = (g.dim[a] + 3) / 4;
-        std::vector<uint64_t> cellkey(N);
-        for (int p = 0; p < N; ++p) {
-            int cell[3];
-            for (int a = 0; a < 3; ++a) {
-                double u = std::floor((x[3 * (size_t)(p0 + p) + a] - g.min_point[a]) / g.dx);
-                if (!(u >= 0.0)) u = 0.0;  // also maps NaN
-                if (u > g.dim[a] - 1) u = g.dim[a] - 1;
-                cell[a] = (int)u;
-            }
-            const uint64_t block = ((uint64_t)(cell[0] >> 2) * nb[1] + (cell[1] >> 2)) * nb[2] + (cell[2] >> 2);
-            const uint32_t local = (uint32_t)((cell[0] & 3) << 4 | (cell[1] & 3) << 2 | (cell[2] & 3));
-            cellkey[p] = (block << 6) | local;
-        }
-        std::vector<int> order(N);
-        for (int p = 0; p < N; ++p) order[p] = p;
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cellkey[a] < cellkey[b]; });
-        // slot = rank inside the cell; final key (block, slot, cell)
-        std::vector<uint64_t> key(N);
-        for (int d = 0; d < N; ++d) {
-            const int p = order[d];
-            const uint32_t sl = (d > 0 && cellkey[order[d - 1]] == cellkey[p]) ? (uint32_t)(key[order[d - 1]] >> 6 & 0xffffu) + 1u : 0u;
-            key[p] = ((cellkey[p] >> 6) << 22) | ((uint64_t)(sl & 0xffffu) << 6) | (cellkey[p] & 63u);
-        }
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
-        for (int d = 0; d < N; ++d) c->pos[p0 + order[d]] = p0 + d;
-        int d0 = 0;
-        for (int d = 1; d <= N; ++d) {
-            if (d == N || (key[order[d]] >> 22) != (key[order[d0]] >> 22)) {
-                c->host_blocks.push_back(make_int4(p0 + d0, p0 + d, in, 0));
-                d0 = d;
-            }
-        }
-    }
-}
-
 // Workgroup ranges: every block split into equal parts of at most range_max
 // particles (the particle kernels hold two particles per thread of a
 // 256-thread workgroup, so range_max <= 512).
@@ -808,15 +787,111 @@ int build_ranges(dmpm_ctx* c)
     return DMPM_OK;
 }
 
+// Re-order the FP32 particles on the device (per instance by 4^3-cell block,
+// rank in cell, cell in block; see dmpm_sort.hip) and permute every array
+// that lives per particle across calls: layer 0, the constants, the API index
+// map, the optimiser states, the controls (controls_live) and the held
+// control gradients.  The trajectory after layer 0 and the layer gradients
+// are invalidated; the block ranges are rebuilt from the new order.
+int resort(dmpm_ctx* c, bool controls_live)
+{
+    const int N = c->N;
+    const size_t S = c->S;
+    const long long nblocks = c->boff[c->B];
+    auto* k0 = static_cast<unsigned long long*>(c->sortbuf);
+    if (!c->sortbuf && N > 0) {
+        c->sort_temp = sort_temp_bytes(N, nblocks);
+        if (c->sort_temp == 0) return fail(DMPM_ERR_HIP, "rocPRIM sort: temporary storage query failed");
+        const int rc = dalloc(c, &c->sortbuf, (size_t)N * (2 * 8 + 4 * 4) + 256 + c->sort_temp);
+        if (rc) return rc;
+        k0 = static_cast<unsigned long long*>(c->sortbuf);
+    }
+    std::vector<int> blk(N), api(N);
+    if (N > 0) {
+        SortArgs a{};
+        a.sps = c->d_sps;
+        a.poff = c->d_poff;
+        a.boff = c->d_boff;
+        a.B = c->B;
+        a.nblocks = nblocks;
+        a.x = slot<float>(c, 0);
+        a.S = (long long)S;
+        a.N = N;
+        a.k0 = k0;
+        a.k1 = k0 + N;
+        a.v0 = reinterpret_cast<int*>(a.k1 + N);
+        a.v1 = a.v0 + N;
+        a.head = a.v1 + N;
+        a.blk = a.head + N;
+        a.temp = reinterpret_cast<char*>(c->sortbuf) + (((size_t)N * (2 * 8 + 4 * 4) + 255) & ~(size_t)255);
+        a.temp_bytes = c->sort_temp;
+        HIP_TRY(launch_sort_order(a, c->stream));
+        const int* perm = a.v1;
+        // every per-particle array through tmp (12 fields at a time) in the new order
+        float* t = static_cast<float*>(c->tmp);
+        auto permute = [&](float* base, long long nf) -> int {
+            for (long long f = 0; f < nf; f += 12) {
+                const int k = (int)std::min<long long>(12, nf - f);
+                launch_gather<float>(base + f * S, t, (long long)S, N, k, perm, c->stream);
+                HIP_TRY(hipMemcpyAsync(base + f * S, t, (size_t)k * S * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+            }
+            return DMPM_OK;
+        };
+        int rc;
+        if ((rc = permute(slot<float>(c, 0), kLayerFields)) || (rc = permute(static_cast<float*>(c->cst), kConstFields)))
+            return rc;
+        if (controls_live && (rc = permute(static_cast<float*>(c->ctrl), (long long)(c->L - 1) * 9))) return rc;
+        if (c->bwd_control >= 0 &&
+            (rc = permute(static_cast<float*>(c->gctrl), (c->cfg.keep_control_grads ? (long long)(c->L - 1) : 1) * 9)))
+            return rc;
+        for (void* o : c->opt)
+            if (o && (rc = permute(static_cast<float*>(o), 27))) return rc;
+        launch_gather<int>(c->d_api, reinterpret_cast<int*>(t), (long long)S, N, 1, perm, c->stream);
+        HIP_TRY(hipMemcpyAsync(c->d_api, t, (size_t)N * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(blk.data(), a.blk, (size_t)N * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(api.data(), c->d_api, (size_t)N * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    c->pos.assign(N, 0);
+    for (int d = 0; d < N; ++d) c->pos[api[d]] = d;
+    c->host_blocks.clear();
+    int in = 0;
+    for (int d0 = 0, d = 1; d <= N; ++d) {
+        if (d == N || blk[d] != blk[d0]) {
+            while (in + 1 < c->B && c->boff[in + 1] <= blk[d0]) ++in;
+            c->host_blocks.push_back(make_int4(d0, d, in, 0));
+            d0 = d;
+        }
+    }
+    const int rc = build_ranges(c);
+    if (rc) return rc;
+    ++c->n_sorts;
+    for (long long& w : c->last_wstats) w = 0;
+    c->fwd_valid = std::min(c->fwd_valid, 0);
+    c->loss_valid = false;
+    c->grads_lo = -1;
+    c->tape_seg = -1;
+    return DMPM_OK;
+}
+
+// stale: in the last forward pass that reached the loss, more than
+// resort_permille per mille of the P2G workgroups left the fixed window
+bool order_stale(const dmpm_ctx* c)
+{
+    const long long tot = c->last_wstats[0] + c->last_wstats[1] + c->last_wstats[2];
+    return tot > 0 && (c->last_wstats[1] + c->last_wstats[2]) * 1000 > (long long)c->resort_permille * tot;
+}
+
 template <typename Real>
 int set_particles(dmpm_ctx* c, const double* x, const double* v, const double* F, const double* C, const double* m,
                   const double* vol, const double* lam, const double* mu)
 {
     if (!x || !v || !F || !C || !m || !lam || !mu) return fail(DMPM_ERR_INVALID, "null particle array");
     if (c->f32) {
-        sort_particles(c, x);
-        const int rc0 = build_ranges(c);
-        if (rc0) return rc0;
+        // uploaded in API order, then ordered on the device
+        c->pos.clear();
+        launch_iota(c->d_api, c->N, c->stream);
     }
     // a new particle set starts with zero controls (MaterialPoint.h:31)
     fill<Real>(c, static_cast<Real*>(c->ctrl), (long long)(c->L - 1) * 9 * c->S, Real(0));
@@ -835,6 +910,7 @@ int set_particles(dmpm_ctx* c, const double* x, const double* v, const double* F
     }
     if ((rc = upload_fields<Real>(c, cst, KLAM, 1, lam, c->N))) return rc;
     if ((rc = upload_fields<Real>(c, cst, KMU, 1, mu, c->N))) return rc;
+    if (c->f32 && (rc = resort(c, false))) return rc;
     c->have_particles = true;
     c->fwd_valid = 0;
     c->loss_valid = false;
@@ -1090,6 +1166,8 @@ int chain_episode(dmpm_ctx* c)
     c->grads_lo = -1;
     c->bwd_control = -1;
     c->tape_seg = -1;
+    // the particles moved over the episode: order the new layer 0 afresh
+    if (c->f32 && c->resort_mode >= 1) return resort(c, false);
     return DMPM_OK;
 }
 
@@ -1246,7 +1324,11 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         (rc = dalloc(c, (void**)&c->dpart, std::max<long long>(kLossBlocks, loss_partials(c->B)) * sizeof(double))) ||
         (rc = dalloc(c, (void**)&c->d_sps, c->B * sizeof(StepParams))) ||
         (rc = dalloc(c, (void**)&c->d_nofs, (c->B + 1) * sizeof(long long))) ||
-        (c->f32 && (rc = dalloc(c, (void**)&c->tflags, (grid_slots + 3) * (size_t)c->ntiles * sizeof(unsigned))))) {
+        (c->f32 && (rc = dalloc(c, (void**)&c->tflags, (grid_slots + 3) * (size_t)c->ntiles * sizeof(unsigned)))) ||
+        (c->f32 && (rc = dalloc(c, (void**)&c->d_api, S * sizeof(int)))) ||
+        (c->f32 && (rc = dalloc(c, (void**)&c->d_poff, (c->B + 1) * sizeof(int)))) ||
+        (c->f32 && (rc = dalloc(c, (void**)&c->d_boff, (c->B + 1) * sizeof(long long)))) ||
+        (c->f32 && (rc = dalloc(c, (void**)&c->wstats, 3 * sizeof(unsigned long long))))) {
         std::string msg = g_err;
         dmpm_destroy(c);
         return fail(rc, msg);
@@ -1269,13 +1351,24 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         launch_fill<double>(static_cast<double*>(c->layers), (long long)kLayerFields * S, 0.0, c->stream);
         launch_fill<double>(static_cast<double*>(c->cst), (long long)kConstFields * S, 0.0, c->stream);
     }
+    c->boff.assign(c->B + 1, 0);
+    for (int i = 0; i < c->B; ++i) {
+        const dmpm_instance& in = insts[i];
+        c->boff[i + 1] = c->boff[i] + (long long)((in.dim[0] + 3) / 4) * ((in.dim[1] + 3) / 4) * ((in.dim[2] + 3) / 4);
+    }
     {
         std::vector<StepParams> sps(c->B);
         for (int i = 0; i < c->B; ++i) sps[i] = inst_params(c, i);
         hipError_t e1 = hipMemcpyAsync(c->d_sps, sps.data(), sps.size() * sizeof(StepParams), hipMemcpyHostToDevice, c->stream);
         hipError_t e2 = hipMemcpyAsync(c->d_nofs, c->nofs.data(), c->nofs.size() * sizeof(long long), hipMemcpyHostToDevice, c->stream);
+        hipError_t e3 = hipSuccess, e4 = hipSuccess;
+        if (c->f32) {
+            e3 = hipMemcpyAsync(c->d_poff, c->poff.data(), c->poff.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+            e4 = hipMemcpyAsync(c->d_boff, c->boff.data(), c->boff.size() * sizeof(long long), hipMemcpyHostToDevice, c->stream);
+            launch_fill<double>(reinterpret_cast<double*>(c->wstats), 3, 0.0, c->stream);
+        }
         he = hipStreamSynchronize(c->stream);
-        if (he == hipSuccess) he = e1 != hipSuccess ? e1 : e2;
+        if (he == hipSuccess) he = e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : (e3 != hipSuccess ? e3 : e4));
     }
     c->batch.sps = c->d_sps;
     c->batch.nofs = c->d_nofs;
@@ -1285,6 +1378,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
     c->batch.Gtot = c->G;
     c->batch.gmax = c->gmax;
     c->batch.cap = c->lds_cap;
+    c->batch.wstats = c->wstats;
     if (he != hipSuccess) {
         dmpm_destroy(c);
         return fail(DMPM_ERR_HIP, std::string("init: ") + hipGetErrorString(he));
@@ -1328,7 +1422,8 @@ int dmpm_destroy(dmpm_ctx* c)
     if (c->stream) hipStreamSynchronize(c->stream);
     clear_graphs(c);
     void* bufs[] = {c->cst,  c->ctrl, c->layers, c->grids, c->wgrid, c->wgrid2, c->agrid, c->gbuf[0], c->gbuf[1],
-                    c->gctrl, c->tmp, c->target, c->dacc,  c->ranges, c->d_sps, c->d_nofs, c->dpart, c->tflags};
+                    c->gctrl, c->tmp, c->target, c->dacc,  c->ranges, c->d_sps, c->d_nofs, c->dpart, c->tflags,
+                    c->d_api, c->d_poff, c->d_boff, c->sortbuf, c->wstats};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (void* b : c->opt)
@@ -1646,6 +1741,14 @@ int dmpm_set_option(dmpm_ctx* c, int32_t option, int64_t value)
         c->batch.cap = c->lds_cap;
         clear_graphs(c);
         return DMPM_OK;
+    case DMPM_OPT_RESORT:
+        if (value < 0 || value > 2) return fail(DMPM_ERR_INVALID, "DMPM_OPT_RESORT is 0, 1 or 2");
+        c->resort_mode = (int)value;
+        return DMPM_OK;
+    case DMPM_OPT_RESORT_PERMILLE:
+        if (value < 0 || value > 1000) return fail(DMPM_ERR_INVALID, "DMPM_OPT_RESORT_PERMILLE must be in [0, 1000]");
+        c->resort_permille = (int)value;
+        return DMPM_OK;
     default:
         return fail(DMPM_ERR_INVALID, "unknown option");
     }
@@ -1658,6 +1761,8 @@ int dmpm_get_option(dmpm_ctx* c, int32_t option, int64_t* value)
     case DMPM_OPT_GRAPHS: *value = c->use_graphs ? 1 : 0; return DMPM_OK;
     case DMPM_OPT_RANGE_MAX: *value = c->range_max; return DMPM_OK;
     case DMPM_OPT_LDS_CAP: *value = c->lds_cap; return DMPM_OK;
+    case DMPM_OPT_RESORT: *value = c->resort_mode; return DMPM_OK;
+    case DMPM_OPT_RESORT_PERMILLE: *value = c->resort_permille; return DMPM_OK;
     default: return fail(DMPM_ERR_INVALID, "unknown option");
     }
 }
@@ -1726,6 +1831,24 @@ int dmpm_autotune(dmpm_ctx* c, double* best_ms)
     if (rc2) return rc2;
     HIP_TRY(hipGetLastError());
     if (best_ms) *best_ms = best;
+    return DMPM_OK;
+}
+
+int dmpm_sort_particles(dmpm_ctx* c)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->f32) return DMPM_OK;
+    if (!c->have_particles) return fail(DMPM_ERR_STATE, "sort before dmpm_set_particles");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return resort(c, true);
+}
+
+int dmpm_get_window_stats(dmpm_ctx* c, int64_t* stats)
+{
+    if (!c || !stats) return fail(DMPM_ERR_INVALID, "null argument");
+    for (int k = 0; k < 3; ++k) stats[k] = c->last_wstats[k];
+    stats[3] = c->n_sorts;
     return DMPM_OK;
 }
 

@@ -192,12 +192,28 @@ int dmpm_chain_episode(dmpm_ctx* ctx);
 #define DMPM_OPT_GRAPHS 1    /* 1: capture each pass into a HIP graph on first use, then replay it (default 0) */
 #define DMPM_OPT_RANGE_MAX 2 /* FP32: most particles per workgroup range, 64..512 (default 512) */
 #define DMPM_OPT_LDS_CAP 3   /* FP32: node capacity of the general LDS window, 512..4096 (default 1024) */
+#define DMPM_OPT_RESORT 4    /* FP32: when the device re-orders the particles by grid block (dmpm_sort_particles):
+                              * 0 at dmpm_set_particles only, 1 also at dmpm_chain_episode, 2 (default) also at
+                              * dmpm_forward_pass(0) when the order is stale (DMPM_OPT_RESORT_PERMILLE) */
+#define DMPM_OPT_RESORT_PERMILLE 5 /* FP32: stale = in the last forward pass that reached the loss, more than this
+                                    * many per mille of the P2G workgroups left the fixed 8^3 window (default 50) */
 int dmpm_set_option(dmpm_ctx* ctx, int32_t option, int64_t value);
 int dmpm_get_option(dmpm_ctx* ctx, int32_t option, int64_t* value);
 /* FP32: time a forward and a backward step of layer 0 for every candidate range size and LDS
  * capacity, keep the fastest (best_ms: its time).  Needs particles; the forward trajectory
  * (layers after 0) and gradients are invalidated. */
 int dmpm_autotune(dmpm_ctx* ctx, double* best_ms);
+
+/* FP32: re-order the particles on the device by the 4^3-cell block of their layer-0 cell (then rank in
+ * cell, then cell in block), as dmpm_set_particles does; every per-particle array follows (controls,
+ * optimiser states, held control gradients), the API order of every other call is unchanged, and the
+ * layers after 0 are invalidated.  A no-op on the FP64 path, which keeps the API order.  No reference
+ * counterpart: the reference's particles have no device order (DiffMPMAnimator3D.cpp:424-429 chains
+ * episodes on the host). */
+int dmpm_sort_particles(dmpm_ctx* ctx);
+/* FP32: stats[0..2] = P2G workgroups on the fixed window, the general LDS window and direct HBM atomics
+ * over the last forward pass that reached dmpm_end_layer_mass_loss; stats[3] = device sorts so far. */
+int dmpm_get_window_stats(dmpm_ctx* ctx, int64_t* stats);
 
 int dmpm_synchronize(dmpm_ctx* ctx);
 /* Device bytes held by the context. */

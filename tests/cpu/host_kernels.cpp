@@ -87,6 +87,20 @@ void launch_loss_particles_b(const Batch&, const float*, const float*, const dou
 {
     fp32_only("loss_particles_b");
 }
+size_t sort_temp_bytes(int, long long)
+{
+    fp32_only("sort_temp_bytes");
+    return 0;
+}
+hipError_t launch_sort_order(const SortArgs&, hipStream_t)
+{
+    fp32_only("sort_order");
+    return hipErrorInvalidValue;
+}
+template <typename T> void launch_gather(const T*, T*, long long, int, int, const int*, hipStream_t) { fp32_only("gather"); }
+template void launch_gather<float>(const float*, float*, long long, int, int, const int*, hipStream_t);
+template void launch_gather<int>(const int*, int*, long long, int, int, const int*, hipStream_t);
+void launch_iota(int*, int, hipStream_t) { fp32_only("iota"); }
 
 }  // namespace dmpm
 
