@@ -123,7 +123,7 @@ struct dmpm_ctx {
     // pass began, and their snapshot at the last EndLayerMassLoss
     unsigned long long* wstats = nullptr;
     long long last_wstats[3] = {0, 0, 0};
-    int resort_mode = 2;            // 0: at set_particles only, 1: also at chain_episode, 2: also when stale
+    int resort_mode = 2;            // 0: at set_particles only; 1: + every chain_episode; 2: chain_episode when stale; 1, 2: forward_pass(0) when stale
     int resort_permille = 50;       // stale: more than this share of P2G workgroups off the fixed window
     long long n_sorts = 0;
     std::vector<int4> host_blocks;  // {begin, end, instance} of every non-empty 4^3-cell block
@@ -545,7 +545,7 @@ int forward_pass(dmpm_ctx* c, int start)
 {
     if (!c->have_particles) return fail(DMPM_ERR_STATE, "forward pass before dmpm_set_particles");
     if (start < 0 || start >= c->L - 1) return fail(DMPM_ERR_INVALID, "start_layer out of range");
-    if (c->f32 && start == 0 && c->resort_mode >= 2 && order_stale(c)) {
+    if (c->f32 && start == 0 && c->resort_mode >= 1 && order_stale(c)) {
         const int rs = resort(c, true);
         if (rs) return rs;
     }
@@ -1167,7 +1167,8 @@ int chain_episode(dmpm_ctx* c)
     c->bwd_control = -1;
     c->tape_seg = -1;
     // the particles moved over the episode: order the new layer 0 afresh
-    if (c->f32 && c->resort_mode >= 1) return resort(c, false);
+    // (mode 1), or when the episode's P2G windows showed a stale order (mode 2)
+    if (c->f32 && (c->resort_mode == 1 || (c->resort_mode == 2 && order_stale(c)))) return resort(c, false);
     return DMPM_OK;
 }
 

@@ -179,6 +179,50 @@ def run_batched(pkg, count, L, dt, warmup, steps):
             "tuned": r["tuned"], "loss": r["loss"]}
 
 
+def run_chained(pkg, sc, vol, L, episodes, resort_mode, seed=11, alpha=1e-3):
+    """Chained episodes as the reference's episode loop runs them (DiffMPMAnimator3D.cpp:421-471),
+    all on the device: an episode is one descent iteration on the layer-0 control (fwd + loss +
+    adjoint, the gradient norm, Descend_Adam) and the forward pass of the updated control, then
+    the chain to the next (layer 0 <- last layer, controls <- 0; with resort_mode >= 1 the particles
+    are re-sorted by block there).  FP32 + checkpoint, graphs, autotuned once; resort_mode is
+    DMPM_OPT_RESORT.  Returns per-episode wall ms (chain included; chain_ms alone), the losses and
+    the P2G window statistics of each episode's first forward pass."""
+    x, v, C, F = sc.state0()
+    mo = pkg.Morph.from_scene(sc, precision=pkg.PREC_F32, tape=pkg.TAPE_CHECKPOINT, num_layers=L, dt=sc.dt)
+    mo.set_particles(x, v, F, C, sc.m, vol, sc.lam, sc.mu)
+    mo.target_from_points(sc.target_x, sc.target_m)
+    mo.set_control(0, pkg.scenes.random_controls(sc.n, 1e-2, seed))
+    mo.autotune()
+    mo.set_graphs(True)
+    mo.set_option(pkg.OPT_RESORT, resort_mode)
+    episode(mo)  # warm-up: graphs captured
+    mo.synchronize()
+    ms, chain_ms, stats, losses = [], [], [], []
+    for e in range(episodes):
+        t0 = time.perf_counter()
+        if e > 0:
+            mo.chain_episode()
+            mo.synchronize()
+        t1 = time.perf_counter()
+        losses.append(episode(mo))
+        stats.append(mo.window_stats())
+        gn = mo.control_grad_norm(0)
+        if gn > 0:
+            mo.descend(0, pkg.DESCEND_ADAM, alpha, gn, timestep=e + 1)
+        mo.forward_pass(0)
+        mo.synchronize()
+        t2 = time.perf_counter()
+        ms.append((t2 - t0) * 1e3)
+        chain_ms.append((t1 - t0) * 1e3)
+    mo.close()
+    tot = [s_["fixed"] + s_["general"] + s_["hbm"] for s_ in stats]
+    return {"ms": [round(t, 3) for t in ms], "chain_ms": [round(t, 3) for t in chain_ms], "loss": losses,
+            "fixed_share": [round(s_["fixed"] / max(t, 1), 4) for s_, t in zip(stats, tot)],
+            "general_share": [round(s_["general"] / max(t, 1), 4) for s_, t in zip(stats, tot)],
+            "hbm_share": [round(s_["hbm"] / max(t, 1), 4) for s_, t in zip(stats, tot)],
+            "sorts": stats[-1]["sorts"] if stats else 0}
+
+
 def loss_curves(pkg, sc, vol, dFc, L, args):
     """Descent on the layer-0 control from the same seed on the FP32+checkpoint and
     plain FP64 (checkpointed, to fit) paths; one JSON line with both loss curves."""
@@ -286,6 +330,8 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=2)
     ap.add_argument("--store-all-steps", type=int, default=3)
     ap.add_argument("--batched-steps", type=int, default=3)
+    ap.add_argument("--episodes", type=int, default=10,
+                    help="chained episodes after the timed region, under each re-sort policy; 0: skip")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--gd-iters", type=int, default=0,
@@ -391,6 +437,15 @@ def main():
         print(f"# batched 24: {batched['ms_per_step']:.2f} ms/episode, {batched['value']:.4e} particle-steps/s "
               f"({batched['particles']} particles)", file=sys.stderr, flush=True)
 
+    chained = None
+    if args.episodes:
+        chained = {"episodes": args.episodes, "episode_steps": args.episode_steps}
+        for key, mode in (("resort_when_stale", 2), ("resort_every_chain", 1), ("no_resort", 0)):
+            r = run_chained(pkg, sc, vol, L, args.episodes, mode)
+            chained[key] = r
+            print(f"# chained {key}: ms {r['ms']}, chain ms {r['chain_ms']}, fixed-window share {r['fixed_share']}, "
+                  f"sorts {r['sorts']}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -438,6 +493,8 @@ def main():
                                  "ms_per_step": rsa["time"] / args.store_all_steps * 1e3, "memory_bytes": rsa["mem"]}
     if batched is not None:
         out["batched"] = batched
+    if chained is not None:
+        out["chained"] = chained
     if r64 is not None:
         v64 = sc.n * args.episode_steps * args.fp64_steps / r64["time"]
         g32, g64 = r32["grad"], r64["grad"]

@@ -193,8 +193,9 @@ int dmpm_chain_episode(dmpm_ctx* ctx);
 #define DMPM_OPT_RANGE_MAX 2 /* FP32: most particles per workgroup range, 64..512 (default 512) */
 #define DMPM_OPT_LDS_CAP 3   /* FP32: node capacity of the general LDS window, 512..4096 (default 1024) */
 #define DMPM_OPT_RESORT 4    /* FP32: when the device re-orders the particles by grid block (dmpm_sort_particles):
-                              * 0 at dmpm_set_particles only, 1 also at dmpm_chain_episode, 2 (default) also at
-                              * dmpm_forward_pass(0) when the order is stale (DMPM_OPT_RESORT_PERMILLE) */
+                              * 0: at dmpm_set_particles only; 1: also at every dmpm_chain_episode and at
+                              * dmpm_forward_pass(0) when the order is stale (DMPM_OPT_RESORT_PERMILLE); 2 (default):
+                              * at dmpm_chain_episode and dmpm_forward_pass(0) when the order is stale */
 #define DMPM_OPT_RESORT_PERMILLE 5 /* FP32: stale = in the last forward pass that reached the loss, more than this
                                     * many per mille of the P2G workgroups left the fixed 8^3 window (default 50) */
 int dmpm_set_option(dmpm_ctx* ctx, int32_t option, int64_t value);

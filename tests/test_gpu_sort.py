@@ -40,7 +40,7 @@ def test_fp32_chained_episode_vs_oracle(resort):
         _episode(mo)
         sorts0 = mo.window_stats()["sorts"]
         mo.chain_episode()
-        assert mo.window_stats()["sorts"] == sorts0 + (1 if resort else 0)
+        assert mo.window_stats()["sorts"] == sorts0 + (1 if resort == 1 else 0)
         x0, v0, F0, C0 = mo.layer_state(0)
         assert np.abs(mo.control(0)).max() == 0.0  # chaining clears the controls
         dFc = pkg.scenes.random_controls(case.sc.n, 1e-2, 77)
@@ -89,6 +89,27 @@ def test_stale_order_is_detected_and_resorted():
         assert st2["sorts"] == sorts + 1 and st2["general"] + st2["hbm"] == 0
         assert loss2 == pytest.approx(ref_loss, rel=1e-6)
         assert rel(g2, gd[0]) < 1e-4
+    finally:
+        mo.close()
+
+
+def test_chain_resorts_a_stale_order_only():
+    """DMPM_OPT_RESORT 2 (the default): a chain after an episode on the fixed window keeps the
+    order; a chain after an episode whose order was stale re-sorts."""
+    case = make_case(seed=7, L=6)
+    mo = gpu_morph(case, precision=pkg.PREC_F32, tape=pkg.TAPE_CHECKPOINT)
+    try:
+        assert mo.option(pkg.OPT_RESORT) == 2
+        _episode(mo)
+        s0 = mo.window_stats()["sorts"]
+        mo.chain_episode()
+        assert mo.window_stats()["sorts"] == s0
+        perm = np.random.default_rng(8).permutation(case.sc.n)
+        mo.set_layer_state(0, *(a[perm] for a in mo.layer_state(0)))
+        _episode(mo)
+        assert mo.window_stats()["general"] + mo.window_stats()["hbm"] > 0
+        mo.chain_episode()
+        assert mo.window_stats()["sorts"] == s0 + 1
     finally:
         mo.close()
 
