@@ -87,7 +87,10 @@ struct dmpm_ctx {
     unsigned* tflags = nullptr;
     long long ntiles = 0;
     size_t grid_slots = 0;
-    void* gbuf[2] = {nullptr, nullptr};
+    // gradient layers: the last layer's (the loss seed, kept across backward
+    // passes as the reference keeps the last layer's dLdx) in gbuf[2], the
+    // others ping-pong between gbuf[0] and gbuf[1] by layer parity
+    void* gbuf[3] = {nullptr, nullptr, nullptr};
     void* gctrl = nullptr;    // (L-1) x 9 x S or 9 x S
     void* tmp = nullptr;      // 12 x S
     void* target = nullptr;   // G
@@ -140,8 +143,16 @@ struct dmpm_ctx {
     long long* d_nofs = nullptr;
     Batch batch{};
     // checkpoint-mode slot layout
-    int n_ckpt = 0, seg_last = 0, tape_base = 0, roll_base = 0, n_slots = 0;
-    int tape_seg = -1;
+    // checkpoint mode: two tapes, segment s kept in tape s & 1 (slots from
+    // tape_base[t], grids from grid slot t K); tape_hold[t]: the segment tape t
+    // holds, or -1.  The backward pass recomputes segment s - 1 into one tape on
+    // stream2 while it steps back through segment s in the other.
+    int n_ckpt = 0, seg_last = 0, roll_base = 0, n_slots = 0;
+    int tape_base[2] = {0, 0};
+    int tape_hold[2] = {-1, -1};
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+    bool overlap = true;
     // validity
     bool have_particles = false;
     bool have_target = false;
@@ -319,21 +330,52 @@ Real* slot(dmpm_ctx* c, int s)
     return static_cast<Real*>(c->layers) + (size_t)s * c->lf * c->S;
 }
 
+// the tape holding segment seg, or -1
+int tape_of(const dmpm_ctx* c, int seg)
+{
+    for (int t = 0; t < 2; ++t)
+        if (c->tape_hold[t] == seg) return t;
+    return -1;
+}
+
+void drop_tapes(dmpm_ctx* c) { c->tape_hold[0] = c->tape_hold[1] = -1; }
+
+// tapes holding a segment that reaches past layer lo (its controls changed) are stale
+void drop_tapes_from(dmpm_ctx* c, int lo)
+{
+    for (int& h : c->tape_hold)
+        if (h >= 0 && h * c->K + c->K > lo) h = -1;
+}
+
+// layer j of segment j / K in tape t (a checkpoint when j is a multiple of K)
 template <typename Real>
-Real* tape_layer(dmpm_ctx* c, int j)  // layer j inside the current tape segment
+Real* tape_slot(dmpm_ctx* c, int t, int j)
+{
+    if (j % c->K == 0) return slot<Real>(c, j / c->K);
+    return slot<Real>(c, c->tape_base[t] + (j - (j / c->K) * c->K));
+}
+
+// grid of step i of segment i / K in tape t
+template <typename Real>
+Real* tape_grid(dmpm_ctx* c, int t, int i)
+{
+    return static_cast<Real*>(c->grids) + ((size_t)t * c->K + (size_t)(i - (i / c->K) * c->K)) * kGridFields * c->G;
+}
+
+template <typename Real>
+Real* tape_layer(dmpm_ctx* c, int j)  // layer j of a segment held by a tape
 {
     if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) return slot<Real>(c, j);
-    if (j % c->K == 0) return slot<Real>(c, j / c->K);
-    return slot<Real>(c, c->tape_base + (j - (j / c->K) * c->K));
+    const int t = tape_of(c, j / c->K);
+    return tape_slot<Real>(c, t < 0 ? 0 : t, j);
 }
 
 template <typename Real>
 Real* grid_of_step(dmpm_ctx* c, int i)
 {
     if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) return static_cast<Real*>(c->grids) + (size_t)i * kGridFields * c->G;
-    int seg = i / c->K;
-    if (seg == c->tape_seg || (c->tape_seg < 0 && seg == c->seg_last))
-        return static_cast<Real*>(c->grids) + (size_t)(i - seg * c->K) * kGridFields * c->G;
+    const int t = tape_of(c, i / c->K);
+    if (t >= 0) return tape_grid<Real>(c, t, i);
     return static_cast<Real*>(c->wgrid);
 }
 
@@ -346,7 +388,7 @@ Real* ctrl_of(dmpm_ctx* c, int layer)
 template <typename Real>
 Real* grads_of(dmpm_ctx* c, int layer)
 {
-    return static_cast<Real*>(c->gbuf[layer & 1]);
+    return static_cast<Real*>(c->gbuf[layer == c->L - 1 ? 2 : (layer & 1)]);
 }
 
 template <typename Real>
@@ -504,7 +546,7 @@ Real* materialize(dmpm_ctx* c, int j)
 {
     if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) return slot<Real>(c, j);
     if (j % c->K == 0) return slot<Real>(c, j / c->K);
-    if (c->tape_seg == j / c->K) return tape_layer<Real>(c, j);
+    if (tape_of(c, j / c->K) >= 0) return tape_layer<Real>(c, j);
     const int j0 = (j / c->K) * c->K;
     Real* prev = slot<Real>(c, j0 / c->K);
     bool clean = false;
@@ -517,23 +559,22 @@ Real* materialize(dmpm_ctx* c, int j)
     return prev;
 }
 
-// Recompute segment seg into the tape (checkpoint mode).
+// Recompute segment seg into its tape, seg & 1 (checkpoint mode), with the
+// launches on c->stream.  The caller records which segment the tape holds.
 template <typename Real>
 void recompute_segment(dmpm_ctx* c, int seg)
 {
+    const int t = seg & 1;
     const int s0 = seg * c->K, s1 = std::min(s0 + c->K, c->L - 1);
-    Real* grids = static_cast<Real*>(c->grids);
     bool clean = false;
     for (int i = s0; i < s1; ++i) {
-        const Real* cur = (i == s0) ? slot<Real>(c, s0 / c->K) : slot<Real>(c, c->tape_base + (i - s0));
         // a segment that ends on a checkpoint needs only the grid of its last
         // step: the backward pass reads the next layer from the checkpoint
         const bool ends_on_ckpt = (i + 1) % c->K == 0;
-        Real* dst = ends_on_ckpt ? slot<Real>(c, c->roll_base) : slot<Real>(c, c->tape_base + (i + 1 - s0));
-        clean = step_forward<Real>(c, cur, dst, ctrl_of<Real>(c, i), grids + (size_t)(i - s0) * kGridFields * c->G, clean,
-                                   i + 1 < s1 ? grids + (size_t)(i + 1 - s0) * kGridFields * c->G : nullptr, true, !ends_on_ckpt);
+        Real* dst = ends_on_ckpt ? slot<Real>(c, c->roll_base) : tape_slot<Real>(c, t, i + 1);
+        clean = step_forward<Real>(c, tape_slot<Real>(c, t, i), dst, ctrl_of<Real>(c, i), tape_grid<Real>(c, t, i), clean,
+                                   i + 1 < s1 ? tape_grid<Real>(c, t, i + 1) : nullptr, true, !ends_on_ckpt);
     }
-    c->tape_seg = seg;
 }
 
 // -------------------------------------------------------------- passes -----
@@ -563,23 +604,19 @@ int forward_pass(dmpm_ctx* c, int start)
     } else {
         from = (from / c->K) * c->K;  // restart from a checkpoint so the tape holds whole segments
         const int s0_last = c->seg_last * c->K;
-        Real* grids = static_cast<Real*>(c->grids);
+        const int tl = c->seg_last & 1;  // the last segment's tape
         rc = run_graph(c, from, [&] {
         zero_wstats(c);
         bool clean = false;
         for (int i = from; i < c->L - 1; ++i) {
             const Real* cur;
-            if (i % c->K == 0) cur = slot<Real>(c, i / c->K);
-            else if (i / c->K == c->seg_last) cur = slot<Real>(c, c->tape_base + (i - s0_last));
+            if (i % c->K == 0 || i / c->K == c->seg_last) cur = tape_slot<Real>(c, tl, i);
             else cur = slot<Real>(c, c->roll_base + (i & 1));
             Real* dst;
             const int j = i + 1;
-            if (j % c->K == 0) dst = slot<Real>(c, j / c->K);
-            else if (j / c->K == c->seg_last) dst = slot<Real>(c, c->tape_base + (j - s0_last));
+            if (j % c->K == 0 || j / c->K == c->seg_last) dst = tape_slot<Real>(c, tl, j);
             else dst = slot<Real>(c, c->roll_base + (j & 1));
-            auto grid_at = [&](int k) {
-                return (k >= s0_last) ? grids + (size_t)(k - s0_last) * kGridFields * c->G : wgrid_of<Real>(c, k);
-            };
+            auto grid_at = [&](int k) { return (k >= s0_last) ? tape_grid<Real>(c, tl, k) : wgrid_of<Real>(c, k); };
             // steps before the last segment are recomputed for the backward pass:
             // their affine matrices are not read
             clean = step_forward<Real>(c, cur, dst, ctrl_of<Real>(c, i), grid_at(i), clean, i + 1 < c->L - 1 ? grid_at(i + 1) : nullptr,
@@ -587,7 +624,8 @@ int forward_pass(dmpm_ctx* c, int start)
         }
         });
         if (rc) return rc;
-        c->tape_seg = c->seg_last;
+        c->tape_hold[tl] = c->seg_last;
+        c->tape_hold[tl ^ 1] = -1;
     }
     HIP_TRY(hipGetLastError());
     c->fwd_valid = c->L - 1;
@@ -665,8 +703,9 @@ int backward_pass(dmpm_ctx* c, int control)
 {
     if (!c->loss_valid) return fail(DMPM_ERR_STATE, "backward pass before EndLayerMassLoss");
     if (control < 0 || control > c->L - 2) return fail(DMPM_ERR_INVALID, "control_layer out of range");
-    // graph key: backward flag, control layer, tape segment held at the start
-    const long long key = (1LL << 40) | ((long long)control << 20) | (long long)(c->tape_seg + 1);
+    // graph key: backward flag, control layer, segments the tapes hold at the start
+    const long long key = (1LL << 62) | ((long long)control << 32) | ((long long)(c->tape_hold[0] + 1) << 16) |
+                          (long long)(c->tape_hold[1] + 1);
     int rc;
     if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) {
         rc = run_graph(c, key, [&] {
@@ -676,22 +715,47 @@ int backward_pass(dmpm_ctx* c, int control)
         });
         if (rc) return rc;
     } else {
-        Real* grids = static_cast<Real*>(c->grids);
+        const int seg_lo = control / c->K;
+        const int hold0[2] = {c->tape_hold[0], c->tape_hold[1]};
+        const bool ov = c->overlap && !c->prof && c->stream2;
         rc = run_graph(c, key, [&] {
             // the adjoint grid chain runs across the recomputed segments (they do not touch it)
+            int hold[2] = {hold0[0], hold0[1]};
+            bool pending[2] = {false, false};
             bool clean = false;
-            for (int seg = c->seg_last; seg >= control / c->K; --seg) {
+            for (int seg = c->seg_last; seg >= seg_lo; --seg) {
+                const int t = seg & 1;
                 const int s0 = seg * c->K, s1 = std::min(s0 + c->K, c->L - 1);
-                if (c->tape_seg != seg) recompute_segment<Real>(c, seg);
-                for (int i = s1 - 1; i >= std::max(s0, control); --i) {
-                    const Real* cur = tape_layer<Real>(c, i);
-                    const Real* nxt = tape_layer<Real>(c, i + 1);
-                    clean = step_backward<Real>(c, i, cur, nxt, grids + (size_t)(i - s0) * kGridFields * c->G, clean, i > control);
+                if (pending[t]) {
+                    hipStreamWaitEvent(c->stream, c->ev_join[t], 0);
+                    pending[t] = false;
+                } else if (hold[t] != seg) {
+                    recompute_segment<Real>(c, seg);
+                    hold[t] = seg;
                 }
+                // recompute the segment below into the other tape on stream2 while
+                // this one steps back (the fork waits for the steps that last read
+                // that tape, those of segment seg + 1)
+                const int sp = seg - 1;
+                if (ov && sp >= seg_lo && hold[sp & 1] != sp) {
+                    hipEventRecord(c->ev_fork, c->stream);
+                    hipStreamWaitEvent(c->stream2, c->ev_fork, 0);
+                    std::swap(c->stream, c->stream2);
+                    recompute_segment<Real>(c, sp);
+                    std::swap(c->stream, c->stream2);
+                    hipEventRecord(c->ev_join[sp & 1], c->stream2);
+                    hold[sp & 1] = sp;
+                    pending[sp & 1] = true;
+                }
+                for (int i = s1 - 1; i >= std::max(s0, control); --i)
+                    clean = step_backward<Real>(c, i, tape_slot<Real>(c, t, i), tape_slot<Real>(c, t, i + 1), tape_grid<Real>(c, t, i),
+                                                clean, i > control);
             }
         });
         if (rc) return rc;
-        c->tape_seg = control / c->K;  // the pass ends holding the control layer's segment
+        // the pass ends holding the control layer's segment and the one above it
+        c->tape_hold[seg_lo & 1] = seg_lo;
+        if (seg_lo + 1 <= c->seg_last) c->tape_hold[(seg_lo + 1) & 1] = seg_lo + 1;
     }
     HIP_TRY(hipGetLastError());
     c->grads_lo = control;
@@ -871,7 +935,7 @@ int resort(dmpm_ctx* c, bool controls_live)
     c->fwd_valid = std::min(c->fwd_valid, 0);
     c->loss_valid = false;
     c->grads_lo = -1;
-    c->tape_seg = -1;
+    drop_tapes(c);
     return DMPM_OK;
 }
 
@@ -915,7 +979,7 @@ int set_particles(dmpm_ctx* c, const double* x, const double* v, const double* F
     c->fwd_valid = 0;
     c->loss_valid = false;
     c->grads_lo = -1;
-    c->tape_seg = -1;
+    drop_tapes(c);
     return DMPM_OK;
 }
 
@@ -1098,15 +1162,17 @@ int set_layer_state(dmpm_ctx* c, int layer, const double* x, const double* v, co
     c->fwd_valid = layer;
     c->loss_valid = false;
     c->grads_lo = -1;
-    if (c->cfg.tape_mode != DMPM_TAPE_STORE_ALL) c->tape_seg = -1;
+    if (c->cfg.tape_mode != DMPM_TAPE_STORE_ALL) drop_tapes(c);
     return DMPM_OK;
 }
 
 template <typename Real>
 int get_layer_grads(dmpm_ctx* c, int layer, double* gx, double* gv, double* gF, double* gC)
 {
-    if (c->grads_lo < 0 || layer < c->grads_lo || layer > std::min(c->grads_lo + 1, c->L - 1))
-        return fail(DMPM_ERR_STATE, "gradients of this layer are not held (only the lowest layer of the last pass and the one above)");
+    const bool seed = layer == c->L - 1 && c->loss_valid;  // the last layer's seed stays held
+    if (!seed && (c->grads_lo < 0 || layer < c->grads_lo || layer > std::min(c->grads_lo + 1, c->L - 1)))
+        return fail(DMPM_ERR_STATE,
+                    "gradients of this layer are not held (only the last layer, the lowest layer of the last pass and the one above)");
     Real* g = grads_of<Real>(c, layer);
     int rc;
     if (gx && (rc = download_fields<Real>(c, g, FX, 3, gx, c->N))) return rc;
@@ -1165,7 +1231,7 @@ int chain_episode(dmpm_ctx* c)
     c->loss_valid = false;
     c->grads_lo = -1;
     c->bwd_control = -1;
-    c->tape_seg = -1;
+    drop_tapes(c);
     // the particles moved over the episode: order the new layer 0 afresh
     // (mode 1), or when the episode's P2G windows showed a stale order (mode 2)
     if (c->f32 && (c->resort_mode == 1 || (c->resort_mode == 2 && order_stale(c)))) return resort(c, false);
@@ -1294,8 +1360,10 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         c->K = K;
         c->n_ckpt = (c->L - 1) / K + 1;  // layers 0, K, 2K, ... <= L-1
         c->seg_last = (c->L - 2) / K;
-        c->tape_base = c->n_ckpt;         // K+1 tape slots (slot 0 unused: the segment start is a checkpoint)
-        c->roll_base = c->tape_base + K + 1;
+        // two tapes of K+1 slots (slot 0 unused: the segment start is a checkpoint)
+        c->tape_base[0] = c->n_ckpt;
+        c->tape_base[1] = c->n_ckpt + K + 1;
+        c->roll_base = c->tape_base[1] + K + 1;
         c->n_slots = c->roll_base + 2;
     } else {
         c->K = steps;
@@ -1310,7 +1378,18 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         delete c;
         return fail(DMPM_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
     }
-    const size_t grid_slots = cfg->tape_mode == DMPM_TAPE_CHECKPOINT ? (size_t)c->K : (size_t)steps;
+    if (cfg->tape_mode == DMPM_TAPE_CHECKPOINT) {
+        // the recompute stream (default priority) and its fork / join events
+        hipError_t e1 = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
+        hipError_t e2 = e1 == hipSuccess ? hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) : e1;
+        hipError_t e3 = e2 == hipSuccess ? hipEventCreateWithFlags(&c->ev_join[0], hipEventDisableTiming) : e2;
+        hipError_t e4 = e3 == hipSuccess ? hipEventCreateWithFlags(&c->ev_join[1], hipEventDisableTiming) : e3;
+        if (e4 != hipSuccess) {
+            dmpm_destroy(c);
+            return fail(DMPM_ERR_HIP, std::string("recompute stream: ") + hipGetErrorString(e4));
+        }
+    }
+    const size_t grid_slots = cfg->tape_mode == DMPM_TAPE_CHECKPOINT ? 2 * (size_t)c->K : (size_t)steps;
     c->grid_slots = grid_slots;
     c->ntiles = c->f32 ? num_tiles((long long)G) : 0;
     const size_t gctrl_layers = cfg->keep_control_grads ? (size_t)steps : 1;
@@ -1319,7 +1398,8 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         (rc = dalloc(c, &c->grids, grid_slots * kGridFields * G * e)) || (rc = dalloc(c, &c->wgrid, kGridFields * G * e)) ||
         (rc = dalloc(c, &c->wgrid2, kGridFields * G * e)) ||
         (rc = dalloc(c, &c->agrid, kGridFields * G * e)) || (rc = dalloc(c, &c->gbuf[0], kLayerFields * S * e)) ||
-        (rc = dalloc(c, &c->gbuf[1], kLayerFields * S * e)) || (rc = dalloc(c, &c->gctrl, gctrl_layers * 9 * S * e)) ||
+        (rc = dalloc(c, &c->gbuf[1], kLayerFields * S * e)) || (rc = dalloc(c, &c->gbuf[2], kLayerFields * S * e)) ||
+        (rc = dalloc(c, &c->gctrl, gctrl_layers * 9 * S * e)) ||
         (rc = dalloc(c, &c->tmp, 12 * S * e)) || (rc = dalloc(c, &c->target, G * e)) ||
         (rc = dalloc(c, (void**)&c->dacc, std::max(64, c->B + 8) * sizeof(double))) ||
         (rc = dalloc(c, (void**)&c->dpart, std::max<long long>(kLossBlocks, loss_partials(c->B)) * sizeof(double))) ||
@@ -1422,7 +1502,7 @@ int dmpm_destroy(dmpm_ctx* c)
     hipSetDevice(c->cfg.device);
     if (c->stream) hipStreamSynchronize(c->stream);
     clear_graphs(c);
-    void* bufs[] = {c->cst,  c->ctrl, c->layers, c->grids, c->wgrid, c->wgrid2, c->agrid, c->gbuf[0], c->gbuf[1],
+    void* bufs[] = {c->cst,  c->ctrl, c->layers, c->grids, c->wgrid, c->wgrid2, c->agrid, c->gbuf[0], c->gbuf[1], c->gbuf[2],
                     c->gctrl, c->tmp, c->target, c->dacc,  c->ranges, c->d_sps, c->d_nofs, c->dpart, c->tflags,
                     c->d_api, c->d_poff, c->d_boff, c->sortbuf, c->wstats};
     for (void* b : bufs)
@@ -1430,6 +1510,12 @@ int dmpm_destroy(dmpm_ctx* c)
     for (void* b : c->opt)
         if (b) hipFree(b);
     for (hipEvent_t ev : c->pool) hipEventDestroy(ev);
+    for (hipEvent_t ev : {c->ev_fork, c->ev_join[0], c->ev_join[1]})
+        if (ev) hipEventDestroy(ev);
+    if (c->stream2) {
+        hipStreamSynchronize(c->stream2);
+        hipStreamDestroy(c->stream2);
+    }
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return DMPM_OK;
@@ -1473,7 +1559,7 @@ int dmpm_set_control(dmpm_ctx* c, int32_t layer, const double* dFc)
     }
     c->fwd_valid = std::min(c->fwd_valid, lo);
     c->loss_valid = false;
-    if (c->cfg.tape_mode != DMPM_TAPE_STORE_ALL && c->tape_seg >= 0 && c->tape_seg * c->K + c->K > lo) c->tape_seg = -1;
+    if (c->cfg.tape_mode != DMPM_TAPE_STORE_ALL) drop_tapes_from(c, lo);
     return DMPM_OK;
 }
 
@@ -1699,7 +1785,7 @@ int dmpm_descend(dmpm_ctx* c, int32_t layer, int32_t rule, float alpha, float gr
     // the control changed: layers after `layer` are stale (as dmpm_set_control)
     c->fwd_valid = std::min(c->fwd_valid, (int)layer);
     c->loss_valid = false;
-    if (c->cfg.tape_mode != DMPM_TAPE_STORE_ALL && c->tape_seg >= 0 && c->tape_seg * c->K + c->K > layer) c->tape_seg = -1;
+    if (c->cfg.tape_mode != DMPM_TAPE_STORE_ALL) drop_tapes_from(c, layer);
     return DMPM_OK;
 }
 
@@ -1750,6 +1836,12 @@ int dmpm_set_option(dmpm_ctx* c, int32_t option, int64_t value)
         if (value < 0 || value > 1000) return fail(DMPM_ERR_INVALID, "DMPM_OPT_RESORT_PERMILLE must be in [0, 1000]");
         c->resort_permille = (int)value;
         return DMPM_OK;
+    case DMPM_OPT_OVERLAP:
+        if (value != 0 && value != 1) return fail(DMPM_ERR_INVALID, "DMPM_OPT_OVERLAP is 0 or 1");
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->overlap = value != 0;
+        clear_graphs(c);
+        return DMPM_OK;
     default:
         return fail(DMPM_ERR_INVALID, "unknown option");
     }
@@ -1764,6 +1856,7 @@ int dmpm_get_option(dmpm_ctx* c, int32_t option, int64_t* value)
     case DMPM_OPT_LDS_CAP: *value = c->lds_cap; return DMPM_OK;
     case DMPM_OPT_RESORT: *value = c->resort_mode; return DMPM_OK;
     case DMPM_OPT_RESORT_PERMILLE: *value = c->resort_permille; return DMPM_OK;
+    case DMPM_OPT_OVERLAP: *value = c->overlap ? 1 : 0; return DMPM_OK;
     default: return fail(DMPM_ERR_INVALID, "unknown option");
     }
 }
@@ -1827,7 +1920,7 @@ int dmpm_autotune(dmpm_ctx* c, double* best_ms)
     c->loss_valid = false;
     c->grads_lo = -1;
     c->bwd_control = -1;
-    c->tape_seg = -1;
+    drop_tapes(c);
     if (rc) return rc;
     if (rc2) return rc2;
     HIP_TRY(hipGetLastError());

@@ -198,6 +198,8 @@ int dmpm_chain_episode(dmpm_ctx* ctx);
                               * at dmpm_chain_episode and dmpm_forward_pass(0) when the order is stale */
 #define DMPM_OPT_RESORT_PERMILLE 5 /* FP32: stale = in the last forward pass that reached the loss, more than this
                                     * many per mille of the P2G workgroups left the fixed 8^3 window (default 50) */
+#define DMPM_OPT_OVERLAP 6   /* checkpoint tape: 1 (default) recomputes the next segment down on a second stream
+                              * while the backward pass steps through the current one (two tapes); 0: one stream */
 int dmpm_set_option(dmpm_ctx* ctx, int32_t option, int64_t value);
 int dmpm_get_option(dmpm_ctx* ctx, int32_t option, int64_t* value);
 /* FP32: time a forward and a backward step of layer 0 for every candidate range size and LDS

@@ -20,6 +20,7 @@ typedef void* hipGraphExec_t;
 typedef void* hipGraphNode_t;
 enum hipMemcpyKind { hipMemcpyHostToHost = 0, hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMemcpyDeviceToDevice = 3 };
 enum { hipStreamNonBlocking = 1 };
+enum { hipEventDisableTiming = 2 };
 enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1 };
 
 struct int2 { int x, y; };
@@ -75,6 +76,8 @@ hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned f);
 hipError_t hipStreamSynchronize(hipStream_t s);
 hipError_t hipStreamDestroy(hipStream_t s);
 hipError_t hipEventCreate(hipEvent_t* e);
+hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned flags);
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned flags);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
 hipError_t hipEventSynchronize(hipEvent_t e);
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);

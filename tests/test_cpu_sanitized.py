@@ -92,6 +92,22 @@ ref_loss, layers, gd, _ = orc.episode(case.prm, case.st0, case.cs, case.controls
 assert abs(loss - ref_loss) <= 1e-12 * abs(ref_loss), (loss, ref_loss)
 assert rel(mo.control_grad(0), gd[0]) < 1e-10
 mo.close()
+# the checkpoint tape (segments of 3 steps in two alternating tapes, the segment below
+# recomputed while the one above steps back), control layer 4: per-layer gradients
+case = make_case(L=11, per_layer_ctrl=True)
+mo = gpu_morph(case, tape=pkg.TAPE_CHECKPOINT, ckpt_stride=3)
+mo.forward_pass(0)
+loss = mo.end_layer_mass_loss()
+mo.backward_pass(4)
+ref_loss, layers, gd, _ = orc.episode(case.prm, case.st0, case.cs, case.controls, case.target, case.L, control_layer=4)
+assert abs(loss - ref_loss) <= 1e-12 * abs(ref_loss), (loss, ref_loss)
+for i in range(4, case.L - 1):
+    assert rel(mo.control_grad(i), gd[i]) < 1e-10, i
+x5 = mo.layer_state(5)[0]
+assert rel(x5, layers[5].x) < 1e-12
+mo.backward_pass(4)  # again, from the tapes the first pass left
+assert rel(mo.control_grad(4), gd[4]) < 1e-10
+mo.close()
 t.test_diverged_particles_touch_no_node(pkg.PREC_F64)
 print("host build ok")
 """

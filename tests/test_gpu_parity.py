@@ -221,6 +221,37 @@ def test_checkpoint_matches_store_all(prec, tol, stride):
     assert maxrel(xb, xa) < (1e-12 if prec == 0 else 1e-6)
 
 
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 2e-5)])
+def test_checkpoint_overlap_and_repeated_backward(prec, tol):
+    """The checkpoint tape's backward pass with the recompute of the segment below on a second
+    stream (default) equals the one-stream pass and the store-all tape; a second backward pass
+    to another control layer after one loss reuses the kept loss seed and the tapes the first
+    pass left (CompGraph's layers keep the last layer's gradients, CompGraph.cpp:258-267)."""
+    case = make_case(L=14, per_layer_ctrl=True)
+    a = gpu_morph(case, precision=prec)
+    a.forward_pass(0)
+    la = a.end_layer_mass_loss()
+    a.backward_pass(0)
+    for overlap in (1, 0):
+        b = gpu_morph(case, precision=prec, tape=pkg.TAPE_CHECKPOINT, ckpt_stride=3)
+        b.set_option(pkg.OPT_OVERLAP, overlap)
+        b.set_graphs(bool(overlap))
+        for rep in range(2):  # a captured pass and its replay
+            b.forward_pass(0)
+            lb = b.end_layer_mass_loss()
+            assert lb == pytest.approx(la, rel=1e-12 if prec == 0 else 1e-6)
+            b.backward_pass(0)
+            for i in range(case.L - 1):
+                assert rel(b.control_grad(i), a.control_grad(i)) < tol, (overlap, rep, i)
+        b.backward_pass(7)  # no new loss: the seed of the last layer is kept
+        for i in range(7, case.L - 1):
+            assert rel(b.control_grad(i), a.control_grad(i)) < tol, (overlap, "again", i)
+        b.backward_pass(0)
+        assert rel(b.control_grad(0), a.control_grad(0)) < tol
+        b.close()
+    a.close()
+
+
 def test_checkpoint_memory_smaller():
     case = make_case(L=41)
     a = gpu_morph(case, keep_control_grads=False)
