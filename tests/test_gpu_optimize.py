@@ -64,6 +64,95 @@ def test_descend_closed_forms(rule, precision):
         mo.end_layer_mass_loss()
 
 
+F32_EPS = float(np.finfo(np.float32).eps)
+
+
+def _adam_lion_rounds(h, alpha, b1, b2, eps, rounds, fail_trial):
+    """PointCloud::Descend_Adam then Descend_Lion (PointCloud.cpp:83-99,134-162) on one shared
+    momentum, as each line-search trial of CompGraph.cpp:391-394 applies them, for a constant
+    normalised gradient h, in float64 with float32 parameters; optionally one failed trial and
+    its undo at -alpha (:406-408) at the end.  Returns the total change of dFc."""
+    a, b1, b2, eps = (np.float64(np.float32(v)) for v in (alpha, b1, b2, eps))
+    m = np.zeros_like(h)
+    v = np.zeros_like(h)
+    d = np.zeros_like(h)
+    sg = np.sign(h)
+
+    def adam(al, t):
+        nonlocal m, v, d
+        m = b1 * m + (1 - b1) * h
+        v = b2 * v + (1 - b2) * h * h
+        d = d - al * (m / np.float64(np.float32(1 - np.float64(b1) ** t))) / (
+            np.sqrt(v / np.float64(np.float32(1 - np.float64(b2) ** t))) + eps)
+
+    def lion(al):
+        nonlocal m, d
+        m = b1 * m + (1 - b1) * sg
+        d = d - al * m
+
+    for t in range(1, rounds + 1):
+        adam(a, t)
+        lion(a)
+    if fail_trial:
+        adam(a, rounds + 1)
+        lion(a)
+        adam(-a, rounds + 1)
+        lion(-a)
+    return d
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 1e-5)])
+def test_adam_then_lion_on_shared_momentum(precision, tol):
+    """Several rounds of the reference's line-search update (Adam then Lion on the same
+    momentum) and a failed trial with its inexact undo, on a fixed gradient:
+    * every round against the float64 restatement of PointCloud.cpp (relative to the largest
+      entry of the total change: Lion's sign term enters Adam's numerator, so entries with
+      |h| << eps step by up to alpha / eps);
+    * the first round from zero moments in closed form: Adam moves dFc by alpha h / (|h| + eps)
+      and leaves m = (1 - beta) h, which Lion then moves to beta (1 - beta) h + (1 - beta) sign h
+      and steps by (Lion first would step by alpha (1 - beta) sign h);
+    * the closed form the rounds approach: the shared momentum tends to
+      m* = (beta h + sign h) / (1 + beta) after Lion, the bias-corrected second moment is h^2
+      exactly, so a round moves dFc by alpha (beta m* + (1 - beta) h) / (|h| + eps) + alpha m*
+      (separate moments would give alpha h / (|h| + eps) + alpha sign h)."""
+    _, mo = _after_backward(precision)
+    g = mo.control_grad(1)
+    gn = mo.control_grad_norm(1)
+    h = g / np.float64(np.float32(gn))  # dmpm_descend takes the norm as a float
+    d0 = mo.control(1)
+    # eps of the order of |h| keeps Adam's steps near alpha for the small entries of h
+    alpha, b1, b2, eps = 1e-3, 0.5, 0.9, 1e-2
+    rounds = 30
+    prev = d0
+    for t in range(1, rounds + 1):
+        mo.descend(1, pkg.DESCEND_ADAM, alpha, gn, b1, b2, eps, t)
+        mo.descend(1, pkg.DESCEND_LION, alpha, gn, b1)
+        if t == 1:
+            first = alpha * h / (np.abs(h) + eps) + alpha * (b1 * (1 - b1) * h + (1 - b1) * np.sign(h))
+            assert np.abs((d0 - mo.control(1)) - first).max() < 2e-6 * alpha + 4 * np.abs(d0).max() * F32_EPS
+        if t in (1, 2, 5, rounds):
+            want = _adam_lion_rounds(h, alpha, b1, b2, eps, t, False)
+            got = mo.control(1) - d0
+            assert np.abs(got - want).max() < tol * np.abs(want).max() + 4 * np.abs(d0).max() * F32_EPS, t
+        if t == rounds - 1:
+            prev = mo.control(1)
+    last = mo.control(1) - prev
+    ms = (b1 * h + np.sign(h)) / (1 + b1)
+    limit = alpha * (b1 * ms + (1 - b1) * h) / (np.abs(h) + eps) + alpha * ms
+    assert np.abs(-last - limit).max() < 1e-6 * np.abs(limit).max() + 4 * np.abs(prev).max() * F32_EPS
+    separate = alpha * h / (np.abs(h) + eps) + alpha * np.sign(h)
+    assert np.abs(-last - separate).max() > 1e-1 * alpha
+    # a failed trial and its undo at -alpha leave dFc moved (the moments moved twice)
+    mo.descend(1, pkg.DESCEND_ADAM, alpha, gn, b1, b2, eps, rounds + 1)
+    mo.descend(1, pkg.DESCEND_LION, alpha, gn, b1)
+    mo.descend(1, pkg.DESCEND_ADAM, -alpha, gn, b1, b2, eps, rounds + 1)
+    mo.descend(1, pkg.DESCEND_LION, -alpha, gn, b1)
+    want = _adam_lion_rounds(h, alpha, b1, b2, eps, rounds, True)
+    got = mo.control(1) - d0
+    assert np.abs(got - want).max() < tol * np.abs(want).max() + 4 * np.abs(d0).max() * F32_EPS
+    assert np.abs(got - _adam_lion_rounds(h, alpha, b1, b2, eps, rounds, False)).max() > 1e-3 * alpha
+
+
 def test_gradient_descent_fp32_tracks_fp64():
     """Normalised GD with line search: the loss decreases monotonically and the
     FP32 curve follows the FP64 one (relative 1e-4 per iteration)."""
