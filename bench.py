@@ -251,6 +251,9 @@ def loss_curves(pkg, sc, vol, dFc, L, args):
 
 
 def cpu_baseline(sc, vol, dFc, threads, steps):
+    """One ``steps``-step episode (forward keeping every layer, loss, adjoint) of the CPU
+    restatement on ``threads`` OpenMP threads, in the float build (the reference's precision)
+    and in double.  Returns {precision: (particle-steps/s, seconds)}."""
     from oracle import oracle as orc
 
     orc.set_threads(threads)
@@ -260,10 +263,16 @@ def cpu_baseline(sc, vol, dFc, threads, steps):
     cs = orc.Consts(sc.m, vol, sc.lam, sc.mu)
     target = orc.p2g_mass(prm, sc.target_x, sc.target_m)
     controls = [dFc] + [np.zeros_like(dFc)] * (steps - 1)
+    out = {}
+    t0 = time.perf_counter()
+    orc.episode_f32(prm, st, cs, controls, target, steps + 1, threads=threads)
+    dt = time.perf_counter() - t0
+    out["f32"] = (sc.n * steps / dt, dt)
     t0 = time.perf_counter()
     orc.episode(prm, st, cs, controls, target, steps + 1)
     dt = time.perf_counter() - t0
-    return sc.n * steps / dt, dt
+    out["f64"] = (sc.n * steps / dt, dt)
+    return out
 
 
 def copy_probe(nbytes):
@@ -448,10 +457,20 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
-        cv, cdt = cpu_baseline(sc, vol, dFc, threads, args.cpu_steps)
-        cpu = {"value": cv, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": f"FP64 oracle, one {args.cpu_steps}-step episode fwd+loss+adjoint of {sc.name} ({cdt:.1f} s)"}
+        # every core this process may run on, at most the 16 host cores that one GPU's share of the
+        # machine provides (OMP_NUM_THREADS)
+        nproc = os.cpu_count() or 1
+        usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+        share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or 16
+        threads = max(1, min(share, usable))
+        cb = cpu_baseline(sc, vol, dFc, threads, args.cpu_steps)
+        cpu = {"value": cb["f32"][0], "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "nproc": nproc, "usable_cores": usable,
+               "sample": f"CPU restatement of the reference step, float build (the reference's precision), one "
+                         f"{args.cpu_steps}-step episode fwd+loss+adjoint of {sc.name} ({cb['f32'][1]:.1f} s)",
+               "f64": {"value": cb["f64"][0], "seconds": cb["f64"][1]}}
+        print(f"# cpu baseline on {threads} of {nproc} cores ({usable} usable): f32 {cb['f32'][0]:.4g}, "
+              f"f64 {cb['f64'][0]:.4g} particle-steps/s", file=sys.stderr, flush=True)
 
     out = {
         "metric": "fwd+adjoint particle-steps/s (FP32+ckpt vs plain FP64) at 1M pts; grad err",

@@ -6,7 +6,8 @@
  * product never links, loads or calls it.
  *
  * Each function names the reference lines it restates (paths relative to the
- * reference repository root).  The arithmetic is double precision throughout;
+ * reference repository root).  The arithmetic is double precision throughout
+ * (orc_real; the float build of the same source is only a timing baseline);
  * the reference itself is float (DiffMPMLib3D/pch.h:24-25).
  *
  * Scatter phases (P2G and the transposed G2P) are parallel over particles
@@ -19,12 +20,14 @@
 #include "dmpm_oracle.h"
 
 #include <float.h>
-#include <math.h>
+#include <tgmath.h> /* fabs, sqrt, floor, ... of the argument's type */
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+
+#define ORC_EPSILON (sizeof(orc_real) == sizeof(float) ? FLT_EPSILON : DBL_EPSILON)
 
 /* std::numeric_limits<float>::epsilon(), the node-mass threshold of
  * ForwardSimulation.cpp:55 and BackPropagation.cpp:55. */
@@ -48,17 +51,17 @@ void orc_set_threads(int nthreads)
 /* ------------------------------------------------------------------ */
 /* Interpolation.h:5-30                                                 */
 /* ------------------------------------------------------------------ */
-double orc_cubic_bspline(double x)
+orc_real orc_cubic_bspline(orc_real x)
 {
-    double a = fabs(x);
+    orc_real a = fabs(x);
     if (a < 1.0) return 0.5 * a * a * a - a * a + 2.0 / 3.0;
     if (a < 2.0) return (2.0 - a) * (2.0 - a) * (2.0 - a) / 6.0;
     return 0.0;
 }
 
-double orc_cubic_bspline_slope(double x)
+orc_real orc_cubic_bspline_slope(orc_real x)
 {
-    double a = fabs(x);
+    orc_real a = fabs(x);
     if (a < 1.0) return 1.5 * x * a - 2.0 * x;
     if (a < 2.0) return -x * a / 2.0 + 2.0 * x - 2.0 * x / a;
     return 0.0;
@@ -67,49 +70,49 @@ double orc_cubic_bspline_slope(double x)
 /* ------------------------------------------------------------------ */
 /* 3x3 helpers (row-major, (i,j) at 3*i+j)                              */
 /* ------------------------------------------------------------------ */
-static void m3_mul(const double* A, const double* B, double* out)
+static void m3_mul(const orc_real* A, const orc_real* B, orc_real* out)
 {
-    double r[9];
+    orc_real r[9];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
             r[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
     memcpy(out, r, sizeof r);
 }
 /* A^T B */
-static void m3_tmul(const double* A, const double* B, double* out)
+static void m3_tmul(const orc_real* A, const orc_real* B, orc_real* out)
 {
-    double r[9];
+    orc_real r[9];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
             r[3 * i + j] = A[i] * B[j] + A[3 + i] * B[3 + j] + A[6 + i] * B[6 + j];
     memcpy(out, r, sizeof r);
 }
 /* A B^T */
-static void m3_mult(const double* A, const double* B, double* out)
+static void m3_mult(const orc_real* A, const orc_real* B, orc_real* out)
 {
-    double r[9];
+    orc_real r[9];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
             r[3 * i + j] = A[3 * i] * B[3 * j] + A[3 * i + 1] * B[3 * j + 1] + A[3 * i + 2] * B[3 * j + 2];
     memcpy(out, r, sizeof r);
 }
-static void m3_transpose(const double* A, double* out)
+static void m3_transpose(const orc_real* A, orc_real* out)
 {
-    double r[9];
+    orc_real r[9];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) r[3 * i + j] = A[3 * j + i];
     memcpy(out, r, sizeof r);
 }
-static double m3_det(const double* A)
+static orc_real m3_det(const orc_real* A)
 {
     return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
            A[2] * (A[3] * A[7] - A[4] * A[6]);
 }
 /* inverse via the adjugate */
-static void m3_inverse(const double* A, double* out)
+static void m3_inverse(const orc_real* A, orc_real* out)
 {
-    double d = m3_det(A);
-    double r[9];
+    orc_real d = m3_det(A);
+    orc_real r[9];
     r[0] = (A[4] * A[8] - A[5] * A[7]) / d;
     r[1] = (A[2] * A[7] - A[1] * A[8]) / d;
     r[2] = (A[1] * A[5] - A[2] * A[4]) / d;
@@ -121,9 +124,9 @@ static void m3_inverse(const double* A, double* out)
     r[8] = (A[0] * A[4] - A[1] * A[3]) / d;
     memcpy(out, r, sizeof r);
 }
-static double m3_inner(const double* A, const double* B)
+static orc_real m3_inner(const orc_real* A, const orc_real* B)
 {
-    double s = 0.0;
+    orc_real s = 0.0;
     for (int i = 0; i < 9; ++i) s += A[i] * B[i];
     return s;
 }
@@ -132,18 +135,18 @@ static double m3_inner(const double* A, const double* B)
 /* SVD / polar: one-sided (Hestenes) Jacobi, then the conventions of    */
 /* qrsvd/ImplicitQRSVD.h:680-686 and :853-875.                          */
 /* ------------------------------------------------------------------ */
-static void swap_col(double* M, int a, int b)
+static void swap_col(orc_real* M, int a, int b)
 {
     for (int k = 0; k < 3; ++k) {
-        double t = M[3 * k + a];
+        orc_real t = M[3 * k + a];
         M[3 * k + a] = M[3 * k + b];
         M[3 * k + b] = t;
     }
 }
 
-int orc_svd3(const double A[9], double U[9], double s[3], double V[9])
+int orc_svd3(const orc_real A[9], orc_real U[9], orc_real s[3], orc_real V[9])
 {
-    double B[9];
+    orc_real B[9];
     memcpy(B, A, sizeof B);
     for (int i = 0; i < 9; ++i) V[i] = (i % 4 == 0) ? 1.0 : 0.0;
     static const int PQ[3][2] = {{0, 1}, {0, 2}, {1, 2}};
@@ -152,22 +155,22 @@ int orc_svd3(const double A[9], double U[9], double s[3], double V[9])
         int rotated = 0;
         for (int r = 0; r < 3; ++r) {
             int p = PQ[r][0], q = PQ[r][1];
-            double al = 0, be = 0, ga = 0;
+            orc_real al = 0, be = 0, ga = 0;
             for (int k = 0; k < 3; ++k) {
                 al += B[3 * k + p] * B[3 * k + p];
                 be += B[3 * k + q] * B[3 * k + q];
                 ga += B[3 * k + p] * B[3 * k + q];
             }
-            if (ga == 0.0 || fabs(ga) <= DBL_EPSILON * sqrt(al * be)) continue;
+            if (ga == 0.0 || fabs(ga) <= ORC_EPSILON * sqrt(al * be)) continue;
             rotated = 1;
-            double zeta = (be - al) / (2.0 * ga);
-            double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-            double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+            orc_real zeta = (be - al) / (2.0 * ga);
+            orc_real t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+            orc_real c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
             for (int k = 0; k < 3; ++k) {
-                double bp = B[3 * k + p], bq = B[3 * k + q];
+                orc_real bp = B[3 * k + p], bq = B[3 * k + q];
                 B[3 * k + p] = c * bp - sn * bq;
                 B[3 * k + q] = sn * bp + c * bq;
-                double vp = V[3 * k + p], vq = V[3 * k + q];
+                orc_real vp = V[3 * k + p], vq = V[3 * k + q];
                 V[3 * k + p] = c * vp - sn * vq;
                 V[3 * k + q] = sn * vp + c * vq;
             }
@@ -180,14 +183,14 @@ int orc_svd3(const double A[9], double U[9], double s[3], double V[9])
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2 - a; ++b)
             if (s[b] < s[b + 1]) {
-                double t = s[b];
+                orc_real t = s[b];
                 s[b] = s[b + 1];
                 s[b + 1] = t;
                 swap_col(B, b, b + 1);
                 swap_col(V, b, b + 1);
             }
-    double smax = s[0] > 0 ? s[0] : 1.0;
-    double tiny = 64.0 * DBL_EPSILON * smax;
+    orc_real smax = s[0] > 0 ? s[0] : 1.0;
+    orc_real tiny = 64.0 * ORC_EPSILON * smax;
     /* left vectors; complete a basis where singular values vanish */
     if (s[0] <= tiny) {
         for (int i = 0; i < 9; ++i) U[i] = (i % 4 == 0) ? 1.0 : 0.0;
@@ -195,14 +198,14 @@ int orc_svd3(const double A[9], double U[9], double s[3], double V[9])
         for (int k = 0; k < 3; ++k) U[3 * k] = B[3 * k] / s[0];
         if (s[1] <= tiny) {
             /* any unit vector orthogonal to u0 */
-            double u0[3] = {U[0], U[3], U[6]};
+            orc_real u0[3] = {U[0], U[3], U[6]};
             int m = (fabs(u0[0]) < fabs(u0[1])) ? ((fabs(u0[0]) < fabs(u0[2])) ? 0 : 2)
                                                 : ((fabs(u0[1]) < fabs(u0[2])) ? 1 : 2);
-            double e[3] = {0, 0, 0};
+            orc_real e[3] = {0, 0, 0};
             e[m] = 1.0;
-            double d = e[0] * u0[0] + e[1] * u0[1] + e[2] * u0[2];
-            double w[3] = {e[0] - d * u0[0], e[1] - d * u0[1], e[2] - d * u0[2]};
-            double nw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+            orc_real d = e[0] * u0[0] + e[1] * u0[1] + e[2] * u0[2];
+            orc_real w[3] = {e[0] - d * u0[0], e[1] - d * u0[1], e[2] - d * u0[2]};
+            orc_real nw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
             for (int k = 0; k < 3; ++k) U[3 * k + 1] = w[k] / nw;
         } else {
             for (int k = 0; k < 3; ++k) U[3 * k + 1] = B[3 * k + 1] / s[1];
@@ -230,12 +233,12 @@ int orc_svd3(const double A[9], double U[9], double s[3], double V[9])
     return sweep;
 }
 
-void orc_polar3(const double A[9], double R[9], double S[9])
+void orc_polar3(const orc_real A[9], orc_real R[9], orc_real S[9])
 {
-    double U[9], s[3], V[9];
+    orc_real U[9], s[3], V[9];
     orc_svd3(A, U, s, V);
     m3_mult(U, V, R); /* U V^T */
-    double VS[9];
+    orc_real VS[9];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) VS[3 * i + j] = V[3 * i + j] * s[j];
     m3_mult(VS, V, S); /* V S V^T */
@@ -245,25 +248,25 @@ void orc_polar3(const double A[9], double R[9], double S[9])
 /* Elasticity.cpp                                                        */
 /* ------------------------------------------------------------------ */
 /* Elasticity.cpp:74-86 */
-double orc_fc_energy(const double F[9], double lam, double mu)
+orc_real orc_fc_energy(const orc_real F[9], orc_real lam, orc_real mu)
 {
-    double U[9], s[3], V[9];
+    orc_real U[9], s[3], V[9];
     orc_svd3(F, U, s, V);
-    double sum = (s[0] - 1) * (s[0] - 1) + (s[1] - 1) * (s[1] - 1) + (s[2] - 1) * (s[2] - 1);
-    double J = m3_det(F);
+    orc_real sum = (s[0] - 1) * (s[0] - 1) + (s[1] - 1) * (s[1] - 1) + (s[2] - 1) * (s[2] - 1);
+    orc_real J = m3_det(F);
     if (J < ORC_J_MIN) J = ORC_J_MIN;
     return mu * sum + 0.5 * lam * (J - 1) * (J - 1);
 }
 
 /* Elasticity.cpp:88-98: P = 2 mu (F - R) + lam (J-1) J F^-T, J clamped. */
-void orc_pk1(const double F[9], double lam, double mu, double P[9])
+void orc_pk1(const orc_real F[9], orc_real lam, orc_real mu, orc_real P[9])
 {
-    double R[9], S[9], Fi[9];
+    orc_real R[9], S[9], Fi[9];
     orc_polar3(F, R, S);
-    double J = m3_det(F);
+    orc_real J = m3_det(F);
     if (J < ORC_J_MIN) J = ORC_J_MIN;
     m3_inverse(F, Fi);
-    double k = lam * (J - 1.0) * J;
+    orc_real k = lam * (J - 1.0) * J;
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
             P[3 * i + j] = 2.0 * mu * (F[3 * i + j] - R[3 * i + j]) + k * Fi[3 * j + i];
@@ -275,63 +278,63 @@ void orc_pk1(const double F[9], double lam, double mu, double P[9])
  * (:144-173), written in the singular basis: with M = U^T dF V,
  * dR = U w V^T, w_ij = (M_ij - M_ji)/(s_i + s_j); a vanishing s_i + s_j
  * takes the pseudo-inverse (w_ij = 0). */
-void orc_hess_mult(const double F[9], double lam, double mu, const double dF[9], double out[9],
+void orc_hess_mult(const orc_real F[9], orc_real lam, orc_real mu, const orc_real dF[9], orc_real out[9],
                    int exact)
 {
-    double U[9], s[3], V[9];
+    orc_real U[9], s[3], V[9];
     orc_svd3(F, U, s, V);
-    double Ju = m3_det(F);
-    double J = Ju < ORC_J_MIN ? ORC_J_MIN : Ju;
-    double Fi[9], Fit[9];
+    orc_real Ju = m3_det(F);
+    orc_real J = Ju < ORC_J_MIN ? ORC_J_MIN : Ju;
+    orc_real Fi[9], Fit[9];
     m3_inverse(F, Fi);
     m3_transpose(Fi, Fit);
     for (int i = 0; i < 9; ++i) out[i] = 2.0 * mu * dF[i];
     /* Fit dF^T Fit */
-    double t1[9], FdF[9];
+    orc_real t1[9], FdF[9];
     m3_mult(Fit, dF, t1); /* Fit dF^T */
     m3_mul(t1, Fit, FdF);
     if (exact && Ju <= ORC_J_MIN) {
-        double k = lam * (J - 1.0) * J;
+        orc_real k = lam * (J - 1.0) * J;
         for (int i = 0; i < 9; ++i) out[i] -= k * FdF[i];
     } else {
-        double inner = J * m3_inner(Fit, dF);
+        orc_real inner = J * m3_inner(Fit, dF);
         for (int i = 0; i < 9; ++i) out[i] += lam * J * Fit[i] * inner;
-        double fd = m3_inner(Fit, dF);
-        double k = lam * (J - 1.0) * Ju;
+        orc_real fd = m3_inner(Fit, dF);
+        orc_real k = lam * (J - 1.0) * Ju;
         for (int i = 0; i < 9; ++i) out[i] += k * (fd * Fit[i] - FdF[i]);
     }
-    double M[9], tmp[9];
+    orc_real M[9], tmp[9];
     m3_tmul(U, dF, tmp);
     m3_mul(tmp, V, M);
-    double w[9] = {0};
+    orc_real w[9] = {0};
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
             if (i == j) continue;
-            double den = s[i] + s[j];
-            double scale = fabs(s[0]) > 1.0 ? fabs(s[0]) : 1.0;
+            orc_real den = s[i] + s[j];
+            orc_real scale = fabs(s[0]) > 1.0 ? fabs(s[0]) : 1.0;
             w[3 * i + j] = (fabs(den) > 1e-12 * scale) ? (M[3 * i + j] - M[3 * j + i]) / den : 0.0;
         }
-    double dR[9];
+    orc_real dR[9];
     m3_mul(U, w, tmp);
     m3_mult(tmp, V, dR);
     for (int i = 0; i < 9; ++i) out[i] -= 2.0 * mu * dR[i];
 }
 
 /* Elasticity.cpp:231-235 */
-void orc_lame(double E, double nu, double* lam, double* mu)
+void orc_lame(orc_real E, orc_real nu, orc_real* lam, orc_real* mu)
 {
     *lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
     *mu = E / (2.0 + 2.0 * nu);
 }
 
-void orc_pk1_batch(int n, const double* F, const double* lam, const double* mu, double* P)
+void orc_pk1_batch(int n, const orc_real* F, const orc_real* lam, const orc_real* mu, orc_real* P)
 {
 #pragma omp parallel for schedule(static)
     for (int p = 0; p < n; ++p) orc_pk1(F + 9 * p, lam[p], mu[p], P + 9 * p);
 }
 
-void orc_hess_mult_batch(int n, const double* F, const double* lam, const double* mu,
-                         const double* dF, double* out, int exact)
+void orc_hess_mult_batch(int n, const orc_real* F, const orc_real* lam, const orc_real* mu,
+                         const orc_real* dF, orc_real* out, int exact)
 {
 #pragma omp parallel for schedule(static)
     for (int p = 0; p < n; ++p)
@@ -343,23 +346,23 @@ void orc_hess_mult_batch(int n, const double* F, const double* lam, const double
 /* ------------------------------------------------------------------ */
 typedef struct {
     int base[3];
-    double N[3][4];  /* N(u) per axis and offset */
-    double dN[3][4]; /* N'(u) */
-    double d[3][4];  /* x_i - x_p per axis and offset */
+    orc_real N[3][4];  /* N(u) per axis and offset */
+    orc_real dN[3][4]; /* N'(u) */
+    orc_real d[3][4];  /* x_i - x_p per axis and offset */
 } orc_stencil;
 
-static void stencil(const orc_params* prm, const double* xp, orc_stencil* st, int with_slope)
+static void stencil(const orc_params* prm, const orc_real* xp, orc_stencil* st, int with_slope)
 {
     for (int a = 0; a < 3; ++a) {
-        double rel = xp[a] - prm->min_point[a];
+        orc_real rel = xp[a] - prm->min_point[a];
         /* clamped to +-2^20 cells (inf and NaN included) before the int
          * conversion: every node of such a particle is outside the grid, and
          * base + 3 cannot overflow */
         st->base[a] = (int)fmin(fmax(floor(rel / prm->dx), -1048576.0), 1048576.0) - 1;
         for (int o = 0; o < 4; ++o) {
-            double xi = prm->min_point[a] + (double)(st->base[a] + o) * prm->dx;
-            double d = xi - xp[a];
-            double u = d / prm->dx;
+            orc_real xi = prm->min_point[a] + (orc_real)(st->base[a] + o) * prm->dx;
+            orc_real d = xi - xp[a];
+            orc_real u = d / prm->dx;
             st->d[a][o] = d;
             st->N[a][o] = orc_cubic_bspline(u);
             st->dN[a][o] = with_slope ? orc_cubic_bspline_slope(u) : 0.0;
@@ -386,18 +389,18 @@ static long grid_size(const orc_params* prm)
  * 11-51), into a shared grid with OpenMP atomics (shared = 1) or into a grid
  * only this thread writes (shared = 0: the proxy grids of
  * MultiThreadForwardSimulation.cpp:171-181). */
-static void p2g_particle(const orc_params* prm, int p, const double* x, const double* v, const double* C,
-                         const double* F, const double* dFc, const double* m, const double* vol,
-                         const double* lam, const double* mu, double dt, double drag,
-                         double* grid_m, double* grid_p, int shared)
+static void p2g_particle(const orc_params* prm, int p, const orc_real* x, const orc_real* v, const orc_real* C,
+                         const orc_real* F, const orc_real* dFc, const orc_real* m, const orc_real* vol,
+                         const orc_real* lam, const orc_real* mu, orc_real dt, orc_real drag,
+                         orc_real* grid_m, orc_real* grid_p, int shared)
 {
-    double Dinv = 3.0 / (prm->dx * prm->dx);
-    double Fc[9], P[9], PF[9], A[9];
+    orc_real Dinv = 3.0 / (prm->dx * prm->dx);
+    orc_real Fc[9], P[9], PF[9], A[9];
     for (int i = 0; i < 9; ++i) Fc[i] = F[9 * p + i] + dFc[9 * p + i];
     orc_pk1(Fc, lam[p], mu[p], P);
     m3_mult(P, Fc, PF); /* P Fc^T */
     for (int i = 0; i < 9; ++i) A[i] = -Dinv * dt * vol[p] * PF[i] + m[p] * C[9 * p + i];
-    double mv[3];
+    orc_real mv[3];
     for (int a = 0; a < 3; ++a) mv[a] = m[p] * v[3 * p + a] * (1.0 - dt * drag);
     orc_stencil st;
     stencil(prm, x + 3 * p, &st, 0);
@@ -406,9 +409,9 @@ static void p2g_particle(const orc_params* prm, int p, const double* x, const do
             for (int k = 0; k < 4; ++k) {
                 long id = node_index(prm, &st, i, j, k);
                 if (id < 0) continue;
-                double w = st.N[0][i] * st.N[1][j] * st.N[2][k];
-                double d[3] = {st.d[0][i], st.d[1][j], st.d[2][k]};
-                double dm = w * m[p];
+                orc_real w = st.N[0][i] * st.N[1][j] * st.N[2][k];
+                orc_real d[3] = {st.d[0][i], st.d[1][j], st.d[2][k]};
+                orc_real dm = w * m[p];
                 if (shared) {
 #pragma omp atomic
                     grid_m[id] += dm;
@@ -416,7 +419,7 @@ static void p2g_particle(const orc_params* prm, int p, const double* x, const do
                     grid_m[id] += dm;
                 }
                 for (int a = 0; a < 3; ++a) {
-                    double dp = w * (mv[a] + A[3 * a] * d[0] + A[3 * a + 1] * d[1] + A[3 * a + 2] * d[2]);
+                    orc_real dp = w * (mv[a] + A[3 * a] * d[0] + A[3 * a + 1] * d[1] + A[3 * a + 2] * d[2]);
                     if (shared) {
 #pragma omp atomic
                         grid_p[3 * id + a] += dp;
@@ -427,23 +430,23 @@ static void p2g_particle(const orc_params* prm, int p, const double* x, const do
             }
 }
 
-void orc_p2g(const orc_params* prm, const double* x, const double* v, const double* C,
-             const double* F, const double* dFc, const double* m, const double* vol,
-             const double* lam, const double* mu, double dt, double drag,
-             double* grid_m, double* grid_p)
+void orc_p2g(const orc_params* prm, const orc_real* x, const orc_real* v, const orc_real* C,
+             const orc_real* F, const orc_real* dFc, const orc_real* m, const orc_real* vol,
+             const orc_real* lam, const orc_real* mu, orc_real dt, orc_real drag,
+             orc_real* grid_m, orc_real* grid_p)
 {
     long G = grid_size(prm);
-    memset(grid_m, 0, sizeof(double) * G);
-    memset(grid_p, 0, sizeof(double) * 3 * G);
+    memset(grid_m, 0, sizeof(orc_real) * G);
+    memset(grid_p, 0, sizeof(orc_real) * 3 * G);
 #pragma omp parallel for schedule(static)
     for (int p = 0; p < prm->n; ++p)
         p2g_particle(prm, p, x, v, C, F, dFc, m, vol, lam, mu, dt, drag, grid_m, grid_p, 1);
 }
 
-void orc_p2g_mass(const orc_params* prm, const double* x, const double* m, double* grid_m)
+void orc_p2g_mass(const orc_params* prm, const orc_real* x, const orc_real* m, orc_real* grid_m)
 {
     long G = grid_size(prm);
-    memset(grid_m, 0, sizeof(double) * G);
+    memset(grid_m, 0, sizeof(orc_real) * G);
 #pragma omp parallel for schedule(static)
     for (int p = 0; p < prm->n; ++p) {
         orc_stencil st;
@@ -453,7 +456,7 @@ void orc_p2g_mass(const orc_params* prm, const double* x, const double* m, doubl
                 for (int k = 0; k < 4; ++k) {
                     long id = node_index(prm, &st, i, j, k);
                     if (id < 0) continue;
-                    double dm = st.N[0][i] * st.N[1][j] * st.N[2][k] * m[p];
+                    orc_real dm = st.N[0][i] * st.N[1][j] * st.N[2][k] * m[p];
 #pragma omp atomic
                     grid_m[id] += dm;
                 }
@@ -465,7 +468,7 @@ void orc_p2g_mass(const orc_params* prm, const double* x, const double* m, doubl
 /* ------------------------------------------------------------------ */
 /* G_op on one node: ForwardSimulation.cpp:53-58,184-200 (v stays 0 from
  * the reset); grid_v holds the momentum on entry. */
-static void grid_op_node(const orc_params* prm, long i, const double* grid_m, double* grid_v)
+static void grid_op_node(const orc_params* prm, long i, const orc_real* grid_m, orc_real* grid_v)
 {
     if (fabs(grid_m[i]) > ORC_MASS_EPS) {
         for (int a = 0; a < 3; ++a) grid_v[3 * i + a] = grid_v[3 * i + a] / grid_m[i] + prm->dt * prm->f_ext[a];
@@ -476,30 +479,30 @@ static void grid_op_node(const orc_params* prm, long i, const double* grid_m, do
 
 /* G2P + P_op_2: ForwardSimulation.cpp:60-106,108-124 (s = 0 is the
  * overload without smoothing, :108-112) */
-static void g2p_update(const orc_params* prm, double s, const double* x, const double* F, const double* dFc,
-                       const double* grid_v, double* x1, double* v1, double* C1, double* F1)
+static void g2p_update(const orc_params* prm, orc_real s, const orc_real* x, const orc_real* F, const orc_real* dFc,
+                       const orc_real* grid_v, orc_real* x1, orc_real* v1, orc_real* C1, orc_real* F1)
 {
-    double Dinv = 3.0 / (prm->dx * prm->dx);
-    double dt = prm->dt;
+    orc_real Dinv = 3.0 / (prm->dx * prm->dx);
+    orc_real dt = prm->dt;
 #pragma omp parallel for schedule(static)
     for (int p = 0; p < prm->n; ++p) {
         orc_stencil st;
         stencil(prm, x + 3 * p, &st, 0);
-        double vp[3] = {0, 0, 0}, Cp[9] = {0};
+        orc_real vp[3] = {0, 0, 0}, Cp[9] = {0};
         for (int i = 0; i < 4; ++i)
             for (int j = 0; j < 4; ++j)
                 for (int k = 0; k < 4; ++k) {
                     long id = node_index(prm, &st, i, j, k);
                     if (id < 0) continue;
-                    double w = st.N[0][i] * st.N[1][j] * st.N[2][k];
-                    double d[3] = {st.d[0][i], st.d[1][j], st.d[2][k]};
+                    orc_real w = st.N[0][i] * st.N[1][j] * st.N[2][k];
+                    orc_real d[3] = {st.d[0][i], st.d[1][j], st.d[2][k]};
                     for (int a = 0; a < 3; ++a) {
-                        double vi = grid_v[3 * id + a];
+                        orc_real vi = grid_v[3 * id + a];
                         vp[a] += w * vi;
                         for (int b = 0; b < 3; ++b) Cp[3 * a + b] += Dinv * w * vi * d[b];
                     }
                 }
-        double Fc[9], IC[9], Fn[9];
+        orc_real Fc[9], IC[9], Fn[9];
         for (int i = 0; i < 9; ++i) {
             Fc[i] = F[9 * p + i] + dFc[9 * p + i];
             IC[i] = ((i % 4 == 0) ? 1.0 : 0.0) + dt * Cp[i];
@@ -517,11 +520,11 @@ static void g2p_update(const orc_params* prm, double s, const double* x, const d
 }
 
 void orc_forward_step(const orc_params* prm,
-                      const double* x, const double* v, const double* C, const double* F,
-                      const double* dFc, const double* m, const double* vol,
-                      const double* lam, const double* mu,
-                      double* grid_m, double* grid_v,
-                      double* x1, double* v1, double* C1, double* F1)
+                      const orc_real* x, const orc_real* v, const orc_real* C, const orc_real* F,
+                      const orc_real* dFc, const orc_real* m, const orc_real* vol,
+                      const orc_real* lam, const orc_real* mu,
+                      orc_real* grid_m, orc_real* grid_v,
+                      orc_real* x1, orc_real* v1, orc_real* C1, orc_real* F1)
 {
     long G = grid_size(prm);
     /* P_op_1 + G_Reset + P2G; grid_v holds the momentum until G_op */
@@ -541,29 +544,29 @@ void orc_forward_step(const orc_params* prm,
  * smoothing follow (:210-225).  With a fixed section count the result does
  * not depend on the thread count or schedule. */
 int orc_forward_step_proxy(const orc_params* prm, int sections,
-                           const double* x, const double* v, const double* C, const double* F,
-                           const double* dFc, const double* m, const double* vol,
-                           const double* lam, const double* mu,
-                           double* grid_m, double* grid_v,
-                           double* x1, double* v1, double* C1, double* F1)
+                           const orc_real* x, const orc_real* v, const orc_real* C, const orc_real* F,
+                           const orc_real* dFc, const orc_real* m, const orc_real* vol,
+                           const orc_real* lam, const orc_real* mu,
+                           orc_real* grid_m, orc_real* grid_v,
+                           orc_real* x1, orc_real* v1, orc_real* C1, orc_real* F1)
 {
     if (sections < 1) return -1;
     long G = grid_size(prm);
-    double* proxy = (double*)calloc((size_t)sections * 4 * (size_t)G, sizeof(double));
+    orc_real* proxy = (orc_real*)calloc((size_t)sections * 4 * (size_t)G, sizeof(orc_real));
     if (!proxy) return -2;
     int per = prm->n / sections, first = per + prm->n % sections;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int t = 0; t < sections; ++t) {
         int begin = t == 0 ? 0 : first + (t - 1) * per, count = t == 0 ? first : per;
-        double* pm = proxy + (size_t)t * 4 * G;
+        orc_real* pm = proxy + (size_t)t * 4 * G;
         for (int p = begin; p < begin + count; ++p)
             p2g_particle(prm, p, x, v, C, F, dFc, m, vol, lam, mu, prm->dt, prm->drag, pm, pm + G, 0);
     }
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < G; ++i) {
-        double gm = 0.0, gp[3] = {0.0, 0.0, 0.0};
+        orc_real gm = 0.0, gp[3] = {0.0, 0.0, 0.0};
         for (int t = 0; t < sections; ++t) {
-            const double* pm = proxy + (size_t)t * 4 * G;
+            const orc_real* pm = proxy + (size_t)t * 4 * G;
             gm += pm[i];
             for (int a = 0; a < 3; ++a) gp[a] += pm[G + 3 * i + a];
         }
@@ -578,16 +581,16 @@ int orc_forward_step_proxy(const orc_params* prm, int sections,
 
 
 /* ForwardSimulation.cpp:262-290: vol = m / sum_i w_ip m_i (no dx^3 factor) */
-void orc_point_volumes(const orc_params* prm, const double* x, const double* m, double* vol)
+void orc_point_volumes(const orc_params* prm, const orc_real* x, const orc_real* m, orc_real* vol)
 {
     long G = grid_size(prm);
-    double* gm = (double*)malloc(sizeof(double) * (size_t)G);
+    orc_real* gm = (orc_real*)malloc(sizeof(orc_real) * (size_t)G);
     orc_p2g_mass(prm, x, m, gm);
 #pragma omp parallel for schedule(static)
     for (int p = 0; p < prm->n; ++p) {
         orc_stencil st;
         stencil(prm, x + 3 * p, &st, 0);
-        double mg = 0.0;
+        orc_real mg = 0.0;
         for (int i = 0; i < 4; ++i)
             for (int j = 0; j < 4; ++j)
                 for (int k = 0; k < 4; ++k) {
@@ -603,39 +606,39 @@ void orc_point_volumes(const orc_params* prm, const double* x, const double* m, 
 /* ------------------------------------------------------------------ */
 /* EndLayerMassLoss: CompGraph.cpp:110-244                              */
 /* ------------------------------------------------------------------ */
-double orc_end_mass_loss(const orc_params* prm, const double* x, const double* m,
-                         const double* target_m, double* grid_m, double* grid_dLdm,
-                         double* dLdx)
+orc_real orc_end_mass_loss(const orc_params* prm, const orc_real* x, const orc_real* m,
+                         const orc_real* target_m, orc_real* grid_m, orc_real* grid_dLdm,
+                         orc_real* dLdx)
 {
     long G = grid_size(prm);
     orc_p2g_mass(prm, x, m, grid_m);
-    double loss = 0.0;
+    orc_real loss = 0.0;
 #pragma omp parallel for schedule(static) reduction(+ : loss)
     for (long i = 0; i < G; ++i) {
-        double cm = grid_m[i], tm = target_m[i];
-        double c = log(cm + 1.0 + ORC_LOSS_EPS) - log(tm + 1.0 + ORC_LOSS_EPS);
-        double l = 0.5 * c * c;
-        double g = c / (cm + 1.0 + ORC_LOSS_EPS);
+        orc_real cm = grid_m[i], tm = target_m[i];
+        orc_real c = log(cm + 1.0 + ORC_LOSS_EPS) - log(tm + 1.0 + ORC_LOSS_EPS);
+        orc_real l = 0.5 * c * c;
+        orc_real g = c / (cm + 1.0 + ORC_LOSS_EPS);
         if (cm < ORC_MIN_MASS) {
-            double diff = ORC_MIN_MASS - cm;
+            orc_real diff = ORC_MIN_MASS - cm;
             l += ORC_PENALTY * diff * diff;
             g += -2.0 * ORC_PENALTY * diff;
         }
         loss += l;
         grid_dLdm[i] = g;
     }
-    double inv_dx = 1.0 / prm->dx;
+    orc_real inv_dx = 1.0 / prm->dx;
 #pragma omp parallel for schedule(static)
     for (int p = 0; p < prm->n; ++p) {
         orc_stencil st;
         stencil(prm, x + 3 * p, &st, 1);
-        double gx[3] = {0, 0, 0};
+        orc_real gx[3] = {0, 0, 0};
         for (int i = 0; i < 4; ++i)
             for (int j = 0; j < 4; ++j)
                 for (int k = 0; k < 4; ++k) {
                     long id = node_index(prm, &st, i, j, k);
                     if (id < 0) continue;
-                    double gw[3] = {-inv_dx * st.dN[0][i] * st.N[1][j] * st.N[2][k],
+                    orc_real gw[3] = {-inv_dx * st.dN[0][i] * st.N[1][j] * st.N[2][k],
                                     -inv_dx * st.N[0][i] * st.dN[1][j] * st.N[2][k],
                                     -inv_dx * st.N[0][i] * st.N[1][j] * st.dN[2][k]};
                     for (int a = 0; a < 3; ++a) gx[a] += m[p] * grid_dLdm[id] * gw[a];
@@ -664,27 +667,27 @@ double orc_end_mass_loss(const orc_params* prm, const double* x, const double* m
 /* = the reference's dLdF, including defect (a).                        */
 /* ------------------------------------------------------------------ */
 void orc_backward_step(const orc_params* prm,
-                       const double* x, const double* v, const double* C, const double* F,
-                       const double* dFc, const double* m, const double* vol,
-                       const double* lam, const double* mu,
-                       const double* grid_m, const double* grid_v,
-                       const double* C1, const double* gx1, const double* gv1,
-                       const double* gF1, const double* gC1,
-                       double* grid_vbar, double* grid_mbar,
-                       double* gx, double* gv, double* gF, double* gC, double* gdFc)
+                       const orc_real* x, const orc_real* v, const orc_real* C, const orc_real* F,
+                       const orc_real* dFc, const orc_real* m, const orc_real* vol,
+                       const orc_real* lam, const orc_real* mu,
+                       const orc_real* grid_m, const orc_real* grid_v,
+                       const orc_real* C1, const orc_real* gx1, const orc_real* gv1,
+                       const orc_real* gF1, const orc_real* gC1,
+                       orc_real* grid_vbar, orc_real* grid_mbar,
+                       orc_real* gx, orc_real* gv, orc_real* gF, orc_real* gC, orc_real* gdFc)
 {
     long G = grid_size(prm);
     const int cp = prm->compat;
-    const double dt = prm->dt, s = prm->smoothing, drag = prm->drag;
-    const double Dinv = 3.0 / (prm->dx * prm->dx);
-    const double inv_dx = 1.0 / prm->dx;
-    const double cs = (cp & ORC_COMPAT_NO_1MS) ? 1.0 : (1.0 - s);
+    const orc_real dt = prm->dt, s = prm->smoothing, drag = prm->drag;
+    const orc_real Dinv = 3.0 / (prm->dx * prm->dx);
+    const orc_real inv_dx = 1.0 / prm->dx;
+    const orc_real cs = (cp & ORC_COMPAT_NO_1MS) ? 1.0 : (1.0 - s);
 
     /* G2P transposed: scatter vbar_i */
-    memset(grid_vbar, 0, sizeof(double) * 3 * G);
+    memset(grid_vbar, 0, sizeof(orc_real) * 3 * G);
 #pragma omp parallel for schedule(static)
     for (int p = 0; p < prm->n; ++p) {
-        double Fc[9], vt[3], Ct[9], t[9];
+        orc_real Fc[9], vt[3], Ct[9], t[9];
         for (int i = 0; i < 9; ++i) Fc[i] = F[9 * p + i] + dFc[9 * p + i];
         for (int a = 0; a < 3; ++a) vt[a] = gv1[3 * p + a] + dt * gx1[3 * p + a];
         m3_mult(gF1 + 9 * p, Fc, t); /* F' Fc^T */
@@ -696,10 +699,10 @@ void orc_backward_step(const orc_params* prm,
                 for (int k = 0; k < 4; ++k) {
                     long id = node_index(prm, &st, i, j, k);
                     if (id < 0) continue;
-                    double w = st.N[0][i] * st.N[1][j] * st.N[2][k];
-                    double d[3] = {st.d[0][i], st.d[1][j], st.d[2][k]};
+                    orc_real w = st.N[0][i] * st.N[1][j] * st.N[2][k];
+                    orc_real d[3] = {st.d[0][i], st.d[1][j], st.d[2][k]};
                     for (int a = 0; a < 3; ++a) {
-                        double val = w * (vt[a] + Dinv * (Ct[3 * a] * d[0] + Ct[3 * a + 1] * d[1] + Ct[3 * a + 2] * d[2]));
+                        orc_real val = w * (vt[a] + Dinv * (Ct[3 * a] * d[0] + Ct[3 * a + 1] * d[1] + Ct[3 * a + 2] * d[2]));
 #pragma omp atomic
                         grid_vbar[3 * id + a] += val;
                     }
@@ -709,11 +712,11 @@ void orc_backward_step(const orc_params* prm,
     /* G_op transposed; pbar overwrites vbar */
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < G; ++i) {
-        double mi = grid_m[i];
+        orc_real mi = grid_m[i];
         if (fabs(mi) > ORC_MASS_EPS) {
-            double dotv = 0.0;
+            orc_real dotv = 0.0;
             for (int a = 0; a < 3; ++a) {
-                double vi = grid_v[3 * i + a];
+                orc_real vi = grid_v[3 * i + a];
                 if (!(cp & ORC_COMPAT_DLDM_NO_FEXT)) vi -= dt * prm->f_ext[a];
                 dotv += vi * grid_vbar[3 * i + a];
             }
@@ -728,22 +731,22 @@ void orc_backward_step(const orc_params* prm,
     /* P2G transposed (gather) + P_op_1 transposed */
 #pragma omp parallel for schedule(static)
     for (int p = 0; p < prm->n; ++p) {
-        const double mp = m[p];
-        double Fc[9], P[9], PF[9], Gm[9], t[9];
+        const orc_real mp = m[p];
+        orc_real Fc[9], P[9], PF[9], Gm[9], t[9];
         for (int i = 0; i < 9; ++i) Fc[i] = F[9 * p + i] + dFc[9 * p + i];
         orc_pk1(Fc, lam[p], mu[p], P);
-        const double kk = Dinv * dt * vol[p];
+        const orc_real kk = Dinv * dt * vol[p];
         m3_mult(P, Fc, PF);
         for (int i = 0; i < 9; ++i) Gm[i] = -kk * PF[i] + mp * C[9 * p + i];
-        double b[3];
-        const double bscale = (cp & ORC_COMPAT_NO_DRAG_X) ? 1.0 : (1.0 - dt * drag);
+        orc_real b[3];
+        const orc_real bscale = (cp & ORC_COMPAT_NO_DRAG_X) ? 1.0 : (1.0 - dt * drag);
         for (int a = 0; a < 3; ++a) b[a] = mp * bscale * v[3 * p + a];
-        double vt[3], Ct[9];
+        orc_real vt[3], Ct[9];
         for (int a = 0; a < 3; ++a) vt[a] = gv1[3 * p + a] + dt * gx1[3 * p + a];
         m3_mult(gF1 + 9 * p, Fc, t);
         for (int i = 0; i < 9; ++i) Ct[i] = gC1[9 * p + i] + cs * dt * t[i];
 
-        double Gb[9] = {0}, xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0};
+        orc_real Gb[9] = {0}, xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0};
         orc_stencil st;
         stencil(prm, x + 3 * p, &st, 1);
         for (int i = 0; i < 4; ++i)
@@ -751,34 +754,34 @@ void orc_backward_step(const orc_params* prm,
                 for (int k = 0; k < 4; ++k) {
                     long id = node_index(prm, &st, i, j, k);
                     if (id < 0) continue;
-                    double w = st.N[0][i] * st.N[1][j] * st.N[2][k];
-                    double gw[3] = {-inv_dx * st.dN[0][i] * st.N[1][j] * st.N[2][k],
+                    orc_real w = st.N[0][i] * st.N[1][j] * st.N[2][k];
+                    orc_real gw[3] = {-inv_dx * st.dN[0][i] * st.N[1][j] * st.N[2][k],
                                     -inv_dx * st.N[0][i] * st.dN[1][j] * st.N[2][k],
                                     -inv_dx * st.N[0][i] * st.N[1][j] * st.dN[2][k]};
-                    double d[3] = {st.d[0][i], st.d[1][j], st.d[2][k]};
-                    const double* pb = grid_vbar + 3 * id;
-                    const double* vi = grid_v + 3 * id;
-                    double mb = grid_mbar[id];
+                    orc_real d[3] = {st.d[0][i], st.d[1][j], st.d[2][k]};
+                    const orc_real* pb = grid_vbar + 3 * id;
+                    const orc_real* vi = grid_v + 3 * id;
+                    orc_real mb = grid_mbar[id];
                     for (int a = 0; a < 3; ++a)
                         for (int c = 0; c < 3; ++c) Gb[3 * a + c] += w * pb[a] * d[c];
                     /* (b + G d) . pbar */
-                    double bgd = 0.0, vdot = 0.0, cvd = 0.0;
+                    orc_real bgd = 0.0, vdot = 0.0, cvd = 0.0;
                     for (int a = 0; a < 3; ++a) {
-                        double gd = Gm[3 * a] * d[0] + Gm[3 * a + 1] * d[1] + Gm[3 * a + 2] * d[2];
+                        orc_real gd = Gm[3 * a] * d[0] + Gm[3 * a + 1] * d[1] + Gm[3 * a + 2] * d[2];
                         bgd += (b[a] + gd) * pb[a];
                         vdot += vi[a] * vt[a];
                         cvd += vi[a] * (Ct[3 * a] * d[0] + Ct[3 * a + 1] * d[1] + Ct[3 * a + 2] * d[2]);
                     }
                     for (int a = 0; a < 3; ++a) {
-                        double GTp = Gm[a] * pb[0] + Gm[3 + a] * pb[1] + Gm[6 + a] * pb[2];
-                        double CTv = Ct[a] * vi[0] + Ct[3 + a] * vi[1] + Ct[6 + a] * vi[2];
+                        orc_real GTp = Gm[a] * pb[0] + Gm[3 + a] * pb[1] + Gm[6 + a] * pb[2];
+                        orc_real CTv = Ct[a] * vi[0] + Ct[3 + a] * vi[1] + Ct[6 + a] * vi[2];
                         xb[a] += mp * mb * gw[a] + gw[a] * bgd - w * GTp + gw[a] * vdot +
                                  Dinv * (cvd * gw[a] - w * CTv);
                         vb[a] += w * mp * (1.0 - dt * drag) * pb[a];
                     }
                 }
         /* Pbar = -k Gb Fc ; stress term of Fcbar = -k Gb^T P */
-        double Pb[9], GTP[9], H[9], IC[9], base[9];
+        orc_real Pb[9], GTP[9], H[9], IC[9], base[9];
         m3_mul(Gb, Fc, t);
         for (int i = 0; i < 9; ++i) Pb[i] = -kk * t[i];
         m3_tmul(Gb, P, GTP);
@@ -786,7 +789,7 @@ void orc_backward_step(const orc_params* prm,
         for (int i = 0; i < 9; ++i) IC[i] = ((i % 4 == 0) ? 1.0 : 0.0) + dt * C1[9 * p + i];
         m3_tmul(IC, gF1 + 9 * p, base); /* (I + dt C')^T F' */
         for (int i = 0; i < 9; ++i) {
-            double fcb;
+            orc_real fcb;
             if (cp & ORC_COMPAT_DUP_FBAR) {
                 /* BackPropagation.cpp:20-21 then :121-122 */
                 fcb = (1.0 - s) * base[i] + s * gF1[9 * p + i] - kk * GTP[i] + base[i] + H[i];

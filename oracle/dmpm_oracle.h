@@ -16,6 +16,11 @@
  *                                   magnitude, sign on the smallest sigma,
  *                                   U and V rotations, R = U V^T)
  *
+ * orc_real is double.  The Makefile also builds liboracle_dmpm_f32.so from
+ * the same source with ORC_REAL=float and single-precision literals, the
+ * reference's own precision (DiffMPMLib3D/pch.h:24-25): it is only timed, as
+ * the FP32 CPU baseline, and never used as a checker.
+ *
  * Layouts (host, row-major AoS): x,v are [n][3]; F,C,dFc are [n][9] with
  * element (i,j) at 3*i+j; grids are flat with index i*dy*dz + j*dz + k
  * (Grid.h:22-31) and node position min + (i,j,k)*dx (Grid.cpp:22).
@@ -27,6 +32,11 @@
 extern "C" {
 #endif
 
+#ifndef ORC_REAL
+#define ORC_REAL double
+#endif
+typedef ORC_REAL orc_real;
+
 /* Reference-compat switches (BackPropagation.cpp / PointCloud.cpp quirks). */
 #define ORC_COMPAT_DUP_FBAR     1  /* (a) (I+dtC')^T dLdF' added twice, :20 and :121 */
 #define ORC_COMPAT_NO_1MS       2  /* (b) dLdC_next lacks (1-s), :24 */
@@ -37,72 +47,72 @@ extern "C" {
 typedef struct orc_params {
     int n;              /* particles */
     int dim[3];         /* grid nodes per axis */
-    double dx;
-    double min_point[3];
-    double dt, drag, smoothing;
-    double f_ext[3];
+    orc_real dx;
+    orc_real min_point[3];
+    orc_real dt, drag, smoothing;
+    orc_real f_ext[3];
     int compat;         /* ORC_COMPAT_* bit set, 0 = exact adjoint */
 } orc_params;
 
 void orc_set_threads(int nthreads);
 
 /* Interpolation.h */
-double orc_cubic_bspline(double x);
-double orc_cubic_bspline_slope(double x);
+orc_real orc_cubic_bspline(orc_real x);
+orc_real orc_cubic_bspline_slope(orc_real x);
 
 /* 3x3 SVD with the JIXIE conventions; returns sweeps used. */
-int orc_svd3(const double A[9], double U[9], double sigma[3], double V[9]);
-void orc_polar3(const double A[9], double R[9], double S[9]);
+int orc_svd3(const orc_real A[9], orc_real U[9], orc_real sigma[3], orc_real V[9]);
+void orc_polar3(const orc_real A[9], orc_real R[9], orc_real S[9]);
 
 /* Elasticity.cpp */
-double orc_fc_energy(const double F[9], double lam, double mu);
-void orc_pk1(const double F[9], double lam, double mu, double P[9]);
+orc_real orc_fc_energy(const orc_real F[9], orc_real lam, orc_real mu);
+void orc_pk1(const orc_real F[9], orc_real lam, orc_real mu, orc_real P[9]);
 /* exact = 1: derivative of the clamped PK1; exact = 0: the reference's
  * formula structure (J clamp ignored in d(J F^-T)). Identical when det F > 1e-6. */
-void orc_hess_mult(const double F[9], double lam, double mu, const double dF[9], double out[9], int exact);
-void orc_lame(double E, double nu, double* lam, double* mu);
+void orc_hess_mult(const orc_real F[9], orc_real lam, orc_real mu, const orc_real dF[9], orc_real out[9], int exact);
+void orc_lame(orc_real E, orc_real nu, orc_real* lam, orc_real* mu);
 
 /* Batched forms for tests. */
-void orc_pk1_batch(int n, const double* F, const double* lam, const double* mu, double* P);
-void orc_hess_mult_batch(int n, const double* F, const double* lam, const double* mu,
-                         const double* dF, double* out, int exact);
+void orc_pk1_batch(int n, const orc_real* F, const orc_real* lam, const orc_real* mu, orc_real* P);
+void orc_hess_mult_batch(int n, const orc_real* F, const orc_real* lam, const orc_real* mu,
+                         const orc_real* dF, orc_real* out, int exact);
 
 /* ForwardSimulation.cpp:11-51 with grid reset; grid_p is [G][3] momentum. */
-void orc_p2g(const orc_params* prm, const double* x, const double* v, const double* C,
-             const double* F, const double* dFc, const double* m, const double* vol,
-             const double* lam, const double* mu, double dt, double drag,
-             double* grid_m, double* grid_p);
+void orc_p2g(const orc_params* prm, const orc_real* x, const orc_real* v, const orc_real* C,
+             const orc_real* F, const orc_real* dFc, const orc_real* m, const orc_real* vol,
+             const orc_real* lam, const orc_real* mu, orc_real dt, orc_real drag,
+             orc_real* grid_m, orc_real* grid_p);
 
 /* One ForwardTimeStep (smoothing overload, ForwardSimulation.cpp:136-157).
  * grid_m / grid_v receive the updated grid (needed by the adjoint). */
 void orc_forward_step(const orc_params* prm,
-                      const double* x, const double* v, const double* C, const double* F,
-                      const double* dFc, const double* m, const double* vol,
-                      const double* lam, const double* mu,
-                      double* grid_m, double* grid_v,
-                      double* x1, double* v1, double* C1, double* F1);
+                      const orc_real* x, const orc_real* v, const orc_real* C, const orc_real* F,
+                      const orc_real* dFc, const orc_real* m, const orc_real* vol,
+                      const orc_real* lam, const orc_real* mu,
+                      orc_real* grid_m, orc_real* grid_v,
+                      orc_real* x1, orc_real* v1, orc_real* C1, orc_real* F1);
 
 /* MultiThreadMPM::ForwardTimeStep (MultiThreadForwardSimulation.cpp:7-225):
  * the proxy-grid forward with `sections` particle runs, no smoothing.
  * Returns 0, or -1 for sections < 1, -2 when the proxy grids cannot be
  * allocated. */
 int orc_forward_step_proxy(const orc_params* prm, int sections,
-                           const double* x, const double* v, const double* C, const double* F,
-                           const double* dFc, const double* m, const double* vol,
-                           const double* lam, const double* mu,
-                           double* grid_m, double* grid_v,
-                           double* x1, double* v1, double* C1, double* F1);
+                           const orc_real* x, const orc_real* v, const orc_real* C, const orc_real* F,
+                           const orc_real* dFc, const orc_real* m, const orc_real* vol,
+                           const orc_real* lam, const orc_real* mu,
+                           orc_real* grid_m, orc_real* grid_v,
+                           orc_real* x1, orc_real* v1, orc_real* C1, orc_real* F1);
 
 /* ForwardSimulation.cpp:262-290 (vol = m / sum_i w m_i, no dx^3). */
-void orc_point_volumes(const orc_params* prm, const double* x, const double* m, double* vol);
+void orc_point_volumes(const orc_params* prm, const orc_real* x, const orc_real* m, orc_real* vol);
 
 /* P2G mass only (target-grid construction, MPMPolyscope.cpp:158-160). */
-void orc_p2g_mass(const orc_params* prm, const double* x, const double* m, double* grid_m);
+void orc_p2g_mass(const orc_params* prm, const orc_real* x, const orc_real* m, orc_real* grid_m);
 
 /* CompGraph.cpp:110-244. Writes grid_m, grid_dLdm and dLdx; returns the loss. */
-double orc_end_mass_loss(const orc_params* prm, const double* x, const double* m,
-                         const double* target_m, double* grid_m, double* grid_dLdm,
-                         double* dLdx);
+orc_real orc_end_mass_loss(const orc_params* prm, const orc_real* x, const orc_real* m,
+                         const orc_real* target_m, orc_real* grid_m, orc_real* grid_dLdm,
+                         orc_real* dLdx);
 
 /* One Back_Timestep (BackPropagation.cpp:6-125).
  * Inputs: layer n state and constants, the updated grid of step n,
@@ -110,14 +120,14 @@ double orc_end_mass_loss(const orc_params* prm, const double* x, const double* m
  * Outputs: layer n gradients (gx, gv, gF, gC) and dL/d(dFc_n) in gdFc.
  * grid_vbar ([G][3]) and grid_mbar ([G]) are work arrays (overwritten). */
 void orc_backward_step(const orc_params* prm,
-                       const double* x, const double* v, const double* C, const double* F,
-                       const double* dFc, const double* m, const double* vol,
-                       const double* lam, const double* mu,
-                       const double* grid_m, const double* grid_v,
-                       const double* C1, const double* gx1, const double* gv1,
-                       const double* gF1, const double* gC1,
-                       double* grid_vbar, double* grid_mbar,
-                       double* gx, double* gv, double* gF, double* gC, double* gdFc);
+                       const orc_real* x, const orc_real* v, const orc_real* C, const orc_real* F,
+                       const orc_real* dFc, const orc_real* m, const orc_real* vol,
+                       const orc_real* lam, const orc_real* mu,
+                       const orc_real* grid_m, const orc_real* grid_v,
+                       const orc_real* C1, const orc_real* gx1, const orc_real* gv1,
+                       const orc_real* gF1, const orc_real* gC1,
+                       orc_real* grid_vbar, orc_real* grid_mbar,
+                       orc_real* gx, orc_real* gv, orc_real* gF, orc_real* gC, orc_real* gdFc);
 
 #ifdef __cplusplus
 }

@@ -379,3 +379,83 @@ def episode(prm: Params, st0: State, cs: Consts, controls, target, num_layers: i
 def loss_only(prm: Params, st0: State, cs: Consts, controls, target, num_layers: int) -> float:
     layers, _ = forward_pass(prm, st0, cs, controls, num_layers)
     return end_mass_loss(prm, layers[-1].x, cs.m, target)[0]
+
+
+# ------------------------------------------------------- float build (timing) --
+_LIB32 = None
+
+
+def _float_field(t):
+    if issubclass(t, ctypes.Array):
+        return ctypes.c_float * t._length_ if t._type_ is ctypes.c_double else t
+    return ctypes.c_float if t is ctypes.c_double else t
+
+
+class _Params32(ctypes.Structure):
+    _fields_ = [(name, _float_field(t)) for name, t in _Params._fields_]
+
+
+def lib32():
+    """``liboracle_dmpm_f32.so``: the same source with ORC_REAL=float.  Only bench.py's
+    CPU-baseline leg times it; no test checks against it."""
+    global _LIB32
+    if _LIB32 is None:
+        path = os.path.join(_HERE, "liboracle_dmpm_f32.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        fp = ctypes.POINTER(ctypes.c_float)
+        pp = ctypes.POINTER(_Params32)
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_forward_step.argtypes = [pp] + [fp] * 9 + [fp, fp] + [fp] * 4
+        L.orc_end_mass_loss.restype = ctypes.c_float
+        L.orc_end_mass_loss.argtypes = [pp, fp, fp, fp, fp, fp, fp]
+        L.orc_backward_step.argtypes = [pp] + [fp] * 9 + [fp, fp] + [fp] * 5 + [fp, fp] + [fp] * 5
+        _LIB32 = L
+    return _LIB32
+
+
+def episode_f32(prm: Params, st0: State, cs: Consts, controls, target, num_layers: int, threads: int = 0):
+    """``episode`` (forward pass keeping every layer, EndLayerMassLoss, backward pass to layer 0)
+    through the float build.  Returns the loss."""
+    L = lib32()
+    if threads > 0:
+        L.orc_set_threads(int(threads))
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+    p = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))  # noqa: E731
+    P = _Params32()
+    P.n = int(prm.n)
+    for a in range(3):
+        P.dim[a] = int(prm.dim[a])
+        P.min_point[a] = float(prm.min_point[a])
+        P.f_ext[a] = float(prm.f_ext[a])
+    P.dx, P.dt, P.drag, P.smoothing, P.compat = prm.dx, prm.dt, prm.drag, prm.smoothing, int(prm.compat)
+    n, G = prm.n, prm.G
+    m, vol, lam, mu = f(cs.m), f(cs.vol), f(cs.lam), f(cs.mu)
+    ctl = [f(c) for c in controls]
+    lay = [[f(st0.x), f(st0.v), f(st0.C), f(st0.F)]]
+    grids = []
+    for i in range(num_layers - 1):
+        x, v, C, F = lay[i]
+        nx = [np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32), np.zeros((n, 9), np.float32),
+              np.zeros((n, 9), np.float32)]
+        gm, gv = np.zeros(G, np.float32), np.zeros(3 * G, np.float32)
+        L.orc_forward_step(ctypes.byref(P), p(x), p(v), p(C), p(F), p(ctl[i]), p(m), p(vol), p(lam), p(mu), p(gm),
+                           p(gv), *(p(a) for a in nx))
+        lay.append(nx)
+        grids.append((gm, gv))
+    gx = np.zeros((n, 3), np.float32)
+    loss = L.orc_end_mass_loss(ctypes.byref(P), p(lay[-1][0]), p(m), p(f(target)), p(np.zeros(G, np.float32)),
+                               p(np.zeros(G, np.float32)), p(gx))
+    g = [gx, np.zeros((n, 3), np.float32), np.zeros((n, 9), np.float32), np.zeros((n, 9), np.float32)]
+    wv, wm = np.zeros(3 * G, np.float32), np.zeros(G, np.float32)
+    for i in range(num_layers - 2, -1, -1):
+        x, v, C, F = lay[i]
+        out = [np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32), np.zeros((n, 9), np.float32),
+               np.zeros((n, 9), np.float32), np.zeros((n, 9), np.float32)]
+        gm, gv = grids[i]
+        L.orc_backward_step(ctypes.byref(P), p(x), p(v), p(C), p(F), p(ctl[i]), p(m), p(vol), p(lam), p(mu), p(gm),
+                            p(gv), p(lay[i + 1][2]), p(g[0]), p(g[1]), p(g[2]), p(g[3]), p(wv), p(wm),
+                            *(p(a) for a in out))
+        g = [out[0], out[1], out[2], out[3]]
+    return float(loss)
