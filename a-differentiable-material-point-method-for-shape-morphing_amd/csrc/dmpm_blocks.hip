@@ -620,18 +620,17 @@ struct P2GPart {
     M3<float> A;
 };
 
-__device__ __forceinline__ void p2g_prepare(const StepParams& sp, const StepIO<float>& io, int p, P2GPart& q, float bound[4])
+// q from Fc = F + dF and the constants; rest(x, v, C, m) supplies the fields
+// the elastic part does not need after it, so that they need not occupy
+// registers across the FP64 polar.  aux (or null): the layer that caches A.
+// bound: running per-channel bounds of the contributions.
+template <typename Rest>
+__device__ __forceinline__ void p2g_part(const StepParams& sp, const M3<float>& F, const M3<float>& dF, double vol, double lam,
+                                         double mu, Rest&& rest, float* aux, int p, P2GPart& q, float bound[4])
 {
-    const int S = sp.stride;
     const float dx = float(sp.dx);
     const float sdrag = 1.f - float(sp.dt) * float(sp.drag);
     const double kst = -3.0 * sp.inv_dx * sp.inv_dx * sp.dt;
-    // the elastic part first; the fields it does not need are loaded after
-    // it, so that they do not occupy registers across the FP64 polar
-    const M3<float> F = ldm(io.cur, FF, S, p);
-    const M3<float> dF = ldm(io.ctrl, 0, S, p);
-    const double vol = io.cst[KVOL * S + p];
-    const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
     M3<double> Fc;
 #pragma unroll
     for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
@@ -641,13 +640,11 @@ __device__ __forceinline__ void p2g_prepare(const StepParams& sp, const StepIO<f
 #pragma unroll
     for (int i = 0; i < 9; ++i) kPF[i] = float(k * PF.a[i]);
     float v[3];
-    ld3(io.cur, FX, S, p, q.x);
-    ld3(io.cur, FV, S, p, v);
-    const M3<float> C = ldm(io.cur, FC, S, p);
-    q.m = io.cst[KM * S + p];
+    M3<float> C;
+    rest(q.x, v, C, q.m);
 #pragma unroll
     for (int i = 0; i < 9; ++i) q.A.a[i] = kPF[i] + q.m * C.a[i];
-    if (io.aux) stm(io.aux, FA, S, p, q.A);
+    if (aux) stm(aux, FA, sp.stride, p, q.A);
 #pragma unroll
     for (int a = 0; a < 3; ++a) q.mv[a] = q.m * v[a] * sdrag;
     // every contribution: |w (mv + A d)| <= |mv| + 2 dx sum_b |A_ab|  (w <= 1, |d| <= 2 dx)
@@ -656,6 +653,22 @@ __device__ __forceinline__ void p2g_prepare(const StepParams& sp, const StepIO<f
     for (int a = 0; a < 3; ++a)
         bound[1 + a] = fmaxf(bound[1 + a], fabsf(q.mv[a]) + 2.f * dx * (fabsf(q.A.a[3 * a]) + fabsf(q.A.a[3 * a + 1]) +
                                                                          fabsf(q.A.a[3 * a + 2])));
+}
+
+__device__ __forceinline__ void p2g_prepare(const StepParams& sp, const StepIO<float>& io, int p, P2GPart& q, float bound[4])
+{
+    const int S = sp.stride;
+    const M3<float> F = ldm(io.cur, FF, S, p);
+    const M3<float> dF = ldm(io.ctrl, 0, S, p);
+    const double vol = io.cst[KVOL * S + p];
+    const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
+    auto rest = [&](float x[3], float v[3], M3<float>& C, float& m) {
+        ld3(io.cur, FX, S, p, x);
+        ld3(io.cur, FV, S, p, v);
+        C = ldm(io.cur, FC, S, p);
+        m = io.cst[KM * S + p];
+    };
+    p2g_part(sp, F, dF, vol, lam, mu, rest, io.aux, p, q, bound);
 }
 
 __device__ __forceinline__ void p2g_scale(P2GPart& q, const Scales<4>& SC)
@@ -677,30 +690,18 @@ __device__ __forceinline__ void p2g_scale(P2GPart& q, const Scales<4>& SC)
 // the two particles one after the other.
 constexpr int kP2G = 256;
 
-__global__ __launch_bounds__(kP2G, 4) void k_p2g_blk(Batch bt, StepIO<float> io)
+// The scatter of a range's prepared particles q0 (valid v0), q1 (v1) into
+// io.grid (already offset to the instance), from their base box and bounds:
+// window and scales, LDS accumulation, flush.  Starts with a barrier, so the
+// caller may have used the LDS before.
+__device__ __forceinline__ void p2g_scatter(const Batch& bt, const StepParams& sp, const StepIO<float>& io, long long nofs, int nP,
+                                            P2GPart& q0, P2GPart& q1, bool v0, bool v1, const float bound[4], int box[6],
+                                            unsigned long long* lds64)
 {
-    extern __shared__ unsigned long long lds64[];
-    const int4 rg = bt.ranges[blockIdx.x];
-    const StepParams sp = bt.sps[rg.z];
-    const long long nofs = bt.nofs[rg.z];
-    io.grid += 4 * nofs;
-    const int p0 = rg.x + (int)threadIdx.x, p1 = p0 + kP2G;
-    const bool v0 = p0 < rg.y, v1 = p1 < rg.y;
     const float dx = float(sp.dx);
-    P2GPart q0{}, q1{};
-    float bound[4] = {0.f, 0.f, 0.f, 0.f};
-    int box[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
-    if (v0) {
-        p2g_prepare(sp, io, p0, q0, bound);
-        box_add(sp, q0.x, box);
-    }
-    if (v1) {
-        p2g_prepare(sp, io, p1, q1, bound);
-        box_add(sp, q1.x, box);
-    }
     Win W;
     Scales<4> SC;
-    block_setup<4, false>(sp, box, bound, rg.y - rg.x, bt.cap, W, SC);
+    block_setup<4, false>(sp, box, bound, nP, bt.cap, W, SC);
     const int mode = (W.fixed && SC.ok) ? 0 : ((W.lds && SC.ok) ? 1 : 2);
     if (threadIdx.x == 0 && bt.wstats) atomicAdd(bt.wstats + mode, 1ull);
     if (mode == 0)
@@ -767,6 +768,34 @@ __global__ __launch_bounds__(kP2G, 4) void k_p2g_blk(Batch bt, StepIO<float> io)
     }
 }
 
+// tc: a grid to clear in appended workgroups (the next step's), or none
+__global__ __launch_bounds__(kP2G, 4) void k_p2g_blk(Batch bt, StepIO<float> io, TileClear tc)
+{
+    if (blockIdx.x >= (unsigned)bt.nranges) {
+        appended_clear(bt, tc, (unsigned)bt.nranges);
+        return;
+    }
+    extern __shared__ unsigned long long lds64[];
+    const int4 rg = bt.ranges[blockIdx.x];
+    const StepParams sp = bt.sps[rg.z];
+    const long long nofs = bt.nofs[rg.z];
+    io.grid += 4 * nofs;
+    const int p0 = rg.x + (int)threadIdx.x, p1 = p0 + kP2G;
+    const bool v0 = p0 < rg.y, v1 = p1 < rg.y;
+    P2GPart q0{}, q1{};
+    float bound[4] = {0.f, 0.f, 0.f, 0.f};
+    int box[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    if (v0) {
+        p2g_prepare(sp, io, p0, q0, bound);
+        box_add(sp, q0.x, box);
+    }
+    if (v1) {
+        p2g_prepare(sp, io, p1, q1, bound);
+        box_add(sp, q1.x, box);
+    }
+    p2g_scatter(bt, sp, io, nofs, rg.y - rg.x, q0, q1, v0, v1, bound, box, lds64);
+}
+
 // --------------------------------------------------- P2G of mass only ----
 // Node masses of a layer (the EndLayerMassLoss grid, CompGraph.cpp:110-141),
 // with the window scatter of k_p2g_blk.
@@ -810,6 +839,124 @@ __global__ __launch_bounds__(kScat) void k_p2g_mass_blk(Batch bt, const float* l
 }
 
 // ------------------------------------------------------------------ G2P ----
+// node velocities of an FP32 grid (m, p) as G2P loads them
+struct NodeVel {
+    const float* gr;
+    long long G;
+    float fdt[3];
+    __device__ NodeVel(const StepParams& sp, const float* grid) : gr(grid), G(sp.G)
+    {
+        const float dt = float(sp.dt);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) fdt[a] = dt * float(sp.f_ext[a]);
+    }
+    __device__ __forceinline__ void operator()(long long node, float* v) const
+    {
+        const float p[3] = {gr[G + node], gr[2 * G + node], gr[3 * G + node]};
+        node_velocity(gr[node], p, fdt, v);
+    }
+};
+
+// G2P and P_op_2 (ForwardSimulation.cpp:60-124) of particle p at x: its next
+// state from the window W of node velocities in lds
+__device__ __forceinline__ void g2p_particle(const StepParams& sp, const StepIO<float>& io, const Win& W, const float* lds,
+                                             const NodeVel& ld, int p, const float x[3], float x1[3], float v1[3], M3<float>& C1,
+                                             M3<float>& F1)
+{
+    const int S = sp.stride;
+    const float dt = float(sp.dt), s = float(sp.smoothing);
+    const float Dd = float(3.0 * sp.inv_dx);
+    Axis<float> st[3];
+    stencil_f(sp, x, st, false);
+    float e0s[4], e1s[4], e2s[4];
+    offsets4(st[0], Dd, e0s);
+    offsets4(st[1], Dd, e1s);
+    offsets4(st[2], Dd, e2s);
+    float vp[3] = {0.f, 0.f, 0.f};
+    M3<float> Cp = m3_zero<float>();
+    // row sums R = sum_k w_k v, Re = sum_k w_k e_k v
+    float we2[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) we2[k] = st[2].w[k] * e2s[k];
+    float R[3] = {0.f, 0.f, 0.f}, Re[3] = {0.f, 0.f, 0.f};
+    auto acc = [&](int, int, int kk, const float* vi) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            R[a] += st[2].w[kk] * vi[a];
+            Re[a] += we2[kk] * vi[a];
+        }
+    };
+    auto row = [&](int i, int j) {
+        const float w01 = st[0].w[i] * st[1].w[j];
+        const float c0 = w01 * e0s[i], c1 = w01 * e1s[j];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            vp[a] += w01 * R[a];
+            Cp.a[3 * a] += c0 * R[a];
+            Cp.a[3 * a + 1] += c1 * R[a];
+            Cp.a[3 * a + 2] += w01 * Re[a];
+            R[a] = 0.f;
+            Re[a] = 0.f;
+        }
+    };
+    if (W.fixed) gather64<3, 0>(sp, W, st, lds, ld, acc, row);
+    else if (W.lds) gather64<3, 1>(sp, W, st, lds, ld, acc, row);
+    else gather64<3, 2>(sp, W, st, lds, ld, acc, row);
+    const M3<float> F = ldm(io.cur, FF, S, p);
+    const M3<float> dF = ldm(io.ctrl, 0, S, p);
+    M3<float> Fc, IC;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        Fc.a[i] = F.a[i] + dF.a[i];
+        IC.a[i] = dt * Cp.a[i];
+    }
+    IC.a[0] += 1.f;
+    IC.a[4] += 1.f;
+    IC.a[8] += 1.f;
+    F1 = mul(IC, Fc);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) F1.a[i] = F1.a[i] * (1.f - s) + F.a[i] * s;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        x1[a] = x[a] + dt * vp[a];
+        v1[a] = vp[a];
+    }
+    C1 = Cp;
+}
+
+__device__ __forceinline__ void store_state(const StepParams& sp, float* lay, int p, const float x[3], const float v[3],
+                                            const M3<float>& C, const M3<float>& F)
+{
+    const int S = sp.stride;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lay[(long long)(FX + a) * S + p] = x[a];
+        lay[(long long)(FV + a) * S + p] = v[a];
+    }
+    stm(lay, FC, S, p, C);
+    stm(lay, FF, S, p, F);
+}
+
+// the window of the node velocities that a range's G2P reads, loaded into lds;
+// xs: the thread's (at most two: ranges hold <= 512) particle positions
+__device__ __forceinline__ Win g2p_window(const Batch& bt, const StepParams& sp, const StepIO<float>& io, int4 rg, float xs[2][3],
+                                          float* lds, const NodeVel& ld)
+{
+    int box[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int p = rg.x + (int)threadIdx.x + h * kGat;
+        if (p < rg.y) {
+            ld3(io.cur, FX, sp.stride, p, xs[h]);
+            box_add(sp, xs[h], box);
+        }
+    }
+    const Win W = window_of(sp, box, bt.cap);
+    win_load<3>(sp, W, lds, ld);
+    __syncthreads();
+    return W;
+}
+
 __global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io, TileClear tc)
 {
     if (blockIdx.x >= (unsigned)bt.nranges) {
@@ -820,31 +967,9 @@ __global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io, Ti
     const int4 rg = bt.ranges[blockIdx.x];
     const StepParams sp = bt.sps[rg.z];
     io.grid += 4 * bt.nofs[rg.z];
-    const int S = sp.stride;
-    // the thread's particles (at most two: ranges hold <= 512): positions
-    // once, for the window and the gathers
+    const NodeVel ld(sp, io.grid);
     float xs[2][3];
-    int box[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int p = rg.x + (int)threadIdx.x + h * kGat;
-        if (p < rg.y) {
-            ld3(io.cur, FX, S, p, xs[h]);
-            box_add(sp, xs[h], box);
-        }
-    }
-    const Win W = window_of(sp, box, bt.cap);
-    const float dt = float(sp.dt), s = float(sp.smoothing);
-    const float fdt[3] = {dt * float(sp.f_ext[0]), dt * float(sp.f_ext[1]), dt * float(sp.f_ext[2])};
-    const float* gr = io.grid;
-    const long long G = sp.G;
-    auto ld = [&](long long node, float* v) {
-        const float p[3] = {gr[G + node], gr[2 * G + node], gr[3 * G + node]};
-        node_velocity(gr[node], p, fdt, v);
-    };
-    win_load<3>(sp, W, lds, ld);
-    __syncthreads();
-    const float Dd = float(3.0 * sp.inv_dx);
+    const Win W = g2p_window(bt, sp, io, rg, xs, lds, ld);
 #pragma unroll 1
     for (int h = 0; h < 2; ++h) {
         const int p = rg.x + (int)threadIdx.x + h * kGat;
@@ -852,64 +977,68 @@ __global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io, Ti
         float x[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) x[a] = h ? xs[1][a] : xs[0][a];
-        Axis<float> st[3];
-        stencil_f(sp, x, st, false);
-        float e0s[4], e1s[4], e2s[4];
-        offsets4(st[0], Dd, e0s);
-        offsets4(st[1], Dd, e1s);
-        offsets4(st[2], Dd, e2s);
-        float vp[3] = {0.f, 0.f, 0.f};
-        M3<float> Cp = m3_zero<float>();
-        // row sums R = sum_k w_k v, Re = sum_k w_k e_k v
-        float we2[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) we2[k] = st[2].w[k] * e2s[k];
-        float R[3] = {0.f, 0.f, 0.f}, Re[3] = {0.f, 0.f, 0.f};
-        auto acc = [&](int, int, int kk, const float* vi) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                R[a] += st[2].w[kk] * vi[a];
-                Re[a] += we2[kk] * vi[a];
-            }
-        };
-        auto row = [&](int i, int j) {
-            const float w01 = st[0].w[i] * st[1].w[j];
-            const float c0 = w01 * e0s[i], c1 = w01 * e1s[j];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                vp[a] += w01 * R[a];
-                Cp.a[3 * a] += c0 * R[a];
-                Cp.a[3 * a + 1] += c1 * R[a];
-                Cp.a[3 * a + 2] += w01 * Re[a];
-                R[a] = 0.f;
-                Re[a] = 0.f;
-            }
-        };
-        if (W.fixed) gather64<3, 0>(sp, W, st, lds, ld, acc, row);
-        else if (W.lds) gather64<3, 1>(sp, W, st, lds, ld, acc, row);
-        else gather64<3, 2>(sp, W, st, lds, ld, acc, row);
-        const M3<float> F = ldm(io.cur, FF, S, p);
-        const M3<float> dF = ldm(io.ctrl, 0, S, p);
-        M3<float> Fc, IC;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            Fc.a[i] = F.a[i] + dF.a[i];
-            IC.a[i] = dt * Cp.a[i];
-        }
-        IC.a[0] += 1.f;
-        IC.a[4] += 1.f;
-        IC.a[8] += 1.f;
-        M3<float> Fn = mul(IC, Fc);
-#pragma unroll
-        for (int i = 0; i < 9; ++i) Fn.a[i] = Fn.a[i] * (1.f - s) + F.a[i] * s;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            io.next[(long long)(FX + a) * S + p] = x[a] + dt * vp[a];
-            io.next[(long long)(FV + a) * S + p] = vp[a];
-        }
-        stm(io.next, FC, S, p, Cp);
-        stm(io.next, FF, S, p, Fn);
+        float x1[3], v1[3];
+        M3<float> C1, F1;
+        g2p_particle(sp, io, W, lds, ld, p, x, x1, v1, C1, F1);
+        store_state(sp, io.next, p, x1, v1, C1, F1);
     }
+}
+
+// ------------------------------------------------------- fused step ------
+// G2P of step i - 1 (gio: layer i - 1 -> layer i, grid of step i - 1) and P2G
+// of step i (pio: dFc_i, grid of step i, A cached in layer i when pio.aux is
+// set) in one launch, one workgroup per range: each thread's new particle
+// state goes from the gather straight into the scatter's inputs, and the
+// VALU-bound scatter of one workgroup overlaps the memory-bound gathers of
+// others.  The grid of step i must be clear; tc clears step i + 1's.  The
+// results equal those of k_g2p_blk then k_p2g_blk: the state written is the
+// state scattered, and the scatter sums are independent of their order.
+static_assert(kP2G == kGat, "the fused step maps particles to threads as both kernels do");
+__global__ __launch_bounds__(kP2G, 4) void k_step_blk(Batch bt, StepIO<float> gio, StepIO<float> pio, TileClear tc)
+{
+    if (blockIdx.x >= (unsigned)bt.nranges) {
+        appended_clear(bt, tc, (unsigned)bt.nranges);
+        return;
+    }
+    extern __shared__ unsigned long long lds64[];
+    const int4 rg = bt.ranges[blockIdx.x];
+    const StepParams sp = bt.sps[rg.z];
+    const long long nofs = bt.nofs[rg.z];
+    gio.grid += 4 * nofs;
+    pio.grid += 4 * nofs;
+    const int S = sp.stride;
+    const NodeVel ld(sp, gio.grid);
+    float xs[2][3];
+    const Win W = g2p_window(bt, sp, gio, rg, xs, reinterpret_cast<float*>(lds64), ld);
+    const int p0 = rg.x + (int)threadIdx.x, p1 = p0 + kP2G;
+    const bool v0 = p0 < rg.y, v1 = p1 < rg.y;
+    P2GPart q0{}, q1{};
+    float bound[4] = {0.f, 0.f, 0.f, 0.f};
+    int box[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    auto one = [&](int p, const float x[3], P2GPart& q) {
+        float xn[3], vn[3];
+        M3<float> Cn, Fn;
+        g2p_particle(sp, gio, W, reinterpret_cast<const float*>(lds64), ld, p, x, xn, vn, Cn, Fn);
+        store_state(sp, gio.next, p, xn, vn, Cn, Fn);
+        const M3<float> dF = ldm(pio.ctrl, 0, S, p);
+        const double vol = pio.cst[KVOL * S + p];
+        const double lam = pio.cst[KLAM * S + p], mu = pio.cst[KMU * S + p];
+        auto rest = [&](float x_[3], float v_[3], M3<float>& C_, float& m_) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                x_[a] = xn[a];
+                v_[a] = vn[a];
+            }
+            C_ = Cn;
+            m_ = pio.cst[KM * S + p];
+        };
+        p2g_part(sp, Fn, dF, vol, lam, mu, rest, pio.aux, p, q, bound);
+        box_add(sp, q.x, box);
+    };
+    if (v0) one(p0, xs[0], q0);
+    if (v1) one(p1, xs[1], q1);
+    // p2g_scatter's first barrier orders every gather above before the LDS is reused
+    p2g_scatter(bt, sp, pio, nofs, rg.y - rg.x, q0, q1, v0, v1, bound, box, lds64);
 }
 
 // -------------------------------------------------------- transposed G2P ---
@@ -1380,11 +1509,6 @@ __global__ __launch_bounds__(kGat) void k_loss_particles_b(Batch bt, const float
 // LDS: max(cap, fixed-window slots) nodes of 64-bit words (scatters: NC, or (NC + 1) / 2 packed) or NC floats (gathers)
 static size_t win_nodes(int cap) { return (size_t)(cap > kFWS ? cap : kFWS); }
 
-void launch_p2g_blk(const Batch& b, const StepIO<float>& io, hipStream_t s)
-{
-    if (b.nranges <= 0) return;
-    hipLaunchKernelGGL(k_p2g_blk, dim3(b.nranges), dim3(kP2G), 4 * win_nodes(b.cap) * sizeof(unsigned long long), s, b, io);
-}
 
 void launch_p2g_mass_blk(const Batch& b, const float* lay, const float* cst, float* gm, hipStream_t s)
 {
@@ -1400,6 +1524,27 @@ static unsigned clear_blocks(const Batch& b, const TileClear& tc)
     long long nb = (num_tiles(b.Gtot) + 4 * 64 - 1) / (4 * 64);
     if (nb > 4096) nb = 4096;
     return (unsigned)nb;
+}
+
+void launch_p2g_blk(const Batch& b, const StepIO<float>& io, const TileClear& tc, hipStream_t s)
+{
+    if (b.nranges <= 0) {
+        if (tc.grid) launch_clear_tiles(b, tc.grid, tc.flags, s);
+        return;
+    }
+    hipLaunchKernelGGL(k_p2g_blk, dim3(b.nranges + clear_blocks(b, tc)), dim3(kP2G), 4 * win_nodes(b.cap) * sizeof(unsigned long long),
+                       s, b, io, tc);
+}
+
+void launch_step_blk(const Batch& b, const StepIO<float>& gio, const StepIO<float>& pio, const TileClear& tc, hipStream_t s)
+{
+    if (b.nranges <= 0) {
+        if (tc.grid) launch_clear_tiles(b, tc.grid, tc.flags, s);
+        return;
+    }
+    // the gather's window (3 float channels) and then the scatter's (4 64-bit words) share the LDS
+    hipLaunchKernelGGL(k_step_blk, dim3(b.nranges + clear_blocks(b, tc)), dim3(kP2G),
+                       4 * win_nodes(b.cap) * sizeof(unsigned long long), s, b, gio, pio, tc);
 }
 
 void launch_g2p_blk(const Batch& b, const StepIO<float>& io, const TileClear& tc, hipStream_t s)

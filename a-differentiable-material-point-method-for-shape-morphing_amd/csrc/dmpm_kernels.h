@@ -127,7 +127,9 @@ struct TileClear {
     unsigned* flags;
 };
 void launch_set_flags(unsigned* flags, long long ntiles, hipStream_t s);
-void launch_p2g_blk(const Batch& b, const StepIO<float>& io, hipStream_t s);
+void launch_p2g_blk(const Batch& b, const StepIO<float>& io, const TileClear& tc, hipStream_t s);
+// G2P of a step (gio) and P2G of the next (pio) in one launch; tc: the grid after
+void launch_step_blk(const Batch& b, const StepIO<float>& gio, const StepIO<float>& pio, const TileClear& tc, hipStream_t s);
 void launch_p2g_mass_blk(const Batch& b, const float* lay, const float* cst, float* gm, hipStream_t s);
 void launch_g2p_blk(const Batch& b, const StepIO<float>& io, const TileClear& tc, hipStream_t s);
 void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s);

@@ -81,9 +81,10 @@ struct dmpm_ctx {
     void* layers = nullptr;   // layer slots x 24 x S
     void* grids = nullptr;    // grid slots x 4 x G
     void* wgrid = nullptr;    // 4 x G: loss mass + dLdm / checkpoint-mode forward grid (even steps)
-    void* wgrid2 = nullptr;   // 4 x G: checkpoint-mode forward grid (odd steps)
+    void* wgrid2 = nullptr;   // 4 x G: checkpoint-mode forward grid (steps i % 3 == 1; FP64: odd steps)
+    void* wgrid3 = nullptr;   // 4 x G (FP32): checkpoint-mode forward grid (steps i % 3 == 2)
     void* agrid = nullptr;    // 4 x G: vbar/pbar + mbar
-    // FP32: tile flags of every grid buffer (grid slots, then wgrid, agrid, wgrid2)
+    // FP32: tile flags of every grid buffer (grid slots, then wgrid, agrid, wgrid2, wgrid3)
     unsigned* tflags = nullptr;
     long long ntiles = 0;
     size_t grid_slots = 0;
@@ -153,6 +154,10 @@ struct dmpm_ctx {
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
     bool overlap = true;
+    // FP32: G2P of a step and P2G of the next in one launch (k_step_blk)
+    bool fuse = true;
+    // FP32: order of the block ranges (launch order): 0 by block, 1 longest first
+    int range_order = 1;
     // validity
     bool have_particles = false;
     bool have_target = false;
@@ -413,6 +418,7 @@ unsigned* flags_of(dmpm_ctx* c, const void* g)
     if (g == c->wgrid) k = c->grid_slots;
     else if (g == c->agrid) k = c->grid_slots + 1;
     else if (g == c->wgrid2) k = c->grid_slots + 2;
+    else if (g == c->wgrid3) k = c->grid_slots + 3;
     else k = (size_t)(static_cast<const char*>(g) - static_cast<const char*>(c->grids)) / gb;
     return c->tflags + k * (size_t)c->ntiles;
 }
@@ -442,7 +448,7 @@ void taint_wgrid(dmpm_ctx* c)
 // with direct HBM atomics; the FP32 path runs one workgroup per block range
 // with the node window staged in LDS.
 inline void p2g_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<double>& io) { launch_p2g<double>(sp, io, c->stream); }
-inline void p2g_dispatch(dmpm_ctx* c, const StepParams&, const StepIO<float>& io) { launch_p2g_blk(c->batch, io, c->stream); }
+inline void p2g_dispatch(dmpm_ctx* c, const StepParams&, const StepIO<float>& io) { launch_p2g_blk(c->batch, io, TileClear{}, c->stream); }
 inline void grid_update_dispatch(dmpm_ctx* c, const StepParams& sp, double* g) { launch_grid_update<double>(sp, g, c->stream); }
 inline void g2p_dispatch(dmpm_ctx* c, const StepParams& sp, const StepIO<double>& io) { launch_g2p<double>(sp, io, c->stream); }
 inline void g2p_dispatch(dmpm_ctx* c, const StepParams&, const StepIO<float>& io, const TileClear& tc)
@@ -530,12 +536,57 @@ bool step_backward(dmpm_ctx* c, int i, const Real* cur, const Real* nxt, const R
     }
 }
 
-// checkpoint-mode forward grid of a step outside the tape: two buffers by
-// parity, so that a step's G2P can clear the next step's grid
+// checkpoint-mode forward grid of a step outside the tape: buffers in
+// rotation, so that a step's G2P can clear the next step's grid (FP64: two),
+// also while it runs as one launch with the next step's P2G (FP32: three)
 template <typename Real>
 Real* wgrid_of(dmpm_ctx* c, int i)
 {
+    if (c->wgrid3) {
+        void* const g[3] = {c->wgrid, c->wgrid2, c->wgrid3};
+        return static_cast<Real*>(g[i % 3]);
+    }
     return static_cast<Real*>((i & 1) ? c->wgrid2 : c->wgrid);
+}
+
+// Steps [i0, i1) of one launch sequence.  lay(j): the slot of layer j (read
+// for j = i0, written for i0 < j <= i1); grid(i): the grid of step i, three
+// consecutive ones distinct buffers; keep(i): the backward pass reads layer i,
+// so P2G caches its affine matrix there; last_g2p: the last step's G2P runs
+// (else only its grid: the layer after it is held already).  FP32 with
+// c->fuse runs the G2P of each step with the P2G of the next as one launch.
+template <typename Real, typename LayFn, typename GridFn, typename KeepFn>
+void forward_seq(dmpm_ctx* c, int i0, int i1, LayFn&& lay, GridFn&& grid, KeepFn&& keep, bool last_g2p = true)
+{
+    if (i1 <= i0) return;
+    if constexpr (std::is_same<Real, float>::value) {
+        if (c->fuse) {
+            const StepParams sp = step_params(c);
+            auto pio_of = [&](int i) {
+                float* g = grid(i);
+                return StepIO<float>{lay(i), nullptr, ctrl_of<float>(c, i), static_cast<const float*>(c->cst), g,
+                                     keep(i) ? lay(i) : nullptr, flags_of(c, g)};
+            };
+            auto clear_of = [&](int i) { return i < i1 ? TileClear{grid(i), flags_of(c, grid(i))} : TileClear{}; };
+            clear_grid(c, grid(i0));
+            timed(c, DMPM_K_P2G, [&] { launch_p2g_blk(c->batch, pio_of(i0), clear_of(i0 + 1), c->stream); });
+            for (int i = i0 + 1; i < i1; ++i) {
+                const StepIO<float> gio{lay(i - 1), lay(i), ctrl_of<float>(c, i - 1), static_cast<const float*>(c->cst), grid(i - 1),
+                                        nullptr, nullptr};
+                timed(c, DMPM_K_STEP, [&] { launch_step_blk(c->batch, gio, pio_of(i), clear_of(i + 1), c->stream); });
+            }
+            if (last_g2p) {
+                const StepIO<float> gio{lay(i1 - 1), lay(i1), ctrl_of<float>(c, i1 - 1), static_cast<const float*>(c->cst),
+                                        grid(i1 - 1), nullptr, nullptr};
+                timed(c, DMPM_K_G2P, [&] { g2p_dispatch(c, sp, gio, TileClear{}); });
+            }
+            return;
+        }
+    }
+    bool clean = false;
+    for (int i = i0; i < i1; ++i)
+        clean = step_forward<Real>(c, lay(i), lay(i + 1), ctrl_of<Real>(c, i), grid(i), clean, i + 1 < i1 ? grid(i + 1) : nullptr,
+                                   keep(i), i + 1 < i1 || last_g2p);
 }
 
 // Bring layer j (<= fwd_valid) into a readable slot and return it.  Checkpoint
@@ -548,15 +599,9 @@ Real* materialize(dmpm_ctx* c, int j)
     if (j % c->K == 0) return slot<Real>(c, j / c->K);
     if (tape_of(c, j / c->K) >= 0) return tape_layer<Real>(c, j);
     const int j0 = (j / c->K) * c->K;
-    Real* prev = slot<Real>(c, j0 / c->K);
-    bool clean = false;
-    for (int i = j0; i < j; ++i) {
-        Real* dst = slot<Real>(c, c->roll_base + ((i + 1) & 1));
-        clean = step_forward<Real>(c, prev, dst, ctrl_of<Real>(c, i), wgrid_of<Real>(c, i), clean,
-                                   i + 1 < j ? wgrid_of<Real>(c, i + 1) : nullptr, false);
-        prev = dst;
-    }
-    return prev;
+    auto lay = [&](int k) { return k == j0 ? slot<Real>(c, j0 / c->K) : slot<Real>(c, c->roll_base + (k & 1)); };
+    forward_seq<Real>(c, j0, j, lay, [&](int i) { return wgrid_of<Real>(c, i); }, [](int) { return false; });
+    return lay(j);
 }
 
 // Recompute segment seg into its tape, seg & 1 (checkpoint mode), with the
@@ -566,15 +611,10 @@ void recompute_segment(dmpm_ctx* c, int seg)
 {
     const int t = seg & 1;
     const int s0 = seg * c->K, s1 = std::min(s0 + c->K, c->L - 1);
-    bool clean = false;
-    for (int i = s0; i < s1; ++i) {
-        // a segment that ends on a checkpoint needs only the grid of its last
-        // step: the backward pass reads the next layer from the checkpoint
-        const bool ends_on_ckpt = (i + 1) % c->K == 0;
-        Real* dst = ends_on_ckpt ? slot<Real>(c, c->roll_base) : tape_slot<Real>(c, t, i + 1);
-        clean = step_forward<Real>(c, tape_slot<Real>(c, t, i), dst, ctrl_of<Real>(c, i), tape_grid<Real>(c, t, i), clean,
-                                   i + 1 < s1 ? tape_grid<Real>(c, t, i + 1) : nullptr, true, !ends_on_ckpt);
-    }
+    // a segment that ends on a checkpoint needs only the grid of its last
+    // step: the backward pass reads the next layer from the checkpoint
+    forward_seq<Real>(c, s0, s1, [&](int j) { return tape_slot<Real>(c, t, j); }, [&](int i) { return tape_grid<Real>(c, t, i); },
+                      [](int) { return true; }, s1 % c->K != 0);
 }
 
 // -------------------------------------------------------------- passes -----
@@ -595,10 +635,8 @@ int forward_pass(dmpm_ctx* c, int start)
     if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) {
         rc = run_graph(c, from, [&] {
             zero_wstats(c);
-            bool clean = false;
-            for (int i = from; i < c->L - 1; ++i)
-                clean = step_forward<Real>(c, slot<Real>(c, i), slot<Real>(c, i + 1), ctrl_of<Real>(c, i), grid_of_step<Real>(c, i), clean,
-                                           i + 1 < c->L - 1 ? grid_of_step<Real>(c, i + 1) : nullptr);
+            forward_seq<Real>(c, from, c->L - 1, [&](int j) { return slot<Real>(c, j); }, [&](int i) { return grid_of_step<Real>(c, i); },
+                              [](int) { return true; });
         });
         if (rc) return rc;
     } else {
@@ -606,22 +644,16 @@ int forward_pass(dmpm_ctx* c, int start)
         const int s0_last = c->seg_last * c->K;
         const int tl = c->seg_last & 1;  // the last segment's tape
         rc = run_graph(c, from, [&] {
-        zero_wstats(c);
-        bool clean = false;
-        for (int i = from; i < c->L - 1; ++i) {
-            const Real* cur;
-            if (i % c->K == 0 || i / c->K == c->seg_last) cur = tape_slot<Real>(c, tl, i);
-            else cur = slot<Real>(c, c->roll_base + (i & 1));
-            Real* dst;
-            const int j = i + 1;
-            if (j % c->K == 0 || j / c->K == c->seg_last) dst = tape_slot<Real>(c, tl, j);
-            else dst = slot<Real>(c, c->roll_base + (j & 1));
+            zero_wstats(c);
+            // checkpoints and the last segment go to the tape, other layers to the rolling slots
+            auto lay = [&](int j) {
+                if (j % c->K == 0 || j / c->K == c->seg_last) return tape_slot<Real>(c, tl, j);
+                return slot<Real>(c, c->roll_base + (j & 1));
+            };
             auto grid_at = [&](int k) { return (k >= s0_last) ? tape_grid<Real>(c, tl, k) : wgrid_of<Real>(c, k); };
             // steps before the last segment are recomputed for the backward pass:
             // their affine matrices are not read
-            clean = step_forward<Real>(c, cur, dst, ctrl_of<Real>(c, i), grid_at(i), clean, i + 1 < c->L - 1 ? grid_at(i + 1) : nullptr,
-                                       i >= s0_last);
-        }
+            forward_seq<Real>(c, from, c->L - 1, lay, grid_at, [&](int i) { return i >= s0_last; });
         });
         if (rc) return rc;
         c->tape_hold[tl] = c->seg_last;
@@ -828,6 +860,10 @@ int build_ranges(dmpm_ctx* c)
         for (int k = 0; k < parts; ++k)
             rg.push_back(make_int4(b.x + (int)((long long)len * k / parts), b.x + (int)((long long)len * (k + 1) / parts), b.z, 0));
     }
+    // longest ranges first: workgroups start in range order, so the short ones
+    // fill the last round of a launch instead of trailing behind long ones
+    if (c->range_order == 1)
+        std::stable_sort(rg.begin(), rg.end(), [](const int4& a, const int4& b) { return a.y - a.x > b.y - b.x; });
     clear_graphs(c);
     if (c->ranges) {
         hipFree(c->ranges);
@@ -1396,7 +1432,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
     if ((rc = dalloc(c, &c->cst, kConstFields * S * e)) || (rc = dalloc(c, &c->ctrl, (size_t)steps * 9 * S * e)) ||
         (rc = dalloc(c, &c->layers, (size_t)c->n_slots * c->lf * S * e)) ||
         (rc = dalloc(c, &c->grids, grid_slots * kGridFields * G * e)) || (rc = dalloc(c, &c->wgrid, kGridFields * G * e)) ||
-        (rc = dalloc(c, &c->wgrid2, kGridFields * G * e)) ||
+        (rc = dalloc(c, &c->wgrid2, kGridFields * G * e)) || (c->f32 && (rc = dalloc(c, &c->wgrid3, kGridFields * G * e))) ||
         (rc = dalloc(c, &c->agrid, kGridFields * G * e)) || (rc = dalloc(c, &c->gbuf[0], kLayerFields * S * e)) ||
         (rc = dalloc(c, &c->gbuf[1], kLayerFields * S * e)) || (rc = dalloc(c, &c->gbuf[2], kLayerFields * S * e)) ||
         (rc = dalloc(c, &c->gctrl, gctrl_layers * 9 * S * e)) ||
@@ -1405,7 +1441,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         (rc = dalloc(c, (void**)&c->dpart, std::max<long long>(kLossBlocks, loss_partials(c->B)) * sizeof(double))) ||
         (rc = dalloc(c, (void**)&c->d_sps, c->B * sizeof(StepParams))) ||
         (rc = dalloc(c, (void**)&c->d_nofs, (c->B + 1) * sizeof(long long))) ||
-        (c->f32 && (rc = dalloc(c, (void**)&c->tflags, (grid_slots + 3) * (size_t)c->ntiles * sizeof(unsigned)))) ||
+        (c->f32 && (rc = dalloc(c, (void**)&c->tflags, (grid_slots + 4) * (size_t)c->ntiles * sizeof(unsigned)))) ||
         (c->f32 && (rc = dalloc(c, (void**)&c->d_api, S * sizeof(int)))) ||
         (c->f32 && (rc = dalloc(c, (void**)&c->d_poff, (c->B + 1) * sizeof(int)))) ||
         (c->f32 && (rc = dalloc(c, (void**)&c->d_boff, (c->B + 1) * sizeof(long long)))) ||
@@ -1424,8 +1460,9 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         launch_fill<float>(static_cast<float*>(c->grids), (long long)(grid_slots * kGridFields * G), 0.f, c->stream);
         launch_fill<float>(static_cast<float*>(c->wgrid), (long long)(kGridFields * G), 0.f, c->stream);
         launch_fill<float>(static_cast<float*>(c->wgrid2), (long long)(kGridFields * G), 0.f, c->stream);
+        launch_fill<float>(static_cast<float*>(c->wgrid3), (long long)(kGridFields * G), 0.f, c->stream);
         launch_fill<float>(static_cast<float*>(c->agrid), (long long)(kGridFields * G), 0.f, c->stream);
-        launch_fill<float>(reinterpret_cast<float*>(c->tflags), (long long)(grid_slots + 3) * c->ntiles, 0.f, c->stream);
+        launch_fill<float>(reinterpret_cast<float*>(c->tflags), (long long)(grid_slots + 4) * c->ntiles, 0.f, c->stream);
     } else {
         launch_fill<double>(static_cast<double*>(c->gctrl), (long long)gctrl_layers * 9 * S, 0.0, c->stream);
         launch_fill<double>(static_cast<double*>(c->ctrl), (long long)steps * 9 * S, 0.0, c->stream);
@@ -1502,7 +1539,7 @@ int dmpm_destroy(dmpm_ctx* c)
     hipSetDevice(c->cfg.device);
     if (c->stream) hipStreamSynchronize(c->stream);
     clear_graphs(c);
-    void* bufs[] = {c->cst,  c->ctrl, c->layers, c->grids, c->wgrid, c->wgrid2, c->agrid, c->gbuf[0], c->gbuf[1], c->gbuf[2],
+    void* bufs[] = {c->cst,  c->ctrl, c->layers, c->grids, c->wgrid, c->wgrid2, c->wgrid3, c->agrid, c->gbuf[0], c->gbuf[1], c->gbuf[2],
                     c->gctrl, c->tmp, c->target, c->dacc,  c->ranges, c->d_sps, c->d_nofs, c->dpart, c->tflags,
                     c->d_api, c->d_poff, c->d_boff, c->sortbuf, c->wstats};
     for (void* b : bufs)
@@ -1842,6 +1879,17 @@ int dmpm_set_option(dmpm_ctx* c, int32_t option, int64_t value)
         c->overlap = value != 0;
         clear_graphs(c);
         return DMPM_OK;
+    case DMPM_OPT_RANGE_ORDER:
+        if (value != 0 && value != 1) return fail(DMPM_ERR_INVALID, "DMPM_OPT_RANGE_ORDER is 0 or 1");
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->range_order = (int)value;
+        return c->f32 && !c->host_blocks.empty() ? build_ranges(c) : DMPM_OK;
+    case DMPM_OPT_FUSE:
+        if (value != 0 && value != 1) return fail(DMPM_ERR_INVALID, "DMPM_OPT_FUSE is 0 or 1");
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->fuse = value != 0;
+        clear_graphs(c);
+        return DMPM_OK;
     default:
         return fail(DMPM_ERR_INVALID, "unknown option");
     }
@@ -1857,6 +1905,8 @@ int dmpm_get_option(dmpm_ctx* c, int32_t option, int64_t* value)
     case DMPM_OPT_RESORT: *value = c->resort_mode; return DMPM_OK;
     case DMPM_OPT_RESORT_PERMILLE: *value = c->resort_permille; return DMPM_OK;
     case DMPM_OPT_OVERLAP: *value = c->overlap ? 1 : 0; return DMPM_OK;
+    case DMPM_OPT_FUSE: *value = c->fuse ? 1 : 0; return DMPM_OK;
+    case DMPM_OPT_RANGE_ORDER: *value = c->range_order; return DMPM_OK;
     default: return fail(DMPM_ERR_INVALID, "unknown option");
     }
 }

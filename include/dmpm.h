@@ -54,7 +54,8 @@ extern "C" {
 #define DMPM_K_P2G_ADJ 5
 #define DMPM_K_LOSS 6
 #define DMPM_K_FILL 7
-#define DMPM_K_COUNT 8
+#define DMPM_K_STEP 8 /* FP32: G2P of a step with P2G of the next in one launch (DMPM_OPT_FUSE) */
+#define DMPM_K_COUNT 9
 
 typedef struct dmpm_ctx dmpm_ctx;
 
@@ -200,6 +201,9 @@ int dmpm_chain_episode(dmpm_ctx* ctx);
                                     * many per mille of the P2G workgroups left the fixed 8^3 window (default 50) */
 #define DMPM_OPT_OVERLAP 6   /* checkpoint tape: 1 (default) recomputes the next segment down on a second stream
                               * while the backward pass steps through the current one (two tapes); 0: one stream */
+#define DMPM_OPT_FUSE 7      /* FP32: 1 (default) runs the G2P of each step and the P2G of the next as one launch
+                              * inside a pass; 0: separate launches (the same results) */
+#define DMPM_OPT_RANGE_ORDER 8 /* FP32: launch order of the block ranges: 1 (default) longest first, 0 block order */
 int dmpm_set_option(dmpm_ctx* ctx, int32_t option, int64_t value);
 int dmpm_get_option(dmpm_ctx* ctx, int32_t option, int64_t* value);
 /* FP32: time a forward and a backward step of layer 0 for every candidate range size and LDS

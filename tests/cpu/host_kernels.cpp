@@ -74,7 +74,8 @@ template void launch_sumsq<float>(const float*, long long, double*, double*, hip
 }
 void launch_clear_tiles(const Batch&, float*, unsigned*, hipStream_t) { fp32_only("clear_tiles"); }
 void launch_set_flags(unsigned*, long long, hipStream_t) { fp32_only("set_flags"); }
-void launch_p2g_blk(const Batch&, const StepIO<float>&, hipStream_t) { fp32_only("p2g_blk"); }
+void launch_p2g_blk(const Batch&, const StepIO<float>&, const TileClear&, hipStream_t) { fp32_only("p2g_blk"); }
+void launch_step_blk(const Batch&, const StepIO<float>&, const StepIO<float>&, const TileClear&, hipStream_t) { fp32_only("step_blk"); }
 void launch_p2g_mass_blk(const Batch&, const float*, const float*, float*, hipStream_t) { fp32_only("p2g_mass_blk"); }
 void launch_g2p_blk(const Batch&, const StepIO<float>&, const TileClear&, hipStream_t) { fp32_only("g2p_blk"); }
 void launch_g2p_adj_blk(const Batch&, const AdjIO<float>&, hipStream_t) { fp32_only("g2p_adj_blk"); }
