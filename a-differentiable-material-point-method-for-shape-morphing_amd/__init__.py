@@ -42,6 +42,7 @@ OPT_RESORT_PERMILLE = 5
 OPT_OVERLAP = 6
 OPT_FUSE = 7
 OPT_RANGE_ORDER = 8
+OPT_SEG_RESORT = 9
 
 K_P2G, K_GRID_UPDATE, K_G2P, K_G2P_ADJ, K_GRID_ADJ, K_P2G_ADJ, K_LOSS, K_FILL = range(8)
 KERNEL_NAMES = ["p2g", "grid_update", "g2p", "g2p_adj", "grid_adj", "p2g_adj", "loss", "fill", "step"]
@@ -61,6 +62,7 @@ EXPORTS = [
     "dmpm_eval_energy", "dmpm_get_point_diagnostics", "dmpm_target_from_points_noalloc",
     "dmpm_phase_p2g", "dmpm_phase_grid_op", "dmpm_phase_grid_op_point", "dmpm_phase_g2p", "dmpm_phase_p_op2",
     "dmpm_phase_p2g_mass", "dmpm_phase_g2g_mass", "dmpm_sort_particles", "dmpm_get_window_stats",
+    "dmpm_get_segment_window_stats",
 ]
 
 # most points of one dmpm_target_from_points_noalloc call (DMPM_NOALLOC_MAX_POINTS)
@@ -189,6 +191,7 @@ def load_library(path: str = LIB_PATH):
         "dmpm_phase_p_op2": ([i32] + [_DP] * 5 + [ctypes.c_double] * 2 + [_DP] * 2, ctypes.c_int),
         "dmpm_sort_particles": ([vp], ctypes.c_int),
         "dmpm_get_window_stats": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "dmpm_get_segment_window_stats": ([vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "dmpm_phase_p2g_mass": ([ctypes.POINTER(GridDesc), i32, _DP, _DP, _DP], ctypes.c_int),
         "dmpm_phase_g2g_mass": ([ctypes.POINTER(GridDesc), _DP, ctypes.POINTER(GridDesc), _DP], ctypes.c_int),
     }
@@ -582,6 +585,13 @@ class Morph:
         a = (ctypes.c_int64 * 4)()
         _check(self._L.dmpm_get_window_stats(self._h, a))
         return {"fixed": int(a[0]), "general": int(a[1]), "hbm": int(a[2]), "sorts": int(a[3])}
+
+    def segment_window_stats(self, segment):
+        """FP32 checkpoint tape: one segment's P2G window modes in the last forward pass that
+        reached the loss, whether it holds an order of its own, and the segment sorts so far."""
+        a = (ctypes.c_int64 * 5)()
+        _check(self._L.dmpm_get_segment_window_stats(self._h, int(segment), a))
+        return {"fixed": int(a[0]), "general": int(a[1]), "hbm": int(a[2]), "own_order": bool(a[3]), "seg_sorts": int(a[4])}
 
     def autotune(self):
         """FP32: pick the fastest range size / LDS capacity; returns its step time (ms).

@@ -76,6 +76,10 @@ __device__ __forceinline__ int fixed_slot_of(int idx)  // idx: x-major node of t
     return (idx >> 6) * kSI + ((idx >> 3) & 7) * kSJ + (idx & 7);
 }
 
+// Slot of particle slot p in the home order of the constants, controls and
+// control gradients (bt.home: null when the particles are in that order).
+__device__ __forceinline__ int home_of(const Batch& bt, int p) { return bt.home ? bt.home[p] : p; }
+
 struct Win {
     int lo[3];   // general window: clamped box
     int w[3];
@@ -655,18 +659,19 @@ __device__ __forceinline__ void p2g_part(const StepParams& sp, const M3<float>& 
                                                                          fabsf(q.A.a[3 * a + 2])));
 }
 
-__device__ __forceinline__ void p2g_prepare(const StepParams& sp, const StepIO<float>& io, int p, P2GPart& q, float bound[4])
+// h: the particle's home slot (constants, control)
+__device__ __forceinline__ void p2g_prepare(const StepParams& sp, const StepIO<float>& io, int p, int h, P2GPart& q, float bound[4])
 {
     const int S = sp.stride;
     const M3<float> F = ldm(io.cur, FF, S, p);
-    const M3<float> dF = ldm(io.ctrl, 0, S, p);
-    const double vol = io.cst[KVOL * S + p];
-    const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
+    const M3<float> dF = ldm(io.ctrl, 0, S, h);
+    const double vol = io.cst[KVOL * S + h];
+    const double lam = io.cst[KLAM * S + h], mu = io.cst[KMU * S + h];
     auto rest = [&](float x[3], float v[3], M3<float>& C, float& m) {
         ld3(io.cur, FX, S, p, x);
         ld3(io.cur, FV, S, p, v);
         C = ldm(io.cur, FC, S, p);
-        m = io.cst[KM * S + p];
+        m = io.cst[KM * S + h];
     };
     p2g_part(sp, F, dF, vol, lam, mu, rest, io.aux, p, q, bound);
 }
@@ -786,11 +791,11 @@ __global__ __launch_bounds__(kP2G, 4) void k_p2g_blk(Batch bt, StepIO<float> io,
     float bound[4] = {0.f, 0.f, 0.f, 0.f};
     int box[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
     if (v0) {
-        p2g_prepare(sp, io, p0, q0, bound);
+        p2g_prepare(sp, io, p0, home_of(bt, p0), q0, bound);
         box_add(sp, q0.x, box);
     }
     if (v1) {
-        p2g_prepare(sp, io, p1, q1, bound);
+        p2g_prepare(sp, io, p1, home_of(bt, p1), q1, bound);
         box_add(sp, q1.x, box);
     }
     p2g_scatter(bt, sp, io, nofs, rg.y - rg.x, q0, q1, v0, v1, bound, box, lds64);
@@ -813,7 +818,7 @@ __global__ __launch_bounds__(kScat) void k_p2g_mass_blk(Batch bt, const float* l
     float bound[1] = {0.f};
     if (valid) {
         ld3(lay, FX, S, p, x);
-        m = cst[KM * S + p];
+        m = cst[KM * S + home_of(bt, p)];
         bound[0] = fabsf(m);
     }
     const Scales<1> SC = block_scales<1, false>(bound, rg.y - rg.x);
@@ -860,7 +865,7 @@ struct NodeVel {
 // G2P and P_op_2 (ForwardSimulation.cpp:60-124) of particle p at x: its next
 // state from the window W of node velocities in lds
 __device__ __forceinline__ void g2p_particle(const StepParams& sp, const StepIO<float>& io, const Win& W, const float* lds,
-                                             const NodeVel& ld, int p, const float x[3], float x1[3], float v1[3], M3<float>& C1,
+                                             const NodeVel& ld, int p, int h, const float x[3], float x1[3], float v1[3], M3<float>& C1,
                                              M3<float>& F1)
 {
     const int S = sp.stride;
@@ -903,7 +908,7 @@ __device__ __forceinline__ void g2p_particle(const StepParams& sp, const StepIO<
     else if (W.lds) gather64<3, 1>(sp, W, st, lds, ld, acc, row);
     else gather64<3, 2>(sp, W, st, lds, ld, acc, row);
     const M3<float> F = ldm(io.cur, FF, S, p);
-    const M3<float> dF = ldm(io.ctrl, 0, S, p);
+    const M3<float> dF = ldm(io.ctrl, 0, S, h);
     M3<float> Fc, IC;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
@@ -979,7 +984,7 @@ __global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io, Ti
         for (int a = 0; a < 3; ++a) x[a] = h ? xs[1][a] : xs[0][a];
         float x1[3], v1[3];
         M3<float> C1, F1;
-        g2p_particle(sp, io, W, lds, ld, p, x, x1, v1, C1, F1);
+        g2p_particle(sp, io, W, lds, ld, p, home_of(bt, p), x, x1, v1, C1, F1);
         store_state(sp, io.next, p, x1, v1, C1, F1);
     }
 }
@@ -1018,11 +1023,12 @@ __global__ __launch_bounds__(kP2G, 4) void k_step_blk(Batch bt, StepIO<float> gi
     auto one = [&](int p, const float x[3], P2GPart& q) {
         float xn[3], vn[3];
         M3<float> Cn, Fn;
-        g2p_particle(sp, gio, W, reinterpret_cast<const float*>(lds64), ld, p, x, xn, vn, Cn, Fn);
+        const int h = home_of(bt, p);
+        g2p_particle(sp, gio, W, reinterpret_cast<const float*>(lds64), ld, p, h, x, xn, vn, Cn, Fn);
         store_state(sp, gio.next, p, xn, vn, Cn, Fn);
-        const M3<float> dF = ldm(pio.ctrl, 0, S, p);
-        const double vol = pio.cst[KVOL * S + p];
-        const double lam = pio.cst[KLAM * S + p], mu = pio.cst[KMU * S + p];
+        const M3<float> dF = ldm(pio.ctrl, 0, S, h);
+        const double vol = pio.cst[KVOL * S + h];
+        const double lam = pio.cst[KLAM * S + h], mu = pio.cst[KMU * S + h];
         auto rest = [&](float x_[3], float v_[3], M3<float>& C_, float& m_) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -1030,7 +1036,7 @@ __global__ __launch_bounds__(kP2G, 4) void k_step_blk(Batch bt, StepIO<float> gi
                 v_[a] = vn[a];
             }
             C_ = Cn;
-            m_ = pio.cst[KM * S + p];
+            m_ = pio.cst[KM * S + h];
         };
         p2g_part(sp, Fn, dF, vol, lam, mu, rest, pio.aux, p, q, bound);
         box_add(sp, q.x, box);
@@ -1051,7 +1057,7 @@ struct G2PAdjPart {
     M3<float> Ct;
 };
 
-__device__ __forceinline__ void g2p_adj_prepare(const StepParams& sp, const AdjIO<float>& io, int p, G2PAdjPart& q, float bound[3])
+__device__ __forceinline__ void g2p_adj_prepare(const StepParams& sp, const AdjIO<float>& io, int p, int h, G2PAdjPart& q, float bound[3])
 {
     const int S = sp.stride;
     const float Dd = float(3.0 * sp.inv_dx);
@@ -1062,7 +1068,7 @@ __device__ __forceinline__ void g2p_adj_prepare(const StepParams& sp, const AdjI
     ld3(io.g1, FX, S, p, gx1);
     ld3(io.g1, FV, S, p, gv1);
     const M3<float> F = ldm(io.cur, FF, S, p);
-    const M3<float> dF = ldm(io.ctrl, 0, S, p);
+    const M3<float> dF = ldm(io.ctrl, 0, S, h);
     const M3<float> gF1 = ldm(io.g1, FF, S, p);
     const M3<float> gC1 = ldm(io.g1, FC, S, p);
     M3<float> Fc;
@@ -1109,11 +1115,11 @@ __global__ __launch_bounds__(kG2PAdj, 4) void k_g2p_adj_blk(Batch bt, AdjIO<floa
     float bound[3] = {0.f, 0.f, 0.f};
     int box[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
     if (v0) {
-        g2p_adj_prepare(sp, io, p0, q0, bound);
+        g2p_adj_prepare(sp, io, p0, home_of(bt, p0), q0, bound);
         box_add(sp, q0.x, box);
     }
     if (v1) {
-        g2p_adj_prepare(sp, io, p1, q1, bound);
+        g2p_adj_prepare(sp, io, p1, home_of(bt, p1), q1, bound);
         box_add(sp, q1.x, box);
     }
     Win W;
@@ -1189,7 +1195,7 @@ __global__ __launch_bounds__(kG2PAdj, 4) void k_g2p_adj_blk(Batch bt, AdjIO<floa
 // U, sigma, V (SVD); each path is its own copy of the code, so that registers
 // are allocated for one decomposition at a time.
 template <bool SVD>
-__device__ __forceinline__ void elastic_adj_path(const StepParams& sp, const AdjIO<float>& io, int p, const M3<float>& Gb,
+__device__ __forceinline__ void elastic_adj_path(const StepParams& sp, const AdjIO<float>& io, int p, int h, const M3<float>& Gb,
                                                  const M3<double>& Fc, FCE& e, double vol, double lam, double mu)
 {
     e.svd = SVD;
@@ -1229,16 +1235,16 @@ __device__ __forceinline__ void elastic_adj_path(const StepParams& sp, const Adj
         }
     }
     stm(io.g0, FF, S, p, gF);
-    stm(io.gctrl, 0, S, p, gdFc);
+    stm(io.gctrl, 0, S, h, gdFc);
 }
 
-__device__ __forceinline__ void elastic_adj(const StepParams& sp, const AdjIO<float>& io, int p, const M3<float>& Gb)
+__device__ __forceinline__ void elastic_adj(const StepParams& sp, const AdjIO<float>& io, int p, int h, const M3<float>& Gb)
 {
     const int S = sp.stride;
     const M3<float> F = ldm(io.cur, FF, S, p);
-    const M3<float> dF = ldm(io.ctrl, 0, S, p);
-    const double vol = io.cst[KVOL * S + p];
-    const double lam = io.cst[KLAM * S + p], mu = io.cst[KMU * S + p];
+    const M3<float> dF = ldm(io.ctrl, 0, S, h);
+    const double vol = io.cst[KVOL * S + h];
+    const double lam = io.cst[KLAM * S + h], mu = io.cst[KMU * S + h];
     M3<double> Fc;
 #pragma unroll
     for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
@@ -1256,11 +1262,11 @@ __device__ __forceinline__ void elastic_adj(const StepParams& sp, const AdjIO<fl
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) e.S.a[3 * i + j] = 0.5 * (RF.a[3 * i + j] + RF.a[3 * j + i]);
-        elastic_adj_path<false>(sp, io, p, Gb, Fc, e, vol, lam, mu);
+        elastic_adj_path<false>(sp, io, p, h, Gb, Fc, e, vol, lam, mu);
     } else {
         svd3(Fc, e.U, e.s, e.V);
         e.R = mult(e.U, e.V);
-        elastic_adj_path<true>(sp, io, p, Gb, Fc, e, vol, lam, mu);
+        elastic_adj_path<true>(sp, io, p, h, Gb, Fc, e, vol, lam, mu);
     }
 }
 
@@ -1320,7 +1326,7 @@ __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
     const float bsc = (cp & 4) ? 1.f : sdrag;
     const double kst = -3.0 * sp.inv_dx * sp.inv_dx * sp.dt;
     for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) {
-        const float m = io.cst[KM * S + p];
+        const float m = io.cst[KM * S + home_of(bt, p)];
         const M3<float> Gm = ldm(io.cur, FA, S, p);
         float x[3], v[3], vt[3];
         ld3(io.cur, FX, S, p, x);
@@ -1398,7 +1404,7 @@ __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
 #pragma unroll
         for (int i = 0; i < 9; ++i) gC.a[i] = m * Gb.a[i];
         stm(io.g0, FC, S, p, gC);
-        if (FUSE) elastic_adj(sp, io, p, Gb);
+        if (FUSE) elastic_adj(sp, io, p, home_of(bt, p), Gb);
         else stm(io.tmp, 0, S, p, Gb);  // vt, Ct are dead for this particle
     }
 }
@@ -1413,7 +1419,7 @@ __global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io,
     const int4 rg = bt.ranges[blockIdx.x];
     const StepParams sp = bt.sps[rg.z];
     const int S = sp.stride;
-    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) elastic_adj(sp, io, p, ldm(io.tmp, 0, S, p));
+    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) elastic_adj(sp, io, p, home_of(bt, p), ldm(io.tmp, 0, S, p));
 }
 
 // --------------------------------------------------- batched grid kernels --
@@ -1497,7 +1503,7 @@ __global__ __launch_bounds__(kGat) void k_loss_particles_b(Batch bt, const float
                 }
             }
         }
-        const double sc = -double(cst[KM * S + p]) * sp.inv_dx;
+        const double sc = -double(cst[KM * S + home_of(bt, p)]) * sp.inv_dx;
 #pragma unroll
         for (int a = 0; a < 3; ++a) g[(long long)(FX + a) * S + p] = float(sc * gx[a]);
 #pragma unroll

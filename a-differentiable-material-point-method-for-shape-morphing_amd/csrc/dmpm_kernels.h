@@ -109,6 +109,8 @@ struct Batch {
     int cap;
     long long gmax;         // nodes of the largest instance
     unsigned long long* wstats;  // P2G workgroups per window mode (fixed, general, HBM), or null
+    const int* home;        // slot of each particle slot in the order of the constants, controls and
+                            // control gradients (null: the same order)
 };
 // 4-channel grids (m or mbar, then 3 vector channels) of instance i start at 4 * nofs[i];
 // 1-channel arrays (target, dL/dm) at nofs[i].
@@ -176,6 +178,10 @@ size_t sort_temp_bytes(int N, long long nblocks);
 hipError_t launch_sort_order(const SortArgs& a, hipStream_t s);
 // dst[f S + d] = src[f S + perm[d]], f < nf
 template <typename T> void launch_gather(const T* src, T* dst, long long S, int N, int nf, const int* perm, hipStream_t s);
+// dst[f S + idx[d]] = src[f S + d] for nf fields
+template <typename T> void launch_scatter(const T* src, T* dst, long long S, int N, int nf, const int* idx, hipStream_t s);
+// inv[a[d]] = d
+void launch_invert(const int* a, int* inv, int N, hipStream_t s);
 void launch_iota(int* a, int N, hipStream_t s);
 
 // batched constitutive evaluations (AoS 9 doubles per matrix)

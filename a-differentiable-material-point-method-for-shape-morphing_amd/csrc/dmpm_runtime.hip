@@ -124,13 +124,37 @@ struct dmpm_ctx {
     void* sortbuf = nullptr;        // sort keys, values, heads, blocks and rocPRIM scratch (first sort)
     size_t sort_temp = 0;
     // P2G workgroups per window mode (fixed, general, HBM) since the forward
-    // pass began, and their snapshot at the last EndLayerMassLoss
+    // pass began (3 per segment, nstat segments), and their sums at the last EndLayerMassLoss
     unsigned long long* wstats = nullptr;
     long long last_wstats[3] = {0, 0, 0};
     int resort_mode = 2;            // 0: at set_particles only; 1: + every chain_episode; 2: chain_episode when stale; 1, 2: forward_pass(0) when stale
     int resort_permille = 50;       // stale: more than this share of P2G workgroups off the fixed window
     long long n_sorts = 0;
     std::vector<int4> host_blocks;  // {begin, end, instance} of every non-empty 4^3-cell block
+    // FP32 checkpoint tape: segment s >= 1 may keep its layers (and the
+    // gradients of its layers) in a particle order of its own, sorted by the
+    // positions of its first layer, when its P2G windows went stale in the last
+    // forward pass that reached the loss (seg_resort).  The constants, controls,
+    // control gradients and optimiser states stay in the home order (the order
+    // of layer 0), which the kernels reach through home[].  A layer on a
+    // segment boundary is gathered into the next segment's order (bidx); the
+    // backward pass scatters its gradients back.
+    struct SegOrder {
+        bool own = false;         // an order of its own (else the home order)
+        int* home = nullptr;      // N: home slot of each slot
+        int* inv = nullptr;       // N: slot of each home slot
+        int* bidx = nullptr;      // N: slot in segment s - 1's order of each slot of this segment's (orders differ)
+        bool bidx_ok = false;
+        int4* ranges = nullptr;   // block ranges of this order
+        int nranges = 0;
+        std::vector<int4> host_blocks;
+        size_t ranges_cap = 0;
+    };
+    std::vector<SegOrder> sord;
+    int seg_resort = 1;
+    long long n_seg_sorts = 0;
+    int nstat = 1;                       // window statistics kept per segment (FP32 checkpoint) or for the pass
+    std::vector<long long> last_swstats; // 3 per segment, snapshot at EndLayerMassLoss
     int4* ranges = nullptr;         // per-workgroup {begin, end, instance}: blocks split to <= range_max
     std::vector<int4> host_ranges;
     int n_ranges = 0;
@@ -435,7 +459,7 @@ void clear_grid(dmpm_ctx* c, double* g) { fill<double>(c, g, (long long)kGridFie
 // restart the window-mode counts (a kernel, so that graphs may hold it)
 void zero_wstats(dmpm_ctx* c)
 {
-    if (c->wstats) launch_fill<double>(reinterpret_cast<double*>(c->wstats), 3, 0.0, c->stream);
+    if (c->wstats) launch_fill<double>(reinterpret_cast<double*>(c->wstats), 3LL * c->nstat, 0.0, c->stream);
 }
 
 // wgrid used as scratch (node masses, dL/dm): every tile may be non-zero
@@ -536,6 +560,77 @@ bool step_backward(dmpm_ctx* c, int i, const Real* cur, const Real* nxt, const R
     }
 }
 
+// --------------------------------------------------------- segment orders --
+bool seg_own(const dmpm_ctx* c, int s) { return s >= 1 && s < (int)c->sord.size() && c->sord[s].own; }
+
+// the orders of segments s - 1 and s differ: layer sK is gathered between them
+bool boundary_permuted(const dmpm_ctx* c, int s) { return seg_own(c, s - 1) || seg_own(c, s); }
+
+// the segment whose order holds layer j as the passes leave it: a checkpoint
+// its own segment's, the last layer the last segment's
+int layer_seg(const dmpm_ctx* c, int j) { return j >= c->L - 1 ? c->seg_last : j / std::max(c->K, 1); }
+
+// the launch configuration of segment s's steps: its block ranges and home
+// map, and its window statistics
+Batch batch_of(const dmpm_ctx* c, int s)
+{
+    Batch b = c->batch;
+    b.ranges = c->ranges;
+    b.nranges = c->n_ranges;
+    b.home = nullptr;
+    b.wstats = c->wstats ? c->wstats + 3 * (size_t)std::min(std::max(s, 0), c->nstat - 1) : nullptr;
+    if (seg_own(c, s)) {
+        const auto& o = c->sord[s];
+        b.ranges = o.ranges;
+        b.nranges = o.nranges;
+        b.home = o.home;
+    }
+    return b;
+}
+
+// c->batch is segment s's for the launches of a scope
+struct BatchScope {
+    dmpm_ctx* c;
+    Batch saved;
+    BatchScope(dmpm_ctx* ctx, int s) : c(ctx), saved(ctx->batch) { c->batch = batch_of(ctx, s); }
+    ~BatchScope() { c->batch = saved; }
+};
+
+// slot in segment s - 1's order of each slot of segment s's order, or null when
+// the two orders are the same; computed on the stream when first needed after
+// either order changed (prepare_bidx does it before a graph capture)
+const int* bidx_of(dmpm_ctx* c, int s)
+{
+    if (!boundary_permuted(c, s)) return nullptr;
+    auto& o = c->sord[s];
+    if (!o.bidx_ok) {
+        const size_t bytes = (size_t)c->N * sizeof(int);
+        if (seg_own(c, s) && seg_own(c, s - 1))
+            launch_gather<int>(c->sord[s - 1].inv, o.bidx, (long long)c->S, c->N, 1, o.home, c->stream);
+        else if (seg_own(c, s))
+            hipMemcpyAsync(o.bidx, o.home, bytes, hipMemcpyDeviceToDevice, c->stream);
+        else
+            hipMemcpyAsync(o.bidx, c->sord[s - 1].inv, bytes, hipMemcpyDeviceToDevice, c->stream);
+        o.bidx_ok = true;
+    }
+    return o.bidx;
+}
+
+void prepare_bidx(dmpm_ctx* c)
+{
+    for (int s = 1; s < (int)c->sord.size(); ++s) bidx_of(c, s);
+}
+
+// segment s's P2G windows in the last forward pass that reached the loss:
+// more than resort_permille per mille of its workgroups off the fixed window
+bool seg_stale(const dmpm_ctx* c, int s)
+{
+    if ((int)c->last_swstats.size() < 3 * (s + 1)) return false;
+    const long long* w = &c->last_swstats[3 * (size_t)s];
+    const long long tot = w[0] + w[1] + w[2];
+    return tot > 0 && (w[1] + w[2]) * 1000 > (long long)c->resort_permille * tot;
+}
+
 // checkpoint-mode forward grid of a step outside the tape: buffers in
 // rotation, so that a step's G2P can clear the next step's grid (FP64: two),
 // also while it runs as one launch with the next step's P2G (FP32: three)
@@ -568,12 +663,18 @@ void forward_seq(dmpm_ctx* c, int i0, int i1, LayFn&& lay, GridFn&& grid, KeepFn
                                      keep(i) ? lay(i) : nullptr, flags_of(c, g)};
             };
             auto clear_of = [&](int i) { return i < i1 ? TileClear{grid(i), flags_of(c, grid(i))} : TileClear{}; };
+            // the P2G of step i counts its window modes for step i's segment
+            auto batch_at = [&](int i) {
+                Batch b = c->batch;
+                if (c->nstat > 1 && c->wstats) b.wstats = c->wstats + 3 * (size_t)std::min(i / c->K, c->nstat - 1);
+                return b;
+            };
             clear_grid(c, grid(i0));
-            timed(c, DMPM_K_P2G, [&] { launch_p2g_blk(c->batch, pio_of(i0), clear_of(i0 + 1), c->stream); });
+            timed(c, DMPM_K_P2G, [&] { launch_p2g_blk(batch_at(i0), pio_of(i0), clear_of(i0 + 1), c->stream); });
             for (int i = i0 + 1; i < i1; ++i) {
                 const StepIO<float> gio{lay(i - 1), lay(i), ctrl_of<float>(c, i - 1), static_cast<const float*>(c->cst), grid(i - 1),
                                         nullptr, nullptr};
-                timed(c, DMPM_K_STEP, [&] { launch_step_blk(c->batch, gio, pio_of(i), clear_of(i + 1), c->stream); });
+                timed(c, DMPM_K_STEP, [&] { launch_step_blk(batch_at(i), gio, pio_of(i), clear_of(i + 1), c->stream); });
             }
             if (last_g2p) {
                 const StepIO<float> gio{lay(i1 - 1), lay(i1), ctrl_of<float>(c, i1 - 1), static_cast<const float*>(c->cst),
@@ -584,9 +685,13 @@ void forward_seq(dmpm_ctx* c, int i0, int i1, LayFn&& lay, GridFn&& grid, KeepFn
         }
     }
     bool clean = false;
-    for (int i = i0; i < i1; ++i)
+    unsigned long long* const ws = c->batch.wstats;
+    for (int i = i0; i < i1; ++i) {
+        if (c->nstat > 1 && c->wstats) c->batch.wstats = c->wstats + 3 * (size_t)std::min(i / c->K, c->nstat - 1);
         clean = step_forward<Real>(c, lay(i), lay(i + 1), ctrl_of<Real>(c, i), grid(i), clean, i + 1 < i1 ? grid(i + 1) : nullptr,
                                    keep(i), i + 1 < i1 || last_g2p);
+    }
+    c->batch.wstats = ws;
 }
 
 // Bring layer j (<= fwd_valid) into a readable slot and return it.  Checkpoint
@@ -600,6 +705,7 @@ Real* materialize(dmpm_ctx* c, int j)
     if (tape_of(c, j / c->K) >= 0) return tape_layer<Real>(c, j);
     const int j0 = (j / c->K) * c->K;
     auto lay = [&](int k) { return k == j0 ? slot<Real>(c, j0 / c->K) : slot<Real>(c, c->roll_base + (k & 1)); };
+    BatchScope bs(c, j / c->K);
     forward_seq<Real>(c, j0, j, lay, [&](int i) { return wgrid_of<Real>(c, i); }, [](int) { return false; });
     return lay(j);
 }
@@ -612,9 +718,220 @@ void recompute_segment(dmpm_ctx* c, int seg)
     const int t = seg & 1;
     const int s0 = seg * c->K, s1 = std::min(s0 + c->K, c->L - 1);
     // a segment that ends on a checkpoint needs only the grid of its last
-    // step: the backward pass reads the next layer from the checkpoint
-    forward_seq<Real>(c, s0, s1, [&](int j) { return tape_slot<Real>(c, t, j); }, [&](int i) { return tape_grid<Real>(c, t, i); },
-                      [](int) { return true; }, s1 % c->K != 0);
+    // step: the backward pass reads the next layer from the checkpoint, unless
+    // the checkpoint holds it in another order; then the last G2P writes it in
+    // this segment's order to the tape's boundary slot
+    const bool perm_b = s1 < c->L - 1 && boundary_permuted(c, seg + 1);
+    Real* bslot = slot<Real>(c, c->tape_base[t] + c->K);
+    BatchScope bs(c, seg);
+    forward_seq<Real>(c, s0, s1, [&](int j) { return (j == s1 && perm_b) ? bslot : tape_slot<Real>(c, t, j); },
+                      [&](int i) { return tape_grid<Real>(c, t, i); }, [](int) { return true; }, s1 % c->K != 0 || perm_b);
+}
+
+// ------------------------------------------------------------ block sort ---
+// Workgroup ranges of blocks {begin, end, instance}: at most range_max
+// particles each, longest first with range_order 1 (workgroups start in range
+// order, so the short ones fill the last round of a launch).
+std::vector<int4> make_ranges(const dmpm_ctx* c, const std::vector<int4>& blocks)
+{
+    std::vector<int4> rg;
+    for (const int4& b : blocks) {
+        const int len = b.y - b.x;
+        const int parts = (len + c->range_max - 1) / c->range_max;
+        for (int k = 0; k < parts; ++k)
+            rg.push_back(make_int4(b.x + (int)((long long)len * k / parts), b.x + (int)((long long)len * (k + 1) / parts), b.z, 0));
+    }
+    if (c->range_order == 1)
+        std::stable_sort(rg.begin(), rg.end(), [](const int4& a, const int4& b) { return a.y - a.x > b.y - b.x; });
+    return rg;
+}
+
+// ranges of a segment order into its device array (grown when short)
+int upload_seg_ranges(dmpm_ctx* c, dmpm_ctx::SegOrder& o)
+{
+    const std::vector<int4> rg = make_ranges(c, o.host_blocks);
+    if (rg.size() > o.ranges_cap) {
+        if (o.ranges) hipFree(o.ranges);
+        o.ranges = nullptr;
+        o.ranges_cap = 0;
+        const int rc = dalloc(c, (void**)&o.ranges, rg.size() * sizeof(int4));
+        if (rc) return rc;
+        o.ranges_cap = rg.size();
+    }
+    if (!rg.empty())
+        HIP_TRY(hipMemcpyAsync(o.ranges, rg.data(), rg.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    o.nranges = (int)rg.size();
+    return DMPM_OK;
+}
+
+// The block order of the particles whose positions are the x field of lay
+// (dmpm_sort.hip): a.v1[d] is the slot of lay whose particle goes to slot d,
+// a.blk[d] its global block.
+int run_sort(dmpm_ctx* c, const float* lay, SortArgs& a)
+{
+    const int N = c->N;
+    const long long nblocks = c->boff[c->B];
+    if (!c->sortbuf) {
+        c->sort_temp = sort_temp_bytes(N, nblocks);
+        if (c->sort_temp == 0) return fail(DMPM_ERR_HIP, "rocPRIM sort: temporary storage query failed");
+        const int rc = dalloc(c, &c->sortbuf, (size_t)N * (2 * 8 + 4 * 4) + 256 + c->sort_temp);
+        if (rc) return rc;
+    }
+    auto* k0 = static_cast<unsigned long long*>(c->sortbuf);
+    a = SortArgs{};
+    a.sps = c->d_sps;
+    a.poff = c->d_poff;
+    a.boff = c->d_boff;
+    a.B = c->B;
+    a.nblocks = nblocks;
+    a.x = lay;
+    a.S = (long long)c->S;
+    a.N = N;
+    a.k0 = k0;
+    a.k1 = k0 + N;
+    a.v0 = reinterpret_cast<int*>(a.k1 + N);
+    a.v1 = a.v0 + N;
+    a.head = a.v1 + N;
+    a.blk = a.head + N;
+    a.temp = reinterpret_cast<char*>(c->sortbuf) + (((size_t)N * (2 * 8 + 4 * 4) + 255) & ~(size_t)255);
+    a.temp_bytes = c->sort_temp;
+    HIP_TRY(launch_sort_order(a, c->stream));
+    return DMPM_OK;
+}
+
+// {begin, end, instance} of every non-empty block from the sorted block ids
+std::vector<int4> blocks_from(const dmpm_ctx* c, const std::vector<int>& blk)
+{
+    std::vector<int4> out;
+    const int N = (int)blk.size();
+    int in = 0;
+    for (int d0 = 0, d = 1; d <= N; ++d) {
+        if (d == N || blk[d] != blk[d0]) {
+            while (in + 1 < c->B && c->boff[in + 1] <= blk[d0]) ++in;
+            out.push_back(make_int4(d0, d, in, 0));
+            d0 = d;
+        }
+    }
+    return out;
+}
+
+// every segment back on the home order: layers held in segment orders are
+// no longer readable, so the caller invalidates the pass
+void drop_seg_orders(dmpm_ctx* c)
+{
+    bool any = false;
+    for (auto& o : c->sord) {
+        any = any || o.own;
+        o.own = false;
+        o.bidx_ok = false;
+    }
+    if (any) clear_graphs(c);
+}
+
+// A new order for segment s >= 1 from the positions of its first layer, held
+// in lay in segment s - 1's order: by block as the home order is, composed
+// with segment s - 1's map to the home order.
+int seg_sort(dmpm_ctx* c, int s, const float* lay)
+{
+    auto& o = c->sord[s];
+    const int N = c->N;
+    const size_t bytes = (size_t)N * sizeof(int);
+    int rc;
+    if (!o.home && ((rc = dalloc(c, (void**)&o.home, bytes)) || (rc = dalloc(c, (void**)&o.inv, bytes)))) return rc;
+    for (int k = s; k <= std::min(s + 1, c->seg_last); ++k)
+        if (!c->sord[k].bidx && (rc = dalloc(c, (void**)&c->sord[k].bidx, bytes))) return rc;
+    SortArgs a;
+    if ((rc = run_sort(c, lay, a))) return rc;
+    HIP_TRY(hipMemcpyAsync(o.bidx, a.v1, bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (seg_own(c, s - 1)) launch_gather<int>(c->sord[s - 1].home, o.home, (long long)c->S, N, 1, a.v1, c->stream);
+    else HIP_TRY(hipMemcpyAsync(o.home, a.v1, bytes, hipMemcpyDeviceToDevice, c->stream));
+    launch_invert(o.home, o.inv, N, c->stream);
+    HIP_TRY(hipGetLastError());
+    std::vector<int> blk(N);
+    HIP_TRY(hipMemcpyAsync(blk.data(), a.blk, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    o.host_blocks = blocks_from(c, blk);
+    if ((rc = upload_seg_ranges(c, o))) return rc;
+    o.own = true;
+    o.bidx_ok = true;
+    if (s + 1 <= c->seg_last) c->sord[s + 1].bidx_ok = false;
+    for (int k = 0; k < 3; ++k) c->last_swstats[3 * (size_t)s + k] = 0;
+    ++c->n_seg_sorts;
+    clear_graphs(c);
+    return DMPM_OK;
+}
+
+// Gradients of layer s K (in segment s's order) into segment s - 1's order,
+// in place, through tmp (12 fields at a time).
+void grads_to_prev(dmpm_ctx* c, float* g, const int* bidx)
+{
+    float* t = static_cast<float*>(c->tmp);
+    for (int f = 0; f < kLayerFields; f += 12) {
+        const int k = std::min(12, kLayerFields - f);
+        launch_scatter<float>(g + (size_t)f * c->S, t, (long long)c->S, c->N, k, bidx, c->stream);
+        hipMemcpyAsync(g + (size_t)f * c->S, t, (size_t)k * c->S * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
+    }
+}
+
+// FP32 checkpoint forward pass from checkpoint `from`, one launch sequence per
+// segment, whose P2G window statistics are kept apart.  Segments whose
+// windows went stale in the last pass that reached the loss are ordered
+// afresh from their first layer (that pass runs without a graph: the sort
+// reads block counts back); each boundary layer is gathered into the next
+// segment's order when the two differ.
+int forward_segments(dmpm_ctx* c, int from)
+{
+    const int K = c->K;
+    const int s0_last = c->seg_last * K;
+    const int tl = c->seg_last & 1;
+    const int s_from = from / K;
+    std::vector<char> sort_at(c->seg_last + 2, 0);
+    bool any_sort = false;
+    if (c->seg_resort)
+        for (int s = s_from + 1; s <= c->seg_last; ++s)
+            if (seg_stale(c, s)) {
+                sort_at[s] = 1;
+                any_sort = true;
+            }
+    // checkpoints and the last segment go to the tape, other layers to the rolling slots
+    auto lay = [&](int j) {
+        if (j % K == 0 || j / K == c->seg_last) return tape_slot<float>(c, tl, j);
+        return slot<float>(c, c->roll_base + (j & 1));
+    };
+    auto grid_at = [&](int k) { return (k >= s0_last) ? tape_grid<float>(c, tl, k) : wgrid_of<float>(c, k); };
+    // steps before the last segment are recomputed for the backward pass: their affine matrices are not read
+    auto keep = [&](int i) { return i >= s0_last; };
+    int err = DMPM_OK;
+    auto pass = [&] {
+        zero_wstats(c);
+        for (int s = s_from; s <= c->seg_last;) {
+            // one launch sequence over segments s .. e, joined by boundaries in one order
+            int e = s;
+            while (e < c->seg_last && !sort_at[e + 1] && !boundary_permuted(c, e + 1)) ++e;
+            const int a = s * K, b = std::min((e + 1) * K, c->L - 1);
+            const bool perm_b = b < c->L - 1;  // the boundary after e changes the order
+            float* scratch = slot<float>(c, c->roll_base + (b & 1));
+            {
+                BatchScope bs(c, s);
+                forward_seq<float>(c, a, b, [&](int j) { return (j == b && perm_b) ? scratch : lay(j); }, grid_at, keep);
+            }
+            if (perm_b) {
+                if (sort_at[e + 1] && (err = seg_sort(c, e + 1, scratch))) return;
+                launch_gather<float>(scratch, lay(b), (long long)c->S, c->N, kLayerFields, bidx_of(c, e + 1), c->stream);
+            }
+            s = e + 1;
+        }
+    };
+    if (any_sort) {
+        pass();
+        if (err) return err;
+        HIP_TRY(hipGetLastError());
+        return DMPM_OK;
+    }
+    prepare_bidx(c);
+    const int rc = run_graph(c, from, pass);
+    return rc ? rc : err;
 }
 
 // -------------------------------------------------------------- passes -----
@@ -626,12 +943,13 @@ int forward_pass(dmpm_ctx* c, int start)
 {
     if (!c->have_particles) return fail(DMPM_ERR_STATE, "forward pass before dmpm_set_particles");
     if (start < 0 || start >= c->L - 1) return fail(DMPM_ERR_INVALID, "start_layer out of range");
-    if (c->f32 && start == 0 && c->resort_mode >= 1 && order_stale(c)) {
+    // the home order: the whole pass's windows, or with segment orders segment 0's
+    if (c->f32 && start == 0 && c->resort_mode >= 1 && (c->nstat > 1 ? seg_stale(c, 0) : order_stale(c))) {
         const int rs = resort(c, true);
         if (rs) return rs;
     }
     int from = std::min(start, c->fwd_valid);
-    int rc;
+    int rc = DMPM_OK;
     if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) {
         rc = run_graph(c, from, [&] {
             zero_wstats(c);
@@ -643,7 +961,7 @@ int forward_pass(dmpm_ctx* c, int start)
         from = (from / c->K) * c->K;  // restart from a checkpoint so the tape holds whole segments
         const int s0_last = c->seg_last * c->K;
         const int tl = c->seg_last & 1;  // the last segment's tape
-        rc = run_graph(c, from, [&] {
+        auto pass = [&] {
             zero_wstats(c);
             // checkpoints and the last segment go to the tape, other layers to the rolling slots
             auto lay = [&](int j) {
@@ -654,7 +972,12 @@ int forward_pass(dmpm_ctx* c, int start)
             // steps before the last segment are recomputed for the backward pass:
             // their affine matrices are not read
             forward_seq<Real>(c, from, c->L - 1, lay, grid_at, [&](int i) { return i >= s0_last; });
-        });
+        };
+        if (c->f32 && c->nstat > 1) {
+            if constexpr (std::is_same<Real, float>::value) rc = forward_segments(c, from);
+        } else {
+            rc = run_graph(c, from, pass);
+        }
         if (rc) return rc;
         c->tape_hold[tl] = c->seg_last;
         c->tape_hold[tl ^ 1] = -1;
@@ -708,18 +1031,25 @@ int end_loss(dmpm_ctx* c, double* loss)
     // 4G-real work grid (8-byte aligned for both precisions)
     double* dLdm = reinterpret_cast<double*>(gm + 2 * c->G);
     fill<Real>(c, gm, c->G, Real(0));
-    timed(c, DMPM_K_LOSS, [&] {
-        end_masses(c, lay, gm);
-        loss_kernels(c, lay, gm, dLdm);
-    });
+    {
+        BatchScope bs(c, layer_seg(c, last));
+        timed(c, DMPM_K_LOSS, [&] {
+            end_masses(c, lay, gm);
+            loss_kernels(c, lay, gm, dLdm);
+        });
+    }
     taint_wgrid(c);
     HIP_TRY(hipGetLastError());
     c->inst_loss.assign(c->B, 0.0);
     HIP_TRY(hipMemcpyAsync(c->inst_loss.data(), c->dacc, c->B * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    unsigned long long ws[3] = {0, 0, 0};
-    if (c->wstats) HIP_TRY(hipMemcpyAsync(ws, c->wstats, sizeof(ws), hipMemcpyDeviceToHost, c->stream));
+    std::vector<unsigned long long> ws(3 * (size_t)c->nstat, 0ull);
+    if (c->wstats) HIP_TRY(hipMemcpyAsync(ws.data(), c->wstats, ws.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    for (int k = 0; k < 3; ++k) c->last_wstats[k] = (long long)ws[k];
+    c->last_swstats.assign(ws.begin(), ws.end());
+    for (int k = 0; k < 3; ++k) {
+        c->last_wstats[k] = 0;
+        for (int i = 0; i < c->nstat; ++i) c->last_wstats[k] += (long long)ws[3 * (size_t)i + k];
+    }
     double h = 0.0;
     for (double v : c->inst_loss) h += v;
     if (loss) *loss = h;
@@ -750,6 +1080,7 @@ int backward_pass(dmpm_ctx* c, int control)
         const int seg_lo = control / c->K;
         const int hold0[2] = {c->tape_hold[0], c->tape_hold[1]};
         const bool ov = c->overlap && !c->prof && c->stream2;
+        prepare_bidx(c);
         rc = run_graph(c, key, [&] {
             // the adjoint grid chain runs across the recomputed segments (they do not touch it)
             int hold[2] = {hold0[0], hold0[1]};
@@ -779,9 +1110,16 @@ int backward_pass(dmpm_ctx* c, int control)
                     hold[sp & 1] = sp;
                     pending[sp & 1] = true;
                 }
-                for (int i = s1 - 1; i >= std::max(s0, control); --i)
-                    clean = step_backward<Real>(c, i, tape_slot<Real>(c, t, i), tape_slot<Real>(c, t, i + 1), tape_grid<Real>(c, t, i),
-                                                clean, i > control);
+                // a boundary layer in another order: its gradients into this
+                // segment's order, the layer itself from the tape's boundary slot
+                const bool perm_b = s1 < c->L - 1 && boundary_permuted(c, seg + 1);
+                if constexpr (std::is_same<Real, float>::value)
+                    if (perm_b) grads_to_prev(c, grads_of<float>(c, s1), bidx_of(c, seg + 1));
+                BatchScope bs(c, seg);
+                for (int i = s1 - 1; i >= std::max(s0, control); --i) {
+                    const Real* nxt = (i + 1 == s1 && perm_b) ? slot<Real>(c, c->tape_base[t] + c->K) : tape_slot<Real>(c, t, i + 1);
+                    clean = step_backward<Real>(c, i, tape_slot<Real>(c, t, i), nxt, tape_grid<Real>(c, t, i), clean, i > control);
+                }
             }
         });
         if (rc) return rc;
@@ -853,18 +1191,13 @@ int download_positions(dmpm_ctx* c, const Real* layer, double* x)
 // 256-thread workgroup, so range_max <= 512).
 int build_ranges(dmpm_ctx* c)
 {
-    std::vector<int4> rg;
-    for (const int4& b : c->host_blocks) {
-        const int len = b.y - b.x;
-        const int parts = (len + c->range_max - 1) / c->range_max;
-        for (int k = 0; k < parts; ++k)
-            rg.push_back(make_int4(b.x + (int)((long long)len * k / parts), b.x + (int)((long long)len * (k + 1) / parts), b.z, 0));
-    }
-    // longest ranges first: workgroups start in range order, so the short ones
-    // fill the last round of a launch instead of trailing behind long ones
-    if (c->range_order == 1)
-        std::stable_sort(rg.begin(), rg.end(), [](const int4& a, const int4& b) { return a.y - a.x > b.y - b.x; });
+    std::vector<int4> rg = make_ranges(c, c->host_blocks);
     clear_graphs(c);
+    for (auto& o : c->sord)
+        if (o.own) {
+            const int rc = upload_seg_ranges(c, o);
+            if (rc) return rc;
+        }
     if (c->ranges) {
         hipFree(c->ranges);
         c->ranges = nullptr;
@@ -897,35 +1230,11 @@ int resort(dmpm_ctx* c, bool controls_live)
 {
     const int N = c->N;
     const size_t S = c->S;
-    const long long nblocks = c->boff[c->B];
-    auto* k0 = static_cast<unsigned long long*>(c->sortbuf);
-    if (!c->sortbuf && N > 0) {
-        c->sort_temp = sort_temp_bytes(N, nblocks);
-        if (c->sort_temp == 0) return fail(DMPM_ERR_HIP, "rocPRIM sort: temporary storage query failed");
-        const int rc = dalloc(c, &c->sortbuf, (size_t)N * (2 * 8 + 4 * 4) + 256 + c->sort_temp);
-        if (rc) return rc;
-        k0 = static_cast<unsigned long long*>(c->sortbuf);
-    }
     std::vector<int> blk(N), api(N);
     if (N > 0) {
-        SortArgs a{};
-        a.sps = c->d_sps;
-        a.poff = c->d_poff;
-        a.boff = c->d_boff;
-        a.B = c->B;
-        a.nblocks = nblocks;
-        a.x = slot<float>(c, 0);
-        a.S = (long long)S;
-        a.N = N;
-        a.k0 = k0;
-        a.k1 = k0 + N;
-        a.v0 = reinterpret_cast<int*>(a.k1 + N);
-        a.v1 = a.v0 + N;
-        a.head = a.v1 + N;
-        a.blk = a.head + N;
-        a.temp = reinterpret_cast<char*>(c->sortbuf) + (((size_t)N * (2 * 8 + 4 * 4) + 255) & ~(size_t)255);
-        a.temp_bytes = c->sort_temp;
-        HIP_TRY(launch_sort_order(a, c->stream));
+        SortArgs a;
+        const int src = run_sort(c, slot<float>(c, 0), a);
+        if (src) return src;
         const int* perm = a.v1;
         // every per-particle array through tmp (12 fields at a time) in the new order
         float* t = static_cast<float*>(c->tmp);
@@ -955,19 +1264,14 @@ int resort(dmpm_ctx* c, bool controls_live)
     }
     c->pos.assign(N, 0);
     for (int d = 0; d < N; ++d) c->pos[api[d]] = d;
-    c->host_blocks.clear();
-    int in = 0;
-    for (int d0 = 0, d = 1; d <= N; ++d) {
-        if (d == N || blk[d] != blk[d0]) {
-            while (in + 1 < c->B && c->boff[in + 1] <= blk[d0]) ++in;
-            c->host_blocks.push_back(make_int4(d0, d, in, 0));
-            d0 = d;
-        }
-    }
+    c->host_blocks = blocks_from(c, blk);
+    // the segment orders map to the old home slots
+    drop_seg_orders(c);
     const int rc = build_ranges(c);
     if (rc) return rc;
     ++c->n_sorts;
     for (long long& w : c->last_wstats) w = 0;
+    std::fill(c->last_swstats.begin(), c->last_swstats.end(), 0LL);
     c->fwd_valid = std::min(c->fwd_valid, 0);
     c->loss_valid = false;
     c->grads_lo = -1;
@@ -1149,12 +1453,28 @@ int grid_upload(dmpm_ctx* c, Real* dst, long long count, const double* src)
     return DMPM_OK;
 }
 
+// A layer (or gradient layer) l held in segment s's order, gathered into the
+// home order in a rolling slot (scratch between passes) other than l; l itself
+// when segment s uses the home order.
+template <typename Real>
+Real* home_copy(dmpm_ctx* c, Real* l, int s)
+{
+    if constexpr (std::is_same<Real, float>::value) {
+        if (!seg_own(c, s)) return l;
+        float* st = slot<float>(c, c->roll_base) == l ? slot<float>(c, c->roll_base + 1) : slot<float>(c, c->roll_base);
+        launch_gather<float>(l, st, (long long)c->S, c->N, kLayerFields, c->sord[s].inv, c->stream);
+        return st;
+    }
+    return l;
+}
+
 template <typename Real>
 int get_layer_state(dmpm_ctx* c, int layer, double* x, double* v, double* F, double* C)
 {
     if (layer < 0 || layer >= c->L) return fail(DMPM_ERR_INVALID, "layer out of range");
     if (layer > c->fwd_valid) return fail(DMPM_ERR_STATE, "layer not computed by a forward pass");
     Real* l = (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) ? slot<Real>(c, layer) : materialize<Real>(c, layer);
+    l = home_copy<Real>(c, l, layer_seg(c, layer));
     int rc;
     if (x && (rc = download_positions<Real>(c, l, x))) return rc;
     if (v && (rc = download_fields<Real>(c, l, FV, 3, v, c->N))) return rc;
@@ -1171,7 +1491,7 @@ int get_point_diagnostics(dmpm_ctx* c, int layer, double* J, double* psi)
 {
     if (layer < 0 || layer >= c->L) return fail(DMPM_ERR_INVALID, "layer out of range");
     if (layer > c->fwd_valid) return fail(DMPM_ERR_STATE, "layer not computed by a forward pass");
-    const Real* l = materialize<Real>(c, layer);
+    const Real* l = home_copy<Real>(c, materialize<Real>(c, layer), layer_seg(c, layer));
     const Real* ctrl = layer < c->L - 1 ? ctrl_of<Real>(c, layer) : nullptr;
     double* out = static_cast<double*>(c->tmp);
     launch_point_diag<Real>(c->S, l, ctrl, static_cast<const Real*>(c->cst), out, c->stream);
@@ -1209,7 +1529,8 @@ int get_layer_grads(dmpm_ctx* c, int layer, double* gx, double* gv, double* gF, 
     if (!seed && (c->grads_lo < 0 || layer < c->grads_lo || layer > std::min(c->grads_lo + 1, c->L - 1)))
         return fail(DMPM_ERR_STATE,
                     "gradients of this layer are not held (only the last layer, the lowest layer of the last pass and the one above)");
-    Real* g = grads_of<Real>(c, layer);
+    // held in the order of the lowest layer's segment (the last layer's: the last segment's)
+    Real* g = home_copy<Real>(c, grads_of<Real>(c, layer), layer == c->L - 1 ? c->seg_last : layer_seg(c, c->grads_lo));
     int rc;
     if (gx && (rc = download_fields<Real>(c, g, FX, 3, gx, c->N))) return rc;
     if (gv && (rc = download_fields<Real>(c, g, FV, 3, gv, c->N))) return rc;
@@ -1223,13 +1544,20 @@ int set_layer_grads(dmpm_ctx* c, int layer, const double* gx, const double* gv, 
 {
     if (layer < 0 || layer >= c->L) return fail(DMPM_ERR_INVALID, "layer out of range");
     if (layer > c->fwd_valid) return fail(DMPM_ERR_STATE, "layer not computed by a forward pass");
-    Real* g = grads_of<Real>(c, layer);
+    Real* dst = grads_of<Real>(c, layer);
+    // in the order of the layer's segment: through a rolling slot in the home order
+    const int sg = layer_seg(c, layer);
+    Real* g = dst;
+    if constexpr (std::is_same<Real, float>::value)
+        if (seg_own(c, sg)) g = slot<float>(c, c->roll_base);
     fill<Real>(c, g, (long long)kLayerFields * c->S, Real(0));
     int rc;
     if (gx && (rc = upload_fields<Real>(c, g, FX, 3, gx, c->N))) return rc;
     if (gv && (rc = upload_fields<Real>(c, g, FV, 3, gv, c->N))) return rc;
     if (gF && (rc = upload_fields<Real>(c, g, FF, 9, gF, c->N))) return rc;
     if (gC && (rc = upload_fields<Real>(c, g, FC, 9, gC, c->N))) return rc;
+    if constexpr (std::is_same<Real, float>::value)
+        if (g != dst) launch_gather<float>(g, dst, (long long)c->S, c->N, kLayerFields, c->sord[sg].home, c->stream);
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->grads_lo = layer;
     c->bwd_control = -1;
@@ -1260,7 +1588,18 @@ int chain_episode(dmpm_ctx* c)
 {
     if (c->fwd_valid < c->L - 1) return fail(DMPM_ERR_STATE, "chain before the forward pass reached the last layer");
     Real* last = (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) ? slot<Real>(c, c->L - 1) : materialize<Real>(c, c->L - 1);
-    HIP_TRY(hipMemcpyAsync(slot<Real>(c, 0), last, (size_t)kLayerFields * c->S * sizeof(Real), hipMemcpyDeviceToDevice, c->stream));
+    bool own_any = false;
+    if constexpr (std::is_same<Real, float>::value) {
+        for (int s = 1; s <= c->seg_last && s < (int)c->sord.size(); ++s) own_any = own_any || seg_own(c, s);
+        if (seg_own(c, c->seg_last))  // back to the home order
+            launch_gather<float>(last, slot<float>(c, 0), (long long)c->S, c->N, kLayerFields, c->sord[c->seg_last].inv, c->stream);
+        else
+            HIP_TRY(hipMemcpyAsync(slot<Real>(c, 0), last, (size_t)kLayerFields * c->S * sizeof(Real), hipMemcpyDeviceToDevice, c->stream));
+        // the segment orders were those of the episode that ended
+        drop_seg_orders(c);
+    } else {
+        HIP_TRY(hipMemcpyAsync(slot<Real>(c, 0), last, (size_t)kLayerFields * c->S * sizeof(Real), hipMemcpyDeviceToDevice, c->stream));
+    }
     fill<Real>(c, static_cast<Real*>(c->ctrl), (long long)(c->L - 1) * 9 * c->S, Real(0));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->fwd_valid = 0;
@@ -1269,8 +1608,9 @@ int chain_episode(dmpm_ctx* c)
     c->bwd_control = -1;
     drop_tapes(c);
     // the particles moved over the episode: order the new layer 0 afresh
-    // (mode 1), or when the episode's P2G windows showed a stale order (mode 2)
-    if (c->f32 && (c->resort_mode == 1 || (c->resort_mode == 2 && order_stale(c)))) return resort(c, false);
+    // (mode 1), or when the episode's P2G windows showed a stale order or a
+    // segment needed an order of its own (mode 2)
+    if (c->f32 && (c->resort_mode == 1 || (c->resort_mode == 2 && (order_stale(c) || own_any)))) return resort(c, false);
     return DMPM_OK;
 }
 
@@ -1396,7 +1736,9 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         c->K = K;
         c->n_ckpt = (c->L - 1) / K + 1;  // layers 0, K, 2K, ... <= L-1
         c->seg_last = (c->L - 2) / K;
-        // two tapes of K+1 slots (slot 0 unused: the segment start is a checkpoint)
+        // two tapes of K+1 slots (slot 0 unused: the segment start is a
+        // checkpoint; slot K: the segment's end layer when the next segment
+        // holds its checkpoint in another particle order)
         c->tape_base[0] = c->n_ckpt;
         c->tape_base[1] = c->n_ckpt + K + 1;
         c->roll_base = c->tape_base[1] + K + 1;
@@ -1405,6 +1747,10 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         c->K = steps;
         c->n_slots = c->L;
     }
+    // FP32 checkpoint tape: window statistics and particle orders per segment
+    c->nstat = (c->f32 && cfg->tape_mode == DMPM_TAPE_CHECKPOINT) ? c->seg_last + 1 : 1;
+    c->sord.resize(c->nstat > 1 ? (size_t)c->nstat : 0);
+    c->last_swstats.assign(3 * (size_t)c->nstat, 0);
     const size_t e = c->esz;
     const size_t S = c->S;
     const size_t G = c->G;
@@ -1445,7 +1791,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         (c->f32 && (rc = dalloc(c, (void**)&c->d_api, S * sizeof(int)))) ||
         (c->f32 && (rc = dalloc(c, (void**)&c->d_poff, (c->B + 1) * sizeof(int)))) ||
         (c->f32 && (rc = dalloc(c, (void**)&c->d_boff, (c->B + 1) * sizeof(long long)))) ||
-        (c->f32 && (rc = dalloc(c, (void**)&c->wstats, 3 * sizeof(unsigned long long))))) {
+        (c->f32 && (rc = dalloc(c, (void**)&c->wstats, 3 * (size_t)c->nstat * sizeof(unsigned long long))))) {
         std::string msg = g_err;
         dmpm_destroy(c);
         return fail(rc, msg);
@@ -1483,7 +1829,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         if (c->f32) {
             e3 = hipMemcpyAsync(c->d_poff, c->poff.data(), c->poff.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
             e4 = hipMemcpyAsync(c->d_boff, c->boff.data(), c->boff.size() * sizeof(long long), hipMemcpyHostToDevice, c->stream);
-            launch_fill<double>(reinterpret_cast<double*>(c->wstats), 3, 0.0, c->stream);
+            launch_fill<double>(reinterpret_cast<double*>(c->wstats), 3LL * c->nstat, 0.0, c->stream);
         }
         he = hipStreamSynchronize(c->stream);
         if (he == hipSuccess) he = e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : (e3 != hipSuccess ? e3 : e4));
@@ -1546,6 +1892,9 @@ int dmpm_destroy(dmpm_ctx* c)
         if (b) hipFree(b);
     for (void* b : c->opt)
         if (b) hipFree(b);
+    for (auto& o : c->sord)
+        for (void* b : {(void*)o.home, (void*)o.inv, (void*)o.bidx, (void*)o.ranges})
+            if (b) hipFree(b);
     for (hipEvent_t ev : c->pool) hipEventDestroy(ev);
     for (hipEvent_t ev : {c->ev_fork, c->ev_join[0], c->ev_join[1]})
         if (ev) hipEventDestroy(ev);
@@ -1879,6 +2228,23 @@ int dmpm_set_option(dmpm_ctx* c, int32_t option, int64_t value)
         c->overlap = value != 0;
         clear_graphs(c);
         return DMPM_OK;
+    case DMPM_OPT_SEG_RESORT:
+        if (value != 0 && value != 1) return fail(DMPM_ERR_INVALID, "DMPM_OPT_SEG_RESORT is 0 or 1");
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->seg_resort = (int)value;
+        if (!value) {
+            bool any = false;
+            for (int sg = 1; sg < (int)c->sord.size(); ++sg) any = any || seg_own(c, sg);
+            if (any) {
+                // layers held in segment orders: recomputed from layer 0
+                drop_seg_orders(c);
+                c->fwd_valid = std::min(c->fwd_valid, 0);
+                c->loss_valid = false;
+                c->grads_lo = -1;
+                drop_tapes(c);
+            }
+        }
+        return DMPM_OK;
     case DMPM_OPT_RANGE_ORDER:
         if (value != 0 && value != 1) return fail(DMPM_ERR_INVALID, "DMPM_OPT_RANGE_ORDER is 0 or 1");
         HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1907,6 +2273,7 @@ int dmpm_get_option(dmpm_ctx* c, int32_t option, int64_t* value)
     case DMPM_OPT_OVERLAP: *value = c->overlap ? 1 : 0; return DMPM_OK;
     case DMPM_OPT_FUSE: *value = c->fuse ? 1 : 0; return DMPM_OK;
     case DMPM_OPT_RANGE_ORDER: *value = c->range_order; return DMPM_OK;
+    case DMPM_OPT_SEG_RESORT: *value = c->seg_resort; return DMPM_OK;
     default: return fail(DMPM_ERR_INVALID, "unknown option");
     }
 }
@@ -1993,6 +2360,16 @@ int dmpm_get_window_stats(dmpm_ctx* c, int64_t* stats)
     if (!c || !stats) return fail(DMPM_ERR_INVALID, "null argument");
     for (int k = 0; k < 3; ++k) stats[k] = c->last_wstats[k];
     stats[3] = c->n_sorts;
+    return DMPM_OK;
+}
+
+int dmpm_get_segment_window_stats(dmpm_ctx* c, int32_t segment, int64_t* stats)
+{
+    if (!c || !stats) return fail(DMPM_ERR_INVALID, "null argument");
+    if (segment < 0 || segment >= c->nstat) return fail(DMPM_ERR_INVALID, "segment out of range");
+    for (int k = 0; k < 3; ++k) stats[k] = c->last_swstats[3 * (size_t)segment + k];
+    stats[3] = seg_own(c, segment) ? 1 : 0;
+    stats[4] = c->n_seg_sorts;
     return DMPM_OK;
 }
 

@@ -102,6 +102,23 @@ __global__ __launch_bounds__(kSortBlock) void k_gather(const T* src, T* dst, lon
     for (int f = 0; f < nf; ++f) dst[(long long)f * S + d] = src[(long long)f * S + s];
 }
 
+// dst[f S + idx[d]] = src[f S + d] for nf fields
+template <typename T>
+__global__ __launch_bounds__(kSortBlock) void k_scatter(const T* src, T* dst, long long S, int N, int nf, const int* idx)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= N) return;
+    const long long t = idx[d];
+    for (int f = 0; f < nf; ++f) dst[(long long)f * S + t] = src[(long long)f * S + d];
+}
+
+// inv[a[d]] = d (a: a permutation of [0, N))
+__global__ __launch_bounds__(kSortBlock) void k_invert(const int* a, int* inv, int N)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < N) inv[a[d]] = d;
+}
+
 __global__ __launch_bounds__(kSortBlock) void k_iota(int* a, int N)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
@@ -159,6 +176,20 @@ void launch_gather(const T* src, T* dst, long long S, int N, int nf, const int* 
 }
 template void launch_gather<float>(const float*, float*, long long, int, int, const int*, hipStream_t);
 template void launch_gather<int>(const int*, int*, long long, int, int, const int*, hipStream_t);
+
+template <typename T>
+void launch_scatter(const T* src, T* dst, long long S, int N, int nf, const int* idx, hipStream_t s)
+{
+    if (N <= 0 || nf <= 0) return;
+    hipLaunchKernelGGL(k_scatter<T>, dim3(sort_blocks(N)), dim3(kSortBlock), 0, s, src, dst, S, N, nf, idx);
+}
+template void launch_scatter<float>(const float*, float*, long long, int, int, const int*, hipStream_t);
+
+void launch_invert(const int* a, int* inv, int N, hipStream_t s)
+{
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_invert, dim3(sort_blocks(N)), dim3(kSortBlock), 0, s, a, inv, N);
+}
 
 void launch_iota(int* a, int N, hipStream_t s)
 {

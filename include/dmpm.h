@@ -204,6 +204,9 @@ int dmpm_chain_episode(dmpm_ctx* ctx);
 #define DMPM_OPT_FUSE 7      /* FP32: 1 (default) runs the G2P of each step and the P2G of the next as one launch
                               * inside a pass; 0: separate launches (the same results) */
 #define DMPM_OPT_RANGE_ORDER 8 /* FP32: launch order of the block ranges: 1 (default) longest first, 0 block order */
+#define DMPM_OPT_SEG_RESORT 9  /* FP32 checkpoint tape: 1 (default) orders a segment's particles afresh from its first
+                                * layer when its P2G windows went stale (DMPM_OPT_RESORT_PERMILLE) in the last forward
+                                * pass that reached the loss, and keeps that order; 0: every segment in layer 0's order */
 int dmpm_set_option(dmpm_ctx* ctx, int32_t option, int64_t value);
 int dmpm_get_option(dmpm_ctx* ctx, int32_t option, int64_t* value);
 /* FP32: time a forward and a backward step of layer 0 for every candidate range size and LDS
@@ -221,6 +224,11 @@ int dmpm_sort_particles(dmpm_ctx* ctx);
 /* FP32: stats[0..2] = P2G workgroups on the fixed window, the general LDS window and direct HBM atomics
  * over the last forward pass that reached dmpm_end_layer_mass_loss; stats[3] = device sorts so far. */
 int dmpm_get_window_stats(dmpm_ctx* ctx, int64_t* stats);
+/* FP32 checkpoint tape (no reference counterpart): stats[0..2] as dmpm_get_window_stats for the steps of one
+ * checkpoint segment (segments 0 .. ceil((num_layers - 1) / stride) - 1; other contexts have segment 0 only);
+ * stats[3] = 1 when the segment holds its particles in an order of its own (DMPM_OPT_SEG_RESORT);
+ * stats[4] = segment sorts so far. */
+int dmpm_get_segment_window_stats(dmpm_ctx* ctx, int32_t segment, int64_t* stats);
 
 int dmpm_synchronize(dmpm_ctx* ctx);
 /* Device bytes held by the context. */

@@ -101,6 +101,9 @@ hipError_t launch_sort_order(const SortArgs&, hipStream_t)
 template <typename T> void launch_gather(const T*, T*, long long, int, int, const int*, hipStream_t) { fp32_only("gather"); }
 template void launch_gather<float>(const float*, float*, long long, int, int, const int*, hipStream_t);
 template void launch_gather<int>(const int*, int*, long long, int, int, const int*, hipStream_t);
+template <typename T> void launch_scatter(const T*, T*, long long, int, int, const int*, hipStream_t) { fp32_only("scatter"); }
+template void launch_scatter<float>(const float*, float*, long long, int, int, const int*, hipStream_t);
+void launch_invert(const int*, int*, int, hipStream_t) { fp32_only("invert"); }
 void launch_iota(int*, int, hipStream_t) { fp32_only("iota"); }
 
 }  // namespace dmpm
