@@ -615,6 +615,46 @@ __device__ __forceinline__ void appended_clear(const Batch& bt, const TileClear&
                 (nb * blockDim.x) >> 6);
 }
 
+// The ranges of a launch.  Without a queue (bt.queue null) workgroup b takes
+// range b and the workgroups past the last range clear tc.  With one the
+// launch is persistent: its workgroups (as many as fit on the chip at once)
+// take the next range from the counter queue[0] until none is left, then
+// share the clear of tc; the last workgroup to finish (counter queue[1])
+// resets both counters for the next launch.
+// The two forms are separate instantiations (PERSIST), so that the loop of
+// the persistent one leaves the register allocation of the other unchanged.
+template <bool PERSIST, typename Body>
+__device__ __forceinline__ void for_ranges(const Batch& bt, const TileClear& tc, Body&& body)
+{
+    if constexpr (!PERSIST) {
+        if (blockIdx.x >= (unsigned)bt.nranges) {
+            appended_clear(bt, tc, (unsigned)bt.nranges);
+            return;
+        }
+        body(bt.ranges[blockIdx.x]);
+        return;
+    }
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(bt.queue, 1);
+        __syncthreads();
+        const int r = next;
+        __syncthreads();  // every thread holds r before thread 0 takes the next range
+        if (r >= bt.nranges) break;
+        body(bt.ranges[r]);
+    }
+    if (tc.grid)
+        clear_tiles(bt, tc.grid, tc.flags, num_tiles(bt.Gtot), ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6,
+                    ((long long)gridDim.x * blockDim.x) >> 6);
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(bt.queue + 1, 1) == (int)gridDim.x - 1) {
+            atomicExch(bt.queue, 0);
+            atomicExch(bt.queue + 1, 0);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ P2G ----
 // One particle's P2G inputs: position, m (1 - dt drag) v, m, and the affine
 // matrix A = -(3/dx^2) dt vol P(Fc) Fc^T + m C (FP64 polar and PK1), which is
@@ -774,14 +814,8 @@ __device__ __forceinline__ void p2g_scatter(const Batch& bt, const StepParams& s
 }
 
 // tc: a grid to clear in appended workgroups (the next step's), or none
-__global__ __launch_bounds__(kP2G, 4) void k_p2g_blk(Batch bt, StepIO<float> io, TileClear tc)
+__device__ __forceinline__ void p2g_range(const Batch& bt, StepIO<float> io, const int4 rg, unsigned long long* lds64)
 {
-    if (blockIdx.x >= (unsigned)bt.nranges) {
-        appended_clear(bt, tc, (unsigned)bt.nranges);
-        return;
-    }
-    extern __shared__ unsigned long long lds64[];
-    const int4 rg = bt.ranges[blockIdx.x];
     const StepParams sp = bt.sps[rg.z];
     const long long nofs = bt.nofs[rg.z];
     io.grid += 4 * nofs;
@@ -799,6 +833,13 @@ __global__ __launch_bounds__(kP2G, 4) void k_p2g_blk(Batch bt, StepIO<float> io,
         box_add(sp, q1.x, box);
     }
     p2g_scatter(bt, sp, io, nofs, rg.y - rg.x, q0, q1, v0, v1, bound, box, lds64);
+}
+
+template <bool PERSIST>
+__global__ __launch_bounds__(kP2G, 4) void k_p2g_blk(Batch bt, StepIO<float> io, TileClear tc)
+{
+    extern __shared__ unsigned long long lds64[];
+    for_ranges<PERSIST>(bt, tc, [&](const int4 rg) { p2g_range(bt, io, rg, lds64); });
 }
 
 // --------------------------------------------------- P2G of mass only ----
@@ -962,14 +1003,8 @@ __device__ __forceinline__ Win g2p_window(const Batch& bt, const StepParams& sp,
     return W;
 }
 
-__global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io, TileClear tc)
+__device__ __forceinline__ void g2p_range(const Batch& bt, StepIO<float> io, const int4 rg, float* lds)
 {
-    if (blockIdx.x >= (unsigned)bt.nranges) {
-        appended_clear(bt, tc, (unsigned)bt.nranges);
-        return;
-    }
-    extern __shared__ float lds[];
-    const int4 rg = bt.ranges[blockIdx.x];
     const StepParams sp = bt.sps[rg.z];
     io.grid += 4 * bt.nofs[rg.z];
     const NodeVel ld(sp, io.grid);
@@ -989,6 +1024,13 @@ __global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io, Ti
     }
 }
 
+template <bool PERSIST>
+__global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io, TileClear tc)
+{
+    extern __shared__ float lds[];
+    for_ranges<PERSIST>(bt, tc, [&](const int4 rg) { g2p_range(bt, io, rg, lds); });
+}
+
 // ------------------------------------------------------- fused step ------
 // G2P of step i - 1 (gio: layer i - 1 -> layer i, grid of step i - 1) and P2G
 // of step i (pio: dFc_i, grid of step i, A cached in layer i when pio.aux is
@@ -999,14 +1041,9 @@ __global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io, Ti
 // results equal those of k_g2p_blk then k_p2g_blk: the state written is the
 // state scattered, and the scatter sums are independent of their order.
 static_assert(kP2G == kGat, "the fused step maps particles to threads as both kernels do");
-__global__ __launch_bounds__(kP2G, 4) void k_step_blk(Batch bt, StepIO<float> gio, StepIO<float> pio, TileClear tc)
+__device__ __forceinline__ void step_range(const Batch& bt, StepIO<float> gio, StepIO<float> pio, const int4 rg,
+                                           unsigned long long* lds64)
 {
-    if (blockIdx.x >= (unsigned)bt.nranges) {
-        appended_clear(bt, tc, (unsigned)bt.nranges);
-        return;
-    }
-    extern __shared__ unsigned long long lds64[];
-    const int4 rg = bt.ranges[blockIdx.x];
     const StepParams sp = bt.sps[rg.z];
     const long long nofs = bt.nofs[rg.z];
     gio.grid += 4 * nofs;
@@ -1045,6 +1082,13 @@ __global__ __launch_bounds__(kP2G, 4) void k_step_blk(Batch bt, StepIO<float> gi
     if (v1) one(p1, xs[1], q1);
     // p2g_scatter's first barrier orders every gather above before the LDS is reused
     p2g_scatter(bt, sp, pio, nofs, rg.y - rg.x, q0, q1, v0, v1, bound, box, lds64);
+}
+
+template <bool PERSIST>
+__global__ __launch_bounds__(kP2G, 4) void k_step_blk(Batch bt, StepIO<float> gio, StepIO<float> pio, TileClear tc)
+{
+    extern __shared__ unsigned long long lds64[];
+    for_ranges<PERSIST>(bt, tc, [&](const int4 rg) { step_range(bt, gio, pio, rg, lds64); });
 }
 
 // -------------------------------------------------------- transposed G2P ---
@@ -1101,10 +1145,8 @@ __device__ __forceinline__ void g2p_adj_scale(G2PAdjPart& q, const Scales<3>& SC
 
 constexpr int kG2PAdj = 256;
 
-__global__ __launch_bounds__(kG2PAdj, 4) void k_g2p_adj_blk(Batch bt, AdjIO<float> io)
+__device__ __forceinline__ void g2p_adj_range(const Batch& bt, AdjIO<float> io, const int4 rg, unsigned long long* lds64)
 {
-    extern __shared__ unsigned long long lds64[];
-    const int4 rg = bt.ranges[blockIdx.x];
     const StepParams sp = bt.sps[rg.z];
     const long long nofs = bt.nofs[rg.z];
     io.agrid += 4 * nofs;
@@ -1187,6 +1229,13 @@ __global__ __launch_bounds__(kG2PAdj, 4) void k_g2p_adj_blk(Batch bt, AdjIO<floa
         __syncthreads();
         flush_fixed<3, false, true>(sp, W, lds64, SC, io.agrid, sp.G, gate, io.aflags, nofs);
     }
+}
+
+template <bool PERSIST>
+__global__ __launch_bounds__(kG2PAdj, 4) void k_g2p_adj_blk(Batch bt, AdjIO<float> io)
+{
+    extern __shared__ unsigned long long lds64[];
+    for_ranges<PERSIST>(bt, TileClear{}, [&](const int4 rg) { g2p_adj_range(bt, io, rg, lds64); });
 }
 
 // Elastic half of P2G^T + P_op_1^T for particle p (FP64 polar, PK1, H:dF):
@@ -1283,10 +1332,8 @@ __device__ __forceinline__ void elastic_adj(const StepParams& sp, const AdjIO<fl
 // then the elastic part (FP64 polar, PK1 and H:dF):
 //   Pbar = -k Gb Fc,  Fcbar = (1-s)(I + dt C')^T F' - k Gb^T P + H(Fc):Pbar.
 template <bool FUSE>
-__global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
+__device__ __forceinline__ void p2g_adj_range(const Batch& bt, AdjIO<float> io, const int4 rg, float* lds)
 {
-    extern __shared__ float lds[];
-    const int4 rg = bt.ranges[blockIdx.x];
     const StepParams sp = bt.sps[rg.z];
     io.grid += 4 * bt.nofs[rg.z];
     io.agrid += 4 * bt.nofs[rg.z];
@@ -1409,17 +1456,25 @@ __global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
     }
 }
 
-// Elastic half as a kernel of its own (Gb from tmp), one thread per particle of a range.
-__global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io, TileClear tc)
+template <bool FUSE, bool PERSIST>
+__global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
 {
-    if (blockIdx.x >= (unsigned)bt.nranges) {
-        appended_clear(bt, tc, (unsigned)bt.nranges);
-        return;
-    }
-    const int4 rg = bt.ranges[blockIdx.x];
+    extern __shared__ float lds[];
+    for_ranges<PERSIST>(bt, TileClear{}, [&](const int4 rg) { p2g_adj_range<FUSE>(bt, io, rg, lds); });
+}
+
+// Elastic half as a kernel of its own (Gb from tmp), one thread per particle of a range.
+__device__ __forceinline__ void elastic_range(const Batch& bt, const AdjIO<float>& io, const int4 rg)
+{
     const StepParams sp = bt.sps[rg.z];
     const int S = sp.stride;
     for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) elastic_adj(sp, io, p, home_of(bt, p), ldm(io.tmp, 0, S, p));
+}
+
+template <bool PERSIST>
+__global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io, TileClear tc)
+{
+    for_ranges<PERSIST>(bt, tc, [&](const int4 rg) { elastic_range(bt, io, rg); });
 }
 
 // --------------------------------------------------- batched grid kernels --
@@ -1532,14 +1587,42 @@ static unsigned clear_blocks(const Batch& b, const TileClear& tc)
     return (unsigned)nb;
 }
 
+// The grid of a range kernel and the batch it takes: one workgroup per range
+// plus the appended clear workgroups, or, with a queue, a persistent launch of
+// as many workgroups as fit on the chip at once (at most one per range) on the
+// counters of the kernel's class (cls 0: forward, 1: adjoint; a forward
+// recompute on the second stream overlaps only adjoint kernels).
+template <typename Kern>
+static dim3 range_grid(const Batch& b, Batch& bq, const TileClear& tc, Kern kernel, int block, size_t lds, int cls)
+{
+    bq = b;
+    if (!b.queue) return dim3(b.nranges + clear_blocks(b, tc));
+    bq.queue = b.queue + 2 * cls;
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus < 1)
+        cus = 1;
+    long long g = (long long)per_cu * cus;
+    if (g > b.nranges) g = b.nranges;
+    return dim3((unsigned)(g < 1 ? 1 : g));
+}
+
 void launch_p2g_blk(const Batch& b, const StepIO<float>& io, const TileClear& tc, hipStream_t s)
 {
     if (b.nranges <= 0) {
         if (tc.grid) launch_clear_tiles(b, tc.grid, tc.flags, s);
         return;
     }
-    hipLaunchKernelGGL(k_p2g_blk, dim3(b.nranges + clear_blocks(b, tc)), dim3(kP2G), 4 * win_nodes(b.cap) * sizeof(unsigned long long),
-                       s, b, io, tc);
+    const size_t lds = 4 * win_nodes(b.cap) * sizeof(unsigned long long);
+    Batch bq;
+    if (b.queue) {
+        const dim3 g = range_grid(b, bq, tc, k_p2g_blk<true>, kP2G, lds, 0);
+        hipLaunchKernelGGL(k_p2g_blk<true>, g, dim3(kP2G), lds, s, bq, io, tc);
+    } else {
+        const dim3 g = range_grid(b, bq, tc, k_p2g_blk<false>, kP2G, lds, 0);
+        hipLaunchKernelGGL(k_p2g_blk<false>, g, dim3(kP2G), lds, s, bq, io, tc);
+    }
 }
 
 void launch_step_blk(const Batch& b, const StepIO<float>& gio, const StepIO<float>& pio, const TileClear& tc, hipStream_t s)
@@ -1549,8 +1632,15 @@ void launch_step_blk(const Batch& b, const StepIO<float>& gio, const StepIO<floa
         return;
     }
     // the gather's window (3 float channels) and then the scatter's (4 64-bit words) share the LDS
-    hipLaunchKernelGGL(k_step_blk, dim3(b.nranges + clear_blocks(b, tc)), dim3(kP2G),
-                       4 * win_nodes(b.cap) * sizeof(unsigned long long), s, b, gio, pio, tc);
+    const size_t lds = 4 * win_nodes(b.cap) * sizeof(unsigned long long);
+    Batch bq;
+    if (b.queue) {
+        const dim3 g = range_grid(b, bq, tc, k_step_blk<true>, kP2G, lds, 0);
+        hipLaunchKernelGGL(k_step_blk<true>, g, dim3(kP2G), lds, s, bq, gio, pio, tc);
+    } else {
+        const dim3 g = range_grid(b, bq, tc, k_step_blk<false>, kP2G, lds, 0);
+        hipLaunchKernelGGL(k_step_blk<false>, g, dim3(kP2G), lds, s, bq, gio, pio, tc);
+    }
 }
 
 void launch_g2p_blk(const Batch& b, const StepIO<float>& io, const TileClear& tc, hipStream_t s)
@@ -1559,14 +1649,29 @@ void launch_g2p_blk(const Batch& b, const StepIO<float>& io, const TileClear& tc
         if (tc.grid) launch_clear_tiles(b, tc.grid, tc.flags, s);
         return;
     }
-    hipLaunchKernelGGL(k_g2p_blk, dim3(b.nranges + clear_blocks(b, tc)), dim3(kGat), 3 * win_nodes(b.cap) * sizeof(float), s, b, io,
-                       tc);
+    const size_t lds = 3 * win_nodes(b.cap) * sizeof(float);
+    Batch bq;
+    if (b.queue) {
+        const dim3 g = range_grid(b, bq, tc, k_g2p_blk<true>, kGat, lds, 0);
+        hipLaunchKernelGGL(k_g2p_blk<true>, g, dim3(kGat), lds, s, bq, io, tc);
+    } else {
+        const dim3 g = range_grid(b, bq, tc, k_g2p_blk<false>, kGat, lds, 0);
+        hipLaunchKernelGGL(k_g2p_blk<false>, g, dim3(kGat), lds, s, bq, io, tc);
+    }
 }
 
 void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
 {
     if (b.nranges <= 0) return;
-    hipLaunchKernelGGL(k_g2p_adj_blk, dim3(b.nranges), dim3(kG2PAdj), 2 * win_nodes(b.cap) * sizeof(unsigned long long), s, b, io);
+    const size_t lds = 2 * win_nodes(b.cap) * sizeof(unsigned long long);
+    Batch bq;
+    if (b.queue) {
+        const dim3 g = range_grid(b, bq, TileClear{}, k_g2p_adj_blk<true>, kG2PAdj, lds, 1);
+        hipLaunchKernelGGL(k_g2p_adj_blk<true>, g, dim3(kG2PAdj), lds, s, bq, io);
+    } else {
+        const dim3 g = range_grid(b, bq, TileClear{}, k_g2p_adj_blk<false>, kG2PAdj, lds, 1);
+        hipLaunchKernelGGL(k_g2p_adj_blk<false>, g, dim3(kG2PAdj), lds, s, bq, io);
+    }
 }
 
 void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear& tc, hipStream_t s)
@@ -1575,12 +1680,21 @@ void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear&
         if (tc.grid) launch_clear_tiles(b, tc.grid, tc.flags, s);
         return;
     }
+    const size_t lds = 7 * win_nodes(b.cap) * sizeof(float);
+    Batch bq;
     if (kFuseElastic) {
-        hipLaunchKernelGGL(k_p2g_adj_blk<true>, dim3(b.nranges), dim3(kGat), 7 * win_nodes(b.cap) * sizeof(float), s, b, io);
+        const dim3 g = range_grid(b, bq, TileClear{}, k_p2g_adj_blk<true, false>, kGat, lds, 1);
+        hipLaunchKernelGGL((k_p2g_adj_blk<true, false>), dim3(b.nranges), dim3(kGat), lds, s, b, io);
+        (void)g;
         if (tc.grid) launch_clear_tiles(b, tc.grid, tc.flags, s);
+    } else if (b.queue) {
+        const dim3 g = range_grid(b, bq, TileClear{}, k_p2g_adj_blk<false, true>, kGat, lds, 1);
+        hipLaunchKernelGGL((k_p2g_adj_blk<false, true>), g, dim3(kGat), lds, s, bq, io);
+        const dim3 ge = range_grid(b, bq, tc, k_elastic_adj<true>, kGat, 0, 1);
+        hipLaunchKernelGGL(k_elastic_adj<true>, ge, dim3(kGat), 0, s, bq, io, tc);
     } else {
-        hipLaunchKernelGGL(k_p2g_adj_blk<false>, dim3(b.nranges), dim3(kGat), 7 * win_nodes(b.cap) * sizeof(float), s, b, io);
-        hipLaunchKernelGGL(k_elastic_adj, dim3(b.nranges + clear_blocks(b, tc)), dim3(kGat), 0, s, b, io, tc);
+        hipLaunchKernelGGL((k_p2g_adj_blk<false, false>), dim3(b.nranges), dim3(kGat), lds, s, b, io);
+        hipLaunchKernelGGL(k_elastic_adj<false>, dim3(b.nranges + clear_blocks(b, tc)), dim3(kGat), 0, s, b, io, tc);
     }
 }
 

@@ -111,6 +111,8 @@ struct Batch {
     unsigned long long* wstats;  // P2G workgroups per window mode (fixed, general, HBM), or null
     const int* home;        // slot of each particle slot in the order of the constants, controls and
                             // control gradients (null: the same order)
+    int* queue;             // persistent launches: 4 zeroed counters (ranges taken, workgroups done) for
+                            // the forward kernels, then for the adjoint kernels; null: a workgroup per range
 };
 // 4-channel grids (m or mbar, then 3 vector channels) of instance i start at 4 * nofs[i];
 // 1-channel arrays (target, dL/dm) at nofs[i].
@@ -156,8 +158,8 @@ void launch_phase_p_op2(int n, const double* x, const double* F, const double* d
                         double dt, double* xn, double* Fn, hipStream_t s);
 void launch_phase_g2g_mass(const StepParams& g1, const StepParams& g2, const double* m1, double* m2, hipStream_t s);
 
-// Device order of the FP32 particles (dmpm_sort.hip): per instance by 4^3-cell
-// block, rank in cell, cell in block.  Buffers of N entries; after
+// Device order of the FP32 particles (dmpm_sort.hip): per instance by T^3-cell
+// block (T = 2^tshift, 4 or 8), rank in cell, cell in block.  Buffers of N entries; after
 // launch_sort_order, v1[d] is the current slot of the particle that goes to
 // slot d and blk[d] its global block (instance i's blocks from boff[i]).
 struct SortArgs {
@@ -165,6 +167,7 @@ struct SortArgs {
     const int* poff;         // slot prefix, B+1 entries (device)
     const long long* boff;   // block prefix, B+1 entries (device)
     int B;
+    int tshift;              // log2 of the block edge in cells (2 or 3)
     long long nblocks;       // boff[B]
     const float* x;          // layer holding the positions (fields FX.., stride S)
     long long S;

@@ -121,6 +121,10 @@ struct dmpm_ctx {
     int* d_poff = nullptr;          // slot prefix per instance (B+1)
     long long* d_boff = nullptr;    // block prefix per instance (B+1)
     std::vector<long long> boff;
+    int tile = 4;                   // block edge in cells (4 or 8): the LDS tile of a workgroup's particles
+    int* queue = nullptr;           // persistent launches: range counters of the forward and adjoint kernels (4)
+    bool persistent = false;
+    long long nblocks4 = 0;         // blocks of every instance at tile 4 (the most of either tile)
     void* sortbuf = nullptr;        // sort keys, values, heads, blocks and rocPRIM scratch (first sort)
     size_t sort_temp = 0;
     // P2G workgroups per window mode (fixed, general, HBM) since the forward
@@ -729,6 +733,17 @@ void recompute_segment(dmpm_ctx* c, int seg)
 }
 
 // ------------------------------------------------------------ block sort ---
+// block prefix of the instances at the current tile (host; the caller uploads it)
+void set_block_prefix(dmpm_ctx* c)
+{
+    const int t = c->tile;
+    c->boff.assign(c->B + 1, 0);
+    for (int i = 0; i < c->B; ++i) {
+        const dmpm_instance& in = c->inst[i];
+        c->boff[i + 1] = c->boff[i] + (long long)((in.dim[0] + t - 1) / t) * ((in.dim[1] + t - 1) / t) * ((in.dim[2] + t - 1) / t);
+    }
+}
+
 // Workgroup ranges of blocks {begin, end, instance}: at most range_max
 // particles each, longest first with range_order 1 (workgroups start in range
 // order, so the short ones fill the last round of a launch).
@@ -773,7 +788,8 @@ int run_sort(dmpm_ctx* c, const float* lay, SortArgs& a)
     const int N = c->N;
     const long long nblocks = c->boff[c->B];
     if (!c->sortbuf) {
-        c->sort_temp = sort_temp_bytes(N, nblocks);
+        // scratch for the widest keys of either tile
+        c->sort_temp = sort_temp_bytes(N, std::max(nblocks, c->nblocks4));
         if (c->sort_temp == 0) return fail(DMPM_ERR_HIP, "rocPRIM sort: temporary storage query failed");
         const int rc = dalloc(c, &c->sortbuf, (size_t)N * (2 * 8 + 4 * 4) + 256 + c->sort_temp);
         if (rc) return rc;
@@ -784,6 +800,7 @@ int run_sort(dmpm_ctx* c, const float* lay, SortArgs& a)
     a.poff = c->d_poff;
     a.boff = c->d_boff;
     a.B = c->B;
+    a.tshift = c->tile == 8 ? 3 : 2;
     a.nblocks = nblocks;
     a.x = lay;
     a.S = (long long)c->S;
@@ -1791,7 +1808,8 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         (c->f32 && (rc = dalloc(c, (void**)&c->d_api, S * sizeof(int)))) ||
         (c->f32 && (rc = dalloc(c, (void**)&c->d_poff, (c->B + 1) * sizeof(int)))) ||
         (c->f32 && (rc = dalloc(c, (void**)&c->d_boff, (c->B + 1) * sizeof(long long)))) ||
-        (c->f32 && (rc = dalloc(c, (void**)&c->wstats, 3 * (size_t)c->nstat * sizeof(unsigned long long))))) {
+        (c->f32 && (rc = dalloc(c, (void**)&c->wstats, 3 * (size_t)c->nstat * sizeof(unsigned long long)))) ||
+        (c->f32 && (rc = dalloc(c, (void**)&c->queue, 4 * sizeof(int))))) {
         std::string msg = g_err;
         dmpm_destroy(c);
         return fail(rc, msg);
@@ -1815,11 +1833,8 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         launch_fill<double>(static_cast<double*>(c->layers), (long long)kLayerFields * S, 0.0, c->stream);
         launch_fill<double>(static_cast<double*>(c->cst), (long long)kConstFields * S, 0.0, c->stream);
     }
-    c->boff.assign(c->B + 1, 0);
-    for (int i = 0; i < c->B; ++i) {
-        const dmpm_instance& in = insts[i];
-        c->boff[i + 1] = c->boff[i] + (long long)((in.dim[0] + 3) / 4) * ((in.dim[1] + 3) / 4) * ((in.dim[2] + 3) / 4);
-    }
+    set_block_prefix(c);
+    c->nblocks4 = c->boff[c->B];
     {
         std::vector<StepParams> sps(c->B);
         for (int i = 0; i < c->B; ++i) sps[i] = inst_params(c, i);
@@ -1830,6 +1845,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
             e3 = hipMemcpyAsync(c->d_poff, c->poff.data(), c->poff.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
             e4 = hipMemcpyAsync(c->d_boff, c->boff.data(), c->boff.size() * sizeof(long long), hipMemcpyHostToDevice, c->stream);
             launch_fill<double>(reinterpret_cast<double*>(c->wstats), 3LL * c->nstat, 0.0, c->stream);
+            launch_fill<float>(reinterpret_cast<float*>(c->queue), 4, 0.f, c->stream);  // the bits of int 0
         }
         he = hipStreamSynchronize(c->stream);
         if (he == hipSuccess) he = e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : (e3 != hipSuccess ? e3 : e4));
@@ -1887,7 +1903,7 @@ int dmpm_destroy(dmpm_ctx* c)
     clear_graphs(c);
     void* bufs[] = {c->cst,  c->ctrl, c->layers, c->grids, c->wgrid, c->wgrid2, c->wgrid3, c->agrid, c->gbuf[0], c->gbuf[1], c->gbuf[2],
                     c->gctrl, c->tmp, c->target, c->dacc,  c->ranges, c->d_sps, c->d_nofs, c->dpart, c->tflags,
-                    c->d_api, c->d_poff, c->d_boff, c->sortbuf, c->wstats};
+                    c->d_api, c->d_poff, c->d_boff, c->sortbuf, c->wstats, c->queue};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (void* b : c->opt)
@@ -2191,6 +2207,19 @@ int dmpm_chain_episode(dmpm_ctx* c)
     return DISPATCH(c, chain_episode, c);
 }
 
+// FP32: the block tile (4 or 8 cells per edge); a context holding particles
+// re-orders them (resort: the pass and the segment orders are dropped)
+int set_tile(dmpm_ctx* c, int t)
+{
+    if (t == c->tile) return DMPM_OK;
+    c->tile = t;
+    set_block_prefix(c);
+    if (!c->f32) return DMPM_OK;
+    HIP_TRY(hipMemcpyAsync(c->d_boff, c->boff.data(), c->boff.size() * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return c->have_particles ? resort(c, true) : DMPM_OK;
+}
+
 int dmpm_set_option(dmpm_ctx* c, int32_t option, int64_t value)
 {
     int rc = check_ctx(c);
@@ -2228,6 +2257,17 @@ int dmpm_set_option(dmpm_ctx* c, int32_t option, int64_t value)
         c->overlap = value != 0;
         clear_graphs(c);
         return DMPM_OK;
+    case DMPM_OPT_PERSISTENT:
+        if (value != 0 && value != 1) return fail(DMPM_ERR_INVALID, "DMPM_OPT_PERSISTENT is 0 or 1");
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->persistent = value != 0 && c->queue;
+        c->batch.queue = c->persistent ? c->queue : nullptr;
+        clear_graphs(c);
+        return DMPM_OK;
+    case DMPM_OPT_TILE:
+        if (value != 4 && value != 8) return fail(DMPM_ERR_INVALID, "DMPM_OPT_TILE is 4 or 8");
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        return set_tile(c, (int)value);
     case DMPM_OPT_SEG_RESORT:
         if (value != 0 && value != 1) return fail(DMPM_ERR_INVALID, "DMPM_OPT_SEG_RESORT is 0 or 1");
         HIP_TRY(hipStreamSynchronize(c->stream));
@@ -2274,6 +2314,8 @@ int dmpm_get_option(dmpm_ctx* c, int32_t option, int64_t* value)
     case DMPM_OPT_FUSE: *value = c->fuse ? 1 : 0; return DMPM_OK;
     case DMPM_OPT_RANGE_ORDER: *value = c->range_order; return DMPM_OK;
     case DMPM_OPT_SEG_RESORT: *value = c->seg_resort; return DMPM_OK;
+    case DMPM_OPT_TILE: *value = c->tile; return DMPM_OK;
+    case DMPM_OPT_PERSISTENT: *value = c->persistent ? 1 : 0; return DMPM_OK;
     default: return fail(DMPM_ERR_INVALID, "unknown option");
     }
 }
@@ -2292,11 +2334,18 @@ int dmpm_autotune(dmpm_ctx* c, double* best_ms)
     float* l0 = slot<float>(c, 0);
     float* l1 = slot<float>(c, 1);
     float* g = static_cast<float*>(c->grids);
+    const int tiles[] = {4, 8};
     const int ranges[] = {128, 256, 512};
     const int caps[] = {512, 1024, 2048};
     double best = 1e30;
-    int best_r = c->range_max, best_c = c->lds_cap;
-    for (int cap : caps) {
+    int best_r = c->range_max, best_c = c->lds_cap, best_t = c->tile;
+    bool best_p = c->persistent;
+    for (int tile : tiles) {
+      // the positions of layer 0 are those of the particles' order: re-ordering moves them with it
+      if ((rc = set_tile(c, tile))) break;
+      for (int pers = 0; pers < 2 && !rc; ++pers) {
+      c->batch.queue = pers ? c->queue : nullptr;
+      for (int cap : caps) {
         c->lds_cap = cap;
         c->batch.cap = cap;
         for (int r : ranges) {
@@ -2322,9 +2371,14 @@ int dmpm_autotune(dmpm_ctx* c, double* best_ms)
                 best = ms_min;
                 best_r = r;
                 best_c = cap;
+                best_t = tile;
+                best_p = pers != 0;
             }
         }
         if (rc) break;
+      }
+      }
+      if (rc) break;
     }
     hipEventDestroy(a);
     hipEventDestroy(b);
@@ -2332,7 +2386,10 @@ int dmpm_autotune(dmpm_ctx* c, double* best_ms)
     c->lds_cap = best_c;
     c->batch.cap = best_c;
     c->range_max = best_r;
-    const int rc2 = build_ranges(c);
+    c->persistent = best_p && c->queue;
+    c->batch.queue = c->persistent ? c->queue : nullptr;
+    int rc2 = rc ? DMPM_OK : set_tile(c, best_t);
+    if (!rc2) rc2 = build_ranges(c);
     c->fwd_valid = 0;
     c->loss_valid = false;
     c->grads_lo = -1;

@@ -1,11 +1,11 @@
 // Device order of the FP32 path's particles, computed on the GPU.
 //
-// Within each instance the particles are ordered by the 4^3-cell block of
-// their cell (blocks x-major), then by their rank inside their cell ("slot",
-// stable in the current order), then by the cell's x-major index 16 x + 4 y + z
-// inside the block.  A 64-lane wave then holds one particle of each of a
-// block's 64 cells, so its LDS atomics hit 64 distinct nodes.  Two stable
-// radix sorts give that order:
+// Within each instance the particles are ordered by the T^3-cell block (tile)
+// of their cell (blocks x-major, T = 2^tshift: 4 or 8), then by their rank
+// inside their cell ("slot", stable in the current order), then by the cell's
+// x-major index inside the block.  With T = 4 a 64-lane wave holds one
+// particle of each of a block's 64 cells, so its LDS atomics hit 64 distinct
+// nodes.  Two stable radix sorts give that order:
 //   1. key (block, cell-in-block), value = current slot: particles grouped by
 //      cell, each cell's particles in their current order;
 //   2. the slot of each particle = its position minus the first position of
@@ -40,7 +40,7 @@ __device__ __forceinline__ int instance_of_slot(const int* poff, int B, int d)
     return lo;
 }
 
-// key (global block << 6 | cell in block), value = slot
+// key (global block << 3 tshift | cell in block), value = slot
 __global__ __launch_bounds__(kSortBlock) void k_key_cell(SortArgs a)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
@@ -56,11 +56,12 @@ __global__ __launch_bounds__(kSortBlock) void k_key_cell(SortArgs a)
         if (!(u >= 0.0)) u = 0.0;
         if (u > double(sp.dim[ax] - 1)) u = double(sp.dim[ax] - 1);
         cell[ax] = int(u);
-        nb[ax] = (sp.dim[ax] + 3) >> 2;
+        nb[ax] = (sp.dim[ax] + (1 << a.tshift) - 1) >> a.tshift;
     }
-    const long long block = a.boff[in] + ((long long)(cell[0] >> 2) * nb[1] + (cell[1] >> 2)) * nb[2] + (cell[2] >> 2);
-    const unsigned local = (unsigned)((cell[0] & 3) << 4 | (cell[1] & 3) << 2 | (cell[2] & 3));
-    a.k0[d] = ((unsigned long long)block << 6) | local;
+    const int t = a.tshift, m = (1 << t) - 1;
+    const long long block = a.boff[in] + ((long long)(cell[0] >> t) * nb[1] + (cell[1] >> t)) * nb[2] + (cell[2] >> t);
+    const unsigned local = (unsigned)((cell[0] & m) << (2 * t) | (cell[1] & m) << t | (cell[2] & m));
+    a.k0[d] = ((unsigned long long)block << (3 * t)) | local;
     a.v0[d] = d;
 }
 
@@ -72,8 +73,9 @@ __global__ __launch_bounds__(kSortBlock) void k_heads(const unsigned long long* 
     head[d] = (d == 0 || key[d] != key[d - 1]) ? d : 0;
 }
 
-// key (global block << 22 | slot << 6 | cell in block), slots beyond 2^16 - 1 clamped
-__global__ __launch_bounds__(kSortBlock) void k_key_slot(const unsigned long long* k1, const int* first, const int* v1, int N,
+// key (global block << (16 + c) | slot << c | cell in block), c = 3 tshift bits
+// of cell, slots beyond 2^16 - 1 clamped
+__global__ __launch_bounds__(kSortBlock) void k_key_slot(const unsigned long long* k1, const int* first, const int* v1, int N, int c,
                                                         unsigned long long* k2, int* v2)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
@@ -81,15 +83,15 @@ __global__ __launch_bounds__(kSortBlock) void k_key_slot(const unsigned long lon
     const unsigned long long k = k1[d];
     int slot = d - first[d];
     if (slot > 65535) slot = 65535;
-    k2[d] = ((k >> 6) << 22) | ((unsigned long long)slot << 6) | (k & 63ull);
+    k2[d] = ((k >> c) << (16 + c)) | ((unsigned long long)slot << c) | (k & ((1ull << c) - 1));
     v2[d] = v1[d];
 }
 
-__global__ __launch_bounds__(kSortBlock) void k_block_of(const unsigned long long* key, int N, int* blk)
+__global__ __launch_bounds__(kSortBlock) void k_block_of(const unsigned long long* key, int N, int shift, int* blk)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= N) return;
-    blk[d] = (int)(key[d] >> 22);
+    blk[d] = (int)(key[d] >> shift);
 }
 
 // dst[f S + d] = src[f S + perm[d]] for nf fields
@@ -139,7 +141,8 @@ size_t sort_temp_bytes(int N, long long nblocks)
     size_t a = 0, b = 0;
     unsigned long long* kp = nullptr;
     int* ip = nullptr;
-    const unsigned hi1 = 6 + bits_for(nblocks), hi2 = 22 + bits_for(nblocks);
+    // the widest keys of either tile (8^3: 9 bits of cell)
+    const unsigned hi1 = 9 + bits_for(nblocks), hi2 = 25 + bits_for(nblocks);
     if (rocprim::radix_sort_pairs(nullptr, a, kp, kp, ip, ip, (size_t)N, 0u, hi1 > hi2 ? hi1 : hi2) != hipSuccess) return 0;
     if (rocprim::inclusive_scan(nullptr, b, ip, ip, (size_t)N, rocprim::maximum<int>()) != hipSuccess) return 0;
     return a > b ? a : b;
@@ -149,7 +152,8 @@ hipError_t launch_sort_order(const SortArgs& a, hipStream_t s)
 {
     if (a.N <= 0) return hipSuccess;
     const unsigned nb = sort_blocks(a.N);
-    const unsigned hi1 = 6 + bits_for(a.nblocks), hi2 = 22 + bits_for(a.nblocks);
+    const int c = 3 * a.tshift;
+    const unsigned hi1 = c + bits_for(a.nblocks), hi2 = 16 + c + bits_for(a.nblocks);
     size_t bytes = a.temp_bytes;
     hipLaunchKernelGGL(k_key_cell, dim3(nb), dim3(kSortBlock), 0, s, a);
     hipError_t e = rocprim::radix_sort_pairs(a.temp, bytes, a.k0, a.k1, a.v0, a.v1, (size_t)a.N, 0u, hi1, s);
@@ -160,11 +164,11 @@ hipError_t launch_sort_order(const SortArgs& a, hipStream_t s)
     bytes = a.temp_bytes;
     e = rocprim::inclusive_scan(a.temp, bytes, a.head, a.v0, (size_t)a.N, rocprim::maximum<int>(), s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_key_slot, dim3(nb), dim3(kSortBlock), 0, s, a.k1, a.v0, a.v1, a.N, a.k0, a.v0);
+    hipLaunchKernelGGL(k_key_slot, dim3(nb), dim3(kSortBlock), 0, s, a.k1, a.v0, a.v1, a.N, c, a.k0, a.v0);
     bytes = a.temp_bytes;
     e = rocprim::radix_sort_pairs(a.temp, bytes, a.k0, a.k1, a.v0, a.v1, (size_t)a.N, 0u, hi2, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_block_of, dim3(nb), dim3(kSortBlock), 0, s, a.k1, a.N, a.blk);
+    hipLaunchKernelGGL(k_block_of, dim3(nb), dim3(kSortBlock), 0, s, a.k1, a.N, 16 + c, a.blk);
     return hipGetLastError();
 }
 

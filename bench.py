@@ -37,7 +37,7 @@ sys.path.insert(0, ROOT)
 
 from __graft_entry__ import _load_pkg  # noqa: E402
 
-OPT_RANGE_MAX, OPT_LDS_CAP = 2, 3
+OPT_RANGE_MAX, OPT_LDS_CAP, OPT_TILE, OPT_PERSISTENT = 2, 3, 10, 11
 
 # Algorithmic HBM bytes per FP32 particle / per touched node / per dense grid node
 # of each kernel (FP64: twice), the per-particle-step figures of the scope
@@ -48,12 +48,15 @@ OPT_RANGE_MAX, OPT_LDS_CAP = 2, 3
 # transposed G2P reads x, F, dFc, x', v', F', C' (180 B) and scatters vbar
 # (12 B per node); the transposed P2G with its elastic half reads v, C, m, vol,
 # lam, mu, C' and writes the x, v, C, F gradients (196 B), per node m, v read,
-# pbar/mbar write and read, reset (60 B).  What the implementation adds (the
-# cached affine matrix, vt/Ct and Gb round trips, window overlap) shows in the
-# measured traffic, not here.
+# pbar/mbar write and read, reset (60 B).  The fused launch of a step's G2P with
+# the next step's P2G does the work of both (328 B + 28 B per touched node).  What
+# the implementation adds (the cached affine matrix, vt/Ct and Gb round trips,
+# window overlap) or saves (the fused launch's state handed over in registers)
+# shows in the measured traffic, not here.
 KBYTES = {
     "p2g": (148, 16, 0),
     "g2p": (180, 12, 0),
+    "step": (328, 28, 0),
     "g2p_adj": (180, 12, 0),
     "p2g_adj": (196, 60, 0),
 }
@@ -124,7 +127,8 @@ def timed_episodes(mo, warmup, steps, prof_steps=0, tune=False, graphs=False):
     """Warm up, time ``steps`` episodes, then (optionally) profile ``prof_steps`` eager episodes."""
     tuned = None
     if tune:
-        tuned = {"step_ms": mo.autotune(), "range_max": mo.option(OPT_RANGE_MAX), "lds_cap": mo.option(OPT_LDS_CAP)}
+        tuned = {"step_ms": mo.autotune(), "range_max": mo.option(OPT_RANGE_MAX), "lds_cap": mo.option(OPT_LDS_CAP),
+                 "tile": mo.option(OPT_TILE), "persistent": mo.option(OPT_PERSISTENT)}
     mo.set_graphs(graphs)
     for _ in range(warmup):
         episode(mo)

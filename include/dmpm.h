@@ -207,14 +207,22 @@ int dmpm_chain_episode(dmpm_ctx* ctx);
 #define DMPM_OPT_SEG_RESORT 9  /* FP32 checkpoint tape: 1 (default) orders a segment's particles afresh from its first
                                 * layer when its P2G windows went stale (DMPM_OPT_RESORT_PERMILLE) in the last forward
                                 * pass that reached the loss, and keeps that order; 0: every segment in layer 0's order */
+#define DMPM_OPT_TILE 10       /* FP32: block tile, the cells per edge of the blocks the particles are ordered by and
+                                * each workgroup's range is drawn from: 4 (default; fixed 8^3-node LDS window) or 8
+                                * (ranges span 9 cells: the general LDS window up to DMPM_OPT_LDS_CAP nodes) */
+#define DMPM_OPT_PERSISTENT 11 /* FP32: 1 runs the particle kernels as persistent launches (as many workgroups as fit
+                                * on the chip, taking every instance's block ranges from a counter); 0 (default): one
+                                * workgroup per range */
 int dmpm_set_option(dmpm_ctx* ctx, int32_t option, int64_t value);
 int dmpm_get_option(dmpm_ctx* ctx, int32_t option, int64_t* value);
-/* FP32: time a forward and a backward step of layer 0 for every candidate range size and LDS
- * capacity, keep the fastest (best_ms: its time).  Needs particles; the forward trajectory
- * (layers after 0) and gradients are invalidated. */
+/* FP32: time a forward and a backward step of layer 0 for every candidate block tile (4, 8), launch form
+ * (a workgroup per range or persistent), range size (128, 256, 512 particles: one or two per thread) and
+ * LDS capacity (512, 1024, 2048 nodes), keep
+ * the fastest (best_ms: its time).  Needs particles; the particles are re-ordered for each tile, the
+ * forward trajectory (layers after 0) and gradients are invalidated. */
 int dmpm_autotune(dmpm_ctx* ctx, double* best_ms);
 
-/* FP32: re-order the particles on the device by the 4^3-cell block of their layer-0 cell (then rank in
+/* FP32: re-order the particles on the device by the DMPM_OPT_TILE block of their layer-0 cell (then rank in
  * cell, then cell in block), as dmpm_set_particles does; every per-particle array follows (controls,
  * optimiser states, held control gradients), the API order of every other call is unchanged, and the
  * layers after 0 are invalidated.  A no-op on the FP64 path, which keeps the API order.  No reference
