@@ -49,14 +49,14 @@ OPT_RANGE_MAX, OPT_LDS_CAP, OPT_TILE, OPT_PERSISTENT = 2, 3, 10, 11
 # (12 B per node); the transposed P2G with its elastic half reads v, C, m, vol,
 # lam, mu, C' and writes the x, v, C, F gradients (196 B), per node m, v read,
 # pbar/mbar write and read, reset (60 B).  The fused launch of a step's G2P with
-# the next step's P2G does the work of both (328 B + 28 B per touched node).  What
-# the implementation adds (the cached affine matrix, vt/Ct and Gb round trips,
-# window overlap) or saves (the fused launch's state handed over in registers)
-# shows in the measured traffic, not here.
+# the next step's P2G reads each particle once: the scope table's 244 B for a
+# fused P2G + G2P (+ 28 B per touched node).  What the implementation adds (the
+# cached affine matrix, vt/Ct and Gb round trips, window overlap) shows in the
+# measured traffic, not here.
 KBYTES = {
     "p2g": (148, 16, 0),
     "g2p": (180, 12, 0),
-    "step": (328, 28, 0),
+    "step": (244, 28, 0),
     "g2p_adj": (180, 12, 0),
     "p2g_adj": (196, 60, 0),
 }
@@ -64,6 +64,8 @@ HBM_PEAK_GBS = 8000.0
 # one forward + one adjoint particle-step: (328 + 376) B per particle, (56 + 72) B
 # per touched node; no dense grid pass (resets cover the flagged tiles only)
 STEP_BYTES = (704, 128, 0)
+# the FP32 path fuses each step's G2P with the next P2G: (244 + 376) B per particle
+STEP_BYTES_FUSED = (620, 128, 0)
 # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a
 # device-to-device copy of known size) of the 1M-particle scene's kernels
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_morph_1m_128.txt")
@@ -107,8 +109,13 @@ def make_scene(pkg, name, dt):
     raise SystemExit(f"unknown workload {name}")
 
 
+OVERLAP = True  # --no-overlap: checkpoint recompute on the main stream (isolated kernel times in a trace)
+
+
 def setup(pkg, sc, vol, dFc, precision, tape, num_layers):
     mo = pkg.Morph.from_scene(sc, precision=precision, tape=tape, num_layers=num_layers, dt=sc.dt)
+    if not OVERLAP and tape == pkg.TAPE_CHECKPOINT:
+        mo.set_option(pkg.OPT_OVERLAP, 0)
     x, v, C, F = sc.state0()
     mo.set_particles(x, v, F, C, sc.m, vol, sc.lam, sc.mu)
     mo.set_control(0, dFc)
@@ -347,6 +354,8 @@ def main():
                     help="chained episodes after the timed region, under each re-sort policy; 0: skip")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="recompute checkpoint segments on the main stream, not beside the adjoint (kernel traces)")
     ap.add_argument("--gd-iters", type=int, default=0,
                     help="loss-curve mode: this many descent iterations on the layer-0 control (FP32+ckpt and FP64)")
     ap.add_argument("--gd-precision", default="both", choices=["both", "f32", "f64"])
@@ -357,6 +366,8 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
+    global OVERLAP
+    OVERLAP = not args.no_overlap
 
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py measures one GPU; --gpus must be 1 (sharding across GPUs is described, not implemented)"}))
@@ -476,6 +487,7 @@ def main():
         print(f"# cpu baseline on {threads} of {nproc} cores ({usable} usable): f32 {cb['f32'][0]:.4g}, "
               f"f64 {cb['f64'][0]:.4g} particle-steps/s", file=sys.stderr, flush=True)
 
+    step32 = STEP_BYTES_FUSED[0] * sc.n + STEP_BYTES_FUSED[1] * touched + STEP_BYTES_FUSED[2] * sc.G
     out = {
         "metric": "fwd+adjoint particle-steps/s (FP32+ckpt vs plain FP64) at 1M pts; grad err",
         "value": value,
@@ -496,18 +508,17 @@ def main():
         "cpu_baseline": cpu,
         "hbm_copy_probe": {"achieved_GBs": copy_gbs, "frac_of_peak": copy_gbs / HBM_PEAK_GBS,
                            "note": "HIP runtime device-to-device copy of 256 MiB, read + write bytes, best of 5"},
-        "step_roofline": {"bytes_per_particle_step": (STEP_BYTES[0] * sc.n + STEP_BYTES[1] * touched + STEP_BYTES[2] * sc.G)
-                          / sc.n,
-                          "achieved_GBs": (STEP_BYTES[0] * sc.n + STEP_BYTES[1] * touched + STEP_BYTES[2] * sc.G)
-                          * args.episode_steps * args.steps / r32["time"] / 1e9,
-                          "frac": (STEP_BYTES[0] * sc.n + STEP_BYTES[1] * touched + STEP_BYTES[2] * sc.G)
-                          * args.episode_steps * args.steps / r32["time"] / 1e9 / HBM_PEAK_GBS,
-                          "note": "one forward + one adjoint per particle-step; recompute counts in the time"},
+        "step_roofline": {"bytes_per_particle_step": step32 / sc.n,
+                          "achieved_GBs": step32 * args.episode_steps * args.steps / r32["time"] / 1e9,
+                          "frac": step32 * args.episode_steps * args.steps / r32["time"] / 1e9 / HBM_PEAK_GBS,
+                          "note": "one fused forward (244 B) + one adjoint (376 B) per particle-step, 128 B per touched "
+                                  "node; recompute counts in the time"},
         "kernels_fp32": kernels,
         "kernel_rooflines": kroof,
         "kernel_timing": f"HIP events around every launch of {args.prof_steps} eager episodes after the timed region",
         "tuned": r32["tuned"],
         "graphs": not args.no_graphs,
+        "overlap": not args.no_overlap,
         "touched_nodes": touched,
         "memory_fp32_ckpt_bytes": r32["mem"],
     }
