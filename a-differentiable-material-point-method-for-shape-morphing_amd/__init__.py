@@ -45,6 +45,7 @@ OPT_RANGE_ORDER = 8
 OPT_SEG_RESORT = 9
 OPT_TILE = 10
 OPT_PERSISTENT = 11
+OPT_ELASTIC = 12
 
 K_P2G, K_GRID_UPDATE, K_G2P, K_G2P_ADJ, K_GRID_ADJ, K_P2G_ADJ, K_LOSS, K_FILL = range(8)
 KERNEL_NAMES = ["p2g", "grid_update", "g2p", "g2p_adj", "grid_adj", "p2g_adj", "loss", "fill", "step"]

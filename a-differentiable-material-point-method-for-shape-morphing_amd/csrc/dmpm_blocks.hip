@@ -55,11 +55,6 @@ namespace {
 constexpr int kGat = 256;    // gather kernels
 constexpr int kScat = 512;   // the loss mass scatter: one particle per thread of a block range
 constexpr int kMaxWaves = 8;
-// The elastic adjoint runs as a kernel of its own after the transposed P2G
-// gather (Gb through tmp): the gather then needs 167 VGPRs instead of 253 and
-// the pair of kernels measured 177 us against 189 for the fused one (1M
-// particles, 128^3).  true fuses them.
-constexpr bool kFuseElastic = false;
 constexpr int kFW = 8;       // fixed window edge (nodes)
 constexpr int kFW3 = kFW * kFW * kFW;
 // LDS layout of the fixed window: node (i, j, k) at slot i*kSI + j*kSJ + k.
@@ -1239,27 +1234,14 @@ __global__ __launch_bounds__(kG2PAdj, 4) void k_g2p_adj_blk(Batch bt, AdjIO<floa
 }
 
 // Elastic half of P2G^T + P_op_1^T for particle p (FP64 polar, PK1, H:dF):
-//   Pbar = -k Gb Fc,  Fcbar = (1-s)(I + dt C')^T F' - k Gb^T P + H(Fc):Pbar.
-// e holds det Fc, F^-T and the rotation with S (polar path, SVD false) or
-// U, sigma, V (SVD); each path is its own copy of the code, so that registers
-// are allocated for one decomposition at a time.
-template <bool SVD>
-__device__ __forceinline__ void elastic_adj_path(const StepParams& sp, const AdjIO<float>& io, int p, int h, const M3<float>& Gb,
-                                                 const M3<double>& Fc, FCE& e, double vol, double lam, double mu)
+//   Pbar = -k Gb Fc,  Fcbar = (1-s)(I + dt C')^T F' - k Gb^T P + H(Fc):Pbar
+// with k = 3 dt vol / dx^2.  rest = -k Gb^T P + H(Fc):Pbar is formed in FP64
+// and rounded; this is the FP32 remainder and the stores.
+__device__ __forceinline__ void elastic_adj_store(const StepParams& sp, const AdjIO<float>& io, int p, int h, const M3<float>& rest)
 {
-    e.svd = SVD;
     const int S = sp.stride;
     const int cp = sp.compat;
     const float dt = float(sp.dt);
-    const double kst = -3.0 * sp.inv_dx * sp.inv_dx * sp.dt;
-    const double kk_ = -kst * vol;  // 3 dt vol / dx^2
-    const M3<double> Gbd = cast<double>(Gb);
-    // Gb^T P first, so that P dies before H:dF
-    const M3<double> GTP = tmul(Gbd, fce_pk1(Fc, e, lam, mu));
-    M3<double> Pb = mul(Gbd, Fc);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Pb.a[i] *= -kk_;
-    const M3<double> H = fce_hess_mult(e, lam, mu, Pb, cp == 0);
     const M3<float> C1 = ldm(io.next, FC, S, p);
     const M3<float> gF1 = ldm(io.g1, FF, S, p);
     M3<float> IC;
@@ -1273,13 +1255,12 @@ __device__ __forceinline__ void elastic_adj_path(const StepParams& sp, const Adj
     M3<float> gF, gdFc;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        const float rest = float(-kk_ * GTP.a[i] + H.a[i]);
         if (cp & 1) {
-            const float f = (1.f - s) * base.a[i] + s * gF1.a[i] + base.a[i] + rest;
+            const float f = (1.f - s) * base.a[i] + s * gF1.a[i] + base.a[i] + rest.a[i];
             gF.a[i] = f;
             gdFc.a[i] = f;
         } else {
-            gdFc.a[i] = (1.f - s) * base.a[i] + rest;
+            gdFc.a[i] = (1.f - s) * base.a[i] + rest.a[i];
             gF.a[i] = gdFc.a[i] + s * gF1.a[i];
         }
     }
@@ -1287,36 +1268,166 @@ __device__ __forceinline__ void elastic_adj_path(const StepParams& sp, const Adj
     stm(io.gctrl, 0, S, h, gdFc);
 }
 
-__device__ __forceinline__ void elastic_adj(const StepParams& sp, const AdjIO<float>& io, int p, int h, const M3<float>& Gb)
+// symmetric 3x3, entries (00, 11, 22, 01, 02, 12)
+struct Sym3 {
+    double a[6];
+    __device__ __forceinline__ double operator()(int i, int j) const { return i == j ? a[i] : a[i + j + 2]; }
+};
+
+__device__ __forceinline__ Sym3 sym_inverse(const Sym3& s)
+{
+    Sym3 c;
+    c.a[0] = s.a[1] * s.a[2] - s.a[5] * s.a[5];
+    c.a[1] = s.a[0] * s.a[2] - s.a[4] * s.a[4];
+    c.a[2] = s.a[0] * s.a[1] - s.a[3] * s.a[3];
+    c.a[3] = s.a[4] * s.a[5] - s.a[3] * s.a[2];
+    c.a[4] = s.a[3] * s.a[5] - s.a[4] * s.a[1];
+    c.a[5] = s.a[3] * s.a[4] - s.a[0] * s.a[5];
+    const double inv = 1.0 / (s.a[0] * c.a[0] + s.a[3] * c.a[3] + s.a[4] * c.a[4]);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) c.a[i] *= inv;
+    return c;
+}
+
+// rest of the polar path (Fc = R S, R from the Newton iteration), in the
+// frame of the rotation: F^-T = R S^-1, P = R Sig with
+// Sig = 2 mu (S - I) + lam (J - 1) J S^-1, and with Gh = R^T Gb R
+//   Gb^T P = R Gh^T Sig,   Pbar = R Pb' with Pb' = -k Gh S,
+//   F^-T:Pbar = tr(S^-1 Pb') = -k tr Gb,   F^-T Pbar^T F^-T = -k R Gh^T S^-1,
+//   dR = R [w]x with (tr S I - S) w = axial(Pb' - Pb'^T),
+// so rest = R (Gh^T T + 2 mu (Pb' - [w]x) + c1 S^-1) with the symmetric
+// T = c2 S^-1 - k Sig: the closed form of fce_hess_mult (Elasticity.cpp:100-178)
+// in terms of S, S^-1 and Gh (c1, c2 below; the J-clamped branch of the exact
+// adjoint keeps only the F^-T variation).  False when Fc needs the SVD
+// (det Fc <= 1e-6 |Fc|^3 or no Newton convergence); the caller then takes
+// elastic_adj_svd.
+__device__ __forceinline__ bool elastic_rest_polar(const M3<double>& Fc, const M3<float>& Gb, double k, double lam, double mu,
+                                                   bool exact, M3<float>& rest)
+{
+    const double Ju = det(Fc);
+    M3<double> R;
+    if (!polar_newton(Fc, Ju, R)) return false;
+    const double J = Ju < kJMin ? kJMin : Ju;
+    Sym3 S;
+    {
+        const M3<double> RF = tmul(R, Fc);
+        S.a[0] = RF.a[0];
+        S.a[1] = RF.a[4];
+        S.a[2] = RF.a[8];
+        S.a[3] = 0.5 * (RF.a[1] + RF.a[3]);
+        S.a[4] = 0.5 * (RF.a[2] + RF.a[6]);
+        S.a[5] = 0.5 * (RF.a[5] + RF.a[7]);
+    }
+    const double fd = -k * (double(Gb.a[0]) + double(Gb.a[4]) + double(Gb.a[8]));
+    const M3<double> Gh = tmul(R, mul(cast<double>(Gb), R));
+    M3<double> N;  // Pb' first
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) N.a[3 * i + j] = -k * (Gh.a[3 * i] * S(0, j) + Gh.a[3 * i + 1] * S(1, j) + Gh.a[3 * i + 2] * S(2, j));
+    double w0, w1, w2;
+    {
+        const double b0 = N.a[7] - N.a[5], b1 = N.a[2] - N.a[6], b2 = N.a[3] - N.a[1];
+        Sym3 A;
+        A.a[0] = S.a[1] + S.a[2];
+        A.a[1] = S.a[0] + S.a[2];
+        A.a[2] = S.a[0] + S.a[1];
+        A.a[3] = -S.a[3];
+        A.a[4] = -S.a[4];
+        A.a[5] = -S.a[5];
+        const Sym3 Ai = sym_inverse(A);
+        w0 = Ai(0, 0) * b0 + Ai(0, 1) * b1 + Ai(0, 2) * b2;
+        w1 = Ai(1, 0) * b0 + Ai(1, 1) * b1 + Ai(1, 2) * b2;
+        w2 = Ai(2, 0) * b0 + Ai(2, 1) * b1 + Ai(2, 2) * b2;
+    }
+    double c1, c2;
+    if (exact && Ju <= kJMin) {
+        c1 = 0.0;
+        c2 = k * lam * (J - 1.0) * J;
+    } else {
+        const double k2 = lam * (J - 1.0) * Ju;
+        c1 = (lam * J * J + k2) * fd;
+        c2 = k2 * k;
+    }
+    const double m2 = 2.0 * mu;
+    const double kJ = lam * (J - 1.0) * J;
+    const Sym3 Si = sym_inverse(S);
+    Sym3 T;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) T.a[d] = c2 * Si.a[d] - k * (m2 * (d < 3 ? S.a[d] - 1.0 : S.a[d]) + kJ * Si.a[d]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            N.a[3 * i + j] = m2 * N.a[3 * i + j] + c1 * Si(i, j) + (Gh.a[i] * T(0, j) + Gh.a[3 + i] * T(1, j) + Gh.a[6 + i] * T(2, j));
+    // - 2 mu [w]x
+    N.a[1] += m2 * w2;
+    N.a[2] -= m2 * w1;
+    N.a[3] -= m2 * w2;
+    N.a[5] += m2 * w0;
+    N.a[6] += m2 * w1;
+    N.a[7] -= m2 * w0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            rest.a[3 * i + j] = float(R.a[3 * i] * N.a[j] + R.a[3 * i + 1] * N.a[3 + j] + R.a[3 * i + 2] * N.a[6 + j]);
+    return true;
+}
+
+__device__ __forceinline__ M3<double> elastic_fc(const StepParams& sp, const AdjIO<float>& io, int p, int h)
 {
     const int S = sp.stride;
     const M3<float> F = ldm(io.cur, FF, S, p);
     const M3<float> dF = ldm(io.ctrl, 0, S, h);
-    const double vol = io.cst[KVOL * S + h];
-    const double lam = io.cst[KLAM * S + h], mu = io.cst[KMU * S + h];
     M3<double> Fc;
 #pragma unroll
     for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);
-    // fce_prepare, split by path
+    return Fc;
+}
+
+// The elastic half on the polar path; false (nothing written) when Fc needs the SVD.
+__device__ __forceinline__ bool elastic_adj_polar(const StepParams& sp, const AdjIO<float>& io, int p, int h, const M3<float>& Gb)
+{
+    const int S = sp.stride;
+    const M3<double> Fc = elastic_fc(sp, io, p, h);
+    const double vol = io.cst[KVOL * S + h];
+    const double lam = io.cst[KLAM * S + h], mu = io.cst[KMU * S + h];
+    M3<float> rest;
+    if (!elastic_rest_polar(Fc, Gb, 3.0 * sp.inv_dx * sp.inv_dx * sp.dt * vol, lam, mu, sp.compat == 0, rest)) return false;
+    elastic_adj_store(sp, io, p, h, rest);
+    return true;
+}
+
+// The elastic half on the SVD path (inverted or nearly flat elements).
+__device__ __forceinline__ void elastic_adj_svd(const StepParams& sp, const AdjIO<float>& io, int p, int h, const M3<float>& Gb)
+{
+    const int S = sp.stride;
+    const M3<double> Fc = elastic_fc(sp, io, p, h);
+    const double vol = io.cst[KVOL * S + h];
+    const double lam = io.cst[KLAM * S + h], mu = io.cst[KMU * S + h];
     FCE e;
+    e.svd = true;
     e.Ju = det(Fc);
     e.J = e.Ju < kJMin ? kJMin : e.Ju;
     const M3<double> cf = cofactor(Fc);
     const double inv = 1.0 / e.Ju;
 #pragma unroll
     for (int i = 0; i < 9; ++i) e.Fit.a[i] = cf.a[i] * inv;
-    if (polar_newton(Fc, e.Ju, e.R)) {
-        const M3<double> RF = tmul(e.R, Fc);
+    svd3(Fc, e.U, e.s, e.V);
+    e.R = mult(e.U, e.V);
+    const double k = 3.0 * sp.inv_dx * sp.inv_dx * sp.dt * vol;
+    const M3<double> Gbd = cast<double>(Gb);
+    // Gb^T P first, so that P dies before H:dF
+    const M3<double> GTP = tmul(Gbd, fce_pk1(Fc, e, lam, mu));
+    M3<double> Pb = mul(Gbd, Fc);
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < 9; ++i) Pb.a[i] *= -k;
+    const M3<double> H = fce_hess_mult(e, lam, mu, Pb, sp.compat == 0);
+    M3<float> rest;
 #pragma unroll
-            for (int j = 0; j < 3; ++j) e.S.a[3 * i + j] = 0.5 * (RF.a[3 * i + j] + RF.a[3 * j + i]);
-        elastic_adj_path<false>(sp, io, p, h, Gb, Fc, e, vol, lam, mu);
-    } else {
-        svd3(Fc, e.U, e.s, e.V);
-        e.R = mult(e.U, e.V);
-        elastic_adj_path<true>(sp, io, p, h, Gb, Fc, e, vol, lam, mu);
-    }
+    for (int i = 0; i < 9; ++i) rest.a[i] = float(-k * GTP.a[i] + H.a[i]);
+    elastic_adj_store(sp, io, p, h, rest);
 }
 
 // ------------------------------------------------------- transposed P2G ----
@@ -1451,30 +1562,68 @@ __device__ __forceinline__ void p2g_adj_range(const Batch& bt, AdjIO<float> io, 
 #pragma unroll
         for (int i = 0; i < 9; ++i) gC.a[i] = m * Gb.a[i];
         stm(io.g0, FC, S, p, gC);
-        if (FUSE) elastic_adj(sp, io, p, home_of(bt, p), Gb);
-        else stm(io.tmp, 0, S, p, Gb);  // vt, Ct are dead for this particle
+        // vt, Ct in tmp are dead for this particle: Gb takes their place when
+        // the elastic half runs in a kernel of its own (FUSE false, or the SVD
+        // path of a particle deferred to k_elastic_defer)
+        if (FUSE) {
+            if (!elastic_adj_polar(sp, io, p, home_of(bt, p), Gb)) {
+                stm(io.tmp, 0, S, p, Gb);
+                io.defer[atomicAdd(io.defer_n, 1)] = make_int2(p, rg.z);
+            }
+        } else {
+            stm(io.tmp, 0, S, p, Gb);
+        }
     }
 }
 
-template <bool FUSE, bool PERSIST>
-__global__ __launch_bounds__(kGat) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
+// OCC: the waves per SIMD the register allocation must allow (1: no limit)
+template <bool FUSE, bool PERSIST, int OCC>
+__global__ __launch_bounds__(kGat, OCC) void k_p2g_adj_blk(Batch bt, AdjIO<float> io)
 {
     extern __shared__ float lds[];
     for_ranges<PERSIST>(bt, TileClear{}, [&](const int4 rg) { p2g_adj_range<FUSE>(bt, io, rg, lds); });
 }
 
-// Elastic half as a kernel of its own (Gb from tmp), one thread per particle of a range.
+// Elastic half as a kernel of its own (Gb from tmp), one thread per particle
+// of a range; the SVD path is deferred to k_elastic_defer as in the fused gather.
 __device__ __forceinline__ void elastic_range(const Batch& bt, const AdjIO<float>& io, const int4 rg)
 {
     const StepParams sp = bt.sps[rg.z];
     const int S = sp.stride;
-    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x) elastic_adj(sp, io, p, home_of(bt, p), ldm(io.tmp, 0, S, p));
+    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += blockDim.x)
+        if (!elastic_adj_polar(sp, io, p, home_of(bt, p), ldm(io.tmp, 0, S, p))) io.defer[atomicAdd(io.defer_n, 1)] = make_int2(p, rg.z);
 }
 
 template <bool PERSIST>
-__global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io, TileClear tc)
+__global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io)
 {
-    for_ranges<PERSIST>(bt, tc, [&](const int4 rg) { elastic_range(bt, io, rg); });
+    for_ranges<PERSIST>(bt, TileClear{}, [&](const int4 rg) { elastic_range(bt, io, rg); });
+}
+
+// The elastic half of the particles deferred to the SVD path (Gb in tmp):
+// workgroups [0, nwork) share the list; the last of them to finish resets
+// its two counters for the next step (every one has read the entry count
+// before it adds to the done count).  Workgroups from nwork on clear tc.
+__global__ __launch_bounds__(kGat) void k_elastic_defer(Batch bt, AdjIO<float> io, TileClear tc, unsigned nwork)
+{
+    if (blockIdx.x >= nwork) {
+        appended_clear(bt, tc, nwork);
+        return;
+    }
+    const int n = io.defer_n[0];
+    for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += (int)(nwork * blockDim.x)) {
+        const int2 d = io.defer[i];
+        const StepParams sp = bt.sps[d.y];
+        elastic_adj_svd(sp, io, d.x, home_of(bt, d.x), ldm(io.tmp, 0, sp.stride, d.x));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(io.defer_n + 1, 1) == (int)nwork - 1) {
+            atomicExch(io.defer_n, 0);
+            atomicExch(io.defer_n + 1, 0);
+        }
+    }
 }
 
 // --------------------------------------------------- batched grid kernels --
@@ -1674,7 +1823,10 @@ void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
     }
 }
 
-void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear& tc, hipStream_t s)
+// the deferred-list workgroups of k_elastic_defer
+constexpr unsigned kDeferBlocks = 64;
+
+void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear& tc, int elastic, hipStream_t s)
 {
     if (b.nranges <= 0) {
         if (tc.grid) launch_clear_tiles(b, tc.grid, tc.flags, s);
@@ -1682,20 +1834,33 @@ void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear&
     }
     const size_t lds = 7 * win_nodes(b.cap) * sizeof(float);
     Batch bq;
-    if (kFuseElastic) {
-        const dim3 g = range_grid(b, bq, TileClear{}, k_p2g_adj_blk<true, false>, kGat, lds, 1);
-        hipLaunchKernelGGL((k_p2g_adj_blk<true, false>), dim3(b.nranges), dim3(kGat), lds, s, b, io);
-        (void)g;
-        if (tc.grid) launch_clear_tiles(b, tc.grid, tc.flags, s);
-    } else if (b.queue) {
-        const dim3 g = range_grid(b, bq, TileClear{}, k_p2g_adj_blk<false, true>, kGat, lds, 1);
-        hipLaunchKernelGGL((k_p2g_adj_blk<false, true>), g, dim3(kGat), lds, s, bq, io);
-        const dim3 ge = range_grid(b, bq, tc, k_elastic_adj<true>, kGat, 0, 1);
-        hipLaunchKernelGGL(k_elastic_adj<true>, ge, dim3(kGat), 0, s, bq, io, tc);
+    dim3 g;
+    if (elastic == 0) {
+        if (b.queue) {
+            g = range_grid(b, bq, TileClear{}, k_p2g_adj_blk<false, true, 1>, kGat, lds, 1);
+            hipLaunchKernelGGL((k_p2g_adj_blk<false, true, 1>), g, dim3(kGat), lds, s, bq, io);
+            g = range_grid(b, bq, TileClear{}, k_elastic_adj<true>, kGat, 0, 1);
+            hipLaunchKernelGGL(k_elastic_adj<true>, g, dim3(kGat), 0, s, bq, io);
+        } else {
+            hipLaunchKernelGGL((k_p2g_adj_blk<false, false, 1>), dim3(b.nranges), dim3(kGat), lds, s, b, io);
+            hipLaunchKernelGGL(k_elastic_adj<false>, dim3(b.nranges), dim3(kGat), 0, s, b, io);
+        }
+    } else if (elastic == 1) {
+        if (b.queue) {
+            g = range_grid(b, bq, TileClear{}, k_p2g_adj_blk<true, true, 1>, kGat, lds, 1);
+            hipLaunchKernelGGL((k_p2g_adj_blk<true, true, 1>), g, dim3(kGat), lds, s, bq, io);
+        } else {
+            hipLaunchKernelGGL((k_p2g_adj_blk<true, false, 1>), dim3(b.nranges), dim3(kGat), lds, s, b, io);
+        }
     } else {
-        hipLaunchKernelGGL((k_p2g_adj_blk<false, false>), dim3(b.nranges), dim3(kGat), lds, s, b, io);
-        hipLaunchKernelGGL(k_elastic_adj<false>, dim3(b.nranges + clear_blocks(b, tc)), dim3(kGat), 0, s, b, io, tc);
+        if (b.queue) {
+            g = range_grid(b, bq, TileClear{}, k_p2g_adj_blk<true, true, 3>, kGat, lds, 1);
+            hipLaunchKernelGGL((k_p2g_adj_blk<true, true, 3>), g, dim3(kGat), lds, s, bq, io);
+        } else {
+            hipLaunchKernelGGL((k_p2g_adj_blk<true, false, 3>), dim3(b.nranges), dim3(kGat), lds, s, b, io);
+        }
     }
+    hipLaunchKernelGGL(k_elastic_defer, dim3(kDeferBlocks + clear_blocks(b, tc)), dim3(kGat), 0, s, b, io, tc, kDeferBlocks);
 }
 
 void launch_clear_tiles(const Batch& b, float* grid, unsigned* flags, hipStream_t s)

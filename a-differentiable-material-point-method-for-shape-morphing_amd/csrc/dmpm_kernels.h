@@ -55,6 +55,11 @@ struct AdjIO {
     Real* tmp;           // per particle vt (3) and Ct (9), from the transposed G2P
     const unsigned* gflags;  // FP32: tile flags of grid, else null
     unsigned* aflags;        // FP32: tile flags of agrid (set where the transposed G2P writes)
+    // FP32: particles whose elastic half needs the SVD, deferred by the
+    // transposed P2G to a kernel of their own ({slot, instance}, S entries),
+    // and two counters (entries, workgroups done) that start and end at 0
+    int2* defer;
+    int* defer_n;
 };
 
 // most workgroups of a one-row FP64 reduction (loss, norm)
@@ -137,7 +142,10 @@ void launch_step_blk(const Batch& b, const StepIO<float>& gio, const StepIO<floa
 void launch_p2g_mass_blk(const Batch& b, const float* lay, const float* cst, float* gm, hipStream_t s);
 void launch_g2p_blk(const Batch& b, const StepIO<float>& io, const TileClear& tc, hipStream_t s);
 void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s);
-void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear& tc, hipStream_t s);
+// elastic: where the elastic half of the transposed P2G runs (DMPM_OPT_ELASTIC): 0 in a
+// kernel of its own, 1 in the gather kernel, 2 there at 3 waves per SIMD; particles that
+// need the SVD run in k_elastic_defer, whose appended workgroups clear tc
+void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear& tc, int elastic, hipStream_t s);
 // loss over all instances from node masses gm (instance i at nofs[i]):
 // per-instance losses in acc[0..B) (fixed-order sums of workgroup partials
 // kept in part, at most loss_partials(B) doubles); gmax = largest instance node count

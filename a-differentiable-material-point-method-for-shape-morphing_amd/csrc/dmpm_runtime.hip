@@ -122,7 +122,9 @@ struct dmpm_ctx {
     long long* d_boff = nullptr;    // block prefix per instance (B+1)
     std::vector<long long> boff;
     int tile = 4;                   // block edge in cells (4 or 8): the LDS tile of a workgroup's particles
-    int* queue = nullptr;           // persistent launches: range counters of the forward and adjoint kernels (4)
+    int* queue = nullptr;           // persistent launches: range counters of the forward and adjoint kernels (4),
+                                    // then the two counters of the deferred elastic list (2)
+    int2* defer = nullptr;          // FP32: particles deferred to the SVD path of the elastic adjoint (S)
     bool persistent = false;
     long long nblocks4 = 0;         // blocks of every instance at tile 4 (the most of either tile)
     void* sortbuf = nullptr;        // sort keys, values, heads, blocks and rocPRIM scratch (first sort)
@@ -186,6 +188,8 @@ struct dmpm_ctx {
     bool fuse = true;
     // FP32: order of the block ranges (launch order): 0 by block, 1 longest first
     int range_order = 1;
+    // FP32: where the elastic half of the transposed P2G runs (DMPM_OPT_ELASTIC)
+    int elastic = 0;
     // validity
     bool have_particles = false;
     bool have_target = false;
@@ -489,7 +493,7 @@ inline void grid_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<dou
 inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<double>& io) { launch_p2g_adj<double>(sp, io, c->stream); }
 inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams&, const AdjIO<float>& io, const TileClear& tc)
 {
-    launch_p2g_adj_blk(c->batch, io, tc, c->stream);
+    launch_p2g_adj_blk(c->batch, io, tc, c->elastic, c->stream);
 }
 
 // One ForwardTimeStep: cur -> nxt, grid of the step in g.  g_clean: the
@@ -546,6 +550,8 @@ bool step_backward(dmpm_ctx* c, int i, const Real* cur, const Real* nxt, const R
     io.tmp = static_cast<Real*>(c->tmp);
     io.gflags = flags_of(c, g);
     io.aflags = flags_of(c, c->agrid);
+    io.defer = c->defer;
+    io.defer_n = c->queue ? c->queue + 4 : nullptr;
     if constexpr (std::is_same<Real, double>::value) {
         fill<Real>(c, io.agrid, 3 * c->G, Real(0));  // the vbar channels
         timed(c, DMPM_K_G2P_ADJ, [&] { g2p_adj_dispatch(c, sp, io); });
@@ -1809,7 +1815,8 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
         (c->f32 && (rc = dalloc(c, (void**)&c->d_poff, (c->B + 1) * sizeof(int)))) ||
         (c->f32 && (rc = dalloc(c, (void**)&c->d_boff, (c->B + 1) * sizeof(long long)))) ||
         (c->f32 && (rc = dalloc(c, (void**)&c->wstats, 3 * (size_t)c->nstat * sizeof(unsigned long long)))) ||
-        (c->f32 && (rc = dalloc(c, (void**)&c->queue, 4 * sizeof(int))))) {
+        (c->f32 && (rc = dalloc(c, (void**)&c->queue, 6 * sizeof(int)))) ||
+        (c->f32 && (rc = dalloc(c, (void**)&c->defer, S * sizeof(int2))))) {
         std::string msg = g_err;
         dmpm_destroy(c);
         return fail(rc, msg);
@@ -1845,7 +1852,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
             e3 = hipMemcpyAsync(c->d_poff, c->poff.data(), c->poff.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
             e4 = hipMemcpyAsync(c->d_boff, c->boff.data(), c->boff.size() * sizeof(long long), hipMemcpyHostToDevice, c->stream);
             launch_fill<double>(reinterpret_cast<double*>(c->wstats), 3LL * c->nstat, 0.0, c->stream);
-            launch_fill<float>(reinterpret_cast<float*>(c->queue), 4, 0.f, c->stream);  // the bits of int 0
+            launch_fill<float>(reinterpret_cast<float*>(c->queue), 6, 0.f, c->stream);  // the bits of int 0
         }
         he = hipStreamSynchronize(c->stream);
         if (he == hipSuccess) he = e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : (e3 != hipSuccess ? e3 : e4));
@@ -1903,7 +1910,7 @@ int dmpm_destroy(dmpm_ctx* c)
     clear_graphs(c);
     void* bufs[] = {c->cst,  c->ctrl, c->layers, c->grids, c->wgrid, c->wgrid2, c->wgrid3, c->agrid, c->gbuf[0], c->gbuf[1], c->gbuf[2],
                     c->gctrl, c->tmp, c->target, c->dacc,  c->ranges, c->d_sps, c->d_nofs, c->dpart, c->tflags,
-                    c->d_api, c->d_poff, c->d_boff, c->sortbuf, c->wstats, c->queue};
+                    c->d_api, c->d_poff, c->d_boff, c->sortbuf, c->wstats, c->queue, c->defer};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (void* b : c->opt)
@@ -2296,6 +2303,12 @@ int dmpm_set_option(dmpm_ctx* c, int32_t option, int64_t value)
         c->fuse = value != 0;
         clear_graphs(c);
         return DMPM_OK;
+    case DMPM_OPT_ELASTIC:
+        if (value < 0 || value > 2) return fail(DMPM_ERR_INVALID, "DMPM_OPT_ELASTIC is 0, 1 or 2");
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->elastic = (int)value;
+        clear_graphs(c);
+        return DMPM_OK;
     default:
         return fail(DMPM_ERR_INVALID, "unknown option");
     }
@@ -2316,6 +2329,7 @@ int dmpm_get_option(dmpm_ctx* c, int32_t option, int64_t* value)
     case DMPM_OPT_SEG_RESORT: *value = c->seg_resort; return DMPM_OK;
     case DMPM_OPT_TILE: *value = c->tile; return DMPM_OK;
     case DMPM_OPT_PERSISTENT: *value = c->persistent ? 1 : 0; return DMPM_OK;
+    case DMPM_OPT_ELASTIC: *value = c->elastic; return DMPM_OK;
     default: return fail(DMPM_ERR_INVALID, "unknown option");
     }
 }

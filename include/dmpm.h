@@ -213,6 +213,10 @@ int dmpm_chain_episode(dmpm_ctx* ctx);
 #define DMPM_OPT_PERSISTENT 11 /* FP32: 1 runs the particle kernels as persistent launches (as many workgroups as fit
                                 * on the chip, taking every instance's block ranges from a counter); 0 (default): one
                                 * workgroup per range */
+#define DMPM_OPT_ELASTIC 12    /* FP32: where the elastic half of the transposed P2G (FP64 polar, PK1 and H:dF) runs:
+                                * 0 in a kernel of its own after the gather, 1 in the gather kernel, 2 there with
+                                * registers for 3 waves per SIMD; particles that need the SVD (inverted or nearly
+                                * flat elements) run in a small kernel after it in every mode (the same results) */
 int dmpm_set_option(dmpm_ctx* ctx, int32_t option, int64_t value);
 int dmpm_get_option(dmpm_ctx* ctx, int32_t option, int64_t* value);
 /* FP32: time a forward and a backward step of layer 0 for every candidate block tile (4, 8), launch form
