@@ -47,6 +47,7 @@
 #include "dmpm_math.h"
 
 #include <climits>
+#include <type_traits>
 
 namespace dmpm {
 
@@ -505,10 +506,13 @@ __device__ __forceinline__ void scatter64(const StepParams& sp, const Win& W, co
     }
 }
 
-// Gather over a particle's 64 nodes: fn(i, j, k, ch) with ch[c] the node's
-// channel c, then row(i, j) after the nodes of each (i, j) row of 4 (the
-// kernels sum a row with the k-weights first and apply the (i, j) factors
-// once per row).  MODE 0: fixed window, 1: general LDS window, 2: HBM.
+// Gather over a particle's 64 nodes: fn(mode, i, j, k, ch) with ch[c] the
+// node's channel c, then row(mode, i, j) after the nodes of each (i, j) row of
+// 4 (the kernels sum a row with the k-weights first and apply the (i, j)
+// factors once per row); mode is MODE as a std::integral_constant.  MODE 0:
+// fixed window (every node visited), 1: general LDS window, 2: HBM (nodes
+// outside the grid skipped: a row may miss its first nodes, or be skipped
+// with its row call when its (i, j) is outside).
 template <int NC, int MODE, typename Ld, typename Fn, typename RowFn>
 __device__ __forceinline__ void gather64(const StepParams& sp, const Win& W, const Axis<float> st[3], const float* lds, Ld&& ld,
                                          Fn&& fn, RowFn&& row)
@@ -538,9 +542,9 @@ __device__ __forceinline__ void gather64(const StepParams& sp, const Win& W, con
                 } else {
                     ld(((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk, ch);
                 }
-                fn(i, j, kk, ch);
+                fn(std::integral_constant<int, MODE>{}, i, j, kk, ch);
             }
-            row(i, j);
+            row(std::integral_constant<int, MODE>{}, i, j);
         }
     }
 }
@@ -915,19 +919,26 @@ __device__ __forceinline__ void g2p_particle(const StepParams& sp, const StepIO<
     offsets4(st[2], Dd, e2s);
     float vp[3] = {0.f, 0.f, 0.f};
     M3<float> Cp = m3_zero<float>();
-    // row sums R = sum_k w_k v, Re = sum_k w_k e_k v
+    // row sums R = sum_k w_k v, Re = sum_k w_k e_k v: the fixed window visits
+    // every node, so a row's first node starts them; the other modes may skip
+    // it, so their sums start at 0 and restart after every row
     float we2[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) we2[k] = st[2].w[k] * e2s[k];
     float R[3] = {0.f, 0.f, 0.f}, Re[3] = {0.f, 0.f, 0.f};
-    auto acc = [&](int, int, int kk, const float* vi) {
+    auto acc = [&](auto mode, int, int, int kk, const float* vi) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            R[a] += st[2].w[kk] * vi[a];
-            Re[a] += we2[kk] * vi[a];
+            if (decltype(mode)::value == 0 && kk == 0) {
+                R[a] = st[2].w[0] * vi[a];
+                Re[a] = we2[0] * vi[a];
+            } else {
+                R[a] += st[2].w[kk] * vi[a];
+                Re[a] += we2[kk] * vi[a];
+            }
         }
     };
-    auto row = [&](int i, int j) {
+    auto row = [&](auto mode, int i, int j) {
         const float w01 = st[0].w[i] * st[1].w[j];
         const float c0 = w01 * e0s[i], c1 = w01 * e1s[j];
 #pragma unroll
@@ -936,8 +947,10 @@ __device__ __forceinline__ void g2p_particle(const StepParams& sp, const StepIO<
             Cp.a[3 * a] += c0 * R[a];
             Cp.a[3 * a + 1] += c1 * R[a];
             Cp.a[3 * a + 2] += w01 * Re[a];
-            R[a] = 0.f;
-            Re[a] = 0.f;
+            if (decltype(mode)::value != 0) {
+                R[a] = 0.f;
+                Re[a] = 0.f;
+            }
         }
     };
     if (W.fixed) gather64<3, 0>(sp, W, st, lds, ld, acc, row);
@@ -1502,22 +1515,26 @@ __device__ __forceinline__ void p2g_adj_range(const Batch& bt, AdjIO<float> io, 
         offsets4(st[2], dx, d2s);
         // the sums over nodes are regrouped so that the node loop does the
         // least work: sum_i w_i G^T pbar_i = G^T (sum_i w_i pbar_i) and
-        // sum_i w_i D Ct^T v_i = D Ct^T (sum_i w_i v_i) leave the loop
+        // sum_i w_i D Ct^T v_i = D Ct^T (sum_i w_i v_i) leave the loop, and the
+        // weights' (i, j) factors leave the k-rows: a row sums only the
+        // k-weighted channels (w2, w2 d2 and dw2 times them), which the row's
+        // (i, j) factors then scale
         M3<float> CtD;
 #pragma unroll
         for (int i = 0; i < 9; ++i) CtD.a[i] = Dinv * Ct.a[i];
         M3<float> Gb = m3_zero<float>();
         float xg[3] = {0.f, 0.f, 0.f}, pacc[3] = {0.f, 0.f, 0.f}, vacc[3] = {0.f, 0.f, 0.f};
-        // per node (a row-wise contraction as in G2P needs 212 VGPRs here,
-        // occupancy 2, and measured 213 us against 181 us)
-        // xg_c = sum_k u_c[k] T_c[k] with T_c[k] = sum_ij g_c(i, j) s(i, j, k): the
-        // gradient weight's k-factor u (w_k or w'_k) leaves the node loop
-        float T[3][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-        auto acc = [&](int i, int j, int kk, const float* ch) {
+        float wzd[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) wzd[k] = st[2].w[k] * d2s[k];
+        // row sums: rs = sum_k w2 pbar, rsd = sum_k w2 d2 pbar, vs = sum_k w2 v,
+        // s0 = sum_k w2 sgw, s2 = sum_k dw2 sgw.  The fixed window visits every
+        // node, so its first node of a row starts the sums; the other modes
+        // may skip it, so their sums start at 0 and restart after every row.
+        float rs[3] = {0.f, 0.f, 0.f}, rsd[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f}, s0 = 0.f, s2 = 0.f;
+        auto acc = [&](auto mode, int i, int j, int kk, const float* ch) {
+            constexpr bool first = decltype(mode)::value == 0;
             const float d0 = d0s[i], d1 = d1s[j], d2 = d2s[kk];
-            const float w01 = st[0].w[i] * st[1].w[j];
-            const float w = w01 * st[2].w[kk];
-            const float g[3] = {mgw * st[0].dw[i] * st[1].w[j], mgw * st[0].w[i] * st[1].dw[j], mgw * w01};
             float sgw = m * ch[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -1525,26 +1542,52 @@ __device__ __forceinline__ void p2g_adj_range(const Batch& bt, AdjIO<float> io, 
                 const float q = (b[a] + Gm.a[3 * a] * d0 + Gm.a[3 * a + 1] * d1) + Gm.a[3 * a + 2] * d2;
                 const float r = (vt[a] + CtD.a[3 * a] * d0 + CtD.a[3 * a + 1] * d1) + CtD.a[3 * a + 2] * d2;
                 sgw += ch[a] * q + ch[4 + a] * r;
-                const float wp = w * ch[a];
-                pacc[a] += wp;
-                Gb.a[3 * a] += wp * d0;
-                Gb.a[3 * a + 1] += wp * d1;
-                Gb.a[3 * a + 2] += wp * d2;
-                vacc[a] += w * ch[4 + a];
+                if (first && kk == 0) {
+                    rs[a] = st[2].w[0] * ch[a];
+                    rsd[a] = wzd[0] * ch[a];
+                    vs[a] = st[2].w[0] * ch[4 + a];
+                } else {
+                    rs[a] += st[2].w[kk] * ch[a];
+                    rsd[a] += wzd[kk] * ch[a];
+                    vs[a] += st[2].w[kk] * ch[4 + a];
+                }
             }
-#pragma unroll
-            for (int a = 0; a < 3; ++a) T[a][kk] += g[a] * sgw;
+            if (first && kk == 0) {
+                s0 = st[2].w[0] * sgw;
+                s2 = st[2].dw[0] * sgw;
+            } else {
+                s0 += st[2].w[kk] * sgw;
+                s2 += st[2].dw[kk] * sgw;
+            }
         };
-        auto row = [](int, int) {};
+        auto row = [&](auto mode, int i, int j) {
+            const float w01 = st[0].w[i] * st[1].w[j];
+            const float a0 = w01 * d0s[i], a1 = w01 * d1s[j];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                pacc[a] += w01 * rs[a];
+                Gb.a[3 * a] += a0 * rs[a];
+                Gb.a[3 * a + 1] += a1 * rs[a];
+                Gb.a[3 * a + 2] += w01 * rsd[a];
+                vacc[a] += w01 * vs[a];
+            }
+            xg[0] += (st[0].dw[i] * st[1].w[j]) * s0;
+            xg[1] += (st[0].w[i] * st[1].dw[j]) * s0;
+            xg[2] += w01 * s2;
+            if (decltype(mode)::value != 0) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) rs[a] = rsd[a] = vs[a] = 0.f;
+                s0 = s2 = 0.f;
+            }
+            // keep the scheduler from interleaving the rows (their hoisted
+            // row terms and loads would raise the register count)
+            __builtin_amdgcn_sched_barrier(0);
+        };
         if (W.fixed) gather64<7, 0>(sp, W, st, lds, ld, acc, row);
         else if (W.lds) gather64<7, 1>(sp, W, st, lds, ld, acc, row);
         else gather64<7, 2>(sp, W, st, lds, ld, acc, row);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            xg[0] += st[2].w[k] * T[0][k];
-            xg[1] += st[2].w[k] * T[1][k];
-            xg[2] += st[2].dw[k] * T[2][k];
-        }
+        for (int a = 0; a < 3; ++a) xg[a] *= mgw;
         float xb[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
@@ -1594,10 +1637,11 @@ __device__ __forceinline__ void elastic_range(const Batch& bt, const AdjIO<float
         if (!elastic_adj_polar(sp, io, p, home_of(bt, p), ldm(io.tmp, 0, S, p))) io.defer[atomicAdd(io.defer_n, 1)] = make_int2(p, rg.z);
 }
 
+// tc: the adjoint grid, cleared for the next step by appended workgroups
 template <bool PERSIST>
-__global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io)
+__global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io, TileClear tc)
 {
-    for_ranges<PERSIST>(bt, TileClear{}, [&](const int4 rg) { elastic_range(bt, io, rg); });
+    for_ranges<PERSIST>(bt, tc, [&](const int4 rg) { elastic_range(bt, io, rg); });
 }
 
 // The elastic half of the particles deferred to the SVD path (Gb in tmp):
@@ -1824,7 +1868,7 @@ void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
 }
 
 // the deferred-list workgroups of k_elastic_defer
-constexpr unsigned kDeferBlocks = 64;
+constexpr unsigned kDeferBlocks = 16;
 
 void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear& tc, int elastic, hipStream_t s)
 {
@@ -1835,16 +1879,18 @@ void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear&
     const size_t lds = 7 * win_nodes(b.cap) * sizeof(float);
     Batch bq;
     dim3 g;
+    TileClear dtc = tc;  // the clear rides on the elastic kernel when it has one, else on k_elastic_defer
     if (elastic == 0) {
         if (b.queue) {
-            g = range_grid(b, bq, TileClear{}, k_p2g_adj_blk<false, true, 1>, kGat, lds, 1);
-            hipLaunchKernelGGL((k_p2g_adj_blk<false, true, 1>), g, dim3(kGat), lds, s, bq, io);
-            g = range_grid(b, bq, TileClear{}, k_elastic_adj<true>, kGat, 0, 1);
-            hipLaunchKernelGGL(k_elastic_adj<true>, g, dim3(kGat), 0, s, bq, io);
+            g = range_grid(b, bq, TileClear{}, k_p2g_adj_blk<false, true, 3>, kGat, lds, 1);
+            hipLaunchKernelGGL((k_p2g_adj_blk<false, true, 3>), g, dim3(kGat), lds, s, bq, io);
+            g = range_grid(b, bq, tc, k_elastic_adj<true>, kGat, 0, 1);
+            hipLaunchKernelGGL(k_elastic_adj<true>, g, dim3(kGat), 0, s, bq, io, tc);
         } else {
-            hipLaunchKernelGGL((k_p2g_adj_blk<false, false, 1>), dim3(b.nranges), dim3(kGat), lds, s, b, io);
-            hipLaunchKernelGGL(k_elastic_adj<false>, dim3(b.nranges), dim3(kGat), 0, s, b, io);
+            hipLaunchKernelGGL((k_p2g_adj_blk<false, false, 3>), dim3(b.nranges), dim3(kGat), lds, s, b, io);
+            hipLaunchKernelGGL(k_elastic_adj<false>, dim3(b.nranges + clear_blocks(b, tc)), dim3(kGat), 0, s, b, io, tc);
         }
+        dtc = TileClear{};
     } else if (elastic == 1) {
         if (b.queue) {
             g = range_grid(b, bq, TileClear{}, k_p2g_adj_blk<true, true, 1>, kGat, lds, 1);
@@ -1860,7 +1906,7 @@ void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear&
             hipLaunchKernelGGL((k_p2g_adj_blk<true, false, 3>), dim3(b.nranges), dim3(kGat), lds, s, b, io);
         }
     }
-    hipLaunchKernelGGL(k_elastic_defer, dim3(kDeferBlocks + clear_blocks(b, tc)), dim3(kGat), 0, s, b, io, tc, kDeferBlocks);
+    hipLaunchKernelGGL(k_elastic_defer, dim3(kDeferBlocks + clear_blocks(b, dtc)), dim3(kGat), 0, s, b, io, dtc, kDeferBlocks);
 }
 
 void launch_clear_tiles(const Batch& b, float* grid, unsigned* flags, hipStream_t s)

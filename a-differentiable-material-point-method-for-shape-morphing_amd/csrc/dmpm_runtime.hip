@@ -909,6 +909,11 @@ int forward_segments(dmpm_ctx* c, int from)
     const int s0_last = c->seg_last * K;
     const int tl = c->seg_last & 1;
     const int s_from = from / K;
+    // the segment before the last goes to the other tape (its backward steps
+    // then need no recompute) when this pass runs it
+    const int sp = c->seg_last - 1;
+    const bool tape_prev = sp >= s_from;
+    const int s0_prev = tape_prev ? sp * K : s0_last;
     std::vector<char> sort_at(c->seg_last + 2, 0);
     bool any_sort = false;
     if (c->seg_resort)
@@ -917,14 +922,19 @@ int forward_segments(dmpm_ctx* c, int from)
                 sort_at[s] = 1;
                 any_sort = true;
             }
-    // checkpoints and the last segment go to the tape, other layers to the rolling slots
+    // checkpoints and the last two segments go to the tapes, other layers to the rolling slots
     auto lay = [&](int j) {
         if (j % K == 0 || j / K == c->seg_last) return tape_slot<float>(c, tl, j);
+        if (j >= s0_prev) return tape_slot<float>(c, tl ^ 1, j);
         return slot<float>(c, c->roll_base + (j & 1));
     };
-    auto grid_at = [&](int k) { return (k >= s0_last) ? tape_grid<float>(c, tl, k) : wgrid_of<float>(c, k); };
-    // steps before the last segment are recomputed for the backward pass: their affine matrices are not read
-    auto keep = [&](int i) { return i >= s0_last; };
+    auto grid_at = [&](int k) {
+        if (k >= s0_last) return tape_grid<float>(c, tl, k);
+        if (k >= s0_prev) return tape_grid<float>(c, tl ^ 1, k);
+        return wgrid_of<float>(c, k);
+    };
+    // steps of earlier segments are recomputed for the backward pass: their affine matrices are not read
+    auto keep = [&](int i) { return i >= s0_prev; };
     int err = DMPM_OK;
     auto pass = [&] {
         zero_wstats(c);
@@ -934,7 +944,9 @@ int forward_segments(dmpm_ctx* c, int from)
             while (e < c->seg_last && !sort_at[e + 1] && !boundary_permuted(c, e + 1)) ++e;
             const int a = s * K, b = std::min((e + 1) * K, c->L - 1);
             const bool perm_b = b < c->L - 1;  // the boundary after e changes the order
-            float* scratch = slot<float>(c, c->roll_base + (b & 1));
+            // a taped segment before the last keeps that boundary layer, in its own
+            // order, in its tape's boundary slot (as a recompute leaves it)
+            float* scratch = (tape_prev && e == sp) ? slot<float>(c, c->tape_base[tl ^ 1] + K) : slot<float>(c, c->roll_base + (b & 1));
             {
                 BatchScope bs(c, s);
                 forward_seq<float>(c, a, b, [&](int j) { return (j == b && perm_b) ? scratch : lay(j); }, grid_at, keep);
@@ -955,6 +967,14 @@ int forward_segments(dmpm_ctx* c, int from)
     prepare_bidx(c);
     const int rc = run_graph(c, from, pass);
     return rc ? rc : err;
+}
+
+// the segment before the last, held by the other tape after a checkpoint
+// forward pass from `from` (that pass ran it), or -1
+int prev_taped(const dmpm_ctx* c, int from)
+{
+    const int sp = c->seg_last - 1;
+    return sp >= 0 && sp >= from / std::max(c->K, 1) ? sp : -1;
 }
 
 // -------------------------------------------------------------- passes -----
@@ -984,17 +1004,24 @@ int forward_pass(dmpm_ctx* c, int start)
         from = (from / c->K) * c->K;  // restart from a checkpoint so the tape holds whole segments
         const int s0_last = c->seg_last * c->K;
         const int tl = c->seg_last & 1;  // the last segment's tape
+        const int sp = prev_taped(c, from);  // the segment before it, in the other tape
+        const int s0_prev = sp >= 0 ? sp * c->K : s0_last;
         auto pass = [&] {
             zero_wstats(c);
-            // checkpoints and the last segment go to the tape, other layers to the rolling slots
+            // checkpoints and the last two segments go to the tapes, other layers to the rolling slots
             auto lay = [&](int j) {
                 if (j % c->K == 0 || j / c->K == c->seg_last) return tape_slot<Real>(c, tl, j);
+                if (j >= s0_prev) return tape_slot<Real>(c, tl ^ 1, j);
                 return slot<Real>(c, c->roll_base + (j & 1));
             };
-            auto grid_at = [&](int k) { return (k >= s0_last) ? tape_grid<Real>(c, tl, k) : wgrid_of<Real>(c, k); };
-            // steps before the last segment are recomputed for the backward pass:
+            auto grid_at = [&](int k) {
+                if (k >= s0_last) return tape_grid<Real>(c, tl, k);
+                if (k >= s0_prev) return tape_grid<Real>(c, tl ^ 1, k);
+                return wgrid_of<Real>(c, k);
+            };
+            // steps of earlier segments are recomputed for the backward pass:
             // their affine matrices are not read
-            forward_seq<Real>(c, from, c->L - 1, lay, grid_at, [&](int i) { return i >= s0_last; });
+            forward_seq<Real>(c, from, c->L - 1, lay, grid_at, [&](int i) { return i >= s0_prev; });
         };
         if (c->f32 && c->nstat > 1) {
             if constexpr (std::is_same<Real, float>::value) rc = forward_segments(c, from);
@@ -1003,7 +1030,7 @@ int forward_pass(dmpm_ctx* c, int start)
         }
         if (rc) return rc;
         c->tape_hold[tl] = c->seg_last;
-        c->tape_hold[tl ^ 1] = -1;
+        c->tape_hold[tl ^ 1] = sp;
     }
     HIP_TRY(hipGetLastError());
     c->fwd_valid = c->L - 1;
