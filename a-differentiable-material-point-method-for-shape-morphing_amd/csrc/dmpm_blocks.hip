@@ -22,21 +22,21 @@
 // P2G apply the grid update (and its transpose) to each node as they load it,
 // so no node-parallel kernel runs between the particle kernels.
 //
-// Scatter accumulation is fixed point in 64-bit LDS words: LDS float atomics
+// Scatter accumulation is fixed point in LDS integer words: LDS float atomics
 // are serialised lane by lane on gfx950 (ds_add_f32 ~0.3 lane-ops/clk/CU
-// against ~9 for ds_add_u64).  Each channel c of a workgroup of nP particles
-// gets a power-of-two scale s_c with s_c * max_p B_pc <= 2^E, where B_pc
-// bounds |val| for every contribution w * val of particle p to channel c;
-// each contribution is rounded to an integer multiple of 1/s_c.
-//   One channel per word (P2G): E = 30, sums in int64.
-//   Two channels per word (transposed G2P, half the LDS atomics): E is the
-//   largest exponent with 2^E nP (2/3)^3 + nP/2 < 2^31 (a node receives at
-//   most nP contributions, each of weight <= (2/3)^3), so every node sum of
-//   every channel fits an int32.  Channel 2q is added in the low 32 bits of
-//   word q and channel 2q+1 in the high 32 bits (the low word's sign borrowed
-//   from the high word): the 64-bit sum is hi * 2^32 + lo and both halves
-//   come back exactly, at a resolution 2^(E-30) coarser than one channel per
-//   word (E = 23 for 512 particles).
+// against ~9 for ds_add_u64 and ~12 for ds_add_u32).  Each channel c of a
+// workgroup of nP particles gets a power-of-two scale s_c with
+// s_c * max_p B_pc <= 2^E, where B_pc bounds |val| for every contribution
+// w * val of particle p to channel c; each contribution is rounded to an
+// integer multiple of 1/s_c.
+//   64-bit words (P2G): E = 30, sums in int64 (a contribution is a rounded
+//   int32, sign-extended).
+//   32-bit words (transposed G2P, I32): E is the largest exponent with
+//   2^E nP (2/3)^3 + nP/2 < 2^31 (a node receives at most nP contributions,
+//   each of weight <= (2/3)^3), so every node sum of every channel fits an
+//   int32 and the wrapping 32-bit adds of the rounded two's complement
+//   contributions are exact, at a resolution 2^(E-30) coarser than the
+//   64-bit words (E = 23 for 512 particles); no sign extension per add.
 // Window sums are thus independent of the order of the additions; the flush
 // converts them back to float for one HBM atomic per node and channel.
 //
@@ -263,15 +263,15 @@ __device__ __forceinline__ void wave_max(float v[NC])
 }
 
 // power-of-two scales from the waves' bounds red[c][wave] (one thread), with
-// s_c * max <= 2^E: E = 30 for one channel per word, else the largest
-// exponent with 2^E nP (2/3)^3 + nP/2 < 2^31 (capped at 30)
-template <int NC, bool PACK>
+// s_c * max <= 2^E: E = 30 for 64-bit words, for 32-bit words (I32) the
+// largest exponent with 2^E nP (2/3)^3 + nP/2 < 2^31 (capped at 30)
+template <int NC, bool I32>
 __device__ Scales<NC> bound_scales(const float (*red)[kMaxWaves], int nw, int nP)
 {
     Scales<NC> S;
     S.ok = true;
     int E = 30;
-    if (PACK) {
+    if (I32) {
         // (2/3)^3 rounded up, so that float weights cannot exceed it
         E = (int)floor(log2((2147483647.0 - 0.5 * nP) / (0.29630 * (nP > 1 ? nP : 1))));
         if (E > 30) E = 30;
@@ -298,7 +298,7 @@ __device__ Scales<NC> bound_scales(const float (*red)[kMaxWaves], int nw, int nP
 }
 
 // block-wide max of NC per-thread contribution bounds -> scales (all threads)
-template <int NC, bool PACK>
+template <int NC, bool I32>
 __device__ Scales<NC> block_scales(const float bound[NC], int nP)
 {
     __shared__ float red[NC][kMaxWaves];
@@ -312,13 +312,13 @@ __device__ Scales<NC> block_scales(const float bound[NC], int nP)
 #pragma unroll
         for (int c = 0; c < NC; ++c) red[c][wid] = v[c];
     __syncthreads();
-    if (threadIdx.x == 0) sc = bound_scales<NC, PACK>(red, (int)(blockDim.x >> 6), nP);
+    if (threadIdx.x == 0) sc = bound_scales<NC, I32>(red, (int)(blockDim.x >> 6), nP);
     __syncthreads();
     return sc;
 }
 
 // The window and the scales of a scatter kernel in one round of barriers.
-template <int NC, bool PACK>
+template <int NC, bool I32>
 __device__ void block_setup(const StepParams& sp, int box[6], const float bound[NC], int nP, int cap, Win& W, Scales<NC>& S)
 {
     __shared__ int redi[6][kMaxWaves];
@@ -341,7 +341,7 @@ __device__ void block_setup(const StepParams& sp, int box[6], const float bound[
     if (threadIdx.x == 0) {
         const int nw = (int)(blockDim.x >> 6);
         win = box_window(sp, redi, nw, cap);
-        sc = bound_scales<NC, PACK>(redf, nw, nP);
+        sc = bound_scales<NC, I32>(redf, nw, nP);
     }
     __syncthreads();
     W = win;
@@ -366,35 +366,32 @@ __device__ __forceinline__ unsigned long long fix32(float y)
     return (unsigned long long)(long long)round_i32(y);
 }
 
-// channels lo, hi (scaled units) rounded to int32 and packed into one word:
-// (hi + borrow) * 2^32 + lo as a 64-bit integer is hi * 2^32 + lo
-__device__ __forceinline__ unsigned long long pack2(float lo, float hi)
+// the integer sum of channel c at window slot `slot` (stride: the channel stride in words), and as a float
+template <bool I32>
+__device__ __forceinline__ long long lds_word(const unsigned long long* lds, int c, int stride, int slot)
 {
-    const int l = round_i32(lo), h = round_i32(hi);
-    return ((unsigned long long)(unsigned)(h + (l >> 31)) << 32) | (unsigned)l;
+    if (I32) return reinterpret_cast<const int*>(lds)[c * stride + slot];
+    return (long long)lds[c * stride + slot];
 }
 
-// a sum T = H * 2^32 + L of packed words with |L|, |H| < 2^31 -> L, H
-__device__ __forceinline__ void unpack2(unsigned long long T, int& lo, int& hi)
+template <bool I32>
+__device__ __forceinline__ float word_value(long long q, float inv)
 {
-    lo = (int)(unsigned)T;
-    hi = (int)(((long long)T - (long long)lo) >> 32);
+    if (I32) return (float)(int)q * inv;
+    return (float)((double)q * (double)inv);
 }
 
 // MASS: channel 0 is the node mass; a node whose window mass is zero adds
 // nothing (its other channels hold only contributions of weights below the
 // fixed-point resolution), so a node left at mass 0 keeps every channel at 0.
-// gm (optional): skip nodes whose mass there is 0 (the transposed G2P skips
-// nodes the forward P2G left empty: their pbar is 0 whatever vbar is).
 // flags (optional): tile flags of the target grid, set for every node
 // flushed (nofs: the instance's first flat node).
-// PACK: two channels per word.
-template <int NC, bool MASS, bool PACK>
+// I32: 32-bit words (lds holds NC int32 channels at the same strides).
+template <int NC, bool MASS, bool I32>
 __device__ __forceinline__ void flush_fixed(const StepParams& sp, const Win& W, const unsigned long long* lds,
-                                            const Scales<NC>& S, float* base, long long cstride, const float* gm = nullptr,
-                                            unsigned* flags = nullptr, long long nofs = 0)
+                                            const Scales<NC>& S, float* base, long long cstride, unsigned* flags = nullptr,
+                                            long long nofs = 0)
 {
-    constexpr int NW = PACK ? (NC + 1) / 2 : NC;
     const int n = W.fixed ? kFW3 : W.vol;
     const int stride = W.fixed ? kFWS : W.vol;
     for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
@@ -410,23 +407,13 @@ __device__ __forceinline__ void flush_fixed(const StepParams& sp, const Win& W, 
         }
         float val[NC];
         bool any = true;
-        if (PACK) {
-            int q[2 * NW];
 #pragma unroll
-            for (int k = 0; k < NW; ++k) unpack2(lds[k * stride + slot], q[2 * k], q[2 * k + 1]);
-            if (MASS) any = q[0] != 0;
-#pragma unroll
-            for (int c = 0; c < NC; ++c) val[c] = (float)q[c] * S.inv[c];
-        } else {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                const long long q = (long long)lds[c * stride + slot];
-                if (MASS && c == 0) any = q != 0;
-                val[c] = (float)((double)q * (double)S.inv[c]);
-            }
+        for (int c = 0; c < NC; ++c) {
+            const long long q = lds_word<I32>(lds, c, stride, slot);
+            if (MASS && c == 0) any = q != 0;
+            val[c] = word_value<I32>(q, S.inv[c]);
         }
         if (!any) continue;
-        if (gm && gm[node] == 0.f) continue;
         if (flags) flags[(nofs + node) >> kTileShift] = 1u;
 #pragma unroll
         for (int c = 0; c < NC; ++c)
@@ -459,15 +446,14 @@ __device__ __forceinline__ void stm(T* base, int field, int S, int p, const M3<T
 
 // Scatter of NC channels over a particle's 64 nodes.  val(i, j, k, out)
 // fills the NC contributions (already in scaled units for the LDS modes).
-// PACK: two channels per LDS word; otherwise channel 0 may be declared
-// non-negative (NONNEG0: the mass).
-// MODE 0: fixed window, 1: general LDS window, 2: HBM float atomics (nodes
-// where gate is 0 are skipped there, as the LDS modes' flush skips them; the
-// tile flags of the nodes written are set when flags is given).
-template <int NC, int MODE, bool PACK, bool NONNEG0 = false, typename ValFn>
+// I32: 32-bit words (NC int32 channels at the strides of the 64-bit layout);
+// otherwise channel 0 may be declared non-negative (NONNEG0: the mass).
+// MODE 0: fixed window, 1: general LDS window, 2: HBM float atomics (the tile
+// flags of the nodes written are set when flags is given).
+template <int NC, int MODE, bool I32, bool NONNEG0 = false, typename ValFn>
 __device__ __forceinline__ void scatter64(const StepParams& sp, const Win& W, const Axis<float> st[3],
-                                          unsigned long long* lds, float* hbm, ValFn&& val, const float* gate = nullptr,
-                                          unsigned* flags = nullptr, long long nofs = 0)
+                                          unsigned long long* lds, float* hbm, ValFn&& val, unsigned* flags = nullptr,
+                                          long long nofs = 0)
 {
     const int slot = MODE == 0 ? fixed_slot(W, st) : 0;
 #pragma unroll
@@ -487,16 +473,16 @@ __device__ __forceinline__ void scatter64(const StepParams& sp, const Win& W, co
                 if (MODE != 2) {
                     const int o = MODE == 0 ? slot + i * kSI + j * kSJ + kk : win_local(W, gi, gj, gk);
                     const int stride = MODE == 0 ? kFWS : W.vol;
-                    if (PACK) {
+                    if (I32) {
+                        unsigned* l32 = reinterpret_cast<unsigned*>(lds);
 #pragma unroll
-                        for (int c = 0; c < NC; c += 2) atomicAdd(&lds[(c / 2) * stride + o], pack2(v[c], c + 1 < NC ? v[c + 1] : 0.f));
+                        for (int c = 0; c < NC; ++c) atomicAdd(&l32[c * stride + o], (unsigned)round_i32(v[c]));
                     } else {
 #pragma unroll
                         for (int c = 0; c < NC; ++c) atomicAdd(&lds[c * stride + o], c == 0 && NONNEG0 ? fix32<true>(v[c]) : fix32(v[c]));
                     }
                 } else {
                     const long long node = ((long long)gi * sp.dim[1] + gj) * sp.dim[2] + gk;
-                    if (gate && gate[node] == 0.f) continue;
                     if (flags) flags[(nofs + node) >> kTileShift] = 1u;
 #pragma unroll
                     for (int c = 0; c < NC; ++c) unsafeAtomicAdd(hbm + c * sp.G + node, v[c]);
@@ -737,7 +723,10 @@ constexpr int kP2G = 256;
 // The scatter of a range's prepared particles q0 (valid v0), q1 (v1) into
 // io.grid (already offset to the instance), from their base box and bounds:
 // window and scales, LDS accumulation, flush.  Starts with a barrier, so the
-// caller may have used the LDS before.
+// caller may have used the LDS before.  ZEROED: the caller zeroed the first
+// 4 kFWS words of lds64 before that barrier (windows then hold at most kFWS
+// nodes: bt.cap <= kFWS).
+template <bool ZEROED = false>
 __device__ __forceinline__ void p2g_scatter(const Batch& bt, const StepParams& sp, const StepIO<float>& io, long long nofs, int nP,
                                             P2GPart& q0, P2GPart& q1, bool v0, bool v1, const float bound[4], int box[6],
                                             unsigned long long* lds64)
@@ -748,11 +737,13 @@ __device__ __forceinline__ void p2g_scatter(const Batch& bt, const StepParams& s
     block_setup<4, false>(sp, box, bound, nP, bt.cap, W, SC);
     const int mode = (W.fixed && SC.ok) ? 0 : ((W.lds && SC.ok) ? 1 : 2);
     if (threadIdx.x == 0 && bt.wstats) atomicAdd(bt.wstats + mode, 1ull);
-    if (mode == 0)
-        for (int i = threadIdx.x; i < 4 * kFWS; i += blockDim.x) lds64[i] = 0ull;
-    else if (mode == 1)
-        for (int i = threadIdx.x; i < 4 * W.vol; i += blockDim.x) lds64[i] = 0ull;
-    __syncthreads();
+    if (!ZEROED) {
+        if (mode == 0)
+            for (int i = threadIdx.x; i < 4 * kFWS; i += blockDim.x) lds64[i] = 0ull;
+        else if (mode == 1)
+            for (int i = threadIdx.x; i < 4 * W.vol; i += blockDim.x) lds64[i] = 0ull;
+        __syncthreads();
+    }
     if (mode != 2) {  // scaled units
         p2g_scale(q0, SC);
         p2g_scale(q1, SC);
@@ -803,12 +794,12 @@ __device__ __forceinline__ void p2g_scatter(const Batch& bt, const StepParams& s
             };
             if (mode == 0) scatter64<4, 0, false, true>(sp, W, st, lds64, io.grid, val);
             else if (mode == 1) scatter64<4, 1, false, true>(sp, W, st, lds64, io.grid, val);
-            else scatter64<4, 2, false>(sp, W, st, lds64, io.grid, val, nullptr, io.gflags, nofs);
+            else scatter64<4, 2, false>(sp, W, st, lds64, io.grid, val, io.gflags, nofs);
         }
     }
     if (mode != 2) {
         __syncthreads();
-        flush_fixed<4, true, false>(sp, W, lds64, SC, io.grid, sp.G, nullptr, io.gflags, nofs);
+        flush_fixed<4, true, false>(sp, W, lds64, SC, io.grid, sp.G, io.gflags, nofs);
     }
 }
 
@@ -902,13 +893,14 @@ struct NodeVel {
     }
 };
 
-// G2P and P_op_2 (ForwardSimulation.cpp:60-124) of particle p at x: its next
-// state from the window W of node velocities in lds
-__device__ __forceinline__ void g2p_particle(const StepParams& sp, const StepIO<float>& io, const Win& W, const float* lds,
-                                             const NodeVel& ld, int p, int h, const float x[3], float x1[3], float v1[3], M3<float>& C1,
-                                             M3<float>& F1)
+// G2P and P_op_2 (ForwardSimulation.cpp:60-124) of a particle at x: its next
+// state from the window W of node velocities in lds; fdf(F, dF) gives its
+// deformation gradient and control after the gather (a load, or values the
+// caller loaded earlier)
+template <typename FdF>
+__device__ __forceinline__ void g2p_particle(const StepParams& sp, const Win& W, const float* lds, const NodeVel& ld,
+                                             const float x[3], FdF&& fdf, float x1[3], float v1[3], M3<float>& C1, M3<float>& F1)
 {
-    const int S = sp.stride;
     const float dt = float(sp.dt), s = float(sp.smoothing);
     const float Dd = float(3.0 * sp.inv_dx);
     Axis<float> st[3];
@@ -956,8 +948,8 @@ __device__ __forceinline__ void g2p_particle(const StepParams& sp, const StepIO<
     if (W.fixed) gather64<3, 0>(sp, W, st, lds, ld, acc, row);
     else if (W.lds) gather64<3, 1>(sp, W, st, lds, ld, acc, row);
     else gather64<3, 2>(sp, W, st, lds, ld, acc, row);
-    const M3<float> F = ldm(io.cur, FF, S, p);
-    const M3<float> dF = ldm(io.ctrl, 0, S, h);
+    M3<float> F, dF;
+    fdf(F, dF);
     M3<float> Fc, IC;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
@@ -1027,7 +1019,12 @@ __device__ __forceinline__ void g2p_range(const Batch& bt, StepIO<float> io, con
         for (int a = 0; a < 3; ++a) x[a] = h ? xs[1][a] : xs[0][a];
         float x1[3], v1[3];
         M3<float> C1, F1;
-        g2p_particle(sp, io, W, lds, ld, p, home_of(bt, p), x, x1, v1, C1, F1);
+        const int hp = home_of(bt, p);
+        auto fdf = [&](M3<float>& F, M3<float>& dF) {
+            F = ldm(io.cur, FF, sp.stride, p);
+            dF = ldm(io.ctrl, 0, sp.stride, hp);
+        };
+        g2p_particle(sp, W, lds, ld, x, fdf, x1, v1, C1, F1);
         store_state(sp, io.next, p, x1, v1, C1, F1);
     }
 }
@@ -1048,6 +1045,9 @@ __global__ __launch_bounds__(kGat) void k_g2p_blk(Batch bt, StepIO<float> io, Ti
 // others.  The grid of step i must be clear; tc clears step i + 1's.  The
 // results equal those of k_g2p_blk then k_p2g_blk: the state written is the
 // state scattered, and the scatter sums are independent of their order.
+// The LDS holds the scatter's window (4 kFWS words, zeroed as the gather's
+// window loads) and after it the gather's (3 kFWS floats): windows hold at
+// most kFWS nodes (bt.cap <= kFWS).
 static_assert(kP2G == kGat, "the fused step maps particles to threads as both kernels do");
 __device__ __forceinline__ void step_range(const Batch& bt, StepIO<float> gio, StepIO<float> pio, const int4 rg,
                                            unsigned long long* lds64)
@@ -1058,8 +1058,10 @@ __device__ __forceinline__ void step_range(const Batch& bt, StepIO<float> gio, S
     pio.grid += 4 * nofs;
     const int S = sp.stride;
     const NodeVel ld(sp, gio.grid);
+    float* glds = reinterpret_cast<float*>(lds64 + 4 * kFWS);
+    for (int i = threadIdx.x; i < 4 * kFWS; i += blockDim.x) lds64[i] = 0ull;
     float xs[2][3];
-    const Win W = g2p_window(bt, sp, gio, rg, xs, reinterpret_cast<float*>(lds64), ld);
+    const Win W = g2p_window(bt, sp, gio, rg, xs, glds, ld);
     const int p0 = rg.x + (int)threadIdx.x, p1 = p0 + kP2G;
     const bool v0 = p0 < rg.y, v1 = p1 < rg.y;
     P2GPart q0{}, q1{};
@@ -1069,7 +1071,11 @@ __device__ __forceinline__ void step_range(const Batch& bt, StepIO<float> gio, S
         float xn[3], vn[3];
         M3<float> Cn, Fn;
         const int h = home_of(bt, p);
-        g2p_particle(sp, gio, W, reinterpret_cast<const float*>(lds64), ld, p, h, x, xn, vn, Cn, Fn);
+        auto fdf = [&](M3<float>& F, M3<float>& dF_) {
+            F = ldm(gio.cur, FF, S, p);
+            dF_ = ldm(gio.ctrl, 0, S, h);
+        };
+        g2p_particle(sp, W, glds, ld, x, fdf, xn, vn, Cn, Fn);
         store_state(sp, gio.next, p, xn, vn, Cn, Fn);
         const M3<float> dF = ldm(pio.ctrl, 0, S, h);
         const double vol = pio.cst[KVOL * S + h];
@@ -1088,8 +1094,8 @@ __device__ __forceinline__ void step_range(const Batch& bt, StepIO<float> gio, S
     };
     if (v0) one(p0, xs[0], q0);
     if (v1) one(p1, xs[1], q1);
-    // p2g_scatter's first barrier orders every gather above before the LDS is reused
-    p2g_scatter(bt, sp, pio, nofs, rg.y - rg.x, q0, q1, v0, v1, bound, box, lds64);
+    // the scatter's window was zeroed before g2p_window's barriers
+    p2g_scatter<true>(bt, sp, pio, nofs, rg.y - rg.x, q0, q1, v0, v1, bound, box, lds64);
 }
 
 template <bool PERSIST>
@@ -1101,9 +1107,10 @@ __global__ __launch_bounds__(kP2G, 4) void k_step_blk(Batch bt, StepIO<float> gi
 
 // -------------------------------------------------------- transposed G2P ---
 // P_op_2^T + G2P^T: vt = v' + dt x', Ct = C' + (1-s) dt F' Fc^T (kept in tmp),
-// vbar_i += w (vt + D Ct d), into the adjoint grid (3 channels; nodes of zero
-// forward mass are skipped: their pbar is 0 whatever vbar is).  Two particles
-// per thread, paired as in P2G.
+// vbar_i += w (vt + D Ct d), into the adjoint grid (3 channels, 32-bit
+// words; a node of zero forward mass may receive vbar: the transposed P2G
+// gives it pbar = mbar = 0 whatever vbar is).  Two particles per thread,
+// paired as in P2G.
 struct G2PAdjPart {
     float x[3], vt[3];
     M3<float> Ct;
@@ -1177,9 +1184,9 @@ __device__ __forceinline__ void g2p_adj_range(const Batch& bt, AdjIO<float> io, 
     block_setup<3, true>(sp, box, bound, rg.y - rg.x, bt.cap, W, SC);
     const int mode = (W.fixed && SC.ok) ? 0 : ((W.lds && SC.ok) ? 1 : 2);
     if (mode == 0)
-        for (int i = threadIdx.x; i < 2 * kFWS; i += blockDim.x) lds64[i] = 0ull;
+        for (int i = threadIdx.x; i < 3 * kFWS; i += blockDim.x) reinterpret_cast<unsigned*>(lds64)[i] = 0u;
     else if (mode == 1)
-        for (int i = threadIdx.x; i < 2 * W.vol; i += blockDim.x) lds64[i] = 0ull;
+        for (int i = threadIdx.x; i < 3 * W.vol; i += blockDim.x) reinterpret_cast<unsigned*>(lds64)[i] = 0u;
     __syncthreads();
     if (mode != 2) {
         g2p_adj_scale(q0, SC);
@@ -1189,7 +1196,6 @@ __device__ __forceinline__ void g2p_adj_range(const Batch& bt, AdjIO<float> io, 
     stencil_f(sp, q0.x, s0, false);
     stencil_f(sp, q1.x, s1, false);
     const bool same = v1 && s0[0].base == s1[0].base && s0[1].base == s1[1].base && s0[2].base == s1[2].base;
-    const float* gate = io.grid + 4 * nofs;
     if (mode == 0 && __all(!v1 || same)) {
         if (v0) {
             float a0[4], a1[4], a2[4], b0[4], b1[4], b2[4];
@@ -1230,12 +1236,12 @@ __device__ __forceinline__ void g2p_adj_range(const Batch& bt, AdjIO<float> io, 
             };
             if (mode == 0) scatter64<3, 0, true>(sp, W, st, lds64, io.agrid, val);
             else if (mode == 1) scatter64<3, 1, true>(sp, W, st, lds64, io.agrid, val);
-            else scatter64<3, 2, true>(sp, W, st, lds64, io.agrid, val, gate, io.aflags, nofs);
+            else scatter64<3, 2, true>(sp, W, st, lds64, io.agrid, val, io.aflags, nofs);
         }
     }
     if (mode != 2) {
         __syncthreads();
-        flush_fixed<3, false, true>(sp, W, lds64, SC, io.agrid, sp.G, gate, io.aflags, nofs);
+        flush_fixed<3, false, true>(sp, W, lds64, SC, io.agrid, sp.G, io.aflags, nofs);
     }
 }
 
@@ -1760,7 +1766,8 @@ __global__ __launch_bounds__(kGat) void k_loss_particles_b(Batch bt, const float
 }
 
 // ------------------------------------------------------------ launchers ----
-// LDS: max(cap, fixed-window slots) nodes of 64-bit words (scatters: NC, or (NC + 1) / 2 packed) or NC floats (gathers)
+// LDS: max(cap, fixed-window slots) nodes of NC 64-bit words (P2G scatters), NC 32-bit words (the transposed
+// G2P) or NC floats (gathers)
 static size_t win_nodes(int cap) { return (size_t)(cap > kFWS ? cap : kFWS); }
 
 
@@ -1824,14 +1831,16 @@ void launch_step_blk(const Batch& b, const StepIO<float>& gio, const StepIO<floa
         if (tc.grid) launch_clear_tiles(b, tc.grid, tc.flags, s);
         return;
     }
-    // the gather's window (3 float channels) and then the scatter's (4 64-bit words) share the LDS
-    const size_t lds = 4 * win_nodes(b.cap) * sizeof(unsigned long long);
+    // the scatter's window (4 64-bit words) and the gather's (3 floats) of at most kFWS nodes each
+    const size_t lds = kFWS * (4 * sizeof(unsigned long long) + 3 * sizeof(float));
+    Batch bc = b;
+    if (bc.cap > kFWS) bc.cap = kFWS;
     Batch bq;
     if (b.queue) {
-        const dim3 g = range_grid(b, bq, tc, k_step_blk<true>, kP2G, lds, 0);
+        const dim3 g = range_grid(bc, bq, tc, k_step_blk<true>, kP2G, lds, 0);
         hipLaunchKernelGGL(k_step_blk<true>, g, dim3(kP2G), lds, s, bq, gio, pio, tc);
     } else {
-        const dim3 g = range_grid(b, bq, tc, k_step_blk<false>, kP2G, lds, 0);
+        const dim3 g = range_grid(bc, bq, tc, k_step_blk<false>, kP2G, lds, 0);
         hipLaunchKernelGGL(k_step_blk<false>, g, dim3(kP2G), lds, s, bq, gio, pio, tc);
     }
 }
@@ -1856,7 +1865,7 @@ void launch_g2p_blk(const Batch& b, const StepIO<float>& io, const TileClear& tc
 void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
 {
     if (b.nranges <= 0) return;
-    const size_t lds = 2 * win_nodes(b.cap) * sizeof(unsigned long long);
+    const size_t lds = 3 * win_nodes(b.cap) * sizeof(unsigned);
     Batch bq;
     if (b.queue) {
         const dim3 g = range_grid(b, bq, TileClear{}, k_g2p_adj_blk<true>, kG2PAdj, lds, 1);
