@@ -357,6 +357,28 @@ StepParams inst_params(const dmpm_ctx* c, int i)
     return sp;
 }
 
+// Default checkpoint stride of `steps` steps: the divisor of steps nearest to
+// sqrt(steps) within a factor 1.5 (the larger of two equally near), else
+// round(sqrt(steps)).  A divisor makes every segment whole, so the last two
+// segments, which the forward pass leaves on the two tapes, hold 2K steps:
+// 50 steps take K = 5 (40 recomputed, 25 layer slots and 10 grids) instead of
+// 7 (42 recomputed, 26 slots and 14 grids).
+int default_stride(int steps)
+{
+    const double r = std::sqrt((double)steps);
+    int best = 0;
+    double bd = 0.0;
+    for (int d = 1; d <= steps; ++d) {
+        if (steps % d || d < r / 1.5 || d > r * 1.5) continue;
+        const double dd = std::fabs(d - r);
+        if (!best || dd <= bd) {
+            best = d;
+            bd = dd;
+        }
+    }
+    return best ? best : (int)std::lround(r);
+}
+
 // Parameters of the single-instance (FP64) path.
 StepParams step_params(const dmpm_ctx* c) { return inst_params(c, 0); }
 
@@ -1780,7 +1802,7 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
     c->G = c->nofs[c->B];
     const int steps = c->L - 1;
     if (cfg->tape_mode == DMPM_TAPE_CHECKPOINT) {
-        int K = cfg->ckpt_stride > 0 ? cfg->ckpt_stride : (int)std::lround(std::sqrt((double)steps));
+        int K = cfg->ckpt_stride > 0 ? cfg->ckpt_stride : default_stride(steps);
         if (K < 1) K = 1;
         if (K > steps) K = steps;
         c->K = K;

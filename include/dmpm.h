@@ -70,7 +70,8 @@ typedef struct dmpm_config {
     double f_ext[3];
     int32_t compat;             /* DMPM_COMPAT_* bits */
     int32_t tape_mode;          /* DMPM_TAPE_* */
-    int32_t ckpt_stride;        /* checkpoint mode: layers between checkpoints, 0 = round(sqrt(L-1)) */
+    int32_t ckpt_stride;        /* checkpoint mode: layers between checkpoints, 0 = the divisor of L-1 nearest
+                                 * sqrt(L-1) within a factor 1.5, else round(sqrt(L-1)) */
     int32_t keep_control_grads; /* 1: keep dL/d(dFc_i) of every layer of the backward pass */
     int32_t device;             /* HIP device ordinal */
     int32_t reserved;
