@@ -101,6 +101,26 @@ def test_elastic_modes_deferred_svd_vs_oracle(mode):
             assert rel(g[i], runs[0][1][i]) < 1e-6, (r, i)
 
 
+def test_deferred_list_longer_than_its_workgroups():
+    """Every particle inverted: about 8.8k particles on the deferred list of each backward step, more
+    than the 16 x 256 threads of the deferred kernel take in one pass (the kernel strides over the
+    list); gradients against the oracle in the split and fused modes."""
+    rng = np.random.default_rng(3)
+    case = make_case(L=3, per_layer_ctrl=True, vel=0.1)
+    n = case.sc.n
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    F = np.broadcast_to(Q @ np.diag([0.9, 1.0, -0.3]) @ Q.T, (n, 3, 3)).copy()
+    f32 = lambda a: np.float32(a).astype(np.float64)  # noqa: E731
+    case.st0 = orc.State(case.st0.x, case.st0.v, case.st0.C, f32(F))
+    ref_loss, _, gd, _ = orc.episode(case.prm, case.st0, case.cs, case.controls, case.target, case.L)
+    for mode in (0, 2):
+        (loss, g, _), = _episode(case, mode)
+        errs = [rel(g[i], gd[i]) for i in range(case.L - 1)]
+        print(f"all inverted, mode {mode}: loss {abs(loss - ref_loss) / abs(ref_loss):.2e} grads {max(errs):.2e}")
+        assert abs(loss - ref_loss) / abs(ref_loss) < 1e-5
+        assert max(errs) < 1e-4
+
+
 def test_elastic_modes_agree():
     """The three modes on one episode (checkpoint tape, persistent launches on and off): gradients
     within 1e-6 of one another."""
