@@ -13,7 +13,7 @@ to alpha (FP32 path); 1e-12 for the norm in FP64.
 import numpy as np
 import pytest
 
-from _scenes import gpu_morph, make_case, pkg, rel
+from _scenes import gpu_morph, make_case, orc, pkg, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -190,3 +190,91 @@ def test_optimize_control_sequence_runs():
     assert np.isfinite(l0) and np.isfinite(l1)
     assert len(hist) >= 3 and all(np.isfinite(hist))
     assert rel(mo.control(0), case.controls[0]) > 0  # the control moved
+
+
+def _host_optimize(case, control_stride, max_gd_iters, max_ls, alpha0, gd_tol, temporal_iters, b1=0.9, b2=0.999,
+                   eps=1e-5):
+    """CompGraph::OptimizeDefGradControlSequence (CompGraph.cpp:280-444) restated on the host over the FP64
+    oracle: every episode, loss and control gradient from oracle.episode, and Descend_Adam then Descend_Lion
+    on one shared momentum per control layer (PointCloud.cpp:83-99,134-162) in float64 with the float32
+    parameters the device takes, a failed trial undone at -alpha (:406-408).  Returns (initial loss, final
+    loss, gradient-norm history, controls, accepted and rejected trials)."""
+    L = case.L
+    ctl = [c.copy() for c in case.controls]
+    mom = [np.zeros_like(c) for c in ctl]
+    vec = [np.zeros_like(c) for c in ctl]
+    f32 = lambda v: np.float64(np.float32(v))  # noqa: E731
+    a1, a2, ae = f32(b1), f32(b2), f32(eps)
+
+    def ep(cf):
+        loss, _, gd, _ = orc.episode(case.prm, case.st0, case.cs, ctl, case.target, L, control_layer=cf)
+        return loss, gd[cf]
+
+    def adam(cf, h, al, t):
+        mom[cf] = a1 * mom[cf] + (1 - a1) * h
+        vec[cf] = a2 * vec[cf] + (1 - a2) * h * h
+        ctl[cf] = ctl[cf] - f32(al) * (mom[cf] / f32(1 - np.float64(a1) ** t)) / (
+            np.sqrt(vec[cf] / f32(1 - np.float64(a2) ** t)) + ae)
+
+    def lion(cf, h, al):
+        mom[cf] = a1 * mom[cf] + (1 - a1) * np.sign(h)
+        ctl[cf] = ctl[cf] - f32(al) * mom[cf]
+
+    loss0, g0 = ep(0)
+    init_norm = np.linalg.norm(g0)
+    hist = []
+    trials = [0, 0]
+    t = 1
+    for _ in range(temporal_iters):
+        for cf in range(0, L - 1, control_stride):
+            alpha = alpha0
+            for _gd in range(max_gd_iters):
+                gd_loss, g = ep(cf)
+                gn = np.linalg.norm(g)
+                hist.append(gn)
+                if gn < gd_tol * init_norm:
+                    break
+                h = g / f32(gn)
+                found = False
+                for _ls in range(max_ls):
+                    adam(cf, h, alpha, t)
+                    lion(cf, h, alpha)
+                    if orc.loss_only(case.prm, case.st0, case.cs, ctl, case.target, L) < gd_loss:
+                        found = True
+                        trials[0] += 1
+                        break
+                    trials[1] += 1
+                    adam(cf, h, -alpha, t)
+                    lion(cf, h, -alpha)
+                    alpha /= 2.0
+                if not found:
+                    break
+                t += 1
+    return loss0, orc.loss_only(case.prm, case.st0, case.cs, ctl, case.target, L), hist, ctl, trials
+
+
+def test_optimize_control_sequence_vs_host_restatement():
+    """The device loop (FP64 path) against its host restatement over the FP64 oracle, every control layer
+    (stride 1), 3 descent iterations each with up to 3 line-search trials at alpha 5e-3 and Adam's eps 1e-2
+    (the trials move dFc by up to about alpha / eps per entry, so that episodes stay in the elastic regime):
+    both accept and reject trials, with the same decisions (the trial losses differ by ~1e-7 relative, the
+    two paths agree to ~1e-12); gradient norms within 1e-9, losses within 1e-10 and every layer's control
+    within 1e-8 of the restatement."""
+    case = make_case(L=4, per_layer_ctrl=True)
+    kw = dict(control_stride=1, max_gd_iters=3, max_ls=3, alpha0=5e-3, gd_tol=1e-3, temporal_iters=1, eps=1e-2)
+    l0, l1, hist, ctl, trials = _host_optimize(case, **kw)
+    assert trials[0] > 0 and trials[1] > 0, trials  # accepted and rejected trials are both exercised
+    mo = gpu_morph(case, precision=pkg.PREC_F64)
+    lines = []
+    d0, d1, dhist = pkg.optimize.optimize_control_sequence(
+        mo, control_stride=1, max_gd_iters=kw["max_gd_iters"], max_line_search_iters=kw["max_ls"],
+        initial_alpha=kw["alpha0"], gd_tol=kw["gd_tol"], temporal_iters=1, epsilon=kw["eps"], log=lines.append)
+    print("\n".join(lines))
+    print(f"host: loss {l0:.12g} -> {l1:.12g}, {len(hist)} gradient norms, trials accepted/rejected {trials}")
+    assert d0 == pytest.approx(l0, rel=1e-10)
+    assert d1 == pytest.approx(l1, rel=1e-10)
+    assert len(dhist) == len(hist)
+    assert np.allclose(dhist, hist, rtol=1e-9, atol=0)
+    for i in range(case.L - 1):
+        assert rel(mo.control(i), ctl[i]) < 1e-8, i
+    mo.close()
