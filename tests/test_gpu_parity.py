@@ -297,7 +297,7 @@ def test_fp32_episode_vs_fp64_oracle():
 
 
 # ------------------------------------------------------------- edge cases --
-def _edge_episode(x, F, prec, seed=7, L=4):
+def _edge_episode(x, F, prec, seed=7, L=4, dt=1e-4):
     """FP64 oracle and the GPU path on one small edge-case episode: returns
     (loss errors, x, F and control-gradient errors) of the GPU path."""
     rng = np.random.default_rng(seed)
@@ -309,7 +309,7 @@ def _edge_episode(x, F, prec, seed=7, L=4):
     if prec == pkg.PREC_F32:  # give the oracle the float inputs the FP32 path stores
         f32 = lambda a: np.float32(a).astype(np.float64)  # noqa: E731
         x, F, m, lam, mu = f32(x), f32(F), f32(m), f32(lam), f32(mu)
-    prm = orc.Params(n=n, dim=sc.dim, dx=sc.dx, dt=1e-4)
+    prm = orc.Params(n=n, dim=sc.dim, dx=sc.dx, dt=dt)
     vol = orc.point_volumes(prm, x, m) * sc.dx ** 3
     v = 0.1 * rng.normal(size=(n, 3))
     dFc = 1e-2 * rng.normal(size=(n, 3, 3))
@@ -319,7 +319,7 @@ def _edge_episode(x, F, prec, seed=7, L=4):
     st = orc.State(x, v, np.zeros((n, 3, 3)), F)
     target = orc.p2g_mass(prm, sc.target_x, sc.target_m)
     controls = [dFc] + [np.zeros_like(dFc)] * (L - 2)
-    mo = pkg.Morph(n, sc.dim, sc.dx, L, dt=1e-4, drag=0.0, smoothing=0.0, precision=prec, keep_control_grads=True)
+    mo = pkg.Morph(n, sc.dim, sc.dx, L, dt=dt, drag=0.0, smoothing=0.0, precision=prec, keep_control_grads=True)
     mo.set_particles(st.x, st.v, st.F, st.C, m, vol, lam, mu)
     mo.set_control(0, dFc)
     mo.set_target_grid(target)
@@ -357,6 +357,58 @@ def test_particles_near_and_outside_grid_fp32():
     el, ex, eF, eg = _edge_episode(*_near_grid_points(), pkg.PREC_F32)
     print(f"fp32 near/outside grid: loss {el:.2e} x {ex:.2e} F {eF:.2e} grad {eg:.2e}")
     assert el < 1e-5 and ex < 1e-6 and eF < 2e-5 and eg < 1e-4
+
+
+# loss, x, F and control-gradient bars of the edge-case episodes: FP64 to rounding, FP32 as for the
+# sphere episodes
+_EDGE_BARS = {pkg.PREC_F64: (1e-12, 1e-12, 1e-12, 1e-10), pkg.PREC_F32: (1e-5, 1e-6, 2e-5, 1e-4)}
+_PRECS = pytest.mark.parametrize("prec", [pkg.PREC_F64, pkg.PREC_F32], ids=["fp64", "fp32"])
+
+
+def _edge_check(name, x, F, prec, dt=1e-4):
+    el, ex, eF, eg = _edge_episode(x, F, prec, dt=dt)
+    print(f"{name} ({'fp32' if prec == pkg.PREC_F32 else 'fp64'}): loss {el:.2e} x {ex:.2e} F {eF:.2e} grad {eg:.2e}")
+    bl, bx, bF, bg = _EDGE_BARS[prec]
+    assert el < bl and ex < bx and eF < bF and eg < bg
+
+
+@_PRECS
+def test_one_particle(prec):
+    """A single particle: one block range of one particle, one 64-node stencil (dt 1e-5: a lone
+    particle's volume is about 9 dx^3, so its own stress is stable over the episode only there)."""
+    _edge_check("one particle", np.array([[0.5, 0.51, 0.49]]), _rand_F(1, 1, 0.01), prec, dt=1e-5)
+
+
+@_PRECS
+def test_coincident_particles(prec):
+    """1 300 particles at one position beside a lattice ball: a block holding three ranges (512, 512,
+    276 particles) whose stencils are all equal, so every LDS and HBM atomic of the scatters meets
+    the same 64 nodes."""
+    sc = pkg.scenes.morph_small()
+    ball = pkg.scenes.lattice_spheres([[0.3, 0.6, 0.5]], [0.08], sc.point_dx)
+    x = np.concatenate([np.tile([[0.55, 0.45, 0.52]], (1300, 1)), ball])
+    _edge_check("coincident", x, _rand_F(len(x), 2, 0.01), prec)
+
+
+@_PRECS
+def test_particles_on_nodes_and_cell_centres(prec):
+    """One particle on each grid node of an 8^3 patch (the stencil's last node then has weight 0) and
+    one at each cell centre of a second patch (weights symmetric about the centre); one particle per
+    cell, so dt 1e-5 keeps the episode stable."""
+    dx = pkg.scenes.morph_small().dx
+    g = np.arange(8, dtype=np.float64)
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    x = np.concatenate([(lat + 10.0) * dx, (lat + [19.5, 10.5, 10.5]) * dx])
+    _edge_check("nodes and centres", x, _rand_F(len(x), 3, 0.01), prec, dt=1e-5)
+
+
+@pytest.mark.parametrize("n", [511, 512, 513, 1025])
+def test_block_range_boundaries_fp32(n):
+    """n particles in one 4^3-cell block of the FP32 path: one range below, at and one past the
+    512-particle range size, and three ranges with a last range of one particle."""
+    rng = np.random.default_rng(n)
+    x = 0.5 + 0.125 * rng.uniform(0.0, 1.0, size=(n, 3))
+    _edge_check(f"{n} in one block", x, _rand_F(n, 4, 0.01), pkg.PREC_F32)
 
 
 @pytest.mark.parametrize("prec", [pkg.PREC_F64, pkg.PREC_F32], ids=["fp64", "fp32"])
