@@ -111,6 +111,7 @@ def make_scene(pkg, name, dt):
 
 OVERLAP = True  # --no-overlap: checkpoint recompute on the main stream (isolated kernel times in a trace)
 ELASTIC = None  # --elastic N: DMPM_OPT_ELASTIC of the FP32 contexts (None: the library's default)
+FUSE = None  # --fuse N: DMPM_OPT_FUSE of the FP32 contexts (None: the library's default)
 
 
 def setup(pkg, sc, vol, dFc, precision, tape, num_layers):
@@ -119,6 +120,8 @@ def setup(pkg, sc, vol, dFc, precision, tape, num_layers):
         mo.set_option(pkg.OPT_OVERLAP, 0)
     if ELASTIC is not None and precision == pkg.PREC_F32:
         mo.set_option(pkg.OPT_ELASTIC, ELASTIC)
+    if FUSE is not None and precision == pkg.PREC_F32:
+        mo.set_option(pkg.OPT_FUSE, FUSE)
     x, v, C, F = sc.state0()
     mo.set_particles(x, v, F, C, sc.m, vol, sc.lam, sc.mu)
     mo.set_control(0, dFc)
@@ -175,6 +178,8 @@ def run_batched(pkg, count, L, dt, warmup, steps):
     mb = pkg.MorphBatch.from_scenes(scs, num_layers=L, dt=dt, tape=pkg.TAPE_CHECKPOINT)
     if ELASTIC is not None:
         mb.set_option(pkg.OPT_ELASTIC, ELASTIC)
+    if FUSE is not None:
+        mb.set_option(pkg.OPT_FUSE, FUSE)
     st = [sc.state0() for sc in scs]
     cat = lambda k: np.concatenate([t[k] for t in st])  # noqa: E731
     m = np.concatenate([sc.m for sc in scs])
@@ -363,6 +368,8 @@ def main():
                     help="recompute checkpoint segments on the main stream, not beside the adjoint (kernel traces)")
     ap.add_argument("--elastic", type=int, default=None, choices=[0, 1, 2],
                     help="where the FP32 transposed P2G's elastic half runs (DMPM_OPT_ELASTIC)")
+    ap.add_argument("--fuse", type=int, default=None, choices=[0, 1],
+                    help="FP32 forward steps: 0 separate G2P and P2G launches, 1 fused (DMPM_OPT_FUSE)")
     ap.add_argument("--gd-iters", type=int, default=0,
                     help="loss-curve mode: this many descent iterations on the layer-0 control (FP32+ckpt and FP64)")
     ap.add_argument("--gd-precision", default="both", choices=["both", "f32", "f64"])
@@ -373,9 +380,10 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
-    global OVERLAP, ELASTIC
+    global OVERLAP, ELASTIC, FUSE
     OVERLAP = not args.no_overlap
     ELASTIC = args.elastic
+    FUSE = args.fuse
 
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py measures one GPU; --gpus must be 1 (sharding across GPUs is described, not implemented)"}))
