@@ -1650,6 +1650,31 @@ __global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io,
     for_ranges<PERSIST>(bt, tc, [&](const int4 rg) { elastic_range(bt, io, rg); });
 }
 
+// The elastic half over every particle slot [0, n), one thread per particle:
+// every lane of a workgroup holds a particle (a block range leaves lanes
+// idle), and a thread issues its particle's loads at once instead of after
+// the previous particle's FP64 work.  poff: the slot prefix of the instances
+// (B + 1 entries; instance i holds slots [poff[i], poff[i + 1])).  Workgroups
+// from nwork on clear tc.
+__global__ __launch_bounds__(kGat) void k_elastic_flat(Batch bt, AdjIO<float> io, TileClear tc, unsigned nwork, int n,
+                                                       const int* poff)
+{
+    if (blockIdx.x >= nwork) {
+        appended_clear(bt, tc, nwork);
+        return;
+    }
+    const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p >= n) return;
+    int lo = 0, hi = bt.B - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (poff[mid] <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    const StepParams sp = bt.sps[lo];
+    if (!elastic_adj_polar(sp, io, p, home_of(bt, p), ldm(io.tmp, 0, sp.stride, p))) io.defer[atomicAdd(io.defer_n, 1)] = make_int2(p, lo);
+}
+
 // The elastic half of the particles deferred to the SVD path (Gb in tmp):
 // workgroups [0, nwork) share the list; the last of them to finish resets
 // its two counters for the next step (every one has read the entry count
@@ -1879,7 +1904,8 @@ void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s)
 // the deferred-list workgroups of k_elastic_defer
 constexpr unsigned kDeferBlocks = 16;
 
-void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear& tc, int elastic, hipStream_t s)
+void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear& tc, int elastic, int n, const int* poff,
+                        hipStream_t s)
 {
     if (b.nranges <= 0) {
         if (tc.grid) launch_clear_tiles(b, tc.grid, tc.flags, s);
@@ -1897,7 +1923,8 @@ void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear&
             hipLaunchKernelGGL(k_elastic_adj<true>, g, dim3(kGat), 0, s, bq, io, tc);
         } else {
             hipLaunchKernelGGL((k_p2g_adj_blk<false, false, 3>), dim3(b.nranges), dim3(kGat), lds, s, b, io);
-            hipLaunchKernelGGL(k_elastic_adj<false>, dim3(b.nranges + clear_blocks(b, tc)), dim3(kGat), 0, s, b, io, tc);
+            const unsigned nwork = (unsigned)((n + kGat - 1) / kGat);
+            hipLaunchKernelGGL(k_elastic_flat, dim3(nwork + clear_blocks(b, tc)), dim3(kGat), 0, s, b, io, tc, nwork, n, poff);
         }
         dtc = TileClear{};
     } else if (elastic == 1) {

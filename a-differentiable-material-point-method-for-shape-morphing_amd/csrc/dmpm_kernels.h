@@ -145,7 +145,10 @@ void launch_g2p_adj_blk(const Batch& b, const AdjIO<float>& io, hipStream_t s);
 // elastic: where the elastic half of the transposed P2G runs (DMPM_OPT_ELASTIC): 0 in a
 // kernel of its own, 1 in the gather kernel, 2 there at 3 waves per SIMD; particles that
 // need the SVD run in k_elastic_defer, whose appended workgroups clear tc
-void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear& tc, int elastic, hipStream_t s);
+// n particle slots, poff the instances' slot prefix (device, B + 1 entries): the elastic half in a kernel of
+// its own runs one thread per particle (k_elastic_flat), or per block range in a persistent launch
+void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear& tc, int elastic, int n, const int* poff,
+                        hipStream_t s);
 // loss over all instances from node masses gm (instance i at nofs[i]):
 // per-instance losses in acc[0..B) (fixed-order sums of workgroup partials
 // kept in part, at most loss_partials(B) doubles); gmax = largest instance node count

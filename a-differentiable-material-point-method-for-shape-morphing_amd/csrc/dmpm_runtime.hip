@@ -515,7 +515,7 @@ inline void grid_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<dou
 inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams& sp, const AdjIO<double>& io) { launch_p2g_adj<double>(sp, io, c->stream); }
 inline void p2g_adj_dispatch(dmpm_ctx* c, const StepParams&, const AdjIO<float>& io, const TileClear& tc)
 {
-    launch_p2g_adj_blk(c->batch, io, tc, c->elastic, c->stream);
+    launch_p2g_adj_blk(c->batch, io, tc, c->elastic, c->N, c->d_poff, c->stream);
 }
 
 // One ForwardTimeStep: cur -> nxt, grid of the step in g.  g_clean: the
