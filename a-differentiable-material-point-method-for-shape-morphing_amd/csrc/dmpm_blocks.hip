@@ -1653,10 +1653,19 @@ __global__ __launch_bounds__(kGat) void k_elastic_adj(Batch bt, AdjIO<float> io,
 // The elastic half over every particle slot [0, n), one thread per particle:
 // every lane of a workgroup holds a particle (a block range leaves lanes
 // idle), and a thread issues its particle's loads at once instead of after
-// the previous particle's FP64 work.  poff: the slot prefix of the instances
-// (B + 1 entries; instance i holds slots [poff[i], poff[i + 1])).  Workgroups
-// from nwork on clear tc.
-__global__ __launch_bounds__(kGat) void k_elastic_flat(Batch bt, AdjIO<float> io, TileClear tc, unsigned nwork, int n,
+// the previous particle's FP64 work.  A particle that needs the SVD takes it
+// here, through a call (no deferred list).  poff: the slot prefix of the
+// instances (B + 1 entries; instance i holds slots [poff[i], poff[i + 1])).
+// Workgroups from nwork on clear tc.
+// The SVD path of one particle out of line: k_elastic_flat's register
+// allocation stays that of its polar path (the call, rare, spills instead).
+__device__ __noinline__ void elastic_svd_call(const StepParams* sps, int inst, AdjIO<float> io, int p, int h)
+{
+    const StepParams sp = sps[inst];
+    elastic_adj_svd(sp, io, p, h, ldm(io.tmp, 0, sp.stride, p));
+}
+
+__global__ __launch_bounds__(kGat, 3) void k_elastic_flat(Batch bt, AdjIO<float> io, TileClear tc, unsigned nwork, int n,
                                                        const int* poff)
 {
     if (blockIdx.x >= nwork) {
@@ -1672,7 +1681,8 @@ __global__ __launch_bounds__(kGat) void k_elastic_flat(Batch bt, AdjIO<float> io
         else hi = mid - 1;
     }
     const StepParams sp = bt.sps[lo];
-    if (!elastic_adj_polar(sp, io, p, home_of(bt, p), ldm(io.tmp, 0, sp.stride, p))) io.defer[atomicAdd(io.defer_n, 1)] = make_int2(p, lo);
+    const int h = home_of(bt, p);
+    if (!elastic_adj_polar(sp, io, p, h, ldm(io.tmp, 0, sp.stride, p))) elastic_svd_call(bt.sps, lo, io, p, h);
 }
 
 // The elastic half of the particles deferred to the SVD path (Gb in tmp):
@@ -1925,6 +1935,7 @@ void launch_p2g_adj_blk(const Batch& b, const AdjIO<float>& io, const TileClear&
             hipLaunchKernelGGL((k_p2g_adj_blk<false, false, 3>), dim3(b.nranges), dim3(kGat), lds, s, b, io);
             const unsigned nwork = (unsigned)((n + kGat - 1) / kGat);
             hipLaunchKernelGGL(k_elastic_flat, dim3(nwork + clear_blocks(b, tc)), dim3(kGat), 0, s, b, io, tc, nwork, n, poff);
+            return;  // k_elastic_flat runs the SVD path itself: nothing is deferred
         }
         dtc = TileClear{};
     } else if (elastic == 1) {

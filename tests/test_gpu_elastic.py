@@ -1,7 +1,9 @@
 """Where the FP32 path runs the elastic half of the transposed P2G (DMPM_OPT_ELASTIC): 0 in a kernel
-of its own after the gather, 1 inside the gather kernel, 2 there with registers for 3 waves per SIMD.
-The polar path is the closed form of H(Fc):Pbar written in the rotation's frame; particles whose Fc
-needs the SVD (inverted or nearly flat elements) are deferred to a small kernel in every mode.
+of its own after the gather (one thread per particle slot; with persistent launches, per block
+range), 1 inside the gather kernel, 2 there with registers for 3 waves per SIMD.
+The polar path is the closed form of H(Fc):Pbar written in the rotation's frame.  Particles whose Fc
+needs the SVD (inverted or nearly flat elements) take it through an out-of-line call in mode 0's
+one-thread-per-particle kernel, and are deferred to a small kernel after the step otherwise.
 
 Bars: FP32 path vs the FP64 oracle as in test_gpu_parity (loss 1e-5, x 1e-6, F 2e-5, control gradient
 1e-4 relative L2); the three modes against one another 1e-6 (the same FP64 arithmetic, compiled into
@@ -85,10 +87,10 @@ def _inverted_case(seed=9, L=4):
 
 @pytest.mark.parametrize("mode", MODES)
 def test_elastic_modes_deferred_svd_vs_oracle(mode):
-    """Inverted and nearly flat elements in every mode: the deferred SVD kernel's gradients against the
-    oracle; a graph-captured pass replayed twice gives the gradients of the first pass each time (the
-    deferred list and its counters reset after every step), within 1e-6 (the float HBM atomics of the
-    window flushes add in any order)."""
+    """Inverted and nearly flat elements in every mode: the SVD path's gradients (inline in mode 0,
+    deferred in modes 1 and 2) against the oracle; a graph-captured pass replayed twice gives the
+    gradients of the first pass each time (the deferred list and its counters reset after every
+    step), within 1e-6 (the float HBM atomics of the window flushes add in any order)."""
     case = _inverted_case()
     runs = _episode(case, mode, graphs=True, reps=3)
     ref_loss, layers, gd, _ = orc.episode(case.prm, case.st0, case.cs, case.controls, case.target, case.L)
@@ -102,9 +104,11 @@ def test_elastic_modes_deferred_svd_vs_oracle(mode):
 
 
 def test_deferred_list_longer_than_its_workgroups():
-    """Every particle inverted: about 8.8k particles on the deferred list of each backward step, more
-    than the 16 x 256 threads of the deferred kernel take in one pass (the kernel strides over the
-    list); gradients against the oracle in the split and fused modes."""
+    """Every particle inverted: in the fused modes about 8.8k particles on the deferred list of each
+    backward step, more than the 16 x 256 threads of the deferred kernel take in one pass (the kernel
+    strides over the list), and every particle through the SVD call of the split mode's kernel;
+    gradients against the oracle in each mode, and the split mode's persistent launch (per block range,
+    with the deferred list)."""
     rng = np.random.default_rng(3)
     case = make_case(L=3, per_layer_ctrl=True, vel=0.1)
     n = case.sc.n
@@ -113,10 +117,11 @@ def test_deferred_list_longer_than_its_workgroups():
     f32 = lambda a: np.float32(a).astype(np.float64)  # noqa: E731
     case.st0 = orc.State(case.st0.x, case.st0.v, case.st0.C, f32(F))
     ref_loss, _, gd, _ = orc.episode(case.prm, case.st0, case.cs, case.controls, case.target, case.L)
-    for mode in (0, 2):
-        (loss, g, _), = _episode(case, mode)
+    for mode, persistent in ((0, False), (0, True), (1, False), (2, False)):
+        (loss, g, _), = _episode(case, mode, persistent=persistent)
         errs = [rel(g[i], gd[i]) for i in range(case.L - 1)]
-        print(f"all inverted, mode {mode}: loss {abs(loss - ref_loss) / abs(ref_loss):.2e} grads {max(errs):.2e}")
+        print(f"all inverted, mode {mode} persistent {persistent}: loss {abs(loss - ref_loss) / abs(ref_loss):.2e} "
+              f"grads {max(errs):.2e}")
         assert abs(loss - ref_loss) / abs(ref_loss) < 1e-5
         assert max(errs) < 1e-4
 
@@ -138,7 +143,8 @@ def test_elastic_modes_agree():
 
 def test_elastic_modes_batch():
     """A batch of instances (one of them with inverted elements) in every mode against the single
-    contexts of the default mode: the deferred list carries each particle's instance."""
+    contexts of the default mode: the split mode's kernel finds each particle's instance from the slot
+    prefix, the deferred list of the fused modes carries it."""
     cases = [make_case(seed=s, L=4, per_layer_ctrl=True) for s in (1, 2)] + [_inverted_case(L=4)]
     singles = [_episode(c, 0)[0] for c in cases]
     cat = lambda f: np.concatenate([f(c) for c in cases])  # noqa: E731
