@@ -215,9 +215,11 @@ int dmpm_chain_episode(dmpm_ctx* ctx);
                                 * on the chip, taking every instance's block ranges from a counter); 0 (default): one
                                 * workgroup per range */
 #define DMPM_OPT_ELASTIC 12    /* FP32: where the elastic half of the transposed P2G (FP64 polar, PK1 and H:dF) runs:
-                                * 0 in a kernel of its own after the gather, 1 in the gather kernel, 2 there with
-                                * registers for 3 waves per SIMD; particles that need the SVD (inverted or nearly
-                                * flat elements) run in a small kernel after it in every mode (the same results) */
+                                * 0 in a kernel of its own after the gather (one thread per particle), 1 in the
+                                * gather kernel, 2 there with registers for 3 waves per SIMD; particles that need
+                                * the SVD (inverted or nearly flat elements) take it inside mode 0's kernel, and in
+                                * a small kernel after the step in modes 1 and 2 and with persistent launches (the
+                                * same results) */
 int dmpm_set_option(dmpm_ctx* ctx, int32_t option, int64_t value);
 int dmpm_get_option(dmpm_ctx* ctx, int32_t option, int64_t* value);
 /* FP32: time a forward and a backward step of layer 0 for every candidate block tile (4, 8), launch form
