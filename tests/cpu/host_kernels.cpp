@@ -79,7 +79,7 @@ void launch_step_blk(const Batch&, const StepIO<float>&, const StepIO<float>&, c
 void launch_p2g_mass_blk(const Batch&, const float*, const float*, float*, hipStream_t) { fp32_only("p2g_mass_blk"); }
 void launch_g2p_blk(const Batch&, const StepIO<float>&, const TileClear&, hipStream_t) { fp32_only("g2p_blk"); }
 void launch_g2p_adj_blk(const Batch&, const AdjIO<float>&, hipStream_t) { fp32_only("g2p_adj_blk"); }
-void launch_p2g_adj_blk(const Batch&, const AdjIO<float>&, const TileClear&, int, hipStream_t) { fp32_only("p2g_adj_blk"); }
+void launch_p2g_adj_blk(const Batch&, const AdjIO<float>&, const TileClear&, int, int, const int*, hipStream_t) { fp32_only("p2g_adj_blk"); }
 void launch_loss_b(const Batch&, const float*, const float*, double*, double*, double*, long long, hipStream_t)
 {
     fp32_only("loss_b");
