@@ -88,6 +88,8 @@ struct dmpm_ctx {
     unsigned* tflags = nullptr;
     long long ntiles = 0;
     size_t grid_slots = 0;
+    // FP32 checkpoint tape: the grid of every checkpoint step (its own buffer after the two tapes' grids)
+    int ck_grids = 0;
     // gradient layers: the last layer's (the loss seed, kept across backward
     // passes as the reference keeps the last layer's dLdx) in gbuf[2], the
     // others ping-pong between gbuf[0] and gbuf[1] by layer parity
@@ -421,6 +423,25 @@ Real* tape_grid(dmpm_ctx* c, int t, int i)
     return static_cast<Real*>(c->grids) + ((size_t)t * c->K + (size_t)(i - (i / c->K) * c->K)) * kGridFields * c->G;
 }
 
+// The grid of checkpoint step i (a multiple of K) when the context keeps one
+// (FP32 checkpoint tape: the forward pass writes it, together with the step's
+// affine matrix in the checkpoint layer, so a recompute of the segment starts
+// with its first G2P), else null.
+template <typename Real>
+Real* ck_grid(dmpm_ctx* c, int i)
+{
+    if (!c->ck_grids || i % c->K != 0) return nullptr;
+    return static_cast<Real*>(c->grids) + (2 * (size_t)c->K + (size_t)(i / c->K)) * kGridFields * c->G;
+}
+
+// grid of step i of segment i / K held in tape t, or its checkpoint grid
+template <typename Real>
+Real* seg_grid(dmpm_ctx* c, int t, int i)
+{
+    Real* g = ck_grid<Real>(c, i);
+    return g ? g : tape_grid<Real>(c, t, i);
+}
+
 template <typename Real>
 Real* tape_layer(dmpm_ctx* c, int j)  // layer j of a segment held by a tape
 {
@@ -433,6 +454,7 @@ template <typename Real>
 Real* grid_of_step(dmpm_ctx* c, int i)
 {
     if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) return static_cast<Real*>(c->grids) + (size_t)i * kGridFields * c->G;
+    if (Real* g = ck_grid<Real>(c, i)) return g;
     const int t = tape_of(c, i / c->K);
     if (t >= 0) return tape_grid<Real>(c, t, i);
     return static_cast<Real*>(c->wgrid);
@@ -681,9 +703,12 @@ Real* wgrid_of(dmpm_ctx* c, int i)
 // consecutive ones distinct buffers; keep(i): the backward pass reads layer i,
 // so P2G caches its affine matrix there; last_g2p: the last step's G2P runs
 // (else only its grid: the layer after it is held already).  FP32 with
-// c->fuse runs the G2P of each step with the P2G of the next as one launch.
+// c->fuse runs the G2P of each step with the P2G of the next as one launch;
+// there p2g0 false starts from grid(i0) and the affine matrix of layer i0 as
+// an earlier pass left them (no P2G of step i0).
 template <typename Real, typename LayFn, typename GridFn, typename KeepFn>
-void forward_seq(dmpm_ctx* c, int i0, int i1, LayFn&& lay, GridFn&& grid, KeepFn&& keep, bool last_g2p = true)
+void forward_seq(dmpm_ctx* c, int i0, int i1, LayFn&& lay, GridFn&& grid, KeepFn&& keep, bool last_g2p = true,
+                 bool p2g0 = true)
 {
     if (i1 <= i0) return;
     if constexpr (std::is_same<Real, float>::value) {
@@ -701,8 +726,12 @@ void forward_seq(dmpm_ctx* c, int i0, int i1, LayFn&& lay, GridFn&& grid, KeepFn
                 if (c->nstat > 1 && c->wstats) b.wstats = c->wstats + 3 * (size_t)std::min(i / c->K, c->nstat - 1);
                 return b;
             };
-            clear_grid(c, grid(i0));
-            timed(c, DMPM_K_P2G, [&] { launch_p2g_blk(batch_at(i0), pio_of(i0), clear_of(i0 + 1), c->stream); });
+            if (p2g0) {
+                clear_grid(c, grid(i0));
+                timed(c, DMPM_K_P2G, [&] { launch_p2g_blk(batch_at(i0), pio_of(i0), clear_of(i0 + 1), c->stream); });
+            } else if (i0 + 1 < i1) {
+                clear_grid(c, grid(i0 + 1));  // the P2G of step i0 would have carried this clear
+            }
             for (int i = i0 + 1; i < i1; ++i) {
                 const StepIO<float> gio{lay(i - 1), lay(i), ctrl_of<float>(c, i - 1), static_cast<const float*>(c->cst), grid(i - 1),
                                         nullptr, nullptr};
@@ -756,8 +785,10 @@ void recompute_segment(dmpm_ctx* c, int seg)
     const bool perm_b = s1 < c->L - 1 && boundary_permuted(c, seg + 1);
     Real* bslot = slot<Real>(c, c->tape_base[t] + c->K);
     BatchScope bs(c, seg);
+    // with checkpoint grids the forward pass left step s0's grid and affine matrix: no P2G of it
     forward_seq<Real>(c, s0, s1, [&](int j) { return (j == s1 && perm_b) ? bslot : tape_slot<Real>(c, t, j); },
-                      [&](int i) { return tape_grid<Real>(c, t, i); }, [](int) { return true; }, s1 % c->K != 0 || perm_b);
+                      [&](int i) { return seg_grid<Real>(c, t, i); }, [](int) { return true; }, s1 % c->K != 0 || perm_b,
+                      c->ck_grids == 0);
 }
 
 // ------------------------------------------------------------ block sort ---
@@ -951,12 +982,14 @@ int forward_segments(dmpm_ctx* c, int from)
         return slot<float>(c, c->roll_base + (j & 1));
     };
     auto grid_at = [&](int k) {
+        if (float* g = ck_grid<float>(c, k)) return g;
         if (k >= s0_last) return tape_grid<float>(c, tl, k);
         if (k >= s0_prev) return tape_grid<float>(c, tl ^ 1, k);
         return wgrid_of<float>(c, k);
     };
-    // steps of earlier segments are recomputed for the backward pass: their affine matrices are not read
-    auto keep = [&](int i) { return i >= s0_prev; };
+    // steps of earlier segments are recomputed for the backward pass: their affine matrices are not
+    // read, but those of checkpoint steps with a grid of their own (the recompute starts after them)
+    auto keep = [&](int i) { return i >= s0_prev || ck_grid<float>(c, i) != nullptr; };
     int err = DMPM_OK;
     auto pass = [&] {
         zero_wstats(c);
@@ -1037,13 +1070,16 @@ int forward_pass(dmpm_ctx* c, int start)
                 return slot<Real>(c, c->roll_base + (j & 1));
             };
             auto grid_at = [&](int k) {
+                if (Real* g = ck_grid<Real>(c, k)) return g;
                 if (k >= s0_last) return tape_grid<Real>(c, tl, k);
                 if (k >= s0_prev) return tape_grid<Real>(c, tl ^ 1, k);
                 return wgrid_of<Real>(c, k);
             };
             // steps of earlier segments are recomputed for the backward pass:
-            // their affine matrices are not read
-            forward_seq<Real>(c, from, c->L - 1, lay, grid_at, [&](int i) { return i >= s0_prev; });
+            // their affine matrices are not read, but those of checkpoint steps
+            // with a grid of their own (the recompute starts after them)
+            forward_seq<Real>(c, from, c->L - 1, lay, grid_at,
+                              [&](int i) { return i >= s0_prev || ck_grid<Real>(c, i) != nullptr; });
         };
         if (c->f32 && c->nstat > 1) {
             if constexpr (std::is_same<Real, float>::value) rc = forward_segments(c, from);
@@ -1190,7 +1226,7 @@ int backward_pass(dmpm_ctx* c, int control)
                 BatchScope bs(c, seg);
                 for (int i = s1 - 1; i >= std::max(s0, control); --i) {
                     const Real* nxt = (i + 1 == s1 && perm_b) ? slot<Real>(c, c->tape_base[t] + c->K) : tape_slot<Real>(c, t, i + 1);
-                    clean = step_backward<Real>(c, i, tape_slot<Real>(c, t, i), nxt, tape_grid<Real>(c, t, i), clean, i > control);
+                    clean = step_backward<Real>(c, i, tape_slot<Real>(c, t, i), nxt, seg_grid<Real>(c, t, i), clean, i > control);
                 }
             }
         });
@@ -1843,7 +1879,9 @@ int create_impl(const dmpm_config* cfg, const std::vector<dmpm_instance>& insts,
             return fail(DMPM_ERR_HIP, std::string("recompute stream: ") + hipGetErrorString(e4));
         }
     }
-    const size_t grid_slots = cfg->tape_mode == DMPM_TAPE_CHECKPOINT ? 2 * (size_t)c->K : (size_t)steps;
+    // FP32 checkpoint tape: the two tapes' grids, then one grid per checkpoint step
+    c->ck_grids = (c->f32 && cfg->tape_mode == DMPM_TAPE_CHECKPOINT) ? c->seg_last + 1 : 0;
+    const size_t grid_slots = cfg->tape_mode == DMPM_TAPE_CHECKPOINT ? 2 * (size_t)c->K + c->ck_grids : (size_t)steps;
     c->grid_slots = grid_slots;
     c->ntiles = c->f32 ? num_tiles((long long)G) : 0;
     const size_t gctrl_layers = cfg->keep_control_grads ? (size_t)steps : 1;

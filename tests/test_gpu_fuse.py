@@ -124,6 +124,35 @@ def test_fused_launch_counts():
         mo.close()
 
 
+def test_checkpoint_recompute_starts_from_checkpoint_grids():
+    """Checkpoint tape, 16 steps in segments of 4 (the forward pass leaves the last two in the tapes):
+    the backward pass recomputes segments 0 and 1 from the grids and affine matrices the forward pass
+    kept for their checkpoint steps, so it launches no P2G, 3 fused steps per segment (the G2P of a
+    checkpoint step with the P2G of the next) and no G2P (each ends on a checkpoint); the gradients
+    equal those of a store-all context (1e-5, the bar of two runs)."""
+    case = make_case(seed=16, L=17, vel=0.3, per_layer_ctrl=True)
+    ref = gpu_morph(case, precision=pkg.PREC_F32, tape=pkg.TAPE_STORE_ALL)
+    mo = gpu_morph(case, precision=pkg.PREC_F32, tape=pkg.TAPE_CHECKPOINT, ckpt_stride=4)
+    try:
+        _, gr = _run(ref, case.L)
+        mo.set_option(pkg.OPT_OVERLAP, 0)  # every launch on the timed stream
+        mo.profile(True)
+        mo.profile_reset()
+        mo.forward_pass(0)
+        t = mo.profile_read()
+        assert t["p2g"][1] == 1 and t["step"][1] == case.L - 2 and t["g2p"][1] == 1, t
+        mo.end_layer_mass_loss()
+        mo.profile_reset()
+        mo.backward_pass(0)
+        t = mo.profile_read()
+        assert t["p2g"][1] == 0 and t["step"][1] == 2 * 3 and t["g2p"][1] == 0, t
+        for i in range(case.L - 1):
+            assert rel(mo.control_grad(i), gr[i]) < 1e-5, i
+    finally:
+        mo.close()
+        ref.close()
+
+
 def test_fused_batch_equals_separate():
     L = 6
     scs = pkg.scenes.batched_instances(4, dt=1e-5)
