@@ -591,8 +591,8 @@ __device__ __forceinline__ void clear_tiles(const Batch& bt, float* grid, unsign
     }
 }
 
-// a kernel's workgroups from `first` on clear tc's grid (k_g2p_blk and
-// k_elastic_adj carry the clear of a grid the next kernels scatter into)
+// a kernel's workgroups from `first` on clear tc's grid (k_g2p_blk and the
+// elastic kernels carry the clear of a grid the next kernels scatter into)
 __device__ __forceinline__ void appended_clear(const Batch& bt, const TileClear& tc, unsigned first)
 {
     const long long nb = (long long)gridDim.x - first;
@@ -1633,8 +1633,9 @@ __global__ __launch_bounds__(kGat, OCC) void k_p2g_adj_blk(Batch bt, AdjIO<float
     for_ranges<PERSIST>(bt, TileClear{}, [&](const int4 rg) { p2g_adj_range<FUSE>(bt, io, rg, lds); });
 }
 
-// Elastic half as a kernel of its own (Gb from tmp), one thread per particle
-// of a range; the SVD path is deferred to k_elastic_defer as in the fused gather.
+// Elastic half as a kernel of its own (Gb from tmp) over the block ranges, for
+// persistent launches (k_elastic_flat otherwise): one thread per particle of
+// a range; the SVD path is deferred to k_elastic_defer as in the fused gather.
 __device__ __forceinline__ void elastic_range(const Batch& bt, const AdjIO<float>& io, const int4 rg)
 {
     const StepParams sp = bt.sps[rg.z];
