@@ -1284,7 +1284,7 @@ __device__ __forceinline__ void elastic_adj_store(const StepParams& sp, const Ad
         }
     }
     stm(io.g0, FF, S, p, gF);
-    stm(io.gctrl, 0, S, h, gdFc);
+    if (io.gctrl) stm(io.gctrl, 0, S, h, gdFc);
 }
 
 // symmetric 3x3, entries (00, 11, 22, 01, 02, 12)

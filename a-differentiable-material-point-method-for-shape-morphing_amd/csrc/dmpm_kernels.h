@@ -51,7 +51,7 @@ struct AdjIO {
     Real* agrid;         // adjoint grid: vbar -> pbar (3G), mbar (G) (FP64); vbar (FP32)
     const Real* g1;      // gradients of layer n+1 (24 fields, same layout as a layer)
     Real* g0;            // gradients of layer n
-    Real* gctrl;         // dL/d(dFc_n) (9 fields)
+    Real* gctrl;         // dL/d(dFc_n) (9 fields), or null: not stored (no reader)
     Real* tmp;           // per particle vt (3) and Ct (9), from the transposed G2P
     const unsigned* gflags;  // FP32: tile flags of grid, else null
     unsigned* aflags;        // FP32: tile flags of agrid (set where the transposed G2P writes)

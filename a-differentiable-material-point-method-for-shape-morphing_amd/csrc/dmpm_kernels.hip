@@ -742,7 +742,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(StepParams sp, AdjIO<Real> i
     for (int i = 0; i < 9; ++i) gC.a[i] = m * Gb.a[i];
     store_mat(io.g0, FC, S, p, gC);
     store_mat(io.g0, FF, S, p, gF);
-    store_mat(io.gctrl, 0, S, p, gdFc);
+    if (io.gctrl) store_mat(io.gctrl, 0, S, p, gdFc);
 }
 
 template <typename Real>

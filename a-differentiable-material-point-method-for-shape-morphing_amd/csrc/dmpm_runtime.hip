@@ -590,7 +590,10 @@ bool step_backward(dmpm_ctx* c, int i, const Real* cur, const Real* nxt, const R
     io.agrid = static_cast<Real*>(c->agrid);
     io.g1 = grads_of<Real>(c, i + 1);
     io.g0 = grads_of<Real>(c, i);
-    io.gctrl = gctrl_of<Real>(c, i);
+    // without keep_control_grads every layer's control gradient goes to one
+    // block and only the last step's (the control layer's, clear_next false)
+    // is read: the steps above it skip that store
+    io.gctrl = (c->cfg.keep_control_grads || !clear_next) ? gctrl_of<Real>(c, i) : nullptr;
     io.tmp = static_cast<Real*>(c->tmp);
     io.gflags = flags_of(c, g);
     io.aflags = flags_of(c, c->agrid);
