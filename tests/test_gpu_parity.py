@@ -277,6 +277,27 @@ def test_checkpoint_control_layer_restart():
     assert rel(b.control_grad(4), gd[4]) < 1e-10
 
 
+@pytest.mark.parametrize("prec,tape", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_single_control_block_holds_the_control_layer(prec, tape):
+    """keep_control_grads off: one block holds the control gradient, written by the backward step of the
+    control layer only (the steps above it skip the store). For control layers 0 and 3 it equals the
+    oracle's (1e-10 FP64, 1e-4 FP32) and every other layer's is refused."""
+    case = make_case(seed=21, L=7, per_layer_ctrl=True)
+    mo = gpu_morph(case, precision=prec, tape=tape, ckpt_stride=2 if tape else 0, keep_control_grads=False)
+    try:
+        mo.forward_pass(0)
+        mo.end_layer_mass_loss()
+        for control in (3, 0):
+            mo.backward_pass(control)
+            _, _, gd, _ = orc.episode(case.prm, case.st0, case.cs, case.controls, case.target, case.L,
+                                      control_layer=control)
+            assert rel(mo.control_grad(control), gd[control]) < (1e-10 if prec == 0 else 1e-4), control
+            with pytest.raises(pkg.DmpmError):
+                mo.control_grad(control + 1)
+    finally:
+        mo.close()
+
+
 # ------------------------------------------------------------------ FP32 ---
 def test_fp32_episode_vs_fp64_oracle():
     case = make_case(L=11)
