@@ -970,16 +970,17 @@ __device__ __forceinline__ void g2p_particle(const StepParams& sp, const Win& W,
     C1 = Cp;
 }
 
+// full false: a layer that only the next step's G2P reads (x and F)
 __device__ __forceinline__ void store_state(const StepParams& sp, float* lay, int p, const float x[3], const float v[3],
-                                            const M3<float>& C, const M3<float>& F)
+                                            const M3<float>& C, const M3<float>& F, bool full = true)
 {
     const int S = sp.stride;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         lay[(long long)(FX + a) * S + p] = x[a];
-        lay[(long long)(FV + a) * S + p] = v[a];
+        if (full) lay[(long long)(FV + a) * S + p] = v[a];
     }
-    stm(lay, FC, S, p, C);
+    if (full) stm(lay, FC, S, p, C);
     stm(lay, FF, S, p, F);
 }
 
@@ -1076,7 +1077,8 @@ __device__ __forceinline__ void step_range(const Batch& bt, StepIO<float> gio, S
             dF_ = ldm(gio.ctrl, 0, S, h);
         };
         g2p_particle(sp, W, glds, ld, x, fdf, xn, vn, Cn, Fn);
-        store_state(sp, gio.next, p, xn, vn, Cn, Fn);
+        // v and C of a lean layer go straight into this step's P2G below
+        store_state(sp, gio.next, p, xn, vn, Cn, Fn, !gio.lean);
         const M3<float> dF = ldm(pio.ctrl, 0, S, h);
         const double vol = pio.cst[KVOL * S + h];
         const double lam = pio.cst[KLAM * S + h], mu = pio.cst[KMU * S + h];

@@ -39,6 +39,7 @@ struct StepIO {
     Real* grid;        // grid m, p -> v (FP64); FP32 keeps m, p (kGridFields * G)
     Real* aux;         // writable alias of cur for derived fields (FP32: A), or null
     unsigned* gflags;  // FP32: tile flags of grid (set where P2G writes), else null
+    int lean = 0;      // FP32 fused step: the layer written is read only for x and F (v, C not stored)
 };
 
 template <typename Real>

@@ -708,10 +708,12 @@ Real* wgrid_of(dmpm_ctx* c, int i)
 // (else only its grid: the layer after it is held already).  FP32 with
 // c->fuse runs the G2P of each step with the P2G of the next as one launch;
 // there p2g0 false starts from grid(i0) and the affine matrix of layer i0 as
-// an earlier pass left them (no P2G of step i0).
+// an earlier pass left them (no P2G of step i0), and the layers j < lean_below
+// that are not checkpoints are written lean (x and F: the next step's G2P is
+// their only reader; the same launch's P2G takes v and C from registers).
 template <typename Real, typename LayFn, typename GridFn, typename KeepFn>
 void forward_seq(dmpm_ctx* c, int i0, int i1, LayFn&& lay, GridFn&& grid, KeepFn&& keep, bool last_g2p = true,
-                 bool p2g0 = true)
+                 bool p2g0 = true, int lean_below = -1)
 {
     if (i1 <= i0) return;
     if constexpr (std::is_same<Real, float>::value) {
@@ -737,7 +739,7 @@ void forward_seq(dmpm_ctx* c, int i0, int i1, LayFn&& lay, GridFn&& grid, KeepFn
             }
             for (int i = i0 + 1; i < i1; ++i) {
                 const StepIO<float> gio{lay(i - 1), lay(i), ctrl_of<float>(c, i - 1), static_cast<const float*>(c->cst), grid(i - 1),
-                                        nullptr, nullptr};
+                                        nullptr, nullptr, i < lean_below && i % c->K != 0};
                 timed(c, DMPM_K_STEP, [&] { launch_step_blk(batch_at(i), gio, pio_of(i), clear_of(i + 1), c->stream); });
             }
             if (last_g2p) {
@@ -1007,7 +1009,8 @@ int forward_segments(dmpm_ctx* c, int from)
             float* scratch = (tape_prev && e == sp) ? slot<float>(c, c->tape_base[tl ^ 1] + K) : slot<float>(c, c->roll_base + (b & 1));
             {
                 BatchScope bs(c, s);
-                forward_seq<float>(c, a, b, [&](int j) { return (j == b && perm_b) ? scratch : lay(j); }, grid_at, keep);
+                forward_seq<float>(c, a, b, [&](int j) { return (j == b && perm_b) ? scratch : lay(j); }, grid_at, keep, true,
+                                   true, s0_prev);
             }
             if (perm_b) {
                 if (sort_at[e + 1] && (err = seg_sort(c, e + 1, scratch))) return;
@@ -1082,7 +1085,7 @@ int forward_pass(dmpm_ctx* c, int start)
             // their affine matrices are not read, but those of checkpoint steps
             // with a grid of their own (the recompute starts after them)
             forward_seq<Real>(c, from, c->L - 1, lay, grid_at,
-                              [&](int i) { return i >= s0_prev || ck_grid<Real>(c, i) != nullptr; });
+                              [&](int i) { return i >= s0_prev || ck_grid<Real>(c, i) != nullptr; }, true, true, s0_prev);
         };
         if (c->f32 && c->nstat > 1) {
             if constexpr (std::is_same<Real, float>::value) rc = forward_segments(c, from);
