@@ -221,6 +221,27 @@ def test_checkpoint_matches_store_all(prec, tol, stride):
     assert maxrel(xb, xa) < (1e-12 if prec == 0 else 1e-6)
 
 
+def test_checkpoint_layer_states_after_a_pass():
+    """FP32 checkpoint tape, 16 steps in segments of 4: after a forward and a backward pass every layer
+    read through the API (checkpoints, taped layers, and layers of recomputed segments, which the
+    forward pass held only in rolling slots of x and F) has the full state (x, v, F, C) of a store-all
+    context (1e-5 max-relative: two runs' float atomics)."""
+    case = make_case(seed=22, L=17, vel=0.3, per_layer_ctrl=True)
+    a = gpu_morph(case, precision=pkg.PREC_F32)
+    b = gpu_morph(case, precision=pkg.PREC_F32, tape=pkg.TAPE_CHECKPOINT, ckpt_stride=4)
+    try:
+        for mo in (a, b):
+            mo.forward_pass(0)
+            mo.end_layer_mass_loss()
+            mo.backward_pass(0)
+        for j in range(case.L):
+            for f, (x, y) in enumerate(zip(b.layer_state(j), a.layer_state(j))):
+                assert maxrel(x, y) < 1e-5, (j, f)
+    finally:
+        a.close()
+        b.close()
+
+
 @pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 2e-5)])
 def test_checkpoint_overlap_and_repeated_backward(prec, tol):
     """The checkpoint tape's backward pass with the recompute of the segment below on a second
