@@ -437,6 +437,16 @@ __device__ __forceinline__ M3<T> ldm(const T* base, int field, int S, int p)
     return M;
 }
 
+// the control dFc of slot h, zero when ctrl is null (the layer's F field holds Fc)
+__device__ __forceinline__ M3<float> ld_ctrl(const float* ctrl, int S, int h)
+{
+    if (ctrl) return ldm(ctrl, 0, S, h);
+    M3<float> z;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) z.a[i] = 0.f;
+    return z;
+}
+
 template <typename T>
 __device__ __forceinline__ void stm(T* base, int field, int S, int p, const M3<T>& M)
 {
@@ -1023,7 +1033,7 @@ __device__ __forceinline__ void g2p_range(const Batch& bt, StepIO<float> io, con
         const int hp = home_of(bt, p);
         auto fdf = [&](M3<float>& F, M3<float>& dF) {
             F = ldm(io.cur, FF, sp.stride, p);
-            dF = ldm(io.ctrl, 0, sp.stride, hp);
+            dF = ld_ctrl(io.ctrl, sp.stride, hp);
         };
         g2p_particle(sp, W, lds, ld, x, fdf, x1, v1, C1, F1);
         store_state(sp, io.next, p, x1, v1, C1, F1);
@@ -1074,12 +1084,16 @@ __device__ __forceinline__ void step_range(const Batch& bt, StepIO<float> gio, S
         const int h = home_of(bt, p);
         auto fdf = [&](M3<float>& F, M3<float>& dF_) {
             F = ldm(gio.cur, FF, S, p);
-            dF_ = ldm(gio.ctrl, 0, S, h);
+            dF_ = ld_ctrl(gio.ctrl, S, h);
         };
         g2p_particle(sp, W, glds, ld, x, fdf, xn, vn, Cn, Fn);
-        // v and C of a lean layer go straight into this step's P2G below
-        store_state(sp, gio.next, p, xn, vn, Cn, Fn, !gio.lean);
         const M3<float> dF = ldm(pio.ctrl, 0, S, h);
+        // v and C of a lean layer go straight into this step's P2G below; an Fc
+        // layer's F field takes Fn + dF, the sum its readers form in FP32
+        M3<float> Fs;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Fs.a[i] = gio.fc ? Fn.a[i] + dF.a[i] : Fn.a[i];
+        store_state(sp, gio.next, p, xn, vn, Cn, Fs, !gio.lean);
         const double vol = pio.cst[KVOL * S + h];
         const double lam = pio.cst[KLAM * S + h], mu = pio.cst[KMU * S + h];
         auto rest = [&](float x_[3], float v_[3], M3<float>& C_, float& m_) {
@@ -1129,7 +1143,7 @@ __device__ __forceinline__ void g2p_adj_prepare(const StepParams& sp, const AdjI
     ld3(io.g1, FX, S, p, gx1);
     ld3(io.g1, FV, S, p, gv1);
     const M3<float> F = ldm(io.cur, FF, S, p);
-    const M3<float> dF = ldm(io.ctrl, 0, S, h);
+    const M3<float> dF = ld_ctrl(io.ctrl, S, h);
     const M3<float> gF1 = ldm(io.g1, FF, S, p);
     const M3<float> gC1 = ldm(io.g1, FC, S, p);
     M3<float> Fc;
@@ -1400,7 +1414,7 @@ __device__ __forceinline__ M3<double> elastic_fc(const StepParams& sp, const Adj
 {
     const int S = sp.stride;
     const M3<float> F = ldm(io.cur, FF, S, p);
-    const M3<float> dF = ldm(io.ctrl, 0, S, h);
+    const M3<float> dF = ld_ctrl(io.ctrl, S, h);
     M3<double> Fc;
 #pragma unroll
     for (int i = 0; i < 9; ++i) Fc.a[i] = double(F.a[i]) + double(dF.a[i]);

@@ -34,19 +34,20 @@ template <typename Real>
 struct StepIO {
     const Real* cur;   // layer n state
     Real* next;        // layer n+1 state (forward)
-    const Real* ctrl;  // dFc_n (9 fields)
+    const Real* ctrl;  // dFc_n (9 fields); FP32 G2P: null when cur's F field holds Fc_n = F_n + dFc_n
     const Real* cst;   // constants
     Real* grid;        // grid m, p -> v (FP64); FP32 keeps m, p (kGridFields * G)
     Real* aux;         // writable alias of cur for derived fields (FP32: A), or null
     unsigned* gflags;  // FP32: tile flags of grid (set where P2G writes), else null
     int lean = 0;      // FP32 fused step: the layer written is read only for x and F (v, C not stored)
+    int fc = 0;        // FP32 fused step: the layer written holds Fc = F + dFc (pio.ctrl) in its F field
 };
 
 template <typename Real>
 struct AdjIO {
     const Real* cur;     // layer n state
     const Real* next;    // layer n+1 state (C' is read)
-    const Real* ctrl;    // dFc_n
+    const Real* ctrl;    // dFc_n; FP32: null when cur's F field holds Fc_n
     const Real* cst;
     const Real* grid;    // grid of step n: m, v after the update (FP64) or m, p (FP32)
     Real* agrid;         // adjoint grid: vbar -> pbar (3G), mbar (G) (FP64); vbar (FP32)

@@ -466,6 +466,19 @@ Real* ctrl_of(dmpm_ctx* c, int layer)
     return static_cast<Real*>(c->ctrl) + (size_t)layer * 9 * c->S;
 }
 
+// FP32 checkpoint tape with the fused step and no smoothing: F enters every
+// kernel only as Fc = F + dFc, so the passes store Fc in the F field of the
+// layers that are neither checkpoints nor the last, and their readers skip
+// the control (a null ctrl).  The API's layer reads recompute true F.
+bool fc_layers(const dmpm_ctx* c)
+{
+    return c->f32 && c->cfg.tape_mode == DMPM_TAPE_CHECKPOINT && c->fuse && c->cfg.smoothing == 0.0;
+}
+bool fc_layer(const dmpm_ctx* c, int j)
+{
+    return fc_layers(c) && j % c->K != 0 && j < c->L - 1;
+}
+
 template <typename Real>
 Real* grads_of(dmpm_ctx* c, int layer)
 {
@@ -584,7 +597,7 @@ bool step_backward(dmpm_ctx* c, int i, const Real* cur, const Real* nxt, const R
     AdjIO<Real> io{};
     io.cur = cur;
     io.next = nxt;
-    io.ctrl = ctrl_of<Real>(c, i);
+    io.ctrl = fc_layer(c, i) ? nullptr : ctrl_of<Real>(c, i);
     io.cst = static_cast<const Real*>(c->cst);
     io.grid = g;
     io.agrid = static_cast<Real*>(c->agrid);
@@ -711,9 +724,11 @@ Real* wgrid_of(dmpm_ctx* c, int i)
 // an earlier pass left them (no P2G of step i0), and the layers j < lean_below
 // that are not checkpoints are written lean (x and F: the next step's G2P is
 // their only reader; the same launch's P2G takes v and C from registers).
+// fc: the layers of fc_layer hold Fc in their F field (written by the fused
+// step, read with a null control); i0 and i1 are checkpoints or the last layer.
 template <typename Real, typename LayFn, typename GridFn, typename KeepFn>
 void forward_seq(dmpm_ctx* c, int i0, int i1, LayFn&& lay, GridFn&& grid, KeepFn&& keep, bool last_g2p = true,
-                 bool p2g0 = true, int lean_below = -1)
+                 bool p2g0 = true, int lean_below = -1, bool fc = false)
 {
     if (i1 <= i0) return;
     if constexpr (std::is_same<Real, float>::value) {
@@ -737,13 +752,15 @@ void forward_seq(dmpm_ctx* c, int i0, int i1, LayFn&& lay, GridFn&& grid, KeepFn
             } else if (i0 + 1 < i1) {
                 clear_grid(c, grid(i0 + 1));  // the P2G of step i0 would have carried this clear
             }
+            auto is_fc = [&](int j) { return fc && fc_layer(c, j); };
+            auto ctrl_in = [&](int j) { return is_fc(j) ? nullptr : ctrl_of<float>(c, j); };
             for (int i = i0 + 1; i < i1; ++i) {
-                const StepIO<float> gio{lay(i - 1), lay(i), ctrl_of<float>(c, i - 1), static_cast<const float*>(c->cst), grid(i - 1),
-                                        nullptr, nullptr, i < lean_below && i % c->K != 0};
+                const StepIO<float> gio{lay(i - 1), lay(i), ctrl_in(i - 1), static_cast<const float*>(c->cst), grid(i - 1),
+                                        nullptr, nullptr, i < lean_below && i % c->K != 0, is_fc(i)};
                 timed(c, DMPM_K_STEP, [&] { launch_step_blk(batch_at(i), gio, pio_of(i), clear_of(i + 1), c->stream); });
             }
             if (last_g2p) {
-                const StepIO<float> gio{lay(i1 - 1), lay(i1), ctrl_of<float>(c, i1 - 1), static_cast<const float*>(c->cst),
+                const StepIO<float> gio{lay(i1 - 1), lay(i1), ctrl_in(i1 - 1), static_cast<const float*>(c->cst),
                                         grid(i1 - 1), nullptr, nullptr};
                 timed(c, DMPM_K_G2P, [&] { g2p_dispatch(c, sp, gio, TileClear{}); });
             }
@@ -762,13 +779,13 @@ void forward_seq(dmpm_ctx* c, int i0, int i1, LayFn&& lay, GridFn&& grid, KeepFn
 
 // Bring layer j (<= fwd_valid) into a readable slot and return it.  Checkpoint
 // mode recomputes from the checkpoint below j into the rolling slots when j
-// is neither a checkpoint nor in the current tape segment.
+// is neither a checkpoint nor in a tape segment, or its tape holds Fc for F.
 template <typename Real>
 Real* materialize(dmpm_ctx* c, int j)
 {
     if (c->cfg.tape_mode == DMPM_TAPE_STORE_ALL) return slot<Real>(c, j);
     if (j % c->K == 0) return slot<Real>(c, j / c->K);
-    if (tape_of(c, j / c->K) >= 0) return tape_layer<Real>(c, j);
+    if (tape_of(c, j / c->K) >= 0 && !fc_layer(c, j)) return tape_layer<Real>(c, j);
     const int j0 = (j / c->K) * c->K;
     auto lay = [&](int k) { return k == j0 ? slot<Real>(c, j0 / c->K) : slot<Real>(c, c->roll_base + (k & 1)); };
     BatchScope bs(c, j / c->K);
@@ -793,7 +810,7 @@ void recompute_segment(dmpm_ctx* c, int seg)
     // with checkpoint grids the forward pass left step s0's grid and affine matrix: no P2G of it
     forward_seq<Real>(c, s0, s1, [&](int j) { return (j == s1 && perm_b) ? bslot : tape_slot<Real>(c, t, j); },
                       [&](int i) { return seg_grid<Real>(c, t, i); }, [](int) { return true; }, s1 % c->K != 0 || perm_b,
-                      c->ck_grids == 0);
+                      c->ck_grids == 0, -1, fc_layers(c));
 }
 
 // ------------------------------------------------------------ block sort ---
@@ -1010,7 +1027,7 @@ int forward_segments(dmpm_ctx* c, int from)
             {
                 BatchScope bs(c, s);
                 forward_seq<float>(c, a, b, [&](int j) { return (j == b && perm_b) ? scratch : lay(j); }, grid_at, keep, true,
-                                   true, s0_prev);
+                                   true, s0_prev, fc_layers(c));
             }
             if (perm_b) {
                 if (sort_at[e + 1] && (err = seg_sort(c, e + 1, scratch))) return;
@@ -1085,7 +1102,8 @@ int forward_pass(dmpm_ctx* c, int start)
             // their affine matrices are not read, but those of checkpoint steps
             // with a grid of their own (the recompute starts after them)
             forward_seq<Real>(c, from, c->L - 1, lay, grid_at,
-                              [&](int i) { return i >= s0_prev || ck_grid<Real>(c, i) != nullptr; }, true, true, s0_prev);
+                              [&](int i) { return i >= s0_prev || ck_grid<Real>(c, i) != nullptr; }, true, true, s0_prev,
+                              fc_layers(c));
         };
         if (c->f32 && c->nstat > 1) {
             if constexpr (std::is_same<Real, float>::value) rc = forward_segments(c, from);
@@ -2393,7 +2411,17 @@ int dmpm_set_option(dmpm_ctx* c, int32_t option, int64_t value)
     case DMPM_OPT_FUSE:
         if (value != 0 && value != 1) return fail(DMPM_ERR_INVALID, "DMPM_OPT_FUSE is 0 or 1");
         HIP_TRY(hipStreamSynchronize(c->stream));
-        c->fuse = value != 0;
+        if (c->fuse != (value != 0)) {
+            const bool fc = fc_layers(c);
+            c->fuse = value != 0;
+            if (fc != fc_layers(c)) {
+                // the tapes' layers hold F or Fc by the old setting: run the pass again
+                c->fwd_valid = std::min(c->fwd_valid, 0);
+                c->loss_valid = false;
+                c->grads_lo = -1;
+                drop_tapes(c);
+            }
+        }
         clear_graphs(c);
         return DMPM_OK;
     case DMPM_OPT_ELASTIC:

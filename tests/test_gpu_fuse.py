@@ -11,7 +11,8 @@ differs, so the bars are those of two runs of one form:
 * a stale order (the windows of both halves on the general and HBM paths): the fused form
   against the oracle at the FP32 bars (loss 1e-6, control gradient 1e-4);
 * the kernel timings count L - 2 fused launches per forward pass, one P2G and one G2P;
-* a batch of 4 instances: fused against separate launches, loss 1e-6, gradient 1e-5.
+* a batch of 4 instances: fused against separate launches, loss 1e-6, gradient 1e-5;
+* the Fc layers of the checkpoint tape (no smoothing): against store-all at the bars above.
 """
 import numpy as np
 import pytest
@@ -181,3 +182,44 @@ def test_fused_batch_equals_separate():
         assert rel(res[1][2], res[0][2]) < 1e-5
     finally:
         mb.close()
+
+
+@pytest.mark.parametrize("smoothing", [0.0, 0.1])
+def test_fc_layers_against_store_all(smoothing):
+    """FP32 checkpoint tape, 12 steps in segments of 4.  Without smoothing the passes keep Fc = F + dFc in
+    the F field of the layers between checkpoints (their readers skip the control); with smoothing they
+    keep F.  Either way the loss (1e-6), every control gradient (1e-5), the state of one layer of each
+    segment and the point diagnostics (J, psi) of a layer between checkpoints (1e-5) equal those of a
+    store-all context.  Switching the fused step off drops the pass when it changes what the tapes
+    hold (the backward pass then raises); the pass run again gives the same gradients."""
+    case = make_case(seed=17, L=13, vel=0.3, per_layer_ctrl=True, smoothing=smoothing)
+    ref = gpu_morph(case, precision=pkg.PREC_F32, tape=pkg.TAPE_STORE_ALL)
+    mo = gpu_morph(case, precision=pkg.PREC_F32, tape=pkg.TAPE_CHECKPOINT, ckpt_stride=4)
+    try:
+        lr, gr = _run(ref, case.L)
+        lm, gm = _run(mo, case.L)
+        assert lm == pytest.approx(lr, rel=1e-6)
+        for i in range(case.L - 1):
+            assert rel(gm[i], gr[i]) < 1e-5, i
+        for j in (10, 5, 2):  # one layer of each segment
+            for a, b in zip(mo.layer_state(j), ref.layer_state(j)):
+                assert maxrel(a, b) < 1e-5, j
+        for a, b in zip(mo.point_diagnostics(9), ref.point_diagnostics(9)):
+            assert maxrel(a, b) < 1e-5
+        mo.forward_pass(0)
+        mo.end_layer_mass_loss()
+        mo.set_option(pkg.OPT_FUSE, 0)
+        if smoothing == 0.0:
+            with pytest.raises(pkg.DmpmError):
+                mo.backward_pass(0)
+        else:
+            mo.backward_pass(0)
+            assert rel(mo.control_grad(0), gr[0]) < 1e-5
+        mo.set_option(pkg.OPT_FUSE, 1)
+        lm2, gm2 = _run(mo, case.L)
+        assert lm2 == pytest.approx(lr, rel=1e-6)
+        for i in range(case.L - 1):
+            assert rel(gm2[i], gr[i]) < 1e-5, i
+    finally:
+        mo.close()
+        ref.close()
