@@ -1143,12 +1143,15 @@ __device__ __forceinline__ void g2p_adj_prepare(const StepParams& sp, const AdjI
     ld3(io.g1, FX, S, p, gx1);
     ld3(io.g1, FV, S, p, gv1);
     const M3<float> F = ldm(io.cur, FF, S, p);
-    const M3<float> dF = ld_ctrl(io.ctrl, S, h);
+    // without a control the load repeats F's (a cache hit) and its value is
+    // dropped: no branch between this particle's loads
+    const bool hc = io.ctrl != nullptr;
+    const M3<float> dF = ldm(hc ? io.ctrl : io.cur, hc ? 0 : FF, S, hc ? h : p);
     const M3<float> gF1 = ldm(io.g1, FF, S, p);
     const M3<float> gC1 = ldm(io.g1, FC, S, p);
     M3<float> Fc;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) Fc.a[i] = F.a[i] + dF.a[i];
+    for (int i = 0; i < 9; ++i) Fc.a[i] = F.a[i] + (hc ? dF.a[i] : 0.f);
     const M3<float> FF_ = mult(gF1, Fc);
 #pragma unroll
     for (int i = 0; i < 9; ++i) q.Ct.a[i] = gC1.a[i] + cs * dt * FF_.a[i];
